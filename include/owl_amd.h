@@ -1,0 +1,209 @@
+/*
+ * owl_amd.h -- C ABI of the MI355X implementation of OWL's acoustic
+ * staggered-grid finite-difference solvers and their FWI gradient.
+ *
+ * Plain C types only: pointers to host memory, sizes, and opaque handles.
+ * Every entry point returns OWL_OK (0) or a negative owl_status; the message
+ * of the last failure on the calling thread is returned by owl_last_error().
+ * Nothing here exits the process (OWL's solver calls `stop` instead).
+ *
+ * Array conventions are those of OWL's Fortran callers:
+ *   models      vp(nz, nx) / rho(nz, nx), column-major, z fastest;
+ *               3D: vp(nz, ny, nx), z fastest, then y, then x;
+ *   traces      per shot [nr][nt] (receiver-major, as the SU traces of
+ *               type(su) hold them: seis_p%trace(ir)%data(1:nt));
+ *   gradients   (nz, nx) column-major, the array written to
+ *               shot_<id>_grad_<name>.grd / grad_<name>.bin.
+ * Reference file:line citations are relative to OWL's source tree.
+ */
+#ifndef OWL_AMD_H
+#define OWL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int owl_status;
+#define OWL_OK 0
+#define OWL_ERR_ARG (-1)       /* invalid argument or unsupported option */
+#define OWL_ERR_CFL (-2)       /* dt above the stability bound (OWL resets it) */
+#define OWL_ERR_DEVICE (-3)    /* HIP runtime failure */
+#define OWL_ERR_NOMEM (-4)     /* device allocation failed */
+#define OWL_ERR_STATE (-5)     /* call out of order (e.g. adjoint before forward) */
+
+/* Source mechanisms and wavelets (module_source_receiver.f90:76-80, 1036-1061). */
+#define OWL_EXPLOSION 0
+#define OWL_FORCE 1
+#define OWL_RICKER 0
+#define OWL_GAUSSIAN 1
+#define OWL_GAUSSIAN_DERIV 2
+#define OWL_RICKER_DERIV 3
+
+/* One source: type(source), module_source_receiver.f90:56-109.  polar and
+ * azimuth are in radians (load_geometry converts degrees, :257-258). */
+typedef struct owl_source {
+    float x, y, z;
+    float f0, amp, t0;
+    float polar, azimuth;
+    int mechanism;
+    int wavelet;
+} owl_source;
+
+/* One receiver: type(receiver), module_source_receiver.f90:29-53. */
+typedef struct owl_receiver {
+    float x, y, z;
+    float weight;
+} owl_receiver;
+
+/* One shot: type(source_receiver_geometry) (module_source_receiver.f90:
+ * 111-142) plus the shot window that set_adaptive_range computes
+ * (module_utility.f90:410-481): 1-based inclusive node ranges in the full
+ * model.  A range of 0..0 means the full model along that axis. */
+typedef struct owl_shot {
+    int id;
+    int ns;
+    const owl_source *sources;
+    int nr;
+    const owl_receiver *receivers;
+    int nxbeg, nxend;
+    int nybeg, nyend;
+    int nzbeg, nzend;
+} owl_shot;
+
+/* Solver configuration: the components of type(wave_solver_acoustic_iso_2d)
+ * and _3d that are not arrays (acoustic_iso_2d/interface.f90:30-53). */
+typedef struct owl_acoustic_config {
+    int nx, ny, nz;          /* full model; ny = 1 in 2D */
+    float dx, dy, dz;
+    float ox, oy, oz;
+    float dt, tmax;          /* nt = nint(tmax/dt + 1) */
+    float data_dt, data_tmax;/* must equal dt, tmax: OWL's cubic resampling is not provided */
+    int pml;                 /* npml, default 15 (module_model.f90:174) */
+    int fdhalf;              /* R: 2..8; OWL's -D_fdorder<2R>_ */
+    int free_surface;        /* 2D only */
+    float f0factor;          /* default 1.0 (module_source_receiver.f90:134) */
+    int cc_step_interval;    /* imaging every k-th reverse step, default 1 */
+} owl_acoustic_config;
+
+/* Gradient algorithms. */
+#define OWL_GRAD_OWL 0    /* OWL's algorithm: boundary-saving reconstruction
+                             with -dt (acoustic_iso_2d/submodule_fwi_gradient.f90:75-167) */
+#define OWL_GRAD_EXACT 1  /* exact discrete adjoint of the forward scheme */
+
+/* Forward-state storage for OWL_GRAD_EXACT. */
+#define OWL_CKPT_FULL 0      /* keep every step's state */
+#define OWL_CKPT_UNIFORM 1   /* snapshots every k steps, segment recompute */
+#define OWL_CKPT_BINOMIAL 2  /* binomial (revolve) schedule with a snapshot budget */
+#define OWL_STORE_F32 0
+#define OWL_STORE_F16 1      /* with a per-snapshot scale */
+#define OWL_STORE_BF16 2     /* with a per-snapshot scale */
+
+typedef struct owl_acoustic2d owl_acoustic2d; /* opaque batch of 2D shots */
+
+/* Version string of the library. */
+const char *owl_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *owl_last_error(void);
+
+/* Stable dt of the scheme (acoustic_iso_2d/module_vars.f90:121-125):
+ * 1/(sum|c_m| * vmax * sqrt(1/dx^2 [+ 1/dy^2] + 1/dz^2)), float32. */
+float owl_dt_stable(int fdhalf, int dims, float vmax, float dx, float dy, float dz);
+
+/* Grid location and Kaiser-windowed sinc weights of one point, as
+ * source_receiver_geometry%prepare_geometry computes them
+ * (module_source_receiver.f90:556-785).  idx[6] = gx, gy, gz, hx, hy, hz
+ * (1-based); w[6*9] = ix, iy, iz, hx, hy, hz weights for taps -4..4. */
+owl_status owl_locate_point(int dims, float x, float y, float z, float ox, float oy, float oz,
+                            float dx, float dy, float dz, int *idx, float *w);
+
+/* Normalised source time function (prepare_stf, module_source_receiver.f90:
+ * 1002-1097).  Returns the sample count; writes up to cap samples and hnt. */
+int owl_source_time_function(int wavelet, float f0, float dt, float f0factor, float *stf,
+                             int cap, int *hnt);
+
+/* CFS-PML coefficients of one axis (compute_cfspml_damping_coef,
+ * acoustic_iso_2d/module_cfspml.f90:61-133): four arrays of n + 2*pml
+ * entries for i = -pml+1 .. n+pml (a, b on integer nodes; a, b on half
+ * nodes).  kappa is 1 everywhere. */
+owl_status owl_cfspml_axis(int n, int pml, float d, float dt, float pmlvp, float f0max, float *ai,
+                           float *bi, float *ah, float *bh);
+
+/* ---- 2D acoustic: batched shots on one GPU --------------------------------
+ * Replaces the per-shot loops of forward_modeling
+ * (src/forward/2d/module_modeling.f90:66-95) and compute_gradient_shots
+ * (src/inverse/2d/module_gradient.f90:93-388) for which_medium =
+ * 'acoustic-iso'.  vp, rho: full model (nz, nx), z fastest.  Every shot runs
+ * on its own window; all shots of the batch advance together on the GPU. */
+owl_status owl_acoustic2d_create(const owl_acoustic_config *cfg, const float *vp,
+                                 const float *rho, int nshot, const owl_shot *shots,
+                                 owl_acoustic2d **out);
+void owl_acoustic2d_destroy(owl_acoustic2d *b);
+/* Number of steps nt and the per-shot receiver counts. */
+int owl_acoustic2d_nt(const owl_acoustic2d *b);
+/* Replace the model (same shape) without re-planning. */
+owl_status owl_acoustic2d_set_model(owl_acoustic2d *b, const float *vp, const float *rho);
+
+/* Forward modelling of every shot (compute_forward,
+ * acoustic_iso_2d/submodule_forward_modeling.f90:31-132).  traces: shot
+ * after shot, each [nr][nt].  save_boundary != 0 keeps the data OWL's
+ * gradient needs (reconstruct = .true.). */
+owl_status owl_acoustic2d_forward(owl_acoustic2d *b, float *traces, int save_boundary);
+
+/* FWI gradient over all shots.  obs: observed traces, laid out like the
+ * forward traces.  Computes the L2 misfit and the adjoint source
+ * syn - obs (compute_adjsrc_waveform,
+ * module_inversion_adjoint_source.f90:371-383, with the trace-discard rule
+ * of :1042-1054), back-propagates it, and sums the per-shot window
+ * gradients into the full model (module_gradient.f90:366-413).
+ *   mode      OWL_GRAD_OWL | OWL_GRAD_EXACT
+ *   ckpt      OWL_CKPT_* (EXACT only); ckpt_param = snapshot interval
+ *             (UNIFORM) or snapshot budget (BINOMIAL)
+ *   store     OWL_STORE_* for snapshots (EXACT only)
+ *   misfit    per-shot misfit values [nshot] (FP64), may be NULL
+ *   grad_vp, grad_rho: (nz, nx) z fastest, summed over shots (FP64).
+ * traces_out (may be NULL): the synthetic traces of the same run. */
+owl_status owl_acoustic2d_gradient(owl_acoustic2d *b, const float *obs, int mode, int ckpt,
+                                   int ckpt_param, int store, double *misfit, double *grad_vp,
+                                   double *grad_rho, float *traces_out);
+
+/* Adjoint pass alone, OWL's algorithm, on data from a preceding
+ * owl_acoustic2d_forward(..., save_boundary = 1): adjoint source traces adj
+ * (laid out like the forward traces) in, per-shot window gradients out
+ * (compute_adjoint, acoustic_iso_2d/submodule_fwi_gradient.f90:31-199).
+ * grad_vp, grad_rho: shot after shot, each (nz_s, nx_s) of the shot's
+ * window, z fastest: the arrays of shot_<id>_grad_vp.grd / _rho.grd. */
+owl_status owl_acoustic2d_adjoint(owl_acoustic2d *b, const float *adj, float *grad_vp,
+                                  float *grad_rho);
+
+/* Window size of shot s: writes nx_s, nz_s (and the number of receivers). */
+owl_status owl_acoustic2d_shot_dims(const owl_acoustic2d *b, int s, int *nx, int *nz, int *nr);
+
+/* Keep observed traces resident on the device (laid out like the forward
+ * traces); owl_acoustic2d_gradient(b, NULL, ...) then uses them.  This is
+ * the data OWL's compute_adjoint_source reads from dir_field
+ * (src/inverse/2d/module_gradient.f90:321-327). */
+owl_status owl_acoustic2d_set_observed(owl_acoustic2d *b, const float *obs);
+/* Model the batch's current model and keep the traces as observed data
+ * (synthetic inversion tests generate d_obs this way). */
+owl_status owl_acoustic2d_observe(owl_acoustic2d *b);
+
+/* Device timing of the last forward/gradient call (ms) and the number of
+ * stencil grid-point updates it performed. */
+owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double *updates);
+
+/* Per-kernel timing with HIP events on the batch's stream.  No OWL
+ * counterpart (OWL has no profiler); used by the benchmark.  kind:
+ * 0 p-sweep, 1 v-sweep, 2 source/receiver step, 3 imaging condition,
+ * 4 boundary injection, 5 adjoint-source injection, 6 misfit, 7 merge.
+ * bytes_per_launch: algorithmic bytes of one launch (0 if not modelled). */
+owl_status owl_acoustic2d_set_profiling(owl_acoustic2d *b, int on);
+owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d *b, int kind, double *total_ms,
+                                       long long *launches, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OWL_AMD_H */

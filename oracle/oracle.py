@@ -1,0 +1,265 @@
+"""ctypes front end of the CPU oracle (oracle/owl_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline, never as part of the product path.
+
+Array conventions follow OWL's callers: models are numpy arrays of shape
+(nz, nx) [2D] or (nz, ny, nx) [3D]; they are handed to C in Fortran order (z
+fastest), i.e. as the memory image of OWL's vp(nz, nx) / vp(nz, ny, nx).
+Gradients come back the same way.  Traces are (nr, nt), receiver-major.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class Grid2D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_float), ("dz", C.c_float),
+                ("ox", C.c_float), ("oz", C.c_float), ("dt", C.c_float), ("nt", C.c_int),
+                ("pml", C.c_int), ("R", C.c_int), ("free_surface", C.c_int),
+                ("cc_step_interval", C.c_int)]
+
+
+class Source2D(C.Structure):
+    _fields_ = [("x", C.c_float), ("z", C.c_float), ("amp", C.c_float), ("t0", C.c_float),
+                ("polar", C.c_float), ("mechanism", C.c_int), ("nt_stf", C.c_int),
+                ("stf", C.c_void_p)]
+
+
+class Receiver2D(C.Structure):
+    _fields_ = [("x", C.c_float), ("z", C.c_float), ("weight", C.c_float)]
+
+
+class Grid3D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
+                ("dy", C.c_float), ("dz", C.c_float), ("ox", C.c_float), ("oy", C.c_float),
+                ("oz", C.c_float), ("dt", C.c_float), ("nt", C.c_int), ("pml", C.c_int),
+                ("R", C.c_int)]
+
+
+class Source3D(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("amp", C.c_float),
+                ("t0", C.c_float), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
+
+
+class Receiver3D(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("weight", C.c_float)]
+
+
+class Point(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float),
+                ("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int),
+                ("hx", C.c_int), ("hy", C.c_int), ("hz", C.c_int),
+                ("ix", C.c_float * 9), ("iy", C.c_float * 9), ("iz", C.c_float * 9),
+                ("hxw", C.c_float * 9), ("hyw", C.c_float * 9), ("hzw", C.c_float * 9)]
+
+
+MECH = {"explosion": 0, "force": 1}
+WAVELET = {"ricker": 0, "gaussian": 1, "gaussian_deriv": 2, "ricker_deriv": 3}
+
+
+def build():
+    """Compile both oracle builds (float32 and float64) in place."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+class Oracle:
+    """One precision of the oracle (32 or 64)."""
+
+    def __init__(self, bits=32):
+        if bits not in (32, 64):
+            raise ValueError("bits must be 32 or 64")
+        path = os.path.join(_HERE, f"liboracle{bits}.so")
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        self.bits = bits
+        self.real = np.float32 if bits == 32 else np.float64
+        self.p = f"ora{bits}_"
+        f = self._fn
+        f("fdcoefs").restype = C.c_int
+        f("dt_stable").restype = C.c_float
+        f("dt_stable").argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+        f("alphamax").restype = C.c_float
+        f("alphamax").argtypes = [C.c_float]
+        f("nsteps").restype = C.c_int
+        f("nsteps").argtypes = [C.c_float, C.c_float]
+        f("stf_length").restype = C.c_int
+        f("stf_length").argtypes = [C.c_float, C.c_float, C.c_float]
+        f("prepare_stf").restype = C.c_int
+        f("prepare_stf").argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p,
+                                     C.POINTER(C.c_int)]
+        f("kaiser_sinc").argtypes = [C.c_float, C.c_void_p]
+        f("locate_point").argtypes = [C.c_int] + [C.c_float] * 9 + [C.POINTER(Point)]
+        f("cfspml_axis").argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                     C.c_float] + [C.c_void_p] * 6
+        f("acoustic2d_forward").restype = C.c_int
+        f("acoustic2d_gradient_owl").restype = C.c_int
+        f("acoustic3d_forward").restype = C.c_int
+        f("adjoint_source_l2").restype = C.c_double
+        f("adjoint_source_l2").argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_float]
+
+    def _fn(self, name):
+        return getattr(self.lib, self.p + name)
+
+    # -- scalar / small helpers ------------------------------------------
+    def fdcoefs(self, R):
+        out = np.zeros(8, np.float32)
+        if self._fn("fdcoefs")(R, out.ctypes.data_as(C.c_void_p)) != 0:
+            raise ValueError("R must be 2..8")
+        return out[:R].copy()
+
+    def dt_stable(self, R, dims, vmax, dx, dy, dz):
+        return self._fn("dt_stable")(R, dims, vmax, dx, dy, dz)
+
+    def nsteps(self, tmax, dt):
+        return self._fn("nsteps")(tmax, dt)
+
+    def alphamax(self, f0max):
+        return self._fn("alphamax")(f0max)
+
+    def kaiser_sinc(self, x):
+        h = np.zeros(9, np.float32)
+        self._fn("kaiser_sinc")(C.c_float(x), h.ctypes.data_as(C.c_void_p))
+        return h
+
+    def locate(self, x, z, ox, oz, dx, dz, y=0.0, oy=0.0, dy=1.0, dims=2):
+        pt = Point()
+        self._fn("locate_point")(dims, x, y, z, ox, oy, oz, dx, dy, dz, C.byref(pt))
+        return pt
+
+    def stf(self, f0, dt, f0factor=1.0, wavelet="ricker"):
+        hnt = C.c_int(0)
+        n = self._fn("prepare_stf")(WAVELET[wavelet], f0, dt, f0factor, None, C.byref(hnt))
+        buf = np.zeros(n, self.real)
+        self._fn("prepare_stf")(WAVELET[wavelet], f0, dt, f0factor,
+                                buf.ctypes.data_as(C.c_void_p), C.byref(hnt))
+        return buf, hnt.value
+
+    def cfspml_axis(self, n, pml, d, dt, pmlvp, f0max):
+        arrs = [np.zeros(n + 2 * pml, np.float32) for _ in range(6)]
+        self._fn("cfspml_axis")(n, pml, d, dt, pmlvp, f0max,
+                                *[a.ctypes.data_as(C.c_void_p) for a in arrs])
+        return dict(zip(["ai", "bi", "ki", "ah", "bh", "kh"], arrs))
+
+    # -- 2D acoustic -------------------------------------------------------
+    def _grid2d(self, g):
+        return Grid2D(int(g["nx"]), int(g["nz"]), g["dx"], g["dz"], g.get("ox", 0.0),
+                      g.get("oz", 0.0), g["dt"], int(g["nt"]), int(g["pml"]), int(g["R"]),
+                      int(bool(g.get("free_surface", False))), int(g.get("cc_step_interval", 1)))
+
+    def _sources2d(self, g, sources):
+        keep, arr, f0s, hnt = [], (Source2D * max(1, len(sources)))(), [], None
+        for k, s in enumerate(sources):
+            stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            if hnt is None:
+                hnt = h
+            stf = np.ascontiguousarray(stf, dtype=self.real)
+            keep.append(stf)
+            arr[k] = Source2D(s["x"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0),
+                              np.float32(s.get("polar", 0.0)), MECH[s.get("mechanism", "explosion")],
+                              len(stf), stf.ctypes.data)
+            f0s.append(s["f0"])
+        f0a = np.asarray(f0s if f0s else [0.0], np.float32)
+        return arr, keep, f0a, hnt
+
+    @staticmethod
+    def _receivers2d(receivers):
+        arr = (Receiver2D * max(1, len(receivers)))()
+        for k, r in enumerate(receivers):
+            arr[k] = Receiver2D(r["x"], r["z"], r.get("weight", 1.0))
+        return arr
+
+    @staticmethod
+    def _model(a, shape):
+        a = np.asfortranarray(a, dtype=np.float32)
+        if a.shape != shape:
+            raise ValueError(f"model shape {a.shape} != {shape}")
+        return a
+
+    def acoustic2d_forward(self, g, vp, rho, sources, receivers):
+        """Traces (nr, nt) of OWL's compute_forward for one shot window."""
+        G = self._grid2d(g)
+        vp = self._model(vp, (G.nz, G.nx))
+        rho = self._model(rho, (G.nz, G.nx))
+        src, keep, f0a, _ = self._sources2d(g, sources)
+        rec = self._receivers2d(receivers)
+        tr = np.zeros((len(receivers), G.nt), self.real)
+        rc = self._fn("acoustic2d_forward")(
+            C.byref(G), vp.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p),
+            len(sources), src, f0a.ctypes.data_as(C.c_void_p), len(receivers), rec,
+            tr.ctypes.data_as(C.c_void_p), None, None)
+        if rc != 0:
+            raise RuntimeError(f"oracle acoustic2d_forward failed ({rc})")
+        return tr
+
+    def acoustic2d_gradient_owl(self, g, vp, rho, sources, receivers, adj):
+        """OWL's (grad_vp, grad_rho) as (nz, nx) arrays, plus the forward traces."""
+        G = self._grid2d(g)
+        vp = self._model(vp, (G.nz, G.nx))
+        rho = self._model(rho, (G.nz, G.nx))
+        src, keep, f0a, hnt = self._sources2d(g, sources)
+        rec = self._receivers2d(receivers)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (len(receivers), G.nt)
+        tr = np.zeros((len(receivers), G.nt), self.real)
+        gv_f = np.zeros((G.nz, G.nx), self.real, order="F")
+        gr_f = np.zeros((G.nz, G.nx), self.real, order="F")
+        rc = self._fn("acoustic2d_gradient_owl")(
+            C.byref(G), vp.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p),
+            len(sources), src, f0a.ctypes.data_as(C.c_void_p), int(hnt), len(receivers), rec,
+            adj.ctypes.data_as(C.c_void_p), gv_f.ctypes.data_as(C.c_void_p),
+            gr_f.ctypes.data_as(C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle acoustic2d_gradient_owl failed ({rc})")
+        return np.ascontiguousarray(gv_f), np.ascontiguousarray(gr_f), tr
+
+    def adjoint_source_l2(self, weight, syn, obs, threshold=1.0e-6):
+        syn = np.ascontiguousarray(syn, dtype=self.real)
+        obs = np.ascontiguousarray(obs, dtype=self.real)
+        w = np.ascontiguousarray(weight, dtype=np.float32)
+        adj = np.zeros_like(syn)
+        nr, nt = syn.shape
+        mis = self._fn("adjoint_source_l2")(nr, nt, w.ctypes.data_as(C.c_void_p),
+                                            syn.ctypes.data_as(C.c_void_p),
+                                            obs.ctypes.data_as(C.c_void_p),
+                                            adj.ctypes.data_as(C.c_void_p), threshold)
+        return mis, adj
+
+    # -- 3D acoustic -------------------------------------------------------
+    def acoustic3d_forward(self, g, vp, rho, sources, receivers):
+        G = Grid3D(int(g["nx"]), int(g["ny"]), int(g["nz"]), g["dx"], g["dy"], g["dz"],
+                   g.get("ox", 0.0), g.get("oy", 0.0), g.get("oz", 0.0), g["dt"], int(g["nt"]),
+                   int(g["pml"]), int(g["R"]))
+        vp = self._model(vp, (G.nz, G.ny, G.nx))
+        rho = self._model(rho, (G.nz, G.ny, G.nx))
+        keep = []
+        arr = (Source3D * max(1, len(sources)))()
+        f0max = 0.0
+        for k, s in enumerate(sources):
+            stf, _ = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            stf = np.ascontiguousarray(stf, dtype=self.real)
+            keep.append(stf)
+            arr[k] = Source3D(s["x"], s["y"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0),
+                              len(stf), stf.ctypes.data)
+            f0max = max(f0max, s["f0"])
+        rec = (Receiver3D * max(1, len(receivers)))()
+        for k, r in enumerate(receivers):
+            rec[k] = Receiver3D(r["x"], r["y"], r["z"], r.get("weight", 1.0))
+        tr = np.zeros((len(receivers), G.nt), self.real)
+        self._fn("acoustic3d_forward").argtypes = [C.POINTER(Grid3D), C.c_void_p, C.c_void_p,
+                                                   C.c_int, C.c_void_p, C.c_float, C.c_int,
+                                                   C.c_void_p, C.c_void_p]
+        rc = self._fn("acoustic3d_forward")(C.byref(G), vp.ctypes.data_as(C.c_void_p),
+                                            rho.ctypes.data_as(C.c_void_p), len(sources),
+                                            C.cast(arr, C.c_void_p), f0max, len(receivers),
+                                            C.cast(rec, C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle acoustic3d_forward failed ({rc})")
+        return tr

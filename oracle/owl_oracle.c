@@ -1,0 +1,965 @@
+/*
+ * owl_oracle.c -- CPU restatement of OWL's acoustic SSG-FD time stepping and
+ * its boundary-saving FWI gradient.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle (and the
+ * timed CPU baseline).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker.  The
+ * product library (owl_amd/) never links, loads or calls anything here.
+ *
+ * The file is compiled twice: with -DORA_REAL=float (the reference's own
+ * precision) and with -DORA_REAL=double (an FP64 build of the same scheme,
+ * used to state the FP32 error).  Integer-valued quantities (grid indices,
+ * STF length, hnt, nt) are always derived in float32 arithmetic exactly as
+ * the Fortran writes them, so both builds share them bit for bit.
+ *
+ * Each function cites the reference file:line it restates; paths are
+ * relative to the OWL source tree.  Operation order follows the Fortran
+ * left-to-right association; build with -ffp-contract=off.
+ *
+ * Parity pinning (what checks each part independently of this file):
+ *   fd coefficients ........ literal constants, consistency sum c_m(2m-1)~1
+ *   nint / grid indices .... half-way cases vs round-half-away-from-zero
+ *   Kaiser-windowed sinc ... numpy sinc * numpy i0 closed form
+ *   Ricker STF ............. numpy closed form
+ *   CFS-PML coefficients ... closed-form numpy evaluation in float64
+ *   forward propagation .... 2D Green's function (Hankel), linearity,
+ *                            reciprocity, mirror symmetry
+ *   OWL gradient ........... dot-product style C-consistency across probes,
+ *                            mirror symmetry, finite-difference sign/scale
+ *   exact adjoint .......... dot-product test <J dm, dd> = <dm, J^T dd>
+ *                            and central finite differences
+ * FLIT internals (const_pi precision, bessel_i0, sinc) are restated from
+ * their mathematical definitions; OWL's FLIT version is unpinned.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+#ifndef ORA_REAL
+#define ORA_REAL float
+#endif
+typedef ORA_REAL real;
+
+#if defined(ORA_F64)
+#define ORA_SYM(name) ora64_##name
+#else
+#define ORA_SYM(name) ora32_##name
+#endif
+
+#define ORA_PI 3.14159265358979323846264338327950288
+#define NKW 4
+
+/* ------------------------------------------------------------------------ */
+/* FD coefficients: src/lib/module_fdcoef.f90:24-71 (R = fdhalf = 2..8).     */
+/* ------------------------------------------------------------------------ */
+static const float FDC[9][8] = {
+    {0}, {0},
+    {0.1129042e+1f, -0.4301412e-1f},
+    {0.1185991e+1f, -0.7249965e-1f, 0.6301572e-2f},
+    {0.1217990e+1f, -0.9382142e-1f, 0.1507536e-1f, -0.1700324e-2f},
+    {0.1236607e+1f, -0.1082265e+0f, 0.2343440e-1f, -0.5033546e-2f, 0.6817483e-3f},
+    {0.1247662e+1f, -0.1175538e+0f, 0.2997970e-1f, -0.8719078e-2f, 0.2215897e-2f,
+     -0.3462075e-3f},
+    {0.1254799e+1f, -0.1238928e+0f, 0.3494371e-1f, -0.1208897e-1f, 0.4132531e-2f,
+     -0.1197110e-2f, 0.2122227e-3f},
+    {0.1259312e+1f, -0.1280347e+0f, 0.3841945e-1f, -0.1473229e-1f, 0.5924913e-2f,
+     -0.2248618e-2f, 0.7179226e-3f, -0.1400855e-3f},
+};
+
+int ORA_SYM(fdcoefs)(int R, float *out) {
+    if (R < 2 || R > 8) return -1;
+    for (int m = 0; m < R; ++m) out[m] = FDC[R][m];
+    return 0;
+}
+
+/* Fortran nint: round half away from zero (float argument). */
+static int fnint(float x) { return (int)roundf(x); }
+
+/* ------------------------------------------------------------------------ */
+/* Kaiser-windowed sinc: module_source_receiver.f90:789-828.                  */
+/* kaiser_b0 is a double parameter initialised from the default-real literal */
+/* 6.31 (:147), i.e. the float 6.31 widened to double.                       */
+/* bessel_i0: modified Bessel function I0 by its power series (FLIT).       */
+/* ------------------------------------------------------------------------ */
+static double bessel_i0(double x) {
+    double sum = 1.0, term = 1.0, q = 0.25 * x * x;
+    for (int k = 1; k < 500; ++k) {
+        term *= q / ((double)k * (double)k);
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+
+static double sinc_d(double x) { return x == 0.0 ? 1.0 : sin(x) / x; }
+
+void ORA_SYM(kaiser_sinc)(float x, float *h /* [9] for -4..4 */) {
+    const double b0 = (double)6.31f;
+    for (int i = 0; i < 2 * NKW + 1; ++i) h[i] = 0.0f;
+    if (x == 0.0f) { h[NKW] = 1.0f; return; }
+    for (int i = -NKW; i <= NKW; ++i) {
+        float t = (float)i - x;
+        float hi = (float)sinc_d(ORA_PI * (double)t);
+        float r;
+        if (t >= -(float)NKW && t <= (float)NKW) {
+            float q = t / (float)NKW;
+            r = sqrtf(1.0f - q * q);
+            r = (float)(bessel_i0(b0 * (double)r) / bessel_i0(b0));
+        } else {
+            r = 0.0f;
+        }
+        h[i + NKW] = hi * r;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Geometry.  prepare_geometry, module_source_receiver.f90:556-785 (uniform  */
+/* mesh branch).  One record per point; indices are OWL's 1-based grid      */
+/* indices within the shot window.                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    float x, y, z;
+    int gx, gy, gz, hx, hy, hz;
+    float ix[9], iy[9], iz[9], hxw[9], hyw[9], hzw[9];
+} ora_point;
+
+/* dims = 2 or 3; ox..dz describe the window's first grid node and spacing. */
+void ORA_SYM(locate_point)(int dims, float x, float y, float z, float ox, float oy,
+                           float oz, float dx, float dy, float dz, ora_point *pt) {
+    memset(pt, 0, sizeof(*pt));
+    pt->x = x; pt->y = y; pt->z = z;
+    pt->gx = fnint((x - ox) / dx) + 1;
+    pt->gz = fnint((z - oz) / dz) + 1;
+    pt->hx = fnint((x - ox + 0.5f * dx) / dx) + 1;
+    pt->hz = fnint((z - oz + 0.5f * dz) / dz) + 1;
+    float dis;
+    dis = (x - ox) / dx - (float)(pt->gx - 1);
+    ORA_SYM(kaiser_sinc)(dis, pt->ix);
+    dis = (z - oz) / dz - (float)(pt->gz - 1);
+    ORA_SYM(kaiser_sinc)(dis, pt->iz);
+    dis = (x - ox + 0.5f * dx) / dx - (float)(pt->hx - 1);
+    ORA_SYM(kaiser_sinc)(dis, pt->hxw);
+    dis = (z - oz + 0.5f * dz) / dz - (float)(pt->hz - 1);
+    ORA_SYM(kaiser_sinc)(dis, pt->hzw);
+    if (dims == 3) {
+        pt->gy = fnint((y - oy) / dy) + 1;
+        pt->hy = fnint((y - oy + 0.5f * dy) / dy) + 1;
+        dis = (y - oy) / dy - (float)(pt->gy - 1);
+        ORA_SYM(kaiser_sinc)(dis, pt->iy);
+        dis = (y - oy + 0.5f * dy) / dy - (float)(pt->hy - 1);
+        ORA_SYM(kaiser_sinc)(dis, pt->hyw);
+    } else {
+        pt->gy = pt->hy = 1;
+        pt->iy[NKW] = pt->hyw[NKW] = 1.0f;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Source time function.  prepare_stf, module_source_receiver.f90:1002-1097, */
+/* ricker_wavelet :883-891, ricker_deriv :896-904, gaussian :847-854.       */
+/* const_pi (FLIT) is taken as double precision pi.                         */
+/* wavelet: 0 ricker, 1 gaussian, 2 gaussian_deriv, 3 ricker_deriv          */
+/* Returns nt_stf; writes hnt.  stf must hold nt_stf values (query with     */
+/* stf = NULL first).                                                       */
+/* ------------------------------------------------------------------------ */
+int ORA_SYM(stf_length)(float f0, float dt, float f0factor) {
+    return (int)ceilf(2.0f * f0factor / f0 / dt);
+}
+
+int ORA_SYM(prepare_stf)(int wavelet, float f0, float dt, float f0factor, real *stf, int *hnt) {
+    int n = ORA_SYM(stf_length)(f0, dt, f0factor);
+    if (hnt) *hnt = fnint((float)n / 2.0f);
+    if (!stf) return n;
+    real mx = 0;
+    for (int t = 1; t <= n; ++t) {
+        real tt = (real)((float)(t - 1) * dt - f0factor / f0);
+        real w, v;
+        switch (wavelet) {
+        case 1: /* gaussian */
+            w = (real)((ORA_PI * (double)f0 * (double)tt) * (ORA_PI * (double)f0 * (double)tt));
+            v = (real)exp(-(double)w);
+#if !defined(ORA_F64)
+            v = expf(-w);
+#endif
+            break;
+        case 2: /* gaussian_deriv */
+            w = (real)((ORA_PI * (double)f0 * (double)tt) * (ORA_PI * (double)f0 * (double)tt));
+#if defined(ORA_F64)
+            v = -tt * exp(-w);
+#else
+            v = -tt * expf(-w);
+#endif
+            break;
+        case 3: /* ricker_deriv */
+            w = (real)((ORA_PI * (double)f0 * (double)tt) * (ORA_PI * (double)f0 * (double)tt));
+#if defined(ORA_F64)
+            v = (((real)2.0 * w - (real)3.0) * exp(-w)) * tt;
+#else
+            v = ((2.0f * w - 3.0f) * expf(-w)) * tt;
+#endif
+            break;
+        default: /* ricker */
+            w = (real)((ORA_PI * (double)f0 * (double)tt) * (ORA_PI * (double)f0 * (double)tt));
+#if defined(ORA_F64)
+            v = ((real)1 - (real)2.0 * w) * exp(-w);
+#else
+            v = (1.0f - 2.0f * w) * expf(-w);
+#endif
+            break;
+        }
+        stf[t - 1] = v;
+        real a = v < 0 ? -v : v;
+        if (a > mx) mx = a;
+    }
+    for (int t = 0; t < n; ++t) stf[t] = stf[t] / mx;
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CFL: prepare_modeling, acoustic_iso_2d/module_vars.f90:121-126 and       */
+/* acoustic_iso_3d/module_vars.f90:174-181.  Computed in float32.           */
+/* ------------------------------------------------------------------------ */
+float ORA_SYM(dt_stable)(int R, int dims, float vmax, float dx, float dy, float dz) {
+    float temp = 0.0f;
+    for (int m = 0; m < R; ++m) temp = temp + fabsf(FDC[R][m]);
+    float s = 1.0f / (dx * dx) + ((dims == 3) ? 1.0f / (dy * dy) : 0.0f);
+    if (dims == 3) s = 1.0f / (dx * dx) + 1.0f / (dy * dy) + 1.0f / (dz * dz);
+    else s = 1.0f / (dx * dx) + 1.0f / (dz * dz);
+    return 1.0f / (temp * vmax * sqrtf(s));
+}
+
+/* nt = nint(tmax/dt + 1): acoustic_iso_2d/module_vars.f90:134 */
+int ORA_SYM(nsteps)(float tmax, float dt) { return fnint(tmax / dt + 1.0f); }
+
+/* ------------------------------------------------------------------------ */
+/* CFS-PML damping profile for one axis: damp_coef + compute_cfspml_damping  */
+/* _coef, acoustic_iso_2d/module_cfspml.f90:37-133.  Arrays are indexed     */
+/* i = -pml+1 .. n+pml (length n+2*pml); entries outside the slabs hold     */
+/* a = b = 0, k = 1 (alloc_array zero-initialises; k set to 1 at :88-91).   */
+/* ------------------------------------------------------------------------ */
+static void damp_coef(float bdvp, float d, float dist, int pml, float dt, float alphamax,
+                      float *a, float *b, float *k) {
+    const float R0 = 1.0e-5f, kmax = 1.0f;
+    float nd = dist / ((float)pml * d);
+    float alpha = alphamax * (1.0f - nd);
+    float alpha_min = 0.25f * alphamax;
+    if (alpha_min > alpha) alpha = alpha_min;
+    float damp = bdvp * (2.0f + 1.0f) * logf(1.0f / R0) / (2.0f * (float)pml * d) * (nd * nd);
+    float kk = 1.0f + (kmax - 1.0f) * (nd * nd);
+    float adt = fabsf(dt);
+    *a = -(2.0f * adt * damp / kk / (2.0f + adt * (alpha + damp / kk)));
+    *b = (2.0f - adt * (alpha + damp / kk)) / (2.0f + adt * (alpha + damp / kk));
+    *k = kk;
+}
+
+float ORA_SYM(alphamax)(float f0max) { return (float)((double)(1.0f * f0max) * ORA_PI); }
+
+void ORA_SYM(cfspml_axis)(int n, int pml, float d, float dt, float pmlvp, float f0max,
+                          float *ai, float *bi, float *ki, float *ah, float *bh, float *kh) {
+    float alphamax = ORA_SYM(alphamax)(f0max);
+    int len = n + 2 * pml;
+    for (int q = 0; q < len; ++q) {
+        ai[q] = bi[q] = ah[q] = bh[q] = 0.0f;
+        ki[q] = kh[q] = 1.0f;
+    }
+    for (int i = -pml + 1; i <= n + pml; ++i) {
+        float di = 0.0f, dh = 0.0f;
+        if (i <= 1) {
+            di = fabsf((float)(i - 1) * d);
+            dh = fabsf(((float)(i - 1) - 0.5f) * d);
+        } else if (i >= n + 1) {
+            di = fabsf((float)(i - n) * d);
+            dh = fabsf(((float)(i - n) - 0.5f) * d);
+        }
+        if (i <= 1 || i >= n + 1) {
+            int q = i + pml - 1;
+            damp_coef(pmlvp, d, di, pml, dt, alphamax, &ai[q], &bi[q], &ki[q]);
+            damp_coef(pmlvp, d, dh, pml, dt, alphamax, &ah[q], &bh[q], &kh[q]);
+        }
+    }
+}
+
+/* ======================================================================== */
+/* 2D acoustic solver                                                        */
+/* ======================================================================== */
+
+typedef struct {
+    int nx, nz;            /* shot window, interior points */
+    float dx, dz, ox, oz;  /* window origin (shot_xbeg, shot_zbeg) */
+    float dt;              /* time step (after the caller's CFL policy) */
+    int nt;                /* number of steps */
+    int pml;               /* npml */
+    int R;                 /* fdhalf */
+    int free_surface;
+    int cc_step_interval;
+} ora_grid2d;
+
+typedef struct {
+    float x, z, amp, t0, polar; /* polar in radians */
+    int mechanism;              /* 0 explosion, 1 force */
+    int nt_stf;
+    const real *stf;            /* nt_stf samples */
+} ora_source2d;
+
+typedef struct {
+    float x, z, weight;
+} ora_receiver2d;
+
+typedef struct {
+    int lo, NX, NZ; /* index lo..lo+N-1 on both axes, x fastest */
+    int nx, nz, pml, R;
+} lay2;
+
+#define I2(L, i, j) ((size_t)((i) - (L).lo) + (size_t)((j) - (L).lo) * (size_t)(L).NX)
+
+static lay2 make_lay2(const ora_grid2d *g) {
+    lay2 L;
+    L.lo = 1 - g->pml - g->R;
+    L.NX = g->nx + 2 * (g->pml + g->R);
+    L.NZ = g->nz + 2 * (g->pml + g->R);
+    L.nx = g->nx; L.nz = g->nz; L.pml = g->pml; L.R = g->R;
+    return L;
+}
+
+typedef struct {
+    lay2 L;
+    real *bk, *rho, *vp;                       /* padded, edge-replicated */
+    float *axi, *bxi, *kxi, *axh, *bxh, *kxh;  /* index i+pml-1 */
+    float *azi, *bzi, *kzi, *azh, *bzh, *kzh;
+    real idx, idz;
+    float cf[8];
+} med2;
+
+typedef struct {
+    real *p, *vx, *vz;
+    real *m_dxvx, *m_dzvz, *m_dxp, *m_dzp;
+} fld2;
+
+static real *zalloc(size_t n) { return (real *)calloc(n, sizeof(real)); }
+
+static void fld2_alloc(fld2 *f, const lay2 *L) {
+    size_t n = (size_t)L->NX * L->NZ;
+    f->p = zalloc(n); f->vx = zalloc(n); f->vz = zalloc(n);
+    f->m_dxvx = zalloc(n); f->m_dzvz = zalloc(n); f->m_dxp = zalloc(n); f->m_dzp = zalloc(n);
+}
+static void fld2_free(fld2 *f) {
+    free(f->p); free(f->vx); free(f->vz);
+    free(f->m_dxvx); free(f->m_dzvz); free(f->m_dxp); free(f->m_dzp);
+}
+
+/* prepare_modeling (module_vars.f90:108-115): transpose the (nz,nx) z-fast
+ * model to x-fast, bk = rho*vp**2, then pad by edge replication (FLIT
+ * pad_array).  PML coefficients: compute_cfspml_damping_coef. */
+static void med2_init(med2 *M, const ora_grid2d *g, const float *vp_zfast, const float *rho_zfast,
+                      float f0max) {
+    lay2 L = make_lay2(g);
+    M->L = L;
+    size_t n = (size_t)L.NX * L.NZ;
+    M->bk = zalloc(n); M->rho = zalloc(n); M->vp = zalloc(n);
+    for (int j = L.lo; j < L.lo + L.NZ; ++j)
+        for (int i = L.lo; i < L.lo + L.NX; ++i) {
+            int ic = i < 1 ? 1 : (i > g->nx ? g->nx : i);
+            int jc = j < 1 ? 1 : (j > g->nz ? g->nz : j);
+            size_t s = (size_t)(jc - 1) + (size_t)(ic - 1) * (size_t)g->nz;
+            real v = (real)vp_zfast[s], r = (real)rho_zfast[s];
+            M->vp[I2(L, i, j)] = v;
+            M->rho[I2(L, i, j)] = r;
+            M->bk[I2(L, i, j)] = r * (v * v);
+        }
+    M->idx = (real)(1.0 / (double)g->dx);
+    M->idz = (real)(1.0 / (double)g->dz);
+#if !defined(ORA_F64)
+    M->idx = (float)(1.0 / (double)g->dx);
+    M->idz = (float)(1.0 / (double)g->dz);
+#endif
+    /* pmlvp: max of vp on the four edge lines (module_cfspml.f90:93) */
+    float pmlvp = 0.0f;
+    for (int j = 1; j <= g->nz; ++j) {
+        float a = vp_zfast[(size_t)(j - 1)];
+        float b = vp_zfast[(size_t)(j - 1) + (size_t)(g->nx - 1) * g->nz];
+        if (a > pmlvp) pmlvp = a;
+        if (b > pmlvp) pmlvp = b;
+    }
+    for (int i = 1; i <= g->nx; ++i) {
+        float a = vp_zfast[(size_t)(i - 1) * g->nz];
+        float b = vp_zfast[(size_t)(g->nz - 1) + (size_t)(i - 1) * g->nz];
+        if (a > pmlvp) pmlvp = a;
+        if (b > pmlvp) pmlvp = b;
+    }
+    int lx = g->nx + 2 * g->pml, lz = g->nz + 2 * g->pml;
+    float *buf = (float *)calloc((size_t)6 * (lx + lz), sizeof(float));
+    M->axi = buf; M->bxi = buf + lx; M->kxi = buf + 2 * lx;
+    M->axh = buf + 3 * lx; M->bxh = buf + 4 * lx; M->kxh = buf + 5 * lx;
+    float *bz = buf + 6 * lx;
+    M->azi = bz; M->bzi = bz + lz; M->kzi = bz + 2 * lz;
+    M->azh = bz + 3 * lz; M->bzh = bz + 4 * lz; M->kzh = bz + 5 * lz;
+    ORA_SYM(cfspml_axis)(g->nx, g->pml, g->dx, g->dt, pmlvp, f0max, M->axi, M->bxi, M->kxi,
+                         M->axh, M->bxh, M->kxh);
+    ORA_SYM(cfspml_axis)(g->nz, g->pml, g->dz, g->dt, pmlvp, f0max, M->azi, M->bzi, M->kzi,
+                         M->azh, M->bzh, M->kzh);
+    for (int m = 0; m < g->R; ++m) M->cf[m] = FDC[g->R][m];
+}
+
+static void med2_free(med2 *M) {
+    free(M->bk); free(M->rho); free(M->vp); free(M->axi);
+}
+
+/* Staggered differences: macro_fd_stencil.f90:20-183.  Forward difference
+ * at integer node i uses f(i+m) - f(i-m+1); accumulated left to right. */
+static inline real dplus_x(const real *f, const lay2 *L, const float *c, int R, int i, int j) {
+    size_t b = I2(*L, i, j);
+    real s = (real)c[0] * (f[b + 1] - f[b]);
+    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (f[b + m] - f[b - (m - 1)]);
+    return s;
+}
+static inline real dplus_z(const real *f, const lay2 *L, const float *c, int R, int i, int j) {
+    size_t b = I2(*L, i, j), st = (size_t)L->NX;
+    real s = (real)c[0] * (f[b + st] - f[b]);
+    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (f[b + m * st] - f[b - (m - 1) * st]);
+    return s;
+}
+
+/* update_wavefield (module_wavefield.f90:40-107) and its free-surface form
+ * update_wavefield_free_surface (:112-187).  Three separate sweeps, in
+ * place, OpenMP collapse(2) like the reference. */
+static void update2(const med2 *M, fld2 *F, real dt, int free_surface) {
+    const lay2 *L = &M->L;
+    const int nx = L->nx, nz = L->nz, pml = L->pml, R = L->R;
+    const float *c = M->cf;
+    const int j0p = free_surface ? 2 : -pml + 1;
+    /* p-sweep */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = j0p; j <= nz + pml; ++j)
+        for (int i = -pml + 1; i <= nx + pml; ++i) {
+            real pdxvx = M->idx * dplus_x(F->vx, L, c, R, i, j);
+            real pdzvz = M->idz * dplus_z(F->vz, L, c, R, i, j);
+            size_t q = I2(*L, i, j);
+            if (i <= 0 || i >= nx + 1) {
+                int a = i + pml - 1;
+                F->m_dxvx[q] = (real)M->axi[a] * pdxvx + (real)M->bxi[a] * F->m_dxvx[q];
+                pdxvx = (pdxvx + F->m_dxvx[q]) / (real)M->kxi[a];
+            }
+            int zp = free_surface ? (j >= nz + 1) : (j <= 0 || j >= nz + 1);
+            if (zp) {
+                int a = j + pml - 1;
+                F->m_dzvz[q] = (real)M->azi[a] * pdzvz + (real)M->bzi[a] * F->m_dzvz[q];
+                pdzvz = (pdzvz + F->m_dzvz[q]) / (real)M->kzi[a];
+            }
+            F->p[q] = F->p[q] - dt * M->bk[q] * (pdxvx + pdzvz);
+        }
+    if (free_surface) {
+        /* p(i, j) = -p(i, 2 - j) for j <= 0 (:145-151) */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int j = -pml + 1; j <= 0; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) F->p[I2(*L, i, j)] = -F->p[I2(*L, i, 2 - j)];
+    }
+    const int j0v = free_surface ? 1 : -pml + 1;
+    /* vx-sweep */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = j0v; j <= nz + pml; ++j)
+        for (int i = -pml; i <= nx + pml - 1; ++i) {
+            real pdxp = M->idx * dplus_x(F->p, L, c, R, i, j);
+            size_t q = I2(*L, i + 1, j);
+            int m = i + 1;
+            if (m <= 1 || m >= nx + 1) {
+                int a = m + pml - 1;
+                F->m_dxp[q] = (real)M->axh[a] * pdxp + (real)M->bxh[a] * F->m_dxp[q];
+                pdxp = (pdxp + F->m_dxp[q]) / (real)M->kxh[a];
+            }
+            real reff = (real)0.5 * (M->rho[I2(*L, i + 1, j)] + M->rho[I2(*L, i, j)]);
+            F->vx[q] = F->vx[q] - dt / reff * pdxp;
+        }
+    /* vz-sweep */
+    const int j0z = free_surface ? 1 : -pml;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = j0z; j <= nz + pml - 1; ++j)
+        for (int i = -pml + 1; i <= nx + pml; ++i) {
+            real pdzp = M->idz * dplus_z(F->p, L, c, R, i, j);
+            size_t q = I2(*L, i, j + 1);
+            int m = j + 1;
+            int zp = free_surface ? (m >= nz + 1) : (m <= 1 || m >= nz + 1);
+            if (zp) {
+                int a = m + pml - 1;
+                F->m_dzp[q] = (real)M->azh[a] * pdzp + (real)M->bzh[a] * F->m_dzp[q];
+                pdzp = (pdzp + F->m_dzp[q]) / (real)M->kzh[a];
+            }
+            real reff = (real)0.5 * (M->rho[I2(*L, i, j + 1)] + M->rho[I2(*L, i, j)]);
+            F->vz[q] = F->vz[q] - dt / reff * pdzp;
+        }
+}
+
+/* add_source: module_wavefield.f90:192-263 */
+static void add_source2(const med2 *M, fld2 *F, const ora_source2d *src, const ora_point *pts,
+                        int ns, int t, real dt, int free_surface) {
+    const lay2 *L = &M->L;
+    for (int k = 0; k < ns; ++k) {
+        const ora_source2d *s = &src[k];
+        const ora_point *P = &pts[k];
+        int nbeg = fnint(s->t0 / (float)dt) + 1;
+        int nend = nbeg + s->nt_stf - 1;
+        if (t < nbeg || t > nend) continue;
+        real amp = s->stf[t - nbeg] * (real)s->amp * dt;
+        if (s->mechanism == 1) {
+            real sp = (real)sin((double)s->polar), cp = (real)cos((double)s->polar);
+#if !defined(ORA_F64)
+            sp = sinf(s->polar); cp = cosf(s->polar);
+#endif
+            real rx = M->rho[I2(*L, P->hx, P->gz)];
+            real rz = M->rho[I2(*L, P->gx, P->hz)];
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    if (!free_surface || P->gz + irz >= 2) {
+                        size_t q = I2(*L, P->hx + irx, P->gz + irz);
+                        F->vx[q] = F->vx[q] + sp * amp / rx * (real)P->hxw[irx + NKW] * (real)P->iz[irz + NKW];
+                    }
+                    if (!free_surface || P->hz + irz >= 2) {
+                        size_t q = I2(*L, P->gx + irx, P->hz + irz);
+                        F->vz[q] = F->vz[q] + cp * amp / rz * (real)P->ix[irx + NKW] * (real)P->hzw[irz + NKW];
+                    }
+                }
+        } else {
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    if (!free_surface || P->gz + irz >= 2) {
+                        size_t q = I2(*L, P->gx + irx, P->gz + irz);
+                        F->p[q] = F->p[q] + amp * (real)P->ix[irx + NKW] * (real)P->iz[irz + NKW];
+                    }
+                }
+        }
+    }
+}
+
+/* Boundary strips: module_boundary_saving.f90:88-153.  Record layout is the
+ * Fortran column-major order of the four array sections. */
+static size_t strip_len2(const lay2 *L) {
+    size_t R = (size_t)L->R, nx = (size_t)L->nx, nz = (size_t)L->nz;
+    return 2 * R * (nz + 2 * R) + 2 * (nx + 2 * R) * R;
+}
+
+static void strips2(const lay2 *L, fld2 *F, real *rec, int save) {
+    const int R = L->R, nx = L->nx, nz = L->nz;
+    size_t k = 0;
+#define XFER(arr, i, j)                                 \
+    do {                                                \
+        size_t q_ = I2(*L, i, j);                       \
+        if (save) rec[k] = (arr)[q_];                   \
+        else (arr)[q_] = rec[k];                        \
+        ++k;                                            \
+    } while (0)
+    for (int j = 1 - R; j <= nz + R; ++j)
+        for (int i = 2 - R; i <= 1; ++i) XFER(F->vx, i, j);
+    for (int j = 1 - R; j <= nz + R; ++j)
+        for (int i = nx + 1; i <= nx + R; ++i) XFER(F->vx, i, j);
+    for (int j = 2 - R; j <= 1; ++j)
+        for (int i = 1 - R; i <= nx + R; ++i) XFER(F->vz, i, j);
+    for (int j = nz + 1; j <= nz + R; ++j)
+        for (int i = 1 - R; i <= nx + R; ++i) XFER(F->vz, i, j);
+#undef XFER
+}
+
+/* Receiver gather: submodule_forward_modeling.f90:58-79 */
+static void record2(const lay2 *L, const real *p, const ora_receiver2d *rec, const ora_point *rp,
+                    int nr, int t, int nt, real *traces) {
+#pragma omp parallel for schedule(static)
+    for (int ir = 0; ir < nr; ++ir) {
+        if (rec[ir].weight == 0.0f) continue;
+        const ora_point *P = &rp[ir];
+        real acc = traces[(size_t)ir * nt + (t - 1)];
+        for (int irz = -NKW; irz <= NKW; ++irz)
+            for (int irx = -NKW; irx <= NKW; ++irx)
+                acc = acc + p[I2(*L, P->gx + irx, P->gz + irz)] * (real)P->ix[irx + NKW] *
+                                (real)P->iz[irz + NKW] * (real)rec[ir].weight;
+        traces[(size_t)ir * nt + (t - 1)] = acc;
+    }
+}
+
+
+static float f0max_of(const float *f0s, int ns) {
+    float m = 0.0f;
+    for (int k = 0; k < ns; ++k) if (f0s[k] > m) m = f0s[k];
+    return m;
+}
+
+/* Forward modelling: compute_forward, submodule_forward_modeling.f90:31-132.
+ * traces: [nr][nt] (receiver-major), zero-initialised by this call.
+ * strips (optional): [nt][strip_len] boundary records, saved at the start
+ * of each step t (reconstruct = .true.).  final_state (optional): vx, vz, p
+ * full padded arrays at the start of step nt (output_final_step_wavefield).
+ * snap_t/snap_p (optional): p interior (nz,nx) z-fast at the start of step
+ * snap_t (before the update), like the snapshot output at :81-92. */
+int ORA_SYM(acoustic2d_forward)(const ora_grid2d *g, const float *vp, const float *rho,
+                                int ns, const ora_source2d *src, const float *f0s,
+                                int nr, const ora_receiver2d *rec, real *traces,
+                                real *strips, real *final_state) {
+    if (g->R < 2 || g->R > 8) return -1;
+    med2 M;
+    med2_init(&M, g, vp, rho, f0max_of(f0s, ns));
+    const lay2 *L = &M.L;
+    fld2 F;
+    fld2_alloc(&F, L);
+    ora_point *sp = (ora_point *)calloc(ns > 0 ? ns : 1, sizeof(ora_point));
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    for (int k = 0; k < ns; ++k)
+        ORA_SYM(locate_point)(2, src[k].x, 0.0f, src[k].z, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, &sp[k]);
+    for (int k = 0; k < nr; ++k)
+        ORA_SYM(locate_point)(2, rec[k].x, 0.0f, rec[k].z, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, &rp[k]);
+    memset(traces, 0, sizeof(real) * (size_t)nr * g->nt);
+    size_t sl = strip_len2(L), n = (size_t)L->NX * L->NZ;
+    const real dt = (real)g->dt;
+    for (int t = 1; t <= g->nt; ++t) {
+        if (strips) {
+            strips2(L, &F, strips + (size_t)(t - 1) * sl, 1);
+            if (t == g->nt && final_state) {
+                memcpy(final_state, F.vx, n * sizeof(real));
+                memcpy(final_state + n, F.vz, n * sizeof(real));
+                memcpy(final_state + 2 * n, F.p, n * sizeof(real));
+            }
+        }
+        record2(L, F.p, rec, rp, nr, t, g->nt, traces);
+        update2(&M, &F, dt, g->free_surface);
+        add_source2(&M, &F, src, sp, ns, t, dt, g->free_surface);
+    }
+    free(sp); free(rp);
+    fld2_free(&F);
+    med2_free(&M);
+    return 0;
+}
+
+/* OWL's gradient: compute_adjoint + compute_gradient,
+ * acoustic_iso_2d/submodule_fwi_gradient.f90:31-269 (energy preconditioning
+ * off).  Re-runs the forward with boundary saving, then the reverse loop
+ * t = nt .. hnt.  adj: [nr][nt] adjoint source traces (syn - obs).
+ * grad_vp, grad_rho: (nz, nx) z-fast outputs = the transposed .grd arrays
+ * (:187-195), NaN/Inf mapped to zero (FLIT return_normal).
+ * traces_out (optional): the forward traces of the same run. */
+int ORA_SYM(acoustic2d_gradient_owl)(const ora_grid2d *g, const float *vp, const float *rho,
+                                     int ns, const ora_source2d *src, const float *f0s, int hnt,
+                                     int nr, const ora_receiver2d *rec, const real *adj,
+                                     real *grad_vp, real *grad_rho, real *traces_out) {
+    if (g->R < 2 || g->R > 8 || ns < 1) return -1;
+    lay2 L0;
+    {
+        ora_grid2d gg = *g;
+        L0 = make_lay2(&gg);
+    }
+    size_t n = (size_t)L0.NX * L0.NZ, sl = strip_len2(&L0);
+    real *strips = zalloc(sl * (size_t)g->nt);
+    real *final_state = zalloc(3 * n);
+    real *tr = traces_out ? traces_out : zalloc((size_t)nr * g->nt);
+    int rc = ORA_SYM(acoustic2d_forward)(g, vp, rho, ns, src, f0s, nr, rec, tr, strips, final_state);
+    if (!traces_out) free(tr);
+    if (rc) { free(strips); free(final_state); return rc; }
+
+    med2 M;
+    med2_init(&M, g, vp, rho, f0max_of(f0s, ns));
+    const lay2 *L = &M.L;
+    fld2 F, A;
+    fld2_alloc(&F, L);
+    fld2_alloc(&A, L);
+    real *prev_p = zalloc(n), *prev_vx = zalloc(n), *prev_vz = zalloc(n);
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    for (int k = 0; k < nr; ++k)
+        ORA_SYM(locate_point)(2, rec[k].x, 0.0f, rec[k].z, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, &rp[k]);
+    const int nx = g->nx, nz = g->nz;
+    real *gv = zalloc((size_t)nx * nz), *gr = zalloc((size_t)nx * nz);
+    const real mdt = -(real)g->dt;
+    const int cc = g->cc_step_interval > 0 ? g->cc_step_interval : 1;
+    for (int t = g->nt; t >= hnt; --t) {
+        memcpy(prev_p, F.p, n * sizeof(real));
+        memcpy(prev_vx, F.vx, n * sizeof(real));
+        memcpy(prev_vz, F.vz, n * sizeof(real));
+        if (t == g->nt) {
+            memcpy(F.vx, final_state, n * sizeof(real));
+            memcpy(F.vz, final_state + n, n * sizeof(real));
+            memcpy(F.p, final_state + 2 * n, n * sizeof(real));
+        }
+        strips2(L, &F, strips + (size_t)(t - 1) * sl, 0);
+        update2(&M, &F, mdt, g->free_surface);
+        update2(&M, &A, mdt, g->free_surface);
+        /* adjoint source: :126-147, receivers serialised */
+        for (int ir = 0; ir < nr; ++ir) {
+            if (rec[ir].weight == 0.0f) continue;
+            const ora_point *P = &rp[ir];
+            real a = adj[(size_t)ir * g->nt + (t - 1)];
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    size_t q = I2(*L, P->gx + irx, P->gz + irz);
+                    A.p[q] = A.p[q] + a * (real)P->ix[irx + NKW] * (real)P->iz[irz + NKW] * (real)rec[ir].weight;
+                }
+        }
+        if (t % cc == 0) {
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int j = 1; j <= nz; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    size_t q = I2(*L, i, j);
+                    size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+                    real v = M.vp[q];
+                    real sp_ = F.p[q] - prev_p[q];
+                    real rp_ = A.p[q];
+                    gv[o] = gv[o] - sp_ * rp_ / (M.rho[q] * (v * v * v));
+                    real sx = (F.vx[q] + F.vx[q + 1]) - (prev_vx[q] + prev_vx[q + 1]);
+                    real rx = A.vx[q] + A.vx[q + 1];
+                    real sz = (F.vz[q] + F.vz[q + L->NX]) - (prev_vz[q] + prev_vz[q + L->NX]);
+                    real rz = A.vz[q] + A.vz[q + L->NX];
+                    gr[o] = gr[o] + sx * rx + sz * rz;
+                }
+        }
+    }
+    for (int j = 1; j <= nz; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            real a = gv[(size_t)(i - 1) + (size_t)(j - 1) * nx];
+            real b = gr[(size_t)(i - 1) + (size_t)(j - 1) * nx];
+            if (!isfinite(a)) a = 0;
+            if (!isfinite(b)) b = 0;
+            grad_vp[(size_t)(j - 1) + (size_t)(i - 1) * nz] = a;
+            grad_rho[(size_t)(j - 1) + (size_t)(i - 1) * nz] = b;
+        }
+    free(gv); free(gr); free(rp);
+    free(prev_p); free(prev_vx); free(prev_vz);
+    fld2_free(&F); fld2_free(&A);
+    med2_free(&M);
+    free(strips); free(final_state);
+    return 0;
+}
+
+/* L2 waveform misfit and adjoint source: compute_adjsrc_waveform,
+ * module_inversion_adjoint_source.f90:371-383, with the trace-discard rule
+ * of compute_adjoint_source (:1042-1054, threshold 1e-6).  syn, obs, adj:
+ * [nr][nt].  Returns the misfit sum (:1167), accumulated in double. */
+double ORA_SYM(adjoint_source_l2)(int nr, int nt, const float *weight, const real *syn,
+                                  const real *obs, real *adj, float threshold) {
+    double *no = (double *)calloc(nr > 0 ? nr : 1, sizeof(double));
+    double *nsy = (double *)calloc(nr > 0 ? nr : 1, sizeof(double));
+    double mo = 0, ms = 0;
+    for (int i = 0; i < nr; ++i) {
+        double a = 0, b = 0;
+        for (int t = 0; t < nt; ++t) {
+            a += (double)obs[(size_t)i * nt + t] * obs[(size_t)i * nt + t];
+            b += (double)syn[(size_t)i * nt + t] * syn[(size_t)i * nt + t];
+        }
+        no[i] = sqrt(a); nsy[i] = sqrt(b);
+        if (no[i] > mo) mo = no[i];
+        if (nsy[i] > ms) ms = nsy[i];
+    }
+    double mis = 0;
+    for (int i = 0; i < nr; ++i) {
+        int keep = !(weight[i] == 0.0f || no[i] < threshold * mo || nsy[i] < threshold * ms ||
+                     no[i] * nsy[i] == 0.0);
+        for (int t = 0; t < nt; ++t) {
+            size_t q = (size_t)i * nt + t;
+            if (keep) {
+                real d = obs[q] - syn[q];
+                mis += (double)d * (double)d;
+                adj[q] = syn[q] - obs[q];
+            } else {
+                adj[q] = 0;
+            }
+        }
+    }
+    free(no); free(nsy);
+    return mis;
+}
+
+/* ======================================================================== */
+/* 3D acoustic forward: acoustic_iso_3d/module_wavefield.f90:40-172 with no */
+/* domain decomposition (nx1 = -pml+1 .. nx2 = nx+pml, etc.), separate     */
+/* min/max memory arrays folded into one array per derivative (they never */
+/* overlap).  Explosion and force sources: add_source :322-444.            */
+/* ======================================================================== */
+typedef struct {
+    int nx, ny, nz;
+    float dx, dy, dz, ox, oy, oz;
+    float dt;
+    int nt, pml, R;
+} ora_grid3d;
+
+typedef struct {
+    float x, y, z, amp, t0;
+    int nt_stf;
+    const real *stf;
+} ora_source3d;
+
+typedef struct {
+    float x, y, z, weight;
+} ora_receiver3d;
+
+#define I3(i, j, k) \
+    ((size_t)((i) - lo) + (size_t)((j) - lo) * (size_t)NX + (size_t)((k) - lo) * (size_t)NX * (size_t)NY)
+
+int ORA_SYM(acoustic3d_forward)(const ora_grid3d *g, const float *vp, const float *rho, int ns,
+                                const ora_source3d *src, float f0max, int nr,
+                                const ora_receiver3d *rec, real *traces) {
+    const int R = g->R, pml = g->pml, nx = g->nx, ny = g->ny, nz = g->nz;
+    if (R < 2 || R > 8) return -1;
+    const int lo = 1 - pml - R;
+    const int NX = nx + 2 * (pml + R), NY = ny + 2 * (pml + R), NZ = nz + 2 * (pml + R);
+    const size_t n = (size_t)NX * NY * NZ;
+    real *bk = zalloc(n), *rh = zalloc(n);
+    /* input models are (nz, ny, nx) z-fastest (Fortran order); permute(...,321) */
+    for (int k = lo; k < lo + NZ; ++k)
+        for (int j = lo; j < lo + NY; ++j)
+            for (int i = lo; i < lo + NX; ++i) {
+                int ic = i < 1 ? 1 : (i > nx ? nx : i);
+                int jc = j < 1 ? 1 : (j > ny ? ny : j);
+                int kc = k < 1 ? 1 : (k > nz ? nz : k);
+                size_t s = (size_t)(kc - 1) + (size_t)(jc - 1) * nz + (size_t)(ic - 1) * nz * ny;
+                real v = (real)vp[s], r = (real)rho[s];
+                rh[I3(i, j, k)] = r;
+                bk[I3(i, j, k)] = r * (v * v);
+            }
+    float pmlvp = 0.0f;
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j)
+            for (int k = 1; k <= nz; ++k)
+                if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
+                    float v = vp[(size_t)(k - 1) + (size_t)(j - 1) * nz + (size_t)(i - 1) * nz * ny];
+                    if (v > pmlvp) pmlvp = v;
+                }
+    int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    float *ax = (float *)calloc(6 * (size_t)(lx + ly + lz), sizeof(float));
+    float *axi = ax, *bxi = ax + lx, *kxi = ax + 2 * lx, *axh = ax + 3 * lx, *bxh = ax + 4 * lx, *kxh = ax + 5 * lx;
+    float *ay = ax + 6 * lx;
+    float *ayi = ay, *byi = ay + ly, *kyi = ay + 2 * ly, *ayh = ay + 3 * ly, *byh = ay + 4 * ly, *kyh = ay + 5 * ly;
+    float *az = ay + 6 * ly;
+    float *azi = az, *bzi = az + lz, *kzi = az + 2 * lz, *azh = az + 3 * lz, *bzh = az + 4 * lz, *kzh = az + 5 * lz;
+    ORA_SYM(cfspml_axis)(nx, pml, g->dx, g->dt, pmlvp, f0max, axi, bxi, kxi, axh, bxh, kxh);
+    ORA_SYM(cfspml_axis)(ny, pml, g->dy, g->dt, pmlvp, f0max, ayi, byi, kyi, ayh, byh, kyh);
+    ORA_SYM(cfspml_axis)(nz, pml, g->dz, g->dt, pmlvp, f0max, azi, bzi, kzi, azh, bzh, kzh);
+    real *p = zalloc(n), *vx = zalloc(n), *vy = zalloc(n), *vz = zalloc(n);
+    real *mvx = zalloc(n), *mvy = zalloc(n), *mvz = zalloc(n);
+    real *mpx = zalloc(n), *mpy = zalloc(n), *mpz = zalloc(n);
+#if defined(ORA_F64)
+    const real idx = 1.0 / (double)g->dx, idy = 1.0 / (double)g->dy, idz = 1.0 / (double)g->dz;
+#else
+    const real idx = (float)(1.0 / (double)g->dx), idy = (float)(1.0 / (double)g->dy),
+               idz = (float)(1.0 / (double)g->dz);
+#endif
+    const float *c = FDC[R];
+    ora_point *sp = (ora_point *)calloc(ns > 0 ? ns : 1, sizeof(ora_point));
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    for (int s = 0; s < ns; ++s)
+        ORA_SYM(locate_point)(3, src[s].x, src[s].y, src[s].z, g->ox, g->oy, g->oz, g->dx, g->dy, g->dz, &sp[s]);
+    for (int r = 0; r < nr; ++r)
+        ORA_SYM(locate_point)(3, rec[r].x, rec[r].y, rec[r].z, g->ox, g->oy, g->oz, g->dx, g->dy, g->dz, &rp[r]);
+    memset(traces, 0, sizeof(real) * (size_t)nr * g->nt);
+    const size_t sx = 1, sy = (size_t)NX, sz = (size_t)NX * NY;
+    const real dt = (real)g->dt;
+    for (int t = 1; t <= g->nt; ++t) {
+        /* receivers (submodule_forward_modeling.f90 in 3D) */
+#pragma omp parallel for schedule(static)
+        for (int ir = 0; ir < nr; ++ir) {
+            if (rec[ir].weight == 0.0f) continue;
+            const ora_point *P = &rp[ir];
+            real acc = traces[(size_t)ir * g->nt + (t - 1)];
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        acc = acc + p[I3(P->gx + irx, P->gy + iry, P->gz + irz)] * (real)P->ix[irx + NKW] *
+                                        (real)P->iy[iry + NKW] * (real)P->iz[irz + NKW] * (real)rec[ir].weight;
+            traces[(size_t)ir * g->nt + (t - 1)] = acc;
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = -pml + 1; k <= nz + pml; ++k)
+            for (int j = -pml + 1; j <= ny + pml; ++j)
+                for (int i = -pml + 1; i <= nx + pml; ++i) {
+                    size_t b = I3(i, j, k);
+                    real s1 = (real)c[0] * (vx[b + sx] - vx[b]);
+                    real s2 = (real)c[0] * (vy[b + sy] - vy[b]);
+                    real s3 = (real)c[0] * (vz[b + sz] - vz[b]);
+                    for (int m = 2; m <= R; ++m) {
+                        s1 = s1 + (real)c[m - 1] * (vx[b + m * sx] - vx[b - (m - 1) * sx]);
+                        s2 = s2 + (real)c[m - 1] * (vy[b + m * sy] - vy[b - (m - 1) * sy]);
+                        s3 = s3 + (real)c[m - 1] * (vz[b + m * sz] - vz[b - (m - 1) * sz]);
+                    }
+                    real dxv = idx * s1, dyv = idy * s2, dzv = idz * s3;
+                    if (i <= 0 || i >= nx + 1) {
+                        int a = i + pml - 1;
+                        mvx[b] = (real)axi[a] * dxv + (real)bxi[a] * mvx[b];
+                        dxv = (dxv + mvx[b]) / (real)kxi[a];
+                    }
+                    if (j <= 0 || j >= ny + 1) {
+                        int a = j + pml - 1;
+                        mvy[b] = (real)ayi[a] * dyv + (real)byi[a] * mvy[b];
+                        dyv = (dyv + mvy[b]) / (real)kyi[a];
+                    }
+                    if (k <= 0 || k >= nz + 1) {
+                        int a = k + pml - 1;
+                        mvz[b] = (real)azi[a] * dzv + (real)bzi[a] * mvz[b];
+                        dzv = (dzv + mvz[b]) / (real)kzi[a];
+                    }
+                    p[b] = p[b] - dt * bk[b] * (dxv + dyv + dzv);
+                }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = -pml + 1; k <= nz + pml; ++k)
+            for (int j = -pml + 1; j <= ny + pml; ++j)
+                for (int i = -pml; i <= nx + pml - 1; ++i) {
+                    size_t b = I3(i, j, k);
+                    real s = (real)c[0] * (p[b + sx] - p[b]);
+                    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (p[b + m * sx] - p[b - (m - 1) * sx]);
+                    real d = idx * s;
+                    int m1 = i + 1;
+                    if (m1 <= 1 || m1 >= nx + 1) {
+                        int a = m1 + pml - 1;
+                        mpx[b + sx] = (real)axh[a] * d + (real)bxh[a] * mpx[b + sx];
+                        d = (d + mpx[b + sx]) / (real)kxh[a];
+                    }
+                    real reff = (real)0.5 * (rh[b + sx] + rh[b]);
+                    vx[b + sx] = vx[b + sx] - dt / reff * d;
+                }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = -pml + 1; k <= nz + pml; ++k)
+            for (int j = -pml; j <= ny + pml - 1; ++j)
+                for (int i = -pml + 1; i <= nx + pml; ++i) {
+                    size_t b = I3(i, j, k);
+                    real s = (real)c[0] * (p[b + sy] - p[b]);
+                    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (p[b + m * sy] - p[b - (m - 1) * sy]);
+                    real d = idy * s;
+                    int m1 = j + 1;
+                    if (m1 <= 1 || m1 >= ny + 1) {
+                        int a = m1 + pml - 1;
+                        mpy[b + sy] = (real)ayh[a] * d + (real)byh[a] * mpy[b + sy];
+                        d = (d + mpy[b + sy]) / (real)kyh[a];
+                    }
+                    real reff = (real)0.5 * (rh[b + sy] + rh[b]);
+                    vy[b + sy] = vy[b + sy] - dt / reff * d;
+                }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = -pml; k <= nz + pml - 1; ++k)
+            for (int j = -pml + 1; j <= ny + pml; ++j)
+                for (int i = -pml + 1; i <= nx + pml; ++i) {
+                    size_t b = I3(i, j, k);
+                    real s = (real)c[0] * (p[b + sz] - p[b]);
+                    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (p[b + m * sz] - p[b - (m - 1) * sz]);
+                    real d = idz * s;
+                    int m1 = k + 1;
+                    if (m1 <= 1 || m1 >= nz + 1) {
+                        int a = m1 + pml - 1;
+                        mpz[b + sz] = (real)azh[a] * d + (real)bzh[a] * mpz[b + sz];
+                        d = (d + mpz[b + sz]) / (real)kzh[a];
+                    }
+                    real reff = (real)0.5 * (rh[b + sz] + rh[b]);
+                    vz[b + sz] = vz[b + sz] - dt / reff * d;
+                }
+        /* explosion sources (add_source, acoustic_iso_3d/module_wavefield.f90:322-444) */
+        for (int s = 0; s < ns; ++s) {
+            int nbeg = fnint(src[s].t0 / g->dt) + 1, nend = nbeg + src[s].nt_stf - 1;
+            if (t < nbeg || t > nend) continue;
+            real amp = src[s].stf[t - nbeg] * (real)src[s].amp * dt;
+            const ora_point *P = &sp[s];
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx) {
+                        size_t q = I3(P->gx + irx, P->gy + iry, P->gz + irz);
+                        p[q] = p[q] + amp * (real)P->ix[irx + NKW] * (real)P->iy[iry + NKW] * (real)P->iz[irz + NKW];
+                    }
+        }
+    }
+    free(sp); free(rp); free(ax);
+    free(p); free(vx); free(vy); free(vz);
+    free(mvx); free(mvy); free(mvz); free(mpx); free(mpy); free(mpz);
+    free(bk); free(rh);
+    return 0;
+}

@@ -1,0 +1,1301 @@
+// 2D acoustic SSG-FD time stepping and FWI gradient for MI355X (gfx950).
+//
+// The scheme is OWL's acoustic_iso_2d solver: pressure p on integer nodes,
+// vx / vz on half nodes, optimised staggered coefficients of half-length R,
+// CFS-PML memory variables in boundary slabs, 9x9 Kaiser-windowed sinc
+// source injection and receiver sampling, and OWL's boundary-saving
+// gradient (acoustic_iso_2d/module_wavefield.f90:40-263,
+// submodule_forward_modeling.f90:31-132, submodule_fwi_gradient.f90:31-269).
+//
+// All shots of a batch live in one set of device buffers; every kernel
+// launch covers every shot, each shot on its own window.  Arithmetic is
+// FP32 in the reference's operation order (built with -ffp-contract=off),
+// gradients and misfits accumulate in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "host_prep.hpp"
+#include "owl_amd.h"
+
+namespace owl {
+namespace a2 {
+
+constexpr int kBX = 64;     // threads along x (one wavefront per row)
+constexpr int kBY = 4;      // rows of threads per block
+constexpr int kRows = 16;   // z rows per block (kRows / kBY per thread)
+constexpr int kPitchAlign = 64;  // row pitch multiple, floats (256 B)
+
+struct Coef {
+    float c[8];
+};
+
+// Device-side description of one shot.
+struct Shot {
+    int nx, nz;           // interior window
+    int pitch;            // floats per padded row
+    int nrow;             // padded rows
+    long long off;        // element offset of node (lo, lo) in every field buffer
+    int tile0, ntx;       // first update tile, tiles per row
+    int prof;             // offset of the shot's PML profiles
+    int nr;               // receivers
+    int rec0;             // first receiver of the shot in the receiver tables
+    int inj0, ninj;       // source injection cells
+    int adj0, nadj;       // adjoint-source injection cells
+    long long strip0;     // offset of the shot's boundary records
+    long long trace0;     // offset of the shot's [nt][nr] trace block
+    long long g0;         // offset of the shot's window gradient (nx*nz)
+    int edge0;            // offset of the shot's saved edge values (2*nz + 2*nx)
+    int nxbeg, nzbeg;     // window origin in the full model (1-based)
+};
+
+struct FieldSet {
+    const float* p_in;
+    float* p_out;
+    const float* vx_in;
+    float* vx_out;
+    const float* vz_in;
+    float* vz_out;
+    float* m_dxvx;
+    float* m_dzvz;
+    float* m_dxp;
+    float* m_dzp;
+};
+struct Sets {
+    FieldSet s[2];
+};
+
+__device__ __forceinline__ int shot_of_tile(const int* tile_shot, int tile) { return tile_shot[tile]; }
+
+// ---------------------------------------------------------------------------
+// p-sweep: module_wavefield.f90:51-71 (and :123-143 with a free surface).
+// p = p - dt*bk*(dx+ vx + dz+ vz), PML memory update where i <= 0 or
+// i >= nx+1 (resp. j).  Out of place when p_out != p_in.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(kBX * kBY) void k_update_p(const Shot* __restrict__ shots,
+                                                        const int* __restrict__ tile_shot, Sets sets,
+                                                        const float* __restrict__ bk,
+                                                        const float* __restrict__ prof, Coef c,
+                                                        float dt, float idx, float idz, int pml,
+                                                        int free_surface) {
+    const int tile = blockIdx.x;
+    const Shot S = shots[shot_of_tile(tile_shot, tile)];
+    const FieldSet F = sets.s[blockIdx.y];
+    const int lt = tile - S.tile0;
+    const int tx = lt % S.ntx, tz = lt / S.ntx;
+    const int i = -pml + 1 + tx * kBX + int(threadIdx.x);
+    const int nx = S.nx, nz = S.nz;
+    if (i > nx + pml) return;
+    const int lo = 1 - pml - R;
+    const int lx = nx + 2 * pml, lz = nz + 2 * pml;
+    const float* axi = prof + S.prof;
+    const float* bxi = axi + lx;
+    const float* azi = prof + S.prof + 4 * lx;
+    const float* bzi = azi + lz;
+    const bool xslab = (i <= 0 || i >= nx + 1);
+    float ax = 0.f, bx = 0.f;
+    if (xslab) {
+        ax = axi[i + pml - 1];
+        bx = bxi[i + pml - 1];
+    }
+    const size_t pitch = size_t(S.pitch);
+#pragma unroll
+    for (int k = 0; k < kRows / kBY; ++k) {
+        const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
+        if (j > nz + pml) break;
+        if (free_surface && j < 2) continue;
+        const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
+        float sx = c.c[0] * (F.vx_in[q + 1] - F.vx_in[q]);
+        float sz = c.c[0] * (F.vz_in[q + pitch] - F.vz_in[q]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) {
+            sx = sx + c.c[m - 1] * (F.vx_in[q + m] - F.vx_in[q - (m - 1)]);
+            sz = sz + c.c[m - 1] * (F.vz_in[q + m * pitch] - F.vz_in[q - (m - 1) * pitch]);
+        }
+        float pdx = idx * sx;
+        float pdz = idz * sz;
+        if (xslab) {
+            const float mem = ax * pdx + bx * F.m_dxvx[q];
+            F.m_dxvx[q] = mem;
+            pdx = pdx + mem;  // division by kappa = 1 is exact and omitted
+        }
+        const bool zslab = free_surface ? (j >= nz + 1) : (j <= 0 || j >= nz + 1);
+        if (zslab) {
+            const float mem = azi[j + pml - 1] * pdz + bzi[j + pml - 1] * F.m_dzvz[q];
+            F.m_dzvz[q] = mem;
+            pdz = pdz + mem;
+        }
+        F.p_out[q] = F.p_in[q] - dt * bk[q] * (pdx + pdz);
+    }
+}
+
+// Free-surface mirror p(i, j) = -p(i, 2 - j), j <= 0 (module_wavefield.f90:145-151).
+template <int R>
+__global__ void k_mirror_p(const Shot* __restrict__ shots, Sets sets, int pml) {
+    const Shot S = shots[blockIdx.y];
+    float* p = sets.s[blockIdx.z].p_out;
+    const int lo = 1 - pml - R;
+    const int lx = S.nx + 2 * pml;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < lx * pml; e += gridDim.x * blockDim.x) {
+        const int i = -pml + 1 + e % lx;
+        const int j = -pml + 1 + e / lx;  // j in [-pml+1, 0]
+        const size_t pitch = size_t(S.pitch);
+        p[size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch] =
+            -p[size_t(S.off) + size_t(i - lo) + size_t(2 - j - lo) * pitch];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// v-sweeps: module_wavefield.f90:73-105 (and :153-185 with a free surface).
+// vx(m) = vx(m) - dt/(0.5*(rho(m)+rho(m-1))) * dx- p, PML where m <= 1 or
+// m >= nx+1; vz likewise along z.  One thread updates vx and vz at (i, j).
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__ shots,
+                                                        const int* __restrict__ tile_shot, Sets sets,
+                                                        const float* __restrict__ rho,
+                                                        const float* __restrict__ prof, Coef c,
+                                                        float dt, float idx, float idz, int pml,
+                                                        int free_surface) {
+    const int tile = blockIdx.x;
+    const Shot S = shots[shot_of_tile(tile_shot, tile)];
+    const FieldSet F = sets.s[blockIdx.y];
+    const int lt = tile - S.tile0;
+    const int tx = lt % S.ntx, tz = lt / S.ntx;
+    const int i = -pml + 1 + tx * kBX + int(threadIdx.x);
+    const int nx = S.nx, nz = S.nz;
+    if (i > nx + pml) return;
+    const int lo = 1 - pml - R;
+    const int lx = nx + 2 * pml, lz = nz + 2 * pml;
+    const float* axh = prof + S.prof + 2 * lx;
+    const float* bxh = axh + lx;
+    const float* azh = prof + S.prof + 4 * lx + 2 * lz;
+    const float* bzh = azh + lz;
+    const float* p = F.p_out;  // the p just updated
+    const bool xslab = (i <= 1 || i >= nx + 1);
+    float ax = 0.f, bx = 0.f;
+    if (xslab) {
+        ax = axh[i + pml - 1];
+        bx = bxh[i + pml - 1];
+    }
+    const size_t pitch = size_t(S.pitch);
+#pragma unroll
+    for (int k = 0; k < kRows / kBY; ++k) {
+        const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
+        if (j > nz + pml) break;
+        const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
+        if (!free_surface || j >= 1) {
+            // vx at half node m = i: difference p(m-1+k) - p(m-k)
+            float s = c.c[0] * (p[q] - p[q - 1]);
+#pragma unroll
+            for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (p[q + (m - 1)] - p[q - m]);
+            float d = idx * s;
+            if (xslab) {
+                const float mem = ax * d + bx * F.m_dxp[q];
+                F.m_dxp[q] = mem;
+                d = d + mem;
+            }
+            const float reff = 0.5f * (rho[q] + rho[q - 1]);
+            F.vx_out[q] = F.vx_in[q] - dt / reff * d;
+        }
+        if (!free_surface || j >= 2) {
+            float s = c.c[0] * (p[q] - p[q - pitch]);
+#pragma unroll
+            for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (p[q + (m - 1) * pitch] - p[q - m * pitch]);
+            float d = idz * s;
+            const bool zslab = free_surface ? (j >= nz + 1) : (j <= 1 || j >= nz + 1);
+            if (zslab) {
+                const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * F.m_dzp[q];
+                F.m_dzp[q] = mem;
+                d = d + mem;
+            }
+            const float reff = 0.5f * (rho[q] + rho[q - pitch]);
+            F.vz_out[q] = F.vz_in[q] - dt / reff * d;
+        }
+    }
+}
+
+// One tap of a point source or receiver: field cell and two weights.
+struct Tap {
+    long long cell;  // absolute element offset
+    float w1, w2;
+    int row;         // amplitude-table row (sources) or receiver index (adjoint)
+    int field;       // 0 p, 1 vx, 2 vz
+    int tbeg, tend;  // steps during which a source tap is active (nbeg..nend)
+};
+
+// ---------------------------------------------------------------------------
+// After the update of step t: add_source(t) (module_wavefield.f90:192-263),
+// then sample the receivers for step t+1
+// (submodule_forward_modeling.f90:58-79) and save the boundary record of
+// step t+1 (module_boundary_saving.f90:119-133).  One block per shot, so
+// the samples see the injected source.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(1024) void k_post_step(
+    const Shot* __restrict__ shots, float* p, float* vx, float* vz,
+    const int* __restrict__ inj_cell_beg, const Tap* __restrict__ inj_taps,
+    const float* __restrict__ amp_tab, int nt, int t, const int* __restrict__ rec_beg,
+    const Tap* __restrict__ rec_taps, const float* __restrict__ rec_w, float* traces, float* strips,
+    int pml) {
+    const Shot S = shots[blockIdx.x];
+    // source injection at step t (t in 1..nt); cells in CSR order
+    for (int c = threadIdx.x; c < S.ninj; c += blockDim.x) {
+        const int b = inj_cell_beg[S.inj0 + c], e = inj_cell_beg[S.inj0 + c + 1];
+        const Tap first = inj_taps[b];
+        float* f = first.field == 0 ? p : (first.field == 1 ? vx : vz);
+        float acc = f[first.cell];
+        for (int k = b; k < e; ++k) {
+            const Tap tp = inj_taps[k];
+            if (t < tp.tbeg || t > tp.tend) continue;
+            acc = acc + amp_tab[size_t(tp.row) * nt + (t - 1)] * tp.w1 * tp.w2;
+        }
+        f[first.cell] = acc;
+    }
+    __syncthreads();
+    if (t + 1 > nt) return;
+    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
+        const int g = S.rec0 + r;
+        const float w = rec_w[g];
+        float acc = 0.0f;
+        for (int k = rec_beg[g]; k < rec_beg[g + 1]; ++k) {
+            const Tap tp = rec_taps[k];
+            acc = acc + p[tp.cell] * tp.w1 * tp.w2 * w;
+        }
+        traces[S.trace0 + size_t(t) * S.nr + r] = acc;
+    }
+    if (strips) {
+        const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+        const int na = R * (nz + 2 * R), nc = (nx + 2 * R) * R;
+        const int len = 2 * na + 2 * nc;
+        float* rec = strips + S.strip0 + size_t(t) * len;
+        for (int k = threadIdx.x; k < len; k += blockDim.x) {
+            int i, j;
+            const float* src;
+            if (k < na) { i = 2 - R + k % R; j = 1 - R + k / R; src = vx; }
+            else if (k < 2 * na) { const int kk = k - na; i = nx + 1 + kk % R; j = 1 - R + kk / R; src = vx; }
+            else if (k < 2 * na + nc) { const int kk = k - 2 * na; i = 1 - R + kk % (nx + 2 * R); j = 2 - R + kk / (nx + 2 * R); src = vz; }
+            else { const int kk = k - 2 * na - nc; i = 1 - R + kk % (nx + 2 * R); j = nz + 1 + kk / (nx + 2 * R); src = vz; }
+            rec[k] = src[size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.pitch)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse pass, OWL's algorithm.  Inject the boundary record of step t into
+// the forward field (module_boundary_saving.f90:139-153) after keeping the
+// pre-injection values of the edge columns/rows the imaging condition reads.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ void k_inject_strips(const Shot* __restrict__ shots, float* vx, float* vz,
+                                const float* __restrict__ strips, float* edge, int t, int pml) {
+    const Shot S = shots[blockIdx.y];
+    const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+    const int na = R * (nz + 2 * R), nc = (nx + 2 * R) * R;
+    const int len = 2 * na + 2 * nc;
+    const float* rec = strips + S.strip0 + size_t(t - 1) * len;
+    const size_t pitch = size_t(S.pitch);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < len; k += gridDim.x * blockDim.x) {
+        int i, j, e = -1;
+        float* dst;
+        if (k < na) {
+            i = 2 - R + k % R; j = 1 - R + k / R; dst = vx;
+            if (i == 1 && j >= 1 && j <= nz) e = j - 1;
+        } else if (k < 2 * na) {
+            const int kk = k - na; i = nx + 1 + kk % R; j = 1 - R + kk / R; dst = vx;
+            if (i == nx + 1 && j >= 1 && j <= nz) e = nz + j - 1;
+        } else if (k < 2 * na + nc) {
+            const int kk = k - 2 * na; i = 1 - R + kk % (nx + 2 * R); j = 2 - R + kk / (nx + 2 * R); dst = vz;
+            if (j == 1 && i >= 1 && i <= nx) e = 2 * nz + i - 1;
+        } else {
+            const int kk = k - 2 * na - nc; i = 1 - R + kk % (nx + 2 * R); j = nz + 1 + kk / (nx + 2 * R); dst = vz;
+            if (j == nz + 1 && i >= 1 && i <= nx) e = 2 * nz + nx + i - 1;
+        }
+        const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
+        if (e >= 0) edge[S.edge0 + e] = dst[q];
+        dst[q] = rec[k];
+    }
+}
+
+// Adjoint source injection (submodule_fwi_gradient.f90:126-147): cells in CSR
+// order, contributions of each cell in receiver order.
+__global__ void k_inject_adjoint(const Shot* __restrict__ shots, float* pr,
+                                 const int* __restrict__ cell_beg, const Tap* __restrict__ taps,
+                                 const float* __restrict__ rec_w, const float* __restrict__ adj,
+                                 int t) {
+    const Shot S = shots[blockIdx.y];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.nadj; c += gridDim.x * blockDim.x) {
+        const int b = cell_beg[S.adj0 + c], e = cell_beg[S.adj0 + c + 1];
+        const long long cell = taps[b].cell;
+        float acc = pr[cell];
+        for (int k = b; k < e; ++k) {
+            const Tap tp = taps[k];
+            const float a = adj[S.trace0 + size_t(t - 1) * S.nr + tp.row];
+            acc = acc + a * tp.w1 * tp.w2 * rec_w[S.rec0 + tp.row];
+        }
+        pr[cell] = acc;
+    }
+}
+
+// Imaging condition of OWL's gradient (submodule_fwi_gradient.f90:201-254,
+// energy preconditioning off): over the window interior
+//   g_vp  -= (p - prev_p) * pr / (rho vp^3)
+//   g_rho += (sum2 vx - sum2 prev_vx)(sum2 vxr) + (z analogue)
+// with prev = 0 on the first reverse step.  FP64 accumulators.
+template <int R>
+__global__ __launch_bounds__(256) void k_image_owl(const Shot* __restrict__ shots,
+                                                   const float* __restrict__ p,
+                                                   const float* __restrict__ vx,
+                                                   const float* __restrict__ vz,
+                                                   const float* __restrict__ pp,
+                                                   const float* __restrict__ pvx,
+                                                   const float* __restrict__ pvz,
+                                                   const float* __restrict__ edge,
+                                                   const float* __restrict__ pr,
+                                                   const float* __restrict__ vxr,
+                                                   const float* __restrict__ vzr,
+                                                   const float* __restrict__ den, double* gv,
+                                                   double* gr, int prev_zero, int pml) {
+    const Shot S = shots[blockIdx.y];
+    const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+    const size_t pitch = size_t(S.pitch);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nx * nz; e += gridDim.x * blockDim.x) {
+        const int i = 1 + e % nx, j = 1 + e / nx;
+        const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
+        const float pc = p[q];
+        const float pv = prev_zero ? 0.0f : pp[q];
+        const float sp = pc - pv;
+        const float inc_v = sp * pr[q] / den[S.g0 + e];
+        float px0 = 0.f, px1 = 0.f, pz0 = 0.f, pz1 = 0.f;
+        if (!prev_zero) {
+            px0 = (i == 1) ? edge[S.edge0 + j - 1] : pvx[q];
+            px1 = (i == nx) ? edge[S.edge0 + nz + j - 1] : pvx[q + 1];
+            pz0 = (j == 1) ? edge[S.edge0 + 2 * nz + i - 1] : pvz[q];
+            pz1 = (j == nz) ? edge[S.edge0 + 2 * nz + nx + i - 1] : pvz[q + pitch];
+        }
+        const float sx = (vx[q] + vx[q + 1]) - (px0 + px1);
+        const float rx = vxr[q] + vxr[q + 1];
+        const float sz = (vz[q] + vz[q + pitch]) - (pz0 + pz1);
+        const float rz = vzr[q] + vzr[q + pitch];
+        gv[S.g0 + e] = gv[S.g0 + e] - double(inc_v);
+        gr[S.g0 + e] = gr[S.g0 + e] + double(sx * rx) + double(sz * rz);
+    }
+}
+
+// L2 waveform misfit and adjoint source for one shot per block
+// (compute_adjsrc_waveform, module_inversion_adjoint_source.f90:371-383;
+// trace discard rule :1042-1054).  Traces are [nt][nr] per shot.
+__global__ __launch_bounds__(256) void k_adjoint_source(const Shot* __restrict__ shots,
+                                                        const float* __restrict__ syn,
+                                                        const float* __restrict__ obs,
+                                                        const float* __restrict__ rec_w,
+                                                        float* adj, double* misfit, int nt,
+                                                        float threshold, double* scratch) {
+    const Shot S = shots[blockIdx.x];
+    __shared__ double red[256];
+    __shared__ double mo_s, ms_s;
+    double* nobs = scratch + 2 * size_t(S.rec0);
+    double* nsyn = nobs + S.nr;
+    double lmo = 0, lms = 0;
+    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
+        double a = 0, b = 0;
+        for (int t = 0; t < nt; ++t) {
+            const double o = obs[S.trace0 + size_t(t) * S.nr + r];
+            const double s = syn[S.trace0 + size_t(t) * S.nr + r];
+            a += o * o;
+            b += s * s;
+        }
+        nobs[r] = sqrt(a);
+        nsyn[r] = sqrt(b);
+        lmo = fmax(lmo, nobs[r]);
+        lms = fmax(lms, nsyn[r]);
+    }
+    red[threadIdx.x] = lmo;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mo_s = red[0];
+    __syncthreads();
+    red[threadIdx.x] = lms;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ms_s = red[0];
+    __syncthreads();
+    const double mo = mo_s, ms = ms_s;
+    // Neumaier-compensated FP64 sum of the squared residuals
+    double sum = 0.0, comp = 0.0;
+    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
+        const bool keep = !(rec_w[S.rec0 + r] == 0.0f || nobs[r] < threshold * mo ||
+                            nsyn[r] < threshold * ms || nobs[r] * nsyn[r] == 0.0);
+        for (int t = 0; t < nt; ++t) {
+            const size_t q = S.trace0 + size_t(t) * S.nr + r;
+            if (keep) {
+                const float d = obs[q] - syn[q];
+                const double v = double(d) * double(d);
+                const double tt = sum + v;
+                comp += (fabs(sum) >= fabs(v)) ? (sum - tt) + v : (v - tt) + sum;
+                sum = tt;
+                adj[q] = syn[q] - obs[q];
+            } else {
+                adj[q] = 0.0f;
+            }
+        }
+    }
+    red[threadIdx.x] = sum + comp;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) misfit[blockIdx.x] = red[0];
+}
+
+// Sum of the shots' window gradients into the full model, shot order fixed
+// (module_gradient.f90:366-388 followed by the all-reduce at :411-413).
+__global__ void k_merge(const Shot* __restrict__ shots, int nshot, const double* __restrict__ gw,
+                        double* gfull, int nx_full, int nz_full) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nx_full * nz_full;
+         e += gridDim.x * blockDim.x) {
+        const int iz = e % nz_full, ix = e / nz_full;  // full model, z fastest
+        double acc = 0.0;
+        for (int s = 0; s < nshot; ++s) {
+            const Shot S = shots[s];
+            const int i = ix - (S.nxbeg - 1), j = iz - (S.nzbeg - 1);
+            if (i >= 0 && i < S.nx && j >= 0 && j < S.nz) acc += gw[S.g0 + size_t(i) + size_t(j) * S.nx];
+        }
+        gfull[e] = acc;
+    }
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct ShotHost {
+    int id = 0;
+    int nx = 0, nz = 0;
+    int nxbeg = 1, nzbeg = 1;
+    float ox = 0, oz = 0;
+    std::vector<owl_source> src;
+    std::vector<owl_receiver> rec;
+    int hnt = 1;
+    float pmlvp = 0;
+    float f0max = 0;
+};
+
+}  // namespace a2
+}  // namespace owl
+
+using namespace owl;
+using namespace owl::a2;
+
+struct owl_acoustic2d {
+    owl_acoustic_config cfg{};
+    int R = 4, pml = 15, nt = 0;
+    float dt = 0, idx = 0, idz = 0;
+    Coef coef{};
+    std::vector<ShotHost> hs;
+    std::vector<Shot> sd;          // device descriptors (host copy)
+    std::vector<int> tile_shot;
+    size_t field_elems = 0;        // elements of one field buffer (all shots)
+    size_t grad_elems = 0;         // sum of nx*nz
+    size_t trace_elems = 0;        // sum of nt*nr
+    size_t strip_elems = 0;        // sum of nt*strip_len
+    size_t edge_elems = 0;
+    int nrec_total = 0;
+    int max_ninj = 0, max_nadj = 0;
+    double pts_full = 0;           // sum over shots of (nx+2pml)(nz+2pml)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0, last_updates = 0;
+    bool have_boundary = false;
+    bool fwd_state_valid = false;  // fA holds the final forward state
+    bool have_obs = false;         // d_obs holds observed traces
+    // per-kernel timing with HIP events on the batch stream
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_used;  // (kernel kind, first event index)
+    size_t ev_next = 0;
+    double kern_ms[8] = {0};
+    long long kern_launches[8] = {0};
+    double kern_bytes[8] = {0};    // algorithmic bytes per launch, by kind
+    std::vector<float> vp_full, rho_full;  // host copy of the full model
+
+    DevBuf<Shot> d_shots;
+    DevBuf<int> d_tile_shot;
+    DevBuf<float> d_bk, d_rho, d_den, d_prof;
+    DevBuf<float> fA[3], fB[3], fM[4];  // forward field (ping-pong) and PML memory
+    DevBuf<float> aF[3], aM[4];         // adjoint field and PML memory
+    DevBuf<float> d_amp;                // source amplitude rows [row][nt]
+    DevBuf<int> d_inj_beg, d_adj_beg, d_rec_beg;
+    DevBuf<Tap> d_inj_taps, d_adj_taps, d_rec_taps;
+    DevBuf<float> d_rec_w;
+    DevBuf<float> d_traces, d_obs, d_adj, d_strips, d_edge;
+    DevBuf<double> d_gv, d_gr, d_gfull_v, d_gfull_r, d_misfit, d_scratch;
+
+    ~owl_acoustic2d() {
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+inline size_t cell_of(const owl_acoustic2d* b, const Shot& S, int i, int j) {
+    const int lo = 1 - b->pml - b->R;
+    // every host-built tap must address the shot's padded array
+    if (i < lo || i > S.nx + b->pml + b->R || j < lo || j > S.nz + b->pml + b->R)
+        fail(OWL_ERR_ARG, "a source/receiver stencil reaches outside the padded grid; increase pml");
+    return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.pitch);
+}
+
+inline float full_at(const owl_acoustic2d* b, const std::vector<float>& m, const ShotHost& H,
+                     int i, int j) {
+    // window node (i, j), clamped to the window (edge replication), in the
+    // full (nz, nx) z-fast model
+    const int ic = std::min(std::max(i, 1), H.nx), jc = std::min(std::max(j, 1), H.nz);
+    return m[size_t(H.nzbeg - 1 + jc - 1) + size_t(H.nxbeg - 1 + ic - 1) * size_t(b->cfg.nz)];
+}
+
+// bk = rho*vp**2 and rho, padded by edge replication (prepare_modeling,
+// acoustic_iso_2d/module_vars.f90:108-115), and rho*vp**3 for the imaging
+// condition (submodule_fwi_gradient.f90:218).
+void upload_model(owl_acoustic2d* b) {
+    std::vector<float> bk(b->field_elems, 0.f), rh(b->field_elems, 0.f), den(b->grad_elems, 0.f);
+    const int lo = 1 - b->pml - b->R;
+    for (size_t s = 0; s < b->hs.size(); ++s) {
+        const ShotHost& H = b->hs[s];
+        const Shot& S = b->sd[s];
+        const int NXp = H.nx + 2 * (b->pml + b->R), NZp = H.nz + 2 * (b->pml + b->R);
+        for (int jj = 0; jj < NZp; ++jj)
+            for (int ii = 0; ii < NXp; ++ii) {
+                const int i = lo + ii, j = lo + jj;
+                const float v = full_at(b, b->vp_full, H, i, j), r = full_at(b, b->rho_full, H, i, j);
+                const size_t q = cell_of(b, S, i, j);
+                rh[q] = r;
+                bk[q] = r * (v * v);
+            }
+        for (int j = 1; j <= H.nz; ++j)
+            for (int i = 1; i <= H.nx; ++i) {
+                const float v = full_at(b, b->vp_full, H, i, j), r = full_at(b, b->rho_full, H, i, j);
+                den[size_t(S.g0) + size_t(i - 1) + size_t(j - 1) * H.nx] = r * (v * v * v);
+            }
+    }
+    b->d_bk.upload(bk, b->stream);
+    b->d_rho.upload(rh, b->stream);
+    b->d_den.upload(den, b->stream);
+    OWL_HIP(hipStreamSynchronize(b->stream));
+}
+
+void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, const float* rho,
+          int nshot, const owl_shot* shots) {
+    if (!cfg || !vp || !rho || nshot < 1 || !shots) fail(OWL_ERR_ARG, "null argument or no shots");
+    if (cfg->ny > 1) fail(OWL_ERR_ARG, "the 2D solver requires ny = 1");
+    if (!fd_coefficients(cfg->fdhalf, b->coef.c)) fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    if (cfg->nx < 1 || cfg->nz < 1 || !(cfg->dx > 0) || !(cfg->dz > 0) || !(cfg->dt > 0))
+        fail(OWL_ERR_ARG, "invalid grid or dt");
+    if (cfg->pml < cfg->fdhalf) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
+    if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+    b->cfg = *cfg;
+    b->R = cfg->fdhalf;
+    b->pml = cfg->pml;
+    b->dt = cfg->dt;
+    b->nt = time_steps(cfg->tmax, cfg->dt);
+    if (b->nt < 2) fail(OWL_ERR_ARG, "tmax/dt gives fewer than 2 steps");
+    b->idx = float(1.0 / double(cfg->dx));
+    b->idz = float(1.0 / double(cfg->dz));
+    const float f0factor = cfg->f0factor > 0 ? cfg->f0factor : 1.0f;
+    b->vp_full.assign(vp, vp + size_t(cfg->nx) * cfg->nz);
+    b->rho_full.assign(rho, rho + size_t(cfg->nx) * cfg->nz);
+    const int R = b->R, pml = b->pml, nt = b->nt;
+    const int fs = cfg->free_surface ? 1 : 0;
+
+    b->hs.assign(nshot, ShotHost{});
+    b->sd.assign(nshot, Shot{});
+    b->tile_shot.clear();
+    long long off = 0, strip = 0, trace = 0, g0 = 0;
+    int edge = 0, prof = 0;
+    std::vector<float> profs, amp;
+    std::vector<int> inj_beg{0}, adj_beg{0}, rec_beg{0};
+    std::vector<Tap> inj_taps, adj_taps, rec_taps;
+    std::vector<float> rec_w;
+    b->pts_full = 0;
+    for (int s = 0; s < nshot; ++s) {
+        const owl_shot& in = shots[s];
+        ShotHost& H = b->hs[s];
+        Shot& S = b->sd[s];
+        H.id = in.id;
+        H.nxbeg = in.nxbeg > 0 ? in.nxbeg : 1;
+        H.nzbeg = in.nzbeg > 0 ? in.nzbeg : 1;
+        const int nxend = in.nxend > 0 ? in.nxend : cfg->nx;
+        const int nzend = in.nzend > 0 ? in.nzend : cfg->nz;
+        if (H.nxbeg > nxend || nxend > cfg->nx || H.nzbeg > nzend || nzend > cfg->nz)
+            fail(OWL_ERR_ARG, "shot window outside the model");
+        H.nx = nxend - H.nxbeg + 1;
+        H.nz = nzend - H.nzbeg + 1;
+        // window origin shot_xbeg (module_utility.f90:430): (nxbeg-1)*dx + ox
+        H.ox = float(H.nxbeg - 1) * cfg->dx + cfg->ox;
+        H.oz = float(H.nzbeg - 1) * cfg->dz + cfg->oz;
+        if (in.ns < 1 || !in.sources) fail(OWL_ERR_ARG, "shot without sources");
+        if (in.nr < 1 || !in.receivers) fail(OWL_ERR_ARG, "shot without receivers");
+        H.src.assign(in.sources, in.sources + in.ns);
+        H.rec.assign(in.receivers, in.receivers + in.nr);
+
+        float vmax = 0.f, pmlvp = 0.f;
+        for (int i = 1; i <= H.nx; ++i)
+            for (int j = 1; j <= H.nz; ++j) {
+                const float v = full_at(b, b->vp_full, H, i, j);
+                if (!(v > 0.f) || !(full_at(b, b->rho_full, H, i, j) > 0.f))
+                    fail(OWL_ERR_ARG, "vp and rho must be positive");
+                vmax = std::max(vmax, v);
+                if (i == 1 || i == H.nx || j == 1 || j == H.nz) pmlvp = std::max(pmlvp, v);
+            }
+        // check_dt_f0 (module_utility.f90:363-404): OWL silently resets dt;
+        // here the call is refused so that dt stays what the caller chose.
+        const float dts = dt_stable(R, 2, vmax, cfg->dx, 1.0f, cfg->dz);
+        if (cfg->dt > dts)
+            fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " +
+                                  std::to_string(dts));
+        H.pmlvp = pmlvp;
+        for (auto& sc : H.src) H.f0max = std::max(H.f0max, sc.f0);
+
+        // ---- layout
+        const int NXp = H.nx + 2 * (pml + R), NZp = H.nz + 2 * (pml + R);
+        S.nx = H.nx;
+        S.nz = H.nz;
+        S.pitch = (NXp + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
+        S.nrow = NZp;
+        S.off = off;
+        off += (long long)S.pitch * NZp;
+        off = (off + 63) / 64 * 64;
+        S.tile0 = int(b->tile_shot.size());
+        S.ntx = (H.nx + 2 * pml + kBX - 1) / kBX;
+        const int ntz = (H.nz + 2 * pml + kRows - 1) / kRows;
+        b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * ntz, s);
+        b->pts_full += double(H.nx + 2 * pml) * double(H.nz + 2 * pml);
+        S.prof = prof;
+        PmlAxis px = cfspml_axis(H.nx, pml, cfg->dx, cfg->dt, pmlvp, H.f0max);
+        PmlAxis pz = cfspml_axis(H.nz, pml, cfg->dz, cfg->dt, pmlvp, H.f0max);
+        for (auto* v : {&px.ai, &px.bi, &px.ah, &px.bh, &pz.ai, &pz.bi, &pz.ah, &pz.bh})
+            profs.insert(profs.end(), v->begin(), v->end());
+        prof += 4 * (H.nx + 2 * pml) + 4 * (H.nz + 2 * pml);
+        S.nxbeg = H.nxbeg;
+        S.nzbeg = H.nzbeg;
+        S.g0 = g0;
+        g0 += (long long)H.nx * H.nz;
+        S.edge0 = edge;
+        edge += 2 * H.nz + 2 * H.nx;
+        const int strip_len = 2 * R * (H.nz + 2 * R) + 2 * (H.nx + 2 * R) * R;
+        S.strip0 = strip;
+        strip += (long long)strip_len * nt;
+        S.trace0 = trace;
+        S.nr = int(H.rec.size());
+        trace += (long long)nt * S.nr;
+
+        // ---- sources (prepare_geometry + prepare_stf + add_source)
+        const float xmax = float(H.nx - 1) * cfg->dx + H.ox, zmax = float(H.nz - 1) * cfg->dz + H.oz;
+        std::vector<Tap> st;  // taps of this shot, ordered by source then tap
+        for (size_t k = 0; k < H.src.size(); ++k) {
+            const owl_source& sc = H.src[k];
+            if (!(sc.x >= H.ox && sc.x <= xmax && sc.z >= H.oz && sc.z <= zmax))
+                fail(OWL_ERR_ARG, "source outside its shot window");
+            if (sc.mechanism != OWL_EXPLOSION && sc.mechanism != OWL_FORCE)
+                fail(OWL_ERR_ARG, "the acoustic solver takes explosion or force sources");
+            const Located L = locate(2, sc.x, 0.f, sc.z, H.ox, 0.f, H.oz, cfg->dx, 1.f, cfg->dz);
+            int hnt = 0;
+            const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            if (k == 0) H.hnt = hnt;
+            const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1;
+            const int nend = nbeg + int(stf.size()) - 1;
+            // amplitude rows: amp = stf(t-nbeg+1)*amp*dt
+            std::vector<float> a(nt, 0.f);
+            for (int t = std::max(nbeg, 1); t <= std::min(nend, nt); ++t)
+                a[t - 1] = stf[t - nbeg] * sc.amp * cfg->dt;
+            if (sc.mechanism == OWL_EXPLOSION) {
+                const int row = int(amp.size() / nt);
+                amp.insert(amp.end(), a.begin(), a.end());
+                for (int irz = -kNkw; irz <= kNkw; ++irz)
+                    for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                        if (fs && !(L.gz + irz >= 2)) continue;
+                        const float w1 = L.ix[irx + kNkw], w2 = L.iz[irz + kNkw];
+                        if (w1 == 0.f || w2 == 0.f) continue;
+                        st.push_back(Tap{(long long)cell_of(b, S, L.gx + irx, L.gz + irz), w1, w2, row, 0, nbeg, nend});
+                    }
+            } else {
+                // force: vx at (hx, gz) and vz at (gx, hz), scaled by sin/cos(polar)/rho
+                const float rx = full_at(b, b->rho_full, H, L.hx, L.gz);
+                const float rz = full_at(b, b->rho_full, H, L.gx, L.hz);
+                const float sp = std::sin(sc.polar), cp = std::cos(sc.polar);
+                const int rowx = int(amp.size() / nt);
+                for (int t = 0; t < nt; ++t) amp.push_back(sp * a[t] / rx);
+                const int rowz = int(amp.size() / nt);
+                for (int t = 0; t < nt; ++t) amp.push_back(cp * a[t] / rz);
+                for (int irz = -kNkw; irz <= kNkw; ++irz)
+                    for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                        if (!fs || L.gz + irz >= 2) {
+                            const float w1 = L.hxw[irx + kNkw], w2 = L.iz[irz + kNkw];
+                            if (w1 != 0.f && w2 != 0.f)
+                                st.push_back(Tap{(long long)cell_of(b, S, L.hx + irx, L.gz + irz), w1, w2, rowx, 1, nbeg, nend});
+                        }
+                        if (!fs || L.hz + irz >= 2) {
+                            const float w1 = L.ix[irx + kNkw], w2 = L.hzw[irz + kNkw];
+                            if (w1 != 0.f && w2 != 0.f)
+                                st.push_back(Tap{(long long)cell_of(b, S, L.gx + irx, L.hz + irz), w1, w2, rowz, 2, nbeg, nend});
+                        }
+                    }
+            }
+        }
+        // group taps by (field, cell), keeping source order within a cell
+        std::stable_sort(st.begin(), st.end(), [](const Tap& a, const Tap& c) {
+            return a.field != c.field ? a.field < c.field : a.cell < c.cell;
+        });
+        S.inj0 = int(inj_beg.size()) - 1;
+        S.ninj = 0;
+        for (size_t k = 0; k < st.size(); ++k) {
+            if (k == 0 || st[k].cell != st[k - 1].cell || st[k].field != st[k - 1].field) {
+                if (k) inj_beg.push_back(int(inj_taps.size()));
+                ++S.ninj;
+            }
+            inj_taps.push_back(st[k]);
+        }
+        if (!st.empty()) inj_beg.push_back(int(inj_taps.size()));
+        b->max_ninj = std::max(b->max_ninj, S.ninj);
+
+        // ---- receivers (prepare_geometry; sampling and adjoint injection)
+        S.rec0 = b->nrec_total;
+        std::vector<Tap> at;
+        for (int r = 0; r < S.nr; ++r) {
+            const owl_receiver& rc = H.rec[r];
+            const bool valid = rc.x >= H.ox && rc.x <= xmax && rc.z >= H.oz && rc.z <= zmax;
+            const float w = (valid && rc.weight != 0.f) ? rc.weight : 0.f;
+            rec_w.push_back(w);
+            if (w != 0.f) {
+                const Located L = locate(2, rc.x, 0.f, rc.z, H.ox, 0.f, H.oz, cfg->dx, 1.f, cfg->dz);
+                for (int irz = -kNkw; irz <= kNkw; ++irz)
+                    for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                        const float w1 = L.ix[irx + kNkw], w2 = L.iz[irz + kNkw];
+                        if (w1 == 0.f || w2 == 0.f) continue;
+                        const Tap tp{(long long)cell_of(b, S, L.gx + irx, L.gz + irz), w1, w2, r, 0, 0, 0};
+                        rec_taps.push_back(tp);
+                        at.push_back(tp);
+                    }
+            }
+            rec_beg.push_back(int(rec_taps.size()));
+        }
+        b->nrec_total += S.nr;
+        std::stable_sort(at.begin(), at.end(), [](const Tap& a, const Tap& c) { return a.cell < c.cell; });
+        S.adj0 = int(adj_beg.size()) - 1;
+        S.nadj = 0;
+        for (size_t k = 0; k < at.size(); ++k) {
+            if (k == 0 || at[k].cell != at[k - 1].cell) {
+                if (k) adj_beg.push_back(int(adj_taps.size()));
+                ++S.nadj;
+            }
+            adj_taps.push_back(at[k]);
+        }
+        if (!at.empty()) adj_beg.push_back(int(adj_taps.size()));
+        b->max_nadj = std::max(b->max_nadj, S.nadj);
+    }
+    b->field_elems = size_t(off);
+    b->grad_elems = size_t(g0);
+    b->trace_elems = size_t(trace);
+    b->strip_elems = size_t(strip);
+    b->edge_elems = size_t(edge);
+
+    if (!b->stream) {
+        OWL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        OWL_HIP(hipEventCreate(&b->ev0));
+        OWL_HIP(hipEventCreate(&b->ev1));
+    }
+    hipStream_t st = b->stream;
+    b->d_shots.upload(b->sd, st);
+    b->d_tile_shot.upload(b->tile_shot, st);
+    b->d_prof.upload(profs, st);
+    if (amp.empty()) amp.push_back(0.f);
+    b->d_amp.upload(amp, st);
+    b->d_inj_beg.upload(inj_beg, st);
+    if (inj_taps.empty()) inj_taps.push_back(Tap{});
+    b->d_inj_taps.upload(inj_taps, st);
+    b->d_rec_beg.upload(rec_beg, st);
+    if (rec_taps.empty()) rec_taps.push_back(Tap{});
+    b->d_rec_taps.upload(rec_taps, st);
+    b->d_adj_beg.upload(adj_beg, st);
+    if (adj_taps.empty()) adj_taps.push_back(Tap{});
+    b->d_adj_taps.upload(adj_taps, st);
+    b->d_rec_w.upload(rec_w, st);
+    for (int k = 0; k < 3; ++k) b->fA[k].alloc(b->field_elems);
+    for (int k = 0; k < 4; ++k) b->fM[k].alloc(b->field_elems);
+    b->d_traces.alloc(b->trace_elems);
+    upload_model(b);
+}
+
+Sets one_set(float* p, float* vx, float* vz, float* const* m) {
+    Sets S{};
+    S.s[0] = FieldSet{p, p, vx, vx, vz, vz, m[0], m[1], m[2], m[3]};
+    S.s[1] = S.s[0];
+    return S;
+}
+
+enum KernelKind { kKP = 0, kKV = 1, kKPost = 2, kKImage = 3, kKStrips = 4, kKAdj = 5, kKMisfit = 6, kKMerge = 7 };
+
+// Begin/end a timed region around one launch when profiling is on.
+int prof_begin(owl_acoustic2d* b, int kind) {
+    if (!b->profiling) return -1;
+    if (b->ev_next + 2 > b->ev_pool.size()) {
+        for (int k = 0; k < 256; ++k) {
+            hipEvent_t e;
+            OWL_HIP(hipEventCreate(&e));
+            b->ev_pool.push_back(e);
+        }
+    }
+    const int first = int(b->ev_next);
+    b->ev_next += 2;
+    OWL_HIP(hipEventRecord(b->ev_pool[first], b->stream));
+    b->ev_used.push_back({kind, first});
+    return first;
+}
+void prof_end(owl_acoustic2d* b, int first) {
+    if (first < 0) return;
+    OWL_HIP(hipEventRecord(b->ev_pool[first + 1], b->stream));
+}
+// After a stream synchronisation: fold the recorded pairs into the totals.
+void prof_collect(owl_acoustic2d* b) {
+    for (auto& u : b->ev_used) {
+        float ms = 0.f;
+        OWL_HIP(hipEventElapsedTime(&ms, b->ev_pool[u.second], b->ev_pool[u.second + 1]));
+        b->kern_ms[u.first] += ms;
+        b->kern_launches[u.first] += 1;
+    }
+    b->ev_used.clear();
+    b->ev_next = 0;
+}
+
+// Algorithmic (compulsory) bytes of one launch of the update kernels over
+// all shots for nsets field sets: each array element read or written once.
+//   p-sweep: vx, vz, p, bk read, p written (20 B/point) + 8 B per slab
+//            cell and damped direction (memory variable read + write);
+//   v-sweep: p, vx, vz, rho read, vx, vz written (24 B/point) + slabs.
+void set_kernel_bytes(owl_acoustic2d* b, int nsets) {
+    double bp = 0, bv = 0;
+    const int pml = b->pml;
+    for (auto& S : b->sd) {
+        const double lx = S.nx + 2.0 * pml, lz = S.nz + 2.0 * pml;
+        const double pts = lx * lz;
+        const double slab_p = 2.0 * pml * lz + 2.0 * pml * lx;              // i<=0|i>=nx+1 ; j likewise
+        const double slab_v = (2.0 * pml + 1) * lz + (2.0 * pml + 1) * lx;  // m<=1|m>=nx+1 ; likewise
+        bp += 20.0 * pts + 8.0 * slab_p;
+        bv += 24.0 * pts + 8.0 * slab_v;
+    }
+    b->kern_bytes[kKP] = bp * nsets;
+    b->kern_bytes[kKV] = bv * nsets;
+}
+
+template <int R>
+void launch_step(owl_acoustic2d* b, const Sets& sets, int nsets, float dt) {
+    const int fs = b->cfg.free_surface ? 1 : 0;
+    const dim3 blk(kBX, kBY), grd(unsigned(b->tile_shot.size()), unsigned(nsets));
+    int ev = prof_begin(b, kKP);
+    hipLaunchKernelGGL(k_update_p<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), sets,
+                       b->d_bk.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, fs);
+    prof_end(b, ev);
+    if (fs) {
+        const dim3 g2(64, unsigned(b->sd.size()), unsigned(nsets));
+        hipLaunchKernelGGL(k_mirror_p<R>, g2, dim3(256), 0, b->stream, b->d_shots.get(), sets, b->pml);
+    }
+    ev = prof_begin(b, kKV);
+    hipLaunchKernelGGL(k_update_v<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), sets,
+                       b->d_rho.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, fs);
+    prof_end(b, ev);
+}
+
+template <int R>
+void run_forward(owl_acoustic2d* b, bool save) {
+    hipStream_t st = b->stream;
+    for (int k = 0; k < 3; ++k) b->fA[k].zero(st);
+    for (int k = 0; k < 4; ++k) b->fM[k].zero(st);
+    b->d_traces.zero(st);
+    if (save) {
+        b->d_strips.alloc(b->strip_elems);
+        b->d_strips.zero(st);
+    }
+    float* m[4] = {b->fM[0].get(), b->fM[1].get(), b->fM[2].get(), b->fM[3].get()};
+    const Sets sets = one_set(b->fA[0].get(), b->fA[1].get(), b->fA[2].get(), m);
+    const unsigned nshot = unsigned(b->sd.size());
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    // steps 1..nt-1; the update and source of step nt change nothing that
+    // OWL records, and skipping them leaves the state of the start of step nt
+    // (OWL's output_final_step_wavefield) in place.
+    for (int t = 1; t <= b->nt - 1; ++t) {
+        launch_step<R>(b, sets, 1, b->dt);
+        const int ev = prof_begin(b, kKPost);
+        hipLaunchKernelGGL(k_post_step<R>, dim3(nshot), dim3(512), 0, st, b->d_shots.get(),
+                           b->fA[0].get(), b->fA[1].get(), b->fA[2].get(), b->d_inj_beg.get(),
+                           b->d_inj_taps.get(), b->d_amp.get(), b->nt, t, b->d_rec_beg.get(),
+                           b->d_rec_taps.get(), b->d_rec_w.get(), b->d_traces.get(),
+                           save ? b->d_strips.get() : nullptr, b->pml);
+        prof_end(b, ev);
+        if (b->profiling && b->ev_used.size() > 20000) {
+            OWL_HIP(hipStreamSynchronize(st));
+            prof_collect(b);
+        }
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    prof_collect(b);
+    set_kernel_bytes(b, 1);
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = double(b->nt - 1) * b->pts_full;
+    b->have_boundary = save;
+    b->fwd_state_valid = true;
+}
+
+// OWL's reverse loop t = nt .. hnt (submodule_fwi_gradient.f90:75-167).
+template <int R>
+void run_reverse_owl(owl_acoustic2d* b) {
+    hipStream_t st = b->stream;
+    if (!b->have_boundary || !b->fwd_state_valid)
+        fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
+    for (int k = 0; k < 3; ++k) {
+        b->fB[k].alloc(b->field_elems);
+        b->fB[k].zero(st);
+        b->aF[k].alloc(b->field_elems);
+        b->aF[k].zero(st);
+    }
+    for (int k = 0; k < 4; ++k) {
+        b->fM[k].zero(st);
+        b->aM[k].alloc(b->field_elems);
+        b->aM[k].zero(st);
+    }
+    b->d_edge.alloc(std::max<size_t>(b->edge_elems, 1));
+    b->d_gv.alloc(b->grad_elems);
+    b->d_gr.alloc(b->grad_elems);
+    b->d_gv.zero(st);
+    b->d_gr.zero(st);
+    const unsigned nshot = unsigned(b->sd.size());
+    int hnt = b->nt;
+    for (auto& H : b->hs) hnt = std::min(hnt, std::max(H.hnt, 1));
+    // OWL loops down to the first source's hnt of each shot; a batch whose
+    // shots disagree runs each shot's imaging only while t >= its own hnt.
+    for (auto& H : b->hs)
+        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one batch must share hnt");
+    hnt = std::max(b->hs[0].hnt, 1);
+    DevBuf<float>* A = b->fA;
+    DevBuf<float>* B = b->fB;
+    const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
+    const unsigned ig = unsigned(std::max(1, std::min(1024, (int)((b->grad_elems / nshot + 255) / 256))));
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    for (int t = b->nt; t >= hnt; --t) {
+        int ev = prof_begin(b, kKStrips);
+        hipLaunchKernelGGL(k_inject_strips<R>, dim3(16, nshot), dim3(256), 0, st, b->d_shots.get(),
+                           A[1].get(), A[2].get(), b->d_strips.get(), b->d_edge.get(), t, b->pml);
+        prof_end(b, ev);
+        Sets sets{};
+        sets.s[0] = FieldSet{A[0].get(), B[0].get(), A[1].get(), B[1].get(), A[2].get(), B[2].get(),
+                             b->fM[0].get(), b->fM[1].get(), b->fM[2].get(), b->fM[3].get()};
+        sets.s[1] = FieldSet{b->aF[0].get(), b->aF[0].get(), b->aF[1].get(), b->aF[1].get(),
+                             b->aF[2].get(), b->aF[2].get(), b->aM[0].get(), b->aM[1].get(),
+                             b->aM[2].get(), b->aM[3].get()};
+        launch_step<R>(b, sets, 2, -b->dt);
+        ev = prof_begin(b, kKAdj);
+        if (b->max_nadj > 0)
+            hipLaunchKernelGGL(k_inject_adjoint, dim3(unsigned((b->max_nadj + 255) / 256), nshot), dim3(256), 0,
+                               st, b->d_shots.get(), b->aF[0].get(), b->d_adj_beg.get(),
+                               b->d_adj_taps.get(), b->d_rec_w.get(), b->d_adj.get(), t);
+        prof_end(b, ev);
+        ev = prof_begin(b, kKImage);
+        if (t % cc == 0)
+            hipLaunchKernelGGL(k_image_owl<R>, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(),
+                               B[0].get(), B[1].get(), B[2].get(), A[0].get(), A[1].get(), A[2].get(),
+                               b->d_edge.get(), b->aF[0].get(), b->aF[1].get(), b->aF[2].get(),
+                               b->d_den.get(), b->d_gv.get(), b->d_gr.get(), t == b->nt ? 1 : 0, b->pml);
+        prof_end(b, ev);
+        std::swap(A, B);
+        if (b->profiling && b->ev_used.size() > 20000) {
+            OWL_HIP(hipStreamSynchronize(st));
+            prof_collect(b);
+        }
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    prof_collect(b);
+    set_kernel_bytes(b, 2);
+    {
+        // imaging: p, prev p, pr, 1/(rho vp^3) and the six velocity fields
+        // read (40 B/point incl. the +1 neighbours once), two FP64
+        // accumulators read and written (32 B/point)
+        double pts = 0;
+        for (auto& S : b->sd) pts += double(S.nx) * S.nz;
+        b->kern_bytes[kKImage] = 72.0 * pts;
+    }
+    // the forward state now lives in whichever buffer A points to; the
+    // boundary records have been consumed
+    if (A != b->fA)
+        for (int k = 0; k < 3; ++k) std::swap(b->fA[k], b->fB[k]);
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = 2.0 * double(b->nt - hnt + 1) * b->pts_full;
+    b->fwd_state_valid = false;
+}
+
+template <class F>
+void dispatch_R(int R, F&& f) {
+    switch (R) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 6: f(std::integral_constant<int, 6>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    }
+}
+
+// Device [shot][t][nr] <-> host [shot][nr][nt]
+void traces_to_host(const owl_acoustic2d* b, const std::vector<float>& dev, float* out) {
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        const Shot& S = b->sd[s];
+        for (int r = 0; r < S.nr; ++r)
+            for (int t = 0; t < b->nt; ++t)
+                out[size_t(S.trace0) + size_t(r) * b->nt + t] = dev[size_t(S.trace0) + size_t(t) * S.nr + r];
+    }
+}
+void traces_to_dev(const owl_acoustic2d* b, const float* in, std::vector<float>& dev) {
+    dev.assign(b->trace_elems, 0.f);
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        const Shot& S = b->sd[s];
+        for (int r = 0; r < S.nr; ++r)
+            for (int t = 0; t < b->nt; ++t)
+                dev[size_t(S.trace0) + size_t(t) * S.nr + r] = in[size_t(S.trace0) + size_t(r) * b->nt + t];
+    }
+}
+
+void download_traces(owl_acoustic2d* b, float* out) {
+    std::vector<float> h(b->trace_elems);
+    OWL_HIP(hipMemcpyAsync(h.data(), b->d_traces.get(), h.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    OWL_HIP(hipStreamSynchronize(b->stream));
+    traces_to_host(b, h, out);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+owl_status owl_acoustic2d_create(const owl_acoustic_config* cfg, const float* vp, const float* rho,
+                                 int nshot, const owl_shot* shots, owl_acoustic2d** out) {
+    return guarded([&] {
+        if (!out) fail(OWL_ERR_ARG, "out is null");
+        *out = nullptr;
+        auto* b = new owl_acoustic2d();
+        try {
+            plan(b, cfg, vp, rho, nshot, shots);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+void owl_acoustic2d_destroy(owl_acoustic2d* b) { delete b; }
+
+int owl_acoustic2d_nt(const owl_acoustic2d* b) { return b ? b->nt : 0; }
+
+owl_status owl_acoustic2d_set_model(owl_acoustic2d* b, const float* vp, const float* rho) {
+    return guarded([&] {
+        if (!b || !vp || !rho) fail(OWL_ERR_ARG, "null argument");
+        std::vector<owl_shot> shots(b->hs.size());
+        for (size_t s = 0; s < b->hs.size(); ++s) {
+            const ShotHost& H = b->hs[s];
+            shots[s] = owl_shot{H.id, int(H.src.size()), H.src.data(), int(H.rec.size()), H.rec.data(),
+                                H.nxbeg, H.nxbeg + H.nx - 1, 0, 0, H.nzbeg, H.nzbeg + H.nz - 1};
+        }
+        std::vector<ShotHost> keep = b->hs;  // shots reference these vectors
+        for (size_t s = 0; s < keep.size(); ++s) {
+            shots[s].sources = keep[s].src.data();
+            shots[s].receivers = keep[s].rec.data();
+        }
+        const owl_acoustic_config cfg = b->cfg;
+        plan(b, &cfg, vp, rho, int(shots.size()), shots.data());
+        b->have_boundary = false;
+        b->fwd_state_valid = false;
+    });
+}
+
+owl_status owl_acoustic2d_forward(owl_acoustic2d* b, float* traces, int save_boundary) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        dispatch_R(b->R, [&](auto r) { run_forward<decltype(r)::value>(b, save_boundary != 0); });
+        if (traces) download_traces(b, traces);
+    });
+}
+
+owl_status owl_acoustic2d_adjoint(owl_acoustic2d* b, const float* adj, float* grad_vp, float* grad_rho) {
+    return guarded([&] {
+        if (!b || !adj) fail(OWL_ERR_ARG, "null argument");
+        std::vector<float> h;
+        traces_to_dev(b, adj, h);
+        b->d_adj.upload(h, b->stream);
+        dispatch_R(b->R, [&](auto r) { run_reverse_owl<decltype(r)::value>(b); });
+        std::vector<double> gv(b->grad_elems), gr(b->grad_elems);
+        OWL_HIP(hipMemcpy(gv.data(), b->d_gv.get(), gv.size() * sizeof(double), hipMemcpyDeviceToHost));
+        OWL_HIP(hipMemcpy(gr.data(), b->d_gr.get(), gr.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // .grd arrays: transpose to (nz, nx) z-fast; NaN/Inf -> 0 (return_normal)
+        for (size_t s = 0; s < b->sd.size(); ++s) {
+            const Shot& S = b->sd[s];
+            for (int i = 0; i < S.nx; ++i)
+                for (int j = 0; j < S.nz; ++j) {
+                    const size_t src = size_t(S.g0) + size_t(i) + size_t(j) * S.nx;
+                    const size_t dst = size_t(S.g0) + size_t(j) + size_t(i) * S.nz;
+                    const float a = float(gv[src]), c = float(gr[src]);
+                    if (grad_vp) grad_vp[dst] = std::isfinite(a) ? a : 0.f;
+                    if (grad_rho) grad_rho[dst] = std::isfinite(c) ? c : 0.f;
+                }
+        }
+    });
+}
+
+owl_status owl_acoustic2d_gradient(owl_acoustic2d* b, const float* obs, int mode, int ckpt, int ckpt_param,
+                                   int store, double* misfit, double* grad_vp, double* grad_rho,
+                                   float* traces_out) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (mode != OWL_GRAD_OWL) fail(OWL_ERR_ARG, "only OWL_GRAD_OWL is available in this build");
+        (void)ckpt;
+        (void)ckpt_param;
+        (void)store;
+        hipStream_t st = b->stream;
+        if (obs) {
+            std::vector<float> h;
+            traces_to_dev(b, obs, h);
+            b->d_obs.upload(h, st);
+            b->have_obs = true;
+        }
+        if (!b->have_obs) fail(OWL_ERR_STATE, "no observed data: pass obs or call owl_acoustic2d_set_observed");
+        b->d_adj.alloc(b->trace_elems);
+        b->d_misfit.alloc(b->sd.size());
+        b->d_scratch.alloc(2 * size_t(std::max(1, b->nrec_total)));
+        double total_ms = 0, total_up = 0;
+        dispatch_R(b->R, [&](auto r) { run_forward<decltype(r)::value>(b, true); });
+        total_ms += b->last_ms;
+        total_up += b->last_updates;
+        OWL_HIP(hipEventRecord(b->ev0, st));
+        int evm = prof_begin(b, kKMisfit);
+        hipLaunchKernelGGL(k_adjoint_source, dim3(unsigned(b->sd.size())), dim3(256), 0, st, b->d_shots.get(),
+                           b->d_traces.get(), b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(),
+                           b->d_misfit.get(), b->nt, 1.0e-6f, b->d_scratch.get());
+        prof_end(b, evm);
+        OWL_HIP(hipEventRecord(b->ev1, st));
+        OWL_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        total_ms += ms;
+        dispatch_R(b->R, [&](auto r) { run_reverse_owl<decltype(r)::value>(b); });
+        total_ms += b->last_ms;
+        total_up += b->last_updates;
+        b->d_gfull_v.alloc(size_t(b->cfg.nx) * b->cfg.nz);
+        b->d_gfull_r.alloc(size_t(b->cfg.nx) * b->cfg.nz);
+        OWL_HIP(hipEventRecord(b->ev0, st));
+        const unsigned mg = unsigned(std::min<size_t>(4096, (size_t(b->cfg.nx) * b->cfg.nz + 255) / 256));
+        hipLaunchKernelGGL(k_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), int(b->sd.size()),
+                           b->d_gv.get(), b->d_gfull_v.get(), b->cfg.nx, b->cfg.nz);
+        hipLaunchKernelGGL(k_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), int(b->sd.size()),
+                           b->d_gr.get(), b->d_gfull_r.get(), b->cfg.nx, b->cfg.nz);
+        OWL_HIP(hipEventRecord(b->ev1, st));
+        OWL_HIP(hipStreamSynchronize(st));
+        OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        total_ms += ms;
+        b->last_ms = total_ms;
+        b->last_updates = total_up;
+        const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
+        if (grad_vp) OWL_HIP(hipMemcpy(grad_vp, b->d_gfull_v.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
+        if (grad_rho) OWL_HIP(hipMemcpy(grad_rho, b->d_gfull_r.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
+        if (misfit) OWL_HIP(hipMemcpy(misfit, b->d_misfit.get(), b->sd.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (traces_out) download_traces(b, traces_out);
+    });
+}
+
+owl_status owl_acoustic2d_shot_dims(const owl_acoustic2d* b, int s, int* nx, int* nz, int* nr) {
+    return guarded([&] {
+        if (!b || s < 0 || s >= int(b->sd.size())) fail(OWL_ERR_ARG, "bad shot index");
+        if (nx) *nx = b->sd[s].nx;
+        if (nz) *nz = b->sd[s].nz;
+        if (nr) *nr = b->sd[s].nr;
+    });
+}
+
+owl_status owl_acoustic2d_set_observed(owl_acoustic2d* b, const float* obs) {
+    return guarded([&] {
+        if (!b || !obs) fail(OWL_ERR_ARG, "null argument");
+        std::vector<float> h;
+        traces_to_dev(b, obs, h);
+        b->d_obs.upload(h, b->stream);
+        OWL_HIP(hipStreamSynchronize(b->stream));
+        b->have_obs = true;
+    });
+}
+
+owl_status owl_acoustic2d_observe(owl_acoustic2d* b) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        dispatch_R(b->R, [&](auto r) { run_forward<decltype(r)::value>(b, false); });
+        b->d_obs.alloc(b->trace_elems);
+        OWL_HIP(hipMemcpyAsync(b->d_obs.get(), b->d_traces.get(), b->trace_elems * sizeof(float),
+                               hipMemcpyDeviceToDevice, b->stream));
+        OWL_HIP(hipStreamSynchronize(b->stream));
+        b->have_obs = true;
+        b->fwd_state_valid = false;
+    });
+}
+
+owl_status owl_acoustic2d_set_profiling(owl_acoustic2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->profiling = on != 0;
+        for (int k = 0; k < 8; ++k) {
+            b->kern_ms[k] = 0;
+            b->kern_launches[k] = 0;
+        }
+    });
+}
+
+owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d* b, int kind, double* total_ms,
+                                       long long* launches, double* bytes_per_launch) {
+    return guarded([&] {
+        if (!b || kind < 0 || kind > 7) fail(OWL_ERR_ARG, "bad kernel kind");
+        if (total_ms) *total_ms = b->kern_ms[kind];
+        if (launches) *launches = b->kern_launches[kind];
+        if (bytes_per_launch) *bytes_per_launch = b->kern_bytes[kind];
+    });
+}
+
+owl_status owl_acoustic2d_last_stats(const owl_acoustic2d* b, double* ms, double* updates) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (ms) *ms = b->last_ms;
+        if (updates) *updates = b->last_updates;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Shared runtime helpers: error reporting through owl_last_error(), HIP
+// status checks that turn into owl_status codes, and a device buffer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "owl_amd.h"
+
+namespace owl {
+
+// Exception carrying an owl_status; caught at the C boundary.
+struct Error : std::runtime_error {
+    owl_status code;
+    Error(owl_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+void clear_last_error();
+
+[[noreturn]] inline void fail(owl_status code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory)
+            fail(OWL_ERR_NOMEM, std::string(what) + ": " + hipGetErrorString(e));
+        fail(OWL_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define OWL_HIP(call) ::owl::hip_check((call), #call)
+
+// Run f() and convert exceptions into a status (used by every entry point).
+template <class F>
+owl_status guarded(F&& f) {
+    try {
+        clear_last_error();
+        f();
+        return OWL_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return OWL_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return OWL_ERR_ARG;
+    }
+}
+
+// Owning device allocation of n elements of T.
+template <class T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        OWL_HIP(hipMalloc(&p_, n * sizeof(T)));
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void upload(const T* h, size_t n, hipStream_t s) {
+        if (n > n_) fail(OWL_ERR_ARG, "upload larger than buffer");
+        if (n) OWL_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) {
+        if (v.size() > n_) alloc(v.size());
+        upload(v.data(), v.size(), s);
+    }
+    void zero(hipStream_t s) {
+        if (n_) OWL_HIP(hipMemsetAsync(p_, 0, n_ * sizeof(T), s));
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+}  // namespace owl

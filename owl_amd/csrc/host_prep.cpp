@@ -1,0 +1,178 @@
+// Host-side preparation for OWL's acoustic solvers.  See host_prep.hpp.
+#include "host_prep.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace owl {
+
+namespace {
+// Optimised staggered coefficients, src/lib/module_fdcoef.f90:24-71.
+const float kCoef[9][8] = {
+    {0},
+    {0},
+    {0.1129042e+1f, -0.4301412e-1f},
+    {0.1185991e+1f, -0.7249965e-1f, 0.6301572e-2f},
+    {0.1217990e+1f, -0.9382142e-1f, 0.1507536e-1f, -0.1700324e-2f},
+    {0.1236607e+1f, -0.1082265e+0f, 0.2343440e-1f, -0.5033546e-2f, 0.6817483e-3f},
+    {0.1247662e+1f, -0.1175538e+0f, 0.2997970e-1f, -0.8719078e-2f, 0.2215897e-2f, -0.3462075e-3f},
+    {0.1254799e+1f, -0.1238928e+0f, 0.3494371e-1f, -0.1208897e-1f, 0.4132531e-2f, -0.1197110e-2f,
+     0.2122227e-3f},
+    {0.1259312e+1f, -0.1280347e+0f, 0.3841945e-1f, -0.1473229e-1f, 0.5924913e-2f, -0.2248618e-2f,
+     0.7179226e-3f, -0.1400855e-3f},
+};
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// Modified Bessel function of the first kind, order 0, from its series
+// sum_k ((x/2)^k / k!)^2; terms are added until they no longer change the sum.
+double bessel_i0(double x) {
+    const double q = 0.25 * x * x;
+    double term = 1.0, total = 1.0;
+    for (int k = 1; k < 500; ++k) {
+        term = term * (q / (double(k) * double(k)));
+        total += term;
+        if (term < 1e-18 * total) break;
+    }
+    return total;
+}
+}  // namespace
+
+bool fd_coefficients(int R, float* c) {
+    if (R < 2 || R > 8) return false;
+    for (int m = 0; m < R; ++m) c[m] = kCoef[R][m];
+    return true;
+}
+
+int fortran_nint(float x) { return int(std::round(x)); }
+
+void kaiser_sinc(float x, float h[9]) {
+    // kaiser_b0 is a double initialised from the default-real literal 6.31.
+    const double beta = double(6.31f);
+    for (int k = 0; k < 9; ++k) h[k] = 0.0f;
+    if (x == 0.0f) {
+        h[kNkw] = 1.0f;
+        return;
+    }
+    const double norm = bessel_i0(beta);
+    for (int i = -kNkw; i <= kNkw; ++i) {
+        const float t = float(i) - x;
+        const double arg = kPi * double(t);
+        const float s = float(arg == 0.0 ? 1.0 : std::sin(arg) / arg);
+        float w = 0.0f;
+        if (t >= -float(kNkw) && t <= float(kNkw)) {
+            const float u = t / float(kNkw);
+            const float r = std::sqrt(1.0f - u * u);
+            w = float(bessel_i0(beta * double(r)) / norm);
+        }
+        h[i + kNkw] = s * w;
+    }
+}
+
+Located locate(int dims, float x, float y, float z, float ox, float oy, float oz, float dx,
+               float dy, float dz) {
+    Located L;
+    L.gx = fortran_nint((x - ox) / dx) + 1;
+    L.gz = fortran_nint((z - oz) / dz) + 1;
+    L.hx = fortran_nint((x - ox + 0.5f * dx) / dx) + 1;
+    L.hz = fortran_nint((z - oz + 0.5f * dz) / dz) + 1;
+    kaiser_sinc((x - ox) / dx - float(L.gx - 1), L.ix);
+    kaiser_sinc((z - oz) / dz - float(L.gz - 1), L.iz);
+    kaiser_sinc((x - ox + 0.5f * dx) / dx - float(L.hx - 1), L.hxw);
+    kaiser_sinc((z - oz + 0.5f * dz) / dz - float(L.hz - 1), L.hzw);
+    if (dims == 3) {
+        L.gy = fortran_nint((y - oy) / dy) + 1;
+        L.hy = fortran_nint((y - oy + 0.5f * dy) / dy) + 1;
+        kaiser_sinc((y - oy) / dy - float(L.gy - 1), L.iy);
+        kaiser_sinc((y - oy + 0.5f * dy) / dy - float(L.hy - 1), L.hyw);
+    } else {
+        kaiser_sinc(0.0f, L.iy);
+        kaiser_sinc(0.0f, L.hyw);
+    }
+    return L;
+}
+
+int stf_length(float f0, float dt, float f0factor) {
+    return int(std::ceil(2.0f * f0factor / f0 / dt));
+}
+
+std::vector<float> source_time_function(int wavelet, float f0, float dt, float f0factor,
+                                        int* hnt) {
+    const int n = stf_length(f0, dt, f0factor);
+    std::vector<float> w(n > 0 ? n : 0);
+    float peak = 0.0f;
+    for (int t = 1; t <= n; ++t) {
+        const float tau = float(t - 1) * dt - f0factor / f0;
+        // (const_pi*f0*t)**2 is formed in double precision and stored as real.
+        const double a = kPi * double(f0) * double(tau);
+        const float q = float(a * a);
+        float v;
+        switch (wavelet) {
+            case kGaussian: v = std::exp(-q); break;
+            case kGaussianDeriv: v = -tau * std::exp(-q); break;
+            case kRickerDeriv: v = ((2.0f * q - 3.0f) * std::exp(-q)) * tau; break;
+            default: v = (1.0f - 2.0f * q) * std::exp(-q); break;
+        }
+        w[t - 1] = v;
+        peak = std::fabs(v) > peak ? std::fabs(v) : peak;
+    }
+    for (float& v : w) v = v / peak;
+    if (hnt) *hnt = fortran_nint(float(n) / 2.0f);
+    return w;
+}
+
+float dt_stable(int R, int dims, float vmax, float dx, float dy, float dz) {
+    float csum = 0.0f;
+    for (int m = 0; m < R; ++m) csum = csum + std::fabs(kCoef[R][m]);
+    const float s = dims == 3 ? 1.0f / (dx * dx) + 1.0f / (dy * dy) + 1.0f / (dz * dz)
+                              : 1.0f / (dx * dx) + 1.0f / (dz * dz);
+    return 1.0f / (csum * vmax * std::sqrt(s));
+}
+
+int time_steps(float tmax, float dt) { return fortran_nint(tmax / dt + 1.0f); }
+
+namespace {
+// damp_coef, acoustic_iso_2d/module_cfspml.f90:37-56 (npower_a = 1,
+// npower_d = 2, kmax = 1, R0 = 1e-5).
+void damping(float pmlvp, float d, float dist, int pml, float dt, float alphamax, float* a,
+             float* b) {
+    const float r0 = 1.0e-5f;
+    const float nd = dist / (float(pml) * d);
+    float alpha = alphamax * (1.0f - nd);
+    const float floor_alpha = 0.25f * alphamax;
+    if (floor_alpha > alpha) alpha = floor_alpha;
+    const float damp = pmlvp * (2.0f + 1.0f) * std::log(1.0f / r0) / (2.0f * float(pml) * d) * (nd * nd);
+    const float kappa = 1.0f;
+    const float adt = std::fabs(dt);
+    *a = -(2.0f * adt * damp / kappa / (2.0f + adt * (alpha + damp / kappa)));
+    *b = (2.0f - adt * (alpha + damp / kappa)) / (2.0f + adt * (alpha + damp / kappa));
+}
+}  // namespace
+
+PmlAxis cfspml_axis(int n, int pml, float d, float dt, float pmlvp, float f0max) {
+    // alphamax = 1.0*max(f0)*const_pi, with const_pi in double precision.
+    const float alphamax = float(double(1.0f * f0max) * kPi);
+    PmlAxis P;
+    const int len = n + 2 * pml;
+    P.ai.assign(len, 0.0f);
+    P.bi.assign(len, 0.0f);
+    P.ah.assign(len, 0.0f);
+    P.bh.assign(len, 0.0f);
+    for (int i = -pml + 1; i <= n + pml; ++i) {
+        if (!(i <= 1 || i >= n + 1)) continue;
+        float di, dh;
+        if (i <= 1) {
+            di = std::fabs(float(i - 1) * d);
+            dh = std::fabs((float(i - 1) - 0.5f) * d);
+        } else {
+            di = std::fabs(float(i - n) * d);
+            dh = std::fabs((float(i - n) - 0.5f) * d);
+        }
+        const int q = i + pml - 1;
+        damping(pmlvp, d, di, pml, dt, alphamax, &P.ai[q], &P.bi[q]);
+        damping(pmlvp, d, dh, pml, dt, alphamax, &P.ah[q], &P.bh[q]);
+    }
+    return P;
+}
+
+}  // namespace owl
