@@ -198,10 +198,11 @@ owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double
  * counterpart (OWL has no profiler); used by the benchmark.  kind:
  * 0 p-sweep, 1 v-sweep, 2 source/receiver step, 3 imaging condition,
  * 4 boundary injection, 5 adjoint-source injection, 6 misfit, 7 merge.
- * bytes_per_launch: algorithmic bytes of one launch (0 if not modelled). */
+ * Totals since profiling was switched on: device time, launch count, and
+ * algorithmic (compulsory) bytes of those launches (0 if not modelled). */
 owl_status owl_acoustic2d_set_profiling(owl_acoustic2d *b, int on);
 owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d *b, int kind, double *total_ms,
-                                       long long *launches, double *bytes_per_launch);
+                                       long long *launches, double *bytes_total);
 
 #ifdef __cplusplus
 }

@@ -141,7 +141,7 @@ class Acoustic2DBatch:
         for k, name in enumerate(self.KERNELS):
             ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
             check(lib().owl_acoustic2d_kernel_stats(self._h, k, C.byref(ms), C.byref(n), C.byref(by)))
-            out[name] = dict(total_ms=ms.value, launches=n.value, bytes_per_launch=by.value)
+            out[name] = dict(total_ms=ms.value, launches=n.value, bytes_total=by.value)
         return out
 
     def gradient(self, obs=None, mode=_lib.OWL_GRAD_OWL, ckpt=_lib.OWL_CKPT_FULL, ckpt_param=0,

@@ -524,11 +524,12 @@ struct owl_acoustic2d {
     // per-kernel timing with HIP events on the batch stream
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, int>> ev_used;  // (kernel kind, first event index)
+    struct EvUse { int kind, first; double bytes; };
+    std::vector<EvUse> ev_used;    // launches awaiting collection
     size_t ev_next = 0;
     double kern_ms[8] = {0};
     long long kern_launches[8] = {0};
-    double kern_bytes[8] = {0};    // algorithmic bytes per launch, by kind
+    double kern_bytes[8] = {0};    // algorithmic bytes of all recorded launches, by kind
     std::vector<float> vp_full, rho_full;  // host copy of the full model
 
     DevBuf<Shot> d_shots;
@@ -633,6 +634,9 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     std::vector<Tap> inj_taps, adj_taps, rec_taps;
     std::vector<float> rec_w;
     b->pts_full = 0;
+    b->nrec_total = 0;   // every per-plan table is rebuilt from zero
+    b->max_ninj = 0;
+    b->max_nadj = 0;
     for (int s = 0; s < nshot; ++s) {
         const owl_shot& in = shots[s];
         ShotHost& H = b->hs[s];
@@ -815,6 +819,29 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     b->strip_elems = size_t(strip);
     b->edge_elems = size_t(edge);
 
+    // Every table a kernel indexes must cover the ranges the descriptors
+    // name; checked here, on the host, before anything is launched.
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        const Shot& S = b->sd[s];
+        const bool ok =
+            S.rec0 >= 0 && size_t(S.rec0) + size_t(S.nr) <= rec_w.size() &&
+            size_t(S.rec0) + size_t(S.nr) + 1 <= rec_beg.size() &&
+            S.inj0 >= 0 && size_t(S.inj0) + size_t(S.ninj) + (S.ninj ? 1 : 0) <= inj_beg.size() &&
+            S.adj0 >= 0 && size_t(S.adj0) + size_t(S.nadj) + (S.nadj ? 1 : 0) <= adj_beg.size() &&
+            size_t(S.trace0) + size_t(nt) * size_t(S.nr) <= b->trace_elems &&
+            size_t(S.strip0) + size_t(nt) * size_t(2 * R * (S.nz + 2 * R) + 2 * (S.nx + 2 * R) * R) <= b->strip_elems &&
+            size_t(S.g0) + size_t(S.nx) * size_t(S.nz) <= b->grad_elems &&
+            size_t(S.edge0) + size_t(2 * S.nx + 2 * S.nz) <= b->edge_elems &&
+            size_t(S.off) + size_t(S.pitch) * size_t(S.nrow) <= b->field_elems &&
+            size_t(S.prof) + size_t(4 * (S.nx + 2 * pml) + 4 * (S.nz + 2 * pml)) <= profs.size();
+        if (!ok) fail(OWL_ERR_ARG, "internal layout check failed for shot " + std::to_string(s));
+    }
+    for (const Tap& tp : inj_taps)
+        if (tp.cell < 0 || size_t(tp.cell) >= b->field_elems || size_t(tp.row + 1) * nt > amp.size())
+            fail(OWL_ERR_ARG, "internal layout check failed for a source tap");
+    for (const Tap& tp : rec_taps)
+        if (tp.cell < 0 || size_t(tp.cell) >= b->field_elems) fail(OWL_ERR_ARG, "internal layout check failed for a receiver tap");
+
     if (!b->stream) {
         OWL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         OWL_HIP(hipEventCreate(&b->ev0));
@@ -852,7 +879,7 @@ Sets one_set(float* p, float* vx, float* vz, float* const* m) {
 enum KernelKind { kKP = 0, kKV = 1, kKPost = 2, kKImage = 3, kKStrips = 4, kKAdj = 5, kKMisfit = 6, kKMerge = 7 };
 
 // Begin/end a timed region around one launch when profiling is on.
-int prof_begin(owl_acoustic2d* b, int kind) {
+int prof_begin(owl_acoustic2d* b, int kind, double bytes = 0.0) {
     if (!b->profiling) return -1;
     if (b->ev_next + 2 > b->ev_pool.size()) {
         for (int k = 0; k < 256; ++k) {
@@ -864,7 +891,7 @@ int prof_begin(owl_acoustic2d* b, int kind) {
     const int first = int(b->ev_next);
     b->ev_next += 2;
     OWL_HIP(hipEventRecord(b->ev_pool[first], b->stream));
-    b->ev_used.push_back({kind, first});
+    b->ev_used.push_back({kind, first, bytes});
     return first;
 }
 void prof_end(owl_acoustic2d* b, int first) {
@@ -875,9 +902,10 @@ void prof_end(owl_acoustic2d* b, int first) {
 void prof_collect(owl_acoustic2d* b) {
     for (auto& u : b->ev_used) {
         float ms = 0.f;
-        OWL_HIP(hipEventElapsedTime(&ms, b->ev_pool[u.second], b->ev_pool[u.second + 1]));
-        b->kern_ms[u.first] += ms;
-        b->kern_launches[u.first] += 1;
+        OWL_HIP(hipEventElapsedTime(&ms, b->ev_pool[u.first], b->ev_pool[u.first + 1]));
+        b->kern_ms[u.kind] += ms;
+        b->kern_launches[u.kind] += 1;
+        b->kern_bytes[u.kind] += u.bytes;
     }
     b->ev_used.clear();
     b->ev_next = 0;
@@ -888,7 +916,8 @@ void prof_collect(owl_acoustic2d* b) {
 //   p-sweep: vx, vz, p, bk read, p written (20 B/point) + 8 B per slab
 //            cell and damped direction (memory variable read + write);
 //   v-sweep: p, vx, vz, rho read, vx, vz written (24 B/point) + slabs.
-void set_kernel_bytes(owl_acoustic2d* b, int nsets) {
+struct StepBytes { double p, v; };
+StepBytes step_bytes(const owl_acoustic2d* b, int nsets) {
     double bp = 0, bv = 0;
     const int pml = b->pml;
     for (auto& S : b->sd) {
@@ -899,15 +928,15 @@ void set_kernel_bytes(owl_acoustic2d* b, int nsets) {
         bp += 20.0 * pts + 8.0 * slab_p;
         bv += 24.0 * pts + 8.0 * slab_v;
     }
-    b->kern_bytes[kKP] = bp * nsets;
-    b->kern_bytes[kKV] = bv * nsets;
+    return StepBytes{bp * nsets, bv * nsets};
 }
 
 template <int R>
 void launch_step(owl_acoustic2d* b, const Sets& sets, int nsets, float dt) {
     const int fs = b->cfg.free_surface ? 1 : 0;
     const dim3 blk(kBX, kBY), grd(unsigned(b->tile_shot.size()), unsigned(nsets));
-    int ev = prof_begin(b, kKP);
+    const StepBytes sb = b->profiling ? step_bytes(b, nsets) : StepBytes{0, 0};
+    int ev = prof_begin(b, kKP, sb.p);
     hipLaunchKernelGGL(k_update_p<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), sets,
                        b->d_bk.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, fs);
     prof_end(b, ev);
@@ -915,7 +944,7 @@ void launch_step(owl_acoustic2d* b, const Sets& sets, int nsets, float dt) {
         const dim3 g2(64, unsigned(b->sd.size()), unsigned(nsets));
         hipLaunchKernelGGL(k_mirror_p<R>, g2, dim3(256), 0, b->stream, b->d_shots.get(), sets, b->pml);
     }
-    ev = prof_begin(b, kKV);
+    ev = prof_begin(b, kKV, sb.v);
     hipLaunchKernelGGL(k_update_v<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), sets,
                        b->d_rho.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, fs);
     prof_end(b, ev);
@@ -956,7 +985,6 @@ void run_forward(owl_acoustic2d* b, bool save) {
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
     prof_collect(b);
-    set_kernel_bytes(b, 1);
     float ms = 0.f;
     OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
@@ -999,6 +1027,9 @@ void run_reverse_owl(owl_acoustic2d* b) {
     DevBuf<float>* B = b->fB;
     const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
     const unsigned ig = unsigned(std::max(1, std::min(1024, (int)((b->grad_elems / nshot + 255) / 256))));
+    // imaging: p, prev p, pr, rho*vp^3 and six velocity fields read once
+    // (40 B/point), two FP64 accumulators read and written (32 B/point)
+    const double image_bytes = 72.0 * double(b->grad_elems);
     OWL_HIP(hipEventRecord(b->ev0, st));
     for (int t = b->nt; t >= hnt; --t) {
         int ev = prof_begin(b, kKStrips);
@@ -1018,7 +1049,7 @@ void run_reverse_owl(owl_acoustic2d* b) {
                                st, b->d_shots.get(), b->aF[0].get(), b->d_adj_beg.get(),
                                b->d_adj_taps.get(), b->d_rec_w.get(), b->d_adj.get(), t);
         prof_end(b, ev);
-        ev = prof_begin(b, kKImage);
+        ev = prof_begin(b, kKImage, (t % cc == 0) ? image_bytes : 0.0);
         if (t % cc == 0)
             hipLaunchKernelGGL(k_image_owl<R>, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(),
                                B[0].get(), B[1].get(), B[2].get(), A[0].get(), A[1].get(), A[2].get(),
@@ -1035,15 +1066,6 @@ void run_reverse_owl(owl_acoustic2d* b) {
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
     prof_collect(b);
-    set_kernel_bytes(b, 2);
-    {
-        // imaging: p, prev p, pr, 1/(rho vp^3) and the six velocity fields
-        // read (40 B/point incl. the +1 neighbours once), two FP64
-        // accumulators read and written (32 B/point)
-        double pts = 0;
-        for (auto& S : b->sd) pts += double(S.nx) * S.nz;
-        b->kern_bytes[kKImage] = 72.0 * pts;
-    }
     // the forward state now lives in whichever buffer A points to; the
     // boundary records have been consumed
     if (A != b->fA)
@@ -1276,17 +1298,18 @@ owl_status owl_acoustic2d_set_profiling(owl_acoustic2d* b, int on) {
         for (int k = 0; k < 8; ++k) {
             b->kern_ms[k] = 0;
             b->kern_launches[k] = 0;
+            b->kern_bytes[k] = 0;
         }
     });
 }
 
 owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d* b, int kind, double* total_ms,
-                                       long long* launches, double* bytes_per_launch) {
+                                       long long* launches, double* bytes_total) {
     return guarded([&] {
         if (!b || kind < 0 || kind > 7) fail(OWL_ERR_ARG, "bad kernel kind");
         if (total_ms) *total_ms = b->kern_ms[kind];
         if (launches) *launches = b->kern_launches[kind];
-        if (bytes_per_launch) *bytes_per_launch = b->kern_bytes[kind];
+        if (bytes_total) *bytes_total = b->kern_bytes[kind];
     });
 }
 
