@@ -196,8 +196,11 @@ owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double
 
 /* Per-kernel timing with HIP events on the batch's stream.  No OWL
  * counterpart (OWL has no profiler); used by the benchmark.  kind:
- * 0 p-sweep, 1 v-sweep, 2 source/receiver step, 3 imaging condition,
- * 4 boundary injection, 5 adjoint-source injection, 6 misfit, 7 merge.
+ * 0 p-sweep and 1 v-sweep over one field set (forward), 2 source/receiver
+ * step, 3 imaging condition, 4 boundary injection, 5 adjoint-source
+ * injection, 6 misfit, 7 merge, 8 p-sweep and 9 v-sweep over two field sets
+ * (reverse pass: reconstruction + adjoint), 10 exact-adjoint p-pass,
+ * 11 exact-adjoint v-pass, 12 state storage, 0 <= kind < 16.
  * Totals since profiling was switched on: device time, launch count, and
  * algorithmic (compulsory) bytes of those launches (0 if not modelled). */
 owl_status owl_acoustic2d_set_profiling(owl_acoustic2d *b, int on);

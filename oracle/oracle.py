@@ -220,6 +220,29 @@ class Oracle:
             raise RuntimeError(f"oracle acoustic2d_gradient_owl failed ({rc})")
         return np.ascontiguousarray(gv_f), np.ascontiguousarray(gr_f), tr
 
+    def acoustic2d_gradient_exact(self, g, vp, rho, sources, receivers, obs):
+        """Exact adjoint gradient of 1/2 sum |d - obs|^2 w.r.t. (vp, rho):
+        returns (grad_vp, grad_rho) as (nz, nx), misfit sum |d - obs|^2, traces."""
+        G = self._grid2d(g)
+        vp = self._model(vp, (G.nz, G.nx))
+        rho = self._model(rho, (G.nz, G.nx))
+        src, keep, f0a, _ = self._sources2d(g, sources)
+        rec = self._receivers2d(receivers)
+        obs = np.ascontiguousarray(obs, dtype=self.real)
+        assert obs.shape == (len(receivers), G.nt)
+        gv = np.zeros((G.nz, G.nx), self.real, order="F")
+        gr = np.zeros((G.nz, G.nx), self.real, order="F")
+        tr = np.zeros((len(receivers), G.nt), self.real)
+        mis = C.c_double(0.0)
+        rc = self._fn("acoustic2d_gradient_exact")(
+            C.byref(G), vp.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p),
+            len(sources), src, f0a.ctypes.data_as(C.c_void_p), len(receivers), rec,
+            obs.ctypes.data_as(C.c_void_p), gv.ctypes.data_as(C.c_void_p),
+            gr.ctypes.data_as(C.c_void_p), C.byref(mis), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle acoustic2d_gradient_exact failed ({rc})")
+        return np.ascontiguousarray(gv), np.ascontiguousarray(gr), mis.value, tr
+
     def adjoint_source_l2(self, weight, syn, obs, threshold=1.0e-6):
         syn = np.ascontiguousarray(syn, dtype=self.real)
         obs = np.ascontiguousarray(obs, dtype=self.real)

@@ -963,3 +963,193 @@ int ORA_SYM(acoustic3d_forward)(const ora_grid3d *g, const float *vp, const floa
     free(bk); free(rh);
     return 0;
 }
+
+/* ======================================================================== */
+/* Exact discrete adjoint of the 2D forward scheme (no OWL counterpart: OWL */
+/* reconstructs the forward field backwards; this is the transpose of the  */
+/* forward step of acoustic_iso_2d/module_wavefield.f90:40-107 including   */
+/* the CFS-PML memory variables).                                           */
+/*                                                                          */
+/* Forward: u_t = V(P(u_{t-1})) + s_t for t = 1..nt-1, d_t = R u_{t-1}.     */
+/* Misfit phi = 1/2 sum_t |d_t - obs_t|^2; returns d phi / d(vp, rho) on   */
+/* the (nz, nx) model, with the edge-replicated padding folded back onto   */
+/* the edge nodes.  PML profiles are treated as constants (they depend on  */
+/* the model only through a max over the edge lines).  Explosion sources.  */
+/* misfit_out receives sum_t |d_t - obs_t|^2 (OWL's convention, no 1/2).    */
+/* ======================================================================== */
+int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, const float *rho,
+                                       int ns, const ora_source2d *src, const float *f0s,
+                                       int nr, const ora_receiver2d *rec, const real *obs,
+                                       real *grad_vp, real *grad_rho, double *misfit_out,
+                                       real *traces_out) {
+    if (g->R < 2 || g->R > 8 || g->free_surface) return -1;
+    for (int k = 0; k < ns; ++k) if (src[k].mechanism != 0) return -2;
+    med2 M;
+    med2_init(&M, g, vp, rho, f0max_of(f0s, ns));
+    const lay2 *L = &M.L;
+    const int nx = g->nx, nz = g->nz, pml = g->pml, R = g->R, nt = g->nt;
+    const size_t n = (size_t)L->NX * L->NZ;
+    const real dt = (real)g->dt;
+    const float *c = M.cf;
+    ora_point *sp = (ora_point *)calloc(ns > 0 ? ns : 1, sizeof(ora_point));
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    for (int k = 0; k < ns; ++k)
+        ORA_SYM(locate_point)(2, src[k].x, 0.0f, src[k].z, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, &sp[k]);
+    for (int k = 0; k < nr; ++k)
+        ORA_SYM(locate_point)(2, rec[k].x, 0.0f, rec[k].z, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, &rp[k]);
+
+    /* forward, keeping every state: P[t], PP[t] (p before the source), VX[t], VZ[t] */
+    real *P = zalloc(n * nt), *PP = zalloc(n * nt), *VX = zalloc(n * nt), *VZ = zalloc(n * nt);
+    real *tr = zalloc((size_t)nr * nt);
+    fld2 F;
+    fld2_alloc(&F, L);
+    for (int t = 1; t <= nt; ++t) {
+        record2(L, F.p, rec, rp, nr, t, nt, tr);
+        if (t == nt) break;
+        update2(&M, &F, dt, 0);
+        memcpy(PP + n * t, F.p, n * sizeof(real));
+        add_source2(&M, &F, src, sp, ns, t, dt, 0);
+        memcpy(P + n * t, F.p, n * sizeof(real));
+        memcpy(VX + n * t, F.vx, n * sizeof(real));
+        memcpy(VZ + n * t, F.vz, n * sizeof(real));
+    }
+    if (traces_out) memcpy(traces_out, tr, sizeof(real) * (size_t)nr * nt);
+    /* residual r_t = d_t - obs_t */
+    double mis = 0.0;
+    real *res = zalloc((size_t)nr * nt);
+    for (int ir = 0; ir < nr; ++ir) {
+        if (rec[ir].weight == 0.0f) continue;
+        for (int t = 0; t < nt; ++t) {
+            size_t q = (size_t)ir * nt + t;
+            res[q] = tr[q] - obs[q];
+            mis += (double)res[q] * (double)res[q];
+        }
+    }
+    if (misfit_out) *misfit_out = mis;
+
+    /* adjoint state */
+    real *ph = zalloc(n), *vxh = zalloc(n), *vzh = zalloc(n);
+    real *psx = zalloc(n), *psz = zalloc(n), *phx = zalloc(n), *phz = zalloc(n);
+    real *ex = zalloc(n), *ez = zalloc(n), *dxa = zalloc(n), *dza = zalloc(n);
+    double *gK = (double *)calloc(n, sizeof(double)), *gR = (double *)calloc(n, sizeof(double));
+#define INJECT_RT(tt)                                                                         \
+    for (int ir = 0; ir < nr; ++ir) {                                                         \
+        if (rec[ir].weight == 0.0f) continue;                                                 \
+        const ora_point *Pt = &rp[ir];                                                        \
+        real a_ = res[(size_t)ir * nt + ((tt) - 1)];                                          \
+        for (int irz = -NKW; irz <= NKW; ++irz)                                               \
+            for (int irx = -NKW; irx <= NKW; ++irx) {                                         \
+                size_t q_ = I2(*L, Pt->gx + irx, Pt->gz + irz);                               \
+                ph[q_] += a_ * (real)Pt->ix[irx + NKW] * (real)Pt->iz[irz + NKW] * (real)rec[ir].weight; \
+            }                                                                                 \
+    }
+    INJECT_RT(nt);
+    for (int t = nt - 1; t >= 1; --t) {
+        const real *pc = PP + n * t, *pv = P + n * (t - 1);
+        const real *vxc = VX + n * t, *vxp = VX + n * (t - 1), *vzc = VZ + n * t, *vzp = VZ + n * (t - 1);
+        /* (a) transpose of the v-update: e = -(dt/rho_eff) v^ (+ PML), phi^ update */
+        for (int j = -pml + 1; j <= nz + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                size_t q = I2(*L, i, j);
+                real rx = (real)0.5 * (M.rho[q] + M.rho[q - 1]);
+                real r = -(dt / rx) * vxh[q];
+                if (i <= 1 || i >= nx + 1) {
+                    int a = i + pml - 1;
+                    real s = phx[q] + r;
+                    ex[q] = r + (real)M.axh[a] * s;
+                    phx[q] = (real)M.bxh[a] * s;
+                } else ex[q] = r;
+                /* rho gradient from vx: (vx_{t-1} - vx_t)/rho_eff * v^x */
+                real gx = (vxp[q] - vxc[q]) / rx * vxh[q];
+                gR[q] += 0.5 * (double)gx;
+                gR[q - 1] += 0.5 * (double)gx;
+                real rz = (real)0.5 * (M.rho[q] + M.rho[q - L->NX]);
+                r = -(dt / rz) * vzh[q];
+                if (j <= 1 || j >= nz + 1) {
+                    int a = j + pml - 1;
+                    real s = phz[q] + r;
+                    ez[q] = r + (real)M.azh[a] * s;
+                    phz[q] = (real)M.bzh[a] * s;
+                } else ez[q] = r;
+                real gz = (vzp[q] - vzc[q]) / rz * vzh[q];
+                gR[q] += 0.5 * (double)gz;
+                gR[q - L->NX] += 0.5 * (double)gz;
+            }
+        for (int j = -pml + 1; j <= nz + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                /* p^ += -D+^T... = -(idx * sum c (e(i+m) - e(i+1-m)) + z) */
+                size_t q = I2(*L, i, j);
+                real sx = 0, sz = 0;
+                for (int m = 1; m <= R; ++m) {
+                    int ia = i + m, ib = i + 1 - m, ja = j + m, jb = j + 1 - m;
+                    real xa = (ia <= nx + pml) ? ex[I2(*L, ia, j)] : 0, xb = (ib >= -pml + 1) ? ex[I2(*L, ib, j)] : 0;
+                    real za = (ja <= nz + pml) ? ez[I2(*L, i, ja)] : 0, zb = (jb >= -pml + 1) ? ez[I2(*L, i, jb)] : 0;
+                    sx += (real)c[m - 1] * (xa - xb);
+                    sz += (real)c[m - 1] * (za - zb);
+                }
+                ph[q] = ph[q] - (M.idx * sx + M.idz * sz);
+            }
+        /* (b) transpose of the p-update: K gradient, d = -dt K p^ (+ PML), v^ update */
+        for (int j = -pml + 1; j <= nz + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                size_t q = I2(*L, i, j);
+                gK[q] += (double)((pc[q] - pv[q]) / M.bk[q] * ph[q]);
+                real qv = -dt * M.bk[q] * ph[q];
+                if (i <= 0 || i >= nx + 1) {
+                    int a = i + pml - 1;
+                    real s = psx[q] + qv;
+                    dxa[q] = qv + (real)M.axi[a] * s;
+                    psx[q] = (real)M.bxi[a] * s;
+                } else dxa[q] = qv;
+                if (j <= 0 || j >= nz + 1) {
+                    int a = j + pml - 1;
+                    real s = psz[q] + qv;
+                    dza[q] = qv + (real)M.azi[a] * s;
+                    psz[q] = (real)M.bzi[a] * s;
+                } else dza[q] = qv;
+            }
+        for (int j = -pml + 1; j <= nz + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                size_t q = I2(*L, i, j);
+                real sx = 0, sz = 0;
+                for (int m = 1; m <= R; ++m) {
+                    int ia = i - 1 + m, ib = i - m, ja = j - 1 + m, jb = j - m;
+                    real xa = (ia <= nx + pml) ? dxa[I2(*L, ia, j)] : 0, xb = (ib >= -pml + 1) ? dxa[I2(*L, ib, j)] : 0;
+                    real za = (ja <= nz + pml) ? dza[I2(*L, i, ja)] : 0, zb = (jb >= -pml + 1) ? dza[I2(*L, i, jb)] : 0;
+                    sx += (real)c[m - 1] * (xa - xb);
+                    sz += (real)c[m - 1] * (za - zb);
+                }
+                vxh[q] = vxh[q] - M.idx * sx;
+                vzh[q] = vzh[q] - M.idz * sz;
+            }
+        /* (c) receivers of d_t = R u_{t-1} */
+        INJECT_RT(t);
+    }
+#undef INJECT_RT
+    /* fold the edge-replicated padding back onto the window and change
+     * variables (K = rho vp^2): d/dvp = 2 rho vp dK, d/drho = drho + vp^2 dK */
+    double *fK = (double *)calloc((size_t)nx * nz, sizeof(double));
+    double *fR = (double *)calloc((size_t)nx * nz, sizeof(double));
+    for (int j = L->lo; j < L->lo + L->NZ; ++j)
+        for (int i = L->lo; i < L->lo + L->NX; ++i) {
+            int ic = i < 1 ? 1 : (i > nx ? nx : i), jc = j < 1 ? 1 : (j > nz ? nz : j);
+            size_t o = (size_t)(ic - 1) + (size_t)(jc - 1) * nx;
+            fK[o] += gK[I2(*L, i, j)];
+            fR[o] += gR[I2(*L, i, j)];
+        }
+    for (int j = 1; j <= nz; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+            double v = vp[(size_t)(j - 1) + (size_t)(i - 1) * nz], r = rho[(size_t)(j - 1) + (size_t)(i - 1) * nz];
+            grad_vp[(size_t)(j - 1) + (size_t)(i - 1) * nz] = (real)(2.0 * r * v * fK[o]);
+            grad_rho[(size_t)(j - 1) + (size_t)(i - 1) * nz] = (real)(fR[o] + v * v * fK[o]);
+        }
+    free(fK); free(fR); free(gK); free(gR);
+    free(ph); free(vxh); free(vzh); free(psx); free(psz); free(phx); free(phz);
+    free(ex); free(ez); free(dxa); free(dza);
+    free(P); free(PP); free(VX); free(VZ); free(tr); free(res);
+    free(sp); free(rp);
+    fld2_free(&F);
+    med2_free(&M);
+    return 0;
+}

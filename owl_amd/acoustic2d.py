@@ -134,7 +134,8 @@ class Acoustic2DBatch:
         check(lib().owl_acoustic2d_set_profiling(self._h, int(bool(on))))
 
     KERNELS = ("update_p", "update_v", "post_step", "image", "inject_strips", "inject_adjoint",
-               "misfit", "merge")
+               "misfit", "merge", "update_p_2sets", "update_v_2sets", "adjoint_p", "adjoint_v",
+               "store")
 
     def kernel_stats(self):
         out = {}

@@ -527,9 +527,9 @@ struct owl_acoustic2d {
     struct EvUse { int kind, first; double bytes; };
     std::vector<EvUse> ev_used;    // launches awaiting collection
     size_t ev_next = 0;
-    double kern_ms[8] = {0};
-    long long kern_launches[8] = {0};
-    double kern_bytes[8] = {0};    // algorithmic bytes of all recorded launches, by kind
+    double kern_ms[16] = {0};
+    long long kern_launches[16] = {0};
+    double kern_bytes[16] = {0};   // algorithmic bytes of all recorded launches, by kind
     std::vector<float> vp_full, rho_full;  // host copy of the full model
 
     DevBuf<Shot> d_shots;
@@ -876,7 +876,8 @@ Sets one_set(float* p, float* vx, float* vz, float* const* m) {
     return S;
 }
 
-enum KernelKind { kKP = 0, kKV = 1, kKPost = 2, kKImage = 3, kKStrips = 4, kKAdj = 5, kKMisfit = 6, kKMerge = 7 };
+enum KernelKind { kKP = 0, kKV = 1, kKPost = 2, kKImage = 3, kKStrips = 4, kKAdj = 5, kKMisfit = 6, kKMerge = 7,
+                  kKP2 = 8, kKV2 = 9, kKinds = 16 };
 
 // Begin/end a timed region around one launch when profiling is on.
 int prof_begin(owl_acoustic2d* b, int kind, double bytes = 0.0) {
@@ -936,7 +937,7 @@ void launch_step(owl_acoustic2d* b, const Sets& sets, int nsets, float dt) {
     const int fs = b->cfg.free_surface ? 1 : 0;
     const dim3 blk(kBX, kBY), grd(unsigned(b->tile_shot.size()), unsigned(nsets));
     const StepBytes sb = b->profiling ? step_bytes(b, nsets) : StepBytes{0, 0};
-    int ev = prof_begin(b, kKP, sb.p);
+    int ev = prof_begin(b, nsets == 2 ? kKP2 : kKP, sb.p);
     hipLaunchKernelGGL(k_update_p<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), sets,
                        b->d_bk.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, fs);
     prof_end(b, ev);
@@ -944,7 +945,7 @@ void launch_step(owl_acoustic2d* b, const Sets& sets, int nsets, float dt) {
         const dim3 g2(64, unsigned(b->sd.size()), unsigned(nsets));
         hipLaunchKernelGGL(k_mirror_p<R>, g2, dim3(256), 0, b->stream, b->d_shots.get(), sets, b->pml);
     }
-    ev = prof_begin(b, kKV, sb.v);
+    ev = prof_begin(b, nsets == 2 ? kKV2 : kKV, sb.v);
     hipLaunchKernelGGL(k_update_v<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), sets,
                        b->d_rho.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, fs);
     prof_end(b, ev);
@@ -1295,7 +1296,7 @@ owl_status owl_acoustic2d_set_profiling(owl_acoustic2d* b, int on) {
     return guarded([&] {
         if (!b) fail(OWL_ERR_ARG, "null batch");
         b->profiling = on != 0;
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < 16; ++k) {
             b->kern_ms[k] = 0;
             b->kern_launches[k] = 0;
             b->kern_bytes[k] = 0;
@@ -1306,7 +1307,7 @@ owl_status owl_acoustic2d_set_profiling(owl_acoustic2d* b, int on) {
 owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d* b, int kind, double* total_ms,
                                        long long* launches, double* bytes_total) {
     return guarded([&] {
-        if (!b || kind < 0 || kind > 7) fail(OWL_ERR_ARG, "bad kernel kind");
+        if (!b || kind < 0 || kind > 15) fail(OWL_ERR_ARG, "bad kernel kind");
         if (total_ms) *total_ms = b->kern_ms[kind];
         if (launches) *launches = b->kern_launches[kind];
         if (bytes_total) *bytes_total = b->kern_bytes[kind];

@@ -261,3 +261,48 @@ def test_owl_gradient_dot_product_consistency(oracle32):
     ratios = np.array(ratios)
     assert np.all(ratios > 0), ratios
     assert np.max(np.abs(ratios - ratios[0])) / abs(ratios[0]) < 0.03, ratios
+
+
+# ---- exact discrete adjoint (oracle, float64) -----------------------------
+def _exact_case(o):
+    from owl_amd.synthetic import gardner, random_field
+    nz, nx = 30, 40
+    vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 1))).astype(np.float32)
+    rho = gardner(vp)
+    g = dict(nx=nx, nz=nz, dx=10.0, dz=10.0, dt=0.001, nt=180, pml=10, R=4)
+    src = [dict(x=200.3, z=150.7, f0=20.0)]
+    rec = [dict(x=15.0 + 10 * k, z=40.5) for k in range(38)]
+    obs = o.acoustic2d_forward(g, (vp * 1.02).astype(np.float32), rho, src, rec)
+    return g, vp, rho, src, rec, obs
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_exact_adjoint_matches_finite_differences(oracle64, seed):
+    """Central differences of phi = 1/2 |d - obs|^2 along 10 seeded smooth
+    perturbations of vp and rho (zero on the window's edge lines, which set
+    the PML strength) agree with <g, dm> to O(eps^2)."""
+    from owl_amd.synthetic import random_field
+    g, vp, rho, src, rec, obs = _exact_case(oracle64)
+    gv, gr, _, _ = oracle64.acoustic2d_gradient_exact(g, vp, rho, src, rec, obs)
+    nz, nx = vp.shape
+    taper = np.zeros((nz, nx))
+    taper[1:-1, 1:-1] = 1.0
+
+    def phi(v, r):
+        d = oracle64.acoustic2d_forward(g, v, r, src, rec)
+        return 0.5 * np.sum((d - obs) ** 2)
+
+    dm = random_field((nz, nx), 3.0, 100 + seed) * taper
+    for which in ("vp", "rho"):
+        base = vp if which == "vp" else rho
+        step = 1e-4 * float(np.abs(base).max()) * dm
+        plus = (base + step).astype(np.float32)
+        minus = (base - step).astype(np.float32)
+        dact = (plus.astype(np.float64) - minus) / 2
+        if which == "vp":
+            fd = (phi(plus, rho) - phi(minus, rho)) / 2
+            an = np.sum(gv * dact)
+        else:
+            fd = (phi(vp, plus) - phi(vp, minus)) / 2
+            an = np.sum(gr * dact)
+        assert abs(fd / an - 1) < 1e-5, (which, fd, an)
