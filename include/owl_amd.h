@@ -169,6 +169,19 @@ owl_status owl_acoustic2d_gradient(owl_acoustic2d *b, const float *obs, int mode
                                    int ckpt_param, int store, double *misfit, double *grad_vp,
                                    double *grad_rho, float *traces_out);
 
+/* Differentiable path (exact adjoint): model every shot and keep the forward
+ * states the adjoint needs under the given storage policy (OWL_CKPT_*,
+ * ckpt_param, OWL_STORE_*); then owl_acoustic2d_backward applies the
+ * transpose of the forward map to dtraces (dL/dtraces, laid out like the
+ * traces) and returns dL/dvp, dL/drho on the full model (nz, nx), z fastest,
+ * summed over shots.  No OWL counterpart (OWL's gradient is the
+ * reconstruction algorithm of OWL_GRAD_OWL); the forward map is OWL's
+ * compute_forward.  Explosion sources, no free surface. */
+owl_status owl_acoustic2d_forward_store(owl_acoustic2d *b, float *traces, int ckpt, int ckpt_param,
+                                        int store);
+owl_status owl_acoustic2d_backward(owl_acoustic2d *b, const float *dtraces, double *grad_vp,
+                                   double *grad_rho);
+
 /* Adjoint pass alone, OWL's algorithm, on data from a preceding
  * owl_acoustic2d_forward(..., save_boundary = 1): adjoint source traces adj
  * (laid out like the forward traces) in, per-shot window gradients out

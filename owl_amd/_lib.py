@@ -91,6 +91,8 @@ _SIGNATURES = {
     "owl_acoustic2d_observe": (C.c_int, [_P]),
     "owl_acoustic2d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_acoustic2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "owl_acoustic2d_forward_store": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int]),
+    "owl_acoustic2d_backward": (C.c_int, [_P, _P, _P, _P]),
 }
 
 _lib = None

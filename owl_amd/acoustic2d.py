@@ -161,6 +161,25 @@ class Acoustic2DBatch:
         return (mis, np.ascontiguousarray(gv), np.ascontiguousarray(gr),
                 self._split(syn) if syn is not None else None)
 
+    def forward_store(self, ckpt=_lib.OWL_CKPT_FULL, ckpt_param=0, store=_lib.OWL_STORE_F32):
+        """Model every shot and keep the states the exact adjoint needs."""
+        flat = np.zeros(sum(self.nr) * self.nt, np.float32)
+        check(lib().owl_acoustic2d_forward_store(self._h, _ptr(flat), ckpt, ckpt_param, store))
+        return self._split(flat)
+
+    def backward(self, dtraces):
+        """Exact adjoint after forward_store: dL/dvp, dL/drho as (nz, nx) float64
+        for the trace gradient dtraces (list of (nr, nt) or one flat array)."""
+        if isinstance(dtraces, (list, tuple)):
+            flat = np.concatenate([np.asarray(a, np.float32).ravel() for a in dtraces])
+        else:
+            flat = np.asarray(dtraces, np.float32).ravel()
+        flat = np.ascontiguousarray(flat)
+        gv = np.zeros((self.nz, self.nx), np.float64, order="F")
+        gr = np.zeros((self.nz, self.nx), np.float64, order="F")
+        check(lib().owl_acoustic2d_backward(self._h, _ptr(flat), _ptr(gv), _ptr(gr)))
+        return np.ascontiguousarray(gv), np.ascontiguousarray(gr)
+
     def last_stats(self):
         ms, up = C.c_double(), C.c_double()
         check(lib().owl_acoustic2d_last_stats(self._h, C.byref(ms), C.byref(up)))

@@ -65,7 +65,10 @@ public:
         if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
         return *this;
     }
+    // (Re)allocate to n elements; keeps the buffer (and its contents) when the
+    // size is unchanged.
     void alloc(size_t n) {
+        if (p_ && n == n_) return;
         release();
         if (n == 0) return;
         OWL_HIP(hipMalloc(&p_, n * sizeof(T)));
