@@ -220,6 +220,33 @@ owl_status owl_acoustic2d_set_profiling(owl_acoustic2d *b, int on);
 owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d *b, int kind, double *total_ms,
                                        long long *launches, double *bytes_total);
 
+/* ---- 3D acoustic: batched shots on one GPU --------------------------------
+ * OWL's acoustic_iso_3d forward solver (compute_forward,
+ * acoustic_iso_3d/submodule_forward_modeling.f90:31-183; update_wavefield
+ * module_wavefield.f90:40-172 without domain decomposition) for
+ * which_medium = 'acoustic-iso' in src/forward/3d/module_modeling.f90.
+ * vp, rho: (nz, ny, nx) z fastest.  Explosion sources, no free surface.
+ * Traces per shot [nr][nt]. */
+typedef struct owl_acoustic3d owl_acoustic3d;
+owl_status owl_acoustic3d_create(const owl_acoustic_config *cfg, const float *vp,
+                                 const float *rho, int nshot, const owl_shot *shots,
+                                 owl_acoustic3d **out);
+void owl_acoustic3d_destroy(owl_acoustic3d *b);
+int owl_acoustic3d_nt(const owl_acoustic3d *b);
+owl_status owl_acoustic3d_forward(owl_acoustic3d *b, float *traces);
+/* Launch shape of the sweeps (results do not depend on it; no OWL
+ * counterpart): z planes one block marches over, and the kernel variant
+ * (0: x-y planes staged in LDS; 1, 2, 3: barrier-free column segments with
+ * 4, 8, 16 rows per block, 5-7 the same with an XCD-contiguous tile order;
+ * 8: four-wide columns (float4 per thread, the default), 24: the same with
+ * the XCD-contiguous order). */
+owl_status owl_acoustic3d_set_launch(owl_acoustic3d *b, int planes, int variant);
+/* Timing as for the 2D batch; kinds: 0 p-sweep, 1 v-sweep, 2 source/receivers. */
+owl_status owl_acoustic3d_set_profiling(owl_acoustic3d *b, int on);
+owl_status owl_acoustic3d_kernel_stats(const owl_acoustic3d *b, int kind, double *total_ms,
+                                       long long *launches, double *bytes_total);
+owl_status owl_acoustic3d_last_stats(const owl_acoustic3d *b, double *ms, double *updates);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1216 @@
+// 3D acoustic SSG-FD time stepping for MI355X (gfx950): OWL's
+// acoustic_iso_3d forward solver (acoustic_iso_3d/module_wavefield.f90:40-172,
+// add_source :322-444, submodule_forward_modeling.f90:31-183) for a batch of
+// shots, each on its own window, in every launch.
+//
+// Layout: per shot a padded grid x fastest, then y, then z; rows padded to a
+// multiple of 64 floats.  The sweeps tile x-y in 64 x 16 blocks and march
+// along z: x and y neighbours come from LDS (one plane with its halo), the
+// z neighbours from a per-thread register queue of 2R values, so each field
+// element is read from HBM once per sweep.  FP32 in the reference's
+// operation order (no FMA contraction).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "host_prep.hpp"
+#include "owl_amd.h"
+#include "profiler.hpp"
+
+namespace owl {
+namespace a3 {
+
+constexpr int kBX = 64;          // threads along x
+constexpr int kBY = 4;           // thread rows
+constexpr int kRY = 4;           // y rows per thread
+constexpr int kTY = kBY * kRY;   // y extent of a tile
+constexpr int kPitchAlign = 64;
+constexpr int kCY = 16;          // rows per block of the column variant
+constexpr int kXcd = 8;          // XCDs of an MI355X (L2 domains)
+
+struct Coef {
+    float c[8];
+};
+
+struct Shot3 {
+    int nx, ny, nz;        // window interior
+    int px;                // row pitch (floats)
+    int nyp, nzp;          // padded rows per plane, padded planes
+    long long pxy;         // plane pitch (floats)
+    long long off;         // element offset of node (lo, lo, lo)
+    int tile0, ntx, nty, nzc;
+    int prof;              // PML profiles: x (ai bi ah bh), y, z
+    int nr, rec0;          // receivers
+    int ninj, inj0;        // source cells
+    long long trace0;      // [nt][nr] block
+    int nxbeg, nybeg, nzbeg;
+    int xb;                // x index stored at row position 0 (<= lo; 16-byte aligned interior)
+};
+
+// Wavefields, updated in place; memory variables m[0..2] for the p-sweep
+// (dx vx, dy vy, dz vz on integer nodes), m[3..5] for the v-sweep (dx p,
+// dy p, dz p on half nodes).  Full-size arrays touched only in the slabs.
+struct F3 {
+    float* p;
+    float* vx;
+    float* vy;
+    float* vz;
+    float* m[6];
+};
+
+struct Tap3 {
+    long long cell;
+    float w1, w2, w3;
+    int row;        // amplitude row (source) or receiver index
+    int tbeg, tend; // active steps of a source tap
+};
+
+// Hardware hands consecutive workgroups to the XCDs round-robin; remap so
+// that each XCD walks a contiguous range of tiles (neighbouring tiles share
+// halos through that XCD's L2).  Returns -1 for the padding workgroups.
+__device__ __forceinline__ int xcd_tile(int ntiles, int remap) {
+    if (!remap) return int(blockIdx.x) < ntiles ? int(blockIdx.x) : -1;
+    const int per = (ntiles + kXcd - 1) / kXcd;
+    const int t = int(blockIdx.x % kXcd) * per + int(blockIdx.x / kXcd);
+    return t < ntiles ? t : -1;
+}
+
+__device__ __forceinline__ size_t at3(const Shot3& S, int lo, int i, int j, int k) {
+    return size_t(S.off) + size_t(i - S.xb) + size_t(j - lo) * size_t(S.px) + size_t(k - lo) * size_t(S.pxy);
+}
+
+// ---------------------------------------------------------------------------
+// p-sweep (module_wavefield.f90:66-102): p -= dt*bk*(dx+ vx + dy+ vy + dz+ vz)
+// with CFS-PML memory updates in the slabs.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(kBX * kBY) void k3_update_p(const Shot3* __restrict__ shots,
+                                                         const int* __restrict__ tile_shot, F3 F,
+                                                         const float* __restrict__ bk,
+                                                         const float* __restrict__ prof, Coef c,
+                                                         float dt, float idx, float idy, float idz,
+                                                         int pml, int kc) {
+    const int tile = blockIdx.x;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R, hx = nx + pml + R, hy = ny + pml + R;
+    const int i0 = -pml + 1 + tx * kBX, j0 = -pml + 1 + ty * kTY;
+    const int k0 = -pml + 1 + tzc * kc, k1 = min(k0 + kc - 1, nz + pml);
+    const int lane = threadIdx.x, tid = threadIdx.y * kBX + threadIdx.x;
+    const int i = i0 + lane;
+    __shared__ float svx[kTY][kBX + 2 * R];
+    __shared__ float svy[kTY + 2 * R][kBX];
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* ax_i = prof + S.prof;
+    const float* bx_i = ax_i + lx;
+    const float* ay_i = prof + S.prof + 4 * lx;
+    const float* by_i = ay_i + ly;
+    const float* az_i = prof + S.prof + 4 * lx + 4 * ly;
+    const float* bz_i = az_i + lz;
+    bool ok[kRY];
+    float q[kRY][2 * R];
+#pragma unroll
+    for (int r = 0; r < kRY; ++r) {
+        const int j = j0 + threadIdx.y + kBY * r;
+        ok[r] = (i <= nx + pml) && (j <= ny + pml);
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) q[r][m] = ok[r] ? F.vz[at3(S, lo, i, j, k0 - R + 1 + m)] : 0.0f;
+    }
+    const bool xslab = (i <= 0 || i >= nx + 1);
+    const float axv = xslab && i <= nx + pml ? ax_i[i + pml - 1] : 0.0f;
+    const float bxv = xslab && i <= nx + pml ? bx_i[i + pml - 1] : 0.0f;
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int r = 0; r < kRY; ++r) {
+            const int j = j0 + threadIdx.y + kBY * r;
+            q[r][2 * R - 1] = ok[r] ? F.vz[at3(S, lo, i, j, k + R)] : 0.0f;
+        }
+        __syncthreads();
+        // vx with its x halo: columns x = i0-R+1 .. i0+63+R at col = x - (i0-R+1)
+#pragma unroll
+        for (int r = 0; r < kRY; ++r) {
+            const int row = threadIdx.y + kBY * r, j = j0 + row;
+            svx[row][lane + R - 1] = (i <= hx && j <= hy) ? F.vx[at3(S, lo, i, j, k)] : 0.0f;
+        }
+        if (tid < (2 * R - 1) * kTY) {
+            const int hc = tid % (2 * R - 1), row = tid / (2 * R - 1);
+            const int col = hc < R - 1 ? hc : kBX + hc;
+            const int xg = i0 - R + 1 + col, j = j0 + row;
+            svx[row][col] = (xg >= lo && xg <= hx && j <= hy) ? F.vx[at3(S, lo, xg, j, k)] : 0.0f;
+        }
+        // vy with its y halo: rows y = j0-R+1 .. j0+15+R at row = y - (j0-R+1)
+        for (int e = tid; e < (kTY + 2 * R - 1) * kBX; e += kBX * kBY) {
+            const int row = e / kBX, col = e % kBX;
+            const int yg = j0 - R + 1 + row, xg = i0 + col;
+            svy[row][col] = (yg >= lo && yg <= hy && xg <= hx) ? F.vy[at3(S, lo, xg, yg, k)] : 0.0f;
+        }
+        __syncthreads();
+        const bool zslab = (k <= 0 || k >= nz + 1);
+        const float azv = zslab ? az_i[k + pml - 1] : 0.0f, bzv = zslab ? bz_i[k + pml - 1] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < kRY; ++r) {
+            if (ok[r]) {
+                const int row = threadIdx.y + kBY * r, j = j0 + row;
+                const int cx = lane + R - 1, cy = row + R - 1;
+                float sx = c.c[0] * (svx[row][cx + 1] - svx[row][cx]);
+                float sy = c.c[0] * (svy[cy + 1][lane] - svy[cy][lane]);
+                float sz = c.c[0] * (q[r][R] - q[r][R - 1]);
+#pragma unroll
+                for (int m = 2; m <= R; ++m) {
+                    sx = sx + c.c[m - 1] * (svx[row][cx + m] - svx[row][cx - m + 1]);
+                    sy = sy + c.c[m - 1] * (svy[cy + m][lane] - svy[cy - m + 1][lane]);
+                    sz = sz + c.c[m - 1] * (q[r][R - 1 + m] - q[r][R - m]);
+                }
+                float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+                const size_t qq = at3(S, lo, i, j, k);
+                if (xslab) {
+                    const float mem = axv * dxv + bxv * F.m[0][qq];
+                    F.m[0][qq] = mem;
+                    dxv = dxv + mem;
+                }
+                if (j <= 0 || j >= ny + 1) {
+                    const float mem = ay_i[j + pml - 1] * dyv + by_i[j + pml - 1] * F.m[1][qq];
+                    F.m[1][qq] = mem;
+                    dyv = dyv + mem;
+                }
+                if (zslab) {
+                    const float mem = azv * dzv + bzv * F.m[2][qq];
+                    F.m[2][qq] = mem;
+                    dzv = dzv + mem;
+                }
+                F.p[qq] = F.p[qq] - dt * bk[qq] * (dxv + dyv + dzv);
+            }
+#pragma unroll
+            for (int m = 0; m < 2 * R - 1; ++m) q[r][m] = q[r][m + 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// v-sweeps (module_wavefield.f90:106-170): v -= dt/rho_eff * d- p with
+// rho_eff the mean of the two nodes, CFS-PML in the half-node slabs.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(kBX * kBY) void k3_update_v(const Shot3* __restrict__ shots,
+                                                         const int* __restrict__ tile_shot, F3 F,
+                                                         const float* __restrict__ rho,
+                                                         const float* __restrict__ prof, Coef c,
+                                                         float dt, float idx, float idy, float idz,
+                                                         int pml, int kc) {
+    const int tile = blockIdx.x;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R, hx = nx + pml + R, hy = ny + pml + R;
+    const int i0 = -pml + 1 + tx * kBX, j0 = -pml + 1 + ty * kTY;
+    const int k0 = -pml + 1 + tzc * kc, k1 = min(k0 + kc - 1, nz + pml);
+    const int lane = threadIdx.x, tid = threadIdx.y * kBX + threadIdx.x;
+    const int i = i0 + lane;
+    // p plane: x = i0-R .. i0+63+R-1 at col = x - (i0-R); y = j0-R .. j0+15+R-1 at row = y - (j0-R)
+    __shared__ float sp[kTY + 2 * R][kBX + 2 * R];
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* ax_h = prof + S.prof + 2 * lx;
+    const float* bx_h = ax_h + lx;
+    const float* ay_h = prof + S.prof + 4 * lx + 2 * ly;
+    const float* by_h = ay_h + ly;
+    const float* az_h = prof + S.prof + 4 * lx + 4 * ly + 2 * lz;
+    const float* bz_h = az_h + lz;
+    bool ok[kRY];
+    float q[kRY][2 * R];
+    float rprev[kRY];
+#pragma unroll
+    for (int r = 0; r < kRY; ++r) {
+        const int j = j0 + threadIdx.y + kBY * r;
+        ok[r] = (i <= nx + pml) && (j <= ny + pml);
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) q[r][m] = ok[r] ? F.p[at3(S, lo, i, j, k0 - R + m)] : 0.0f;
+        rprev[r] = ok[r] ? rho[at3(S, lo, i, j, k0 - 1)] : 0.0f;
+    }
+    const bool xslab = (i <= 1 || i >= nx + 1);
+    const float axv = xslab && i <= nx + pml ? ax_h[i + pml - 1] : 0.0f;
+    const float bxv = xslab && i <= nx + pml ? bx_h[i + pml - 1] : 0.0f;
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int r = 0; r < kRY; ++r) {
+            const int j = j0 + threadIdx.y + kBY * r;
+            q[r][2 * R - 1] = ok[r] ? F.p[at3(S, lo, i, j, k + R - 1)] : 0.0f;
+        }
+        __syncthreads();
+        // rows of the tile with the x halo (R left, R-1 right)
+#pragma unroll
+        for (int r = 0; r < kRY; ++r) {
+            const int row = threadIdx.y + kBY * r, j = j0 + row;
+            sp[row + R][lane + R] = (i <= hx && j <= hy) ? F.p[at3(S, lo, i, j, k)] : 0.0f;
+        }
+        if (tid < (2 * R - 1) * kTY) {
+            const int hc = tid % (2 * R - 1), row = tid / (2 * R - 1);
+            const int col = hc < R ? hc : kBX + hc;  // 0..R-1 and 64+R..64+2R-2
+            const int xg = i0 - R + col, j = j0 + row;
+            sp[row + R][col] = (xg >= lo && xg <= hx && j <= hy) ? F.p[at3(S, lo, xg, j, k)] : 0.0f;
+        }
+        // y halo rows (R above, R-1 below) over the tile's columns
+        for (int e = tid; e < (2 * R - 1) * kBX; e += kBX * kBY) {
+            const int hr = e / kBX, col = e % kBX;
+            const int row = hr < R ? hr : kTY + hr;  // 0..R-1 and 16+R..16+2R-2
+            const int yg = j0 - R + row, xg = i0 + col;
+            sp[row][col + R] = (yg >= lo && yg <= hy && xg <= hx) ? F.p[at3(S, lo, xg, yg, k)] : 0.0f;
+        }
+        __syncthreads();
+        const bool zslab = (k <= 1 || k >= nz + 1);
+        const float azv = zslab ? az_h[k + pml - 1] : 0.0f, bzv = zslab ? bz_h[k + pml - 1] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < kRY; ++r) {
+            if (ok[r]) {
+                const int row = threadIdx.y + kBY * r, j = j0 + row;
+                const int cx = lane + R, cy = row + R;
+                float sx = c.c[0] * (sp[cy][cx] - sp[cy][cx - 1]);
+                float sy = c.c[0] * (sp[cy][cx] - sp[cy - 1][cx]);
+                float sz = c.c[0] * (q[r][R] - q[r][R - 1]);
+#pragma unroll
+                for (int m = 2; m <= R; ++m) {
+                    sx = sx + c.c[m - 1] * (sp[cy][cx + m - 1] - sp[cy][cx - m]);
+                    sy = sy + c.c[m - 1] * (sp[cy + m - 1][cx] - sp[cy - m][cx]);
+                    sz = sz + c.c[m - 1] * (q[r][R - 1 + m] - q[r][R - m]);
+                }
+                float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+                const size_t qq = at3(S, lo, i, j, k);
+                const float r0 = rho[qq];
+                const float rx = 0.5f * (r0 + rho[qq - 1]);
+                const float ry = 0.5f * (r0 + rho[qq - size_t(S.px)]);
+                const float rz = 0.5f * (r0 + rprev[r]);
+                rprev[r] = r0;
+                if (xslab) {
+                    const float mem = axv * dxv + bxv * F.m[3][qq];
+                    F.m[3][qq] = mem;
+                    dxv = dxv + mem;
+                }
+                if (j <= 1 || j >= ny + 1) {
+                    const float mem = ay_h[j + pml - 1] * dyv + by_h[j + pml - 1] * F.m[4][qq];
+                    F.m[4][qq] = mem;
+                    dyv = dyv + mem;
+                }
+                if (zslab) {
+                    const float mem = azv * dzv + bzv * F.m[5][qq];
+                    F.m[5][qq] = mem;
+                    dzv = dzv + mem;
+                }
+                F.vx[qq] = F.vx[qq] - dt / rx * dxv;
+                F.vy[qq] = F.vy[qq] - dt / ry * dyv;
+                F.vz[qq] = F.vz[qq] - dt / rz * dzv;
+            }
+#pragma unroll
+            for (int m = 0; m < 2 * R - 1; ++m) q[r][m] = q[r][m + 1];
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Barrier-free variant: one thread per (x, y) column segment of kc planes,
+// z neighbours from the register queue, x and y neighbours read directly
+// (L1/L2 hits: the wavefront covers 64 consecutive x nodes, the block 4
+// rows).  No LDS, no barriers, so many waves keep loads in flight.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(1024) void k3c_update_p(const Shot3* __restrict__ shots,
+                                                    const int* __restrict__ tile_shot, F3 F,
+                                                    const float* __restrict__ bk,
+                                                    const float* __restrict__ prof, Coef c, float dt,
+                                                    float idx, float idy, float idz, int pml, int kc,
+                                                    int ntiles, int remap) {
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    if (i > nx + pml || j > ny + pml) return;
+    const int k0 = -pml + 1 + tzc * kc, k1 = min(k0 + kc - 1, nz + pml);
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* az_i = prof + S.prof + 4 * lx + 4 * ly;
+    const float* bz_i = az_i + lz;
+    const bool xslab = (i <= 0 || i >= nx + 1), yslab = (j <= 0 || j >= ny + 1);
+    const float axv = xslab ? prof[S.prof + i + pml - 1] : 0.0f;
+    const float bxv = xslab ? prof[S.prof + lx + i + pml - 1] : 0.0f;
+    const float ayv = yslab ? prof[S.prof + 4 * lx + j + pml - 1] : 0.0f;
+    const float byv = yslab ? prof[S.prof + 4 * lx + ly + j + pml - 1] : 0.0f;
+    const size_t px = S.px, pxy = S.pxy;
+    size_t q = at3(S, lo, i, j, k0);
+    float qz[2 * R];
+#pragma unroll
+    for (int m = 0; m < 2 * R - 1; ++m) qz[m] = F.vz[q + (m - R + 1) * pxy];
+    for (int k = k0; k <= k1; ++k, q += pxy) {
+        qz[2 * R - 1] = F.vz[q + R * pxy];
+        float sx = c.c[0] * (F.vx[q + 1] - F.vx[q]);
+        float sy = c.c[0] * (F.vy[q + px] - F.vy[q]);
+        float sz = c.c[0] * (qz[R] - qz[R - 1]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) {
+            sx = sx + c.c[m - 1] * (F.vx[q + m] - F.vx[q - (m - 1)]);
+            sy = sy + c.c[m - 1] * (F.vy[q + m * px] - F.vy[q - (m - 1) * px]);
+            sz = sz + c.c[m - 1] * (qz[R - 1 + m] - qz[R - m]);
+        }
+        float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+        if (xslab) {
+            const float mem = axv * dxv + bxv * F.m[0][q];
+            F.m[0][q] = mem;
+            dxv = dxv + mem;
+        }
+        if (yslab) {
+            const float mem = ayv * dyv + byv * F.m[1][q];
+            F.m[1][q] = mem;
+            dyv = dyv + mem;
+        }
+        if (k <= 0 || k >= nz + 1) {
+            const float mem = az_i[k + pml - 1] * dzv + bz_i[k + pml - 1] * F.m[2][q];
+            F.m[2][q] = mem;
+            dzv = dzv + mem;
+        }
+        F.p[q] = F.p[q] - dt * bk[q] * (dxv + dyv + dzv);
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) qz[m] = qz[m + 1];
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(1024) void k3c_update_v(const Shot3* __restrict__ shots,
+                                                    const int* __restrict__ tile_shot, F3 F,
+                                                    const float* __restrict__ rho,
+                                                    const float* __restrict__ prof, Coef c, float dt,
+                                                    float idx, float idy, float idz, int pml, int kc,
+                                                    int ntiles, int remap) {
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    if (i > nx + pml || j > ny + pml) return;
+    const int k0 = -pml + 1 + tzc * kc, k1 = min(k0 + kc - 1, nz + pml);
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* az_h = prof + S.prof + 4 * lx + 4 * ly + 2 * lz;
+    const float* bz_h = az_h + lz;
+    const bool xslab = (i <= 1 || i >= nx + 1), yslab = (j <= 1 || j >= ny + 1);
+    const float axv = xslab ? prof[S.prof + 2 * lx + i + pml - 1] : 0.0f;
+    const float bxv = xslab ? prof[S.prof + 3 * lx + i + pml - 1] : 0.0f;
+    const float ayv = yslab ? prof[S.prof + 4 * lx + 2 * ly + j + pml - 1] : 0.0f;
+    const float byv = yslab ? prof[S.prof + 4 * lx + 3 * ly + j + pml - 1] : 0.0f;
+    const size_t px = S.px, pxy = S.pxy;
+    size_t q = at3(S, lo, i, j, k0);
+    float qz[2 * R];
+#pragma unroll
+    for (int m = 0; m < 2 * R - 1; ++m) qz[m] = F.p[q + (m - R) * pxy];
+    float rprev = rho[q - pxy];
+    for (int k = k0; k <= k1; ++k, q += pxy) {
+        qz[2 * R - 1] = F.p[q + (R - 1) * pxy];
+        float sx = c.c[0] * (F.p[q] - F.p[q - 1]);
+        float sy = c.c[0] * (F.p[q] - F.p[q - px]);
+        float sz = c.c[0] * (qz[R] - qz[R - 1]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) {
+            sx = sx + c.c[m - 1] * (F.p[q + (m - 1)] - F.p[q - m]);
+            sy = sy + c.c[m - 1] * (F.p[q + (m - 1) * px] - F.p[q - m * px]);
+            sz = sz + c.c[m - 1] * (qz[R - 1 + m] - qz[R - m]);
+        }
+        float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+        const float r0 = rho[q];
+        const float rx = 0.5f * (r0 + rho[q - 1]);
+        const float ry = 0.5f * (r0 + rho[q - px]);
+        const float rz = 0.5f * (r0 + rprev);
+        rprev = r0;
+        if (xslab) {
+            const float mem = axv * dxv + bxv * F.m[3][q];
+            F.m[3][q] = mem;
+            dxv = dxv + mem;
+        }
+        if (yslab) {
+            const float mem = ayv * dyv + byv * F.m[4][q];
+            F.m[4][q] = mem;
+            dyv = dyv + mem;
+        }
+        if (k <= 1 || k >= nz + 1) {
+            const float mem = az_h[k + pml - 1] * dzv + bz_h[k + pml - 1] * F.m[5][q];
+            F.m[5][q] = mem;
+            dzv = dzv + mem;
+        }
+        F.vx[q] = F.vx[q] - dt / rx * dxv;
+        F.vy[q] = F.vy[q] - dt / ry * dyv;
+        F.vz[q] = F.vz[q] - dt / rz * dzv;
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) qz[m] = qz[m + 1];
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Four-wide column variant: each thread updates 4 consecutive x nodes with
+// 16-byte loads and stores (x neighbours from A/4*2+1 aligned float4s, y
+// neighbours one float4 per row, z neighbours from a float4 register
+// queue).  Block = 16 x 16 threads = a 64 x 16 x-y tile.
+// ---------------------------------------------------------------------------
+constexpr int kWX = 16, kWY = 16;
+
+__device__ __forceinline__ float4 ld4(const float* p, size_t q) { return *reinterpret_cast<const float4*>(p + q); }
+__device__ __forceinline__ void st4(float* p, size_t q, float4 v) { *reinterpret_cast<float4*>(p + q) = v; }
+__device__ __forceinline__ float c4(const float4& v, int e) { return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w)); }
+__device__ __forceinline__ void s4(float4& v, int e, float x) {
+    if (e == 0) v.x = x; else if (e == 1) v.y = x; else if (e == 2) v.z = x; else v.w = x;
+}
+
+template <int R>
+__global__ __launch_bounds__(kWX * kWY) void k3w_update_p(const Shot3* __restrict__ shots,
+                                                          const int* __restrict__ tile_shot, F3 F,
+                                                          const float* __restrict__ bk,
+                                                          const float* __restrict__ prof, Coef c, float dt,
+                                                          float idx, float idy, float idz, int pml, int kc,
+                                                          int ntiles, int remap) {
+    constexpr int A = (R + 3) / 4 * 4, NW = 2 * A / 4 + 1;
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int x0 = -pml + 1 + tx * (4 * kWX) + 4 * int(threadIdx.x);
+    const int j = -pml + 1 + ty * kWY + int(threadIdx.y);
+    if (x0 > nx + pml || j > ny + pml) return;
+    const int k0 = -pml + 1 + tzc * kc, k1 = min(k0 + kc - 1, nz + pml);
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* ax_i = prof + S.prof;
+    const float* bx_i = ax_i + lx;
+    const float* az_i = prof + S.prof + 4 * lx + 4 * ly;
+    const float* bz_i = az_i + lz;
+    bool xs[4];
+    float axv[4], bxv[4];
+    bool anyx = false;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x = x0 + e;
+        xs[e] = (x <= 0 || x >= nx + 1) && x <= nx + pml;
+        axv[e] = xs[e] ? ax_i[x + pml - 1] : 0.0f;
+        bxv[e] = xs[e] ? bx_i[x + pml - 1] : 0.0f;
+        anyx = anyx || xs[e];
+    }
+    const bool yslab = (j <= 0 || j >= ny + 1);
+    const float ayv = yslab ? prof[S.prof + 4 * lx + j + pml - 1] : 0.0f;
+    const float byv = yslab ? prof[S.prof + 4 * lx + ly + j + pml - 1] : 0.0f;
+    const size_t px = S.px, pxy = S.pxy;
+    size_t q = at3(S, lo, x0, j, k0);
+    float4 qz[2 * R];
+#pragma unroll
+    for (int m = 0; m < 2 * R - 1; ++m) qz[m] = ld4(F.vz, q + (m - R + 1) * pxy);
+    for (int k = k0; k <= k1; ++k, q += pxy) {
+        qz[2 * R - 1] = ld4(F.vz, q + R * pxy);
+        float w[4 * NW];
+#pragma unroll
+        for (int h = 0; h < NW; ++h) {
+            const float4 v = ld4(F.vx, q - A + 4 * h);
+            w[4 * h] = v.x; w[4 * h + 1] = v.y; w[4 * h + 2] = v.z; w[4 * h + 3] = v.w;
+        }
+        float4 yp[R], ym[R];  // vy(j+m) and vy(j-m+1), m = 1..R
+#pragma unroll
+        for (int m = 1; m <= R; ++m) {
+            yp[m - 1] = ld4(F.vy, q + m * px);
+            ym[m - 1] = ld4(F.vy, q - (m - 1) * px);
+        }
+        float4 pv = ld4(F.p, q);
+        const float4 kv = ld4(bk, q);
+        const bool zslab = (k <= 0 || k >= nz + 1);
+        float4 mx = make_float4(0, 0, 0, 0), my = mx, mz = mx;
+        if (anyx) mx = ld4(F.m[0], q);
+        if (yslab) my = ld4(F.m[1], q);
+        if (zslab) mz = ld4(F.m[2], q);
+        const float azv = zslab ? az_i[k + pml - 1] : 0.0f, bzv = zslab ? bz_i[k + pml - 1] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float sx = c.c[0] * (w[A + e + 1] - w[A + e]);
+            float sy = c.c[0] * (c4(yp[0], e) - c4(ym[0], e));
+            float sz = c.c[0] * (c4(qz[R], e) - c4(qz[R - 1], e));
+#pragma unroll
+            for (int m = 2; m <= R; ++m) {
+                sx = sx + c.c[m - 1] * (w[A + e + m] - w[A + e - m + 1]);
+                sy = sy + c.c[m - 1] * (c4(yp[m - 1], e) - c4(ym[m - 1], e));
+                sz = sz + c.c[m - 1] * (c4(qz[R - 1 + m], e) - c4(qz[R - m], e));
+            }
+            float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+            if (xs[e]) {
+                const float mem = axv[e] * dxv + bxv[e] * c4(mx, e);
+                s4(mx, e, mem);
+                dxv = dxv + mem;
+            }
+            if (yslab) {
+                const float mem = ayv * dyv + byv * c4(my, e);
+                s4(my, e, mem);
+                dyv = dyv + mem;
+            }
+            if (zslab) {
+                const float mem = azv * dzv + bzv * c4(mz, e);
+                s4(mz, e, mem);
+                dzv = dzv + mem;
+            }
+            s4(pv, e, c4(pv, e) - dt * c4(kv, e) * (dxv + dyv + dzv));
+        }
+        // nodes past nx+pml in the last float4 are halo: they hold zeros and
+        // the update of a zero stencil input leaves them zero only if the
+        // stencil is zero there, so mask the write
+        if (x0 + 3 > nx + pml) {
+            const float4 old = ld4(F.p, q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (x0 + e > nx + pml) s4(pv, e, c4(old, e));
+        }
+        st4(F.p, q, pv);
+        if (anyx) st4(F.m[0], q, mx);
+        if (yslab) st4(F.m[1], q, my);
+        if (zslab) st4(F.m[2], q, mz);
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) qz[m] = qz[m + 1];
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kWX * kWY) void k3w_update_v(const Shot3* __restrict__ shots,
+                                                          const int* __restrict__ tile_shot, F3 F,
+                                                          const float* __restrict__ rho,
+                                                          const float* __restrict__ prof, Coef c, float dt,
+                                                          float idx, float idy, float idz, int pml, int kc,
+                                                          int ntiles, int remap) {
+    constexpr int A = (R + 3) / 4 * 4, NW = 2 * A / 4 + 1;
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int x0 = -pml + 1 + tx * (4 * kWX) + 4 * int(threadIdx.x);
+    const int j = -pml + 1 + ty * kWY + int(threadIdx.y);
+    if (x0 > nx + pml || j > ny + pml) return;
+    const int k0 = -pml + 1 + tzc * kc, k1 = min(k0 + kc - 1, nz + pml);
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* ax_h = prof + S.prof + 2 * lx;
+    const float* bx_h = ax_h + lx;
+    const float* az_h = prof + S.prof + 4 * lx + 4 * ly + 2 * lz;
+    const float* bz_h = az_h + lz;
+    bool xs[4];
+    float axv[4], bxv[4];
+    bool anyx = false;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x = x0 + e;
+        xs[e] = (x <= 1 || x >= nx + 1) && x <= nx + pml;
+        axv[e] = xs[e] ? ax_h[x + pml - 1] : 0.0f;
+        bxv[e] = xs[e] ? bx_h[x + pml - 1] : 0.0f;
+        anyx = anyx || xs[e];
+    }
+    const bool yslab = (j <= 1 || j >= ny + 1);
+    const float ayv = yslab ? prof[S.prof + 4 * lx + 2 * ly + j + pml - 1] : 0.0f;
+    const float byv = yslab ? prof[S.prof + 4 * lx + 3 * ly + j + pml - 1] : 0.0f;
+    const size_t px = S.px, pxy = S.pxy;
+    size_t q = at3(S, lo, x0, j, k0);
+    float4 qz[2 * R];
+#pragma unroll
+    for (int m = 0; m < 2 * R - 1; ++m) qz[m] = ld4(F.p, q + (m - R) * pxy);
+    float4 rprev = ld4(rho, q - pxy);
+    for (int k = k0; k <= k1; ++k, q += pxy) {
+        qz[2 * R - 1] = ld4(F.p, q + (R - 1) * pxy);
+        float w[4 * NW];
+#pragma unroll
+        for (int h = 0; h < NW; ++h) {
+            const float4 v = ld4(F.p, q - A + 4 * h);
+            w[4 * h] = v.x; w[4 * h + 1] = v.y; w[4 * h + 2] = v.z; w[4 * h + 3] = v.w;
+        }
+        float4 yp[R], ym[R];  // p(j-1+m) and p(j-m), m = 1..R
+#pragma unroll
+        for (int m = 1; m <= R; ++m) {
+            yp[m - 1] = ld4(F.p, q + (m - 1) * px);
+            ym[m - 1] = ld4(F.p, q - m * px);
+        }
+        const float4 r0 = ld4(rho, q), rl = ld4(rho, q - 4), rym = ld4(rho, q - px);
+        float4 vx = ld4(F.vx, q), vy = ld4(F.vy, q), vz = ld4(F.vz, q);
+        const bool zslab = (k <= 1 || k >= nz + 1);
+        float4 mx = make_float4(0, 0, 0, 0), my = mx, mz = mx;
+        if (anyx) mx = ld4(F.m[3], q);
+        if (yslab) my = ld4(F.m[4], q);
+        if (zslab) mz = ld4(F.m[5], q);
+        const float azv = zslab ? az_h[k + pml - 1] : 0.0f, bzv = zslab ? bz_h[k + pml - 1] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float sx = c.c[0] * (w[A + e] - w[A + e - 1]);
+            float sy = c.c[0] * (c4(yp[0], e) - c4(ym[0], e));
+            float sz = c.c[0] * (c4(qz[R], e) - c4(qz[R - 1], e));
+#pragma unroll
+            for (int m = 2; m <= R; ++m) {
+                sx = sx + c.c[m - 1] * (w[A + e + m - 1] - w[A + e - m]);
+                sy = sy + c.c[m - 1] * (c4(yp[m - 1], e) - c4(ym[m - 1], e));
+                sz = sz + c.c[m - 1] * (c4(qz[R - 1 + m], e) - c4(qz[R - m], e));
+            }
+            float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+            const float rr = c4(r0, e);
+            const float rx = 0.5f * (rr + (e == 0 ? rl.w : c4(r0, e - 1)));
+            const float ry = 0.5f * (rr + c4(rym, e));
+            const float rz = 0.5f * (rr + c4(rprev, e));
+            if (xs[e]) {
+                const float mem = axv[e] * dxv + bxv[e] * c4(mx, e);
+                s4(mx, e, mem);
+                dxv = dxv + mem;
+            }
+            if (yslab) {
+                const float mem = ayv * dyv + byv * c4(my, e);
+                s4(my, e, mem);
+                dyv = dyv + mem;
+            }
+            if (zslab) {
+                const float mem = azv * dzv + bzv * c4(mz, e);
+                s4(mz, e, mem);
+                dzv = dzv + mem;
+            }
+            if (x0 + e <= nx + pml) {
+                s4(vx, e, c4(vx, e) - dt / rx * dxv);
+                s4(vy, e, c4(vy, e) - dt / ry * dyv);
+                s4(vz, e, c4(vz, e) - dt / rz * dzv);
+            }
+        }
+        rprev = r0;
+        st4(F.vx, q, vx);
+        st4(F.vy, q, vy);
+        st4(F.vz, q, vz);
+        if (anyx) st4(F.m[3], q, mx);
+        if (yslab) st4(F.m[4], q, my);
+        if (zslab) st4(F.m[5], q, mz);
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) qz[m] = qz[m + 1];
+    }
+}
+
+// Source injection of step t (add_source, explosion) then receivers of step
+// t+1; one block per shot so the samples see the injected source.
+__global__ __launch_bounds__(1024) void k3_post_step(const Shot3* __restrict__ shots, float* p,
+                                                     const int* __restrict__ inj_beg,
+                                                     const Tap3* __restrict__ inj,
+                                                     const float* __restrict__ amp_tab, int nt, int t,
+                                                     const int* __restrict__ rec_beg,
+                                                     const Tap3* __restrict__ rec,
+                                                     const float* __restrict__ rec_w, float* traces) {
+    const Shot3 S = shots[blockIdx.x];
+    for (int c = threadIdx.x; c < S.ninj; c += blockDim.x) {
+        const int b = inj_beg[S.inj0 + c], e = inj_beg[S.inj0 + c + 1];
+        const long long cell = inj[b].cell;
+        float acc = p[cell];
+        for (int k = b; k < e; ++k) {
+            const Tap3 tp = inj[k];
+            if (t < tp.tbeg || t > tp.tend) continue;
+            acc = acc + amp_tab[size_t(tp.row) * nt + (t - 1)] * tp.w1 * tp.w2 * tp.w3;
+        }
+        p[cell] = acc;
+    }
+    __syncthreads();
+    if (t + 1 > nt || !traces) return;
+    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
+        const int g = S.rec0 + r;
+        const float w = rec_w[g];
+        float acc = 0.0f;
+        for (int k = rec_beg[g]; k < rec_beg[g + 1]; ++k) {
+            const Tap3 tp = rec[k];
+            acc = acc + p[tp.cell] * tp.w1 * tp.w2 * tp.w3 * w;
+        }
+        traces[S.trace0 + size_t(t) * S.nr + r] = acc;
+    }
+}
+
+struct ShotHost3 {
+    int id = 0;
+    int nx = 0, ny = 0, nz = 0, nxbeg = 1, nybeg = 1, nzbeg = 1;
+    float ox = 0, oy = 0, oz = 0;
+    std::vector<owl_source> src;
+    std::vector<owl_receiver> rec;
+};
+
+}  // namespace a3
+}  // namespace owl
+
+using namespace owl;
+using namespace owl::a3;
+
+struct owl_acoustic3d {
+    owl_acoustic_config cfg{};
+    int R = 4, pml = 15, nt = 0, kc = 64;
+    int variant = 1;   // 0: LDS-staged planes with barriers, 1: barrier-free columns, 2: four-wide columns
+    int cy = 4;        // rows per block of the column variant
+    int remap = 0;     // XCD-contiguous tile order
+    float dt = 0, idx = 0, idy = 0, idz = 0;
+    a3::Coef coef{};
+    std::vector<ShotHost3> hs;
+    std::vector<Shot3> sd;
+    std::vector<int> tile_shot;
+    size_t field_elems = 0, trace_elems = 0;
+    int nrec_total = 0;
+    double pts_full = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0, last_updates = 0;
+    Profiler prof;
+    std::vector<float> vp_full, rho_full;
+    DevBuf<Shot3> d_shots;
+    DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg;
+    DevBuf<float> d_bk, d_rho, d_prof, d_amp, d_rec_w, d_traces;
+    DevBuf<float> f[4], m[6];
+    DevBuf<Tap3> d_inj, d_rec;
+    ~owl_acoustic3d() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+inline size_t cell3(const owl_acoustic3d* b, const Shot3& S, int i, int j, int k) {
+    const int lo = 1 - b->pml - b->R;
+    const int h = b->pml + b->R;
+    if (i < lo || i > S.nx + h || j < lo || j > S.ny + h || k < lo || k > S.nz + h)
+        fail(OWL_ERR_ARG, "a source/receiver stencil reaches outside the padded grid; increase pml");
+    return size_t(S.off) + size_t(i - S.xb) + size_t(j - lo) * size_t(S.px) + size_t(k - lo) * size_t(S.pxy);
+}
+
+inline float model_at(const owl_acoustic3d* b, const std::vector<float>& mdl, const ShotHost3& H, int i, int j,
+                      int k) {
+    const int ic = std::min(std::max(i, 1), H.nx), jc = std::min(std::max(j, 1), H.ny),
+              kc = std::min(std::max(k, 1), H.nz);
+    const size_t nz = size_t(b->cfg.nz), ny = size_t(b->cfg.ny);
+    return mdl[size_t(H.nzbeg - 1 + kc - 1) + size_t(H.nybeg - 1 + jc - 1) * nz +
+               size_t(H.nxbeg - 1 + ic - 1) * nz * ny];
+}
+
+void plan3(owl_acoustic3d* b, const owl_acoustic_config* cfg, const float* vp, const float* rho, int nshot,
+           const owl_shot* shots) {
+    if (!cfg || !vp || !rho || nshot < 1 || !shots) fail(OWL_ERR_ARG, "null argument or no shots");
+    if (!fd_coefficients(cfg->fdhalf, b->coef.c)) fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || !(cfg->dx > 0) || !(cfg->dy > 0) || !(cfg->dz > 0) ||
+        !(cfg->dt > 0))
+        fail(OWL_ERR_ARG, "invalid grid or dt");
+    if (cfg->pml < cfg->fdhalf) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
+    if (cfg->free_surface) fail(OWL_ERR_ARG, "the 3D solver here has no free surface");
+    if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+    b->cfg = *cfg;
+    b->R = cfg->fdhalf;
+    b->pml = cfg->pml;
+    b->dt = cfg->dt;
+    b->nt = time_steps(cfg->tmax, cfg->dt);
+    if (b->nt < 2) fail(OWL_ERR_ARG, "tmax/dt gives fewer than 2 steps");
+    b->idx = float(1.0 / double(cfg->dx));
+    b->idy = float(1.0 / double(cfg->dy));
+    b->idz = float(1.0 / double(cfg->dz));
+    const size_t nfull = size_t(cfg->nx) * cfg->ny * cfg->nz;
+    b->vp_full.assign(vp, vp + nfull);
+    b->rho_full.assign(rho, rho + nfull);
+    const float f0factor = cfg->f0factor > 0 ? cfg->f0factor : 1.0f;
+    const int R = b->R, pml = b->pml, nt = b->nt;
+    b->hs.assign(nshot, ShotHost3{});
+    b->sd.assign(nshot, Shot3{});
+    b->tile_shot.clear();
+    b->nrec_total = 0;
+    b->pts_full = 0;
+    long long off = 0, trace = 0;
+    int prof = 0;
+    std::vector<float> profs, amp, rec_w;
+    std::vector<int> inj_beg{0}, rec_beg{0};
+    std::vector<Tap3> inj, rec;
+    for (int s = 0; s < nshot; ++s) {
+        const owl_shot& in = shots[s];
+        ShotHost3& H = b->hs[s];
+        Shot3& S = b->sd[s];
+        H.id = in.id;
+        H.nxbeg = in.nxbeg > 0 ? in.nxbeg : 1;
+        H.nybeg = in.nybeg > 0 ? in.nybeg : 1;
+        H.nzbeg = in.nzbeg > 0 ? in.nzbeg : 1;
+        const int nxend = in.nxend > 0 ? in.nxend : cfg->nx, nyend = in.nyend > 0 ? in.nyend : cfg->ny,
+                  nzend = in.nzend > 0 ? in.nzend : cfg->nz;
+        if (H.nxbeg > nxend || nxend > cfg->nx || H.nybeg > nyend || nyend > cfg->ny || H.nzbeg > nzend ||
+            nzend > cfg->nz)
+            fail(OWL_ERR_ARG, "shot window outside the model");
+        H.nx = nxend - H.nxbeg + 1;
+        H.ny = nyend - H.nybeg + 1;
+        H.nz = nzend - H.nzbeg + 1;
+        H.ox = float(H.nxbeg - 1) * cfg->dx + cfg->ox;
+        H.oy = float(H.nybeg - 1) * cfg->dy + cfg->oy;
+        H.oz = float(H.nzbeg - 1) * cfg->dz + cfg->oz;
+        if (in.ns < 1 || !in.sources || in.nr < 1 || !in.receivers)
+            fail(OWL_ERR_ARG, "shot without sources or receivers");
+        H.src.assign(in.sources, in.sources + in.ns);
+        H.rec.assign(in.receivers, in.receivers + in.nr);
+        float vmax = 0.f, pmlvp = 0.f, f0max = 0.f;
+        for (int i = 1; i <= H.nx; ++i)
+            for (int j = 1; j <= H.ny; ++j)
+                for (int k = 1; k <= H.nz; ++k) {
+                    const float v = model_at(b, b->vp_full, H, i, j, k);
+                    if (!(v > 0.f) || !(model_at(b, b->rho_full, H, i, j, k) > 0.f))
+                        fail(OWL_ERR_ARG, "vp and rho must be positive");
+                    vmax = std::max(vmax, v);
+                    if (i == 1 || i == H.nx || j == 1 || j == H.ny || k == 1 || k == H.nz) pmlvp = std::max(pmlvp, v);
+                }
+        const float dts = dt_stable(R, 3, vmax, cfg->dx, cfg->dy, cfg->dz);
+        if (cfg->dt > dts)
+            fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " + std::to_string(dts));
+        for (auto& sc : H.src) f0max = std::max(f0max, sc.f0);
+        const int NXp = H.nx + 2 * (pml + R), NYp = H.ny + 2 * (pml + R), NZp = H.nz + 2 * (pml + R);
+        S.nx = H.nx;
+        S.ny = H.ny;
+        S.nz = H.nz;
+        // x positions start at xb = -pml+1-A (A = R rounded up to 4): the first
+        // updated node sits 16-byte aligned and float4 reads of the halo stay
+        // in the row; 4 spare floats on the right
+        const int A = (R + 3) / 4 * 4;
+        S.xb = -pml + 1 - A;
+        S.px = (H.nx + 2 * pml + 2 * A + 4 + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
+        (void)NXp;
+        S.nyp = NYp;
+        S.nzp = NZp;
+        S.pxy = (long long)S.px * NYp;
+        S.off = off;
+        off += S.pxy * NZp;
+        off = (off + 63) / 64 * 64;
+        const int ty_rows = b->variant == 0 ? kTY : (b->variant == 1 ? b->cy : kWY);
+        const int tx_cols = b->variant == 2 ? 4 * kWX : kBX;
+        S.nty = (H.ny + 2 * pml + ty_rows - 1) / ty_rows;
+        S.ntx = (H.nx + 2 * pml + tx_cols - 1) / tx_cols;
+        S.nzc = (H.nz + 2 * pml + b->kc - 1) / b->kc;
+        S.tile0 = int(b->tile_shot.size());
+        b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nty * S.nzc, s);
+        b->pts_full += double(H.nx + 2 * pml) * double(H.ny + 2 * pml) * double(H.nz + 2 * pml);
+        S.prof = prof;
+        const PmlAxis PX = cfspml_axis(H.nx, pml, cfg->dx, cfg->dt, pmlvp, f0max);
+        const PmlAxis PY = cfspml_axis(H.ny, pml, cfg->dy, cfg->dt, pmlvp, f0max);
+        const PmlAxis PZ = cfspml_axis(H.nz, pml, cfg->dz, cfg->dt, pmlvp, f0max);
+        for (const PmlAxis* P : {&PX, &PY, &PZ})
+            for (auto* v : {&P->ai, &P->bi, &P->ah, &P->bh}) profs.insert(profs.end(), v->begin(), v->end());
+        prof += 4 * (H.nx + 2 * pml) + 4 * (H.ny + 2 * pml) + 4 * (H.nz + 2 * pml);
+        S.nxbeg = H.nxbeg;
+        S.nybeg = H.nybeg;
+        S.nzbeg = H.nzbeg;
+        S.nr = int(H.rec.size());
+        S.trace0 = trace;
+        trace += (long long)nt * S.nr;
+        // sources: explosions (add_source, acoustic_iso_3d/module_wavefield.f90:416-436)
+        const float xmax = float(H.nx - 1) * cfg->dx + H.ox, ymax = float(H.ny - 1) * cfg->dy + H.oy,
+                    zmax = float(H.nz - 1) * cfg->dz + H.oz;
+        std::vector<Tap3> st;
+        for (auto& sc : H.src) {
+            if (sc.mechanism != OWL_EXPLOSION) fail(OWL_ERR_ARG, "the 3D solver here takes explosion sources");
+            if (!(sc.x >= H.ox && sc.x <= xmax && sc.y >= H.oy && sc.y <= ymax && sc.z >= H.oz && sc.z <= zmax))
+                fail(OWL_ERR_ARG, "source outside its shot window");
+            const Located L = locate(3, sc.x, sc.y, sc.z, H.ox, H.oy, H.oz, cfg->dx, cfg->dy, cfg->dz);
+            int hnt = 0;
+            const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
+            const int row = int(amp.size() / nt);
+            std::vector<float> a(nt, 0.f);
+            for (int t = std::max(nbeg, 1); t <= std::min(nend, nt); ++t) a[t - 1] = stf[t - nbeg] * sc.amp * cfg->dt;
+            amp.insert(amp.end(), a.begin(), a.end());
+            for (int irz = -kNkw; irz <= kNkw; ++irz)
+                for (int iry = -kNkw; iry <= kNkw; ++iry)
+                    for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                        const float w1 = L.ix[irx + kNkw], w2 = L.iy[iry + kNkw], w3 = L.iz[irz + kNkw];
+                        if (w1 == 0.f || w2 == 0.f || w3 == 0.f) continue;
+                        st.push_back(Tap3{(long long)cell3(b, S, L.gx + irx, L.gy + iry, L.gz + irz), w1, w2, w3, row,
+                                          nbeg, nend});
+                    }
+        }
+        std::stable_sort(st.begin(), st.end(), [](const Tap3& x, const Tap3& y) { return x.cell < y.cell; });
+        S.inj0 = int(inj_beg.size()) - 1;
+        S.ninj = 0;
+        for (size_t k = 0; k < st.size(); ++k) {
+            if (k == 0 || st[k].cell != st[k - 1].cell) {
+                if (k) inj_beg.push_back(int(inj.size()));
+                ++S.ninj;
+            }
+            inj.push_back(st[k]);
+        }
+        if (!st.empty()) inj_beg.push_back(int(inj.size()));
+        // receivers (outside the window: weight 0)
+        S.rec0 = b->nrec_total;
+        for (int r = 0; r < S.nr; ++r) {
+            const owl_receiver& rc = H.rec[r];
+            const bool valid = rc.x >= H.ox && rc.x <= xmax && rc.y >= H.oy && rc.y <= ymax && rc.z >= H.oz &&
+                               rc.z <= zmax;
+            const float w = (valid && rc.weight != 0.f) ? rc.weight : 0.f;
+            rec_w.push_back(w);
+            if (w != 0.f) {
+                const Located L = locate(3, rc.x, rc.y, rc.z, H.ox, H.oy, H.oz, cfg->dx, cfg->dy, cfg->dz);
+                for (int irz = -kNkw; irz <= kNkw; ++irz)
+                    for (int iry = -kNkw; iry <= kNkw; ++iry)
+                        for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                            const float w1 = L.ix[irx + kNkw], w2 = L.iy[iry + kNkw], w3 = L.iz[irz + kNkw];
+                            if (w1 == 0.f || w2 == 0.f || w3 == 0.f) continue;
+                            rec.push_back(Tap3{(long long)cell3(b, S, L.gx + irx, L.gy + iry, L.gz + irz), w1, w2,
+                                               w3, r, 0, 0});
+                        }
+            }
+            rec_beg.push_back(int(rec.size()));
+        }
+        b->nrec_total += S.nr;
+    }
+    b->field_elems = size_t(off);
+    b->trace_elems = size_t(trace);
+    for (auto& S : b->sd) {
+        const bool ok = size_t(S.rec0) + size_t(S.nr) <= rec_w.size() && size_t(S.rec0) + S.nr + 1 <= rec_beg.size() &&
+                        size_t(S.inj0) + size_t(S.ninj) + (S.ninj ? 1 : 0) <= inj_beg.size() &&
+                        size_t(S.trace0) + size_t(nt) * S.nr <= b->trace_elems &&
+                        size_t(S.off) + size_t(S.pxy) * S.nzp <= b->field_elems;
+        if (!ok) fail(OWL_ERR_ARG, "internal layout check failed");
+    }
+    if (!b->stream) {
+        OWL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        OWL_HIP(hipEventCreate(&b->ev0));
+        OWL_HIP(hipEventCreate(&b->ev1));
+    }
+    hipStream_t stm = b->stream;
+    b->d_shots.upload(b->sd, stm);
+    b->d_tile_shot.upload(b->tile_shot, stm);
+    b->d_prof.upload(profs, stm);
+    if (amp.empty()) amp.push_back(0.f);
+    b->d_amp.upload(amp, stm);
+    b->d_inj_beg.upload(inj_beg, stm);
+    if (inj.empty()) inj.push_back(Tap3{});
+    b->d_inj.upload(inj, stm);
+    b->d_rec_beg.upload(rec_beg, stm);
+    if (rec.empty()) rec.push_back(Tap3{});
+    b->d_rec.upload(rec, stm);
+    b->d_rec_w.upload(rec_w, stm);
+    // padded models (edge replication) -- permute(...,321): x fastest
+    std::vector<float> bk(b->field_elems, 0.f), rh(b->field_elems, 0.f);
+    const int lo = 1 - pml - R;
+    for (size_t s = 0; s < b->hs.size(); ++s) {
+        const ShotHost3& H = b->hs[s];
+        const Shot3& S = b->sd[s];
+        for (int k = lo; k < lo + S.nzp; ++k)
+            for (int j = lo; j < lo + S.nyp; ++j)
+                for (int i = S.xb; i < S.xb + S.px; ++i) {
+                    const float v = model_at(b, b->vp_full, H, i, j, k), r = model_at(b, b->rho_full, H, i, j, k);
+                    const size_t q = size_t(S.off) + size_t(i - S.xb) + size_t(j - lo) * size_t(S.px) +
+                                     size_t(k - lo) * size_t(S.pxy);
+                    rh[q] = r;
+                    bk[q] = r * (v * v);
+                }
+    }
+    b->d_bk.upload(bk, stm);
+    b->d_rho.upload(rh, stm);
+    for (auto& x : b->f) x.alloc(b->field_elems);
+    for (auto& x : b->m) x.alloc(b->field_elems);
+    b->d_traces.alloc(b->trace_elems);
+    OWL_HIP(hipStreamSynchronize(stm));
+}
+
+template <int R>
+void run_forward3(owl_acoustic3d* b) {
+    hipStream_t st = b->stream;
+    for (auto& x : b->f) x.zero(st);
+    for (auto& x : b->m) x.zero(st);
+    b->d_traces.zero(st);
+    F3 F{b->f[0].get(), b->f[1].get(), b->f[2].get(), b->f[3].get(), {}};
+    for (int k = 0; k < 6; ++k) F.m[k] = b->m[k].get();
+    const dim3 blk(kBX, kBY), grd(unsigned(b->tile_shot.size()));
+    const int ntiles = int(b->tile_shot.size());
+    const dim3 cblk(64, unsigned(b->cy)), cgrd(unsigned(b->remap ? (ntiles + kXcd - 1) / kXcd * kXcd : ntiles));
+    // algorithmic bytes: p-sweep reads vx, vy, vz, p, bk and writes p (24 B);
+    // v-sweep reads p, vx, vy, vz, rho and writes vx, vy, vz (32 B); each
+    // damped slab direction adds a memory variable read + write (8 B)
+    double bp = 0, bv = 0;
+    for (auto& S : b->sd) {
+        const double lx = S.nx + 2.0 * b->pml, ly = S.ny + 2.0 * b->pml, lz = S.nz + 2.0 * b->pml;
+        const double pts = lx * ly * lz;
+        const double sp = 2.0 * b->pml * (ly * lz + lx * lz + lx * ly);
+        const double sv = (2.0 * b->pml + 1) * (ly * lz + lx * lz + lx * ly);
+        bp += 24.0 * pts + 8.0 * sp;
+        bv += 32.0 * pts + 8.0 * sv;
+    }
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    for (int t = 1; t <= b->nt - 1; ++t) {
+        int ev = b->prof.begin(st, 0, bp);
+        if (b->variant == 2)
+            hipLaunchKernelGGL(k3w_update_p<R>, cgrd, dim3(kWX, kWY), 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                               b->d_bk.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc,
+                               ntiles, b->remap);
+        else if (b->variant == 0)
+            hipLaunchKernelGGL(k3_update_p<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                               b->d_bk.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc);
+        else
+            hipLaunchKernelGGL(k3c_update_p<R>, cgrd, cblk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                               b->d_bk.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc,
+                               ntiles, b->remap);
+        b->prof.end(st, ev);
+        ev = b->prof.begin(st, 1, bv);
+        if (b->variant == 2)
+            hipLaunchKernelGGL(k3w_update_v<R>, cgrd, dim3(kWX, kWY), 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                               b->d_rho.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc,
+                               ntiles, b->remap);
+        else if (b->variant == 0)
+            hipLaunchKernelGGL(k3_update_v<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                               b->d_rho.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc);
+        else
+            hipLaunchKernelGGL(k3c_update_v<R>, cgrd, cblk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                               b->d_rho.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc,
+                               ntiles, b->remap);
+        b->prof.end(st, ev);
+        ev = b->prof.begin(st, 2, 0);
+        hipLaunchKernelGGL(k3_post_step, dim3(unsigned(b->sd.size())), dim3(1024), 0, st, b->d_shots.get(), F.p,
+                           b->d_inj_beg.get(), b->d_inj.get(), b->d_amp.get(), b->nt, t, b->d_rec_beg.get(),
+                           b->d_rec.get(), b->d_rec_w.get(), b->d_traces.get());
+        b->prof.end(st, ev);
+        if (b->prof.pending() > 20000) {
+            OWL_HIP(hipStreamSynchronize(st));
+            b->prof.collect();
+        }
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    b->prof.collect();
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = double(b->nt - 1) * b->pts_full;
+}
+
+template <class F>
+void dispatch_R3(int R, F&& f) {
+    switch (R) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 6: f(std::integral_constant<int, 6>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+owl_status owl_acoustic3d_create(const owl_acoustic_config* cfg, const float* vp, const float* rho, int nshot,
+                                 const owl_shot* shots, owl_acoustic3d** out) {
+    return guarded([&] {
+        if (!out) fail(OWL_ERR_ARG, "out is null");
+        *out = nullptr;
+        auto* b = new owl_acoustic3d();
+        try {
+            plan3(b, cfg, vp, rho, nshot, shots);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+void owl_acoustic3d_destroy(owl_acoustic3d* b) { delete b; }
+
+int owl_acoustic3d_nt(const owl_acoustic3d* b) { return b ? b->nt : 0; }
+
+owl_status owl_acoustic3d_set_launch(owl_acoustic3d* b, int planes, int variant) {
+    return guarded([&] {
+        // variant: 0 LDS planes; 1, 2, 3 columns with 4, 8, 16 rows per block;
+        // 8 four-wide columns; +4 (or +16 for 8): XCD-contiguous tile order
+        if (!b || planes < 1 || variant < 0) fail(OWL_ERR_ARG, "bad argument");
+        int kind, cy = b->cy, rm;
+        if (variant == 8 || variant == 24) {
+            kind = 2;
+            rm = variant == 24;
+        } else if (variant <= 7) {
+            const int v = variant & 3;
+            kind = v == 0 ? 0 : 1;
+            cy = v == 1 ? 4 : (v == 2 ? 8 : 16);
+            rm = variant >> 2;
+        } else {
+            fail(OWL_ERR_ARG, "bad launch variant");
+        }
+        if (planes == b->kc && kind == b->variant && cy == b->cy && rm == b->remap) return;
+        b->variant = kind;
+        b->cy = cy;
+        b->remap = rm;
+        std::vector<owl_shot> shots(b->hs.size());
+        std::vector<ShotHost3> keep = b->hs;
+        for (size_t s = 0; s < keep.size(); ++s) {
+            const ShotHost3& H = keep[s];
+            shots[s] = owl_shot{H.id, int(H.src.size()), H.src.data(), int(H.rec.size()), H.rec.data(),
+                                H.nxbeg, H.nxbeg + H.nx - 1, H.nybeg, H.nybeg + H.ny - 1, H.nzbeg, H.nzbeg + H.nz - 1};
+        }
+        b->kc = planes;
+        const owl_acoustic_config cfg = b->cfg;
+        std::vector<float> vp = b->vp_full, rho = b->rho_full;
+        plan3(b, &cfg, vp.data(), rho.data(), int(shots.size()), shots.data());
+    });
+}
+
+owl_status owl_acoustic3d_forward(owl_acoustic3d* b, float* traces) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        dispatch_R3(b->R, [&](auto r) { run_forward3<decltype(r)::value>(b); });
+        if (traces) {
+            std::vector<float> h(b->trace_elems);
+            OWL_HIP(hipMemcpy(h.data(), b->d_traces.get(), h.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (auto& S : b->sd)
+                for (int r = 0; r < S.nr; ++r)
+                    for (int t = 0; t < b->nt; ++t)
+                        traces[size_t(S.trace0) + size_t(r) * b->nt + t] = h[size_t(S.trace0) + size_t(t) * S.nr + r];
+        }
+    });
+}
+
+owl_status owl_acoustic3d_set_profiling(owl_acoustic3d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->prof.enable(on != 0);
+    });
+}
+
+owl_status owl_acoustic3d_kernel_stats(const owl_acoustic3d* b, int kind, double* total_ms, long long* launches,
+                                       double* bytes_total) {
+    return guarded([&] {
+        if (!b || kind < 0 || kind >= Profiler::kKinds) fail(OWL_ERR_ARG, "bad kernel kind");
+        if (total_ms) *total_ms = b->prof.ms(kind);
+        if (launches) *launches = b->prof.launches(kind);
+        if (bytes_total) *bytes_total = b->prof.bytes(kind);
+    });
+}
+
+owl_status owl_acoustic3d_last_stats(const owl_acoustic3d* b, double* ms, double* updates) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (ms) *ms = b->last_ms;
+        if (updates) *updates = b->last_updates;
+    });
+}
+
+}  // extern "C"

@@ -72,3 +72,56 @@ def acoustic2d_shots(nshot=32, x0=140.0, dxs=120.0, zs=20.0, f0=10.0, rx0=50.0, 
                      recr=surface_line(rx0, drx, nrec, zr))
         shots.append(g)
     return shots
+
+
+# ---- 3D acoustic batched forward workload ---------------------------------
+LAYER_VP = (1500.0, 2200.0, 2900.0, 3700.0, 4500.0)
+LAYER_DEPTHS = (400.0, 900.0, 1400.0, 1900.0)
+
+
+def acoustic3d_layered_model(n=256, d=10.0, seed=0, undulation=50.0):
+    """Five layers (vp 1500..4500 m/s) whose four interfaces undulate by
+    +-`undulation` m (smoothed seeded random surfaces); Gardner rho.
+    Returns (vp, rho) shaped (nz, ny, nx)."""
+    z = np.arange(n, dtype=np.float64) * d
+    vp = np.full((n, n, n), LAYER_VP[0], np.float64)
+    for k, z0 in enumerate(LAYER_DEPTHS):
+        surf = z0 + undulation * random_field((n, n), 12.0, seed + 10 * (k + 1))  # (ny, nx)
+        below = z[:, None, None] >= surf[None, :, :]
+        vp[below] = LAYER_VP[k + 1]
+    vp = vp.astype(np.float32)
+    return vp, gardner(vp)
+
+
+def acoustic3d_shots(n=256, d=10.0, seed=0, f0=8.0, zs=20.0, zr=20.0, nrec_side=100,
+                     wmin=128, wmax=256):
+    """16 shots on a 4x4 surface grid, each with its own window of wmin..wmax
+    cells per horizontal side (seeded, centred on the source, clipped to the
+    model) and a 100 x 100 receiver patch at zr spanning the window on grid
+    nodes.  Returns a list of (Geometry, window dict)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    sid = 1
+    for jy in range(4):
+        for jx in range(4):
+            sx, sy = 320.0 + 640.0 * jx, 320.0 + 640.0 * jy
+            wx, wy = (int(v) for v in rng.integers(wmin, wmax + 1, size=2))
+            win = {}
+            for ax, s, w in (("x", sx, wx), ("y", sy, wy)):
+                c = int(round(s / d)) + 1
+                beg = max(1, min(c - w // 2, n - w + 1))
+                win[f"n{ax}beg"], win[f"n{ax}end"] = beg, beg + w - 1
+            win["nzbeg"], win["nzend"] = 1, n
+            recs = []
+            stepx = max(1, (wx - 1) // (nrec_side - 1))
+            stepy = max(1, (wy - 1) // (nrec_side - 1))
+            x0 = (win["nxbeg"] - 1 + (wx - 1 - stepx * (nrec_side - 1)) // 2) * d
+            y0 = (win["nybeg"] - 1 + (wy - 1 - stepy * (nrec_side - 1)) // 2) * d
+            for ry in range(nrec_side):
+                for rx in range(nrec_side):
+                    recs.append(Receiver(x=float(x0 + rx * stepx * d), y=float(y0 + ry * stepy * d),
+                                         z=float(zr)))
+            g = Geometry(id=sid, srcr=[Source(x=sx, y=sy, z=zs, f0=f0)], recr=recs)
+            out.append((g, win))
+            sid += 1
+    return out
