@@ -52,8 +52,9 @@ def test_forward3d_matches_oracle(oracle32, gpu, R):
     print(f"case: 3D forward R={R}", flush=True)
     b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=R)
     outs = [b.forward()]
-    b.set_launch(16, 0)
-    outs.append(b.forward())
+    for kc, var in ((16, 0), (64, 8), (5, 24)):
+        b.set_launch(kc, var)
+        outs.append(b.forward())
     for o in outs[1:]:
         for a, c in zip(o, outs[0]):
             assert np.array_equal(a, c)
@@ -74,7 +75,7 @@ def test_forward3d_zchunk_invariance(gpu):
     print("case: 3D z-chunk invariance", flush=True)
     b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, [(geom, None)], 0.001, 0.1, pml=8)
     ref = b.forward()[0]
-    for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (64, 1)):
+    for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (1, 8), (13, 24), (64, 8), (64, 1)):
         b.set_launch(kc, var)
         assert np.array_equal(b.forward()[0], ref), (kc, var)
     b.close()
