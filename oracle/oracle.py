@@ -38,12 +38,13 @@ class Grid3D(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
                 ("dy", C.c_float), ("dz", C.c_float), ("ox", C.c_float), ("oy", C.c_float),
                 ("oz", C.c_float), ("dt", C.c_float), ("nt", C.c_int), ("pml", C.c_int),
-                ("R", C.c_int)]
+                ("R", C.c_int), ("free_surface", C.c_int), ("cc_step_interval", C.c_int)]
 
 
 class Source3D(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("amp", C.c_float),
-                ("t0", C.c_float), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
+                ("t0", C.c_float), ("polar", C.c_float), ("azimuth", C.c_float),
+                ("mechanism", C.c_int), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
 
 
 class Receiver3D(C.Structure):
@@ -101,6 +102,7 @@ class Oracle:
         f("acoustic2d_forward").restype = C.c_int
         f("acoustic2d_gradient_owl").restype = C.c_int
         f("acoustic3d_forward").restype = C.c_int
+        f("acoustic3d_gradient_owl").restype = C.c_int
         f("adjoint_source_l2").restype = C.c_double
         f("adjoint_source_l2").argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_float]
@@ -256,25 +258,43 @@ class Oracle:
         return mis, adj
 
     # -- 3D acoustic -------------------------------------------------------
-    def acoustic3d_forward(self, g, vp, rho, sources, receivers):
-        G = Grid3D(int(g["nx"]), int(g["ny"]), int(g["nz"]), g["dx"], g["dy"], g["dz"],
-                   g.get("ox", 0.0), g.get("oy", 0.0), g.get("oz", 0.0), g["dt"], int(g["nt"]),
-                   int(g["pml"]), int(g["R"]))
-        vp = self._model(vp, (G.nz, G.ny, G.nx))
-        rho = self._model(rho, (G.nz, G.ny, G.nx))
+    def _grid3(self, g):
+        return Grid3D(int(g["nx"]), int(g["ny"]), int(g["nz"]), g["dx"], g["dy"], g["dz"],
+                      g.get("ox", 0.0), g.get("oy", 0.0), g.get("oz", 0.0), g["dt"], int(g["nt"]),
+                      int(g["pml"]), int(g["R"]), int(bool(g.get("free_surface", False))),
+                      int(g.get("cc_step_interval", 1)))
+
+    def _sources3(self, g, sources):
         keep = []
         arr = (Source3D * max(1, len(sources)))()
         f0max = 0.0
+        hnt = 0
         for k, s in enumerate(sources):
-            stf, _ = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
             stf = np.ascontiguousarray(stf, dtype=self.real)
             keep.append(stf)
+            mech = s.get("mechanism", "explosion")
             arr[k] = Source3D(s["x"], s["y"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0),
-                              len(stf), stf.ctypes.data)
+                              s.get("polar", 0.0), s.get("azimuth", 0.0),
+                              MECH[mech] if isinstance(mech, str) else int(mech), len(stf),
+                              stf.ctypes.data)
             f0max = max(f0max, s["f0"])
+            if k == 0:
+                hnt = h
+        return arr, keep, f0max, hnt
+
+    def _receivers3(self, receivers):
         rec = (Receiver3D * max(1, len(receivers)))()
         for k, r in enumerate(receivers):
             rec[k] = Receiver3D(r["x"], r["y"], r["z"], r.get("weight", 1.0))
+        return rec
+
+    def acoustic3d_forward(self, g, vp, rho, sources, receivers):
+        G = self._grid3(g)
+        vp = self._model(vp, (G.nz, G.ny, G.nx))
+        rho = self._model(rho, (G.nz, G.ny, G.nx))
+        arr, keep, f0max, _ = self._sources3(g, sources)
+        rec = self._receivers3(receivers)
         tr = np.zeros((len(receivers), G.nt), self.real)
         self._fn("acoustic3d_forward").argtypes = [C.POINTER(Grid3D), C.c_void_p, C.c_void_p,
                                                    C.c_int, C.c_void_p, C.c_float, C.c_int,
@@ -286,3 +306,29 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle acoustic3d_forward failed ({rc})")
         return tr
+
+    def acoustic3d_gradient_owl(self, g, vp, rho, sources, receivers, adj):
+        """OWL's 3D gradient algorithm.  Returns (grad_vp, grad_rho) shaped
+        (nz, ny, nx) and the forward traces (nr, nt)."""
+        G = self._grid3(g)
+        vp = self._model(vp, (G.nz, G.ny, G.nx))
+        rho = self._model(rho, (G.nz, G.ny, G.nx))
+        arr, keep, f0max, hnt = self._sources3(g, sources)
+        rec = self._receivers3(receivers)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (len(receivers), G.nt)
+        gv = np.zeros(G.nx * G.ny * G.nz, self.real)
+        gr = np.zeros(G.nx * G.ny * G.nz, self.real)
+        tr = np.zeros((len(receivers), G.nt), self.real)
+        self._fn("acoustic3d_gradient_owl").argtypes = [
+            C.POINTER(Grid3D), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_int,
+            C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        rc = self._fn("acoustic3d_gradient_owl")(
+            C.byref(G), vp.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p), len(sources),
+            C.cast(arr, C.c_void_p), f0max, int(g.get("hnt", hnt)), len(receivers),
+            C.cast(rec, C.c_void_p), adj.ctypes.data_as(C.c_void_p), gv.ctypes.data_as(C.c_void_p),
+            gr.ctypes.data_as(C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle acoustic3d_gradient_owl failed ({rc})")
+        shp = (G.nx, G.ny, G.nz)
+        return (gv.reshape(shp).transpose(2, 1, 0).copy(), gr.reshape(shp).transpose(2, 1, 0).copy(), tr)

@@ -764,20 +764,23 @@ double ORA_SYM(adjoint_source_l2)(int nr, int nt, const float *weight, const rea
 }
 
 /* ======================================================================== */
-/* 3D acoustic forward: acoustic_iso_3d/module_wavefield.f90:40-172 with no */
-/* domain decomposition (nx1 = -pml+1 .. nx2 = nx+pml, etc.), separate     */
-/* min/max memory arrays folded into one array per derivative (they never */
-/* overlap).  Explosion and force sources: add_source :322-444.            */
+/* 3D acoustic solver: acoustic_iso_3d/module_wavefield.f90:40-444 with no  */
+/* domain decomposition (nx1 = -pml+1 .. nx2 = nx+pml, etc.).  The separate */
+/* min/max memory arrays are folded into one array per derivative (their   */
+/* index ranges never overlap).                                             */
 /* ======================================================================== */
 typedef struct {
     int nx, ny, nz;
     float dx, dy, dz, ox, oy, oz;
     float dt;
     int nt, pml, R;
+    int free_surface;
+    int cc_step_interval;
 } ora_grid3d;
 
 typedef struct {
-    float x, y, z, amp, t0;
+    float x, y, z, amp, t0, polar, azimuth; /* angles in radians */
+    int mechanism;                          /* 0 explosion, 1 force */
     int nt_stf;
     const real *stf;
 } ora_source3d;
@@ -786,29 +789,70 @@ typedef struct {
     float x, y, z, weight;
 } ora_receiver3d;
 
-#define I3(i, j, k) \
-    ((size_t)((i) - lo) + (size_t)((j) - lo) * (size_t)NX + (size_t)((k) - lo) * (size_t)NX * (size_t)NY)
+typedef struct {
+    int lo, NX, NY, NZ;
+    int nx, ny, nz, pml, R;
+} lay3;
 
-int ORA_SYM(acoustic3d_forward)(const ora_grid3d *g, const float *vp, const float *rho, int ns,
-                                const ora_source3d *src, float f0max, int nr,
-                                const ora_receiver3d *rec, real *traces) {
-    const int R = g->R, pml = g->pml, nx = g->nx, ny = g->ny, nz = g->nz;
-    if (R < 2 || R > 8) return -1;
-    const int lo = 1 - pml - R;
-    const int NX = nx + 2 * (pml + R), NY = ny + 2 * (pml + R), NZ = nz + 2 * (pml + R);
-    const size_t n = (size_t)NX * NY * NZ;
-    real *bk = zalloc(n), *rh = zalloc(n);
-    /* input models are (nz, ny, nx) z-fastest (Fortran order); permute(...,321) */
-    for (int k = lo; k < lo + NZ; ++k)
-        for (int j = lo; j < lo + NY; ++j)
-            for (int i = lo; i < lo + NX; ++i) {
+#define I3L(L, i, j, k)                                                                \
+    ((size_t)((i) - (L).lo) + (size_t)((j) - (L).lo) * (size_t)(L).NX +                \
+     (size_t)((k) - (L).lo) * (size_t)(L).NX * (size_t)(L).NY)
+
+typedef struct {
+    lay3 L;
+    real *bk, *rho, *vp;
+    float *prof;                                   /* owns the profile arrays */
+    float *axi, *bxi, *kxi, *axh, *bxh, *kxh;
+    float *ayi, *byi, *kyi, *ayh, *byh, *kyh;
+    float *azi, *bzi, *kzi, *azh, *bzh, *kzh;
+    real idx, idy, idz;
+    float cf[8];
+} med3;
+
+typedef struct {
+    real *p, *vx, *vy, *vz;
+    real *mvx, *mvy, *mvz, *mpx, *mpy, *mpz;
+} fld3;
+
+static size_t lay3_n(const lay3 *L) { return (size_t)L->NX * L->NY * L->NZ; }
+
+static void fld3_alloc(fld3 *f, const lay3 *L) {
+    size_t n = lay3_n(L);
+    f->p = zalloc(n); f->vx = zalloc(n); f->vy = zalloc(n); f->vz = zalloc(n);
+    f->mvx = zalloc(n); f->mvy = zalloc(n); f->mvz = zalloc(n);
+    f->mpx = zalloc(n); f->mpy = zalloc(n); f->mpz = zalloc(n);
+}
+static void fld3_free(fld3 *f) {
+    free(f->p); free(f->vx); free(f->vy); free(f->vz);
+    free(f->mvx); free(f->mvy); free(f->mvz); free(f->mpx); free(f->mpy); free(f->mpz);
+}
+
+/* prepare_modeling, acoustic_iso_3d/module_vars.f90:108-168: the (nz,ny,nx)
+ * z-fast model permuted to x-fast (permute(...,321)), bk = rho*vp**2, padded
+ * by edge replication; PML profiles from the max vp on the six faces. */
+static void med3_init(med3 *M, const ora_grid3d *g, const float *vp, const float *rho, float f0max) {
+    lay3 L;
+    L.lo = 1 - g->pml - g->R;
+    L.NX = g->nx + 2 * (g->pml + g->R);
+    L.NY = g->ny + 2 * (g->pml + g->R);
+    L.NZ = g->nz + 2 * (g->pml + g->R);
+    L.nx = g->nx; L.ny = g->ny; L.nz = g->nz; L.pml = g->pml; L.R = g->R;
+    M->L = L;
+    const int nx = g->nx, ny = g->ny, nz = g->nz, pml = g->pml;
+    size_t n = lay3_n(&L);
+    M->bk = zalloc(n); M->rho = zalloc(n); M->vp = zalloc(n);
+    for (int k = L.lo; k < L.lo + L.NZ; ++k)
+        for (int j = L.lo; j < L.lo + L.NY; ++j)
+            for (int i = L.lo; i < L.lo + L.NX; ++i) {
                 int ic = i < 1 ? 1 : (i > nx ? nx : i);
                 int jc = j < 1 ? 1 : (j > ny ? ny : j);
                 int kc = k < 1 ? 1 : (k > nz ? nz : k);
                 size_t s = (size_t)(kc - 1) + (size_t)(jc - 1) * nz + (size_t)(ic - 1) * nz * ny;
                 real v = (real)vp[s], r = (real)rho[s];
-                rh[I3(i, j, k)] = r;
-                bk[I3(i, j, k)] = r * (v * v);
+                size_t q = I3L(L, i, j, k);
+                M->vp[q] = v;
+                M->rho[q] = r;
+                M->bk[q] = r * (v * v);
             }
     float pmlvp = 0.0f;
     for (int i = 1; i <= nx; ++i)
@@ -820,147 +864,392 @@ int ORA_SYM(acoustic3d_forward)(const ora_grid3d *g, const float *vp, const floa
                 }
     int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
     float *ax = (float *)calloc(6 * (size_t)(lx + ly + lz), sizeof(float));
-    float *axi = ax, *bxi = ax + lx, *kxi = ax + 2 * lx, *axh = ax + 3 * lx, *bxh = ax + 4 * lx, *kxh = ax + 5 * lx;
+    M->prof = ax;
+    M->axi = ax; M->bxi = ax + lx; M->kxi = ax + 2 * lx; M->axh = ax + 3 * lx; M->bxh = ax + 4 * lx; M->kxh = ax + 5 * lx;
     float *ay = ax + 6 * lx;
-    float *ayi = ay, *byi = ay + ly, *kyi = ay + 2 * ly, *ayh = ay + 3 * ly, *byh = ay + 4 * ly, *kyh = ay + 5 * ly;
+    M->ayi = ay; M->byi = ay + ly; M->kyi = ay + 2 * ly; M->ayh = ay + 3 * ly; M->byh = ay + 4 * ly; M->kyh = ay + 5 * ly;
     float *az = ay + 6 * ly;
-    float *azi = az, *bzi = az + lz, *kzi = az + 2 * lz, *azh = az + 3 * lz, *bzh = az + 4 * lz, *kzh = az + 5 * lz;
-    ORA_SYM(cfspml_axis)(nx, pml, g->dx, g->dt, pmlvp, f0max, axi, bxi, kxi, axh, bxh, kxh);
-    ORA_SYM(cfspml_axis)(ny, pml, g->dy, g->dt, pmlvp, f0max, ayi, byi, kyi, ayh, byh, kyh);
-    ORA_SYM(cfspml_axis)(nz, pml, g->dz, g->dt, pmlvp, f0max, azi, bzi, kzi, azh, bzh, kzh);
-    real *p = zalloc(n), *vx = zalloc(n), *vy = zalloc(n), *vz = zalloc(n);
-    real *mvx = zalloc(n), *mvy = zalloc(n), *mvz = zalloc(n);
-    real *mpx = zalloc(n), *mpy = zalloc(n), *mpz = zalloc(n);
+    M->azi = az; M->bzi = az + lz; M->kzi = az + 2 * lz; M->azh = az + 3 * lz; M->bzh = az + 4 * lz; M->kzh = az + 5 * lz;
+    ORA_SYM(cfspml_axis)(nx, pml, g->dx, g->dt, pmlvp, f0max, M->axi, M->bxi, M->kxi, M->axh, M->bxh, M->kxh);
+    ORA_SYM(cfspml_axis)(ny, pml, g->dy, g->dt, pmlvp, f0max, M->ayi, M->byi, M->kyi, M->ayh, M->byh, M->kyh);
+    ORA_SYM(cfspml_axis)(nz, pml, g->dz, g->dt, pmlvp, f0max, M->azi, M->bzi, M->kzi, M->azh, M->bzh, M->kzh);
 #if defined(ORA_F64)
-    const real idx = 1.0 / (double)g->dx, idy = 1.0 / (double)g->dy, idz = 1.0 / (double)g->dz;
+    M->idx = 1.0 / (double)g->dx; M->idy = 1.0 / (double)g->dy; M->idz = 1.0 / (double)g->dz;
 #else
-    const real idx = (float)(1.0 / (double)g->dx), idy = (float)(1.0 / (double)g->dy),
-               idz = (float)(1.0 / (double)g->dz);
+    M->idx = (float)(1.0 / (double)g->dx); M->idy = (float)(1.0 / (double)g->dy);
+    M->idz = (float)(1.0 / (double)g->dz);
 #endif
-    const float *c = FDC[R];
-    ora_point *sp = (ora_point *)calloc(ns > 0 ? ns : 1, sizeof(ora_point));
-    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    for (int m = 0; m < g->R; ++m) M->cf[m] = FDC[g->R][m];
+}
+
+static void med3_free(med3 *M) {
+    free(M->bk); free(M->rho); free(M->vp); free(M->prof);
+}
+
+/* update_wavefield (module_wavefield.f90:40-172) and
+ * update_wavefield_free_surface (:177-317): p-sweep, then vx, vy, vz sweeps
+ * with the new p.  With a free surface p starts at k = 2 and is mirrored,
+ * p(:,:,k) = -p(:,:,2-k) for k <= 0; v starts at k = 1 (vz at k = 2) and
+ * the top PML is off. */
+static void update3(const med3 *M, fld3 *F, real dt, int fs) {
+    const lay3 L = M->L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz, pml = L.pml, R = L.R;
+    const float *c = M->cf;
+    const size_t sx = 1, sy = (size_t)L.NX, sz = (size_t)L.NX * L.NY;
+    const int k0p = fs ? 2 : -pml + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = k0p; k <= nz + pml; ++k)
+        for (int j = -pml + 1; j <= ny + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                size_t b = I3L(L, i, j, k);
+                real s1 = (real)c[0] * (F->vx[b + sx] - F->vx[b]);
+                real s2 = (real)c[0] * (F->vy[b + sy] - F->vy[b]);
+                real s3 = (real)c[0] * (F->vz[b + sz] - F->vz[b]);
+                for (int m = 2; m <= R; ++m) {
+                    s1 = s1 + (real)c[m - 1] * (F->vx[b + m * sx] - F->vx[b - (m - 1) * sx]);
+                    s2 = s2 + (real)c[m - 1] * (F->vy[b + m * sy] - F->vy[b - (m - 1) * sy]);
+                    s3 = s3 + (real)c[m - 1] * (F->vz[b + m * sz] - F->vz[b - (m - 1) * sz]);
+                }
+                real dxv = M->idx * s1, dyv = M->idy * s2, dzv = M->idz * s3;
+                if (i <= 0 || i >= nx + 1) {
+                    int a = i + pml - 1;
+                    F->mvx[b] = (real)M->axi[a] * dxv + (real)M->bxi[a] * F->mvx[b];
+                    dxv = (dxv + F->mvx[b]) / (real)M->kxi[a];
+                }
+                if (j <= 0 || j >= ny + 1) {
+                    int a = j + pml - 1;
+                    F->mvy[b] = (real)M->ayi[a] * dyv + (real)M->byi[a] * F->mvy[b];
+                    dyv = (dyv + F->mvy[b]) / (real)M->kyi[a];
+                }
+                if (fs ? (k >= nz + 1) : (k <= 0 || k >= nz + 1)) {
+                    int a = k + pml - 1;
+                    F->mvz[b] = (real)M->azi[a] * dzv + (real)M->bzi[a] * F->mvz[b];
+                    dzv = (dzv + F->mvz[b]) / (real)M->kzi[a];
+                }
+                F->p[b] = F->p[b] - dt * M->bk[b] * (dxv + dyv + dzv);
+            }
+    if (fs) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = -pml + 1; k <= 0; ++k)
+            for (int j = -pml + 1; j <= ny + pml; ++j)
+                for (int i = -pml + 1; i <= nx + pml; ++i) F->p[I3L(L, i, j, k)] = -F->p[I3L(L, i, j, 2 - k)];
+    }
+    const int k0v = fs ? 1 : -pml + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = k0v; k <= nz + pml; ++k)
+        for (int j = -pml + 1; j <= ny + pml; ++j)
+            for (int i = -pml; i <= nx + pml - 1; ++i) {
+                size_t b = I3L(L, i, j, k);
+                real s = (real)c[0] * (F->p[b + sx] - F->p[b]);
+                for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (F->p[b + m * sx] - F->p[b - (m - 1) * sx]);
+                real d = M->idx * s;
+                int m1 = i + 1;
+                if (m1 <= 1 || m1 >= nx + 1) {
+                    int a = m1 + pml - 1;
+                    F->mpx[b + sx] = (real)M->axh[a] * d + (real)M->bxh[a] * F->mpx[b + sx];
+                    d = (d + F->mpx[b + sx]) / (real)M->kxh[a];
+                }
+                real reff = (real)0.5 * (M->rho[b + sx] + M->rho[b]);
+                F->vx[b + sx] = F->vx[b + sx] - dt / reff * d;
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = k0v; k <= nz + pml; ++k)
+        for (int j = -pml; j <= ny + pml - 1; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                size_t b = I3L(L, i, j, k);
+                real s = (real)c[0] * (F->p[b + sy] - F->p[b]);
+                for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (F->p[b + m * sy] - F->p[b - (m - 1) * sy]);
+                real d = M->idy * s;
+                int m1 = j + 1;
+                if (m1 <= 1 || m1 >= ny + 1) {
+                    int a = m1 + pml - 1;
+                    F->mpy[b + sy] = (real)M->ayh[a] * d + (real)M->byh[a] * F->mpy[b + sy];
+                    d = (d + F->mpy[b + sy]) / (real)M->kyh[a];
+                }
+                real reff = (real)0.5 * (M->rho[b + sy] + M->rho[b]);
+                F->vy[b + sy] = F->vy[b + sy] - dt / reff * d;
+            }
+    const int k0z = fs ? 1 : -pml;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = k0z; k <= nz + pml - 1; ++k)
+        for (int j = -pml + 1; j <= ny + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                size_t b = I3L(L, i, j, k);
+                real s = (real)c[0] * (F->p[b + sz] - F->p[b]);
+                for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (F->p[b + m * sz] - F->p[b - (m - 1) * sz]);
+                real d = M->idz * s;
+                int m1 = k + 1;
+                if (fs ? (m1 >= nz + 1) : (m1 <= 1 || m1 >= nz + 1)) {
+                    int a = m1 + pml - 1;
+                    F->mpz[b + sz] = (real)M->azh[a] * d + (real)M->bzh[a] * F->mpz[b + sz];
+                    d = (d + F->mpz[b + sz]) / (real)M->kzh[a];
+                }
+                real reff = (real)0.5 * (M->rho[b + sz] + M->rho[b]);
+                F->vz[b + sz] = F->vz[b + sz] - dt / reff * d;
+            }
+}
+
+/* add_source, module_wavefield.f90:322-444.  Force: amp / rho(gx,gy,gz),
+ * then vx at (hx, gy, gz) scaled by sin(polar) cos(azimuth), vy at
+ * (gx, hy, gz) by sin(polar) sin(azimuth), vz at (gx, gy, hz) by
+ * cos(polar).  With a free surface, only cells with z index >= 2. */
+static void add_source3(const med3 *M, fld3 *F, const ora_source3d *src, const ora_point *pts, int ns, int t,
+                        real dt, int fs) {
+    const lay3 L = M->L;
+    for (int s = 0; s < ns; ++s) {
+        int nbeg = fnint(src[s].t0 / (float)dt) + 1, nend = nbeg + src[s].nt_stf - 1;
+        if (t < nbeg || t > nend) continue;
+        real amp = src[s].stf[t - nbeg] * (real)src[s].amp * dt;
+        const ora_point *P = &pts[s];
+        if (src[s].mechanism == 1) {
+            amp = amp / M->rho[I3L(L, P->gx, P->gy, P->gz)];
+#if defined(ORA_F64)
+            real sp = sin((double)src[s].polar), cp = cos((double)src[s].polar);
+            real sa = sin((double)src[s].azimuth), ca = cos((double)src[s].azimuth);
+#else
+            real sp = sinf(src[s].polar), cp = cosf(src[s].polar);
+            real sa = sinf(src[s].azimuth), ca = cosf(src[s].azimuth);
+#endif
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || P->gz + irz >= 2) {
+                            size_t q = I3L(L, P->hx + irx, P->gy + iry, P->gz + irz);
+                            F->vx[q] = F->vx[q] + sp * ca * amp * (real)P->hxw[irx + NKW] * (real)P->iy[iry + NKW] *
+                                                      (real)P->iz[irz + NKW];
+                        }
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || P->gz + irz >= 2) {
+                            size_t q = I3L(L, P->gx + irx, P->hy + iry, P->gz + irz);
+                            F->vy[q] = F->vy[q] + sp * sa * amp * (real)P->ix[irx + NKW] * (real)P->hyw[iry + NKW] *
+                                                      (real)P->iz[irz + NKW];
+                        }
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || P->hz + irz >= 2) {
+                            size_t q = I3L(L, P->gx + irx, P->gy + iry, P->hz + irz);
+                            F->vz[q] = F->vz[q] + cp * amp * (real)P->ix[irx + NKW] * (real)P->iy[iry + NKW] *
+                                                      (real)P->hzw[irz + NKW];
+                        }
+        } else {
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || P->gz + irz >= 2) {
+                            size_t q = I3L(L, P->gx + irx, P->gy + iry, P->gz + irz);
+                            F->p[q] = F->p[q] + amp * (real)P->ix[irx + NKW] * (real)P->iy[iry + NKW] *
+                                                    (real)P->iz[irz + NKW];
+                        }
+        }
+    }
+}
+
+/* Receiver gather of state t-1 (submodule_forward_modeling.f90:74-110 in 3D). */
+static void record3(const lay3 *L, const real *p, const ora_receiver3d *rec, const ora_point *rp, int nr, int t,
+                    int nt, real *traces) {
+#pragma omp parallel for schedule(static)
+    for (int ir = 0; ir < nr; ++ir) {
+        if (rec[ir].weight == 0.0f) continue;
+        const ora_point *P = &rp[ir];
+        real acc = traces[(size_t)ir * nt + (t - 1)];
+        for (int irz = -NKW; irz <= NKW; ++irz)
+            for (int iry = -NKW; iry <= NKW; ++iry)
+                for (int irx = -NKW; irx <= NKW; ++irx)
+                    acc = acc + p[I3L(*L, P->gx + irx, P->gy + iry, P->gz + irz)] * (real)P->ix[irx + NKW] *
+                                    (real)P->iy[iry + NKW] * (real)P->iz[irz + NKW] * (real)rec[ir].weight;
+        traces[(size_t)ir * nt + (t - 1)] = acc;
+    }
+}
+
+/* Boundary strips, acoustic_iso_3d/module_boundary_saving.f90:83-207: vx on
+ * x in [2-R, 1] and [nx+1, nx+R], vy on the y faces, vz on the z faces, each
+ * over [1-R, n+R] on the other two axes; record layout = the Fortran order
+ * of the six array sections. */
+static size_t strip_len3(const lay3 *L) {
+    size_t R = (size_t)L->R, ex = L->nx + 2 * R, ey = L->ny + 2 * R, ez = L->nz + 2 * R;
+    return 2 * R * (ey * ez + ex * ez + ex * ey);
+}
+
+static void strips3(const lay3 *L, fld3 *F, real *rec, int save) {
+    const int R = L->R, nx = L->nx, ny = L->ny, nz = L->nz;
+    size_t k = 0;
+#define XFER3(arr, i, j, kk)                         \
+    do {                                             \
+        size_t q_ = I3L(*L, i, j, kk);               \
+        if (save) rec[k] = (arr)[q_];                \
+        else (arr)[q_] = rec[k];                     \
+        ++k;                                         \
+    } while (0)
+    for (int kk = 1 - R; kk <= nz + R; ++kk)
+        for (int j = 1 - R; j <= ny + R; ++j)
+            for (int i = 2 - R; i <= 1; ++i) XFER3(F->vx, i, j, kk);
+    for (int kk = 1 - R; kk <= nz + R; ++kk)
+        for (int j = 1 - R; j <= ny + R; ++j)
+            for (int i = nx + 1; i <= nx + R; ++i) XFER3(F->vx, i, j, kk);
+    for (int kk = 1 - R; kk <= nz + R; ++kk)
+        for (int j = 2 - R; j <= 1; ++j)
+            for (int i = 1 - R; i <= nx + R; ++i) XFER3(F->vy, i, j, kk);
+    for (int kk = 1 - R; kk <= nz + R; ++kk)
+        for (int j = ny + 1; j <= ny + R; ++j)
+            for (int i = 1 - R; i <= nx + R; ++i) XFER3(F->vy, i, j, kk);
+    for (int kk = 2 - R; kk <= 1; ++kk)
+        for (int j = 1 - R; j <= ny + R; ++j)
+            for (int i = 1 - R; i <= nx + R; ++i) XFER3(F->vz, i, j, kk);
+    for (int kk = nz + 1; kk <= nz + R; ++kk)
+        for (int j = 1 - R; j <= ny + R; ++j)
+            for (int i = 1 - R; i <= nx + R; ++i) XFER3(F->vz, i, j, kk);
+#undef XFER3
+}
+
+static void locate3_all(const ora_grid3d *g, int ns, const ora_source3d *src, ora_point *sp, int nr,
+                        const ora_receiver3d *rec, ora_point *rp) {
     for (int s = 0; s < ns; ++s)
         ORA_SYM(locate_point)(3, src[s].x, src[s].y, src[s].z, g->ox, g->oy, g->oz, g->dx, g->dy, g->dz, &sp[s]);
     for (int r = 0; r < nr; ++r)
         ORA_SYM(locate_point)(3, rec[r].x, rec[r].y, rec[r].z, g->ox, g->oy, g->oz, g->dx, g->dy, g->dz, &rp[r]);
+}
+
+/* compute_forward, acoustic_iso_3d/submodule_forward_modeling.f90:31-183.
+ * Step order: [save strips(t); at t = nt the final vx, vy, vz, p] ->
+ * record -> update -> source.  traces [nr][nt]; strips [nt][strip_len3] and
+ * final_state (vx, vy, vz, p padded arrays) optional. */
+static int forward3(const ora_grid3d *g, const float *vp, const float *rho, int ns, const ora_source3d *src,
+                    float f0max, int nr, const ora_receiver3d *rec, real *traces, real *strips, real *final_state) {
+    if (g->R < 2 || g->R > 8) return -1;
+    med3 M;
+    med3_init(&M, g, vp, rho, f0max);
+    fld3 F;
+    fld3_alloc(&F, &M.L);
+    ora_point *sp = (ora_point *)calloc(ns > 0 ? ns : 1, sizeof(ora_point));
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    locate3_all(g, ns, src, sp, nr, rec, rp);
     memset(traces, 0, sizeof(real) * (size_t)nr * g->nt);
-    const size_t sx = 1, sy = (size_t)NX, sz = (size_t)NX * NY;
+    const size_t n = lay3_n(&M.L), sl = strip_len3(&M.L);
     const real dt = (real)g->dt;
     for (int t = 1; t <= g->nt; ++t) {
-        /* receivers (submodule_forward_modeling.f90 in 3D) */
-#pragma omp parallel for schedule(static)
+        if (strips) {
+            strips3(&M.L, &F, strips + (size_t)(t - 1) * sl, 1);
+            if (t == g->nt && final_state) {
+                memcpy(final_state, F.vx, n * sizeof(real));
+                memcpy(final_state + n, F.vy, n * sizeof(real));
+                memcpy(final_state + 2 * n, F.vz, n * sizeof(real));
+                memcpy(final_state + 3 * n, F.p, n * sizeof(real));
+            }
+        }
+        record3(&M.L, F.p, rec, rp, nr, t, g->nt, traces);
+        update3(&M, &F, dt, g->free_surface);
+        add_source3(&M, &F, src, sp, ns, t, dt, g->free_surface);
+    }
+    free(sp); free(rp);
+    fld3_free(&F);
+    med3_free(&M);
+    return 0;
+}
+
+int ORA_SYM(acoustic3d_forward)(const ora_grid3d *g, const float *vp, const float *rho, int ns,
+                                const ora_source3d *src, float f0max, int nr,
+                                const ora_receiver3d *rec, real *traces) {
+    return forward3(g, vp, rho, ns, src, f0max, nr, rec, traces, NULL, NULL);
+}
+
+/* OWL's 3D gradient: compute_adjoint + compute_gradient,
+ * acoustic_iso_3d/submodule_fwi_gradient.f90:33-361 (energy preconditioning
+ * off).  Forward with boundary saving, then t = nt .. hnt: prev <- state;
+ * at t = nt load the final state; inject strips(t); update(-dt) on the
+ * forward and the adjoint fields; add adj(t) to pr; image over the interior.
+ * grad_vp, grad_rho: (nz, ny, nx) z-fast (the .grd arrays, permute 321),
+ * NaN/Inf mapped to zero. */
+int ORA_SYM(acoustic3d_gradient_owl)(const ora_grid3d *g, const float *vp, const float *rho, int ns,
+                                     const ora_source3d *src, float f0max, int hnt, int nr,
+                                     const ora_receiver3d *rec, const real *adj, real *grad_vp, real *grad_rho,
+                                     real *traces_out) {
+    if (g->R < 2 || g->R > 8 || ns < 1) return -1;
+    med3 M;
+    med3_init(&M, g, vp, rho, f0max);
+    const lay3 L = M.L;
+    const size_t n = lay3_n(&L), sl = strip_len3(&L);
+    real *strips = zalloc(sl * (size_t)g->nt);
+    real *final_state = zalloc(4 * n);
+    real *tr = traces_out ? traces_out : zalloc((size_t)nr * g->nt);
+    int rc = forward3(g, vp, rho, ns, src, f0max, nr, rec, tr, strips, final_state);
+    if (!traces_out) free(tr);
+    if (rc) { free(strips); free(final_state); med3_free(&M); return rc; }
+    fld3 F, A;
+    fld3_alloc(&F, &L);
+    fld3_alloc(&A, &L);
+    real *pp = zalloc(n), *pvx = zalloc(n), *pvy = zalloc(n), *pvz = zalloc(n);
+    ora_point *sp = (ora_point *)calloc((size_t)ns, sizeof(ora_point));
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    locate3_all(g, ns, src, sp, nr, rec, rp);
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t ni = (size_t)nx * ny * nz;
+    real *gv = zalloc(ni), *gr = zalloc(ni);
+    const real mdt = -(real)g->dt;
+    const int cc = g->cc_step_interval > 0 ? g->cc_step_interval : 1;
+    const size_t sy = (size_t)L.NX, sz = (size_t)L.NX * L.NY;
+    for (int t = g->nt; t >= hnt; --t) {
+        memcpy(pp, F.p, n * sizeof(real));
+        memcpy(pvx, F.vx, n * sizeof(real));
+        memcpy(pvy, F.vy, n * sizeof(real));
+        memcpy(pvz, F.vz, n * sizeof(real));
+        if (t == g->nt) {
+            memcpy(F.vx, final_state, n * sizeof(real));
+            memcpy(F.vy, final_state + n, n * sizeof(real));
+            memcpy(F.vz, final_state + 2 * n, n * sizeof(real));
+            memcpy(F.p, final_state + 3 * n, n * sizeof(real));
+        }
+        strips3(&L, &F, strips + (size_t)(t - 1) * sl, 0);
+        update3(&M, &F, mdt, g->free_surface);
+        update3(&M, &A, mdt, g->free_surface);
         for (int ir = 0; ir < nr; ++ir) {
             if (rec[ir].weight == 0.0f) continue;
             const ora_point *P = &rp[ir];
-            real acc = traces[(size_t)ir * g->nt + (t - 1)];
-            for (int irz = -NKW; irz <= NKW; ++irz)
-                for (int iry = -NKW; iry <= NKW; ++iry)
-                    for (int irx = -NKW; irx <= NKW; ++irx)
-                        acc = acc + p[I3(P->gx + irx, P->gy + iry, P->gz + irz)] * (real)P->ix[irx + NKW] *
-                                        (real)P->iy[iry + NKW] * (real)P->iz[irz + NKW] * (real)rec[ir].weight;
-            traces[(size_t)ir * g->nt + (t - 1)] = acc;
-        }
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int k = -pml + 1; k <= nz + pml; ++k)
-            for (int j = -pml + 1; j <= ny + pml; ++j)
-                for (int i = -pml + 1; i <= nx + pml; ++i) {
-                    size_t b = I3(i, j, k);
-                    real s1 = (real)c[0] * (vx[b + sx] - vx[b]);
-                    real s2 = (real)c[0] * (vy[b + sy] - vy[b]);
-                    real s3 = (real)c[0] * (vz[b + sz] - vz[b]);
-                    for (int m = 2; m <= R; ++m) {
-                        s1 = s1 + (real)c[m - 1] * (vx[b + m * sx] - vx[b - (m - 1) * sx]);
-                        s2 = s2 + (real)c[m - 1] * (vy[b + m * sy] - vy[b - (m - 1) * sy]);
-                        s3 = s3 + (real)c[m - 1] * (vz[b + m * sz] - vz[b - (m - 1) * sz]);
-                    }
-                    real dxv = idx * s1, dyv = idy * s2, dzv = idz * s3;
-                    if (i <= 0 || i >= nx + 1) {
-                        int a = i + pml - 1;
-                        mvx[b] = (real)axi[a] * dxv + (real)bxi[a] * mvx[b];
-                        dxv = (dxv + mvx[b]) / (real)kxi[a];
-                    }
-                    if (j <= 0 || j >= ny + 1) {
-                        int a = j + pml - 1;
-                        mvy[b] = (real)ayi[a] * dyv + (real)byi[a] * mvy[b];
-                        dyv = (dyv + mvy[b]) / (real)kyi[a];
-                    }
-                    if (k <= 0 || k >= nz + 1) {
-                        int a = k + pml - 1;
-                        mvz[b] = (real)azi[a] * dzv + (real)bzi[a] * mvz[b];
-                        dzv = (dzv + mvz[b]) / (real)kzi[a];
-                    }
-                    p[b] = p[b] - dt * bk[b] * (dxv + dyv + dzv);
-                }
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int k = -pml + 1; k <= nz + pml; ++k)
-            for (int j = -pml + 1; j <= ny + pml; ++j)
-                for (int i = -pml; i <= nx + pml - 1; ++i) {
-                    size_t b = I3(i, j, k);
-                    real s = (real)c[0] * (p[b + sx] - p[b]);
-                    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (p[b + m * sx] - p[b - (m - 1) * sx]);
-                    real d = idx * s;
-                    int m1 = i + 1;
-                    if (m1 <= 1 || m1 >= nx + 1) {
-                        int a = m1 + pml - 1;
-                        mpx[b + sx] = (real)axh[a] * d + (real)bxh[a] * mpx[b + sx];
-                        d = (d + mpx[b + sx]) / (real)kxh[a];
-                    }
-                    real reff = (real)0.5 * (rh[b + sx] + rh[b]);
-                    vx[b + sx] = vx[b + sx] - dt / reff * d;
-                }
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int k = -pml + 1; k <= nz + pml; ++k)
-            for (int j = -pml; j <= ny + pml - 1; ++j)
-                for (int i = -pml + 1; i <= nx + pml; ++i) {
-                    size_t b = I3(i, j, k);
-                    real s = (real)c[0] * (p[b + sy] - p[b]);
-                    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (p[b + m * sy] - p[b - (m - 1) * sy]);
-                    real d = idy * s;
-                    int m1 = j + 1;
-                    if (m1 <= 1 || m1 >= ny + 1) {
-                        int a = m1 + pml - 1;
-                        mpy[b + sy] = (real)ayh[a] * d + (real)byh[a] * mpy[b + sy];
-                        d = (d + mpy[b + sy]) / (real)kyh[a];
-                    }
-                    real reff = (real)0.5 * (rh[b + sy] + rh[b]);
-                    vy[b + sy] = vy[b + sy] - dt / reff * d;
-                }
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int k = -pml; k <= nz + pml - 1; ++k)
-            for (int j = -pml + 1; j <= ny + pml; ++j)
-                for (int i = -pml + 1; i <= nx + pml; ++i) {
-                    size_t b = I3(i, j, k);
-                    real s = (real)c[0] * (p[b + sz] - p[b]);
-                    for (int m = 2; m <= R; ++m) s = s + (real)c[m - 1] * (p[b + m * sz] - p[b - (m - 1) * sz]);
-                    real d = idz * s;
-                    int m1 = k + 1;
-                    if (m1 <= 1 || m1 >= nz + 1) {
-                        int a = m1 + pml - 1;
-                        mpz[b + sz] = (real)azh[a] * d + (real)bzh[a] * mpz[b + sz];
-                        d = (d + mpz[b + sz]) / (real)kzh[a];
-                    }
-                    real reff = (real)0.5 * (rh[b + sz] + rh[b]);
-                    vz[b + sz] = vz[b + sz] - dt / reff * d;
-                }
-        /* explosion sources (add_source, acoustic_iso_3d/module_wavefield.f90:322-444) */
-        for (int s = 0; s < ns; ++s) {
-            int nbeg = fnint(src[s].t0 / g->dt) + 1, nend = nbeg + src[s].nt_stf - 1;
-            if (t < nbeg || t > nend) continue;
-            real amp = src[s].stf[t - nbeg] * (real)src[s].amp * dt;
-            const ora_point *P = &sp[s];
+            real a = adj[(size_t)ir * g->nt + (t - 1)];
             for (int irz = -NKW; irz <= NKW; ++irz)
                 for (int iry = -NKW; iry <= NKW; ++iry)
                     for (int irx = -NKW; irx <= NKW; ++irx) {
-                        size_t q = I3(P->gx + irx, P->gy + iry, P->gz + irz);
-                        p[q] = p[q] + amp * (real)P->ix[irx + NKW] * (real)P->iy[iry + NKW] * (real)P->iz[irz + NKW];
+                        size_t q = I3L(L, P->gx + irx, P->gy + iry, P->gz + irz);
+                        A.p[q] = A.p[q] + a * (real)P->ix[irx + NKW] * (real)P->iy[iry + NKW] *
+                                              (real)P->iz[irz + NKW] * (real)rec[ir].weight;
+                    }
+        }
+        if (t % cc == 0) {
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int k = 1; k <= nz; ++k)
+                for (int j = 1; j <= ny; ++j)
+                    for (int i = 1; i <= nx; ++i) {
+                        size_t q = I3L(L, i, j, k);
+                        size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx + (size_t)(k - 1) * nx * ny;
+                        real v = M.vp[q];
+                        real spp = F.p[q] - pp[q];
+                        gv[o] = gv[o] - spp * A.p[q] / (M.rho[q] * (v * v * v));
+                        real sxv = (F.vx[q] + F.vx[q + 1]) - (pvx[q] + pvx[q + 1]);
+                        real rxv = A.vx[q] + A.vx[q + 1];
+                        real syv = (F.vy[q] + F.vy[q + sy]) - (pvy[q] + pvy[q + sy]);
+                        real ryv = A.vy[q] + A.vy[q + sy];
+                        real szv = (F.vz[q] + F.vz[q + sz]) - (pvz[q] + pvz[q + sz]);
+                        real rzv = A.vz[q] + A.vz[q + sz];
+                        gr[o] = gr[o] + sxv * rxv + syv * ryv + szv * rzv;
                     }
         }
     }
-    free(sp); free(rp); free(ax);
-    free(p); free(vx); free(vy); free(vz);
-    free(mvx); free(mvy); free(mvz); free(mpx); free(mpy); free(mpz);
-    free(bk); free(rh);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx + (size_t)(k - 1) * nx * ny;
+                size_t d = (size_t)(k - 1) + (size_t)(j - 1) * nz + (size_t)(i - 1) * nz * ny;
+                real a = gv[o], b = gr[o];
+                grad_vp[d] = isfinite(a) ? a : 0;
+                grad_rho[d] = isfinite(b) ? b : 0;
+            }
+    free(gv); free(gr); free(sp); free(rp);
+    free(pp); free(pvx); free(pvy); free(pvz);
+    fld3_free(&F); fld3_free(&A);
+    med3_free(&M);
+    free(strips); free(final_state);
     return 0;
 }
 
