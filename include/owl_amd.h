@@ -221,27 +221,56 @@ owl_status owl_acoustic2d_kernel_stats(const owl_acoustic2d *b, int kind, double
                                        long long *launches, double *bytes_total);
 
 /* ---- 3D acoustic: batched shots on one GPU --------------------------------
- * OWL's acoustic_iso_3d forward solver (compute_forward,
- * acoustic_iso_3d/submodule_forward_modeling.f90:31-183; update_wavefield
- * module_wavefield.f90:40-172 without domain decomposition) for
- * which_medium = 'acoustic-iso' in src/forward/3d/module_modeling.f90.
- * vp, rho: (nz, ny, nx) z fastest.  Explosion sources, no free surface.
- * Traces per shot [nr][nt]. */
+ * OWL's acoustic_iso_3d solver without domain decomposition, for
+ * which_medium = 'acoustic-iso' in src/forward/3d/module_modeling.f90 and
+ * src/inverse/3d/module_gradient.f90.  vp, rho: (nz, ny, nx) z fastest.
+ * Explosion and force sources; free surface at the top when
+ * cfg->free_surface (every window then starts at z index 1).  Traces per
+ * shot [nr][nt]. */
 typedef struct owl_acoustic3d owl_acoustic3d;
 owl_status owl_acoustic3d_create(const owl_acoustic_config *cfg, const float *vp,
                                  const float *rho, int nshot, const owl_shot *shots,
                                  owl_acoustic3d **out);
 void owl_acoustic3d_destroy(owl_acoustic3d *b);
 int owl_acoustic3d_nt(const owl_acoustic3d *b);
-owl_status owl_acoustic3d_forward(owl_acoustic3d *b, float *traces);
+/* Forward modelling of every shot (compute_forward,
+ * acoustic_iso_3d/submodule_forward_modeling.f90:31-183; update_wavefield
+ * module_wavefield.f90:40-317, add_source :322-444).  traces may be NULL.
+ * save_boundary != 0 keeps the boundary records and the final state that
+ * OWL's gradient needs (reconstruct = .true.,
+ * acoustic_iso_3d/module_boundary_saving.f90), in device memory. */
+owl_status owl_acoustic3d_forward(owl_acoustic3d *b, float *traces, int save_boundary);
+/* Adjoint pass alone, OWL's algorithm (compute_adjoint,
+ * acoustic_iso_3d/submodule_fwi_gradient.f90:33-274, compute_gradient
+ * :276-361, energy preconditioning off), after a forward with
+ * save_boundary = 1.  adj: adjoint source traces laid out like the forward
+ * traces.  grad_vp, grad_rho: shot after shot, each (nz_s, ny_s, nx_s) of the
+ * shot's window, z fastest: the arrays of shot_<id>_grad_vp.grd / _rho.grd. */
+owl_status owl_acoustic3d_adjoint(owl_acoustic3d *b, const float *adj, float *grad_vp,
+                                  float *grad_rho);
+/* FWI gradient over all shots: forward with boundary saving, L2 misfit and
+ * adjoint source syn - obs (module_inversion_adjoint_source.f90:371-383,
+ * trace-discard rule :1042-1054), OWL's reverse pass, and the sum of the
+ * window gradients on the full model (nz, ny, nx), z fastest, FP64.
+ * obs: observed traces (kept on the device; NULL reuses the last ones).
+ * misfit[nshot], grad_vp, grad_rho and traces_out may be NULL. */
+owl_status owl_acoustic3d_gradient(owl_acoustic3d *b, const float *obs, double *misfit,
+                                   double *grad_vp, double *grad_rho, float *traces_out);
 /* Launch shape of the sweeps (results do not depend on it; no OWL
  * counterpart): z planes one block marches over, and the kernel variant
  * (0: x-y planes staged in LDS; 1, 2, 3: barrier-free column segments with
  * 4, 8, 16 rows per block, 5-7 the same with an XCD-contiguous tile order;
- * 8: four-wide columns (float4 per thread, the default), 24: the same with
- * the XCD-contiguous order). */
+ * 8: four-wide columns (float4 per thread), 24: the same with the
+ * XCD-contiguous order). */
 owl_status owl_acoustic3d_set_launch(owl_acoustic3d *b, int planes, int variant);
-/* Timing as for the 2D batch; kinds: 0 p-sweep, 1 v-sweep, 2 source/receivers. */
+/* Launch-shape autotuner (no OWL counterpart): times `steps` sweeps of the
+ * batch under each candidate (variant, planes) and keeps the fastest;
+ * writes the chosen shape (either pointer may be NULL).  Leaves the wave
+ * state zeroed; results do not depend on the shape. */
+owl_status owl_acoustic3d_autotune(owl_acoustic3d *b, int steps, int *planes, int *variant);
+/* Timing as for the 2D batch; kinds: 0 p-sweep, 1 v-sweep, 2 source and
+ * receivers, 3 imaging condition, 4 boundary records, 5 adjoint-source
+ * injection. */
 owl_status owl_acoustic3d_set_profiling(owl_acoustic3d *b, int on);
 owl_status owl_acoustic3d_kernel_stats(const owl_acoustic3d *b, int kind, double *total_ms,
                                        long long *launches, double *bytes_total);

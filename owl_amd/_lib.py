@@ -96,8 +96,11 @@ _SIGNATURES = {
     "owl_acoustic3d_create": (C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
     "owl_acoustic3d_destroy": (None, [_P]),
     "owl_acoustic3d_nt": (C.c_int, [_P]),
-    "owl_acoustic3d_forward": (C.c_int, [_P, _P]),
+    "owl_acoustic3d_forward": (C.c_int, [_P, _P, C.c_int]),
+    "owl_acoustic3d_adjoint": (C.c_int, [_P, _P, _P, _P]),
+    "owl_acoustic3d_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "owl_acoustic3d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_acoustic3d_autotune": (C.c_int, [_P, C.c_int, _P, _P]),
     "owl_acoustic3d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_acoustic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_acoustic3d_last_stats": (C.c_int, [_P, _P, _P]),

@@ -1,7 +1,9 @@
-"""GPU parity of the 3D acoustic forward (C ABI -> HIP) against the oracle:
-traces bit-identical to the float32 oracle (same operation order, no FMA
-contraction), per shot, for a batch with uneven windows; independent of
-the z-chunk launch shape."""
+"""GPU parity of the 3D acoustic path (C ABI -> HIP) against the oracle:
+forward traces bit-identical to the float32 oracle (same operation order,
+no FMA contraction), per shot, for a batch with uneven windows, with
+explosion and force sources, with and without a free surface, independent of
+the launch shape; OWL's 3D gradient within 1e-4 relative L2 of the float32
+oracle (the GPU accumulates in FP64, the oracle in FP32)."""
 import numpy as np
 import pytest
 
@@ -17,7 +19,7 @@ def _model(nz, ny, nx, seed=0):
     return vp, gardner(vp)
 
 
-def _oracle(o, vp, rho, win, geom, dx, dt, nt, pml, R):
+def _oracle_case(vp, rho, win, geom, dx, dt, nt, pml, R, fs=False):
     nz, ny, nx = vp.shape
     w = dict(nxbeg=1, nxend=nx, nybeg=1, nyend=ny, nzbeg=1, nzend=nz)
     w.update(win or {})
@@ -26,7 +28,7 @@ def _oracle(o, vp, rho, win, geom, dx, dt, nt, pml, R):
     f = np.float32
     g = dict(nx=sub_vp.shape[2], ny=sub_vp.shape[1], nz=sub_vp.shape[0], dx=dx, dy=dx, dz=dx,
              ox=float(f(w["nxbeg"] - 1) * f(dx)), oy=float(f(w["nybeg"] - 1) * f(dx)),
-             oz=float(f(w["nzbeg"] - 1) * f(dx)), dt=dt, nt=nt, pml=pml, R=R)
+             oz=float(f(w["nzbeg"] - 1) * f(dx)), dt=dt, nt=nt, pml=pml, R=R, free_surface=fs)
     x1 = f(g["nx"] - 1) * f(dx) + f(g["ox"])
     y1 = f(g["ny"] - 1) * f(dx) + f(g["oy"])
     z1 = f(g["nz"] - 1) * f(dx) + f(g["oz"])
@@ -34,7 +36,13 @@ def _oracle(o, vp, rho, win, geom, dx, dt, nt, pml, R):
     for r in geom.recr:
         ok = g["ox"] <= f(r.x) <= x1 and g["oy"] <= f(r.y) <= y1 and g["oz"] <= f(r.z) <= z1
         recs.append(dict(x=r.x, y=r.y, z=r.z, weight=r.weight if ok else 0.0))
-    srcs = [dict(x=s.x, y=s.y, z=s.z, f0=s.f0) for s in geom.srcr]
+    srcs = [dict(x=s.x, y=s.y, z=s.z, f0=s.f0, amp=s.amp, mechanism=s.mechanism, polar=s.polar,
+                 azimuth=s.azimuth) for s in geom.srcr]
+    return g, sub_vp, sub_rho, srcs, recs
+
+
+def _oracle(o, vp, rho, win, geom, dx, dt, nt, pml, R, fs=False):
+    g, sub_vp, sub_rho, srcs, recs = _oracle_case(vp, rho, win, geom, dx, dt, nt, pml, R, fs)
     return o.acoustic3d_forward(g, sub_vp, sub_rho, srcs, recs)
 
 
@@ -78,4 +86,91 @@ def test_forward3d_zchunk_invariance(gpu):
     for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (1, 8), (13, 24), (64, 8), (64, 1)):
         b.set_launch(kc, var)
         assert np.array_equal(b.forward()[0], ref), (kc, var)
+    kc, var = b.autotune(1)
+    assert np.array_equal(b.forward()[0], ref), ("autotuned", kc, var)
+    b.close()
+
+
+@pytest.mark.parametrize("variant", [1, 8, 0])
+def test_forward3d_free_surface_and_force(oracle32, gpu, variant):
+    """Free surface (p from k = 2, mirror, no top PML) with an explosion and
+    an oblique force source near the surface; bit-exact per launch variant."""
+    nz, ny, nx, dx, dt, tmax, pml, R = 24, 22, 26, 10.0, 0.001, 0.11, 8, 4
+    vp, rho = _model(nz, ny, nx, seed=11)
+    geoms = [
+        (Geometry(id=1, srcr=[Source(x=123.0, y=101.7, z=15.0, f0=20.0)],
+                  recr=[Receiver(x=20.0 + 9.0 * k, y=60.5, z=12.0) for k in range(25)]), None),
+        (Geometry(id=2, srcr=[Source(x=140.0, y=110.0, z=31.3, f0=20.0, mechanism="force", polar=0.7,
+                                     azimuth=2.1, amp=1e3)],
+                  recr=[Receiver(x=100.0 + 10.0 * k, y=100.0, z=25.0) for k in range(12)]),
+         dict(nxbeg=5, nxend=26, nybeg=3, nyend=22, nzbeg=1, nzend=20)),
+    ]
+    print(f"case: 3D free surface + force, variant {variant}", flush=True)
+    b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=R, free_surface=True)
+    b.set_launch(7, variant)
+    out = b.forward()
+    for (geom, win), gtr in zip(geoms, out):
+        ref = _oracle(oracle32, vp, rho, win, geom, dx, dt, b.nt, pml, R, fs=True)
+        assert np.abs(ref).max() > 0
+        assert np.array_equal(gtr, ref), f"shot {geom.id}: max diff {np.abs(gtr - ref).max()}"
+    b.close()
+
+
+def _gradient_case(fs):
+    nz, ny, nx, dx, dt, tmax, pml, R = 22, 24, 26, 10.0, 0.001, 0.15, 8, 4
+    vp, rho = _model(nz, ny, nx, seed=21)
+    geoms = [
+        (Geometry(id=1, srcr=[Source(x=120.0, y=110.0, z=70.0 if not fs else 30.0, f0=20.0)],
+                  recr=[Receiver(x=15.0 + 10.0 * k, y=40.0 + 20.0 * (k % 3), z=30.5) for k in range(24)]), None),
+        (Geometry(id=2, srcr=[Source(x=170.0, y=140.0, z=60.0, f0=20.0)],
+                  recr=[Receiver(x=100.0 + 10.0 * k, y=150.0, z=20.0) for k in range(15)]),
+         dict(nxbeg=8, nxend=26, nybeg=5, nyend=24, nzbeg=1, nzend=20)),
+    ]
+    return nz, ny, nx, dx, dt, tmax, pml, R, vp, rho, geoms
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_owl_gradient3d_matches_oracle(oracle32, gpu, fs):
+    nz, ny, nx, dx, dt, tmax, pml, R, vp, rho, geoms = _gradient_case(fs)
+    print(f"case: 3D OWL gradient fs={fs}", flush=True)
+    b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=R, free_surface=fs)
+    syn = b.forward(save_boundary=True)
+    rng = np.random.default_rng(5)
+    adj = [(0.5 * t + 1e-3 * rng.standard_normal(t.shape)).astype(np.float32) for t in syn]
+    grads = b.adjoint(adj)
+    for (geom, win), (gv, gr), a, tr in zip(geoms, grads, adj, syn):
+        g, svp, srho, srcs, recs = _oracle_case(vp, rho, win, geom, dx, dt, b.nt, pml, R, fs)
+        rv, rr, rt = oracle32.acoustic3d_gradient_owl(g, svp, srho, srcs, recs, a)
+        assert np.array_equal(tr, rt)
+        assert gv.shape == rv.shape
+        assert np.abs(rv).max() > 0
+        assert rel_l2(gv, rv) < 1e-4, rel_l2(gv, rv)
+        assert rel_l2(gr, rr) < 1e-4, rel_l2(gr, rr)
+    b.close()
+
+
+def test_owl_gradient3d_batch_sum_and_misfit(oracle32, gpu):
+    nz, ny, nx, dx, dt, tmax, pml, R, vp, rho, geoms = _gradient_case(False)
+    print("case: 3D gradient batch sum", flush=True)
+    vp_true = (vp * 1.03).astype(np.float32)
+    bt = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp_true, rho, geoms, dt, tmax, pml=pml, fdhalf=R)
+    obs = bt.forward()
+    bt.close()
+    b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=R)
+    mis, gv, gr, syn = b.gradient(obs)
+    ref_v = np.zeros((nz, ny, nx))
+    ref_r = np.zeros((nz, ny, nx))
+    for k, ((geom, win), o, s) in enumerate(zip(geoms, obs, syn)):
+        g, svp, srho, srcs, recs = _oracle_case(vp, rho, win, geom, dx, dt, b.nt, pml, R)
+        w = np.array([r["weight"] for r in recs], np.float32)
+        m_ref, adj = oracle32.adjoint_source_l2(w, s, o)
+        assert abs(mis[k] - m_ref) <= 1e-6 * m_ref
+        rv, rr, _ = oracle32.acoustic3d_gradient_owl(g, svp, srho, srcs, recs, adj)
+        w0 = win or dict(nxbeg=1, nxend=nx, nybeg=1, nyend=ny, nzbeg=1, nzend=nz)
+        sl = (slice(w0["nzbeg"] - 1, w0["nzend"]), slice(w0["nybeg"] - 1, w0["nyend"]),
+              slice(w0["nxbeg"] - 1, w0["nxend"]))
+        ref_v[sl] += rv
+        ref_r[sl] += rr
+    assert rel_l2(gv, ref_v) < 1e-4
+    assert rel_l2(gr, ref_r) < 1e-4
     b.close()
