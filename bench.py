@@ -7,14 +7,19 @@ CFS-PML; true Vp = 1500->4000 m/s linear gradient with a seeded +-5 %
 Gaussian-smoothed perturbation (8-cell correlation), Gardner rho; initial
 model = true model smoothed with sigma = 10 cells; 32 Ricker (10 Hz)
 explosion shots at z = 20 m, x = 140 + 120 k m; 391 receivers at z = 20 m
-every 10 m.  Observed data are modelled on the GPU from the true model
-before the timed region; all inputs are resident in HBM when it starts.
+every 10 m; L2 misfit.  Observed data are modelled on the GPU from the true
+model before the timed region; all inputs are resident in HBM when it starts.
 
-Reported: grid-point updates per second of the batched gradient (all 32
-shots in every launch), FWI-gradient seconds per shot, the same for the
-plain per-shot GPU path (one shot per batch, shots in sequence), the
-dominant kernel's HBM roofline fraction from HIP events over the timed
-region, and the CPU restatement of OWL's algorithm timed on this host.
+Headline: the batched + checkpointed gradient -- all 32 shots in every
+launch, exact discrete adjoint, forward states recomputed from 40 binomial
+checkpoints (FP32) -- in grid-point updates per second (useful updates: one
+forward and one adjoint sweep per step and shot; the recomputation is extra
+work inside the timed region) and seconds per shot.  Next to it: FP16
+checkpoints, the plain per-shot GPU path (one shot per launch, full FP32
+storage, no checkpointing), OWL's own gradient algorithm (batched and per
+shot), the same gradient through torch.autograd, the 3D acoustic batched
+forward, the dominant kernels' HBM rooflines from HIP events over the timed
+regions, and OWL's algorithm restated in C timed on this host's cores.
 """
 import argparse
 import json
@@ -29,6 +34,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
+CFG = dict(nx=401, nz=401, dx=10.0, dz=10.0, dt=0.001, tmax=1.999, pml=40, fdhalf=4)
+SNAPSHOTS = 40
+
 
 def workload():
     from owl_amd.synthetic import acoustic2d_initial_model, acoustic2d_shots, acoustic2d_true_model
@@ -36,9 +44,6 @@ def workload():
     vp0, rho0 = acoustic2d_initial_model(vp, rho, 10.0)
     shots = acoustic2d_shots(32)
     return vp, rho, vp0, rho0, shots
-
-
-CFG = dict(nx=401, nz=401, dx=10.0, dz=10.0, dt=0.001, tmax=1.999, pml=40, fdhalf=4)
 
 
 def make_batch(vp, rho, vp0, rho0, shots):
@@ -51,29 +56,46 @@ def make_batch(vp, rho, vp0, rho0, shots):
     return b
 
 
-def time_gradients(b, steps, warmup, sync):
+def roofline_of(stats):
+    """Dominant kernel (by device time among those with modelled bytes):
+    algorithmic bytes per launch over its average launch time."""
+    name, st = max(((k, v) for k, v in stats.items() if v["bytes_total"] > 0 and v["launches"]),
+                   key=lambda kv: kv[1]["total_ms"])
+    avg_ms = st["total_ms"] / st["launches"]
+    bytes_launch = st["bytes_total"] / st["launches"]
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    return name, dict(bound="hbm", kernel=name, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=achieved / HBM_PEAK_GBS, avg_launch_ms=avg_ms,
+                      algorithmic_bytes_per_launch=bytes_launch)
+
+
+def kernel_table(stats, steps):
+    return {k: {"ms": round(v["total_ms"] / steps, 3), "launches": v["launches"] // steps,
+                "GB/s": round(v["bytes_total"] / (v["total_ms"] * 1e-3) / 1e9, 1) if v["bytes_total"] else None}
+            for k, v in stats.items() if v["launches"]}
+
+
+def time_gradient(b, steps, warmup, sync, **kw):
     for _ in range(warmup):
-        b.gradient(want_traces=False)
+        b.gradient(want_traces=False, **kw)
     b.set_profiling(True)
     sync()
     t0 = time.perf_counter()
     updates = 0.0
-    dev_ms = 0.0
     for _ in range(steps):
-        b.gradient(want_traces=False)
-        ms, up = b.last_stats()
-        updates += up
-        dev_ms += ms
+        _, gv, _, _ = b.gradient(want_traces=False, **kw)
+        updates += b.last_stats()[1]
     sync()
     wall = time.perf_counter() - t0
     stats = b.kernel_stats()
     b.set_profiling(False)
-    return wall, updates, dev_ms, stats
+    return wall, updates, stats, gv
 
 
 def cpu_baseline(vp, rho, vp0, rho0, shot, nt):
     """OWL's algorithm restated in C (oracle, float32, OpenMP), one shot of the
-    same gradient workload: forward with boundary saving + reverse pass."""
+    same gradient workload: forward for the misfit, forward with boundary
+    saving, OWL's reverse pass."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     o = Oracle(32)
@@ -89,34 +111,52 @@ def cpu_baseline(vp, rho, vp0, rho0, shot, nt):
     wall = time.perf_counter() - t0
     _, hnt = o.stf(shot.srcr[0].f0, CFG["dt"])
     pts = (CFG["nx"] + 2 * CFG["pml"]) * (CFG["nz"] + 2 * CFG["pml"])
-    # the oracle runs nt forward steps twice (misfit run + boundary-saving
-    # run inside the gradient) and 2 x (nt - hnt + 1) reverse updates
     updates = pts * (2 * nt + 2 * (nt - hnt + 1))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=updates / wall, unit="grid-pt updates/s", cores=threads, kind="port",
-                sample="1 of 32 shots of the same gradient (full 2000 steps): forward for the "
-                       "misfit, forward with boundary saving, OWL reverse pass",
+                sample="1 of the 32 shots, full 2000 steps: forward for the misfit, forward with "
+                       "boundary saving, OWL reverse pass",
                 seconds=round(wall, 3), s_per_shot=round(wall, 3))
 
 
-def load_traffic(kernel):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
-    if os.path.exists(path):
-        try:
-            return json.load(open(path)).get("bytes_per_launch")
-        except Exception:
-            return None
-    return None
+def bench_3d(sync):
+    """3D acoustic batched forward: 256^3, 10 m, R = 4, 1500 steps, 20-cell
+    PML, 5-layer model, 16 shots with uneven 128..256 windows, 100 x 100
+    receivers each.  dt = 0.9 ms: 1 ms exceeds the scheme's stable step
+    (0.966 ms at vmax = 4500 m/s), which the library refuses."""
+    from owl_amd import Acoustic3DBatch
+    from owl_amd.synthetic import acoustic3d_layered_model, acoustic3d_shots
+    vp, rho = acoustic3d_layered_model()
+    shots = acoustic3d_shots()
+    dt = 0.0009
+    b = Acoustic3DBatch(256, 256, 256, 10.0, 10.0, 10.0, vp, rho, shots, dt, 1499 * dt, pml=20, fdhalf=4)
+    kc, var = b.autotune(3)
+    b.set_profiling(True)
+    sync()
+    t0 = time.perf_counter()
+    b.forward(want_traces=False)
+    sync()
+    wall = time.perf_counter() - t0
+    stats = b.kernel_stats()
+    ms, updates = b.last_stats()
+    b.close()
+    _, roof = roofline_of(stats)
+    return {"workload": "3D acoustic batched forward, 256^3, 16 shots (windows 128-256), 1500 steps, "
+                        "dt 0.9 ms, R 4, pml 20",
+            "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
+            "launch_shape": {"planes": kc, "variant": var}, "roofline": roof,
+            "kernels": kernel_table(stats, 1)}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed FWI gradients")
+    ap.add_argument("--steps", type=int, default=2, help="timed FWI gradients")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--plain-shots", type=int, default=32,
+    ap.add_argument("--plain-shots", type=int, default=4,
                     help="shots timed through the plain per-shot path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -128,47 +168,75 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    from owl_amd import (OWL_CKPT_BINOMIAL, OWL_CKPT_FULL, OWL_GRAD_EXACT, OWL_GRAD_OWL, OWL_STORE_F16,
+                         OWL_STORE_F32)
     sync = torch.cuda.synchronize
-    print("bench: building workload", file=sys.stderr, flush=True)
+    log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
+    log("building workload")
     vp, rho, vp0, rho0, shots = workload()
+    nshot = len(shots)
+    exact_b40 = dict(mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_BINOMIAL, ckpt_param=SNAPSHOTS, store=OWL_STORE_F32)
 
-    # ---- batched path: all 32 shots in every launch
+    # ---- batched + checkpointed (headline)
     b = make_batch(vp, rho, vp0, rho0, shots)
     nt = b.nt
-    print("bench: batched gradient", file=sys.stderr, flush=True)
-    wall, updates, dev_ms, stats = time_gradients(b, args.steps, args.warmup, sync)
-    nshot = len(shots)
+    log("batched exact gradient, binomial 40 snapshots, FP32")
+    wall, updates, stats, g32 = time_gradient(b, args.steps, args.warmup, sync, **exact_b40)
     value = updates / wall
-    ms_per_step = 1e3 * wall / args.steps
+    s_shot = wall / args.steps / nshot
+    name, roof = roofline_of(stats)
+    log("batched exact gradient, binomial 40 snapshots, FP16")
+    w16, _, st16, g16 = time_gradient(b, 1, 1, sync, **dict(exact_b40, store=OWL_STORE_F16))
+    log("batched OWL-algorithm gradient")
+    wowl, _, stowl, _ = time_gradient(b, 1, 1, sync, mode=OWL_GRAD_OWL)
 
-    # dominant kernel by device time, with its algorithmic bytes
-    name, st = max(((k, v) for k, v in stats.items() if v["bytes_total"] > 0),
-                   key=lambda kv: kv[1]["total_ms"])
-    avg_ms = st["total_ms"] / st["launches"]
-    bytes_launch = st["bytes_total"] / st["launches"]
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    log("torch.autograd gradient (binomial 40, FP32)")
+    from owl_amd.autograd import propagate
+    b.set_model(vp, rho)
+    obs = torch.from_numpy(np.concatenate(b.forward(), axis=0))
+    tv = torch.tensor(vp0, requires_grad=True)
+    tr_ = torch.tensor(rho0, requires_grad=True)
+    sync()
+    t0 = time.perf_counter()
+    syn = propagate(tv, tr_, b, OWL_CKPT_BINOMIAL, SNAPSHOTS, OWL_STORE_F32)
+    loss = 0.5 * ((syn - obs) ** 2).sum()
+    loss.backward()
+    sync()
+    w_autograd = time.perf_counter() - t0
     b.close()
 
-    # ---- plain per-shot path: one shot per batch, shots in sequence
-    print("bench: plain per-shot gradient", file=sys.stderr, flush=True)
-    plain = []
+    # ---- plain per-shot: one shot per launch, full FP32 storage
+    log("plain per-shot exact gradient (full storage) and OWL gradient")
+    plain, plain_owl = [], []
     for k in range(min(args.plain_shots, nshot)):
         bs = make_batch(vp, rho, vp0, rho0, [shots[k]])
         if k == 0:
-            bs.gradient(want_traces=False)
+            bs.gradient(want_traces=False, mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_FULL, store=OWL_STORE_F32)
+            bs.gradient(want_traces=False, mode=OWL_GRAD_OWL)
         sync()
         t0 = time.perf_counter()
-        bs.gradient(want_traces=False)
+        bs.gradient(want_traces=False, mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_FULL, store=OWL_STORE_F32)
         sync()
         plain.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        bs.gradient(want_traces=False, mode=OWL_GRAD_OWL)
+        sync()
+        plain_owl.append(time.perf_counter() - t0)
         bs.close()
     plain_s = float(np.mean(plain))
+    plain_owl_s = float(np.mean(plain_owl))
+
+    three_d = None
+    if not args.no_3d:
+        log("3D batched forward")
+        three_d = bench_3d(sync)
 
     cpu = None
     if not args.no_cpu:
-        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        log("CPU baseline")
         cpu = cpu_baseline(vp, rho, vp0, rho0, shots[len(shots) // 2], nt)
 
+    rel16 = float(np.linalg.norm(g16 - g32) / np.linalg.norm(g32))
     out = {
         "metric": "grid-pt updates/s & FWI gradient s/shot, batched+ckpt vs plain, 1 MI355X",
         "value": value,
@@ -176,28 +244,29 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "2D acoustic FWI gradient, OWL algorithm, 32 shots batched",
+        "config": {"workload": "2D acoustic FWI gradient, 32 shots batched, exact adjoint, binomial "
+                               "checkpoints (40 snapshots, FP32)",
                    "grid": "401x401", "dx_m": 10.0, "fd_order": 8, "dt_s": 0.001, "nt": nt,
-                   "pml": 40, "shots": nshot, "receivers": 391, "gradient_mode": "owl",
+                   "pml": 40, "shots": nshot, "receivers": 391, "misfit": "L2",
                    "accumulation": "fp64"},
-        "fwi_gradient_s_per_shot": wall / args.steps / nshot,
-        "plain_per_shot": {"s_per_shot": plain_s, "shots_timed": len(plain)},
-        "speedup_batched_vs_plain": plain_s / (wall / args.steps / nshot),
-        "device_ms_per_step": dev_ms / args.steps,
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic(name),
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch,
-                     "note": "working set of the 32-shot batch (~0.2 GB) is resident in the "
-                             "256 MB Infinity Cache, so HBM fraction can exceed what HBM alone allows"},
-        "kernels": {k: {"ms": round(v["total_ms"] / args.steps, 3), "launches": v["launches"] // args.steps}
-                    for k, v in stats.items() if v["launches"]},
+        "fwi_gradient_s_per_shot": s_shot,
+        "batched_fp16_checkpoints": {"s_per_shot": w16 / nshot,
+                                     "gradient_rel_l2_vs_fp32": rel16},
+        "plain_per_shot": {"s_per_shot": plain_s, "shots_timed": len(plain),
+                           "path": "one shot per launch, exact adjoint, full FP32 storage"},
+        "speedup_batched_ckpt_vs_plain": plain_s / s_shot,
+        "owl_algorithm": {"batched_s_per_shot": wowl / nshot, "plain_s_per_shot": plain_owl_s,
+                          "speedup": plain_owl_s / (wowl / nshot)},
+        "autograd_s_per_shot": w_autograd / nshot,
+        "roofline": dict(roof, traffic=None),
+        "kernels": kernel_table(stats, args.steps),
+        "acoustic3d_forward": three_d,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1586,54 +1586,89 @@ void binomial_table(owl_acoustic2d* b, int N, int S) {
 
 struct RevCtx {
     bool first = true;           // still on the first forward descent (recording)
+    bool direct = false;         // FP32 snapshots: steps read and write the slots directly
     std::function<void()> on_first_adjoint;
 };
 
-template <int R>
-void advance(owl_acoustic2d* b, int from, int to, bool record) {
-    const W7 w = work1(b);
-    for (int t = from + 1; t <= to; ++t) fwd_step<R>(b, w, w, t, record);
+// Where a forward state is: kW1 = the working set w1, kZero = the zero
+// initial state (materialised by clearing w1), >= 0 = snapshot slot.
+constexpr int kW1 = -2, kZero = -1;
+
+W7 slot_set(owl_acoustic2d* b, int slot) {
+    W7 w;
+    for (int k = 0; k < 7; ++k) w.f[k] = const_cast<float*>(slot_ptr(b, b->snaps, slot, k));
+    return w;
 }
 
-template <int R>
-void restore_top(owl_acoustic2d* b, int slot) {
-    W7 w = work1(b);
-    if (slot < 0) zero_set(b, w);
-    else slot_load(b, b->snaps, slot, w.f);
+// The state at `loc` as a readable set (clearing w1 for the zero state).
+W7 state_at(owl_acoustic2d* b, int loc) {
+    if (loc >= 0) return slot_set(b, loc);
+    const W7 w1 = work1(b);
+    if (loc == kZero) zero_set(b, w1);
+    return w1;
 }
 
-// Reverse sweep over steps b..a+1 with the working set holding u_a (stored in
-// `slot`, or the zero initial state when slot < 0) and sfree free snapshots.
+// Advance the state u_from (at `loc`) to u_to.  In direct mode the first
+// step reads the snapshot in place and, when `into` >= 0, the last step
+// writes snapshot `into` directly; otherwise the state is restored into w1
+// and saved by copy.  Returns where u_to is.
 template <int R>
-void rev(owl_acoustic2d* bt, int a, int b, int sfree, int slot, RevCtx& ctx) {
-    const W7 w1 = work1(bt), w2 = work2(bt);
-    auto last_step = [&](int t) {
-        fwd_step<R>(bt, w1, w2, t, ctx.first);
+int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record, bool direct) {
+    const W7 w1 = work1(b);
+    if (!direct) {
+        if (loc >= 0) slot_load(b, b->snaps, loc, w1.f);
+        else if (loc == kZero) zero_set(b, w1);
+        for (int t = from + 1; t <= to; ++t) fwd_step<R>(b, w1, w1, t, record);
+        if (into >= 0) slot_save(b, b->snaps, into, w1.f);
+        return kW1;
+    }
+    if (to == from) {
+        if (into >= 0 && into != loc) {
+            const W7 src = state_at(b, loc);
+            slot_save(b, b->snaps, into, src.f);
+            return into;
+        }
+        return loc;
+    }
+    W7 cur = state_at(b, loc);
+    for (int t = from + 1; t <= to; ++t) {
+        const W7 out = (t == to && into >= 0) ? slot_set(b, into) : w1;
+        fwd_step<R>(b, cur, out, t, record);
+        cur = out;
+    }
+    return into >= 0 ? into : kW1;
+}
+
+// Reverse sweep over steps b..a+1: u_a is stored at `slot` (snapshot, or
+// kZero) and currently at `loc`; sfree free snapshots.
+template <int R>
+void rev(owl_acoustic2d* bt, int a, int b, int sfree, int slot, int loc, RevCtx& ctx) {
+    const W7 w2 = work2(bt);
+    auto last_step = [&](int t, int at) {
+        const W7 prev = state_at(bt, at);
+        fwd_step<R>(bt, prev, w2, t, ctx.first);
         if (ctx.first) {
             ctx.first = false;
             ctx.on_first_adjoint();
         }
-        adjoint_step<R>(bt, t, w2.f, w1.f);
+        adjoint_step<R>(bt, t, w2.f, prev.f);
     };
     if (b == a + 1) {
-        last_step(b);
+        last_step(b, ctx.direct ? loc : advance<R>(bt, a, a, loc, -1, ctx.first, false));
         return;
     }
     if (sfree == 0) {
         for (int t = b; t >= a + 1; --t) {
-            if (t != b) restore_top<R>(bt, slot);
-            advance<R>(bt, a, t - 1, ctx.first);
-            last_step(t);
+            const int at = advance<R>(bt, a, t - 1, t == b ? loc : slot, -1, ctx.first, ctx.direct);
+            last_step(t, at);
         }
         return;
     }
     const int j = bt->binom_split[sfree][b - a];
-    advance<R>(bt, a, a + j, ctx.first);
     const int mine = bt->snaps.nslot - sfree;
-    slot_save(bt, bt->snaps, mine, w1.f);
-    rev<R>(bt, a + j, b, sfree - 1, mine, ctx);
-    restore_top<R>(bt, slot);
-    rev<R>(bt, a, a + j, sfree, slot, ctx);
+    const int at = advance<R>(bt, a, a + j, loc, mine, ctx.first, ctx.direct);
+    rev<R>(bt, a + j, b, sfree - 1, mine, at, ctx);
+    rev<R>(bt, a, a + j, sfree, slot, slot, ctx);
 }
 
 // Exact-adjoint gradient over all shots.  residual(): fills d_adj once all
@@ -1662,10 +1697,24 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     if (ckpt == OWL_CKPT_FULL) {
         if (do_fwd) {
             slots_alloc(b, b->full, N + 1, 3, fmt);
-            slot_save(b, b->full, 0, w1.f);
-            for (int t = 1; t <= N; ++t) {
-                fwd_step<R>(b, w1, w1, t, true);
-                slot_save(b, b->full, t, w1.f);
+            if (fmt == OWL_STORE_F32) {
+                // each step reads slot t-1 and writes slot t directly (memory
+                // variables in place in w1); the slots' outer halo stays zero
+                OWL_HIP(hipMemsetAsync(b->full.f32.get(), 0, size_t(N + 1) * 3 * b->field_elems * sizeof(float), st));
+                for (int t = 1; t <= N; ++t) {
+                    W7 in = w1, out = w1;
+                    for (int k = 0; k < 3; ++k) {
+                        in.f[k] = const_cast<float*>(slot_ptr(b, b->full, t - 1, k));
+                        out.f[k] = const_cast<float*>(slot_ptr(b, b->full, t, k));
+                    }
+                    fwd_step<R>(b, in, out, t, true);
+                }
+            } else {
+                slot_save(b, b->full, 0, w1.f);
+                for (int t = 1; t <= N; ++t) {
+                    fwd_step<R>(b, w1, w1, t, true);
+                    slot_save(b, b->full, t, w1.f);
+                }
             }
             fwd_steps += N;
         }
@@ -1735,13 +1784,18 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
         if (phase == kForwardOnly) {
             // the schedule starts from u_0 in the reverse call; here only the
             // traces are modelled
-            advance<R>(b, 0, N, true);
+            advance<R>(b, 0, N, kW1, -1, true, false);
             goto done;
         }
         binomial_table(b, N, S);
         slots_alloc(b, b->snaps, std::max(S, 1), 7, fmt);
         RevCtx ctx;
         ctx.first = do_fwd;
+        ctx.direct = fmt == OWL_STORE_F32;
+        // steps write FP32 snapshots over the update domain only: their outer
+        // halo must read as zero
+        if (ctx.direct)
+            OWL_HIP(hipMemsetAsync(b->snaps.f32.get(), 0, size_t(std::max(S, 1)) * 7 * b->field_elems * sizeof(float), st));
         ctx.on_first_adjoint = [&] {
             residual();
             adjoint_alloc_zero(b);
@@ -1752,7 +1806,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
             zero_set(b, w2);   // out-of-place target: its halo must read as zero
             ctx.on_first_adjoint();
         }
-        rev<R>(b, 0, N, S, -1, ctx);
+        rev<R>(b, 0, N, S, kZero, kW1, ctx);
         adj_steps += N;
         fwd_steps += 0;  // counted by the schedule; reported through the updates figure below
     } else {
