@@ -261,7 +261,8 @@ owl_status owl_acoustic3d_gradient(owl_acoustic3d *b, const float *obs, double *
  * (0: x-y planes staged in LDS; 1, 2, 3: barrier-free column segments with
  * 4, 8, 16 rows per block, 5-7 the same with an XCD-contiguous tile order;
  * 8: four-wide columns (float4 per thread), 24: the same with the
- * XCD-contiguous order). */
+ * XCD-contiguous order; 9, 10, 11: software-pipelined columns with 4, 8, 16
+ * rows per block, 13-15 the same with the XCD-contiguous order). */
 owl_status owl_acoustic3d_set_launch(owl_acoustic3d *b, int planes, int variant);
 /* Launch-shape autotuner (no OWL counterpart): times `steps` sweeps of the
  * batch under each candidate (variant, planes) and keeps the fastest;
@@ -275,6 +276,33 @@ owl_status owl_acoustic3d_set_profiling(owl_acoustic3d *b, int on);
 owl_status owl_acoustic3d_kernel_stats(const owl_acoustic3d *b, int kind, double *total_ms,
                                        long long *launches, double *bytes_total);
 owl_status owl_acoustic3d_last_stats(const owl_acoustic3d *b, double *ms, double *updates);
+
+/* ---- 3D isotropic elastic: batched shots on one GPU -----------------------
+ * OWL's elastic_vhtiort_3d forward solver for anisotropy_type = 'iso'
+ * (compute_forward, elastic_vhtiort_3d/submodule_forward_modeling.f90:31-140;
+ * update_wavefield module_wavefield.f90:52-545, free surface :549-1090,
+ * add_source :1095-1200; CFS-MPML module_cfspml.f90:428-625) for
+ * which_medium = 'elastic-iso' in src/forward/3d/module_modeling.f90.
+ * vp, vs, rho: (nz, ny, nx) z fastest; cfg as for the acoustic solvers
+ * (free_surface: flat free surface at z index 1 on a uniform mesh).  Force
+ * sources (polar, azimuth).  Traces: shot after shot, each three blocks
+ * (vx, vy, vz) of [nr][nt]. */
+typedef struct owl_elastic3d owl_elastic3d;
+owl_status owl_elastic3d_create(const owl_acoustic_config *cfg, const float *vp, const float *vs,
+                                const float *rho, int nshot, const owl_shot *shots,
+                                owl_elastic3d **out);
+void owl_elastic3d_destroy(owl_elastic3d *b);
+int owl_elastic3d_nt(const owl_elastic3d *b);
+owl_status owl_elastic3d_forward(owl_elastic3d *b, float *traces);
+/* Launch shape (no OWL counterpart; results do not depend on it): z planes
+ * per block and thread rows per block (1, 2, 4, 8 or 16). */
+owl_status owl_elastic3d_set_launch(owl_elastic3d *b, int planes, int rows);
+/* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
+ * sweep, 2 sources and receivers. */
+owl_status owl_elastic3d_set_profiling(owl_elastic3d *b, int on);
+owl_status owl_elastic3d_kernel_stats(const owl_elastic3d *b, int kind, double *total_ms,
+                                      long long *launches, double *bytes_total);
+owl_status owl_elastic3d_last_stats(const owl_elastic3d *b, double *ms, double *updates);
 
 #ifdef __cplusplus
 }

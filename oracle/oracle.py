@@ -51,6 +51,13 @@ class Receiver3D(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("weight", C.c_float)]
 
 
+class EGrid3D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
+                ("dy", C.c_float), ("dz", C.c_float), ("ox", C.c_float), ("oy", C.c_float),
+                ("oz", C.c_float), ("dt", C.c_float), ("nt", C.c_int), ("pml", C.c_int),
+                ("R", C.c_int), ("free_surface", C.c_int), ("cc_step_interval", C.c_int)]
+
+
 class Point(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float),
                 ("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int),
@@ -103,6 +110,7 @@ class Oracle:
         f("acoustic2d_gradient_owl").restype = C.c_int
         f("acoustic3d_forward").restype = C.c_int
         f("acoustic3d_gradient_owl").restype = C.c_int
+        f("elastic3d_forward").restype = C.c_int
         f("adjoint_source_l2").restype = C.c_double
         f("adjoint_source_l2").argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_float]
@@ -332,3 +340,26 @@ class Oracle:
             raise RuntimeError(f"oracle acoustic3d_gradient_owl failed ({rc})")
         shp = (G.nx, G.ny, G.nz)
         return (gv.reshape(shp).transpose(2, 1, 0).copy(), gr.reshape(shp).transpose(2, 1, 0).copy(), tr)
+
+    # -- 3D elastic (isotropic) ---------------------------------------------
+    def elastic3d_forward(self, g, vp, vs, rho, sources, receivers):
+        """Velocity traces of OWL's elastic_vhtiort_3d compute_forward for one
+        shot window: (tx, ty, tz), each (nr, nt).  Force sources only."""
+        G = EGrid3D(*[getattr(self._grid3(g), f) for f, _ in Grid3D._fields_])
+        shape = (G.nz, G.ny, G.nx)
+        vp = self._model(vp, shape)
+        vs = self._model(vs, shape)
+        rho = self._model(rho, shape)
+        arr, keep, f0max, _ = self._sources3(g, [dict(s, mechanism="force") for s in sources])
+        rec = self._receivers3(receivers)
+        tr = np.zeros((3, len(receivers), G.nt), self.real)
+        self._fn("elastic3d_forward").argtypes = [C.POINTER(EGrid3D), C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_int, C.c_void_p, C.c_float, C.c_int, C.c_void_p,
+                                                  C.c_void_p]
+        rc = self._fn("elastic3d_forward")(C.byref(G), vp.ctypes.data_as(C.c_void_p),
+                                           vs.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p),
+                                           len(sources), C.cast(arr, C.c_void_p), f0max, len(receivers),
+                                           C.cast(rec, C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle elastic3d_forward failed ({rc})")
+        return tr[0], tr[1], tr[2]

@@ -500,6 +500,241 @@ __global__ __launch_bounds__(1024) void k3c_update_v(const Shot3* __restrict__ s
 
 
 // ---------------------------------------------------------------------------
+// Software-pipelined columns: as the barrier-free columns, but every operand
+// of plane k+1 (the x and y stencil rows, the z queue's new plane, the node's
+// own values) is loaded before plane k is computed and stored, so each wave
+// keeps two planes of loads in flight.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(1024) void k3q_update_p(const Shot3* __restrict__ shots,
+                                                    const int* __restrict__ tile_shot, F3 F,
+                                                    const float* __restrict__ bk,
+                                                    const float* __restrict__ prof, Coef c, float dt,
+                                                    float idx, float idy, float idz, int pml, int kc,
+                                                    int ntiles, int remap, int fs) {
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    if (i > nx + pml || j > ny + pml) return;
+    const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
+    const int k0 = fs ? max(kt, 2) : kt;
+    if (k0 > k1) return;
+    const float* __restrict__ vx = F.vx;
+    const float* __restrict__ vy = F.vy;
+    const float* __restrict__ vz = F.vz;
+    const float* __restrict__ pin = F.p;
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* az_i = prof + S.prof + 4 * lx + 4 * ly;
+    const float* bz_i = az_i + lz;
+    const bool xslab = (i <= 0 || i >= nx + 1), yslab = (j <= 0 || j >= ny + 1);
+    const float axv = xslab ? prof[S.prof + i + pml - 1] : 0.0f;
+    const float bxv = xslab ? prof[S.prof + lx + i + pml - 1] : 0.0f;
+    const float ayv = yslab ? prof[S.prof + 4 * lx + j + pml - 1] : 0.0f;
+    const float byv = yslab ? prof[S.prof + 4 * lx + ly + j + pml - 1] : 0.0f;
+    const size_t px = S.px, pxy = S.pxy;
+    size_t q = at3(S, lo, i, j, k0);
+    float qz[2 * R];
+#pragma unroll
+    for (int m = 0; m < 2 * R - 1; ++m) qz[m] = vz[q + (m - R + 1) * pxy];
+    // operands of the current plane: X[m] = vx(i - R + 1 + m), Y[m] = vy(j - R + 1 + m)
+    float X[2 * R], Y[2 * R], Zn, P, K;
+#pragma unroll
+    for (int m = 0; m < 2 * R; ++m) {
+        X[m] = vx[q + m - (R - 1)];
+        Y[m] = vy[q + (m - (R - 1)) * px];
+    }
+    Zn = vz[q + R * pxy];
+    P = pin[q];
+    K = bk[q];
+    for (int k = k0; k <= k1; ++k, q += pxy) {
+        float Xn[2 * R], Yn[2 * R], Zn2 = 0.f, Pn = 0.f, Kn = 0.f;
+        const bool more = k < k1;
+        if (more) {
+            const size_t qn = q + pxy;
+#pragma unroll
+            for (int m = 0; m < 2 * R; ++m) {
+                Xn[m] = vx[qn + m - (R - 1)];
+                Yn[m] = vy[qn + (m - (R - 1)) * px];
+            }
+            Zn2 = vz[qn + R * pxy];
+            Pn = pin[qn];
+            Kn = bk[qn];
+        }
+        qz[2 * R - 1] = Zn;
+        float sx = c.c[0] * (X[R] - X[R - 1]);
+        float sy = c.c[0] * (Y[R] - Y[R - 1]);
+        float sz = c.c[0] * (qz[R] - qz[R - 1]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) {
+            sx = sx + c.c[m - 1] * (X[R - 1 + m] - X[R - m]);
+            sy = sy + c.c[m - 1] * (Y[R - 1 + m] - Y[R - m]);
+            sz = sz + c.c[m - 1] * (qz[R - 1 + m] - qz[R - m]);
+        }
+        float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+        if (xslab) {
+            const float mem = axv * dxv + bxv * F.m[0][q];
+            F.m[0][q] = mem;
+            dxv = dxv + mem;
+        }
+        if (yslab) {
+            const float mem = ayv * dyv + byv * F.m[1][q];
+            F.m[1][q] = mem;
+            dyv = dyv + mem;
+        }
+        if (zslab_p(k, nz, fs)) {
+            const float mem = az_i[k + pml - 1] * dzv + bz_i[k + pml - 1] * F.m[2][q];
+            F.m[2][q] = mem;
+            dzv = dzv + mem;
+        }
+        F.po[q] = P - dt * K * (dxv + dyv + dzv);
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) qz[m] = qz[m + 1];
+        if (more) {
+#pragma unroll
+            for (int m = 0; m < 2 * R; ++m) {
+                X[m] = Xn[m];
+                Y[m] = Yn[m];
+            }
+            Zn = Zn2;
+            P = Pn;
+            K = Kn;
+        }
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(1024) void k3q_update_v(const Shot3* __restrict__ shots,
+                                                    const int* __restrict__ tile_shot, F3 F,
+                                                    const float* __restrict__ rho,
+                                                    const float* __restrict__ prof, Coef c, float dt,
+                                                    float idx, float idy, float idz, int pml, int kc,
+                                                    int ntiles, int remap, int fs) {
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;
+    const Shot3 S = shots[tile_shot[tile]];
+    int lt = tile - S.tile0;
+    const int tx = lt % S.ntx;
+    lt /= S.ntx;
+    const int ty = lt % S.nty, tzc = lt / S.nty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    if (i > nx + pml || j > ny + pml) return;
+    const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
+    const int k0 = fs ? max(kt, 1) : kt;
+    if (k0 > k1) return;
+    const float* __restrict__ pn = F.p;
+    const float* __restrict__ vx = F.vx;
+    const float* __restrict__ vy = F.vy;
+    const float* __restrict__ vz = F.vz;
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* az_h = prof + S.prof + 4 * lx + 4 * ly + 2 * lz;
+    const float* bz_h = az_h + lz;
+    const bool xslab = (i <= 1 || i >= nx + 1), yslab = (j <= 1 || j >= ny + 1);
+    const float axv = xslab ? prof[S.prof + 2 * lx + i + pml - 1] : 0.0f;
+    const float bxv = xslab ? prof[S.prof + 3 * lx + i + pml - 1] : 0.0f;
+    const float ayv = yslab ? prof[S.prof + 4 * lx + 2 * ly + j + pml - 1] : 0.0f;
+    const float byv = yslab ? prof[S.prof + 4 * lx + 3 * ly + j + pml - 1] : 0.0f;
+    const size_t px = S.px, pxy = S.pxy;
+    size_t q = at3(S, lo, i, j, k0);
+    float qz[2 * R];
+#pragma unroll
+    for (int m = 0; m < 2 * R - 1; ++m) qz[m] = pn[q + (m - R) * pxy];
+    float rprev = rho[q - pxy];
+    // X[m] = p(i - R + m), Y[m] = p(j - R + m); m = 0 .. 2R-1
+    float X[2 * R], Y[2 * R], Zn, r0, rl, rym, ux, uy, uz;
+#pragma unroll
+    for (int m = 0; m < 2 * R; ++m) {
+        X[m] = pn[q + m - R];
+        Y[m] = pn[q + (m - R) * px];
+    }
+    Zn = pn[q + (R - 1) * pxy];
+    r0 = rho[q];
+    rl = rho[q - 1];
+    rym = rho[q - px];
+    ux = vx[q];
+    uy = vy[q];
+    uz = vz[q];
+    for (int k = k0; k <= k1; ++k, q += pxy) {
+        float Xn[2 * R], Yn[2 * R], Zn2 = 0.f, r0n = 0.f, rln = 0.f, rymn = 0.f, uxn = 0.f, uyn = 0.f, uzn = 0.f;
+        const bool more = k < k1;
+        if (more) {
+            const size_t qn = q + pxy;
+#pragma unroll
+            for (int m = 0; m < 2 * R; ++m) {
+                Xn[m] = pn[qn + m - R];
+                Yn[m] = pn[qn + (m - R) * px];
+            }
+            Zn2 = pn[qn + (R - 1) * pxy];
+            r0n = rho[qn];
+            rln = rho[qn - 1];
+            rymn = rho[qn - px];
+            uxn = vx[qn];
+            uyn = vy[qn];
+            uzn = vz[qn];
+        }
+        qz[2 * R - 1] = Zn;
+        float sx = c.c[0] * (X[R] - X[R - 1]);
+        float sy = c.c[0] * (Y[R] - Y[R - 1]);
+        float sz = c.c[0] * (qz[R] - qz[R - 1]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) {
+            sx = sx + c.c[m - 1] * (X[R + m - 1] - X[R - m]);
+            sy = sy + c.c[m - 1] * (Y[R + m - 1] - Y[R - m]);
+            sz = sz + c.c[m - 1] * (qz[R - 1 + m] - qz[R - m]);
+        }
+        float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+        const float rx = 0.5f * (r0 + rl);
+        const float ry = 0.5f * (r0 + rym);
+        const float rz = 0.5f * (r0 + rprev);
+        rprev = r0;
+        if (xslab) {
+            const float mem = axv * dxv + bxv * F.m[3][q];
+            F.m[3][q] = mem;
+            dxv = dxv + mem;
+        }
+        if (yslab) {
+            const float mem = ayv * dyv + byv * F.m[4][q];
+            F.m[4][q] = mem;
+            dyv = dyv + mem;
+        }
+        if (zslab_v(k, nz, fs)) {
+            const float mem = az_h[k + pml - 1] * dzv + bz_h[k + pml - 1] * F.m[5][q];
+            F.m[5][q] = mem;
+            dzv = dzv + mem;
+        }
+        F.vxo[q] = ux - dt / rx * dxv;
+        F.vyo[q] = uy - dt / ry * dyv;
+        if (!fs || k >= 2) F.vzo[q] = uz - dt / rz * dzv;
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; ++m) qz[m] = qz[m + 1];
+        if (more) {
+#pragma unroll
+            for (int m = 0; m < 2 * R; ++m) {
+                X[m] = Xn[m];
+                Y[m] = Yn[m];
+            }
+            Zn = Zn2;
+            r0 = r0n;
+            rl = rln;
+            rym = rymn;
+            ux = uxn;
+            uy = uyn;
+            uz = uzn;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Four-wide column variant: each thread updates 4 consecutive x nodes with
 // 16-byte loads and stores (x neighbours from A/4*2+1 aligned float4s, y
 // neighbours one float4 per row, z neighbours from a float4 register
@@ -1054,7 +1289,7 @@ using namespace owl::a3;
 struct owl_acoustic3d {
     owl_acoustic_config cfg{};
     int R = 4, pml = 15, nt = 0, kc = 64;
-    int variant = 1;   // 0: LDS-staged planes with barriers, 1: barrier-free columns, 2: four-wide columns
+    int variant = 1;   // 0: LDS planes with barriers, 1: barrier-free columns, 2: four-wide, 3: pipelined columns
     int cy = 4;        // rows per block of the column variant
     int remap = 0;     // XCD-contiguous tile order
     int launch_code = 1;  // the public variant code of the current shape
@@ -1130,7 +1365,7 @@ int csr_by_cell(std::vector<Tap3>& st, std::vector<int>& beg, std::vector<Tap3>&
 void retile(owl_acoustic3d* b) {
     const int pml = b->pml;
     b->tile_shot.clear();
-    const int ty_rows = b->variant == 0 ? kTY : (b->variant == 1 ? b->cy : kWY);
+    const int ty_rows = b->variant == 0 ? kTY : (b->variant == 2 ? kWY : b->cy);
     const int tx_cols = b->variant == 2 ? 4 * kWX : kBX;
     for (size_t s = 0; s < b->sd.size(); ++s) {
         Shot3& S = b->sd[s];
@@ -1440,6 +1675,10 @@ void sweeps3(owl_acoustic3d* b, const F3& in_place_or_pair, float dt) {
         hipLaunchKernelGGL(k3w_update_p<R>, cgrd, dim3(kWX, kWY), 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
                            b->d_bk.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, ntiles,
                            b->remap, b->fs);
+    else if (b->variant == 3)
+        hipLaunchKernelGGL(k3q_update_p<R>, cgrd, cblk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F,
+                           b->d_bk.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, ntiles,
+                           b->remap, b->fs);
     else if (b->variant == 0)
         hipLaunchKernelGGL(k3_update_p<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, b->d_bk.get(),
                            b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
@@ -1458,6 +1697,10 @@ void sweeps3(owl_acoustic3d* b, const F3& in_place_or_pair, float dt) {
     ev = b->prof.begin(st, 1, bv);
     if (b->variant == 2)
         hipLaunchKernelGGL(k3w_update_v<R>, cgrd, dim3(kWX, kWY), 0, st, b->d_shots.get(), b->d_tile_shot.get(), V,
+                           b->d_rho.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, ntiles,
+                           b->remap, b->fs);
+    else if (b->variant == 3)
+        hipLaunchKernelGGL(k3q_update_v<R>, cgrd, cblk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), V,
                            b->d_rho.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, ntiles,
                            b->remap, b->fs);
     else if (b->variant == 0)
@@ -1665,7 +1908,7 @@ void set_shape(owl_acoustic3d* b, int planes, int variant);
 // does not depend on the field values.
 template <int R>
 void autotune3(owl_acoustic3d* b, int steps) {
-    static const int kVariants[] = {1, 2, 3, 5, 6, 8, 24, 0};
+    static const int kVariants[] = {1, 2, 3, 5, 6, 8, 24, 9, 10, 11, 13, 0};
     static const int kPlanes[] = {32, 64, 128};
     hipStream_t st = b->stream;
     for (auto& x : b->f) x.zero(st);
@@ -1740,14 +1983,19 @@ int owl_acoustic3d_nt(const owl_acoustic3d* b) { return b ? b->nt : 0; }
 
 namespace {
 // Launch shape from the public variant code: 0 LDS planes; 1, 2, 3 columns
-// with 4, 8, 16 rows per block; 8 four-wide columns; +4 (or +16 for 8):
-// XCD-contiguous tile order.
+// with 4, 8, 16 rows per block; 8 four-wide columns; 9, 10, 11 pipelined
+// columns with 4, 8, 16 rows; +4 (or +16 for 8): XCD-contiguous tile order.
 void set_shape(owl_acoustic3d* b, int planes, int variant) {
     if (planes < 1 || variant < 0) fail(OWL_ERR_ARG, "bad launch shape");
     int kind, cy = b->cy, rm;
     if (variant == 8 || variant == 24) {
         kind = 2;
         rm = variant == 24;
+    } else if (variant >= 9 && variant <= 15 && variant != 12) {
+        const int v = variant & 3;
+        kind = 3;
+        cy = v == 1 ? 4 : (v == 2 ? 8 : 16);
+        rm = (variant >> 2) & 1;
     } else if (variant <= 7) {
         const int v = variant & 3;
         kind = v == 0 ? 0 : 1;

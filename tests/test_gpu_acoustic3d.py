@@ -60,7 +60,7 @@ def test_forward3d_matches_oracle(oracle32, gpu, R):
     print(f"case: 3D forward R={R}", flush=True)
     b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=R)
     outs = [b.forward()]
-    for kc, var in ((16, 0), (64, 8), (5, 24)):
+    for kc, var in ((16, 0), (64, 8), (5, 24), (64, 9), (3, 15)):
         b.set_launch(kc, var)
         outs.append(b.forward())
     for o in outs[1:]:
@@ -83,7 +83,7 @@ def test_forward3d_zchunk_invariance(gpu):
     print("case: 3D z-chunk invariance", flush=True)
     b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, [(geom, None)], 0.001, 0.1, pml=8)
     ref = b.forward()[0]
-    for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (1, 8), (13, 24), (64, 8), (64, 1)):
+    for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (1, 8), (13, 24), (64, 8), (1, 9), (13, 10), (64, 13), (64, 1)):
         b.set_launch(kc, var)
         assert np.array_equal(b.forward()[0], ref), (kc, var)
     kc, var = b.autotune(1)
@@ -91,7 +91,7 @@ def test_forward3d_zchunk_invariance(gpu):
     b.close()
 
 
-@pytest.mark.parametrize("variant", [1, 8, 0])
+@pytest.mark.parametrize("variant", [1, 8, 0, 9])
 def test_forward3d_free_surface_and_force(oracle32, gpu, variant):
     """Free surface (p from k = 2, mirror, no top PML) with an explosion and
     an oblique force source near the surface; bit-exact per launch variant."""

@@ -1,0 +1,115 @@
+"""Pin the isotropic elastic oracle (oracle/owl_oracle_elastic.c, OWL's
+elastic_vhtiort_3d restated) against facts that do not come from it:
+
+* the closed-form elastic Green's function of a point force in a
+  homogeneous whole space (Stokes' solution: P, S and near-field terms;
+  Aki & Richards eq. 4.23), for all three velocity components at four
+  geometries, with the injection volume dx dy dz and a half-sample shift
+  of the staggered velocity grid;
+* the Rayleigh-wave speed along a free surface (root of the Rayleigh
+  equation), from the cross-correlation moveout of vertical velocities at
+  the surface: the free-surface mirrors and the traction condition set it;
+* the float32 and float64 builds agree on a heterogeneous model with a free
+  surface and an oblique force.
+CPU only.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import brentq
+
+from helpers import rel_l2
+
+
+def _ricker_spectrum(dt, f0, npad):
+    nstf = int(round(2.0 / f0 / dt))
+    stf = np.zeros(npad)
+    for i in range(min(nstf + 1, npad)):
+        a = (np.pi * f0 * (i * dt - 1.0 / f0)) ** 2
+        stf[i] = (1 - 2 * a) * np.exp(-a)
+    stf /= np.max(np.abs(stf))
+    return np.fft.rfft(stf) * dt, 2 * np.pi * np.fft.rfftfreq(npad, dt)
+
+
+def _stokes_velocity(xr, j, amp, vol, vp, vs, rho, nt, dt, f0, shift):
+    """Velocity (3 components) at offset xr for a force amp*stf(t) in
+    direction j: v = d/dt of u_i = G_ij * F, with
+    G_ij = (3 g_i g_j - d_ij)/(4 pi rho r^3) int_{r/a}^{r/b} tau F(t - tau) dtau
+         + g_i g_j/(4 pi rho a^2 r) F(t - r/a) - (g_i g_j - d_ij)/(4 pi rho b^2 r) F(t - r/b)."""
+    npad = 8 * nt
+    S, w = _ricker_spectrum(dt, f0, npad)
+    r = float(np.linalg.norm(xr))
+    gam = np.asarray(xr, float) / r
+    ta, tb = r / vp, r / vs
+    with np.errstate(divide="ignore", invalid="ignore"):
+        integral = (np.exp(-1j * w * tb) * (1 + 1j * w * tb) - np.exp(-1j * w * ta) * (1 + 1j * w * ta)) / w ** 2
+    integral[0] = (tb * tb - ta * ta) / 2
+    out = []
+    for i in range(3):
+        dij = 1.0 if i == j else 0.0
+        G = ((3 * gam[i] * gam[j] - dij) / (4 * np.pi * rho * r ** 3)) * integral \
+            + gam[i] * gam[j] / (4 * np.pi * rho * vp ** 2 * r) * np.exp(-1j * w * ta) \
+            - (gam[i] * gam[j] - dij) / (4 * np.pi * rho * vs ** 2 * r) * np.exp(-1j * w * tb)
+        V = (1j * w) * amp * vol * S * G * np.exp(-1j * w * shift)
+        out.append(np.fft.irfft(V, n=npad)[:nt] / dt)
+    return out
+
+
+def test_point_force_matches_stokes_solution(oracle32):
+    n, d, vp, vs, rho, f0, dt = 61, 10.0, 2000.0, 1155.0, 1000.0, 10.0, 1e-3
+    g = dict(nx=n, ny=n, nz=n, dx=d, dy=d, dz=d, dt=dt, nt=400, pml=12, R=4)
+    c = (n // 2) * d
+    geo = [(150.0, 0.0, 100.0), (0.0, 0.0, 200.0), (-120.0, 90.0, -140.0), (170.0, -60.0, 0.0)]
+    recs = [dict(x=c + a, y=c + b, z=c + e) for a, b, e in geo]
+    m = lambda v: np.full((n, n, n), v, np.float32)  # noqa: E731
+    tx, ty, tz = oracle32.elastic3d_forward(g, m(vp), m(vs), m(rho),
+                                            [dict(x=c, y=c, z=c, f0=f0, polar=0.0, azimuth=0.0, amp=1e3)], recs)
+    for k, xr in enumerate(geo):
+        ref = _stokes_velocity(xr, 2, 1e3, d ** 3, vp, vs, rho, g["nt"], dt, f0, shift=0.5e-3)
+        num = np.concatenate([tx[k], ty[k], tz[k]])
+        err = rel_l2(num, np.concatenate(ref))
+        assert err < 0.02, (k, err)
+
+
+def _rayleigh_speed(vp, vs):
+    k = (vs / vp) ** 2
+    x = brentq(lambda x: (2 - x) ** 2 - 4 * np.sqrt(1 - x * k) * np.sqrt(1 - x), 0.5, 0.99)
+    return vs * np.sqrt(x)
+
+
+def test_free_surface_rayleigh_speed(oracle32):
+    """Vertical force at the surface (moved to z = dz as OWL does); vz at the
+    surface 800, 1200 and 1600 m away; moveout speed from the
+    cross-correlation peak within 1 % of the Rayleigh speed (8.6 points per
+    Rayleigh wavelength at the Ricker peak frequency)."""
+    nx, ny, nz, d, vp, vs, rho, f0, dt = 111, 15, 25, 20.0, 2000.0, 1155.0, 1000.0, 2.5, 2e-3
+    g = dict(nx=nx, ny=ny, nz=nz, dx=d, dy=d, dz=d, dt=dt, nt=1100, pml=12, R=4, free_surface=True)
+    m = lambda v: np.full((nz, ny, nx), v, np.float32)  # noqa: E731
+    yc, xs = (ny // 2) * d, 100.0
+    offs = [800.0, 1200.0, 1600.0]
+    _, _, tz = oracle32.elastic3d_forward(g, m(vp), m(vs), m(rho),
+                                          [dict(x=xs, y=yc, z=0.0, f0=f0, polar=0.0, amp=1e6)],
+                                          [dict(x=xs + o, y=yc, z=0.0) for o in offs])
+    cr = _rayleigh_speed(vp, vs)
+    n, up = tz.shape[1], 16
+    for i in (1, 2):
+        X = np.conj(np.fft.rfft(tz[0], 2 * n)) * np.fft.rfft(tz[i], 2 * n)
+        cc = np.fft.irfft(X, 2 * n * up)
+        j = int(np.argmax(cc))
+        lag = (j if j < n * up else j - 2 * n * up) * dt / up
+        speed = (offs[i] - offs[0]) / lag
+        assert abs(speed / cr - 1) < 0.01, (offs[i], speed, cr)
+
+
+def test_float64_build_agrees(oracle32, oracle64):
+    from owl_amd.synthetic import random_field
+    nz, ny, nx = 20, 18, 22
+    vp = (2500.0 * (1 + 0.05 * random_field((nz, ny, nx), 3.0, 2))).astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = (310.0 * vp ** 0.25).astype(np.float32)
+    g = dict(nx=nx, ny=ny, nz=nz, dx=10.0, dy=10.0, dz=10.0, dt=1e-3, nt=150, pml=8, R=4, free_surface=True)
+    src = [dict(x=103.0, y=88.5, z=31.0, f0=20.0, polar=0.6, azimuth=1.1, amp=1e4)]
+    rec = [dict(x=20.0 + 9 * k, y=60.0, z=5.0 + k) for k in range(20)]
+    a = np.concatenate(oracle32.elastic3d_forward(g, vp, vs, rho, src, rec))
+    b = np.concatenate(oracle64.elastic3d_forward(g, vp, vs, rho, src, rec))
+    assert np.abs(b).max() > 0
+    assert rel_l2(a, b) < 1e-5
