@@ -111,6 +111,7 @@ class Oracle:
         f("acoustic3d_forward").restype = C.c_int
         f("acoustic3d_gradient_owl").restype = C.c_int
         f("elastic3d_forward").restype = C.c_int
+        f("elastic3d_gradient_owl").restype = C.c_int
         f("adjoint_source_l2").restype = C.c_double
         f("adjoint_source_l2").argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_float]
@@ -363,3 +364,33 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle elastic3d_forward failed ({rc})")
         return tr[0], tr[1], tr[2]
+
+    def elastic3d_gradient_owl(self, g, vp, vs, rho, sources, receivers, adj):
+        """OWL's isotropic elastic gradient: (grad_vp, grad_vs, grad_rho) as
+        (nz, ny, nx), plus the forward traces (tx, ty, tz).  adj: (3, nr, nt)."""
+        G = EGrid3D(*[getattr(self._grid3(g), f) for f, _ in Grid3D._fields_])
+        shape = (G.nz, G.ny, G.nx)
+        vp = self._model(vp, shape)
+        vs = self._model(vs, shape)
+        rho = self._model(rho, shape)
+        arr, keep, f0max, hnt = self._sources3(g, [dict(s, mechanism="force") for s in sources])
+        rec = self._receivers3(receivers)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (3, len(receivers), G.nt)
+        n = G.nx * G.ny * G.nz
+        gv, gs, gr = (np.zeros(n, self.real) for _ in range(3))
+        tr = np.zeros((3, len(receivers), G.nt), self.real)
+        self._fn("elastic3d_gradient_owl").argtypes = [
+            C.POINTER(EGrid3D), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_int,
+            C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        rc = self._fn("elastic3d_gradient_owl")(
+            C.byref(G), vp.ctypes.data_as(C.c_void_p), vs.ctypes.data_as(C.c_void_p),
+            rho.ctypes.data_as(C.c_void_p), len(sources), C.cast(arr, C.c_void_p), f0max,
+            int(g.get("hnt", hnt)), len(receivers), C.cast(rec, C.c_void_p), adj.ctypes.data_as(C.c_void_p),
+            gv.ctypes.data_as(C.c_void_p), gs.ctypes.data_as(C.c_void_p), gr.ctypes.data_as(C.c_void_p),
+            tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle elastic3d_gradient_owl failed ({rc})")
+        shp = (G.nx, G.ny, G.nz)
+        out = tuple(a.reshape(shp[::-1][::-1]).reshape(G.nx, G.ny, G.nz).transpose(2, 1, 0).copy() for a in (gv, gs, gr))
+        return out + ((tr[0], tr[1], tr[2]),)

@@ -540,3 +540,309 @@ int ORA_SYM(elastic3d_forward)(const ora_egrid3d *g, const float *vp, const floa
     emed_free(&M);
     return 0;
 }
+
+/* ======================================================================== */
+/* OWL's elastic gradient (isotropic): compute_adjoint + compute_gradient,   */
+/* elastic_vhtiort_3d/submodule_fwi_gradient.f90:31-420, 702-827, energy   */
+/* preconditioning off, medium parameters only.                             */
+/* ======================================================================== */
+
+/* Boundary strips of the three velocities over the six face regions
+ * (elastic_vhtiort_3d/module_boundary_saving.f90:83-236): per velocity the
+ * x faces [2-R, 1] and [nx+1, nx+R], y faces, z faces, each over
+ * [1-R, n+R] on the other two axes; vx's six sections, then vy's, vz's. */
+static size_t estrip_len(const elay *L) {
+    const size_t R = (size_t)L->R, ex = L->nx + 2 * R, ey = L->ny + 2 * R, ez = L->nz + 2 * R;
+    return 3 * 2 * R * (ey * ez + ex * ez + ex * ey);
+}
+
+static void estrips(const elay *L, efld *F, real *rec, int save) {
+    const int R = L->R, nx = L->nx, ny = L->ny, nz = L->nz;
+    size_t k = 0;
+    real *v3[3] = {F->vx, F->vy, F->vz};
+    for (int c = 0; c < 3; ++c) {
+        real *f = v3[c];
+#define XF(i, j, kk)                         \
+    do {                                     \
+        size_t q_ = IE(*L, i, j, kk);        \
+        if (save) rec[k] = f[q_];            \
+        else f[q_] = rec[k];                 \
+        ++k;                                 \
+    } while (0)
+        for (int kk = 1 - R; kk <= nz + R; ++kk)
+            for (int j = 1 - R; j <= ny + R; ++j)
+                for (int i = 2 - R; i <= 1; ++i) XF(i, j, kk);
+        for (int kk = 1 - R; kk <= nz + R; ++kk)
+            for (int j = 1 - R; j <= ny + R; ++j)
+                for (int i = nx + 1; i <= nx + R; ++i) XF(i, j, kk);
+        for (int kk = 1 - R; kk <= nz + R; ++kk)
+            for (int j = 2 - R; j <= 1; ++j)
+                for (int i = 1 - R; i <= nx + R; ++i) XF(i, j, kk);
+        for (int kk = 1 - R; kk <= nz + R; ++kk)
+            for (int j = ny + 1; j <= ny + R; ++j)
+                for (int i = 1 - R; i <= nx + R; ++i) XF(i, j, kk);
+        for (int kk = 2 - R; kk <= 1; ++kk)
+            for (int j = 1 - R; j <= ny + R; ++j)
+                for (int i = 1 - R; i <= nx + R; ++i) XF(i, j, kk);
+        for (int kk = nz + 1; kk <= nz + R; ++kk)
+            for (int j = 1 - R; j <= ny + R; ++j)
+                for (int i = 1 - R; i <= nx + R; ++i) XF(i, j, kk);
+#undef XF
+    }
+}
+
+/* Isotropic compliances at a node (submodule_fwi_gradient.f90:73-95), with
+ * c22 = c33 = c11 and c13 = c23 = c12, in the Fortran's expression order. */
+typedef struct {
+    real s11, s12, s13, s22, s23, s33, s44;
+} ecompl;
+
+static ecompl compliances(real c11, real c12, real c44) {
+    const real c13 = c12, c22 = c11, c23 = c12, c33 = c11;
+    ecompl s;
+    const real den1 = c13 * c13 * c22 - 2 * c12 * c13 * c23 + c11 * (c23 * c23) + c12 * c12 * c33 - c11 * c22 * c33;
+    const real den2 = c13 * c13 * c22 - 2 * c12 * c13 * c23 + c12 * c12 * c33 + c11 * (c23 * c23 - c22 * c33);
+    s.s11 = (c23 * c23 - c22 * c33) / den1;
+    s.s12 = (-c13 * c23 + c12 * c33) / den2;
+    s.s13 = (c13 * c22 - c12 * c23) / den1;
+    s.s22 = (c13 * c13 - c11 * c33) / den1;
+    s.s23 = (-c12 * c13 + c11 * c23) / den2;
+    s.s33 = (c12 * c12 - c11 * c22) / den1;
+    s.s44 = c44 != 0 ? (real)1.0f / c44 : (real)0;
+    return s;
+}
+
+/* adj: three [nr][nt] blocks (x, y, z).  Outputs (nz, ny, nx) z-fast:
+ * grad_vp, grad_vs (chain rule with vp, vs in km/s and rho in g/cm^3,
+ * :486-495) and grad_rho, NaN/Inf mapped to zero. */
+int ORA_SYM(elastic3d_gradient_owl)(const ora_egrid3d *g, const float *vp, const float *vs, const float *rho,
+                                    int ns, const ora_esource3d *src, float f0max, int hnt, int nr,
+                                    const ora_ereceiver3d *rec, const real *adj, real *grad_vp, real *grad_vs,
+                                    real *grad_rho, real *traces_out) {
+    if (g->R < 2 || g->R > 8 || g->pml < g->R || ns < 1) return -1;
+    for (int s = 0; s < ns; ++s)
+        if (src[s].mechanism != 1) return -2;
+    emed M;
+    emed_init(&M, g, vp, vs, rho, f0max);
+    const elay L = M.L;
+    const size_t n = (size_t)L.NX * L.NY * L.NZ, sl = estrip_len(&L);
+    const int nt = g->nt, fs = g->free_surface;
+    real *strips = ezalloc(sl * (size_t)nt);
+    efld F, A, P;
+    efld_alloc(&F, &L);
+    efld_alloc(&A, &L);
+    efld_alloc(&P, &L);
+    ora_point *sp = (ora_point *)calloc((size_t)ns, sizeof(ora_point));
+    ora_point *rp = (ora_point *)calloc(nr > 0 ? nr : 1, sizeof(ora_point));
+    ora_esource3d *s2 = (ora_esource3d *)calloc((size_t)ns, sizeof(ora_esource3d));
+    for (int s = 0; s < ns; ++s) {
+        s2[s] = src[s];
+        if (fs && s2[s].z < g->dz) {
+            s2[s].z = g->dz;
+            s2[s].amp = s2[s].amp * 1.2f;
+        }
+        ORA_SYM(locate_point)(3, s2[s].x, s2[s].y, s2[s].z, g->ox, g->oy, g->oz, g->dx, g->dy, g->dz, &sp[s]);
+    }
+    for (int r = 0; r < nr; ++r)
+        ORA_SYM(locate_point)(3, rec[r].x, rec[r].y, rec[r].z, g->ox, g->oy, g->oz, g->dx, g->dy, g->dz, &rp[r]);
+    real *tr = traces_out ? traces_out : ezalloc(3 * (size_t)nr * nt);
+    memset(tr, 0, sizeof(real) * 3 * (size_t)nr * nt);
+    const real dt = (real)g->dt;
+    /* forward with boundary saving; the final state stays in F */
+    for (int t = 1; t <= nt; ++t) {
+        estrips(&L, &F, strips + (size_t)(t - 1) * sl, 1);
+        if (t == nt) break;   /* final state: the state at the start of step nt */
+        if (fs)
+            for (int kk = 1; kk <= NKW; ++kk)
+                for (int j = L.lo; j < L.lo + L.NY; ++j)
+                    for (int i = L.lo; i < L.lo + L.NX; ++i) {
+                        F.vx[IE(L, i, j, 1 - kk)] = F.vx[IE(L, i, j, 1 + kk)];
+                        F.vy[IE(L, i, j, 1 - kk)] = F.vy[IE(L, i, j, 1 + kk)];
+                        F.vz[IE(L, i, j, 2 - kk)] = F.vz[IE(L, i, j, 1 + kk)];
+                    }
+        for (int r = 0; r < nr; ++r) {
+            if (rec[r].weight == 0.0f) continue;
+            const ora_point *Q = &rp[r];
+            const real w = (real)rec[r].weight;
+            real ax = 0, ay = 0, az = 0;
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx) {
+                        ax = ax + F.vx[IE(L, Q->hx + irx, Q->gy + iry, Q->gz + irz)] * (real)Q->hxw[irx + NKW] *
+                                      (real)Q->iy[iry + NKW] * (real)Q->iz[irz + NKW] * w;
+                    }
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        ay = ay + F.vy[IE(L, Q->gx + irx, Q->hy + iry, Q->gz + irz)] * (real)Q->ix[irx + NKW] *
+                                      (real)Q->hyw[iry + NKW] * (real)Q->iz[irz + NKW] * w;
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        az = az + F.vz[IE(L, Q->gx + irx, Q->gy + iry, Q->hz + irz)] * (real)Q->ix[irx + NKW] *
+                                      (real)Q->iy[iry + NKW] * (real)Q->hzw[irz + NKW] * w;
+            tr[(size_t)r * nt + (t - 1)] = ax;
+            tr[((size_t)nr + r) * nt + (t - 1)] = ay;
+            tr[(2 * (size_t)nr + r) * nt + (t - 1)] = az;
+        }
+        eupdate(&M, &F, dt);
+        eadd_source(&M, &F, s2, sp, ns, t, dt);
+    }
+    /* final state: the nine wavefields at the start of step nt, before the
+     * receivers' mirror (module_boundary_saving.f90:62-70) */
+    real *fin = ezalloc(9 * n);
+    real **fw = &F.sxx;
+    for (int c = 0; c < 9; ++c) memcpy(fin + (size_t)c * n, fw[c], n * sizeof(real));
+    /* the record of step nt (state nt-1, FS-mirrored) */
+    if (fs)
+        for (int kk = 1; kk <= NKW; ++kk)
+            for (int j = L.lo; j < L.lo + L.NY; ++j)
+                for (int i = L.lo; i < L.lo + L.NX; ++i) {
+                    F.vx[IE(L, i, j, 1 - kk)] = F.vx[IE(L, i, j, 1 + kk)];
+                    F.vy[IE(L, i, j, 1 - kk)] = F.vy[IE(L, i, j, 1 + kk)];
+                    F.vz[IE(L, i, j, 2 - kk)] = F.vz[IE(L, i, j, 1 + kk)];
+                }
+    for (int r = 0; r < nr; ++r) {
+        if (rec[r].weight == 0.0f) continue;
+        const ora_point *Q = &rp[r];
+        const real w = (real)rec[r].weight;
+        real ax = 0, ay = 0, az = 0;
+        for (int irz = -NKW; irz <= NKW; ++irz)
+            for (int iry = -NKW; iry <= NKW; ++iry)
+                for (int irx = -NKW; irx <= NKW; ++irx)
+                    ax = ax + F.vx[IE(L, Q->hx + irx, Q->gy + iry, Q->gz + irz)] * (real)Q->hxw[irx + NKW] *
+                                  (real)Q->iy[iry + NKW] * (real)Q->iz[irz + NKW] * w;
+        for (int irz = -NKW; irz <= NKW; ++irz)
+            for (int iry = -NKW; iry <= NKW; ++iry)
+                for (int irx = -NKW; irx <= NKW; ++irx)
+                    ay = ay + F.vy[IE(L, Q->gx + irx, Q->hy + iry, Q->gz + irz)] * (real)Q->ix[irx + NKW] *
+                                  (real)Q->hyw[iry + NKW] * (real)Q->iz[irz + NKW] * w;
+        for (int irz = -NKW; irz <= NKW; ++irz)
+            for (int iry = -NKW; iry <= NKW; ++iry)
+                for (int irx = -NKW; irx <= NKW; ++irx)
+                    az = az + F.vz[IE(L, Q->gx + irx, Q->gy + iry, Q->hz + irz)] * (real)Q->ix[irx + NKW] *
+                                  (real)Q->iy[iry + NKW] * (real)Q->hzw[irz + NKW] * w;
+        tr[(size_t)r * nt + (nt - 1)] = ax;
+        tr[((size_t)nr + r) * nt + (nt - 1)] = ay;
+        tr[(2 * (size_t)nr + r) * nt + (nt - 1)] = az;
+    }
+    efld_free(&F);
+    efld_alloc(&F, &L);
+    fw = &F.sxx;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t ni = (size_t)nx * ny * nz;
+    real *gc[10];
+    for (int c = 0; c < 10; ++c) gc[c] = ezalloc(ni);  /* c11 c12 c13 c22 c23 c33 c44 c55 c66 rho */
+    const real mdt = -dt;
+    const int cc = g->cc_step_interval > 0 ? g->cc_step_interval : 1;
+    const size_t sx = 1, sy = L.sy, sz = L.sz;
+    for (int t = nt; t >= hnt; --t) {
+        real **pw = &P.sxx;
+        for (int c = 0; c < 9; ++c) memcpy(pw[c], fw[c], n * sizeof(real));
+        eupdate(&M, &F, mdt);
+        if (t == nt)
+            for (int c = 0; c < 9; ++c) memcpy(fw[c], fin + (size_t)c * n, n * sizeof(real));
+        estrips(&L, &F, strips + (size_t)(t - 1) * sl, 0);
+        eupdate(&M, &A, mdt);
+        /* adjoint sources into the adjoint velocities (:256-330) */
+        for (int r = 0; r < nr; ++r) {
+            if (rec[r].weight == 0.0f) continue;
+            const ora_point *Q = &rp[r];
+            const real w = (real)rec[r].weight;
+            const real ax = adj[(size_t)r * nt + (t - 1)], ay = adj[((size_t)nr + r) * nt + (t - 1)],
+                       az = adj[(2 * (size_t)nr + r) * nt + (t - 1)];
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || Q->gz + irz >= 2) {
+                            const size_t q = IE(L, Q->hx + irx, Q->gy + iry, Q->gz + irz);
+                            A.vx[q] = A.vx[q] + ax * (real)Q->hxw[irx + NKW] * (real)Q->iy[iry + NKW] *
+                                                    (real)Q->iz[irz + NKW] * w;
+                        }
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || Q->gz + irz >= 2) {
+                            const size_t q = IE(L, Q->gx + irx, Q->hy + iry, Q->gz + irz);
+                            A.vy[q] = A.vy[q] + ay * (real)Q->ix[irx + NKW] * (real)Q->hyw[iry + NKW] *
+                                                    (real)Q->iz[irz + NKW] * w;
+                        }
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || Q->hz + irz >= 2) {
+                            const size_t q = IE(L, Q->gx + irx, Q->gy + iry, Q->hz + irz);
+                            A.vz[q] = A.vz[q] + az * (real)Q->ix[irx + NKW] * (real)Q->iy[iry + NKW] *
+                                                    (real)Q->hzw[irz + NKW] * w;
+                        }
+        }
+        if (t % cc != 0) continue;
+        /* compute_gradient (:702-827) over the interior */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    const size_t q = IE(L, i, j, k);
+                    const size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx + (size_t)(k - 1) * nx * ny;
+                    const ecompl s = compliances(M.c11[q], M.c12[q], M.c44[q]);
+                    const real txx = F.sxx[q] - P.sxx[q], tyy = F.syy[q] - P.syy[q], tzz = F.szz[q] - P.szz[q];
+                    const real tyz = (real)0.25f * (F.syz[q] + F.syz[q + sy] + F.syz[q + sz] + F.syz[q + sy + sz]) -
+                                     (real)0.25f * (P.syz[q] + P.syz[q + sy] + P.syz[q + sz] + P.syz[q + sy + sz]);
+                    const real txz = (real)0.25f * (F.sxz[q] + F.sxz[q + sx] + F.sxz[q + sz] + F.sxz[q + sx + sz]) -
+                                     (real)0.25f * (P.sxz[q] + P.sxz[q + sx] + P.sxz[q + sz] + P.sxz[q + sx + sz]);
+                    const real txy = (real)0.25f * (F.sxy[q] + F.sxy[q + sx] + F.sxy[q + sy] + F.sxy[q + sx + sy]) -
+                                     (real)0.25f * (P.sxy[q] + P.sxy[q + sx] + P.sxy[q + sy] + P.sxy[q + sx + sy]);
+                    const real rxx = A.sxx[q], ryy = A.syy[q], rzz = A.szz[q];
+                    const real ryz = (real)0.25f * (A.syz[q] + A.syz[q + sy] + A.syz[q + sz] + A.syz[q + sy + sz]);
+                    const real rxz = (real)0.25f * (A.sxz[q] + A.sxz[q + sx] + A.sxz[q + sz] + A.sxz[q + sx + sz]);
+                    const real rxy = (real)0.25f * (A.sxy[q] + A.sxy[q + sx] + A.sxy[q + sy] + A.sxy[q + sx + sy]);
+                    const real exx = s.s11 * txx + s.s12 * tyy + s.s13 * tzz;
+                    const real eyy = s.s12 * txx + s.s22 * tyy + s.s23 * tzz;
+                    const real ezz = s.s13 * txx + s.s23 * tyy + s.s33 * tzz;
+                    const real eyz = s.s44 * tyz, exz = s.s44 * txz, exy = s.s44 * txy;
+                    const real fxx = s.s11 * rxx + s.s12 * ryy + s.s13 * rzz;
+                    const real fyy = s.s12 * rxx + s.s22 * ryy + s.s23 * rzz;
+                    const real fzz = s.s13 * rxx + s.s23 * ryy + s.s33 * rzz;
+                    const real fyz = s.s44 * ryz, fxz = s.s44 * rxz, fxy = s.s44 * rxy;
+                    gc[0][o] = gc[0][o] - exx * fxx;
+                    gc[1][o] = gc[1][o] - exx * fyy - eyy * fxx;
+                    gc[2][o] = gc[2][o] - exx * fzz - ezz * fxx;
+                    gc[3][o] = gc[3][o] - eyy * fyy;
+                    gc[4][o] = gc[4][o] - eyy * fzz - ezz * fyy;
+                    gc[5][o] = gc[5][o] - ezz * fzz;
+                    gc[6][o] = gc[6][o] - eyz * fyz;
+                    gc[7][o] = gc[7][o] - exz * fxz;
+                    gc[8][o] = gc[8][o] - exy * fxy;
+                    const real svx = (F.vx[q] + F.vx[q + sx]) - (P.vx[q] + P.vx[q + sx]);
+                    const real rvx = A.vx[q] + A.vx[q + sx];
+                    const real svy = (F.vy[q] + F.vy[q + sy]) - (P.vy[q] + P.vy[q + sy]);
+                    const real rvy = A.vy[q] + A.vy[q + sy];
+                    const real svz = (F.vz[q] + F.vz[q + sz]) - (P.vz[q] + P.vz[q + sz]);
+                    const real rvz = A.vz[q] + A.vz[q + sz];
+                    gc[9][o] = gc[9][o] + svx * rvx + svy * rvy + svz * rvz;
+                }
+    }
+    /* chain rule (:486-495) and output (nz, ny, nx) z-fast */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx + (size_t)(k - 1) * nx * ny;
+                const size_t m = (size_t)(k - 1) + (size_t)(j - 1) * nz + (size_t)(i - 1) * nz * ny;
+                const real r = (real)rho[m] * (real)1.0e-3f, p = (real)vp[m] * (real)1.0e-3f,
+                           v = (real)vs[m] * (real)1.0e-3f;
+                real a = gc[0][o] + gc[1][o] + gc[2][o] + gc[3][o] + gc[4][o] + gc[5][o];
+                a = 2 * r * p * a;
+                real b = gc[6][o] + gc[7][o] + gc[8][o] - 2 * gc[1][o] - 2 * gc[2][o] - 2 * gc[4][o];
+                b = 2 * r * v * b;
+                const real c = gc[9][o];
+                grad_vp[m] = isfinite(a) ? a : 0;
+                grad_vs[m] = isfinite(b) ? b : 0;
+                grad_rho[m] = isfinite(c) ? c : 0;
+            }
+    for (int c = 0; c < 10; ++c) free(gc[c]);
+    if (!traces_out) free(tr);
+    free(fin); free(strips); free(sp); free(rp); free(s2);
+    efld_free(&F); efld_free(&A); efld_free(&P);
+    emed_free(&M);
+    return 0;
+}

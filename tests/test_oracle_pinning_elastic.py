@@ -10,7 +10,12 @@ elastic_vhtiort_3d restated) against facts that do not come from it:
   equation), from the cross-correlation moveout of vertical velocities at
   the surface: the free-surface mirrors and the traction condition set it;
 * the float32 and float64 builds agree on a heterogeneous model with a free
-  surface and an oblique force.
+  surface and an oblique force;
+* OWL's elastic gradient: exact x <-> y symmetry for a symmetric problem,
+  and dot-product consistency of the vp gradient with finite-difference
+  Born data for probes near a reference perturbation (OWL's own test
+  recipe, which compares ratios rather than an absolute adjoint identity
+  because OWL's gradient is not the exact discrete adjoint).
 CPU only.
 """
 import numpy as np
@@ -113,3 +118,66 @@ def test_float64_build_agrees(oracle32, oracle64):
     b = np.concatenate(oracle64.elastic3d_forward(g, vp, vs, rho, src, rec))
     assert np.abs(b).max() > 0
     assert rel_l2(a, b) < 1e-5
+
+
+# ---- OWL's elastic gradient ------------------------------------------------
+def test_owl_elastic_gradient_symmetry(oracle32):
+    """A model and acquisition symmetric under x <-> y give gradients with
+    the same symmetry (to rounding); the traces of the gradient's own
+    forward equal forward()'s."""
+    n, d = 31, 10.0
+    z = np.arange(n)[:, None, None] * d
+    vp = np.broadcast_to(2400.0 + 2.0 * z, (n, n, n)).astype(np.float32)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = np.full((n, n, n), 2000.0, np.float32)
+    g = dict(nx=n, ny=n, nz=n, dx=d, dy=d, dz=d, dt=1e-3, nt=200, pml=8, R=4)
+    c = (n // 2) * d
+    rec = [dict(x=10.0 * i, y=10.0 * j, z=20.0) for j in range(0, n, 3) for i in range(0, n, 3)]
+    src = [dict(x=c, y=c, z=100.0, f0=20.0, polar=0.0, amp=1e6)]
+    obs = np.array(oracle32.elastic3d_forward(g, (vp * 1.02).astype(np.float32), vs, rho, src, rec))
+    syn = np.array(oracle32.elastic3d_forward(g, vp, vs, rho, src, rec))
+    gv, gs, gr, tr = oracle32.elastic3d_gradient_owl(g, vp, vs, rho, src, rec, (syn - obs).astype(np.float32))
+    assert np.array_equal(np.array(tr), syn)
+    for a in (gv, gs, gr):
+        assert np.abs(a).max() > 0
+        assert rel_l2(a.transpose(0, 2, 1), a) < 1e-6
+
+
+def test_owl_elastic_gradient_dot_product_consistency(oracle32):
+    """OWL test 6 recipe (test/test_jacobian_adjoint.py:55-170) on the
+    elastic vp gradient: vertical force at the centre, receivers on two
+    lines above and below it, d_obs = F(m) - dd from a Gaussian vp
+    perturbation beside the source; probes around that perturbation give
+    ratios <J dm, dd> / <dm, g_vp> that agree within 5 % and are positive."""
+    n, D, VP0 = 41, 20.0, 2500.0
+    g = dict(nx=n, ny=n, nz=n, dx=D, dy=D, dz=D, dt=2e-3, nt=300, pml=10, R=4)
+    c = n // 2
+    src = [dict(x=c * D, y=c * D, z=c * D, f0=8.0, polar=0.0, amp=1e9)]
+    rec = [dict(x=D * (4 + 4 * k), y=c * D, z=(c - 8) * D) for k in range(9)] + \
+          [dict(x=D * (4 + 4 * k), y=c * D, z=(c + 8) * D) for k in range(9)]
+    rho = np.full((n, n, n), 2000.0, np.float32)
+    vsm = np.full((n, n, n), VP0 / 1.73, np.float32)
+
+    def gauss(k0, j0, i0, sig, amp):
+        k = np.arange(n)[:, None, None]
+        j = np.arange(n)[None, :, None]
+        i = np.arange(n)[None, None, :]
+        return amp * np.exp(-((k - k0) ** 2 + (j - j0) ** 2 + (i - i0) ** 2) / sig ** 2)
+
+    def fwd(v):
+        return np.array(oracle32.elastic3d_forward(g, v.astype(np.float32), vsm, rho, src, rec), np.float64)
+
+    vp0 = np.full((n, n, n), VP0)
+    eps = 0.02
+    dref = gauss(c, c, c + 6, 3.0, 0.1 * VP0)
+    dd = (fwd(vp0 + eps * dref) - fwd(vp0 - eps * dref)) / (2 * eps)
+    gv, _, _, _ = oracle32.elastic3d_gradient_owl(g, vp0.astype(np.float32), vsm, rho, src, rec,
+                                                  dd.astype(np.float32))
+    ratios = []
+    for off in ((-4, 0, 10), (3, 2, 8), (4, -2, 4)):
+        dm = gauss(c + off[0], c + off[1], c + off[2], 2.5, 0.05 * VP0)
+        born = (fwd(vp0 + eps * dm) - fwd(vp0 - eps * dm)) / (2 * eps)
+        ratios.append(float(np.dot(born.ravel(), dd.ravel()) / np.dot(dm.ravel(), gv.ravel())))
+    ratios = np.array(ratios)
+    assert np.all(ratios > 0), ratios
+    assert np.max(np.abs(ratios - ratios[0])) / abs(ratios[0]) < 0.05, ratios
