@@ -293,12 +293,32 @@ owl_status owl_elastic3d_create(const owl_acoustic_config *cfg, const float *vp,
                                 owl_elastic3d **out);
 void owl_elastic3d_destroy(owl_elastic3d *b);
 int owl_elastic3d_nt(const owl_elastic3d *b);
-owl_status owl_elastic3d_forward(owl_elastic3d *b, float *traces);
+/* save_boundary != 0 keeps the boundary records of every step and the
+ * final state for owl_elastic3d_adjoint (OWL's forward part of
+ * compute_gradient, elastic_vhtiort_3d/module_boundary_saving.f90:62-236). */
+owl_status owl_elastic3d_forward(owl_elastic3d *b, float *traces, int save_boundary);
+/* OWL's reverse pass for the isotropic elastic gradient (compute_adjoint and
+ * compute_gradient, elastic_vhtiort_3d/submodule_fwi_gradient.f90:31-827,
+ * chain rule :486-495): adj in the trace layout (three components) is
+ * injected into the adjoint velocities below the free surface; the
+ * wavefield runs backwards from the final state with the boundary records;
+ * per-shot window gradients grad_vp, grad_vs, grad_rho (nz_s, ny_s, nx_s)
+ * z fastest, shot after shot (the .grd arrays), NaN/Inf -> 0.  Needs a
+ * preceding owl_elastic3d_forward(..., save_boundary = 1). */
+owl_status owl_elastic3d_adjoint(owl_elastic3d *b, const float *adj, float *grad_vp, float *grad_vs,
+                                 float *grad_rho);
+/* Forward with boundary saving, L2 waveform adjoint source per component
+ * (module_inversion_adjoint_source.f90:1020-1125; discard threshold 1e-6 of
+ * each component's largest trace norm), reverse pass, and the sum of the
+ * shots' gradients on the full model (module_gradient.f90:366-388), FP64.
+ * misfit: one value per shot (sum over the three components). */
+owl_status owl_elastic3d_gradient(owl_elastic3d *b, const float *obs, double *misfit, double *grad_vp,
+                                  double *grad_vs, double *grad_rho, float *traces_out);
 /* Launch shape (no OWL counterpart; results do not depend on it): z planes
- * per block and thread rows per block (1, 2, 4, 8 or 16). */
+ * per block and thread rows per block of 64 threads (1, 2 or 4). */
 owl_status owl_elastic3d_set_launch(owl_elastic3d *b, int planes, int rows);
 /* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
- * sweep, 2 sources and receivers. */
+ * sweep, 2 sources and receivers, 3 imaging, 4 boundary records. */
 owl_status owl_elastic3d_set_profiling(owl_elastic3d *b, int on);
 owl_status owl_elastic3d_kernel_stats(const owl_elastic3d *b, int kind, double *total_ms,
                                       long long *launches, double *bytes_total);

@@ -107,7 +107,9 @@ _SIGNATURES = {
     "owl_elastic3d_create": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
     "owl_elastic3d_destroy": (None, [_P]),
     "owl_elastic3d_nt": (C.c_int, [_P]),
-    "owl_elastic3d_forward": (C.c_int, [_P, _P]),
+    "owl_elastic3d_forward": (C.c_int, [_P, _P, C.c_int]),
+    "owl_elastic3d_adjoint": (C.c_int, [_P, _P, _P, _P, _P]),
+    "owl_elastic3d_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "owl_elastic3d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
     "owl_elastic3d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),

@@ -31,7 +31,7 @@ constexpr int kBY = 4;           // thread rows
 constexpr int kRY = 4;           // y rows per thread
 constexpr int kTY = kBY * kRY;   // y extent of a tile
 constexpr int kPitchAlign = 64;
-constexpr int kCY = 16;          // rows per block of the column variant
+
 constexpr int kXcd = 8;          // XCDs of an MI355X (L2 domains)
 
 struct Coef {

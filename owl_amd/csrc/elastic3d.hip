@@ -43,6 +43,9 @@ struct ShotE {
     int nr, rec0;
     int ninj, inj0;
     long long trace0;      // three [nt][nr] blocks (vx, vy, vz)
+    long long strip0;      // boundary records: nt blocks of strip_len
+    long long g0;          // window gradient accumulators (nx*ny*nz, x fastest)
+    int nadj, adj0;        // adjoint-source cells (CSR)
 };
 
 // Wavefields and memory variables (named by derivative and field as in OWL).
@@ -50,6 +53,14 @@ struct EF {
     float *sxx, *syy, *szz, *syz, *sxz, *sxy, *vx, *vy, *vz;
     float *m_dxvx, *m_dyvy, *m_dzvz, *m_dyvz, *m_dzvy, *m_dxvz, *m_dzvx, *m_dxvy, *m_dyvx;
     float *m_dxxx, *m_dyxy, *m_dzxz, *m_dxxy, *m_dyyy, *m_dzyz, *m_dxxz, *m_dyyz, *m_dzzz;
+};
+
+// The nine wavefields a sweep writes.  The stress sweep reads the old
+// stresses and the velocities from EF and writes the new stresses here; the
+// velocity sweep reads the new stresses from here and the old velocities
+// from EF.  In-place stepping passes EF's own pointers.
+struct EV {
+    float *sxx, *syy, *szz, *syz, *sxz, *sxy, *vx, *vy, *vz;
 };
 
 // Medium: c11 (= c22 = c33), c12 (= c13 = c23), the harmonic 4-point shear
@@ -111,7 +122,7 @@ __device__ __forceinline__ Prof prof_of(const float* prof, const ShotE& S, int p
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots, const int* __restrict__ tile_shot,
-                                                 EF F, EM M, const float* __restrict__ prof, Coef c, float dt,
+                                                 EF F, EV O, EM M, const float* __restrict__ prof, Coef c, float dt,
                                                  float idx, float idy, float idz, int pml, int kc, int fs) {
     const int tile = blockIdx.x;
     const ShotE S = shots[tile_shot[tile]];
@@ -178,9 +189,9 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
                     pdzvz = damp(F.m_dzvz, q, pdzvz, P.diz[kk], P.aiz[kk], adt);
                 }
             }
-            F.sxx[q] = F.sxx[q] + dt * (c11 * pdxvx + c12 * pdyvy + c12 * pdzvz);
-            F.syy[q] = F.syy[q] + dt * (c12 * pdxvx + c11 * pdyvy + c12 * pdzvz);
-            F.szz[q] = F.szz[q] + dt * (c12 * pdxvx + c12 * pdyvy + c11 * pdzvz);
+            O.sxx[q] = F.sxx[q] + dt * (c11 * pdxvx + c12 * pdyvy + c12 * pdzvz);
+            O.syy[q] = F.syy[q] + dt * (c12 * pdxvx + c11 * pdyvy + c12 * pdzvz);
+            O.szz[q] = F.szz[q] + dt * (c12 * pdxvx + c12 * pdyvy + c11 * pdzvz);
         }
         // ---- shear stresses with a z derivative: syz, sxz (Z = k)
         if (!fs || k >= 2) {
@@ -201,12 +212,12 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
                 pdyvz = damp(F.m_dyvz, q, pdyvz, P.dhy[jj], P.ahy[jj], adt);
                 pdzvy = damp(F.m_dzvy, q, pdzvy, P.dhz[kk], P.ahz[kk], adt);
             }
-            F.syz[q] = F.syz[q] + dt * M.mu_yz[q] * (pdzvy + pdyvz);
+            O.syz[q] = F.syz[q] + dt * M.mu_yz[q] * (pdzvy + pdyvz);
             if (!Xin || !yin || zpml) {
                 pdxvz = damp(F.m_dxvz, q, pdxvz, P.dhx[ii], P.ahx[ii], adt);
                 pdzvx = damp(F.m_dzvx, q, pdzvx, P.dhz[kk], P.ahz[kk], adt);
             }
-            F.sxz[q] = F.sxz[q] + dt * M.mu_xz[q] * (pdzvx + pdxvz);
+            O.sxz[q] = F.sxz[q] + dt * M.mu_xz[q] * (pdzvx + pdxvz);
         }
         // ---- sxy (X, Y, k)
         {
@@ -222,7 +233,7 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
                 pdxvy = damp(F.m_dxvy, q, pdxvy, P.dhx[ii], P.ahx[ii], adt);
                 pdyvx = damp(F.m_dyvx, q, pdyvx, P.dhy[jj], P.ahy[jj], adt);
             }
-            F.sxy[q] = F.sxy[q] + dt * M.mu_xy[q] * (pdyvx + pdxvy);
+            O.sxy[q] = F.sxy[q] + dt * M.mu_xy[q] * (pdyvx + pdxvy);
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ shots,
-                                                   const int* __restrict__ tile_shot, EF F, EM M,
+                                                   const int* __restrict__ tile_shot, EF F, EV O, EM M,
                                                    const float* __restrict__ prof, Coef c, float dt, float idx,
                                                    float idy, float idz, int pml, int kc, int fs) {
     const int tile = blockIdx.x;
@@ -259,12 +270,12 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
     if (k0 > k1) return;
     const Prof P = prof_of(prof, S, pml);
     const float adt = fabsf(dt);
-    const float* __restrict__ sxx = F.sxx;
-    const float* __restrict__ syy = F.syy;
-    const float* __restrict__ szz = F.szz;
-    const float* __restrict__ syz = F.syz;
-    const float* __restrict__ sxz = F.sxz;
-    const float* __restrict__ sxy = F.sxy;
+    const float* __restrict__ sxx = O.sxx;
+    const float* __restrict__ syy = O.syy;
+    const float* __restrict__ szz = O.szz;
+    const float* __restrict__ syz = O.syz;
+    const float* __restrict__ sxz = O.sxz;
+    const float* __restrict__ sxy = O.sxy;
     const size_t sx = 1, sy = S.px, sz = S.pxy;
     const int ii = i + pml - 1, jj = j + pml - 1;
     size_t q = ate(S, lo, i, j, k0);
@@ -300,7 +311,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                 pdyxy = damp(F.m_dyxy, q, pdyxy, P.diy[jj], P.aiy[jj], adt);
                 pdzxz = damp(F.m_dzxz, q, pdzxz, P.diz[kk], P.aiz[kk], adt);
             }
-            F.vx[q] = F.vx[q] + dt / M.rx[q] * (pdxxx + pdyxy + pdzxz);
+            O.vx[q] = F.vx[q] + dt / M.rx[q] * (pdxxx + pdyxy + pdzxz);
         }
         // ---- vy (i, Y, k)
         {
@@ -319,7 +330,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                 pdyyy = damp(F.m_dyyy, q, pdyyy, P.dhy[jj], P.ahy[jj], adt);
                 pdzyz = damp(F.m_dzyz, q, pdzyz, P.diz[kk], P.aiz[kk], adt);
             }
-            F.vy[q] = F.vy[q] + dt / M.ry[q] * (pdxxy + pdyyy + pdzyz);
+            O.vy[q] = F.vy[q] + dt / M.ry[q] * (pdxxy + pdyyy + pdzyz);
         }
         // ---- vz (i, j, Z = k)
         if (!fs || k >= 2) {
@@ -339,7 +350,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                 pdyyz = damp(F.m_dyyz, q, pdyyz, P.diy[jj], P.aiy[jj], adt);
                 pdzzz = damp(F.m_dzzz, q, pdzzz, P.dhz[kk], P.ahz[kk], adt);
             }
-            F.vz[q] = F.vz[q] + dt / M.rz[q] * (pdxxz + pdyyz + pdzzz);
+            O.vz[q] = F.vz[q] + dt / M.rz[q] * (pdxxz + pdyyz + pdzzz);
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -353,18 +364,19 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
 // Free surface, whole padded x-y planes (module_wavefield.f90:580-590,
 // 823-845; submodule_forward_modeling.f90:64-73).  mode 0: zero the
 // velocities above the surface; 1: mirror the stresses; 2: mirror the
-// velocities for the receivers (NKW planes).
+// velocities for the receivers (NKW planes); 3: zero the velocities above
+// it over the planes modes 0 and 2 touch.
 template <int R>
 __global__ void ke_surface(const ShotE* __restrict__ shots, EF F, int pml, int mode) {
     const ShotE S = shots[blockIdx.y];
     const int lo = 1 - pml - R;
     const long long plane = (long long)S.px * S.nyp;
-    const int nk = mode == 2 ? kNkw : R;
+    const int nk = mode == 2 ? kNkw : (mode == 3 ? (R > kNkw ? R : kNkw) : R);
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < plane; e += (long long)gridDim.x * blockDim.x) {
         const size_t b = size_t(S.off) + size_t(e);
         auto at = [&](int k) { return b + size_t(k - lo) * size_t(S.pxy); };
-        if (mode == 0) {
-            for (int m = 1; m <= R; ++m) {
+        if (mode == 0 || mode == 3) {
+            for (int m = 1; m <= nk; ++m) {
                 F.vx[at(1 - m)] = 0.0f;
                 F.vy[at(1 - m)] = 0.0f;
                 F.vz[at(2 - m)] = 0.0f;
@@ -430,8 +442,270 @@ __global__ __launch_bounds__(256) void ke_record(const ShotE* __restrict__ shots
     }
 }
 
+// ---------------------------------------------------------------------------
+// OWL's elastic gradient (elastic_vhtiort_3d/submodule_fwi_gradient.f90).
+// ---------------------------------------------------------------------------
+
+// Boundary records (module_boundary_saving.f90:83-236): per velocity
+// component the x faces i in [2-R, 1] and [nx+1, nx+R], the y faces, the z
+// faces, each over [1-R, n+R] on the other two axes, first index fastest;
+// vx's six sections, then vy's, then vz's.
+struct EStrip {
+    long long a, b, c;  // section sizes: R*ey*ez, ex*R*ez, ex*ey*R
+    int ex, ey;
+};
+
+__host__ __device__ inline EStrip estrip_geo(int nx, int ny, int nz, int R) {
+    EStrip g;
+    g.ex = nx + 2 * R;
+    g.ey = ny + 2 * R;
+    const int ez = nz + 2 * R;
+    g.a = (long long)R * g.ey * ez;
+    g.b = (long long)g.ex * R * ez;
+    g.c = (long long)g.ex * g.ey * R;
+    return g;
+}
+
+// save != 0: record of step t <- (vx, vy, vz); else the reverse.
+template <int R>
+__global__ void ke_strips(const ShotE* __restrict__ shots, float* vx, float* vy, float* vz, float* strips, int t,
+                          int pml, int save) {
+    const ShotE S = shots[blockIdx.y];
+    const EStrip g = estrip_geo(S.nx, S.ny, S.nz, R);
+    const long long one = 2 * (g.a + g.b + g.c), len = 3 * one;
+    const int lo = 1 - pml - R, nx = S.nx, ny = S.ny, nz = S.nz;
+    float* rec = strips + S.strip0 + (long long)(t - 1) * len;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int comp = int(e / one);
+        long long r = e - comp * one;
+        int i, j, k;
+        if (r < 2 * g.a) {
+            const bool hi = r >= g.a;
+            if (hi) r -= g.a;
+            i = (hi ? nx + 1 : 2 - R) + int(r % R);
+            j = 1 - R + int((r / R) % g.ey);
+            k = 1 - R + int(r / (R * (long long)g.ey));
+        } else if ((r -= 2 * g.a) < 2 * g.b) {
+            const bool hi = r >= g.b;
+            if (hi) r -= g.b;
+            i = 1 - R + int(r % g.ex);
+            j = (hi ? ny + 1 : 2 - R) + int((r / g.ex) % R);
+            k = 1 - R + int(r / (g.ex * (long long)R));
+        } else {
+            r -= 2 * g.b;
+            const bool hi = r >= g.c;
+            if (hi) r -= g.c;
+            i = 1 - R + int(r % g.ex);
+            j = 1 - R + int((r / g.ex) % g.ey);
+            k = (hi ? nz + 1 : 2 - R) + int(r / (g.ex * (long long)g.ey));
+        }
+        float* f = comp == 0 ? vx : (comp == 1 ? vy : vz);
+        const size_t q = ate(S, lo, i, j, k);
+        if (save)
+            rec[e] = f[q];
+        else
+            f[q] = rec[e];
+    }
+}
+
+// Adjoint sources into the adjoint velocities (submodule_fwi_gradient.f90:
+// 256-330): cells grouped by (component, cell), contributions of one cell
+// in receiver order; adj holds three [nt][nr] blocks per shot.
+__global__ void ke_inject_adjoint(const ShotE* __restrict__ shots, float* vx, float* vy, float* vz,
+                                  const int* __restrict__ cell_beg, const Tap* __restrict__ taps,
+                                  const float* __restrict__ rec_w, const float* __restrict__ adj, int nt, int t) {
+    const ShotE S = shots[blockIdx.y];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.nadj; c += gridDim.x * blockDim.x) {
+        const int b = cell_beg[S.adj0 + c], e = cell_beg[S.adj0 + c + 1];
+        const long long cell = taps[b].cell;
+        const int fld = taps[b].field;
+        float* f = fld == 0 ? vx : (fld == 1 ? vy : vz);
+        const float* a_t = adj + S.trace0 + size_t(fld) * nt * S.nr + size_t(t - 1) * S.nr;
+        float acc = f[cell];
+        for (int k = b; k < e; ++k) {
+            const Tap tp = taps[k];
+            acc = acc + a_t[tp.row] * tp.w1 * tp.w2 * tp.w3 * rec_w[S.rec0 + tp.row];
+        }
+        f[cell] = acc;
+    }
+}
+
+// Isotropic compliances (submodule_fwi_gradient.f90:73-95) with c22 = c33 =
+// c11 and c13 = c23 = c12, in the reference's expression order: s22 = s33 =
+// s11 and s23 = s12 bit for bit, s13 has the other denominator.
+__device__ __forceinline__ void compliances(float c11, float c12, float c44, float& s11, float& s12, float& s13,
+                                            float& s44) {
+    const float c13 = c12, c22 = c11, c23 = c12, c33 = c11;
+    const float den1 = c13 * c13 * c22 - 2.0f * c12 * c13 * c23 + c11 * (c23 * c23) + c12 * c12 * c33 - c11 * c22 * c33;
+    const float den2 = c13 * c13 * c22 - 2.0f * c12 * c13 * c23 + c12 * c12 * c33 + c11 * (c23 * c23 - c22 * c33);
+    s11 = (c23 * c23 - c22 * c33) / den1;
+    s12 = (-c13 * c23 + c12 * c33) / den2;
+    s13 = (c13 * c22 - c12 * c23) / den1;
+    s44 = c44 != 0.0f ? 1.0f / c44 : 0.0f;
+}
+
+// Imaging condition (compute_gradient, submodule_fwi_gradient.f90:702-827)
+// over the window interior.  Accumulates, in FP64, the three combinations
+// the chain rule (:486-495) needs:
+//   A = g11 + g12 + g13 + g22 + g23 + g33      (vp)
+//   B = g44 + g55 + g66 - 2 (g12 + g13 + g23)  (vs)
+//   C = sum over x, y, z of (v(i)+v(i+1) - prev)(vr(i)+vr(i+1))  (rho)
+// with g_ij = -(strain of the forward increment) . (adjoint strain).
+__global__ __launch_bounds__(256) void ke_image(const ShotE* __restrict__ shots, EV cur, EV prev, EV adj,
+                                                const float* __restrict__ c11, const float* __restrict__ c12,
+                                                const float* __restrict__ c44, double* gA, double* gB, double* gC,
+                                                int prev_zero, int pml, int R) {
+    const ShotE S = shots[blockIdx.y];
+    const int nx = S.nx, ny = S.ny, nz = S.nz, lo = 1 - pml - R;
+    const long long n = (long long)nx * ny * nz;
+    const size_t sx = 1, sy = S.px, sz = S.pxy;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = 1 + int(e % nx), j = 1 + int((e / nx) % ny), k = 1 + int(e / ((long long)nx * ny));
+        const size_t q = ate(S, lo, i, j, k);
+        float s11, s12, s13, s44;
+        compliances(c11[q], c12[q], c44[q], s11, s12, s13, s44);
+        const float s22 = s11, s33 = s11, s23 = s12;
+        auto avg4 = [](const float* f, size_t b, size_t u, size_t w) {
+            return 0.25f * (f[b] + f[b + u] + f[b + w] + f[b + u + w]);
+        };
+        float txx = cur.sxx[q], tyy = cur.syy[q], tzz = cur.szz[q];
+        float tyz = avg4(cur.syz, q, sy, sz), txz = avg4(cur.sxz, q, sx, sz), txy = avg4(cur.sxy, q, sx, sy);
+        float svx = cur.vx[q] + cur.vx[q + sx], svy = cur.vy[q] + cur.vy[q + sy], svz = cur.vz[q] + cur.vz[q + sz];
+        if (!prev_zero) {
+            txx = txx - prev.sxx[q];
+            tyy = tyy - prev.syy[q];
+            tzz = tzz - prev.szz[q];
+            tyz = tyz - avg4(prev.syz, q, sy, sz);
+            txz = txz - avg4(prev.sxz, q, sx, sz);
+            txy = txy - avg4(prev.sxy, q, sx, sy);
+            svx = svx - (prev.vx[q] + prev.vx[q + sx]);
+            svy = svy - (prev.vy[q] + prev.vy[q + sy]);
+            svz = svz - (prev.vz[q] + prev.vz[q + sz]);
+        }
+        const float rxx = adj.sxx[q], ryy = adj.syy[q], rzz = adj.szz[q];
+        const float ryz = avg4(adj.syz, q, sy, sz), rxz = avg4(adj.sxz, q, sx, sz), rxy = avg4(adj.sxy, q, sx, sy);
+        const float exx = s11 * txx + s12 * tyy + s13 * tzz;
+        const float eyy = s12 * txx + s22 * tyy + s23 * tzz;
+        const float ezz = s13 * txx + s23 * tyy + s33 * tzz;
+        const float fxx = s11 * rxx + s12 * ryy + s13 * rzz;
+        const float fyy = s12 * rxx + s22 * ryy + s23 * rzz;
+        const float fzz = s13 * rxx + s23 * ryy + s33 * rzz;
+        const double i11 = double(exx * fxx), i22 = double(eyy * fyy), i33 = double(ezz * fzz);
+        const double i12 = double(exx * fyy) + double(eyy * fxx);
+        const double i13 = double(exx * fzz) + double(ezz * fxx);
+        const double i23 = double(eyy * fzz) + double(ezz * fyy);
+        const double i44 = double((s44 * tyz) * (s44 * ryz)), i55 = double((s44 * txz) * (s44 * rxz)),
+                     i66 = double((s44 * txy) * (s44 * rxy));
+        const float rvx = adj.vx[q] + adj.vx[q + sx], rvy = adj.vy[q] + adj.vy[q + sy], rvz = adj.vz[q] + adj.vz[q + sz];
+        const size_t o = size_t(S.g0 + e);
+        gA[o] = gA[o] - (i11 + i12 + i13 + i22 + i23 + i33);
+        gB[o] = gB[o] - (i44 + i55 + i66) + 2.0 * (i12 + i13 + i23);
+        gC[o] = gC[o] + double(svx * rvx) + double(svy * rvy) + double(svz * rvz);
+    }
+}
+
+// L2 waveform misfit and adjoint source per shot and component
+// (compute_adjsrc_waveform, module_inversion_adjoint_source.f90:1020-1125,
+// 371-383): the trace discard rule uses each component's own maximum trace
+// norms; traces [nt][nr] per component; FP64 Neumaier sums.
+__global__ __launch_bounds__(256) void ke_adjoint_source(const ShotE* __restrict__ shots,
+                                                         const float* __restrict__ syn, const float* __restrict__ obs,
+                                                         const float* __restrict__ rec_w, float* adj, double* misfit,
+                                                         int nt, float threshold, double* scratch) {
+    const ShotE S = shots[blockIdx.x];
+    const int comp = blockIdx.y;
+    const size_t base = S.trace0 + size_t(comp) * nt * S.nr;
+    __shared__ double red[256];
+    __shared__ double mo_s, ms_s;
+    double* nobs = scratch + 2 * (3 * size_t(S.rec0) + size_t(comp) * S.nr);
+    double* nsyn = nobs + S.nr;
+    double lmo = 0, lms = 0;
+    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
+        double a = 0, b = 0;
+        for (int t = 0; t < nt; ++t) {
+            const double o = obs[base + size_t(t) * S.nr + r];
+            const double sv = syn[base + size_t(t) * S.nr + r];
+            a += o * o;
+            b += sv * sv;
+        }
+        nobs[r] = sqrt(a);
+        nsyn[r] = sqrt(b);
+        lmo = fmax(lmo, nobs[r]);
+        lms = fmax(lms, nsyn[r]);
+    }
+    red[threadIdx.x] = lmo;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mo_s = red[0];
+    __syncthreads();
+    red[threadIdx.x] = lms;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ms_s = red[0];
+    __syncthreads();
+    const double mo = mo_s, ms = ms_s;
+    double sum = 0, cmp = 0;
+    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
+        const bool keep = !(rec_w[S.rec0 + r] == 0.0f || nobs[r] < threshold * mo || nsyn[r] < threshold * ms ||
+                            nobs[r] * nsyn[r] == 0.0);
+        for (int t = 0; t < nt; ++t) {
+            const size_t q = base + size_t(t) * S.nr + r;
+            if (keep) {
+                const float d = obs[q] - syn[q];
+                const double d2 = double(d) * double(d);
+                const double y = sum + d2;
+                cmp += (fabs(sum) >= fabs(d2)) ? (sum - y) + d2 : (d2 - y) + sum;
+                sum = y;
+                adj[q] = syn[q] - obs[q];
+            } else {
+                adj[q] = 0.0f;
+            }
+        }
+    }
+    red[threadIdx.x] = sum + cmp;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) misfit[3 * blockIdx.x + comp] = red[0];
+}
+
+// Sum of the per-shot window accumulators on the full model (nz, ny, nx),
+// z fastest (module_gradient.f90:366-388): the three arrays in one pass.
+__global__ void ke_merge(const ShotE* __restrict__ shots, const int* __restrict__ win, int nshot,
+                         const double* __restrict__ gA, const double* __restrict__ gB, const double* __restrict__ gC,
+                         double* fA, double* fB, double* fC, int nx, int ny, int nz) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int kz = int(e % nz), jy = int((e / nz) % ny), ix = int(e / ((long long)nz * ny));
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int s = 0; s < nshot; ++s) {
+            const ShotE& S = shots[s];
+            const int i = ix - (win[3 * s] - 1), j = jy - (win[3 * s + 1] - 1), k = kz - (win[3 * s + 2] - 1);
+            if (i < 0 || i >= S.nx || j < 0 || j >= S.ny || k < 0 || k >= S.nz) continue;
+            const size_t o = size_t(S.g0) + i + size_t(j) * S.nx + size_t(k) * S.nx * S.ny;
+            a += gA[o];
+            b += gB[o];
+            c += gC[o];
+        }
+        fA[e] = a;
+        fB[e] = b;
+        fC[e] = c;
+    }
+}
+
 struct ShotHostE {
-    int id = 0, nx = 0, ny = 0, nz = 0, nxbeg = 1, nybeg = 1, nzbeg = 1;
+    int id = 0, nx = 0, ny = 0, nz = 0, nxbeg = 1, nybeg = 1, nzbeg = 1, hnt = 1;
     float ox = 0, oy = 0, oz = 0;
     std::vector<owl_source> src;
     std::vector<owl_receiver> rec;
@@ -451,9 +725,10 @@ struct owl_elastic3d {
     std::vector<ShotHostE> hs;
     std::vector<ShotE> sd;
     std::vector<int> tile_shot;
-    size_t field_elems = 0, trace_elems = 0;
-    int nrec_total = 0, max_nr = 0;
+    size_t field_elems = 0, trace_elems = 0, strip_elems = 0, grad_elems = 0;
+    int nrec_total = 0, max_nr = 0, max_nadj = 0;
     double pts_full = 0;
+    bool have_boundary = false, fwd_state_valid = false, have_obs = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0, last_updates = 0;
@@ -462,9 +737,17 @@ struct owl_elastic3d {
     DevBuf<ShotE> d_shots;
     DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg;
     DevBuf<float> d_prof, d_amp, d_rec_w, d_traces;
-    DevBuf<float> med[8];
-    DevBuf<float> fld[27];
+    DevBuf<float> med[9];   // c11, c12, mu_yz, mu_xz, mu_xy, rho_x, rho_y, rho_z, c44 at the nodes
+    DevBuf<float> fld[27];  // nine wavefields, eighteen memory variables
     DevBuf<Tap> d_inj, d_rec;
+    // gradient: boundary records, the other half of the reverse ping-pong,
+    // the adjoint field, adjoint-source CSR, accumulators
+    DevBuf<float> d_strips, d_adj, d_obs;
+    DevBuf<float> fB[9];
+    DevBuf<float> aF[27];
+    DevBuf<int> d_adj_beg, d_win;
+    DevBuf<Tap> d_adj_taps;
+    DevBuf<double> d_gA, d_gB, d_gC, d_fA, d_fB, d_fC, d_misfit, d_scratch;
     ~owl_elastic3d() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -543,12 +826,15 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
     b->tile_shot.clear();
     b->nrec_total = b->max_nr = 0;
     b->pts_full = 0;
-    long long off = 0, trace = 0;
+    long long off = 0, trace = 0, strip = 0, grad = 0;
     int prof = 0;
     std::vector<float> profs, amp, rec_w;
     std::vector<int> inj_beg{0};
     std::vector<Tap> inj;
     std::vector<std::vector<Tap>> rec_taps;  // per (shot, component, receiver)
+    std::vector<int> adj_beg{0};
+    std::vector<Tap> adj_taps;
+    b->max_nadj = 0;
     for (int s = 0; s < nshot; ++s) {
         const owl_shot& in = shots[s];
         ShotHostE& H = b->hs[s];
@@ -615,6 +901,13 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
         S.nr = int(H.rec.size());
         S.trace0 = trace;
         trace += 3LL * nt * S.nr;
+        {
+            const EStrip g = estrip_geo(H.nx, H.ny, H.nz, R);
+            S.strip0 = strip;
+            strip += 3LL * 2 * (g.a + g.b + g.c) * nt;
+        }
+        S.g0 = grad;
+        grad += (long long)H.nx * H.ny * H.nz;
         // force sources; above the first node below a free surface they move
         // to z = dz with amplitude x 1.2 (module_vars.f90:650-655)
         const float xmax = float(H.nx - 1) * cfg->dx + H.ox, ymax = float(H.ny - 1) * cfg->dy + H.oy,
@@ -632,6 +925,7 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
             const Located L = locate(3, sc.x, sc.y, sc.z, H.ox, H.oy, H.oz, cfg->dx, cfg->dy, cfg->dz);
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            if (&sc0 == &H.src[0]) H.hnt = hnt;
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
             const float rs = b->rho_full[model_index(b, H, L.gx, L.gy, L.gz)];
             const float sp = std::sin(sc.polar), cp = std::cos(sc.polar);
@@ -697,12 +991,42 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
                                     c});
                         }
         }
+        // adjoint sources: the receivers' taps, below the free surface only
+        // (submodule_fwi_gradient.f90:256-330), grouped by (component, cell)
+        // in receiver order
+        {
+            std::vector<Tap> at;
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < S.nr; ++r)
+                    for (const Tap& tp : comp[size_t(c) * S.nr + r]) {
+                        const long long rel = tp.cell - S.off;
+                        const int kk = int(rel / S.pxy) + lo;
+                        if (b->fs && kk < 2) continue;
+                        at.push_back(tp);
+                    }
+            std::stable_sort(at.begin(), at.end(), [](const Tap& x, const Tap& y) {
+                return x.field != y.field ? x.field < y.field : x.cell < y.cell;
+            });
+            S.adj0 = int(adj_beg.size()) - 1;
+            S.nadj = 0;
+            for (size_t k = 0; k < at.size(); ++k) {
+                if (k == 0 || at[k].cell != at[k - 1].cell || at[k].field != at[k - 1].field) {
+                    if (k) adj_beg.push_back(int(adj_taps.size()));
+                    ++S.nadj;
+                }
+                adj_taps.push_back(at[k]);
+            }
+            if (!at.empty()) adj_beg.push_back(int(adj_taps.size()));
+            b->max_nadj = std::max(b->max_nadj, S.nadj);
+        }
         for (auto& v : comp) rec_taps.push_back(std::move(v));
         b->nrec_total += S.nr;
         b->max_nr = std::max(b->max_nr, S.nr);
     }
     b->field_elems = size_t(off);
     b->trace_elems = size_t(trace);
+    b->strip_elems = size_t(strip);
+    b->grad_elems = size_t(grad);
     // receiver CSR in (shot, component, receiver) order: entry 3*rec0 + c*nr + r
     std::vector<int> rec_beg{0};
     std::vector<Tap> rec;
@@ -728,6 +1052,14 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
     if (rec.empty()) rec.push_back(Tap{});
     b->d_rec.upload(rec, stm);
     b->d_rec_w.upload(rec_w, stm);
+    b->d_adj_beg.upload(adj_beg, stm);
+    if (adj_taps.empty()) adj_taps.push_back(Tap{});
+    b->d_adj_taps.upload(adj_taps, stm);
+    {
+        std::vector<int> win;
+        for (auto& H : b->hs) win.insert(win.end(), {H.nxbeg, H.nybeg, H.nzbeg});
+        b->d_win.upload(win, stm);
+    }
     // padded media (edge replication, permute(...,321)), then the effective
     // values at the staggered positions in OWL's operation order
     const size_t n = b->field_elems;
@@ -764,8 +1096,8 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
                     rz[q] = 0.5f * (rh[q - sz] + rh[q]);
                 }
     }
-    const std::vector<float>* media[8] = {&c11, &c12, &myz, &mxz, &mxy, &rx, &ry, &rz};
-    for (int k = 0; k < 8; ++k) b->med[k].upload(*media[k], stm);
+    const std::vector<float>* media[9] = {&c11, &c12, &myz, &mxz, &mxy, &rx, &ry, &rz, &c44};
+    for (int k = 0; k < 9; ++k) b->med[k].upload(*media[k], stm);
     for (auto& f : b->fld) f.alloc(b->field_elems);
     b->d_traces.alloc(std::max<size_t>(b->trace_elems, 1));
     OWL_HIP(hipStreamSynchronize(stm));
@@ -799,56 +1131,6 @@ void sweep_bytes(const owl_elastic3d* b, double& bs, double& bv) {
     }
 }
 
-template <int R>
-void run_forward_e(owl_elastic3d* b) {
-    hipStream_t st = b->stream;
-    for (auto& f : b->fld) f.zero(st);
-    b->d_traces.zero(st);
-    const EF F = fields(b);
-    const EM M = media(b);
-    const unsigned nshot = unsigned(b->sd.size());
-    const dim3 blk(64, unsigned(b->cy)), grd(unsigned(b->tile_shot.size()));
-    double bs, bv;
-    sweep_bytes(b, bs, bv);
-    const unsigned rg = unsigned(std::max(1, (3 * b->max_nr + 255) / 256));
-    OWL_HIP(hipEventRecord(b->ev0, st));
-    // step t: [free surface: mirror v above it] -> record(t) -> update -> source
-    for (int t = 1; t <= b->nt; ++t) {
-        if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 2);
-        int ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(ke_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->d_rec_beg.get(),
-                           b->d_rec.get(), b->d_rec_w.get(), b->d_traces.get(), b->nt, t);
-        b->prof.end(st, ev);
-        if (t == b->nt) break;  // the last update changes nothing that is recorded
-        if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 0);
-        ev = b->prof.begin(st, 0, bs);
-        hipLaunchKernelGGL(ke_stress<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, M, b->d_prof.get(),
-                           b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
-        b->prof.end(st, ev);
-        if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 1);
-        ev = b->prof.begin(st, 1, bv);
-        hipLaunchKernelGGL(ke_velocity<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, M,
-                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
-        b->prof.end(st, ev);
-        ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(ke_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), F, b->d_inj_beg.get(),
-                           b->d_inj.get(), b->d_amp.get(), b->nt, t);
-        b->prof.end(st, ev);
-        if (b->prof.pending() > 20000) {
-            OWL_HIP(hipStreamSynchronize(st));
-            b->prof.collect();
-        }
-    }
-    OWL_HIP(hipGetLastError());
-    OWL_HIP(hipEventRecord(b->ev1, st));
-    OWL_HIP(hipStreamSynchronize(st));
-    b->prof.collect();
-    float ms = 0.f;
-    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    b->last_ms = ms;
-    b->last_updates = double(b->nt - 1) * b->pts_full;
-}
-
 template <class Fn>
 void dispatch_Re(int R, Fn&& f) {
     switch (R) {
@@ -861,6 +1143,237 @@ void dispatch_Re(int R, Fn&& f) {
         case 8: f(std::integral_constant<int, 8>{}); break;
         default: fail(OWL_ERR_ARG, "fdhalf must be 2..8");
     }
+}
+
+EV out_of(const EF& F) { return EV{F.sxx, F.syy, F.szz, F.syz, F.sxz, F.sxy, F.vx, F.vy, F.vz}; }
+
+EV ev_of(DevBuf<float>* f) {
+    return EV{f[0].get(), f[1].get(), f[2].get(), f[3].get(), f[4].get(), f[5].get(), f[6].get(), f[7].get(), f[8].get()};
+}
+
+EF with_fields(EF F, const EV& V) {
+    F.sxx = V.sxx; F.syy = V.syy; F.szz = V.szz; F.syz = V.syz; F.sxz = V.sxz; F.sxy = V.sxy;
+    F.vx = V.vx; F.vy = V.vy; F.vz = V.vz;
+    return F;
+}
+
+void maybe_collect(owl_elastic3d* b) {
+    if (b->prof.pending() > 20000) {
+        OWL_HIP(hipStreamSynchronize(b->stream));
+        b->prof.collect();
+    }
+}
+
+// One time step of the wavefields F (memory variables in F) into O (O may
+// be F's own fields): [free surface: zero v above it] -> stresses ->
+// [mirror stresses] -> velocities (module_wavefield.f90:52-1090).
+template <int R>
+void step_e(owl_elastic3d* b, const EF& F, const EV& O, float dt, double bs, double bv) {
+    hipStream_t st = b->stream;
+    const unsigned nshot = unsigned(b->sd.size());
+    if (64 * b->cy > 256) fail(OWL_ERR_ARG, "launch shape exceeds the sweeps' 256-thread blocks");
+    const dim3 blk(64, unsigned(b->cy)), grd(unsigned(b->tile_shot.size()));
+    const EM M = media(b);
+    if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 0);
+    int ev = b->prof.begin(st, 0, bs);
+    hipLaunchKernelGGL(ke_stress<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M, b->d_prof.get(),
+                       b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
+    b->prof.end(st, ev);
+    if (b->fs)
+        hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), with_fields(F, O),
+                           b->pml, 1);
+    ev = b->prof.begin(st, 1, bv);
+    hipLaunchKernelGGL(ke_velocity<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
+                       b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
+    b->prof.end(st, ev);
+}
+
+void strips_e(owl_elastic3d* b, int R, float* vx, float* vy, float* vz, int t, int save);
+
+template <int R>
+void run_forward_e(owl_elastic3d* b, bool save_boundary) {
+    hipStream_t st = b->stream;
+    for (auto& f : b->fld) f.zero(st);
+    b->d_traces.zero(st);
+    if (save_boundary) b->d_strips.alloc(std::max<size_t>(b->strip_elems, 1));
+    const EF F = fields(b);
+    const unsigned nshot = unsigned(b->sd.size());
+    double bs, bv;
+    sweep_bytes(b, bs, bv);
+    const unsigned rg = unsigned(std::max(1, (3 * b->max_nr + 255) / 256));
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    // step t: [boundary record of the state at the start of the step] ->
+    // [free surface: mirror v above it] -> record(t) -> update -> source
+    for (int t = 1; t <= b->nt; ++t) {
+        if (save_boundary) strips_e(b, R, F.vx, F.vy, F.vz, t, 1);
+        if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 2);
+        int ev = b->prof.begin(st, 2, 0);
+        hipLaunchKernelGGL(ke_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->d_rec_beg.get(),
+                           b->d_rec.get(), b->d_rec_w.get(), b->d_traces.get(), b->nt, t);
+        b->prof.end(st, ev);
+        if (t == b->nt) {
+            // the last update changes nothing that is recorded; the state at
+            // the start of step nt is the reverse pass's final state, whose
+            // velocities above a free surface are zero (the mirror undone)
+            if (b->fs)
+                hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 3);
+            break;
+        }
+        step_e<R>(b, F, out_of(F), b->dt, bs, bv);
+        ev = b->prof.begin(st, 2, 0);
+        hipLaunchKernelGGL(ke_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), F, b->d_inj_beg.get(),
+                           b->d_inj.get(), b->d_amp.get(), b->nt, t);
+        b->prof.end(st, ev);
+        maybe_collect(b);
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    b->prof.collect();
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = double(b->nt - 1) * b->pts_full;
+    b->have_boundary = save_boundary;
+    b->fwd_state_valid = save_boundary;
+}
+
+void strips_e(owl_elastic3d* b, int R, float* vx, float* vy, float* vz, int t, int save) {
+    size_t mx = 1;
+    for (auto& S : b->sd) {
+        const EStrip g = estrip_geo(S.nx, S.ny, S.nz, R);
+        mx = std::max<size_t>(mx, size_t(6 * (g.a + g.b + g.c)));
+    }
+    const dim3 grd(unsigned(std::min<size_t>(4096, (mx + 255) / 256)), unsigned(b->sd.size()));
+    const int ev = b->prof.begin(b->stream, 4, 0);
+    dispatch_Re(R, [&](auto r) {
+        hipLaunchKernelGGL(ke_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(), vx, vy, vz,
+                           b->d_strips.get(), t, b->pml, save);
+    });
+    b->prof.end(b->stream, ev);
+}
+
+// OWL's reverse pass (compute_adjoint, elastic_vhtiort_3d/
+// submodule_fwi_gradient.f90): t = nt .. hnt, prev = state; state =
+// update(state, -dt); at t = nt the final state; boundary record t into the
+// state; adjoint = update(adjoint, -dt); adjoint sources; imaging.  The
+// forward field steps out of place (the previous state stays in the other
+// buffer for the imaging); at t = nt the update of the zero field is the
+// zero field, so the final state is used as is with prev = 0.
+template <int R>
+void run_reverse_e(owl_elastic3d* b) {
+    hipStream_t st = b->stream;
+    if (!b->have_boundary || !b->fwd_state_valid)
+        fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
+    if (!b->d_adj.get() || b->d_adj.size() < b->trace_elems) fail(OWL_ERR_STATE, "no adjoint source");
+    // OWL's loop ends at the first source's hnt
+    for (auto& H : b->hs)
+        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one gradient batch must share hnt");
+    for (int k = 9; k < 27; ++k) b->fld[k].zero(st);  // the reverse run's memory variables start at zero
+    for (auto& f : b->fB) {
+        f.alloc(b->field_elems);
+        f.zero(st);
+    }
+    for (auto& f : b->aF) {
+        f.alloc(b->field_elems);
+        f.zero(st);
+    }
+    const size_t ng = std::max<size_t>(b->grad_elems, 1);
+    for (auto* g : {&b->d_gA, &b->d_gB, &b->d_gC}) {
+        g->alloc(ng);
+        g->zero(st);
+    }
+    const unsigned nshot = unsigned(b->sd.size());
+    const int hnt = std::max(b->hs[0].hnt, 1);
+    const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
+    double bs, bv;
+    sweep_bytes(b, bs, bv);
+    EF X = fields(b);              // current state (with memory variables)
+    EV Y = ev_of(b->fB);           // the other half of the ping-pong
+    EF A;
+    {
+        float** p = &A.sxx;
+        for (int k = 0; k < 27; ++k) p[k] = b->aF[k].get();
+    }
+    const EV adj = out_of(A);
+    // imaging: 27 wavefields (three with up to 3 extra neighbours, mostly
+    // cached), c11, c12, c44, three FP64 accumulators read and written
+    const double image_bytes = (27.0 * 4 + 12.0 + 48.0) * double(b->grad_elems);
+    const unsigned ig = unsigned(std::max<size_t>(1, std::min<size_t>(2048, (b->grad_elems / nshot + 255) / 256)));
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    for (int t = b->nt; t >= hnt; --t) {
+        EV cur = out_of(X), prev = out_of(X);
+        if (t != b->nt) {
+            step_e<R>(b, X, Y, -b->dt, bs, bv);
+            cur = Y;
+        }
+        strips_e(b, R, cur.vx, cur.vy, cur.vz, t, 0);
+        step_e<R>(b, A, adj, -b->dt, bs, bv);
+        int ev = b->prof.begin(st, 2, 0);
+        if (b->max_nadj > 0)
+            hipLaunchKernelGGL(ke_inject_adjoint, dim3(unsigned((b->max_nadj + 255) / 256), nshot), dim3(256), 0, st,
+                               b->d_shots.get(), adj.vx, adj.vy, adj.vz, b->d_adj_beg.get(), b->d_adj_taps.get(),
+                               b->d_rec_w.get(), b->d_adj.get(), b->nt, t);
+        b->prof.end(st, ev);
+        if (t % cc == 0) {
+            ev = b->prof.begin(st, 3, image_bytes);
+            hipLaunchKernelGGL(ke_image, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(), cur, prev, adj,
+                               b->med[0].get(), b->med[1].get(), b->med[8].get(), b->d_gA.get(), b->d_gB.get(),
+                               b->d_gC.get(), t == b->nt ? 1 : 0, b->pml, R);
+            b->prof.end(st, ev);
+        }
+        if (t != b->nt) {
+            // the new state becomes current; its old buffers take the next step
+            const EV old = out_of(X);
+            X = with_fields(X, Y);
+            Y = old;
+        }
+        maybe_collect(b);
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    b->prof.collect();
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = 2.0 * double(b->nt - hnt) * b->pts_full;
+    b->fwd_state_valid = false;
+}
+
+void traces_to_host_e(const owl_elastic3d* b, const float* dev, float* out) {
+    std::vector<float> h(b->trace_elems);
+    OWL_HIP(hipMemcpy(h.data(), dev, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    // device [3][nt][nr] per shot -> host [3][nr][nt] per shot
+    for (auto& S : b->sd)
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < S.nr; ++r)
+                for (int t = 0; t < b->nt; ++t)
+                    out[size_t(S.trace0) + (size_t(c) * S.nr + r) * b->nt + t] =
+                        h[size_t(S.trace0) + size_t(c) * b->nt * S.nr + size_t(t) * S.nr + r];
+}
+
+void traces_to_device_e(owl_elastic3d* b, const float* in, DevBuf<float>& dev) {
+    std::vector<float> h(b->trace_elems);
+    for (auto& S : b->sd)
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < S.nr; ++r)
+                for (int t = 0; t < b->nt; ++t)
+                    h[size_t(S.trace0) + size_t(c) * b->nt * S.nr + size_t(t) * S.nr + r] =
+                        in[size_t(S.trace0) + (size_t(c) * S.nr + r) * b->nt + t];
+    dev.alloc(std::max<size_t>(h.size(), 1));
+    dev.upload(h.data(), h.size(), b->stream);
+    OWL_HIP(hipStreamSynchronize(b->stream));
+}
+
+// Chain rule (submodule_fwi_gradient.f90:486-495) with vp, vs in km/s and
+// rho in g/cm^3: grad_vp = 2 rho vp A, grad_vs = 2 rho vs B, grad_rho = C.
+inline void chain_rule(float vp, float vs, float rho, double A, double B, double C, double& gv, double& gs,
+                       double& gr) {
+    const double r = double(rho * 1.0e-3f), p = double(vp * 1.0e-3f), v = double(vs * 1.0e-3f);
+    gv = 2.0 * r * p * A;
+    gs = 2.0 * r * v * B;
+    gr = C;
 }
 
 }  // namespace
@@ -887,28 +1400,102 @@ void owl_elastic3d_destroy(owl_elastic3d* b) { delete b; }
 
 int owl_elastic3d_nt(const owl_elastic3d* b) { return b ? b->nt : 0; }
 
-owl_status owl_elastic3d_forward(owl_elastic3d* b, float* traces) {
+owl_status owl_elastic3d_forward(owl_elastic3d* b, float* traces, int save_boundary) {
     return guarded([&] {
         if (!b) fail(OWL_ERR_ARG, "null batch");
-        dispatch_Re(b->R, [&](auto r) { run_forward_e<decltype(r)::value>(b); });
-        if (traces) {
-            std::vector<float> h(b->trace_elems);
-            OWL_HIP(hipMemcpy(h.data(), b->d_traces.get(), h.size() * sizeof(float), hipMemcpyDeviceToHost));
-            // device [3][nt][nr] per shot -> host [3][nr][nt] per shot
-            for (auto& S : b->sd)
-                for (int c = 0; c < 3; ++c)
-                    for (int r = 0; r < S.nr; ++r)
-                        for (int t = 0; t < b->nt; ++t)
-                            traces[size_t(S.trace0) + (size_t(c) * S.nr + r) * b->nt + t] =
-                                h[size_t(S.trace0) + size_t(c) * b->nt * S.nr + size_t(t) * S.nr + r];
+        dispatch_Re(b->R, [&](auto r) { run_forward_e<decltype(r)::value>(b, save_boundary != 0); });
+        if (traces) traces_to_host_e(b, b->d_traces.get(), traces);
+    });
+}
+
+owl_status owl_elastic3d_adjoint(owl_elastic3d* b, const float* adj, float* grad_vp, float* grad_vs,
+                                 float* grad_rho) {
+    return guarded([&] {
+        if (!b || !adj) fail(OWL_ERR_ARG, "null argument");
+        traces_to_device_e(b, adj, b->d_adj);
+        dispatch_Re(b->R, [&](auto r) { run_reverse_e<decltype(r)::value>(b); });
+        std::vector<double> gA(b->grad_elems), gB(b->grad_elems), gC(b->grad_elems);
+        OWL_HIP(hipMemcpy(gA.data(), b->d_gA.get(), gA.size() * sizeof(double), hipMemcpyDeviceToHost));
+        OWL_HIP(hipMemcpy(gB.data(), b->d_gB.get(), gB.size() * sizeof(double), hipMemcpyDeviceToHost));
+        OWL_HIP(hipMemcpy(gC.data(), b->d_gC.get(), gC.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // per-shot window gradients (nz_s, ny_s, nx_s), z fastest, NaN/Inf -> 0
+        for (size_t s = 0; s < b->sd.size(); ++s) {
+            const ShotE& S = b->sd[s];
+            const ShotHostE& H = b->hs[s];
+            for (int k = 0; k < S.nz; ++k)
+                for (int j = 0; j < S.ny; ++j)
+                    for (int i = 0; i < S.nx; ++i) {
+                        const size_t src = size_t(S.g0) + i + size_t(j) * S.nx + size_t(k) * S.nx * S.ny;
+                        const size_t dst = size_t(S.g0) + k + size_t(j) * S.nz + size_t(i) * S.nz * S.ny;
+                        const size_t m = model_index(b, H, i + 1, j + 1, k + 1);
+                        double gv, gs, gr;
+                        chain_rule(b->vp_full[m], b->vs_full[m], b->rho_full[m], gA[src], gB[src], gC[src], gv, gs,
+                                   gr);
+                        const float a = float(gv), c = float(gs), e = float(gr);
+                        if (grad_vp) grad_vp[dst] = std::isfinite(a) ? a : 0.f;
+                        if (grad_vs) grad_vs[dst] = std::isfinite(c) ? c : 0.f;
+                        if (grad_rho) grad_rho[dst] = std::isfinite(e) ? e : 0.f;
+                    }
         }
+    });
+}
+
+owl_status owl_elastic3d_gradient(owl_elastic3d* b, const float* obs, double* misfit, double* grad_vp,
+                                  double* grad_vs, double* grad_rho, float* traces_out) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (obs) {
+            traces_to_device_e(b, obs, b->d_obs);
+            b->have_obs = true;
+        }
+        if (!b->have_obs) fail(OWL_ERR_STATE, "no observed data");
+        dispatch_Re(b->R, [&](auto r) { run_forward_e<decltype(r)::value>(b, true); });
+        const double fwd_ms = b->last_ms, fwd_up = b->last_updates;
+        hipStream_t st = b->stream;
+        const unsigned nshot = unsigned(b->sd.size());
+        b->d_adj.alloc(std::max<size_t>(b->trace_elems, 1));
+        b->d_misfit.alloc(3 * size_t(nshot));
+        b->d_scratch.alloc(std::max<size_t>(6 * size_t(b->nrec_total), 1));
+        hipLaunchKernelGGL(ke_adjoint_source, dim3(nshot, 3), dim3(256), 0, st, b->d_shots.get(), b->d_traces.get(),
+                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, 1.0e-6f,
+                           b->d_scratch.get());
+        dispatch_Re(b->R, [&](auto r) { run_reverse_e<decltype(r)::value>(b); });
+        const size_t nfull = size_t(b->cfg.nx) * b->cfg.ny * b->cfg.nz;
+        for (auto* g : {&b->d_fA, &b->d_fB, &b->d_fC}) g->alloc(nfull);
+        const unsigned mg = unsigned(std::min<size_t>(8192, (nfull + 255) / 256));
+        hipLaunchKernelGGL(ke_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), b->d_win.get(), int(nshot),
+                           b->d_gA.get(), b->d_gB.get(), b->d_gC.get(), b->d_fA.get(), b->d_fB.get(), b->d_fC.get(),
+                           b->cfg.nx, b->cfg.ny, b->cfg.nz);
+        OWL_HIP(hipGetLastError());
+        OWL_HIP(hipStreamSynchronize(st));
+        if (misfit) {
+            std::vector<double> m3(3 * size_t(nshot));
+            OWL_HIP(hipMemcpy(m3.data(), b->d_misfit.get(), m3.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (unsigned s = 0; s < nshot; ++s) misfit[s] = m3[3 * s] + m3[3 * s + 1] + m3[3 * s + 2];
+        }
+        if (grad_vp || grad_vs || grad_rho) {
+            std::vector<double> fA(nfull), fB(nfull), fC(nfull);
+            OWL_HIP(hipMemcpy(fA.data(), b->d_fA.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
+            OWL_HIP(hipMemcpy(fB.data(), b->d_fB.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
+            OWL_HIP(hipMemcpy(fC.data(), b->d_fC.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t e = 0; e < nfull; ++e) {
+                double gv, gs, gr;
+                chain_rule(b->vp_full[e], b->vs_full[e], b->rho_full[e], fA[e], fB[e], fC[e], gv, gs, gr);
+                if (grad_vp) grad_vp[e] = std::isfinite(gv) ? gv : 0.0;
+                if (grad_vs) grad_vs[e] = std::isfinite(gs) ? gs : 0.0;
+                if (grad_rho) grad_rho[e] = std::isfinite(gr) ? gr : 0.0;
+            }
+        }
+        if (traces_out) traces_to_host_e(b, b->d_traces.get(), traces_out);
+        b->last_ms += fwd_ms;
+        b->last_updates += fwd_up;
     });
 }
 
 owl_status owl_elastic3d_set_launch(owl_elastic3d* b, int planes, int rows) {
     return guarded([&] {
-        if (!b || planes < 1 || (rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16))
-            fail(OWL_ERR_ARG, "bad launch shape");
+        // blocks of 64 x rows threads; the sweeps are compiled for at most 256
+        if (!b || planes < 1 || (rows != 1 && rows != 2 && rows != 4)) fail(OWL_ERR_ARG, "bad launch shape");
         b->kc = planes;
         b->cy = rows;
         int lt = 0;

@@ -2,7 +2,9 @@
 include/owl_amd.h): a batch of shots with per-shot windows sharing one
 (nz, ny, nx) model, forward modelling as OWL's elastic_vhtiort_3d
 compute_forward (src/solver/elastic_vhtiort_3d/submodule_forward_modeling.f90:31-140)
-for isotropic media, with force sources and three-component receivers."""
+for isotropic media, with force sources and three-component receivers, and
+OWL's gradient (compute_adjoint / compute_gradient,
+src/solver/elastic_vhtiort_3d/submodule_fwi_gradient.f90:31-827)."""
 import ctypes as C
 
 import numpy as np
@@ -19,7 +21,7 @@ class Elastic3DBatch:
     """vp, vs, rho: (nz, ny, nx).  shots: list of (Geometry, window) where
     window is None (full model) or a dict with nxbeg..nzend (1-based)."""
 
-    KERNELS = ("stress", "velocity", "sources_receivers")
+    KERNELS = ("stress", "velocity", "sources_receivers", "imaging", "boundary_records")
 
     def __init__(self, nx, ny, nz, dx, dy, dz, vp, vs, rho, shots, dt, tmax, ox=0.0, oy=0.0, oz=0.0,
                  pml=15, fdhalf=4, f0factor=1.0, free_surface=False):
@@ -33,6 +35,7 @@ class Elastic3DBatch:
         arr = (_lib.owl_shot * len(shots))()
         self._keep = []
         self.nr = []
+        self._windows = []
         for k, (geom, win) in enumerate(shots):
             srcs = (_lib.owl_source * geom.ns)()
             for i, s in enumerate(geom.srcr):
@@ -48,6 +51,8 @@ class Elastic3DBatch:
                                    w.get("nxbeg", 0), w.get("nxend", 0), w.get("nybeg", 0),
                                    w.get("nyend", 0), w.get("nzbeg", 0), w.get("nzend", 0))
             self.nr.append(geom.nr)
+            self._windows.append(tuple((w.get(ax + "end", 0) or n) - (w.get(ax + "beg", 0) or 1) + 1
+                                       for ax, n in (("nz", nz), ("ny", ny), ("nx", nx))))
         self._shots = arr
         h = C.c_void_p()
         check(lib().owl_elastic3d_create(C.byref(self.cfg), *[_ptr(a) for a in self._m], len(shots),
@@ -69,18 +74,61 @@ class Elastic3DBatch:
     def set_launch(self, planes, rows=4):
         check(lib().owl_elastic3d_set_launch(self._h, int(planes), int(rows)))
 
-    def forward(self, want_traces=True):
-        """Per shot (vx, vy, vz) traces, each (nr, nt)."""
-        flat = np.zeros(3 * sum(self.nr) * self.nt, np.float32) if want_traces else None
-        check(lib().owl_elastic3d_forward(self._h, None if flat is None else _ptr(flat)))
-        if flat is None:
-            return None
+    def _split(self, flat):
         out, o = [], 0
         for n in self.nr:
             blk = flat[o:o + 3 * n * self.nt].reshape(3, n, self.nt)
             out.append((blk[0].copy(), blk[1].copy(), blk[2].copy()))
             o += 3 * n * self.nt
         return out
+
+    def _flat(self, per_shot):
+        parts = []
+        for (tx, ty, tz), n in zip(per_shot, self.nr):
+            a = np.stack([np.asarray(tx), np.asarray(ty), np.asarray(tz)]).astype(np.float32)
+            if a.shape != (3, n, self.nt):
+                raise ValueError(f"traces must be three (nr, nt) = ({n}, {self.nt}) arrays per shot")
+            parts.append(a.ravel())
+        return np.ascontiguousarray(np.concatenate(parts))
+
+    def forward(self, want_traces=True, save_boundary=False):
+        """Per shot (vx, vy, vz) traces, each (nr, nt)."""
+        flat = np.zeros(3 * sum(self.nr) * self.nt, np.float32) if want_traces else None
+        check(lib().owl_elastic3d_forward(self._h, None if flat is None else _ptr(flat), int(bool(save_boundary))))
+        return None if flat is None else self._split(flat)
+
+    def adjoint(self, adj):
+        """OWL's reverse pass after forward(save_boundary=True).  adj: per
+        shot (ax, ay, az), each (nr, nt).  Returns per shot (grad_vp,
+        grad_vs, grad_rho) on the shot's window, each (nz_s, ny_s, nx_s)."""
+        flat = self._flat(adj)
+        gv, gs, gr = (np.zeros(self._grad_elems(), np.float32) for _ in range(3))
+        check(lib().owl_elastic3d_adjoint(self._h, _ptr(flat), _ptr(gv), _ptr(gs), _ptr(gr)))
+        out, o = [], 0
+        for (nz, ny, nx) in self._windows:
+            n = nz * ny * nx
+            out.append(tuple(a[o:o + n].reshape(nx, ny, nz).transpose(2, 1, 0).copy() for a in (gv, gs, gr)))
+            o += n
+        return out
+
+    def gradient(self, obs=None):
+        """Forward, L2 adjoint source per component, reverse pass and the sum
+        over shots.  obs: per shot (ox, oy, oz) or None to reuse the last.
+        Returns (misfit per shot, grad_vp, grad_vs, grad_rho as (nz, ny, nx)
+        float64, synthetic traces per shot)."""
+        flat = None if obs is None else self._flat(obs)
+        n = int(np.prod(self.shape))
+        mis = np.zeros(len(self.nr))
+        gv, gs, gr = (np.zeros(n) for _ in range(3))
+        tr = np.zeros(3 * sum(self.nr) * self.nt, np.float32)
+        check(lib().owl_elastic3d_gradient(self._h, None if flat is None else _ptr(flat), _ptr(mis), _ptr(gv),
+                                           _ptr(gs), _ptr(gr), _ptr(tr)))
+        nz, ny, nx = self.shape
+        g = tuple(a.reshape(nx, ny, nz).transpose(2, 1, 0).copy() for a in (gv, gs, gr))
+        return (mis,) + g + (self._split(tr),)
+
+    def _grad_elems(self):
+        return sum(a * b * c for a, b, c in self._windows)
 
     def set_profiling(self, on=True):
         check(lib().owl_elastic3d_set_profiling(self._h, int(bool(on))))
