@@ -207,13 +207,22 @@ owl_status owl_acoustic2d_observe(owl_acoustic2d *b);
  * stencil grid-point updates it performed. */
 owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double *updates);
 
+/* Forward-step kernel of the exact-adjoint paths (no OWL counterpart;
+ * results are bit-identical either way): mode 0 two sweeps (p, then v),
+ * 1 one fused z-marching kernel per step, 2 fused for batches of 4 shots or
+ * more (default).  zrows: z rows per fused tile (default 32).  The fused
+ * kernel needs distinct input and output states and no free surface;
+ * elsewhere the two sweeps run. */
+owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d *b, int mode, int zrows);
+
 /* Per-kernel timing with HIP events on the batch's stream.  No OWL
  * counterpart (OWL has no profiler); used by the benchmark.  kind:
  * 0 p-sweep and 1 v-sweep over one field set (forward), 2 source/receiver
  * step, 3 imaging condition, 4 boundary injection, 5 adjoint-source
  * injection, 6 misfit, 7 merge, 8 p-sweep and 9 v-sweep over two field sets
  * (reverse pass: reconstruction + adjoint), 10 exact-adjoint p-pass,
- * 11 exact-adjoint v-pass, 12 state storage, 0 <= kind < 16.
+ * 11 exact-adjoint v-pass, 12 state storage, 13 fused forward step,
+ * 0 <= kind < 16.
  * Totals since profiling was switched on: device time, launch count, and
  * algorithmic (compulsory) bytes of those launches (0 if not modelled). */
 owl_status owl_acoustic2d_set_profiling(owl_acoustic2d *b, int on);

@@ -135,7 +135,14 @@ class Acoustic2DBatch:
 
     KERNELS = ("update_p", "update_v", "post_step", "image", "inject_strips", "inject_adjoint",
                "misfit", "merge", "update_p_2sets", "update_v_2sets", "adjoint_p", "adjoint_v",
-               "store")
+               "store", "step_fused")
+
+    def set_step_kernel(self, mode="auto", zrows=32):
+        """Forward-step kernel of the exact-adjoint paths: "sweeps" (p then
+        v), "fused" (one z-marching kernel per step) or "auto" (fused for 4
+        shots or more).  Results are bit-identical."""
+        m = {"sweeps": 0, "fused": 1, "auto": 2}[mode]
+        check(lib().owl_acoustic2d_set_step_kernel(self._h, m, int(zrows)))
 
     def kernel_stats(self):
         out = {}

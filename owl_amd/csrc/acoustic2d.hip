@@ -53,6 +53,7 @@ struct Shot {
     long long g0;         // offset of the shot's window gradient (nx*nz)
     int edge0;            // offset of the shot's saved edge values (2*nz + 2*nx)
     int nxbeg, nzbeg;     // window origin in the full model (1-based)
+    int ftile0, fntx;     // fused step: first column-strip tile, strips per row
 };
 
 // One set of wavefields for a sweep.  Inputs and outputs may alias (in
@@ -221,6 +222,142 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
             const float reff = 0.5f * (rho[q] + rho[q - pitch]);
             F.vz_out[q] = F.vz_in[q] - dt / reff * d;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward step (p-sweep, then v-sweep with the new p) for the
+// exact-adjoint path, out of place, no free surface.  One wavefront per
+// tile: 64 lanes own 64 consecutive columns c = x0 - R + lane and march down
+// z over the tile's rows [z0, z1], the front row r running R - 1 rows ahead
+// of the v rows.  The new p of the last 2R rows stays in a register queue
+// (the z-stencil of vz), its x-neighbours come from the other lanes
+// (shuffles), and vz_in's z-stencil is a register queue too.  Lanes
+// 0..R-1 and 65-R..63 compute p redundantly for their neighbours' vx; only
+// lanes R..64-R write, and only the tile's own rows.  Every input must be
+// distinct from every output (the redundant lanes read what a neighbouring
+// tile may already have written); arithmetic as k_update_p / k_update_v,
+// so the results are bit-identical to the two-sweep form.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ shots,
+                                                    const int* __restrict__ ftile_shot, int nftile,
+                                                    FieldSet F, const float* __restrict__ bk,
+                                                    const float* __restrict__ rho,
+                                                    const float* __restrict__ prof, Coef c, float dt,
+                                                    float idx, float idz, int pml, int zc) {
+    constexpr int W = 64 - (2 * R - 1);
+    const int tile = int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y);
+    if (tile >= nftile) return;
+    const Shot S = shots[ftile_shot[tile]];
+    const int lt = tile - S.ftile0;
+    const int tx = lt % S.fntx, tz = lt / S.fntx;
+    const int l = int(threadIdx.x);
+    const int nx = S.nx, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int lx = nx + 2 * pml, lz = nz + 2 * pml;
+    const int col = -pml + 1 + tx * W - R + l;
+    const int z0 = -pml + 1 + tz * zc, z1 = min(z0 + zc - 1, nz + pml);
+    const bool pcol = col >= -pml + 1 && col <= nx + pml;     // p is updated in this column
+    const bool mcol = col <= nx + pml + R;                     // column inside the padded grid
+    const bool own = l >= R && l <= 64 - R && col <= nx + pml;
+    const float* axi = prof + S.prof;
+    const float* bxi = axi + lx;
+    const float* axh = prof + S.prof + 2 * lx;
+    const float* bxh = axh + lx;
+    const float* azi = prof + S.prof + 4 * lx;
+    const float* bzi = azi + lz;
+    const float* azh = azi + 2 * lz;
+    const float* bzh = azh + lz;
+    const bool xs_i = col <= 0 || col >= nx + 1, xs_h = col <= 1 || col >= nx + 1;
+    float ax = 0.f, bx = 0.f, axh_ = 0.f, bxh_ = 0.f;
+    if (pcol && xs_i) {
+        ax = axi[col + pml - 1];
+        bx = bxi[col + pml - 1];
+    }
+    if (own && xs_h) {
+        axh_ = axh[col + pml - 1];
+        bxh_ = bxh[col + pml - 1];
+    }
+    const size_t pitch = size_t(S.pitch);
+    const size_t base = size_t(S.off) + size_t(col - lo);  // (col, lo)
+    auto vz_row = [&](int row) -> float {
+        return (mcol && row >= lo && row <= nz + pml + R) ? F.vz_in[base + size_t(row - lo) * pitch] : 0.0f;
+    };
+    // V[k] = vz_in at row r - R + 1 + k;  P[k] = new p at row r - 2R + 1 + k
+    float V[2 * R], P[2 * R];
+    const int rs = z0 - R;
+#pragma unroll
+    for (int k = 0; k < 2 * R - 1; ++k) V[k] = vz_row(rs - R + 1 + k);
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) P[k] = 0.0f;
+    for (int r = rs; r <= z1 + R - 1; ++r) {
+        V[2 * R - 1] = vz_row(r + R);
+        // ---- p at (col, r)
+        float pn = 0.0f;
+        if (pcol && r >= -pml + 1 && r <= nz + pml) {
+            const size_t q = base + size_t(r - lo) * pitch;
+            float sx = c.c[0] * (F.vx_in[q + 1] - F.vx_in[q]);
+            float sz = c.c[0] * (V[R] - V[R - 1]);
+#pragma unroll
+            for (int m = 2; m <= R; ++m) {
+                sx = sx + c.c[m - 1] * (F.vx_in[q + m] - F.vx_in[q - (m - 1)]);
+                sz = sz + c.c[m - 1] * (V[R - 1 + m] - V[R - m]);
+            }
+            float pdx = idx * sx;
+            float pdz = idz * sz;
+            const bool wr = own && r >= z0 && r <= z1;
+            if (xs_i) {
+                const float mem = ax * pdx + bx * F.m_in[0][q];
+                if (wr) F.m_out[0][q] = mem;
+                pdx = pdx + mem;
+            }
+            if (r <= 0 || r >= nz + 1) {
+                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * F.m_in[1][q];
+                if (wr) F.m_out[1][q] = mem;
+                pdz = pdz + mem;
+            }
+            pn = F.p_in[q] - dt * bk[q] * (pdx + pdz);
+            if (wr) F.p_out[q] = pn;
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * R - 1; ++k) P[k] = P[k + 1];
+        P[2 * R - 1] = pn;
+        // ---- v at row j = r - R + 1
+        const int j = r - R + 1;
+        if (j >= z0) {
+            float px[2 * R];  // new p of row j at columns col - R .. col + R - 1
+#pragma unroll
+            for (int k = 0; k < 2 * R; ++k) px[k] = __shfl(P[R], l - R + k, 64);
+            if (own) {
+                const size_t q = base + size_t(j - lo) * pitch;
+                float s = c.c[0] * (px[R] - px[R - 1]);
+#pragma unroll
+                for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (px[R + m - 1] - px[R - m]);
+                float d = idx * s;
+                if (xs_h) {
+                    const float mem = axh_ * d + bxh_ * F.m_in[2][q];
+                    F.m_out[2][q] = mem;
+                    d = d + mem;
+                }
+                const float rq = rho[q];
+                float reff = 0.5f * (rq + rho[q - 1]);
+                F.vx_out[q] = F.vx_in[q] - dt / reff * d;
+                s = c.c[0] * (P[R] - P[R - 1]);
+#pragma unroll
+                for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (P[R + m - 1] - P[R - m]);
+                d = idz * s;
+                if (j <= 1 || j >= nz + 1) {
+                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * F.m_in[3][q];
+                    F.m_out[3][q] = mem;
+                    d = d + mem;
+                }
+                reff = 0.5f * (rq + rho[q - pitch]);
+                F.vz_out[q] = V[0] - dt / reff * d;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * R - 1; ++k) V[k] = V[k + 1];
     }
 }
 
@@ -817,6 +954,12 @@ struct owl_acoustic2d {
     DevBuf<float> d_vpw, d_rhow;      // window models (x fastest, per shot at g0)
     // exact adjoint: second working set, adjoint state, FP64 accumulators
     DevBuf<float> fM2[4];
+    DevBuf<float> fC[3], fM3[4];      // third working set (fused steps ping-pong)
+    // fused forward step: 0 two sweeps, 1 fused, 2 fused for batches of
+    // 4 shots or more; z rows per tile
+    int step_kernel = 2, fzc = 32, nftile = 0;
+    std::vector<int> ftile_shot;
+    DevBuf<int> d_ftile_shot;
     DevBuf<float> adj[11];            // ph vxh vzh psx psz phx phz ex ez dx dz
     DevBuf<double> gK, gX, gZ;
     struct Slots {                    // stored forward states
@@ -889,6 +1032,22 @@ void upload_model(owl_acoustic2d* b) {
     b->d_vpw.upload(vpw, b->stream);
     b->d_rhow.upload(rhow, b->stream);
     OWL_HIP(hipStreamSynchronize(b->stream));
+}
+
+// Column-strip tiles of the fused step: 64 - (2R - 1) output columns and
+// fzc rows each, one wavefront per tile.
+void fused_tiling(owl_acoustic2d* b) {
+    const int W = 64 - (2 * b->R - 1);
+    b->ftile_shot.clear();
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        Shot& S = b->sd[s];
+        S.ftile0 = int(b->ftile_shot.size());
+        S.fntx = (S.nx + 2 * b->pml + W - 1) / W;
+        const int nzc = (S.nz + 2 * b->pml + b->fzc - 1) / b->fzc;
+        b->ftile_shot.insert(b->ftile_shot.end(), size_t(S.fntx) * nzc, int(s));
+    }
+    b->nftile = int(b->ftile_shot.size());
+    if (b->ftile_shot.empty()) b->ftile_shot.push_back(0);
 }
 
 void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, const float* rho,
@@ -1139,8 +1298,10 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         OWL_HIP(hipEventCreate(&b->ev1));
     }
     hipStream_t st = b->stream;
+    fused_tiling(b);
     b->d_shots.upload(b->sd, st);
     b->d_tile_shot.upload(b->tile_shot, st);
+    b->d_ftile_shot.upload(b->ftile_shot, st);
     b->d_prof.upload(profs, st);
     if (amp.empty()) amp.push_back(0.f);
     b->d_amp.upload(amp, st);
@@ -1180,7 +1341,7 @@ Sets one_set(float* p, float* vx, float* vz, float* const* m) {
 }
 
 enum KernelKind { kKP = 0, kKV = 1, kKPost = 2, kKImage = 3, kKStrips = 4, kKAdj = 5, kKMisfit = 6, kKMerge = 7,
-                  kKP2 = 8, kKV2 = 9, kKinds = 16 };
+                  kKP2 = 8, kKV2 = 9, kKFused = 13, kKinds = 16 };
 
 // Begin/end a timed region around one launch when profiling is on.
 int prof_begin(owl_acoustic2d* b, int kind, double bytes = 0.0) {
@@ -1400,6 +1561,35 @@ W7 work2(owl_acoustic2d* b) {
     return W7{{b->fB[0].get(), b->fB[1].get(), b->fB[2].get(), b->fM2[0].get(), b->fM2[1].get(),
                b->fM2[2].get(), b->fM2[3].get()}};
 }
+W7 work3(owl_acoustic2d* b) {
+    return W7{{b->fC[0].get(), b->fC[1].get(), b->fC[2].get(), b->fM3[0].get(), b->fM3[1].get(),
+               b->fM3[2].get(), b->fM3[3].get()}};
+}
+// The other half of the w1/w3 ping-pong.
+W7 other_of(owl_acoustic2d* b, const W7& w) { return w.f[0] == b->fA[0].get() ? work3(b) : work1(b); }
+
+// Algorithmic bytes of one fused step over all shots: p, vx, vz, bk, rho
+// read, p, vx, vz written (32 B/point), plus 8 B per slab cell and damped
+// direction (memory variable read + write) for the p and the v parts.
+double fused_bytes(const owl_acoustic2d* b) {
+    double by = 0;
+    for (auto& S : b->sd) {
+        const double lx = S.nx + 2.0 * b->pml, lz = S.nz + 2.0 * b->pml;
+        const double slab_p = 2.0 * b->pml * lz + 2.0 * b->pml * lx;
+        const double slab_v = (2.0 * b->pml + 1) * lz + (2.0 * b->pml + 1) * lx;
+        by += 32.0 * lx * lz + 8.0 * (slab_p + slab_v);
+    }
+    return by;
+}
+
+bool use_fused(const owl_acoustic2d* b, const W7& in, const W7& out) {
+    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
+    if (b->step_kernel == 2 && b->sd.size() < 4) return false;
+    for (int k = 0; k < 7; ++k)
+        for (int m = 0; m < 7; ++m)
+            if (in.f[k] == out.f[m]) return false;
+    return true;
+}
 
 // One forward step t from `in` into `out` (in place when they alias):
 // sweeps, then source injection of step t (and receivers of step t+1 when
@@ -1419,7 +1609,15 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
         F.m_out[k] = out.f[3 + k];
     }
     sets.s[1] = F;
-    launch_step<R>(b, sets, 1, b->dt);
+    if (use_fused(b, in, out)) {
+        const int ev = prof_begin(b, kKFused, b->profiling ? fused_bytes(b) : 0.0);
+        hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
+                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
+                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
+        prof_end(b, ev);
+    } else {
+        launch_step<R>(b, sets, 1, b->dt);
+    }
     const int ev = prof_begin(b, kKPost);
     hipLaunchKernelGGL(k_post_step<R>, dim3(unsigned(b->sd.size())), dim3(512), 0, b->stream, b->d_shots.get(),
                        out.f[0], out.f[1], out.f[2], b->d_inj_beg.get(), b->d_inj_taps.get(), b->d_amp.get(),
@@ -1590,9 +1788,10 @@ struct RevCtx {
     std::function<void()> on_first_adjoint;
 };
 
-// Where a forward state is: kW1 = the working set w1, kZero = the zero
-// initial state (materialised by clearing w1), >= 0 = snapshot slot.
-constexpr int kW1 = -2, kZero = -1;
+// Where a forward state is: kW1 / kW3 = the working sets w1 / w3 (the
+// steps ping-pong between them), kZero = the zero initial state
+// (materialised by clearing w1), >= 0 = snapshot slot.
+constexpr int kW1 = -2, kZero = -1, kW3 = -3;
 
 W7 slot_set(owl_acoustic2d* b, int slot) {
     W7 w;
@@ -1603,24 +1802,33 @@ W7 slot_set(owl_acoustic2d* b, int slot) {
 // The state at `loc` as a readable set (clearing w1 for the zero state).
 W7 state_at(owl_acoustic2d* b, int loc) {
     if (loc >= 0) return slot_set(b, loc);
+    if (loc == kW3) return work3(b);
     const W7 w1 = work1(b);
     if (loc == kZero) zero_set(b, w1);
     return w1;
 }
 
-// Advance the state u_from (at `loc`) to u_to.  In direct mode the first
-// step reads the snapshot in place and, when `into` >= 0, the last step
-// writes snapshot `into` directly; otherwise the state is restored into w1
-// and saved by copy.  Returns where u_to is.
+int loc_of(owl_acoustic2d* b, const W7& w) { return w.f[0] == b->fA[0].get() ? kW1 : kW3; }
+
+// Advance the state u_from (at `loc`) to u_to, every step out of place
+// (w1 <-> w3).  In direct mode the first step reads the snapshot in place
+// and, when `into` >= 0, the last step writes snapshot `into` directly;
+// otherwise the state is restored into w1 and saved by copy.  Returns
+// where u_to is.
 template <int R>
 int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record, bool direct) {
-    const W7 w1 = work1(b);
     if (!direct) {
-        if (loc >= 0) slot_load(b, b->snaps, loc, w1.f);
-        else if (loc == kZero) zero_set(b, w1);
-        for (int t = from + 1; t <= to; ++t) fwd_step<R>(b, w1, w1, t, record);
-        if (into >= 0) slot_save(b, b->snaps, into, w1.f);
-        return kW1;
+        W7 cur = work1(b);
+        if (loc >= 0) slot_load(b, b->snaps, loc, cur.f);
+        else if (loc == kZero) zero_set(b, cur);
+        else if (loc == kW3) cur = work3(b);
+        for (int t = from + 1; t <= to; ++t) {
+            const W7 out = other_of(b, cur);
+            fwd_step<R>(b, cur, out, t, record);
+            cur = out;
+        }
+        if (into >= 0) slot_save(b, b->snaps, into, cur.f);
+        return loc_of(b, cur);
     }
     if (to == from) {
         if (into >= 0 && into != loc) {
@@ -1631,12 +1839,16 @@ int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record,
         return loc;
     }
     W7 cur = state_at(b, loc);
+    bool in_slot = loc >= 0;
     for (int t = from + 1; t <= to; ++t) {
-        const W7 out = (t == to && into >= 0) ? slot_set(b, into) : w1;
+        W7 out;
+        if (t == to && into >= 0) out = slot_set(b, into);
+        else out = in_slot ? work1(b) : other_of(b, cur);
         fwd_step<R>(b, cur, out, t, record);
         cur = out;
+        in_slot = false;
     }
-    return into >= 0 ? into : kW1;
+    return into >= 0 ? into : loc_of(b, cur);
 }
 
 // Reverse sweep over steps b..a+1: u_a is stored at `slot` (snapshot, or
@@ -1682,6 +1894,8 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     const int N = b->nt - 1;  // forward steps
     for (int k = 0; k < 3; ++k) b->fB[k].alloc(b->field_elems);
     for (int k = 0; k < 4; ++k) b->fM2[k].alloc(b->field_elems);
+    for (int k = 0; k < 3; ++k) b->fC[k].alloc(b->field_elems);
+    for (int k = 0; k < 4; ++k) b->fM3[k].alloc(b->field_elems);
     const W7 w1 = work1(b), w2 = work2(b);
     OWL_HIP(hipEventRecord(b->ev0, st));
     const bool do_fwd = phase != kReverseOnly, do_rev = phase != kForwardOnly;
@@ -1693,27 +1907,36 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
         zero_set(b, w2);
         b->d_traces.zero(st);
     }
+    // w3 is a step target: its halo must read as zero, its memory variables
+    // start from zero wherever a schedule first writes them
+    zero_set(b, work3(b));
     double fwd_steps = 0, adj_steps = 0;
     if (ckpt == OWL_CKPT_FULL) {
         if (do_fwd) {
             slots_alloc(b, b->full, N + 1, 3, fmt);
             if (fmt == OWL_STORE_F32) {
                 // each step reads slot t-1 and writes slot t directly (memory
-                // variables in place in w1); the slots' outer halo stays zero
+                // variables alternate between w1 and w3); the slots' outer
+                // halo stays zero
                 OWL_HIP(hipMemsetAsync(b->full.f32.get(), 0, size_t(N + 1) * 3 * b->field_elems * sizeof(float), st));
+                W7 mA = w1, mB = work3(b);
                 for (int t = 1; t <= N; ++t) {
-                    W7 in = w1, out = w1;
+                    W7 in = mA, out = mB;
                     for (int k = 0; k < 3; ++k) {
                         in.f[k] = const_cast<float*>(slot_ptr(b, b->full, t - 1, k));
                         out.f[k] = const_cast<float*>(slot_ptr(b, b->full, t, k));
                     }
                     fwd_step<R>(b, in, out, t, true);
+                    std::swap(mA, mB);
                 }
             } else {
                 slot_save(b, b->full, 0, w1.f);
+                W7 cur = w1;
                 for (int t = 1; t <= N; ++t) {
-                    fwd_step<R>(b, w1, w1, t, true);
-                    slot_save(b, b->full, t, w1.f);
+                    const W7 out = other_of(b, cur);
+                    fwd_step<R>(b, cur, out, t, true);
+                    slot_save(b, b->full, t, out.f);
+                    cur = out;
                 }
             }
             fwd_steps += N;
@@ -1744,9 +1967,12 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
         const int nseg = (N + k - 1) / k;
         if (do_fwd) {
             slots_alloc(b, b->snaps, nseg, 7, fmt);
+            W7 cur = w1;
             for (int t = 0; t < N; ++t) {
-                if (t % k == 0) slot_save(b, b->snaps, t / k, w1.f);
-                fwd_step<R>(b, w1, w1, t + 1, true);
+                if (t % k == 0) slot_save(b, b->snaps, t / k, cur.f);
+                const W7 out = other_of(b, cur);
+                fwd_step<R>(b, cur, out, t + 1, true);
+                cur = out;
             }
             fwd_steps += N;
         }
@@ -1764,12 +1990,15 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
             float* into[7] = {const_cast<float*>(slot_ptr(b, b->seg, 0, 0)), const_cast<float*>(slot_ptr(b, b->seg, 0, 1)),
                               const_cast<float*>(slot_ptr(b, b->seg, 0, 2)), w1.f[3], w1.f[4], w1.f[5], w1.f[6]};
             slot_load(b, b->snaps, sgi, into);
+            // memory variables alternate between w1 and w3
+            W7 mA = w1, mB = work3(b);
             for (int t = a + 1; t <= e; ++t) {
                 W7 in{{const_cast<float*>(slot_ptr(b, b->seg, t - 1 - a, 0)), const_cast<float*>(slot_ptr(b, b->seg, t - 1 - a, 1)),
-                       const_cast<float*>(slot_ptr(b, b->seg, t - 1 - a, 2)), w1.f[3], w1.f[4], w1.f[5], w1.f[6]}};
+                       const_cast<float*>(slot_ptr(b, b->seg, t - 1 - a, 2)), mA.f[3], mA.f[4], mA.f[5], mA.f[6]}};
                 W7 out{{const_cast<float*>(slot_ptr(b, b->seg, t - a, 0)), const_cast<float*>(slot_ptr(b, b->seg, t - a, 1)),
-                        const_cast<float*>(slot_ptr(b, b->seg, t - a, 2)), w1.f[3], w1.f[4], w1.f[5], w1.f[6]}};
+                        const_cast<float*>(slot_ptr(b, b->seg, t - a, 2)), mB.f[3], mB.f[4], mB.f[5], mB.f[6]}};
                 fwd_step<R>(b, in, out, t, false);
+                std::swap(mA, mB);
             }
             fwd_steps += e - a;
             for (int t = e; t >= a + 1; --t) {
@@ -2115,6 +2344,20 @@ owl_status owl_acoustic2d_observe(owl_acoustic2d* b) {
         OWL_HIP(hipStreamSynchronize(b->stream));
         b->have_obs = true;
         b->fwd_state_valid = false;
+    });
+}
+
+owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d* b, int mode, int zrows) {
+    return guarded([&] {
+        if (!b || mode < 0 || mode > 2 || zrows < 1) fail(OWL_ERR_ARG, "bad step kernel choice");
+        b->step_kernel = mode;
+        if (zrows != b->fzc) {
+            b->fzc = zrows;
+            fused_tiling(b);
+            b->d_shots.upload(b->sd, b->stream);
+            b->d_ftile_shot.upload(b->ftile_shot, b->stream);
+            OWL_HIP(hipStreamSynchronize(b->stream));
+        }
     });
 }
 

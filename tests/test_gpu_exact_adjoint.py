@@ -144,3 +144,49 @@ def test_autograd_backward_equals_gradient(oracle32, gpu):
     assert rel_l2(tv.grad.cpu().numpy(), gv0) < 1e-6
     assert rel_l2(tr_.grad.cpu().numpy(), gr0) < 1e-6
     b.close()
+
+
+@pytest.mark.parametrize("R,zrows", [(4, 32), (2, 7), (8, 16), (5, 1)])
+def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows):
+    """The fused z-marching forward step against the two sweeps (p, then v):
+    five shots on uneven windows, traces, misfits and gradients bit-identical
+    for full FP32 and FP16 storage, uniform and binomial checkpoints."""
+    from owl_amd.synthetic import gardner, random_field
+    nz, nx, dx = 60, 140, 10.0
+    vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 11))).astype(np.float32)
+    rho = gardner(vp)
+    wins = [Window(1, 140, 1, 60, 0.0, 0.0), Window(11, 70, 1, 45, 100.0, 0.0),
+            Window(50, 131, 5, 60, 490.0, 40.0), Window(30, 62, 1, 33, 290.0, 0.0),
+            Window(3, 96, 2, 58, 20.0, 10.0)]
+    geoms = []
+    for k, w in enumerate(wins):
+        x0 = (w.nxbeg - 1) * dx
+        rx = x0 + 5.0 + 10.0 * np.arange(w.nx - 1)
+        sx = x0 + 0.5 * (w.nx - 1) * dx + 3.3
+        sz = (w.nzbeg - 1) * dx + 80.0
+        geoms.append(simple_geometry(k + 1, sx, sz, rx, np.full(len(rx), (w.nzbeg - 1) * dx + 30.0), f0=20.0))
+    pml = max(10, R)
+    b = Acoustic2DBatch(nx, nz, dx, dx, vp, rho, list(zip(geoms, wins)), 0.001, 0.15, pml=pml, fdhalf=R)
+    obs = b.forward()
+    b.set_model((vp * 0.98).astype(np.float32), rho)
+    cases = [(OWL_CKPT_FULL, 0, OWL_STORE_F32), (OWL_CKPT_FULL, 0, OWL_STORE_F16),
+             (OWL_CKPT_UNIFORM, 9, OWL_STORE_F32), (OWL_CKPT_BINOMIAL, 6, OWL_STORE_F32),
+             (OWL_CKPT_BINOMIAL, 5, OWL_STORE_F16)]
+    for ckpt, param, store in cases:
+        print(f"case: fused vs sweeps R={R} zrows={zrows} ckpt={ckpt} param={param} store={store}", flush=True)
+        out = []
+        for mode in ("sweeps", "fused"):
+            b.set_step_kernel(mode, zrows)
+            b.set_profiling(True)
+            out.append(b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store))
+            st = b.kernel_stats()
+            b.set_profiling(False)
+            assert (st["step_fused"]["launches"] > 0) == (mode == "fused")
+        (m0, gv0, gr0, s0), (m1, gv1, gr1, s1) = out
+        assert np.array_equal(m0, m1)
+        for x, y in zip(s0, s1):
+            assert np.array_equal(x, y)
+        assert np.abs(gv0).max() > 0
+        assert np.array_equal(gv0, gv1)
+        assert np.array_equal(gr0, gr1)
+    b.close()
