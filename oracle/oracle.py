@@ -58,6 +58,21 @@ class EGrid3D(C.Structure):
                 ("R", C.c_int), ("free_surface", C.c_int), ("cc_step_interval", C.c_int)]
 
 
+class EGrid2D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_float), ("dz", C.c_float),
+                ("ox", C.c_float), ("oz", C.c_float), ("dt", C.c_float), ("nt", C.c_int),
+                ("pml", C.c_int), ("R", C.c_int), ("cc_step_interval", C.c_int)]
+
+
+class ESource2D(C.Structure):
+    _fields_ = [("x", C.c_float), ("z", C.c_float), ("amp", C.c_float), ("t0", C.c_float),
+                ("polar", C.c_float), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
+
+
+class EReceiver2D(C.Structure):
+    _fields_ = [("x", C.c_float), ("z", C.c_float), ("weight", C.c_float)]
+
+
 class Point(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float),
                 ("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int),
@@ -394,3 +409,74 @@ class Oracle:
         shp = (G.nx, G.ny, G.nz)
         out = tuple(a.reshape(shp[::-1][::-1]).reshape(G.nx, G.ny, G.nz).transpose(2, 1, 0).copy() for a in (gv, gs, gr))
         return out + ((tr[0], tr[1], tr[2]),)
+
+    # ---- 2D isotropic elastic (elastic_vhtiort_2d, force sources, no free surface)
+    def _egrid2(self, g):
+        return EGrid2D(int(g["nx"]), int(g["nz"]), g["dx"], g["dz"], g.get("ox", 0.0), g.get("oz", 0.0),
+                       g["dt"], int(g["nt"]), int(g["pml"]), int(g["R"]), int(g.get("cc_step_interval", 1)))
+
+    def _esources2(self, g, sources):
+        keep = []
+        arr = (ESource2D * max(1, len(sources)))()
+        f0max, hnt = 0.0, 0
+        for k, s in enumerate(sources):
+            stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            stf = np.ascontiguousarray(stf, dtype=self.real)
+            keep.append(stf)
+            arr[k] = ESource2D(s["x"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0), s.get("polar", 0.0),
+                               len(stf), stf.ctypes.data)
+            f0max = max(f0max, s["f0"])
+            if k == 0:
+                hnt = h
+        return arr, keep, f0max, hnt
+
+    def _ereceivers2(self, receivers):
+        rec = (EReceiver2D * max(1, len(receivers)))()
+        for k, r in enumerate(receivers):
+            rec[k] = EReceiver2D(r["x"], r["z"], r.get("weight", 1.0))
+        return rec
+
+    def elastic2d_forward(self, g, vp, vs, rho, sources, receivers):
+        """Velocity traces of OWL's elastic_vhtiort_2d compute_forward for one
+        shot window: (tx, tz), each (nr, nt).  Force sources, no free surface."""
+        G = self._egrid2(g)
+        shape = (G.nz, G.nx)
+        vp, vs, rho = (self._model(a, shape) for a in (vp, vs, rho))
+        arr, keep, f0max, _ = self._esources2(g, sources)
+        rec = self._ereceivers2(receivers)
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("elastic2d_forward")
+        fn.argtypes = [C.POINTER(EGrid2D), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float,
+                       C.c_int, C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), vp.ctypes.data_as(C.c_void_p), vs.ctypes.data_as(C.c_void_p),
+                rho.ctypes.data_as(C.c_void_p), len(sources), C.cast(arr, C.c_void_p), f0max, len(receivers),
+                C.cast(rec, C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle elastic2d_forward failed ({rc})")
+        return tr[0], tr[1]
+
+    def elastic2d_gradient_owl(self, g, vp, vs, rho, sources, receivers, adj):
+        """OWL's isotropic 2D elastic gradient: (grad_vp, grad_vs, grad_rho) as
+        (nz, nx), plus the forward traces (tx, tz).  adj: (2, nr, nt)."""
+        G = self._egrid2(g)
+        shape = (G.nz, G.nx)
+        vp, vs, rho = (self._model(a, shape) for a in (vp, vs, rho))
+        arr, keep, f0max, hnt = self._esources2(g, sources)
+        rec = self._ereceivers2(receivers)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (2, len(receivers), G.nt)
+        n = G.nx * G.nz
+        gv, gs, gr = (np.zeros(n, self.real) for _ in range(3))
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("elastic2d_gradient_owl")
+        fn.argtypes = [C.POINTER(EGrid2D), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float,
+                       C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), vp.ctypes.data_as(C.c_void_p), vs.ctypes.data_as(C.c_void_p),
+                rho.ctypes.data_as(C.c_void_p), len(sources), C.cast(arr, C.c_void_p), f0max,
+                int(g.get("hnt", hnt)), len(receivers), C.cast(rec, C.c_void_p), adj.ctypes.data_as(C.c_void_p),
+                gv.ctypes.data_as(C.c_void_p), gs.ctypes.data_as(C.c_void_p), gr.ctypes.data_as(C.c_void_p),
+                tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle elastic2d_gradient_owl failed ({rc})")
+        out = tuple(a.reshape(G.nx, G.nz).T.copy() for a in (gv, gs, gr))
+        return out + ((tr[0], tr[1]),)
