@@ -333,6 +333,49 @@ owl_status owl_elastic3d_kernel_stats(const owl_elastic3d *b, int kind, double *
                                       long long *launches, double *bytes_total);
 owl_status owl_elastic3d_last_stats(const owl_elastic3d *b, double *ms, double *updates);
 
+/* ---- 2D isotropic elastic: batched shots on one GPU -----------------------
+ * OWL's elastic_vhtiort_2d solver for anisotropy_type = 'iso' (compute_forward,
+ * elastic_vhtiort_2d/submodule_forward_modeling.f90:31-180; update_wavefield
+ * module_wavefield.f90:42-134; CFS-MPML module_cfspml.f90:285-578) and its
+ * gradient (compute_adjoint / compute_gradient, submodule_fwi_gradient.f90;
+ * boundary records module_boundary_saving.f90:53-160), for which_medium =
+ * 'elastic-iso' in src/forward/2d/module_modeling.f90 and
+ * src/inverse/2d/module_gradient.f90.  vp, vs, rho: (nz, nx) z fastest; cfg
+ * as for the acoustic solvers with ny = 1 and no free surface (OWL's 2D
+ * elastic free surface runs on its depth-refined mesh).  Force sources
+ * (polar).  Traces: shot after shot, two blocks (vx, vz) of [nr][nt]. */
+typedef struct owl_elastic2d owl_elastic2d;
+owl_status owl_elastic2d_create(const owl_acoustic_config *cfg, const float *vp, const float *vs,
+                                const float *rho, int nshot, const owl_shot *shots,
+                                owl_elastic2d **out);
+void owl_elastic2d_destroy(owl_elastic2d *b);
+int owl_elastic2d_nt(const owl_elastic2d *b);
+/* save_boundary != 0 keeps the boundary records of every step and the final
+ * state for owl_elastic2d_adjoint. */
+owl_status owl_elastic2d_forward(owl_elastic2d *b, float *traces, int save_boundary);
+/* OWL's reverse pass: adj (trace layout, two components) into the adjoint
+ * velocities; per-shot window gradients grad_vp, grad_vs, grad_rho
+ * (nz_s, nx_s) z fastest, shot after shot (the .grd arrays; chain rule
+ * submodule_fwi_gradient.f90:330-360), NaN/Inf -> 0.  Needs a preceding
+ * owl_elastic2d_forward(..., save_boundary = 1). */
+owl_status owl_elastic2d_adjoint(owl_elastic2d *b, const float *adj, float *grad_vp, float *grad_vs,
+                                 float *grad_rho);
+/* Forward with boundary saving, L2 waveform adjoint source per component
+ * (module_inversion_adjoint_source.f90:1020-1125), reverse pass, sum of the
+ * shots' gradients on the full model (module_gradient.f90:366-388), FP64.
+ * misfit: one value per shot (sum over the two components). */
+owl_status owl_elastic2d_gradient(owl_elastic2d *b, const float *obs, double *misfit, double *grad_vp,
+                                  double *grad_vs, double *grad_rho, float *traces_out);
+/* Launch shape (no OWL counterpart; results do not depend on it): z rows
+ * per tile and 64-thread tiles per block (1, 2 or 4). */
+owl_status owl_elastic2d_set_launch(owl_elastic2d *b, int planes, int rows);
+/* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
+ * sweep, 2 sources and receivers, 3 imaging, 4 boundary records. */
+owl_status owl_elastic2d_set_profiling(owl_elastic2d *b, int on);
+owl_status owl_elastic2d_kernel_stats(const owl_elastic2d *b, int kind, double *total_ms,
+                                      long long *launches, double *bytes_total);
+owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *updates);
+
 #ifdef __cplusplus
 }
 #endif

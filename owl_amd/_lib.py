@@ -105,6 +105,16 @@ _SIGNATURES = {
     "owl_acoustic3d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_acoustic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_acoustic3d_last_stats": (C.c_int, [_P, _P, _P]),
+    "owl_elastic2d_create": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
+    "owl_elastic2d_destroy": (None, [_P]),
+    "owl_elastic2d_nt": (C.c_int, [_P]),
+    "owl_elastic2d_forward": (C.c_int, [_P, _P, C.c_int]),
+    "owl_elastic2d_adjoint": (C.c_int, [_P, _P, _P, _P, _P]),
+    "owl_elastic2d_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "owl_elastic2d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_elastic2d_set_profiling": (C.c_int, [_P, C.c_int]),
+    "owl_elastic2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "owl_elastic2d_last_stats": (C.c_int, [_P, _P, _P]),
     "owl_elastic3d_create": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
     "owl_elastic3d_destroy": (None, [_P]),
     "owl_elastic3d_nt": (C.c_int, [_P]),

@@ -281,43 +281,76 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
     }
     const size_t pitch = size_t(S.pitch);
     const size_t base = size_t(S.off) + size_t(col - lo);  // (col, lo)
+    // Inputs of one loop iteration: the p row r and the v row j = r - R + 1.
+    struct Row {
+        float vx[2 * R];  // vx_in at row r, columns col - R + 1 .. col + R
+        float vzf;        // vz_in at row r + R (the front of the vz queue)
+        float p, bk, m0, m1;
+        float rq, rl, ru, vxj, m2, m3;  // rho (j), rho (j, col - 1), rho (j - 1), vx_in (j)
+    };
+    const int rs = z0 - R, re = z1 + R - 1;
+    auto load = [&](int r, Row& w) {
+        const int vr = r + R;
+        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? F.vz_in[base + size_t(vr - lo) * pitch] : 0.0f;
+        if (pcol && r >= -pml + 1 && r <= nz + pml) {
+            const size_t q = base + size_t(r - lo) * pitch;
+#pragma unroll
+            for (int k = 0; k < 2 * R; ++k) w.vx[k] = F.vx_in[q - (R - 1) + k];
+            w.p = F.p_in[q];
+            w.bk = bk[q];
+            w.m0 = xs_i ? F.m_in[0][q] : 0.0f;
+            w.m1 = (r <= 0 || r >= nz + 1) ? F.m_in[1][q] : 0.0f;
+        }
+        const int j = r - R + 1;
+        if (own && j >= z0) {
+            const size_t q = base + size_t(j - lo) * pitch;
+            w.rq = rho[q];
+            w.rl = rho[q - 1];
+            w.ru = rho[q - pitch];
+            w.vxj = F.vx_in[q];
+            w.m2 = xs_h ? F.m_in[2][q] : 0.0f;
+            w.m3 = (j <= 1 || j >= nz + 1) ? F.m_in[3][q] : 0.0f;
+        }
+    };
     auto vz_row = [&](int row) -> float {
         return (mcol && row >= lo && row <= nz + pml + R) ? F.vz_in[base + size_t(row - lo) * pitch] : 0.0f;
     };
     // V[k] = vz_in at row r - R + 1 + k;  P[k] = new p at row r - 2R + 1 + k
     float V[2 * R], P[2 * R];
-    const int rs = z0 - R;
 #pragma unroll
     for (int k = 0; k < 2 * R - 1; ++k) V[k] = vz_row(rs - R + 1 + k);
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) P[k] = 0.0f;
-    for (int r = rs; r <= z1 + R - 1; ++r) {
-        V[2 * R - 1] = vz_row(r + R);
+    Row cur, nxt;
+    load(rs, cur);
+    for (int r = rs; r <= re; ++r) {
+        if (r < re) load(r + 1, nxt);  // in flight while row r is computed
+        V[2 * R - 1] = cur.vzf;
         // ---- p at (col, r)
         float pn = 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
             const size_t q = base + size_t(r - lo) * pitch;
-            float sx = c.c[0] * (F.vx_in[q + 1] - F.vx_in[q]);
+            float sx = c.c[0] * (cur.vx[R] - cur.vx[R - 1]);
             float sz = c.c[0] * (V[R] - V[R - 1]);
 #pragma unroll
             for (int m = 2; m <= R; ++m) {
-                sx = sx + c.c[m - 1] * (F.vx_in[q + m] - F.vx_in[q - (m - 1)]);
+                sx = sx + c.c[m - 1] * (cur.vx[R - 1 + m] - cur.vx[R - m]);
                 sz = sz + c.c[m - 1] * (V[R - 1 + m] - V[R - m]);
             }
             float pdx = idx * sx;
             float pdz = idz * sz;
             const bool wr = own && r >= z0 && r <= z1;
             if (xs_i) {
-                const float mem = ax * pdx + bx * F.m_in[0][q];
+                const float mem = ax * pdx + bx * cur.m0;
                 if (wr) F.m_out[0][q] = mem;
                 pdx = pdx + mem;
             }
             if (r <= 0 || r >= nz + 1) {
-                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * F.m_in[1][q];
+                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * cur.m1;
                 if (wr) F.m_out[1][q] = mem;
                 pdz = pdz + mem;
             }
-            pn = F.p_in[q] - dt * bk[q] * (pdx + pdz);
+            pn = cur.p - dt * cur.bk * (pdx + pdz);
             if (wr) F.p_out[q] = pn;
         }
 #pragma unroll
@@ -336,28 +369,28 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (px[R + m - 1] - px[R - m]);
                 float d = idx * s;
                 if (xs_h) {
-                    const float mem = axh_ * d + bxh_ * F.m_in[2][q];
+                    const float mem = axh_ * d + bxh_ * cur.m2;
                     F.m_out[2][q] = mem;
                     d = d + mem;
                 }
-                const float rq = rho[q];
-                float reff = 0.5f * (rq + rho[q - 1]);
-                F.vx_out[q] = F.vx_in[q] - dt / reff * d;
+                float reff = 0.5f * (cur.rq + cur.rl);
+                F.vx_out[q] = cur.vxj - dt / reff * d;
                 s = c.c[0] * (P[R] - P[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (P[R + m - 1] - P[R - m]);
                 d = idz * s;
                 if (j <= 1 || j >= nz + 1) {
-                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * F.m_in[3][q];
+                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * cur.m3;
                     F.m_out[3][q] = mem;
                     d = d + mem;
                 }
-                reff = 0.5f * (rq + rho[q - pitch]);
+                reff = 0.5f * (cur.rq + cur.ru);
                 F.vz_out[q] = V[0] - dt / reff * d;
             }
         }
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) V[k] = V[k + 1];
+        cur = nxt;
     }
 }
 
