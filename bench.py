@@ -76,9 +76,15 @@ def kernel_table(stats, steps):
 
 
 def time_gradient(b, steps, warmup, sync, **kw):
+    """Warm up (an exact gradient runs eagerly once, then its HIP graph is
+    captured on the next call), time `steps` gradients with profiling off
+    (graph replays), then run one more gradient eagerly with per-kernel HIP
+    events for the roofline and the kernel table."""
+    from owl_amd import OWL_GRAD_EXACT
+    if kw.get("mode") == OWL_GRAD_EXACT:
+        warmup = max(warmup, 2)
     for _ in range(warmup):
         b.gradient(want_traces=False, **kw)
-    b.set_profiling(True)
     sync()
     t0 = time.perf_counter()
     updates = 0.0
@@ -87,6 +93,8 @@ def time_gradient(b, steps, warmup, sync, **kw):
         updates += b.last_stats()[1]
     sync()
     wall = time.perf_counter() - t0
+    b.set_profiling(True)
+    b.gradient(want_traces=False, **kw)
     stats = b.kernel_stats()
     b.set_profiling(False)
     return wall, updates, stats, gv
@@ -157,6 +165,12 @@ def main():
                     help="shots timed through the plain per-shot path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the batched exact gradient (for profiler runs)")
+    ap.add_argument("--eager", action="store_true", help="no HIP graphs (for counter runs)")
+    ap.add_argument("--only-3d", action="store_true", help="only the 3D batched forward (for profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device five times (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -168,10 +182,22 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(5):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return
     from owl_amd import (OWL_CKPT_BINOMIAL, OWL_CKPT_FULL, OWL_GRAD_EXACT, OWL_GRAD_OWL, OWL_STORE_F16,
                          OWL_STORE_F32)
     sync = torch.cuda.synchronize
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.only_3d:
+        print(json.dumps({"acoustic3d_forward": bench_3d(sync)}))
+        return
     log("building workload")
     vp, rho, vp0, rho0, shots = workload()
     nshot = len(shots)
@@ -179,12 +205,17 @@ def main():
 
     # ---- batched + checkpointed (headline)
     b = make_batch(vp, rho, vp0, rho0, shots)
+    b.set_graphs(not args.eager)
     nt = b.nt
     log("batched exact gradient, binomial 40 snapshots, FP32")
     wall, updates, stats, g32 = time_gradient(b, args.steps, args.warmup, sync, **exact_b40)
     value = updates / wall
     s_shot = wall / args.steps / nshot
     name, roof = roofline_of(stats)
+    if args.headline_only:
+        print(json.dumps({"value": value, "s_per_shot": s_shot, "roofline": roof, "kernels": kernel_table(stats, 1)}))
+        b.close()
+        return
     log("batched exact gradient, binomial 40 snapshots, FP16")
     w16, _, st16, g16 = time_gradient(b, 1, 1, sync, **dict(exact_b40, store=OWL_STORE_F16))
     log("batched OWL-algorithm gradient")
@@ -210,6 +241,7 @@ def main():
     plain, plain_owl = [], []
     for k in range(min(args.plain_shots, nshot)):
         bs = make_batch(vp, rho, vp0, rho0, [shots[k]])
+        bs.set_graphs(False)   # the plain path launches eagerly
         if k == 0:
             bs.gradient(want_traces=False, mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_FULL, store=OWL_STORE_F32)
             bs.gradient(want_traces=False, mode=OWL_GRAD_OWL)
@@ -259,13 +291,15 @@ def main():
         "batched_fp16_checkpoints": {"s_per_shot": w16 / nshot,
                                      "gradient_rel_l2_vs_fp32": rel16},
         "plain_per_shot": {"s_per_shot": plain_s, "shots_timed": len(plain),
-                           "path": "one shot per launch, exact adjoint, full FP32 storage"},
+                           "path": "one shot per launch, exact adjoint, full FP32 storage, eager launches"},
         "speedup_batched_ckpt_vs_plain": plain_s / s_shot,
         "owl_algorithm": {"batched_s_per_shot": wowl / nshot, "plain_s_per_shot": plain_owl_s,
                           "speedup": plain_owl_s / (wowl / nshot)},
         "autograd_s_per_shot": w_autograd / nshot,
         "roofline": dict(roof, traffic=None),
-        "kernels": kernel_table(stats, args.steps),
+        "kernels": kernel_table(stats, 1),
+        "timing": "value and ms_per_step: graph-replayed gradients; roofline and kernels: HIP events "
+                  "of one eager gradient run right after the timed region",
         "acoustic3d_forward": three_d,
         "cpu_baseline": cpu,
     }

@@ -210,10 +210,18 @@ owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double
 /* Forward-step kernel of the exact-adjoint paths (no OWL counterpart;
  * results are bit-identical either way): mode 0 two sweeps (p, then v),
  * 1 one fused z-marching kernel per step, 2 fused for batches of 4 shots or
- * more (default).  zrows: z rows per fused tile (default 32).  The fused
- * kernel needs distinct input and output states and no free surface;
- * elsewhere the two sweeps run. */
+ * more (default), 3 fused with the inputs of two rows ahead in flight.
+ * zrows: z rows per fused tile (default 16).  The fused kernel needs
+ * distinct input and output states and no free surface; elsewhere the two
+ * sweeps run. */
 owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d *b, int mode, int zrows);
+
+/* HIP graphs for owl_acoustic2d_gradient in OWL_GRAD_EXACT mode (no OWL
+ * counterpart; results are bit-identical): on (default), the second call
+ * with the same schedule and buffers captures the whole gradient (storage
+ * schedule, steps, residual, fold) into a graph that later calls replay;
+ * calls with profiling on run eagerly.  Turning it off drops the graph. */
+owl_status owl_acoustic2d_set_graphs(owl_acoustic2d *b, int on);
 
 /* Per-kernel timing with HIP events on the batch's stream.  No OWL
  * counterpart (OWL has no profiler); used by the benchmark.  kind:

@@ -89,6 +89,7 @@ _SIGNATURES = {
     "owl_acoustic2d_last_stats": (C.c_int, [_P, _P, _P]),
     "owl_acoustic2d_set_observed": (C.c_int, [_P, _P]),
     "owl_acoustic2d_observe": (C.c_int, [_P]),
+    "owl_acoustic2d_set_graphs": (C.c_int, [_P, C.c_int]),
     "owl_acoustic2d_set_step_kernel": (C.c_int, [_P, C.c_int, C.c_int]),
     "owl_acoustic2d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_acoustic2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),

@@ -137,11 +137,16 @@ class Acoustic2DBatch:
                "misfit", "merge", "update_p_2sets", "update_v_2sets", "adjoint_p", "adjoint_v",
                "store", "step_fused")
 
-    def set_step_kernel(self, mode="auto", zrows=32):
+    def set_graphs(self, on=True):
+        """HIP graph capture and replay of the exact-adjoint gradient
+        (bit-identical to eager execution)."""
+        check(lib().owl_acoustic2d_set_graphs(self._h, int(bool(on))))
+
+    def set_step_kernel(self, mode="auto", zrows=16):
         """Forward-step kernel of the exact-adjoint paths: "sweeps" (p then
         v), "fused" (one z-marching kernel per step) or "auto" (fused for 4
         shots or more).  Results are bit-identical."""
-        m = {"sweeps": 0, "fused": 1, "auto": 2}[mode]
+        m = {"sweeps": 0, "fused": 1, "auto": 2, "fused2": 3}[mode]
         check(lib().owl_acoustic2d_set_step_kernel(self._h, m, int(zrows)))
 
     def kernel_stats(self):

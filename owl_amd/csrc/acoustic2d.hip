@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
 // tile may already have written); arithmetic as k_update_p / k_update_v,
 // so the results are bit-identical to the two-sweep form.
 // ---------------------------------------------------------------------------
-template <int R>
+template <int R, int PD>
 __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ shots,
                                                     const int* __restrict__ ftile_shot, int nftile,
                                                     FieldSet F, const float* __restrict__ bk,
@@ -321,10 +321,16 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
     for (int k = 0; k < 2 * R - 1; ++k) V[k] = vz_row(rs - R + 1 + k);
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) P[k] = 0.0f;
-    Row cur, nxt;
+    // inputs of the next PD rows are in flight while row r is computed
+    Row cur, nxt, nx2;
     load(rs, cur);
+    if (PD == 2 && rs + 1 <= re) load(rs + 1, nxt);
     for (int r = rs; r <= re; ++r) {
-        if (r < re) load(r + 1, nxt);  // in flight while row r is computed
+        if (PD == 1) {
+            if (r < re) load(r + 1, nxt);
+        } else {
+            if (r + 2 <= re) load(r + 2, nx2);
+        }
         V[2 * R - 1] = cur.vzf;
         // ---- p at (col, r)
         float pn = 0.0f;
@@ -391,6 +397,7 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) V[k] = V[k + 1];
         cur = nxt;
+        if (PD == 2) nxt = nx2;
     }
 }
 
@@ -856,6 +863,11 @@ __global__ void k_fold_exact(const Shot* __restrict__ shots, const double* __res
 
 // Per-shot maximum |x| of one field buffer (bit pattern of a non-negative
 // float, so unsigned max is float max).
+// Zero n 32-bit words (stands in for a memset inside captured graphs).
+__global__ void k_zero_words(unsigned* p, size_t n) {
+    for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x) p[e] = 0u;
+}
+
 __global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) p[e] = v;
 }
@@ -955,6 +967,7 @@ struct owl_acoustic2d {
     size_t edge_elems = 0;
     int nrec_total = 0;
     int max_ninj = 0, max_nadj = 0;
+    int src_t0 = 1, src_t1 = 0;    // steps during which any source tap is active
     double pts_full = 0;           // sum over shots of (nx+2pml)(nz+2pml)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -990,7 +1003,7 @@ struct owl_acoustic2d {
     DevBuf<float> fC[3], fM3[4];      // third working set (fused steps ping-pong)
     // fused forward step: 0 two sweeps, 1 fused, 2 fused for batches of
     // 4 shots or more; z rows per tile
-    int step_kernel = 2, fzc = 32, nftile = 0;
+    int step_kernel = 2, fzc = 16, nftile = 0;
     std::vector<int> ftile_shot;
     DevBuf<int> d_ftile_shot;
     DevBuf<float> adj[11];            // ph vxh vzh psx psz phx phz ex ez dx dz
@@ -1005,8 +1018,15 @@ struct owl_acoustic2d {
     bool have_store = false;
     std::vector<std::vector<int>> binom_split;  // cached optimal splits [s][l]
     int binom_n = 0, binom_s = -1;
+    // HIP graph of the exact-adjoint gradient: captured on the second call
+    // with the same schedule and buffers, replayed afterwards
+    int use_graphs = 1;
+    bool capturing = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gsig, gready;
 
     ~owl_acoustic2d() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1120,6 +1140,8 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     b->nrec_total = 0;   // every per-plan table is rebuilt from zero
     b->max_ninj = 0;
     b->max_nadj = 0;
+    b->src_t0 = 1 << 30;
+    b->src_t1 = -1;
     for (int s = 0; s < nshot; ++s) {
         const owl_shot& in = shots[s];
         ShotHost& H = b->hs[s];
@@ -1207,6 +1229,8 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
             if (k == 0) H.hnt = hnt;
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1;
             const int nend = nbeg + int(stf.size()) - 1;
+            b->src_t0 = std::min(b->src_t0, nbeg);
+            b->src_t1 = std::max(b->src_t1, nend);
             // amplitude rows: amp = stf(t-nbeg+1)*amp*dt
             std::vector<float> a(nt, 0.f);
             for (int t = std::max(nbeg, 1); t <= std::min(nend, nt); ++t)
@@ -1644,13 +1668,21 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
     sets.s[1] = F;
     if (use_fused(b, in, out)) {
         const int ev = prof_begin(b, kKFused, b->profiling ? fused_bytes(b) : 0.0);
-        hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
-                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
-                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
+        if (b->step_kernel == 3)
+            hipLaunchKernelGGL((k_step_fused<R, 2>), dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
+                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
+                               b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
+        else
+            hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
+                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
+                               b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
         prof_end(b, ev);
     } else {
         launch_step<R>(b, sets, 1, b->dt);
     }
+    // without receivers to sample and outside every source's active steps the
+    // post-step kernel would change nothing
+    if (!record && (t < b->src_t0 || t > b->src_t1)) return;
     const int ev = prof_begin(b, kKPost);
     hipLaunchKernelGGL(k_post_step<R>, dim3(unsigned(b->sd.size())), dim3(512), 0, b->stream, b->d_shots.get(),
                        out.f[0], out.f[1], out.f[2], b->d_inj_beg.get(), b->d_inj_taps.get(), b->d_amp.get(),
@@ -1659,21 +1691,35 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
     prof_end(b, ev);
 }
 
+// Clear device memory on the batch stream: a memset, or a fill kernel while
+// a graph is being captured (memset nodes are kept out of graphs).
+void dzero(owl_acoustic2d* b, void* p, size_t bytes) {
+    if (!p || !bytes) return;
+    if (b->capturing) {
+        const size_t n = bytes / 4;
+        const unsigned g = unsigned(std::min<size_t>(4096, (n + 255) / 256));
+        hipLaunchKernelGGL(k_zero_words, dim3(g), dim3(256), 0, b->stream, static_cast<unsigned*>(p), n);
+    } else {
+        OWL_HIP(hipMemsetAsync(p, 0, bytes, b->stream));
+    }
+}
+
 void zero_set(owl_acoustic2d* b, const W7& w) {
-    for (int k = 0; k < 7; ++k)
-        OWL_HIP(hipMemsetAsync(w.f[k], 0, b->field_elems * sizeof(float), b->stream));
+    for (int k = 0; k < 7; ++k) dzero(b, w.f[k], b->field_elems * sizeof(float));
 }
 
 void slots_alloc(owl_acoustic2d* b, owl_acoustic2d::Slots& S, int nslot, int nfield, int fmt) {
     const size_t fe = b->field_elems;
     const size_t n = size_t(nslot) * nfield * fe;
     const size_t bytes = n * (fmt == OWL_STORE_F32 ? 4 : 2);
-    size_t freeb = 0, total = 0;
-    OWL_HIP(hipMemGetInfo(&freeb, &total));
     const size_t have = (fmt == OWL_STORE_F32 ? S.f32.size() * 4 : S.h16.size() * 2);
-    if (bytes > have && bytes - have + (size_t(1) << 30) > freeb)
-        fail(OWL_ERR_NOMEM, "state storage needs " + std::to_string(bytes >> 20) + " MiB; " +
-                                std::to_string(freeb >> 20) + " MiB free");
+    if (bytes > have) {
+        size_t freeb = 0, total = 0;
+        OWL_HIP(hipMemGetInfo(&freeb, &total));
+        if (bytes - have + (size_t(1) << 30) > freeb)
+            fail(OWL_ERR_NOMEM, "state storage needs " + std::to_string(bytes >> 20) + " MiB; " +
+                                    std::to_string(freeb >> 20) + " MiB free");
+    }
     S.fmt = fmt;
     S.nfield = nfield;
     S.nslot = nslot;
@@ -1742,11 +1788,11 @@ AdjSet adj_set(owl_acoustic2d* b) {
 void adjoint_alloc_zero(owl_acoustic2d* b) {
     for (int k = 0; k < 11; ++k) {
         b->adj[k].alloc(b->field_elems);
-        b->adj[k].zero(b->stream);
+        dzero(b, b->adj[k].get(), b->field_elems * sizeof(float));
     }
     for (DevBuf<double>* g : {&b->gK, &b->gX, &b->gZ}) {
         g->alloc(b->field_elems);
-        g->zero(b->stream);
+        dzero(b, g->get(), b->field_elems * sizeof(double));
     }
 }
 
@@ -1916,29 +1962,23 @@ void rev(owl_acoustic2d* bt, int a, int b, int sfree, int slot, int loc, RevCtx&
     rev<R>(bt, a, a + j, sfree, slot, slot, ctx);
 }
 
-// Exact-adjoint gradient over all shots.  residual(): fills d_adj once all
-// traces are known (misfit adjoint source, or the caller's trace gradient).
+// Phases of an exact-adjoint call: forward and reverse, or one of them.
 enum ExactPhase { kBoth = 0, kForwardOnly = 1, kReverseOnly = 2 };
 
+// The stream work of one exact-adjoint call (everything between the timing
+// events): the storage schedule, forward and adjoint steps, the residual,
+// the fold onto the windows.  Issues no synchronisation, so it can be
+// captured into a graph.
 template <int R>
-void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::function<void()>& residual,
-               int phase = kBoth) {
+void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::function<void()>& residual, int phase) {
     hipStream_t st = b->stream;
     const int N = b->nt - 1;  // forward steps
-    for (int k = 0; k < 3; ++k) b->fB[k].alloc(b->field_elems);
-    for (int k = 0; k < 4; ++k) b->fM2[k].alloc(b->field_elems);
-    for (int k = 0; k < 3; ++k) b->fC[k].alloc(b->field_elems);
-    for (int k = 0; k < 4; ++k) b->fM3[k].alloc(b->field_elems);
     const W7 w1 = work1(b), w2 = work2(b);
-    OWL_HIP(hipEventRecord(b->ev0, st));
     const bool do_fwd = phase != kReverseOnly, do_rev = phase != kForwardOnly;
-    if (do_rev && !do_fwd &&
-        (!b->have_store || b->store_policy != ckpt || b->store_param != param || b->store_fmt != fmt))
-        fail(OWL_ERR_STATE, "backward needs a preceding forward_store with the same storage policy");
     if (do_fwd) {
         zero_set(b, w1);
         zero_set(b, w2);
-        b->d_traces.zero(st);
+        dzero(b, b->d_traces.get(), b->d_traces.size() * sizeof(float));
     }
     // w3 is a step target: its halo must read as zero, its memory variables
     // start from zero wherever a schedule first writes them
@@ -1951,7 +1991,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
                 // each step reads slot t-1 and writes slot t directly (memory
                 // variables alternate between w1 and w3); the slots' outer
                 // halo stays zero
-                OWL_HIP(hipMemsetAsync(b->full.f32.get(), 0, size_t(N + 1) * 3 * b->field_elems * sizeof(float), st));
+                dzero(b, b->full.f32.get(), size_t(N + 1) * 3 * b->field_elems * sizeof(float));
                 W7 mA = w1, mB = work3(b);
                 for (int t = 1; t <= N; ++t) {
                     W7 in = mA, out = mB;
@@ -1974,7 +2014,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
             }
             fwd_steps += N;
         }
-        if (!do_rev) goto done;
+        if (!do_rev) return;
         residual();
         adjoint_alloc_zero(b);
         inject_residual(b, b->nt);
@@ -2009,11 +2049,11 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
             }
             fwd_steps += N;
         }
-        if (!do_rev) goto done;
+        if (!do_rev) return;
         slots_alloc(b, b->seg, k + 1, 3, OWL_STORE_F32);
         // the recompute writes each slot out of place over the update domain
         // only; the R-wide outer halo must read as zero, as in the forward
-        OWL_HIP(hipMemsetAsync(b->seg.f32.get(), 0, size_t(k + 1) * 3 * b->field_elems * sizeof(float), st));
+        dzero(b, b->seg.f32.get(), size_t(k + 1) * 3 * b->field_elems * sizeof(float));
         residual();
         adjoint_alloc_zero(b);
         inject_residual(b, b->nt);
@@ -2047,7 +2087,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
             // the schedule starts from u_0 in the reverse call; here only the
             // traces are modelled
             advance<R>(b, 0, N, kW1, -1, true, false);
-            goto done;
+            return;
         }
         binomial_table(b, N, S);
         slots_alloc(b, b->snaps, std::max(S, 1), 7, fmt);
@@ -2057,7 +2097,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
         // steps write FP32 snapshots over the update domain only: their outer
         // halo must read as zero
         if (ctx.direct)
-            OWL_HIP(hipMemsetAsync(b->snaps.f32.get(), 0, size_t(std::max(S, 1)) * 7 * b->field_elems * sizeof(float), st));
+            dzero(b, b->snaps.f32.get(), size_t(std::max(S, 1)) * 7 * b->field_elems * sizeof(float));
         ctx.on_first_adjoint = [&] {
             residual();
             adjoint_alloc_zero(b);
@@ -2083,7 +2123,88 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     hipLaunchKernelGGL(k_fold_exact<R>, dim3(fg, nshot), dim3(256), 0, st, b->d_shots.get(), b->gK.get(), b->gX.get(),
                        b->gZ.get(), b->d_vpw.get(), b->d_rhow.get(), b->d_gv.get(), b->d_gr.get(), b->pml);
     }
-done:
+    (void)fwd_steps;
+    (void)adj_steps;
+}
+
+// Every device pointer and launch parameter an exact-adjoint call bakes
+// into its launches: a graph captured under one signature is replayed only
+// under the same one.
+std::vector<uintptr_t> exact_signature(owl_acoustic2d* b, int ckpt, int param, int fmt) {
+    std::vector<uintptr_t> v{uintptr_t(ckpt), uintptr_t(param), uintptr_t(fmt), uintptr_t(b->R), uintptr_t(b->nt),
+                             uintptr_t(b->step_kernel), uintptr_t(b->fzc), uintptr_t(b->nftile),
+                             uintptr_t(b->tile_shot.size()), uintptr_t(b->field_elems)};
+    auto add = [&](const void* p) { v.push_back(reinterpret_cast<uintptr_t>(p)); };
+    for (int k = 0; k < 3; ++k) { add(b->fA[k].get()); add(b->fB[k].get()); add(b->fC[k].get()); }
+    for (int k = 0; k < 4; ++k) { add(b->fM[k].get()); add(b->fM2[k].get()); add(b->fM3[k].get()); }
+    for (auto& a : b->adj) add(a.get());
+    for (auto* g : {&b->gK, &b->gX, &b->gZ, &b->d_gv, &b->d_gr, &b->d_misfit, &b->d_scratch}) add(g->get());
+    for (auto* S : {&b->full, &b->snaps, &b->seg}) { add(S->f32.get()); add(S->h16.get()); add(S->amax.get()); }
+    for (auto* f : {&b->d_traces, &b->d_obs, &b->d_adj, &b->d_bk, &b->d_rho, &b->d_prof, &b->d_amp, &b->d_rec_w,
+                    &b->d_vpw, &b->d_rhow})
+        add(f->get());
+    add(b->d_shots.get()); add(b->d_tile_shot.get()); add(b->d_ftile_shot.get());
+    add(b->d_inj_beg.get()); add(b->d_adj_beg.get()); add(b->d_rec_beg.get());
+    add(b->d_inj_taps.get()); add(b->d_adj_taps.get()); add(b->d_rec_taps.get());
+    return v;
+}
+
+// Exact-adjoint gradient over all shots.  residual(): fills d_adj once all
+// traces are known (misfit adjoint source, or the caller's trace gradient).
+// A full gradient call (both phases, profiling off) runs eagerly the first
+// time; the next call with the same schedule and buffers is captured into a
+// HIP graph, which later calls replay.
+template <int R>
+void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::function<void()>& residual,
+               int phase = kBoth) {
+    hipStream_t st = b->stream;
+    const int N = b->nt - 1;  // forward steps
+    for (int k = 0; k < 3; ++k) b->fB[k].alloc(b->field_elems);
+    for (int k = 0; k < 4; ++k) b->fM2[k].alloc(b->field_elems);
+    for (int k = 0; k < 3; ++k) b->fC[k].alloc(b->field_elems);
+    for (int k = 0; k < 4; ++k) b->fM3[k].alloc(b->field_elems);
+    const bool do_fwd = phase != kReverseOnly, do_rev = phase != kForwardOnly;
+    if (do_rev && !do_fwd &&
+        (!b->have_store || b->store_policy != ckpt || b->store_param != param || b->store_fmt != fmt))
+        fail(OWL_ERR_STATE, "backward needs a preceding forward_store with the same storage policy");
+    const bool graph_ok = phase == kBoth && b->use_graphs && !b->profiling;
+    std::vector<uintptr_t> sig;
+    if (graph_ok) sig = exact_signature(b, ckpt, param, fmt);
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    if (graph_ok && b->gexec && sig == b->gsig) {
+        OWL_HIP(hipGraphLaunch(b->gexec, st));
+    } else if (graph_ok && sig == b->gready) {
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            b->gsig.clear();
+        }
+        hipGraph_t g = nullptr;
+        b->capturing = true;
+        OWL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            exact_body<R>(b, ckpt, param, fmt, residual, phase);
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            b->capturing = false;
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        b->capturing = false;
+        if (e != hipSuccess || !g) fail(OWL_ERR_DEVICE, "graph capture of the exact gradient failed");
+        const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b->gexec = nullptr;
+            fail(OWL_ERR_DEVICE, "graph instantiation of the exact gradient failed");
+        }
+        b->gsig = sig;
+        OWL_HIP(hipGraphLaunch(b->gexec, st));
+    } else {
+        exact_body<R>(b, ckpt, param, fmt, residual, phase);
+        if (graph_ok) b->gready = exact_signature(b, ckpt, param, fmt);
+    }
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
@@ -2093,8 +2214,6 @@ done:
     b->last_ms = ms;
     // grid-point updates of the useful work: N forward steps + N adjoint steps
     b->last_updates = double(N) * (double(do_fwd) + double(do_rev)) * b->pts_full;
-    (void)fwd_steps;
-    (void)adj_steps;
     b->fwd_state_valid = false;
     b->have_boundary = false;
     if (phase == kForwardOnly) {
@@ -2380,9 +2499,22 @@ owl_status owl_acoustic2d_observe(owl_acoustic2d* b) {
     });
 }
 
+owl_status owl_acoustic2d_set_graphs(owl_acoustic2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->use_graphs = on != 0;
+        if (b->gexec) {
+            OWL_HIP(hipGraphExecDestroy(b->gexec));
+            b->gexec = nullptr;
+        }
+        b->gsig.clear();
+        b->gready.clear();
+    });
+}
+
 owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d* b, int mode, int zrows) {
     return guarded([&] {
-        if (!b || mode < 0 || mode > 2 || zrows < 1) fail(OWL_ERR_ARG, "bad step kernel choice");
+        if (!b || mode < 0 || mode > 3 || zrows < 1) fail(OWL_ERR_ARG, "bad step kernel choice");
         b->step_kernel = mode;
         if (zrows != b->fzc) {
             b->fzc = zrows;

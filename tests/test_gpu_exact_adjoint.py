@@ -175,18 +175,51 @@ def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows):
     for ckpt, param, store in cases:
         print(f"case: fused vs sweeps R={R} zrows={zrows} ckpt={ckpt} param={param} store={store}", flush=True)
         out = []
-        for mode in ("sweeps", "fused"):
+        for mode in ("sweeps", "fused", "fused2"):
             b.set_step_kernel(mode, zrows)
             b.set_profiling(True)
             out.append(b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store))
             st = b.kernel_stats()
             b.set_profiling(False)
-            assert (st["step_fused"]["launches"] > 0) == (mode == "fused")
-        (m0, gv0, gr0, s0), (m1, gv1, gr1, s1) = out
-        assert np.array_equal(m0, m1)
-        for x, y in zip(s0, s1):
-            assert np.array_equal(x, y)
+            assert (st["step_fused"]["launches"] > 0) == (mode != "sweeps")
+        m0, gv0, gr0, s0 = out[0]
         assert np.abs(gv0).max() > 0
-        assert np.array_equal(gv0, gv1)
-        assert np.array_equal(gr0, gr1)
+        for m1, gv1, gr1, s1 in out[1:]:
+            assert np.array_equal(m0, m1)
+            for x, y in zip(s0, s1):
+                assert np.array_equal(x, y)
+            assert np.array_equal(gv0, gv1)
+            assert np.array_equal(gr0, gr1)
+    b.close()
+
+
+def test_graph_replays_are_bit_identical_to_eager(gpu):
+    """The exact-adjoint gradient captured into a HIP graph: four replays,
+    with eager forward modelling and synchronisations between them, each
+    bit-identical to the eager result, for binomial FP32 and uniform FP16
+    storage on a five-shot batch (fused steps)."""
+    from owl_amd.synthetic import gardner, random_field
+    nz, nx, dx = 50, 90, 10.0
+    vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 12))).astype(np.float32)
+    rho = gardner(vp)
+    rx = 15.0 + 10.0 * np.arange(85)
+    geoms = [simple_geometry(k + 1, 100.0 + 150.0 * k, 60.0, rx, np.full(85, 30.0), f0=20.0) for k in range(5)]
+    b = Acoustic2DBatch(nx, nz, dx, dx, vp, rho, [(g, None) for g in geoms], 0.001, 0.15, pml=10)
+    obs = b.forward()
+    b.set_model((vp * 0.98).astype(np.float32), rho)
+    for ckpt, param, store in [(OWL_CKPT_BINOMIAL, 7, OWL_STORE_F32), (OWL_CKPT_UNIFORM, 11, OWL_STORE_F16)]:
+        print(f"case: graph replays ckpt={ckpt} param={param} store={store}", flush=True)
+        b.set_graphs(False)
+        ref = b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store)
+        b.set_graphs(True)
+        for k in range(6):   # eager (signature), capture + launch, four replays
+            got = b.gradient(None, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store)
+            assert np.array_equal(got[0], ref[0]), k
+            assert np.array_equal(got[1], ref[1]), k
+            assert np.array_equal(got[2], ref[2]), k
+            for x, y in zip(got[3], ref[3]):
+                assert np.array_equal(x, y), k
+            if k % 2:
+                b.forward()   # eager work between replays
+    b.set_graphs(False)
     b.close()
