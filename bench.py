@@ -46,10 +46,10 @@ def workload():
     return vp, rho, vp0, rho0, shots
 
 
-def make_batch(vp, rho, vp0, rho0, shots):
+def make_batch(vp, rho, vp0, rho0, shots, tmax=None):
     from owl_amd import Acoustic2DBatch
     b = Acoustic2DBatch(CFG["nx"], CFG["nz"], CFG["dx"], CFG["dz"], vp, rho,
-                        [(g, None) for g in shots], CFG["dt"], CFG["tmax"], pml=CFG["pml"],
+                        [(g, None) for g in shots], CFG["dt"], tmax or CFG["tmax"], pml=CFG["pml"],
                         fdhalf=CFG["fdhalf"])
     b.observe()                # observed data from the true model
     b.set_model(vp0, rho0)     # gradient at the initial model
@@ -168,6 +168,9 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the batched exact gradient (for profiler runs)")
     ap.add_argument("--eager", action="store_true", help="no HIP graphs (for counter runs)")
+    ap.add_argument("--time-steps", type=int, default=0,
+                    help="with --headline-only: that many time steps instead of 2000 (short counter runs "
+                         "with the same launches)")
     ap.add_argument("--only-3d", action="store_true", help="only the 3D batched forward (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device five times (profiler counter calibration)")
@@ -204,7 +207,8 @@ def main():
     exact_b40 = dict(mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_BINOMIAL, ckpt_param=SNAPSHOTS, store=OWL_STORE_F32)
 
     # ---- batched + checkpointed (headline)
-    b = make_batch(vp, rho, vp0, rho0, shots)
+    b = make_batch(vp, rho, vp0, rho0, shots,
+                   tmax=(args.time_steps - 1) * CFG["dt"] if args.headline_only and args.time_steps > 1 else None)
     b.set_graphs(not args.eager)
     nt = b.nt
     log("batched exact gradient, binomial 40 snapshots, FP32")

@@ -208,10 +208,10 @@ owl_status owl_acoustic2d_observe(owl_acoustic2d *b);
 owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double *updates);
 
 /* Forward-step kernel of the exact-adjoint paths (no OWL counterpart;
- * results are bit-identical either way): mode 0 two sweeps (p, then v),
- * 1 one fused z-marching kernel per step, 2 fused for batches of 4 shots or
- * more (default), 3 fused with the inputs of two rows ahead in flight.
- * zrows: z rows per fused tile (default 16).  The fused kernel needs
+ * results are bit-identical either way): mode 0 two sweeps (p, then v)
+ * and the two-pass adjoint, 1 one fused z-marching kernel per forward step
+ * and per adjoint step, 2 fused for batches of 4 shots or more
+ * (default).  zrows: z rows per fused tile (default 16).  The fused kernel needs
  * distinct input and output states and no free surface; elsewhere the two
  * sweeps run. */
 owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d *b, int mode, int zrows);
@@ -230,7 +230,7 @@ owl_status owl_acoustic2d_set_graphs(owl_acoustic2d *b, int on);
  * injection, 6 misfit, 7 merge, 8 p-sweep and 9 v-sweep over two field sets
  * (reverse pass: reconstruction + adjoint), 10 exact-adjoint p-pass,
  * 11 exact-adjoint v-pass, 12 state storage, 13 fused forward step,
- * 0 <= kind < 16.
+ * 14 fused exact-adjoint step, 0 <= kind < 16.
  * Totals since profiling was switched on: device time, launch count, and
  * algorithmic (compulsory) bytes of those launches (0 if not modelled). */
 owl_status owl_acoustic2d_set_profiling(owl_acoustic2d *b, int on);

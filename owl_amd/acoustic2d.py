@@ -135,7 +135,7 @@ class Acoustic2DBatch:
 
     KERNELS = ("update_p", "update_v", "post_step", "image", "inject_strips", "inject_adjoint",
                "misfit", "merge", "update_p_2sets", "update_v_2sets", "adjoint_p", "adjoint_v",
-               "store", "step_fused")
+               "store", "step_fused", "adjoint_fused")
 
     def set_graphs(self, on=True):
         """HIP graph capture and replay of the exact-adjoint gradient
@@ -146,7 +146,7 @@ class Acoustic2DBatch:
         """Forward-step kernel of the exact-adjoint paths: "sweeps" (p then
         v), "fused" (one z-marching kernel per step) or "auto" (fused for 4
         shots or more).  Results are bit-identical."""
-        m = {"sweeps": 0, "fused": 1, "auto": 2, "fused2": 3}[mode]
+        m = {"sweeps": 0, "fused": 1, "auto": 2}[mode]
         check(lib().owl_acoustic2d_set_step_kernel(self._h, m, int(zrows)))
 
     def kernel_stats(self):

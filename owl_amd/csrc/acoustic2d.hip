@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
 // tile may already have written); arithmetic as k_update_p / k_update_v,
 // so the results are bit-identical to the two-sweep form.
 // ---------------------------------------------------------------------------
-template <int R, int PD>
+template <int R>
 __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ shots,
                                                     const int* __restrict__ ftile_shot, int nftile,
                                                     FieldSet F, const float* __restrict__ bk,
@@ -321,16 +321,10 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
     for (int k = 0; k < 2 * R - 1; ++k) V[k] = vz_row(rs - R + 1 + k);
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) P[k] = 0.0f;
-    // inputs of the next PD rows are in flight while row r is computed
-    Row cur, nxt, nx2;
+    Row cur, nxt;
     load(rs, cur);
-    if (PD == 2 && rs + 1 <= re) load(rs + 1, nxt);
     for (int r = rs; r <= re; ++r) {
-        if (PD == 1) {
-            if (r < re) load(r + 1, nxt);
-        } else {
-            if (r + 2 <= re) load(r + 2, nx2);
-        }
+        if (r < re) load(r + 1, nxt);  // in flight while row r is computed
         V[2 * R - 1] = cur.vzf;
         // ---- p at (col, r)
         float pn = 0.0f;
@@ -397,7 +391,6 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) V[k] = V[k + 1];
         cur = nxt;
-        if (PD == 2) nxt = nx2;
     }
 }
 
@@ -807,6 +800,200 @@ __global__ __launch_bounds__(kBX * kBY) void k_adj_v(const Shot* __restrict__ sh
     }
 }
 
+// Fused exact-adjoint step (the p-pass, then the v-pass with this step's
+// dx, dz), organised like k_step_fused: one wavefront per tile of
+// 64 - (2R - 1) columns, marching down z with the p-pass R - 1 rows ahead.
+// dx of a row reaches the neighbouring lanes by shuffles, dz stays in a
+// register queue, so neither is stored.  The p-pass is recomputed by the
+// halo lanes and rows, so its inputs ph, psx, psz, ex, ez come from one
+// buffer set and its outputs go to the other; vxh, vzh, phx, phz and the
+// FP64 accumulators are touched by the owning lanes only and stay in place.
+// Arithmetic as k_adj_p / k_adj_v, so the results are bit-identical.
+struct AdjIO {
+    const float *ph, *psx, *psz, *ex, *ez;   // p-pass inputs
+    float *pho, *psxo, *pszo, *exo, *ezo;    // p-pass outputs, v-pass ex/ez outputs
+};
+
+template <int R>
+__global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shots, const int* __restrict__ ftile_shot,
+                                                   int nftile, AdjIO io, AdjSet A, const float* __restrict__ bk,
+                                                   const float* __restrict__ rho, const float* __restrict__ prof,
+                                                   Coef c, float dt, float idx, float idz, int pml, int zc,
+                                                   const float* __restrict__ p_cur, const float* __restrict__ p_prev,
+                                                   const float* __restrict__ vx_cur, const float* __restrict__ vx_prev,
+                                                   const float* __restrict__ vz_cur, const float* __restrict__ vz_prev) {
+    constexpr int W = 64 - (2 * R - 1);
+    const int tile = int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y);
+    if (tile >= nftile) return;
+    const Shot S = shots[ftile_shot[tile]];
+    const int lt = tile - S.ftile0;
+    const int tx = lt % S.fntx, tz = lt / S.fntx;
+    const int l = int(threadIdx.x);
+    const int nx = S.nx, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int lx = nx + 2 * pml, lz = nz + 2 * pml;
+    const int col = -pml + 1 + tx * W - R + l;
+    const int z0 = -pml + 1 + tz * zc, z1 = min(z0 + zc - 1, nz + pml);
+    const bool pcol = col >= -pml + 1 && col <= nx + pml;
+    const bool mcol = col <= nx + pml + R;
+    const bool own = l >= R && l <= 64 - R && col <= nx + pml;
+    const float* axi = prof + S.prof;
+    const float* bxi = axi + lx;
+    const float* axh = prof + S.prof + 2 * lx;
+    const float* bxh = axh + lx;
+    const float* azi = prof + S.prof + 4 * lx;
+    const float* bzi = azi + lz;
+    const float* azh = azi + 2 * lz;
+    const float* bzh = azh + lz;
+    const bool xs_i = col <= 0 || col >= nx + 1, xs_h = col <= 1 || col >= nx + 1;
+    float ax = 0.f, bx = 0.f, axh_ = 0.f, bxh_ = 0.f;
+    if (pcol && xs_i) {
+        ax = axi[col + pml - 1];
+        bx = bxi[col + pml - 1];
+    }
+    if (own && xs_h) {
+        axh_ = axh[col + pml - 1];
+        bxh_ = bxh[col + pml - 1];
+    }
+    const size_t pitch = size_t(S.pitch);
+    const size_t base = size_t(S.off) + size_t(col - lo);
+    struct Row {
+        float ex[2 * R];  // ex at row r, columns col - R + 1 .. col + R
+        float ezf;        // ez at row r + R
+        float ph, psx, psz, k, pc, pp;
+        float rq, rl, ru, vxh, vzh, phx, phz, xc, xp, zc_, zp;
+    };
+    const int rs = z0 - R, re = z1 + R - 1;
+    auto load = [&](int r, Row& w) {
+        const int er = r + R;
+        w.ezf = (mcol && er >= lo && er <= nz + pml + R) ? io.ez[base + size_t(er - lo) * pitch] : 0.0f;
+        if (pcol && r >= -pml + 1 && r <= nz + pml) {
+            const size_t q = base + size_t(r - lo) * pitch;
+#pragma unroll
+            for (int k = 0; k < 2 * R; ++k) w.ex[k] = io.ex[q - (R - 1) + k];
+            w.ph = io.ph[q];
+            w.psx = xs_i ? io.psx[q] : 0.0f;
+            w.psz = (r <= 0 || r >= nz + 1) ? io.psz[q] : 0.0f;
+            w.k = bk[q];
+            w.pc = p_cur[q];
+            w.pp = p_prev[q];
+        }
+        const int j = r - R + 1;
+        if (own && j >= z0) {
+            const size_t q = base + size_t(j - lo) * pitch;
+            w.rq = rho[q];
+            w.rl = rho[q - 1];
+            w.ru = rho[q - pitch];
+            w.vxh = A.vxh[q];
+            w.vzh = A.vzh[q];
+            w.phx = xs_h ? A.phx[q] : 0.0f;
+            w.phz = (j <= 1 || j >= nz + 1) ? A.phz[q] : 0.0f;
+            w.xc = vx_cur[q];
+            w.xp = vx_prev[q];
+            w.zc_ = vz_cur[q];
+            w.zp = vz_prev[q];
+        }
+    };
+    auto ez_row = [&](int row) -> float {
+        return (mcol && row >= lo && row <= nz + pml + R) ? io.ez[base + size_t(row - lo) * pitch] : 0.0f;
+    };
+    // E[k] = ez at row r - R + 1 + k;  D[k] = this step's dz at row r - 2R + 1 + k
+    float E[2 * R], D[2 * R];
+#pragma unroll
+    for (int k = 0; k < 2 * R - 1; ++k) E[k] = ez_row(rs - R + 1 + k);
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) D[k] = 0.0f;
+    float dxq[R];  // this step's dx of rows r - R + 1 .. r (own column)
+#pragma unroll
+    for (int k = 0; k < R; ++k) dxq[k] = 0.0f;
+    Row cur, nxt;
+    load(rs, cur);
+    for (int r = rs; r <= re; ++r) {
+        if (r < re) load(r + 1, nxt);
+        E[2 * R - 1] = cur.ezf;
+        // ---- p-pass at (col, r)
+        float dxv = 0.0f, dzv = 0.0f;
+        if (pcol && r >= -pml + 1 && r <= nz + pml) {
+            const size_t q = base + size_t(r - lo) * pitch;
+            float sx = 0.0f, sz = 0.0f;
+#pragma unroll
+            for (int m = 1; m <= R; ++m) {
+                sx = sx + c.c[m - 1] * (cur.ex[R - 1 + m] - cur.ex[R - m]);
+                sz = sz + c.c[m - 1] * (E[R - 1 + m] - E[R - m]);
+            }
+            const float ph = cur.ph - (idx * sx + idz * sz);
+            const bool wr = own && r >= z0 && r <= z1;
+            if (wr) {
+                io.pho[q] = ph;
+                A.gK[q] += double((cur.pc - cur.pp) / cur.k * ph);
+            }
+            const float qv = -dt * cur.k * ph;
+            dxv = qv;
+            dzv = qv;
+            if (xs_i) {
+                const float sv = cur.psx + qv;
+                dxv = qv + ax * sv;
+                if (wr) io.psxo[q] = bx * sv;
+            }
+            if (r <= 0 || r >= nz + 1) {
+                const float sv = cur.psz + qv;
+                dzv = qv + azi[r + pml - 1] * sv;
+                if (wr) io.pszo[q] = bzi[r + pml - 1] * sv;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * R - 1; ++k) D[k] = D[k + 1];
+        D[2 * R - 1] = dzv;
+#pragma unroll
+        for (int k = 0; k < R - 1; ++k) dxq[k] = dxq[k + 1];
+        dxq[R - 1] = dxv;
+        // ---- v-pass at row j = r - R + 1
+        const int j = r - R + 1;
+        if (j >= z0) {
+            float xs[2 * R];  // dx of row j at columns col - R .. col + R - 1
+#pragma unroll
+            for (int k = 0; k < 2 * R; ++k) xs[k] = __shfl(dxq[0], l - R + k, 64);
+            if (own) {
+                const size_t q = base + size_t(j - lo) * pitch;
+                const float rx = 0.5f * (cur.rq + cur.rl);
+                const float rz = 0.5f * (cur.rq + cur.ru);
+                float vxh = cur.vxh, vzh = cur.vzh;
+                A.gX[q] += double((cur.xp - cur.xc) / rx * vxh);
+                A.gZ[q] += double((cur.zp - cur.zc_) / rz * vzh);
+                float sx = 0.0f, sz = 0.0f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m) {
+                    sx = sx + c.c[m - 1] * (xs[R + m - 1] - xs[R - m]);
+                    sz = sz + c.c[m - 1] * (D[R + m - 1] - D[R - m]);
+                }
+                vxh = vxh - idx * sx;
+                vzh = vzh - idz * sz;
+                A.vxh[q] = vxh;
+                A.vzh[q] = vzh;
+                float rr = -(dt / rx) * vxh;
+                float e = rr;
+                if (xs_h) {
+                    const float sv = cur.phx + rr;
+                    e = rr + axh_ * sv;
+                    A.phx[q] = bxh_ * sv;
+                }
+                io.exo[q] = e;
+                rr = -(dt / rz) * vzh;
+                e = rr;
+                if (j <= 1 || j >= nz + 1) {
+                    const float sv = cur.phz + rr;
+                    e = rr + azh[j + pml - 1] * sv;
+                    A.phz[q] = bzh[j + pml - 1] * sv;
+                }
+                io.ezo[q] = e;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * R - 1; ++k) E[k] = E[k + 1];
+        cur = nxt;
+    }
+}
+
 // The stored p_t includes the source injected at step t; the K gradient
 // needs the increment of the update alone: remove source/K * ph at the
 // source taps active at step t (one block per shot).
@@ -1007,6 +1194,8 @@ struct owl_acoustic2d {
     std::vector<int> ftile_shot;
     DevBuf<int> d_ftile_shot;
     DevBuf<float> adj[11];            // ph vxh vzh psx psz phx phz ex ez dx dz
+    DevBuf<float> adj2[5];            // fused adjoint: second ph psx psz ex ez
+    int adj_cur = 0;                  // which of the two holds the current ph psx psz ex ez
     DevBuf<double> gK, gX, gZ;
     struct Slots {                    // stored forward states
         int fmt = 0, nfield = 3, nslot = 0;
@@ -1639,6 +1828,13 @@ double fused_bytes(const owl_acoustic2d* b) {
     return by;
 }
 
+// The fused adjoint step (k_adj_fused) runs wherever the fused forward step
+// may: batches of 4 shots or more by default, no free surface.
+bool adj_fused(const owl_acoustic2d* b) {
+    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
+    return !(b->step_kernel == 2 && b->sd.size() < 4);
+}
+
 bool use_fused(const owl_acoustic2d* b, const W7& in, const W7& out) {
     if (b->step_kernel == 0 || b->cfg.free_surface) return false;
     if (b->step_kernel == 2 && b->sd.size() < 4) return false;
@@ -1668,14 +1864,9 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
     sets.s[1] = F;
     if (use_fused(b, in, out)) {
         const int ev = prof_begin(b, kKFused, b->profiling ? fused_bytes(b) : 0.0);
-        if (b->step_kernel == 3)
-            hipLaunchKernelGGL((k_step_fused<R, 2>), dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
-                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
-                               b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
-        else
-            hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
-                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
-                               b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
+        hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
+                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
+                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
         prof_end(b, ev);
     } else {
         launch_step<R>(b, sets, 1, b->dt);
@@ -1794,14 +1985,24 @@ void adjoint_alloc_zero(owl_acoustic2d* b) {
         g->alloc(b->field_elems);
         dzero(b, g->get(), b->field_elems * sizeof(double));
     }
+    if (adj_fused(b))
+        for (auto& a : b->adj2) {
+            a.alloc(b->field_elems);
+            dzero(b, a.get(), b->field_elems * sizeof(float));
+        }
+    b->adj_cur = 0;
 }
+
+// The adjoint pressure currently in use (the fused adjoint alternates it
+// between adj[0] and adj2[0]).
+float* cur_ph(owl_acoustic2d* b) { return b->adj_cur ? b->adj2[0].get() : b->adj[0].get(); }
 
 void inject_residual(owl_acoustic2d* b, int t) {
     const unsigned nshot = unsigned(b->sd.size());
     if (b->max_nadj <= 0) return;
     const int ev = prof_begin(b, kKAdj);
     hipLaunchKernelGGL(k_inject_adjoint, dim3(unsigned((b->max_nadj + 255) / 256), nshot), dim3(256), 0, b->stream,
-                       b->d_shots.get(), b->adj[0].get(), b->d_adj_beg.get(), b->d_adj_taps.get(),
+                       b->d_shots.get(), cur_ph(b), b->d_adj_beg.get(), b->d_adj_taps.get(),
                        b->d_rec_w.get(), b->d_adj.get(), t);
     prof_end(b, ev);
 }
@@ -1813,6 +2014,30 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
     const AdjSet A = adj_set(b);
     double pts = 0;
     for (auto& S : b->sd) pts += double(S.nx + 2 * b->pml) * double(S.nz + 2 * b->pml);
+    if (adj_fused(b)) {
+        DevBuf<float>* a = b->adj;
+        DevBuf<float>* a2 = b->adj2;
+        float* s0[5] = {a[0].get(), a[3].get(), a[4].get(), a[7].get(), a[8].get()};
+        float* s1[5] = {a2[0].get(), a2[1].get(), a2[2].get(), a2[3].get(), a2[4].get()};
+        float** in = b->adj_cur ? s1 : s0;
+        float** out = b->adj_cur ? s0 : s1;
+        const AdjIO io{in[0], in[1], in[2], in[3], in[4], out[0], out[1], out[2], out[3], out[4]};
+        // ex, ez, ph, bk, p_t, p_{t-1}, rho, vxh, vzh, 4 states read; ph, vxh, vzh, ex, ez
+        // written; gK, gX, gZ r+w (120 B per point, PML memories aside)
+        const int ev = prof_begin(b, 14, pts * 120.0);
+        hipLaunchKernelGGL(k_adj_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
+                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, io, A, b->d_bk.get(), b->d_rho.get(),
+                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc, cur[0], prev[0], cur[1],
+                           prev[1], cur[2], prev[2]);
+        prof_end(b, ev);
+        b->adj_cur = 1 - b->adj_cur;
+        if (b->max_ninj > 0)
+            hipLaunchKernelGGL(k_src_correction, dim3(unsigned(b->sd.size())), dim3(256), 0, b->stream,
+                               b->d_shots.get(), b->d_inj_beg.get(), b->d_inj_taps.get(), b->d_amp.get(),
+                               b->d_bk.get(), cur_ph(b), b->gK.get(), b->nt, t);
+        inject_residual(b, t);
+        return;
+    }
     // p-pass: ex, ez, ph, bk, p_t, p_{t-1}, psx/psz read; ph, dx, dz written; gK r+w
     int ev = prof_begin(b, 10, pts * (6 * 4 + 3 * 4 + 16));
     hipLaunchKernelGGL(k_adj_p<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), A, b->d_bk.get(),
@@ -2138,6 +2363,7 @@ std::vector<uintptr_t> exact_signature(owl_acoustic2d* b, int ckpt, int param, i
     for (int k = 0; k < 3; ++k) { add(b->fA[k].get()); add(b->fB[k].get()); add(b->fC[k].get()); }
     for (int k = 0; k < 4; ++k) { add(b->fM[k].get()); add(b->fM2[k].get()); add(b->fM3[k].get()); }
     for (auto& a : b->adj) add(a.get());
+    for (auto& a : b->adj2) add(a.get());
     for (auto* g : {&b->gK, &b->gX, &b->gZ, &b->d_gv, &b->d_gr, &b->d_misfit, &b->d_scratch}) add(g->get());
     for (auto* S : {&b->full, &b->snaps, &b->seg}) { add(S->f32.get()); add(S->h16.get()); add(S->amax.get()); }
     for (auto* f : {&b->d_traces, &b->d_obs, &b->d_adj, &b->d_bk, &b->d_rho, &b->d_prof, &b->d_amp, &b->d_rec_w,
@@ -2514,7 +2740,7 @@ owl_status owl_acoustic2d_set_graphs(owl_acoustic2d* b, int on) {
 
 owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d* b, int mode, int zrows) {
     return guarded([&] {
-        if (!b || mode < 0 || mode > 3 || zrows < 1) fail(OWL_ERR_ARG, "bad step kernel choice");
+        if (!b || mode < 0 || mode > 2 || zrows < 1) fail(OWL_ERR_ARG, "bad step kernel choice");
         b->step_kernel = mode;
         if (zrows != b->fzc) {
             b->fzc = zrows;

@@ -175,7 +175,7 @@ def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows):
     for ckpt, param, store in cases:
         print(f"case: fused vs sweeps R={R} zrows={zrows} ckpt={ckpt} param={param} store={store}", flush=True)
         out = []
-        for mode in ("sweeps", "fused", "fused2"):
+        for mode in ("sweeps", "fused"):
             b.set_step_kernel(mode, zrows)
             b.set_profiling(True)
             out.append(b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store))
