@@ -73,6 +73,20 @@ class EReceiver2D(C.Structure):
     _fields_ = [("x", C.c_float), ("z", C.c_float), ("weight", C.c_float)]
 
 
+class FGrid2D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_float), ("dz", C.c_float),
+                ("ox", C.c_float), ("oz", C.c_float), ("dt", C.c_float), ("nt", C.c_int),
+                ("pml", C.c_int), ("R", C.c_int), ("cc_step_interval", C.c_int), ("aniso", C.c_int)]
+
+
+class FSource2D(C.Structure):
+    _fields_ = [("x", C.c_float), ("z", C.c_float), ("amp", C.c_float), ("t0", C.c_float),
+                ("polar", C.c_float), ("mechanism", C.c_int), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
+
+
+FSG_CIJ = ("c11", "c13", "c15", "c33", "c35", "c55")
+
+
 class Point(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float),
                 ("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int),
@@ -480,3 +494,110 @@ class Oracle:
             raise RuntimeError(f"oracle elastic2d_gradient_owl failed ({rc})")
         out = tuple(a.reshape(G.nx, G.nz).T.copy() for a in (gv, gs, gr))
         return out + ((tr[0], tr[1]),)
+
+    # ---- 2D Lebedev FSG, general anisotropy (elastic_tti_2d, no free surface)
+    def _fgrid2(self, g):
+        return FGrid2D(int(g["nx"]), int(g["nz"]), g["dx"], g["dz"], g.get("ox", 0.0), g.get("oz", 0.0),
+                       g["dt"], int(g["nt"]), int(g["pml"]), int(g["R"]), int(g.get("cc_step_interval", 1)),
+                       int(g.get("aniso", 1)))
+
+    def _fsources2(self, g, sources):
+        keep = []
+        arr = (FSource2D * max(1, len(sources)))()
+        f0max, hnt = 0.0, 0
+        for k, s in enumerate(sources):
+            stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            stf = np.ascontiguousarray(stf, dtype=self.real)
+            keep.append(stf)
+            mech = {"force": 0, "explosion": 1}[s.get("mechanism", "force")]
+            arr[k] = FSource2D(s["x"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0), s.get("polar", 0.0), mech,
+                               len(stf), stf.ctypes.data)
+            f0max = max(f0max, s["f0"])
+            if k == 0:
+                hnt = h
+        return arr, keep, f0max, hnt
+
+    def _cij_ptrs(self, cij, shape):
+        ms = [self._model(cij[k], shape) for k in FSG_CIJ]
+        ptrs = (C.c_void_p * 6)(*[m.ctypes.data for m in ms])
+        return ms, ptrs
+
+    def fsg2d_forward(self, g, cij, rho, sources, receivers):
+        """Velocity traces (tx, tz), each (nr, nt), of OWL's elastic_tti_2d
+        compute_forward for one shot window.  cij: dict of c11, c13, c15, c33,
+        c35, c55 as (nz, nx); g["aniso"] 0 sets the M-PML ratios to 0."""
+        G = self._fgrid2(g)
+        shape = (G.nz, G.nx)
+        ms, ptrs = self._cij_ptrs(cij, shape)
+        rho = self._model(rho, shape)
+        arr, keep, f0max, _ = self._fsources2(g, sources)
+        rec = self._ereceivers2(receivers)
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("fsg2d_forward")
+        fn.argtypes = [C.POINTER(FGrid2D), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_int,
+                       C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), C.cast(ptrs, C.c_void_p), rho.ctypes.data_as(C.c_void_p), len(sources),
+                C.cast(arr, C.c_void_p), f0max, len(receivers), C.cast(rec, C.c_void_p),
+                tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle fsg2d_forward failed ({rc})")
+        return tr[0], tr[1]
+
+    def fsg2d_ratios(self, g, cij, rho):
+        """M-PML damping-ratio profiles (left, right, top, bottom) of the
+        Christoffel scan over the padded edges."""
+        G = self._fgrid2(g)
+        shape = (G.nz, G.nx)
+        ms, ptrs = self._cij_ptrs(cij, shape)
+        rho = self._model(rho, shape)
+        lr = [np.zeros(G.nz + 2 * G.pml + 2, np.float32) for _ in range(2)]
+        tb = [np.zeros(G.nx + 2 * G.pml + 2, np.float32) for _ in range(2)]
+        fn = self._fn("fsg2d_ratios")
+        fn.argtypes = [C.POINTER(FGrid2D), C.c_void_p, C.c_void_p] + [C.c_void_p] * 4
+        fn(C.byref(G), C.cast(ptrs, C.c_void_p), rho.ctypes.data_as(C.c_void_p),
+           *[a.ctypes.data_as(C.c_void_p) for a in lr + tb])
+        return lr[0], lr[1], tb[0], tb[1]
+
+    def fsg_damp_ratio(self, cvals, rho, direction):
+        fn = self._fn("fsg_damp_ratio")
+        fn.argtypes = [C.c_float] * 7 + [C.c_int]
+        fn.restype = C.c_float
+        return float(fn(*[float(v) for v in cvals], float(rho), 1 if direction == "x" else 0))
+
+    def fsg_mpml_eig(self, cvals, rho, direction, ratio, k1, k3):
+        """(eig1^2, eig2^2, deriv1, deriv2) of the M-PML Christoffel scan."""
+        c = np.asarray(cvals, np.float32)
+        out = np.zeros(4, np.float32)
+        fn = self._fn("fsg_mpml_eig")
+        fn.argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p]
+        fn(c.ctypes.data_as(C.c_void_p), float(rho), 1 if direction == "x" else 0, float(ratio), float(k1),
+           float(k3), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def fsg2d_gradient_owl(self, g, cij, rho, sources, receivers, adj, restore_all=False):
+        """OWL's FSG gradient: dict of grad_c11 .. grad_c55 and grad_rho as
+        (nz, nx), plus the forward traces (tx, tz).  adj: (2, nr, nt)."""
+        G = self._fgrid2(g)
+        shape = (G.nz, G.nx)
+        ms, ptrs = self._cij_ptrs(cij, shape)
+        rho = self._model(rho, shape)
+        arr, keep, f0max, hnt = self._fsources2(g, sources)
+        rec = self._ereceivers2(receivers)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (2, len(receivers), G.nt)
+        n = G.nx * G.nz
+        gr = [np.zeros(n, self.real) for _ in range(7)]
+        gptr = (C.c_void_p * 7)(*[a.ctypes.data for a in gr])
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("fsg2d_gradient_owl")
+        fn.argtypes = [C.POINTER(FGrid2D), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_int,
+                       C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), C.cast(ptrs, C.c_void_p), rho.ctypes.data_as(C.c_void_p), len(sources),
+                C.cast(arr, C.c_void_p), f0max, int(g.get("hnt", hnt)), len(receivers), C.cast(rec, C.c_void_p),
+                adj.ctypes.data_as(C.c_void_p), int(bool(restore_all)), C.cast(gptr, C.c_void_p),
+                tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle fsg2d_gradient_owl failed ({rc})")
+        names = ["grad_" + k for k in FSG_CIJ] + ["grad_rho"]
+        out = {k: a.reshape(G.nx, G.nz).T.copy() for k, a in zip(names, gr)}
+        return out, (tr[0], tr[1])
