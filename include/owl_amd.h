@@ -215,6 +215,17 @@ owl_status owl_acoustic2d_last_stats(const owl_acoustic2d *b, double *ms, double
  * distinct input and output states and no free surface; elsewhere the two
  * sweeps run. */
 owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d *b, int mode, int zrows);
+/* z rows per tile of the fused adjoint step (default 8; no OWL counterpart,
+ * results are bit-identical for any value). */
+owl_status owl_acoustic2d_set_adjoint_rows(owl_acoustic2d *b, int zrows);
+/* Autotuner of the fused kernels (no OWL counterpart): on (default), the
+ * first exact gradient times z-row candidates of the fused forward (8..32)
+ * and adjoint (4..16) steps on the batch's buffers and keeps the fastest.
+ * set_step_kernel and set_adjoint_rows turn it off; turning it on again
+ * re-tunes at the next exact gradient. */
+owl_status owl_acoustic2d_set_autotune(owl_acoustic2d *b, int on);
+/* The z rows in use by the fused forward and adjoint steps. */
+owl_status owl_acoustic2d_fused_rows(const owl_acoustic2d *b, int *zrows, int *adjoint_rows);
 
 /* HIP graphs for owl_acoustic2d_gradient in OWL_GRAD_EXACT mode (no OWL
  * counterpart; results are bit-identical): on (default), the second call
@@ -383,6 +394,63 @@ owl_status owl_elastic2d_set_profiling(owl_elastic2d *b, int on);
 owl_status owl_elastic2d_kernel_stats(const owl_elastic2d *b, int kind, double *total_ms,
                                       long long *launches, double *bytes_total);
 owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *updates);
+
+/* ---- 2D general-anisotropic elastic on the Lebedev fully staggered grid ----
+ * OWL's elastic_tti_2d solver (compute_forward,
+ * elastic_tti_2d/submodule_forward_modeling.f90:30-300; update_wavefield
+ * module_wavefield.f90:35-252; add_source :593-736; CFS-MPML with damping
+ * ratios module_cfspml.f90:80-576) and its gradient (compute_adjoint /
+ * compute_gradient, submodule_fwi_gradient.f90:40-727; boundary records
+ * module_boundary_saving.f90:100-200), for which_medium = 'elastic-tti' with
+ * anisotropy_type = 'cij' in src/forward/2d/module_modeling.f90 and
+ * src/inverse/2d/module_gradient.f90.  cij: six arrays c11, c13, c15, c33,
+ * c35, c55 and rho, each (nz, nx) z fastest.  aniso = 0 sets the M-PML
+ * damping ratios to 0 (OWL's anisotropy_type = 'iso', module_cfspml.f90:
+ * 344-352), 1 takes them from the Christoffel scan.  Force and explosion
+ * sources; no free surface (OWL's FSG free surface runs on its FLIT-built
+ * topography-following mesh).  Traces: shot after shot, two blocks (vx, vz)
+ * of [nr][nt], each the half-and-half mean of the two velocity grids. */
+typedef struct owl_fsg2d owl_fsg2d;
+owl_status owl_fsg2d_create(const owl_acoustic_config *cfg, const float *const *cij, const float *rho,
+                            int aniso, int nshot, const owl_shot *shots, owl_fsg2d **out);
+void owl_fsg2d_destroy(owl_fsg2d *b);
+int owl_fsg2d_nt(const owl_fsg2d *b);
+/* The damping-ratio profiles of one shot (compute_damping_profile_ratio,
+ * module_cfspml.f90:222-276): left and right over j = -pml .. nz+pml+1
+ * (nz + 2 pml + 2 values), top and bottom over i = -pml .. nx+pml+1. */
+owl_status owl_fsg2d_damping_ratios(const owl_fsg2d *b, int shot, float *left, float *right, float *top,
+                                    float *bottom);
+/* damp_ratio (module_cfspml.f90:164-219) of one medium, host only: c =
+ * c11, c13, c15, c33, c35, c55; dirx != 0 for the x direction (left and
+ * right edges), 0 for z (top and bottom). */
+owl_status owl_fsg2d_damp_ratio(const float *c, float rho, int dirx, float *ratio);
+/* OWL reads the final state back with vz_hxiz named twice
+ * (module_boundary_saving.f90:85-90): the saved vz_ixhz lands in vz_hxiz and
+ * vz_ixhz stays zero.  on = 0 (default) keeps that order; on = 1 restores
+ * every field (no OWL counterpart). */
+owl_status owl_fsg2d_set_restore_all(owl_fsg2d *b, int on);
+/* save_boundary != 0 keeps the boundary records and the final state for
+ * owl_fsg2d_adjoint. */
+owl_status owl_fsg2d_forward(owl_fsg2d *b, float *traces, int save_boundary);
+/* OWL's reverse pass: adj (trace layout, two components) into both adjoint
+ * velocity grids; grads[0..6] = per-shot window gradients of c11, c13,
+ * c15, c33, c35, c55 and rho (nz_s, nx_s) z fastest, shot after shot (the
+ * .grd arrays of anisotropy_type 'cij'), NaN/Inf -> 0; null entries are
+ * skipped.  Needs a preceding owl_fsg2d_forward(..., save_boundary = 1). */
+owl_status owl_fsg2d_adjoint(owl_fsg2d *b, const float *adj, float *const *grads);
+/* Forward with boundary saving, L2 waveform adjoint source per component,
+ * reverse pass, sum of the shots' gradients on the full model, FP64.
+ * grads[0..6] as above on (nz, nx); misfit: one value per shot. */
+owl_status owl_fsg2d_gradient(owl_fsg2d *b, const float *obs, double *misfit, double *const *grads,
+                              float *traces_out);
+/* Launch shape (no OWL counterpart; results do not depend on it). */
+owl_status owl_fsg2d_set_launch(owl_fsg2d *b, int planes, int rows);
+/* Timing; kinds: 0 stress sweep, 1 velocity sweep, 2 sources and
+ * receivers, 3 imaging, 4 boundary records. */
+owl_status owl_fsg2d_set_profiling(owl_fsg2d *b, int on);
+owl_status owl_fsg2d_kernel_stats(const owl_fsg2d *b, int kind, double *total_ms, long long *launches,
+                                  double *bytes_total);
+owl_status owl_fsg2d_last_stats(const owl_fsg2d *b, double *ms, double *updates);
 
 #ifdef __cplusplus
 }

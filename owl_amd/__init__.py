@@ -8,10 +8,11 @@ from .acoustic2d import Acoustic2DBatch, WaveSolverAcousticIso2D
 from .acoustic3d import Acoustic3DBatch
 from .elastic2d import Elastic2DBatch
 from .elastic3d import Elastic3DBatch
+from .fsg2d import FSG2DBatch
 from .geometry import Geometry, Receiver, Source, Window, set_adaptive_range
 
 __all__ = [
-    "Acoustic2DBatch", "Acoustic3DBatch", "Elastic2DBatch", "Elastic3DBatch", "WaveSolverAcousticIso2D",
+    "Acoustic2DBatch", "Acoustic3DBatch", "Elastic2DBatch", "Elastic3DBatch", "FSG2DBatch", "WaveSolverAcousticIso2D",
     "Geometry", "Receiver", "Source", "Window",
     "set_adaptive_range", "OwlError", "CflError", "lib", "OWL_GRAD_OWL", "OWL_GRAD_EXACT",
     "OWL_CKPT_FULL", "OWL_CKPT_UNIFORM", "OWL_CKPT_BINOMIAL", "OWL_STORE_F32", "OWL_STORE_F16",

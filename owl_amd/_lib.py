@@ -91,6 +91,9 @@ _SIGNATURES = {
     "owl_acoustic2d_observe": (C.c_int, [_P]),
     "owl_acoustic2d_set_graphs": (C.c_int, [_P, C.c_int]),
     "owl_acoustic2d_set_step_kernel": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_acoustic2d_set_adjoint_rows": (C.c_int, [_P, C.c_int]),
+    "owl_acoustic2d_set_autotune": (C.c_int, [_P, C.c_int]),
+    "owl_acoustic2d_fused_rows": (C.c_int, [_P, _P, _P]),
     "owl_acoustic2d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_acoustic2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_acoustic2d_forward_store": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int]),
@@ -116,6 +119,19 @@ _SIGNATURES = {
     "owl_elastic2d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_elastic2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_elastic2d_last_stats": (C.c_int, [_P, _P, _P]),
+    "owl_fsg2d_create": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, _P]),
+    "owl_fsg2d_destroy": (None, [_P]),
+    "owl_fsg2d_nt": (C.c_int, [_P]),
+    "owl_fsg2d_damping_ratios": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
+    "owl_fsg2d_damp_ratio": (C.c_int, [_P, C.c_float, C.c_int, _P]),
+    "owl_fsg2d_set_restore_all": (C.c_int, [_P, C.c_int]),
+    "owl_fsg2d_forward": (C.c_int, [_P, _P, C.c_int]),
+    "owl_fsg2d_adjoint": (C.c_int, [_P, _P, _P]),
+    "owl_fsg2d_gradient": (C.c_int, [_P, _P, _P, _P, _P]),
+    "owl_fsg2d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_fsg2d_set_profiling": (C.c_int, [_P, C.c_int]),
+    "owl_fsg2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "owl_fsg2d_last_stats": (C.c_int, [_P, _P, _P]),
     "owl_elastic3d_create": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
     "owl_elastic3d_destroy": (None, [_P]),
     "owl_elastic3d_nt": (C.c_int, [_P]),

@@ -22,6 +22,7 @@
 #include "host_prep.hpp"
 #include "owl_amd.h"
 #include "profiler.hpp"
+#include "trace_misfit.hpp"
 
 namespace owl {
 namespace e2 {
@@ -399,78 +400,6 @@ __global__ __launch_bounds__(256) void k2e_image(const ShotE2* __restrict__ shot
         gB[o] = gB[o] - i55 + 2.0 * i13;
         gC[o] = gC[o] + double(svx * rvx) + double(svz * rvz);
     }
-}
-
-// L2 waveform misfit and adjoint source per shot and component (as in 3D):
-// discard rule on each component's own largest trace norms, FP64 Neumaier
-// sums, traces [nt][nr] per component.
-__global__ __launch_bounds__(256) void k2e_adjoint_source(const ShotE2* __restrict__ shots,
-                                                          const float* __restrict__ syn, const float* __restrict__ obs,
-                                                          const float* __restrict__ rec_w, float* adj,
-                                                          double* misfit, int nt, float threshold, double* scratch) {
-    const ShotE2 S = shots[blockIdx.x];
-    const int comp = blockIdx.y;
-    const size_t base = S.trace0 + size_t(comp) * nt * S.nr;
-    __shared__ double red[256];
-    __shared__ double mo_s, ms_s;
-    double* nobs = scratch + 2 * (2 * size_t(S.rec0) + size_t(comp) * S.nr);
-    double* nsyn = nobs + S.nr;
-    double lmo = 0, lms = 0;
-    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
-        double a = 0, b = 0;
-        for (int t = 0; t < nt; ++t) {
-            const double o = obs[base + size_t(t) * S.nr + r];
-            const double sv = syn[base + size_t(t) * S.nr + r];
-            a += o * o;
-            b += sv * sv;
-        }
-        nobs[r] = sqrt(a);
-        nsyn[r] = sqrt(b);
-        lmo = fmax(lmo, nobs[r]);
-        lms = fmax(lms, nsyn[r]);
-    }
-    red[threadIdx.x] = lmo;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) mo_s = red[0];
-    __syncthreads();
-    red[threadIdx.x] = lms;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) ms_s = red[0];
-    __syncthreads();
-    const double mo = mo_s, ms = ms_s;
-    double sum = 0, cmp = 0;
-    for (int r = threadIdx.x; r < S.nr; r += blockDim.x) {
-        const bool keep = !(rec_w[S.rec0 + r] == 0.0f || nobs[r] < threshold * mo || nsyn[r] < threshold * ms ||
-                            nobs[r] * nsyn[r] == 0.0);
-        for (int t = 0; t < nt; ++t) {
-            const size_t q = base + size_t(t) * S.nr + r;
-            if (keep) {
-                const float d = obs[q] - syn[q];
-                const double d2 = double(d) * double(d);
-                const double y = sum + d2;
-                cmp += (fabs(sum) >= fabs(d2)) ? (sum - y) + d2 : (d2 - y) + sum;
-                sum = y;
-                adj[q] = syn[q] - obs[q];
-            } else {
-                adj[q] = 0.0f;
-            }
-        }
-    }
-    red[threadIdx.x] = sum + cmp;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) misfit[2 * blockIdx.x + comp] = red[0];
 }
 
 // Sum of the per-shot window accumulators on the full model (nz, nx), z
@@ -1179,7 +1108,7 @@ owl_status owl_elastic2d_gradient(owl_elastic2d* b, const float* obs, double* mi
         b->d_adj.alloc(std::max<size_t>(b->trace_elems, 1));
         b->d_misfit.alloc(2 * size_t(nshot));
         b->d_scratch.alloc(std::max<size_t>(4 * size_t(b->nrec_total), 1));
-        hipLaunchKernelGGL(k2e_adjoint_source, dim3(nshot, 2), dim3(256), 0, st, b->d_shots.get(), b->d_traces.get(),
+        hipLaunchKernelGGL(k_adjoint_source_2c<ShotE2>, dim3(nshot, 2), dim3(256), 0, st, b->d_shots.get(), b->d_traces.get(),
                            b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, 1.0e-6f,
                            b->d_scratch.get());
         dispatch_R2e(b->R, [&](auto r) { run_reverse2<decltype(r)::value>(b); });

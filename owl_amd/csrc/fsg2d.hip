@@ -1,0 +1,1422 @@
+// 2D Lebedev fully-staggered-grid (FSG) elastic time stepping for general
+// (TTI) anisotropy and OWL's gradient on it, for MI355X (gfx950): OWL's
+// elastic_tti_2d (module_wavefield.f90:35-252 update_wavefield, :593-736
+// add_source, module_cfspml.f90:80-576 M-PML, submodule_forward_modeling.f90
+// receivers, module_boundary_saving.f90, submodule_fwi_gradient.f90
+// compute_adjoint and compute_gradient) for a batch of shots, each on its
+// own window, in every launch.  Moduli given as c11, c13, c15, c33, c35,
+// c55 (OWL's 'cij'), force and explosion sources, two-component velocity
+// receivers (half of each velocity grid), CFS-MPML with the damping ratios
+// of the Christoffel scan, no free surface.
+//
+// Grids, in storage index (i, j) of one padded array per field:
+//   ii  stresses at (i, j)                       sxx_ii szz_ii sxz_ii
+//   hh  stresses at (i - 1/2, j - 1/2)           sxx_hh szz_hh sxz_hh
+//   hi  velocities at (i - 1/2, j)               vx_hi  vz_hi
+//   ih  velocities at (i, j - 1/2)               vx_ih  vz_ih
+// x fastest, rows padded to a multiple of 64 floats.  Each sweep tiles x
+// with 64-wide thread rows and marches along z with register queues for the
+// z stencils.  FP32 in the reference's operation order (derivatives divide
+// by dx and dz, no FMA contraction); gradients in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+#include "host_prep.hpp"
+#include "owl_amd.h"
+#include "profiler.hpp"
+#include "trace_misfit.hpp"
+
+namespace owl {
+namespace fsg {
+
+constexpr int kPitchAlign = 64;
+
+struct Coef {
+    float c[8];
+};
+
+struct ShotF {
+    int nx, nz;
+    int px, nzp;           // row pitch (floats), padded rows
+    long long off;         // element offset of node (lo, lo)
+    int tile0, ntx, nzc;
+    int nr, rec0;
+    int ninj, inj0;
+    int nadj, adj0;
+    long long trace0;      // two [nt][nr] blocks (vx, vz)
+    long long strip0;      // boundary records: nt blocks
+    long long g0;          // window gradient accumulators (nx*nz, x fastest)
+};
+
+// Wavefields (0..9) and CFS-PML memory variables (10..25), named as in OWL.
+constexpr int kNW = 10, kNM = 16, kNF = kNW + kNM;
+struct FF {
+    float *sxx_ii, *szz_ii, *sxz_ii, *sxx_hh, *szz_hh, *sxz_hh, *vx_hi, *vz_hi, *vx_ih, *vz_ih;
+    float *m_dxvx_ii, *m_dxvz_ii, *m_dzvx_ii, *m_dzvz_ii;  // stress sweep, integer grid
+    float *m_dxvx_hh, *m_dxvz_hh, *m_dzvx_hh, *m_dzvz_hh;  // stress sweep, corner grid
+    float *m_dxxx_hi, *m_dxxz_hi, *m_dzxz_hi, *m_dzzz_hi;  // velocity sweep, (i-1/2, j)
+    float *m_dxxx_ih, *m_dxxz_ih, *m_dzxz_ih, *m_dzzz_ih;  // velocity sweep, (i, j-1/2)
+};
+
+// The ten wavefields a step writes.
+struct FV {
+    float *sxx_ii, *szz_ii, *sxz_ii, *sxx_hh, *szz_hh, *sxz_hh, *vx_hi, *vz_hi, *vx_ih, *vz_ih;
+};
+
+// Moduli on the integer grid and their 4-point means on the corner grid
+// (module_wavefield.f90:166-171), |dt| over the 2-point densities at the
+// velocity positions (:205, :238).
+struct FM {
+    const float* ci[6];
+    const float* ch[6];
+    const float *qhx, *qhz;
+};
+
+// Per-cell CFS-MPML coefficients (a, b) in x and z on the four grids
+// (module_cfspml.f90:520-567), zero outside the damping frame.
+struct FP {
+    const float *axii, *bxii, *azii, *bzii;
+    const float *axhh, *bxhh, *azhh, *bzhh;
+    const float *axhi, *bxhi, *azhi, *bzhi;
+    const float *axih, *bxih, *azih, *bzih;
+};
+
+struct Tap {
+    long long cell;
+    float w;   // w1 w2 for sources, w1 w2 weight 0.5 for receivers
+    float w2;  // second weight of a source tap
+    int row;
+    int tbeg, tend;
+    int field;  // 0 vx_hi, 1 vz_hi, 2 vx_ih, 3 vz_ih, 4 sxx_ii, 5 szz_ii, 6 sxx_hh, 7 szz_hh
+};
+
+struct Targets {
+    float* f[8];
+};
+
+__device__ __forceinline__ size_t atf(const ShotF& S, int lo, int i, int j) {
+    return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.px);
+}
+
+// m = a d + b m; d = (d + m) / kappa with kappa = 1
+__device__ __forceinline__ float mpml(float* m, size_t q, float d, float a, float b) {
+    const float v = a * d + b * m[q];
+    m[q] = v;
+    return d + v;
+}
+
+// pdxw_stencil(w, i + off, j) (macro_fd_stencil.f90): sum_m c_m (w(i+off+m)
+// - w(i+off-m+1)) in the macro's order
+template <int R>
+__device__ __forceinline__ float sten_x(const float* __restrict__ w, size_t q, int off, const Coef& c) {
+    const float* p = w + q + off;
+    float s = c.c[0] * (p[1] - p[0]);
+#pragma unroll
+    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (p[m] - p[1 - m]);
+    return s;
+}
+
+// the z stencil from a queue: slot k holds row j + k - R
+template <int R>
+__device__ __forceinline__ float sten_q(const float* qv, int off, const Coef& c) {
+    float s = c.c[0] * (qv[R + off + 1] - qv[R + off]);
+#pragma unroll
+    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (qv[R + off + m] - qv[R + off + 1 - m]);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Stress sweep (module_wavefield.f90:87-186): the integer-grid stresses at
+// (i, j) from dx of vx_hi, vz_hi and dz of vx_ih, vz_ih; the corner stresses
+// at storage (i, j) from dx of vx_ih, vz_ih and dz of vx_hi, vz_hi, each
+// one node back.  Reads velocities and old stresses from F, writes O.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ shots, const int* __restrict__ tile_shot,
+                                                    FF F, FV O, FM M, FP P, Coef c, float dt, float dx, float dz,
+                                                    int pml, int kc) {
+    const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
+    const ShotF S = shots[tile_shot[tile]];
+    const int lt = tile - S.tile0;
+    const int tx = lt % S.ntx, tz = lt / S.ntx;
+    const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    if (i > nx + pml) return;
+    const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
+    if (j0 > j1) return;  // padding tile beyond the shot's rows
+    const size_t sz = S.px;
+    size_t q = atf(S, lo, i, j0);
+    // queues over rows j-R .. j+R
+    float qxh[2 * R + 1], qzh[2 * R + 1], qxi[2 * R + 1], qzi[2 * R + 1];  // vx_hi vz_hi vx_ih vz_ih
+#pragma unroll
+    for (int m = 0; m < 2 * R; ++m) {
+        const size_t r = q + (m - R) * sz;
+        qxh[m] = F.vx_hi[r];
+        qzh[m] = F.vz_hi[r];
+        qxi[m] = F.vx_ih[r];
+        qzi[m] = F.vz_ih[r];
+    }
+    const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
+    for (int j = j0; j <= j1; ++j, q += sz) {
+        {
+            const size_t r = q + R * sz;
+            qxh[2 * R] = F.vx_hi[r];
+            qzh[2 * R] = F.vz_hi[r];
+            qxi[2 * R] = F.vx_ih[r];
+            qzi[2 * R] = F.vz_ih[r];
+        }
+        {   // integer grid
+            float pdxvx = sten_x<R>(F.vx_hi, q, 0, c) / dx;
+            float pdxvz = sten_x<R>(F.vz_hi, q, 0, c) / dx;
+            float pdzvx = sten_q<R>(qxi, 0, c) / dz;
+            float pdzvz = sten_q<R>(qzi, 0, c) / dz;
+            if (xi || j <= 0 || j >= nz + 1) {
+                const float ax = P.axii[q], bx = P.bxii[q], az = P.azii[q], bz = P.bzii[q];
+                pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
+                pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
+                pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
+                pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
+            }
+            const float c11 = M.ci[0][q], c13 = M.ci[1][q], c15 = M.ci[2][q];
+            const float c33 = M.ci[3][q], c35 = M.ci[4][q], c55 = M.ci[5][q];
+            const float sh = pdxvz + pdzvx;
+            O.sxx_ii[q] = F.sxx_ii[q] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh);
+            O.szz_ii[q] = F.szz_ii[q] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh);
+            O.sxz_ii[q] = F.sxz_ii[q] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh);
+        }
+        {   // corner grid
+            float pdxvx = sten_x<R>(F.vx_ih, q, -1, c) / dx;
+            float pdxvz = sten_x<R>(F.vz_ih, q, -1, c) / dx;
+            float pdzvx = sten_q<R>(qxh, -1, c) / dz;
+            float pdzvz = sten_q<R>(qzh, -1, c) / dz;
+            if (xh || j <= 1 || j >= nz + 1) {
+                const float ax = P.axhh[q], bx = P.bxhh[q], az = P.azhh[q], bz = P.bzhh[q];
+                pdxvx = mpml(F.m_dxvx_hh, q, pdxvx, ax, bx);
+                pdxvz = mpml(F.m_dxvz_hh, q, pdxvz, ax, bx);
+                pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
+                pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
+            }
+            const float c11 = M.ch[0][q], c13 = M.ch[1][q], c15 = M.ch[2][q];
+            const float c33 = M.ch[3][q], c35 = M.ch[4][q], c55 = M.ch[5][q];
+            const float sh = pdxvz + pdzvx;
+            O.sxx_hh[q] = F.sxx_hh[q] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh);
+            O.szz_hh[q] = F.szz_hh[q] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh);
+            O.sxz_hh[q] = F.sxz_hh[q] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh);
+        }
+#pragma unroll
+        for (int m = 0; m < 2 * R; ++m) {
+            qxh[m] = qxh[m + 1];
+            qzh[m] = qzh[m + 1];
+            qxi[m] = qxi[m + 1];
+            qzi[m] = qzi[m + 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity sweep (module_wavefield.f90:188-250): (i - 1/2, j) from dx of the
+// integer-grid stresses one node back and dz of the corner stresses;
+// (i, j - 1/2) from dx of the corner stresses and dz of the integer-grid
+// stresses one node back.  New stresses from O, old velocities from F.
+// dt / rho_h is sgn * |dt| / rho_h, exactly.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ shots,
+                                                      const int* __restrict__ tile_shot, FF F, FV O, FM M, FP P,
+                                                      Coef c, float sgn, float dx, float dz, int pml, int kc) {
+    const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
+    const ShotF S = shots[tile_shot[tile]];
+    const int lt = tile - S.tile0;
+    const int tx = lt % S.ntx, tz = lt / S.ntx;
+    const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    if (i > nx + pml) return;
+    const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
+    if (j0 > j1) return;
+    const size_t sz = S.px;
+    size_t q = atf(S, lo, i, j0);
+    float qah[2 * R + 1], qch[2 * R + 1], qai[2 * R + 1], qci[2 * R + 1];  // sxz_hh szz_hh sxz_ii szz_ii
+#pragma unroll
+    for (int m = 0; m < 2 * R; ++m) {
+        const size_t r = q + (m - R) * sz;
+        qah[m] = O.sxz_hh[r];
+        qch[m] = O.szz_hh[r];
+        qai[m] = O.sxz_ii[r];
+        qci[m] = O.szz_ii[r];
+    }
+    const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
+    for (int j = j0; j <= j1; ++j, q += sz) {
+        {
+            const size_t r = q + R * sz;
+            qah[2 * R] = O.sxz_hh[r];
+            qch[2 * R] = O.szz_hh[r];
+            qai[2 * R] = O.sxz_ii[r];
+            qci[2 * R] = O.szz_ii[r];
+        }
+        {   // (i - 1/2, j)
+            float pdxxx = sten_x<R>(O.sxx_ii, q, -1, c) / dx;
+            float pdxxz = sten_x<R>(O.sxz_ii, q, -1, c) / dx;
+            float pdzxz = sten_q<R>(qah, 0, c) / dz;
+            float pdzzz = sten_q<R>(qch, 0, c) / dz;
+            if (xh || j <= 0 || j >= nz + 1) {
+                const float ax = P.axhi[q], bx = P.bxhi[q], az = P.azhi[q], bz = P.bzhi[q];
+                pdxxx = mpml(F.m_dxxx_hi, q, pdxxx, ax, bx);
+                pdxxz = mpml(F.m_dxxz_hi, q, pdxxz, ax, bx);
+                pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
+                pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
+            }
+            const float w = sgn * M.qhx[q];
+            O.vx_hi[q] = F.vx_hi[q] + w * (pdxxx + pdzxz);
+            O.vz_hi[q] = F.vz_hi[q] + w * (pdxxz + pdzzz);
+        }
+        {   // (i, j - 1/2)
+            float pdxxx = sten_x<R>(O.sxx_hh, q, 0, c) / dx;
+            float pdxxz = sten_x<R>(O.sxz_hh, q, 0, c) / dx;
+            float pdzxz = sten_q<R>(qai, -1, c) / dz;
+            float pdzzz = sten_q<R>(qci, -1, c) / dz;
+            if (xi || j <= 1 || j >= nz + 1) {
+                const float ax = P.axih[q], bx = P.bxih[q], az = P.azih[q], bz = P.bzih[q];
+                pdxxx = mpml(F.m_dxxx_ih, q, pdxxx, ax, bx);
+                pdxxz = mpml(F.m_dxxz_ih, q, pdxxz, ax, bx);
+                pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
+                pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
+            }
+            const float w = sgn * M.qhz[q];
+            O.vx_ih[q] = F.vx_ih[q] + w * (pdxxx + pdzxz);
+            O.vz_ih[q] = F.vz_ih[q] + w * (pdxxz + pdzzz);
+        }
+#pragma unroll
+        for (int m = 0; m < 2 * R; ++m) {
+            qah[m] = qah[m + 1];
+            qch[m] = qch[m + 1];
+            qai[m] = qai[m + 1];
+            qci[m] = qci[m + 1];
+        }
+    }
+}
+
+// Sources of step t (add_source): one block per shot, taps grouped by
+// (field, cell) in source order; amp_tab holds sin/cos(polar) amp / rho for
+// forces and -amp / (dx dz) for explosions, so every tap adds amp w1 w2.
+__global__ __launch_bounds__(1024) void k_fsg_inject(const ShotF* __restrict__ shots, Targets T,
+                                                     const int* __restrict__ inj_beg, const Tap* __restrict__ inj,
+                                                     const float* __restrict__ amp_tab, int nt, int t) {
+    const ShotF S = shots[blockIdx.x];
+    for (int c = threadIdx.x; c < S.ninj; c += blockDim.x) {
+        const int b = inj_beg[S.inj0 + c], e = inj_beg[S.inj0 + c + 1];
+        const long long cell = inj[b].cell;
+        float* f = T.f[inj[b].field];
+        float acc = f[cell];
+        for (int k = b; k < e; ++k) {
+            const Tap tp = inj[k];
+            if (t < tp.tbeg || t > tp.tend) continue;
+            acc = acc + amp_tab[size_t(tp.row) * nt + (t - 1)] * tp.w * tp.w2;
+        }
+        f[cell] = acc;
+    }
+}
+
+// Receivers of step t: for each stencil tap (irz, irx), vx_hi at (hx, gz)
+// times amp1 then vx_ih at (gx, hz) times amp2, amp = w1 w2 weight 0.5
+// (submodule_forward_modeling.f90:78-118); likewise vz.
+__global__ __launch_bounds__(256) void k_fsg_record(const ShotF* __restrict__ shots, Targets T,
+                                                    const int* __restrict__ rec_beg, const Tap* __restrict__ rec,
+                                                    float* traces, int nt, int t) {
+    const ShotF S = shots[blockIdx.y];
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < 2 * S.nr; r += gridDim.x * blockDim.x) {
+        const int comp = r / S.nr, ir = r % S.nr;
+        const int g = S.rec0 + ir;
+        float acc = 0.0f;
+        for (int k = rec_beg[g]; k < rec_beg[g + 1]; ++k) {
+            const Tap tp = rec[k];  // field 0 (hi grid) or 2 (ih grid)
+            acc = acc + T.f[tp.field + comp][tp.cell] * tp.w;
+        }
+        traces[S.trace0 + size_t(comp) * nt * S.nr + size_t(t - 1) * S.nr + ir] = acc;
+    }
+}
+
+// Boundary records (module_boundary_saving.f90:100-200): vx_hi, vz_hi,
+// vx_ih, vz_ih, each left [2-R, 1] x [1-R, nz+R], right [nx+1, nx+R] x
+// [1-R, nz+R], top [1-R, nx+R] x [2-R, 1], bottom [1-R, nx+R] x [nz+1, nz+R],
+// x fastest.  save != 0: record <- fields; else fields <- record.
+__host__ __device__ inline long long stripf_len(int nx, int nz, int R) {
+    return 4LL * (2LL * R * (nz + 2 * R) + 2LL * R * (nx + 2 * R));
+}
+
+template <int R>
+__global__ void k_fsg_strips(const ShotF* __restrict__ shots, Targets T, float* strips, int t, int pml, int save) {
+    const ShotF S = shots[blockIdx.y];
+    const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+    const long long ex = nx + 2 * R, ez = nz + 2 * R;
+    const long long a = R * ez, b = ex * R, one = 2 * a + 2 * b, len = 4 * one;
+    float* rec = strips + S.strip0 + (long long)(t - 1) * len;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int comp = int(e / one);
+        long long r = e - comp * one;
+        int i, j;
+        if (r < 2 * a) {
+            const bool hi = r >= a;
+            if (hi) r -= a;
+            i = (hi ? nx + 1 : 2 - R) + int(r % R);
+            j = 1 - R + int(r / R);
+        } else {
+            r -= 2 * a;
+            const bool hi = r >= b;
+            if (hi) r -= b;
+            i = 1 - R + int(r % ex);
+            j = (hi ? nz + 1 : 2 - R) + int(r / ex);
+        }
+        float* f = T.f[comp];
+        const size_t q = atf(S, lo, i, j);
+        if (save)
+            rec[e] = f[q];
+        else
+            f[q] = rec[e];
+    }
+}
+
+// Adjoint sources into both adjoint velocity grids (submodule_fwi_gradient.
+// f90:348-390): taps grouped by (field, cell) in receiver order; adj holds
+// two [nt][nr] blocks per shot.
+__global__ void k_fsg_inject_adjoint(const ShotF* __restrict__ shots, Targets T, const int* __restrict__ cell_beg,
+                                     const Tap* __restrict__ taps, const float* __restrict__ adj, int nt, int t) {
+    const ShotF S = shots[blockIdx.y];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.nadj; c += gridDim.x * blockDim.x) {
+        const int b = cell_beg[S.adj0 + c], e = cell_beg[S.adj0 + c + 1];
+        const long long cell = taps[b].cell;
+        const int fld = taps[b].field;  // 0..3
+        float* f = T.f[fld];
+        const float* a_t = adj + S.trace0 + size_t(fld & 1) * nt * S.nr + size_t(t - 1) * S.nr;
+        float acc = f[cell];
+        for (int k = b; k < e; ++k) acc = acc + a_t[taps[k].row] * taps[k].w;
+        f[cell] = acc;
+    }
+}
+
+// Imaging (compute_gradient, submodule_fwi_gradient.f90:628-727) over the
+// window interior: sub-grid sums of the stress increment and of the adjoint
+// stresses, strains by the 2D compliance (cij_to_sij_2d), g_cij -= products
+// of adjoint and forward strains, g_rho += velocity increment . adjoint
+// velocity, into seven FP64 accumulators.
+struct Comp {
+    const float* s[6];  // s11 s13 s15 s33 s35 s55
+};
+struct Acc7 {
+    double* g[7];
+};
+
+__device__ __forceinline__ float hsum(const float* a, size_t q, size_t sz) {
+    return a[q] + a[q + 1] + a[q + sz] + a[q + 1 + sz];
+}
+
+__global__ __launch_bounds__(256) void k_fsg_image(const ShotF* __restrict__ shots, FV cur, FV prev, FV adj, Comp C,
+                                                   Acc7 G, int prev_zero, int pml, int R) {
+    const ShotF S = shots[blockIdx.y];
+    const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
+    const long long n = (long long)nx * nz;
+    const size_t sz = S.px;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = 1 + int(e % nx), j = 1 + int(e / nx);
+        const size_t q = atf(S, lo, i, j);
+        float txx = cur.sxx_ii[q] + 0.25f * hsum(cur.sxx_hh, q, sz);
+        float tzz = cur.szz_ii[q] + 0.25f * hsum(cur.szz_hh, q, sz);
+        float txz = cur.sxz_ii[q] + 0.25f * hsum(cur.sxz_hh, q, sz);
+        float svx = 0.5f * (cur.vx_hi[q] + cur.vx_hi[q + 1]) + 0.5f * (cur.vx_ih[q] + cur.vx_ih[q + sz]);
+        float svz = 0.5f * (cur.vz_hi[q] + cur.vz_hi[q + 1]) + 0.5f * (cur.vz_ih[q] + cur.vz_ih[q + sz]);
+        if (!prev_zero) {
+            txx = txx - (prev.sxx_ii[q] + 0.25f * hsum(prev.sxx_hh, q, sz));
+            tzz = tzz - (prev.szz_ii[q] + 0.25f * hsum(prev.szz_hh, q, sz));
+            txz = txz - (prev.sxz_ii[q] + 0.25f * hsum(prev.sxz_hh, q, sz));
+            svx = svx - (0.5f * (prev.vx_hi[q] + prev.vx_hi[q + 1]) + 0.5f * (prev.vx_ih[q] + prev.vx_ih[q + sz]));
+            svz = svz - (0.5f * (prev.vz_hi[q] + prev.vz_hi[q + 1]) + 0.5f * (prev.vz_ih[q] + prev.vz_ih[q + sz]));
+        }
+        const float rxx = adj.sxx_ii[q] + 0.25f * hsum(adj.sxx_hh, q, sz);
+        const float rzz = adj.szz_ii[q] + 0.25f * hsum(adj.szz_hh, q, sz);
+        const float rxz = adj.sxz_ii[q] + 0.25f * hsum(adj.sxz_hh, q, sz);
+        const float s11 = C.s[0][q], s13 = C.s[1][q], s15 = C.s[2][q];
+        const float s33 = C.s[3][q], s35 = C.s[4][q], s55 = C.s[5][q];
+        const float exx = s11 * txx + s13 * tzz + s15 * txz, ezz = s13 * txx + s33 * tzz + s35 * txz;
+        const float exz = s15 * txx + s35 * tzz + s55 * txz;
+        const float fxx = s11 * rxx + s13 * rzz + s15 * rxz, fzz = s13 * rxx + s33 * rzz + s35 * rxz;
+        const float fxz = s15 * rxx + s35 * rzz + s55 * rxz;
+        const float rvx = 0.5f * (adj.vx_hi[q] + adj.vx_hi[q + 1]) + 0.5f * (adj.vx_ih[q] + adj.vx_ih[q + sz]);
+        const float rvz = 0.5f * (adj.vz_hi[q] + adj.vz_hi[q + 1]) + 0.5f * (adj.vz_ih[q] + adj.vz_ih[q + sz]);
+        const size_t o = size_t(S.g0 + e);
+        G.g[0][o] -= double(fxx * exx);
+        G.g[1][o] -= double(fxx * ezz + fzz * exx);
+        G.g[2][o] -= double(fxx * exz + exx * fxz);
+        G.g[3][o] -= double(fzz * ezz);
+        G.g[4][o] -= double(fzz * exz + ezz * fxz);
+        G.g[5][o] -= double(fxz * exz);
+        G.g[6][o] += double(svx * rvx) + double(svz * rvz);
+    }
+}
+
+// Sum of the per-shot window accumulators on the full model (nz, nx), z
+// fastest (module_gradient.f90:366-388), one parameter per launch.
+__global__ void k_fsg_merge(const ShotF* __restrict__ shots, const int* __restrict__ win, int nshot,
+                            const double* __restrict__ g, double* full, int nx, int nz) {
+    const long long n = (long long)nx * nz;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int kz = int(e % nz), ix = int(e / nz);
+        double a = 0.0;
+        for (int s = 0; s < nshot; ++s) {
+            const ShotF& S = shots[s];
+            const int i = ix - (win[2 * s] - 1), k = kz - (win[2 * s + 1] - 1);
+            if (i < 0 || i >= S.nx || k < 0 || k >= S.nz) continue;
+            a += g[size_t(S.g0) + i + size_t(k) * S.nx];
+        }
+        full[e] = a;
+    }
+}
+
+struct ShotHostF {
+    int id = 0, nx = 0, nz = 0, nxbeg = 1, nzbeg = 1, hnt = 1;
+    float ox = 0, oz = 0;
+    std::vector<owl_source> src;
+    std::vector<owl_receiver> rec;
+    std::vector<float> ratio[4];  // left, right (j = -pml .. nz+pml+1), top, bottom (i = -pml .. nx+pml+1)
+};
+
+}  // namespace fsg
+}  // namespace owl
+
+using namespace owl;
+using namespace owl::fsg;
+
+struct owl_fsg2d {
+    owl_acoustic_config cfg{};
+    int R = 4, pml = 15, nt = 0, kc = 32, cy = 4, aniso = 1, restore_all = 0;
+    float dt = 0;
+    fsg::Coef coef{};
+    std::vector<ShotHostF> hs;
+    std::vector<ShotF> sd;
+    std::vector<int> tile_shot;
+    size_t field_elems = 0, trace_elems = 0, strip_elems = 0, grad_elems = 0;
+    int nrec_total = 0, max_nr = 0, max_nadj = 0;
+    double pts_full = 0;
+    bool have_boundary = false, fwd_state_valid = false, have_obs = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0, last_updates = 0;
+    Profiler prof;
+    std::vector<float> cij_full[6], rho_full;
+    DevBuf<ShotF> d_shots;
+    DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg, d_adj_beg, d_win;
+    DevBuf<float> d_amp, d_rec_w, d_traces, d_strips, d_adj, d_obs;
+    DevBuf<float> med[14];  // ci[6], ch[6], qhx, qhz
+    DevBuf<float> pmlc[16];
+    DevBuf<float> comp[6];  // compliances at the integer nodes
+    DevBuf<float> fld[kNF];
+    DevBuf<float> fB[kNW];  // the other half of the reverse ping-pong
+    DevBuf<float> aF[kNF];  // adjoint field
+    DevBuf<Tap> d_inj, d_rec, d_adj_taps;
+    DevBuf<double> d_g[7], d_full, d_misfit, d_scratch;
+    ~owl_fsg2d() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+inline size_t cellf(const owl_fsg2d* b, const ShotF& S, int i, int j) {
+    const int lo = 1 - b->pml - b->R, h = b->pml + b->R;
+    if (i < lo || i > S.nx + h || j < lo || j > S.nz + h)
+        fail(OWL_ERR_ARG, "a source/receiver stencil reaches outside the padded grid; increase pml");
+    return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.px);
+}
+
+inline size_t model_indexf(const owl_fsg2d* b, const ShotHostF& H, int i, int j) {
+    const int ic = std::min(std::max(i, 1), H.nx), jc = std::min(std::max(j, 1), H.nz);
+    return size_t(H.nzbeg - 1 + jc - 1) + size_t(H.nxbeg - 1 + ic - 1) * size_t(b->cfg.nz);
+}
+
+// damp_coef (module_cfspml.f90:282-324) with the M-PML mixing wx = dampx +
+// ratioz dampz, wz = dampz + ratiox dampx, kmax = 1.
+void damp_coef_mpml(float pmlvp, float xd, float zd, float dx, float dz, int pml, float adt, float am, float rx,
+                    float rz, float* ax, float* bx, float* az, float* bz) {
+    const float R0 = 1.0e-5f, kx = 1.0f, kz = 1.0f;
+    float dampx = pmlvp * (2 + 1.0f) * std::log(1.0f / R0) / (2.0f * float(pml) * dx) * (xd * xd);
+    float dampz = pmlvp * (2 + 1.0f) * std::log(1.0f / R0) / (2.0f * float(pml) * dz) * (zd * zd);
+    const float wx = dampx + rz * dampz, wz = dampz + rx * dampx;
+    dampx = wx;
+    dampz = wz;
+    const float alx = std::max(am * (1.0f - xd), 0.25f * am);
+    *ax = -2.0f * adt * dampx / kx / (2.0f + adt * (alx + dampx / kx));
+    *bx = (2.0f - adt * (alx + dampx / kx)) / (2.0f + adt * (alx + dampx / kx));
+    const float alz = std::max(am * (1.0f - zd), 0.25f * am);
+    *az = -2.0f * adt * dampz / kz / (2.0f + adt * (alz + dampz / kz));
+    *bz = (2.0f - adt * (alz + dampz / kz)) / (2.0f + adt * (alz + dampz / kz));
+}
+
+// Damping ratios of every shot's four padded edge lines
+// (compute_damping_profile_ratio, module_cfspml.f90:222-276), memoised on
+// the medium values and spread over host threads.
+void damping_ratios(owl_fsg2d* b) {
+    struct Job {
+        float c[6], rho;
+        bool dirx;
+        float* out;
+    };
+    std::vector<Job> jobs;
+    const int pml = b->pml;
+    for (auto& H : b->hs) {
+        H.ratio[0].assign(H.nz + 2 * pml + 2, 0.f);
+        H.ratio[1].assign(H.nz + 2 * pml + 2, 0.f);
+        H.ratio[2].assign(H.nx + 2 * pml + 2, 0.f);
+        H.ratio[3].assign(H.nx + 2 * pml + 2, 0.f);
+        if (!b->aniso) continue;
+        auto add = [&](int i, int j, bool dirx, float* out) {
+            const size_t m = model_indexf(b, H, i, j);
+            Job J;
+            for (int k = 0; k < 6; ++k) J.c[k] = b->cij_full[k][m];
+            J.rho = b->rho_full[m];
+            J.dirx = dirx;
+            J.out = out;
+            jobs.push_back(J);
+        };
+        for (int j = -pml; j <= H.nz + pml + 1; ++j) {
+            add(1, j, true, &H.ratio[0][j + pml]);
+            add(H.nx, j, true, &H.ratio[1][j + pml]);
+        }
+        for (int i = -pml; i <= H.nx + pml + 1; ++i) {
+            add(i, 1, false, &H.ratio[2][i + pml]);
+            add(i, H.nz, false, &H.ratio[3][i + pml]);
+        }
+    }
+    if (jobs.empty()) return;
+    // distinct (medium, direction) keys
+    using Key = std::tuple<float, float, float, float, float, float, float, bool>;
+    std::map<Key, size_t> uniq;
+    std::vector<size_t> slot(jobs.size());
+    std::vector<const Job*> todo;
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        const Job& J = jobs[k];
+        const Key key{J.c[0], J.c[1], J.c[2], J.c[3], J.c[4], J.c[5], J.rho, J.dirx};
+        auto it = uniq.find(key);
+        if (it == uniq.end()) {
+            it = uniq.emplace(key, todo.size()).first;
+            todo.push_back(&J);
+        }
+        slot[k] = it->second;
+    }
+    std::vector<float> val(todo.size());
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nth; ++w)
+        pool.emplace_back([&, w] {
+            for (size_t k = w; k < todo.size(); k += nth) val[k] = mpml_damp_ratio(todo[k]->c, todo[k]->rho, todo[k]->dirx);
+        });
+    for (auto& th : pool) th.join();
+    for (size_t k = 0; k < jobs.size(); ++k) *jobs[k].out = val[slot[k]];
+}
+
+void retilef(owl_fsg2d* b) {
+    b->tile_shot.clear();
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        ShotF& S = b->sd[s];
+        S.ntx = (S.nx + 2 * b->pml + 63) / 64;
+        S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
+        S.tile0 = int(b->tile_shot.size());
+        b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nzc, int(s));
+    }
+    while (b->tile_shot.size() % size_t(b->cy)) b->tile_shot.push_back(int(b->sd.size()) - 1);
+}
+
+// Largest phase velocity of the medium over 36 directions (the 2D
+// Christoffel matrix's larger eigenvalue), for the stability check.
+float max_phase_velocity(const float* c, float rho) {
+    double vmax = 0.0;
+    for (int k = 0; k < 36; ++k) {
+        const double th = M_PI * k / 36.0, k1 = std::sin(th), k3 = std::cos(th);
+        const double g11 = c[0] * k1 * k1 + 2.0 * c[2] * k1 * k3 + c[5] * k3 * k3;
+        const double g12 = c[2] * k1 * k1 + (double(c[1]) + c[5]) * k1 * k3 + c[4] * k3 * k3;
+        const double g22 = c[5] * k1 * k1 + 2.0 * c[4] * k1 * k3 + c[3] * k3 * k3;
+        const double tr = g11 + g22, det = g11 * g22 - g12 * g12;
+        const double lam = 0.5 * (tr + std::sqrt(std::max(0.0, tr * tr - 4.0 * det)));
+        vmax = std::max(vmax, std::sqrt(std::max(0.0, lam / rho)));
+    }
+    return float(vmax);
+}
+
+void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij, const float* rho, int aniso,
+           int nshot, const owl_shot* shots) {
+    if (!cfg || !cij || !rho || nshot < 1 || !shots) fail(OWL_ERR_ARG, "null argument or no shots");
+    for (int k = 0; k < 6; ++k)
+        if (!cij[k]) fail(OWL_ERR_ARG, "null modulus array");
+    if (!fd_coefficients(cfg->fdhalf, b->coef.c)) fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    if (cfg->nx < 1 || cfg->nz < 1 || !(cfg->dx > 0) || !(cfg->dz > 0) || !(cfg->dt > 0))
+        fail(OWL_ERR_ARG, "invalid grid or dt");
+    if (cfg->pml < cfg->fdhalf || cfg->pml + cfg->fdhalf < kNkw) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
+    if (cfg->free_surface) fail(OWL_ERR_ARG, "the FSG solver here has no free surface");
+    if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+    b->cfg = *cfg;
+    b->R = cfg->fdhalf;
+    b->pml = cfg->pml;
+    b->dt = cfg->dt;
+    b->aniso = aniso ? 1 : 0;
+    b->nt = time_steps(cfg->tmax, cfg->dt);
+    if (b->nt < 2) fail(OWL_ERR_ARG, "tmax/dt gives fewer than 2 steps");
+    const size_t nfull = size_t(cfg->nx) * cfg->nz;
+    for (int k = 0; k < 6; ++k) b->cij_full[k].assign(cij[k], cij[k] + nfull);
+    b->rho_full.assign(rho, rho + nfull);
+    for (size_t e = 0; e < nfull; ++e)
+        if (!(b->rho_full[e] > 0.f)) fail(OWL_ERR_ARG, "rho must be positive");
+    const float f0factor = cfg->f0factor > 0 ? cfg->f0factor : 1.0f;
+    const int R = b->R, pml = b->pml, nt = b->nt, lo = 1 - pml - R;
+    b->hs.assign(nshot, ShotHostF{});
+    b->sd.assign(nshot, ShotF{});
+    b->nrec_total = b->max_nr = b->max_nadj = 0;
+    b->pts_full = 0;
+    long long off = 0, trace = 0, strip = 0, grad = 0;
+    std::vector<float> amp, rec_w;
+    std::vector<int> inj_beg{0}, adj_beg{0};
+    std::vector<Tap> inj, adj_taps;
+    std::vector<std::vector<Tap>> rec_taps;  // per (shot, receiver)
+    std::vector<float> vcell(nfull, -1.f);   // per-cell largest phase velocity, on demand
+    for (int s = 0; s < nshot; ++s) {
+        const owl_shot& in = shots[s];
+        ShotHostF& H = b->hs[s];
+        ShotF& S = b->sd[s];
+        H.id = in.id;
+        H.nxbeg = in.nxbeg > 0 ? in.nxbeg : 1;
+        H.nzbeg = in.nzbeg > 0 ? in.nzbeg : 1;
+        const int nxend = in.nxend > 0 ? in.nxend : cfg->nx, nzend = in.nzend > 0 ? in.nzend : cfg->nz;
+        if (H.nxbeg > nxend || nxend > cfg->nx || H.nzbeg > nzend || nzend > cfg->nz)
+            fail(OWL_ERR_ARG, "shot window outside the model");
+        H.nx = nxend - H.nxbeg + 1;
+        H.nz = nzend - H.nzbeg + 1;
+        H.ox = float(H.nxbeg - 1) * cfg->dx + cfg->ox;
+        H.oz = float(H.nzbeg - 1) * cfg->dz + cfg->oz;
+        if (in.ns < 1 || !in.sources || in.nr < 1 || !in.receivers)
+            fail(OWL_ERR_ARG, "shot without sources or receivers");
+        H.src.assign(in.sources, in.sources + in.ns);
+        H.rec.assign(in.receivers, in.receivers + in.nr);
+        float vmax = 0.f;
+        for (int i = 1; i <= H.nx; ++i)
+            for (int j = 1; j <= H.nz; ++j) {
+                const size_t m = model_indexf(b, H, i, j);
+                if (vcell[m] < 0.f) {
+                    float c[6];
+                    for (int k = 0; k < 6; ++k) c[k] = b->cij_full[k][m];
+                    vcell[m] = max_phase_velocity(c, b->rho_full[m]);
+                }
+                vmax = std::max(vmax, vcell[m]);
+            }
+        const float dts = dt_stable(R, 2, vmax, cfg->dx, 1.0f, cfg->dz);
+        if (cfg->dt > dts)
+            fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " + std::to_string(dts));
+        S.nx = H.nx;
+        S.nz = H.nz;
+        S.px = (H.nx + 2 * (pml + R) + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
+        S.nzp = H.nz + 2 * (pml + R);
+        S.off = off;
+        off += (long long)S.px * S.nzp;
+        off = (off + 63) / 64 * 64;
+        b->pts_full += double(H.nx + 2 * pml) * double(H.nz + 2 * pml);
+        S.nr = int(H.rec.size());
+        S.trace0 = trace;
+        trace += 2LL * nt * S.nr;
+        S.strip0 = strip;
+        strip += stripf_len(H.nx, H.nz, R) * nt;
+        S.g0 = grad;
+        grad += (long long)H.nx * H.nz;
+        // sources (add_source): a force adds sin/cos(polar) amp / rho(gx, gz)
+        // to vx/vz on (hx, gz) with hx iz and on (gx, hz) with ix hz; an
+        // explosion subtracts amp / (dx dz) from sxx, szz on (gx, gz) with
+        // ix iz and on (hx, hz) with hx hz; amp = stf amp dt
+        const float xmax = float(H.nx - 1) * cfg->dx + H.ox, zmax = float(H.nz - 1) * cfg->dz + H.oz;
+        std::vector<Tap> st;
+        for (const owl_source& sc : H.src) {
+            if (sc.mechanism != OWL_FORCE && sc.mechanism != OWL_EXPLOSION)
+                fail(OWL_ERR_ARG, "the FSG solver takes force and explosion sources");
+            if (!(sc.x >= H.ox && sc.x <= xmax && sc.z >= H.oz && sc.z <= zmax))
+                fail(OWL_ERR_ARG, "source outside its shot window");
+            const Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            int hnt = 0;
+            const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            if (&sc == &H.src[0]) H.hnt = hnt;
+            const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
+            auto taps = [&](int gx, int gz, const float* wx, const float* wz, int field, int row) {
+                for (int irz = -kNkw; irz <= kNkw; ++irz)
+                    for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                        const float w1 = wx[irx + kNkw], w2 = wz[irz + kNkw];
+                        if (w1 == 0.f || w2 == 0.f) continue;
+                        st.push_back(Tap{(long long)cellf(b, S, gx + irx, gz + irz), w1, w2, row, nbeg, nend, field});
+                    }
+            };
+            auto table = [&](float fac, bool divide_rho, float r) {
+                const int row = int(amp.size() / nt);
+                for (int t = 1; t <= nt; ++t) {
+                    float a = (t >= nbeg && t <= nend) ? stf[t - nbeg] * sc.amp * cfg->dt : 0.0f;
+                    if (divide_rho) {
+                        a = a / r;
+                        amp.push_back(fac * a);
+                    } else {
+                        amp.push_back(-(a / (cfg->dx * cfg->dz)));
+                    }
+                }
+                return row;
+            };
+            if (sc.mechanism == OWL_FORCE) {
+                const float r = b->rho_full[model_indexf(b, H, L.gx, L.gz)];
+                const int rs = table(std::sin(sc.polar), true, r), rc = table(std::cos(sc.polar), true, r);
+                taps(L.hx, L.gz, L.hxw, L.iz, 0, rs);
+                taps(L.hx, L.gz, L.hxw, L.iz, 1, rc);
+                taps(L.gx, L.hz, L.ix, L.hzw, 2, rs);
+                taps(L.gx, L.hz, L.ix, L.hzw, 3, rc);
+            } else {
+                const int re = table(0.f, false, 1.f);
+                taps(L.gx, L.gz, L.ix, L.iz, 4, re);
+                taps(L.gx, L.gz, L.ix, L.iz, 5, re);
+                taps(L.hx, L.hz, L.hxw, L.hzw, 6, re);
+                taps(L.hx, L.hz, L.hxw, L.hzw, 7, re);
+            }
+        }
+        std::stable_sort(st.begin(), st.end(), [](const Tap& x, const Tap& y) {
+            return x.field != y.field ? x.field < y.field : x.cell < y.cell;
+        });
+        S.inj0 = int(inj_beg.size()) - 1;
+        S.ninj = 0;
+        for (size_t k = 0; k < st.size(); ++k) {
+            if (k == 0 || st[k].cell != st[k - 1].cell || st[k].field != st[k - 1].field) {
+                if (k) inj_beg.push_back(int(inj.size()));
+                ++S.ninj;
+            }
+            inj.push_back(st[k]);
+        }
+        if (!st.empty()) inj_beg.push_back(int(inj.size()));
+        // receivers: per stencil tap the (hx, gz) cell with amp1, then the
+        // (gx, hz) cell with amp2 (outside the window: weight 0, no taps)
+        S.rec0 = b->nrec_total;
+        std::vector<Tap> at;
+        for (int r = 0; r < S.nr; ++r) {
+            const owl_receiver& rc = H.rec[r];
+            std::vector<Tap> taps;
+            const bool valid = rc.x >= H.ox && rc.x <= xmax && rc.z >= H.oz && rc.z <= zmax;
+            rec_w.push_back((valid && rc.weight != 0.f) ? rc.weight : 0.f);
+            if (valid && rc.weight != 0.f) {
+                const Located L = locate(2, rc.x, 0.0f, rc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+                for (int irz = -kNkw; irz <= kNkw; ++irz)
+                    for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                        const float a1 = L.hxw[irx + kNkw] * L.iz[irz + kNkw] * rc.weight * 0.5f;
+                        const float a2 = L.ix[irx + kNkw] * L.hzw[irz + kNkw] * rc.weight * 0.5f;
+                        if (a1 != 0.f) taps.push_back(Tap{(long long)cellf(b, S, L.hx + irx, L.gz + irz), a1, 0.f, r, 0, 0, 0});
+                        if (a2 != 0.f) taps.push_back(Tap{(long long)cellf(b, S, L.gx + irx, L.hz + irz), a2, 0.f, r, 0, 0, 2});
+                    }
+            }
+            for (const Tap& tp : taps)
+                for (int c = 0; c < 2; ++c) {
+                    Tap u = tp;
+                    u.field = tp.field + c;
+                    at.push_back(u);
+                }
+            rec_taps.push_back(std::move(taps));
+        }
+        // adjoint sources: grouped by (field, cell), receiver order within a cell
+        std::stable_sort(at.begin(), at.end(), [](const Tap& x, const Tap& y) {
+            return x.field != y.field ? x.field < y.field : x.cell < y.cell;
+        });
+        S.adj0 = int(adj_beg.size()) - 1;
+        S.nadj = 0;
+        for (size_t k = 0; k < at.size(); ++k) {
+            if (k == 0 || at[k].cell != at[k - 1].cell || at[k].field != at[k - 1].field) {
+                if (k) adj_beg.push_back(int(adj_taps.size()));
+                ++S.nadj;
+            }
+            adj_taps.push_back(at[k]);
+        }
+        if (!at.empty()) adj_beg.push_back(int(adj_taps.size()));
+        b->max_nadj = std::max(b->max_nadj, S.nadj);
+        b->nrec_total += S.nr;
+        b->max_nr = std::max(b->max_nr, S.nr);
+    }
+    b->field_elems = size_t(off);
+    b->trace_elems = size_t(trace);
+    b->strip_elems = size_t(strip);
+    b->grad_elems = size_t(grad);
+    retilef(b);
+    damping_ratios(b);
+    std::vector<int> rec_beg{0};
+    std::vector<Tap> rec;
+    for (auto& v : rec_taps) {
+        rec.insert(rec.end(), v.begin(), v.end());
+        rec_beg.push_back(int(rec.size()));
+    }
+    if (!b->stream) {
+        OWL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        OWL_HIP(hipEventCreate(&b->ev0));
+        OWL_HIP(hipEventCreate(&b->ev1));
+    }
+    hipStream_t stm = b->stream;
+    b->d_shots.upload(b->sd, stm);
+    b->d_tile_shot.upload(b->tile_shot, stm);
+    if (amp.empty()) amp.push_back(0.f);
+    b->d_amp.upload(amp, stm);
+    b->d_inj_beg.upload(inj_beg, stm);
+    if (inj.empty()) inj.push_back(Tap{});
+    b->d_inj.upload(inj, stm);
+    b->d_rec_beg.upload(rec_beg, stm);
+    if (rec.empty()) rec.push_back(Tap{});
+    b->d_rec.upload(rec, stm);
+    b->d_rec_w.upload(rec_w, stm);
+    b->d_adj_beg.upload(adj_beg, stm);
+    if (adj_taps.empty()) adj_taps.push_back(Tap{});
+    b->d_adj_taps.upload(adj_taps, stm);
+    {
+        std::vector<int> win;
+        for (auto& H : b->hs) win.insert(win.end(), {H.nxbeg, H.nzbeg});
+        b->d_win.upload(win, stm);
+    }
+    // padded media (edge replication), corner means, |dt| / rho_h,
+    // compliances, per-cell M-PML coefficients
+    const size_t n = b->field_elems;
+    std::vector<float> ci[6], ch[6], rh(n, 1.f), qhx(n, 0.f), qhz(n, 0.f), cs[6], pc[16];
+    for (int k = 0; k < 6; ++k) {
+        ci[k].assign(n, 0.f);
+        ch[k].assign(n, 0.f);
+        cs[k].assign(n, 0.f);
+    }
+    for (auto& v : pc) v.assign(n, 0.f);
+    const float adt = std::fabs(cfg->dt);
+    for (size_t s = 0; s < b->hs.size(); ++s) {
+        const ShotHostF& H = b->hs[s];
+        const ShotF& S = b->sd[s];
+        auto at = [&](int i, int j) { return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * S.px; };
+        for (int j = lo; j < lo + S.nzp; ++j)
+            for (int i = lo; i < lo + S.px; ++i) {
+                const size_t m = model_indexf(b, H, i, j), q = at(i, j);
+                for (int k = 0; k < 6; ++k) ci[k][q] = b->cij_full[k][m];
+                rh[q] = b->rho_full[m];
+            }
+        for (int j = lo + 1; j < lo + S.nzp; ++j)
+            for (int i = lo + 1; i < lo + S.nx + 2 * (pml + R); ++i) {
+                const size_t q = at(i, j), sz = S.px;
+                for (int k = 0; k < 6; ++k)
+                    ch[k][q] = 0.25f * (ci[k][q - 1 - sz] + ci[k][q - sz] + ci[k][q - 1] + ci[k][q]);
+                qhx[q] = adt / (0.5f * (rh[q - 1] + rh[q]));
+                qhz[q] = adt / (0.5f * (rh[q - sz] + rh[q]));
+            }
+        for (int j = 1; j <= S.nz; ++j)
+            for (int i = 1; i <= S.nx; ++i) {
+                const size_t q = at(i, j);
+                const float c11 = ci[0][q], c13 = ci[1][q], c15 = ci[2][q], c33 = ci[3][q], c35 = ci[4][q],
+                            c55 = ci[5][q];
+                const float t1 = c15 * c15 * c33 - 2 * c13 * c15 * c35 + c11 * (c35 * c35) + c13 * c13 * c55 -
+                                 c11 * c33 * c55;
+                const float t2 = c15 * c15 * c33 - 2 * c13 * c15 * c35 + c13 * c13 * c55 + c11 * (c35 * c35 - c33 * c55);
+                cs[0][q] = (c35 * c35 - c33 * c55) / t1;
+                cs[1][q] = (-(c15 * c35) + c13 * c55) / t2;
+                cs[2][q] = (c15 * c33 - c13 * c35) / t1;
+                cs[3][q] = (c15 * c15 - c11 * c55) / t1;
+                cs[4][q] = (-(c13 * c15) + c11 * c35) / t2;
+                cs[5][q] = (c13 * c13 - c11 * c33) / t1;
+            }
+        // pmlvp = sqrt(max c11/rho, c33/rho over the window's edge lines)
+        // (module_vars.f90:345-349); alphamax = max f0 pi
+        float pm = 0.f, f0max = 0.f;
+        for (int i = 1; i <= H.nx; ++i)
+            for (int j = 1; j <= H.nz; ++j) {
+                if (!(i == 1 || i == H.nx || j == 1 || j == H.nz)) continue;
+                const size_t m = model_indexf(b, H, i, j);
+                pm = std::max(pm, b->cij_full[0][m] / b->rho_full[m]);
+                pm = std::max(pm, b->cij_full[3][m] / b->rho_full[m]);
+            }
+        for (auto& sc : H.src) f0max = std::max(f0max, sc.f0);
+        const float pmlvp = std::sqrt(pm), am = float(double(1.0f * f0max) * M_PI);
+        const int nx = H.nx, nz = H.nz;
+        for (int j = -pml + 1; j <= nz + pml; ++j)
+            for (int i = -pml + 1; i <= nx + pml; ++i) {
+                if (i > 1 && i < nx && j > 1 && j < nz) continue;
+                float xi = 0, xh = 0, zi = 0, zh = 0;
+                if (i <= 1) {
+                    xi = float(std::fabs((i - 1 + 0.0) / pml));
+                    xh = float(std::fabs((i - 1 - 0.5) / pml));
+                } else if (i >= nx) {
+                    xi = float(std::fabs((i - nx + 0.0) / pml));
+                    xh = float(std::fabs((i - nx - 0.5) / pml));
+                }
+                if (j <= 1) {
+                    zi = float(std::fabs((j - 1 + 0.0) / pml));
+                    zh = float(std::fabs((j - 1 - 0.5) / pml));
+                } else if (j >= nz) {
+                    zi = float(std::fabs((j - nz + 0.0) / pml));
+                    zh = float(std::fabs((j - nz - 0.5) / pml));
+                }
+                float rx = 0.f, rz = 0.f;
+                if (i <= 1)
+                    rx = H.ratio[0][j + pml];
+                else if (i >= nx)
+                    rx = H.ratio[1][j + pml];
+                if (j <= 1)
+                    rz = H.ratio[2][i + pml];
+                else if (j >= nz)
+                    rz = H.ratio[3][i + pml];
+                const size_t q = at(i, j);
+                const float dxf = cfg->dx, dzf = cfg->dz;
+                // grids: ii (xi, zi), hh (xh, zh), hi (xh, zi), ih (xi, zh)
+                damp_coef_mpml(pmlvp, xi, zi, dxf, dzf, pml, adt, am, rx, rz, &pc[0][q], &pc[1][q], &pc[2][q], &pc[3][q]);
+                damp_coef_mpml(pmlvp, xh, zh, dxf, dzf, pml, adt, am, rx, rz, &pc[4][q], &pc[5][q], &pc[6][q], &pc[7][q]);
+                damp_coef_mpml(pmlvp, xh, zi, dxf, dzf, pml, adt, am, rx, rz, &pc[8][q], &pc[9][q], &pc[10][q], &pc[11][q]);
+                damp_coef_mpml(pmlvp, xi, zh, dxf, dzf, pml, adt, am, rx, rz, &pc[12][q], &pc[13][q], &pc[14][q],
+                               &pc[15][q]);
+            }
+    }
+    for (int k = 0; k < 6; ++k) {
+        b->med[k].upload(ci[k], stm);
+        b->med[6 + k].upload(ch[k], stm);
+        b->comp[k].upload(cs[k], stm);
+    }
+    b->med[12].upload(qhx, stm);
+    b->med[13].upload(qhz, stm);
+    for (int k = 0; k < 16; ++k) b->pmlc[k].upload(pc[k], stm);
+    for (auto& f : b->fld) f.alloc(b->field_elems);
+    b->d_traces.alloc(std::max<size_t>(b->trace_elems, 1));
+    OWL_HIP(hipStreamSynchronize(stm));
+}
+
+FF fieldsf(DevBuf<float>* f) {
+    FF F;
+    float** p = &F.sxx_ii;
+    for (int k = 0; k < kNF; ++k) p[k] = f[k].get();
+    return F;
+}
+FV out_of(const FF& F) {
+    FV V;
+    float* const* s = &F.sxx_ii;
+    float** d = &V.sxx_ii;
+    for (int k = 0; k < kNW; ++k) d[k] = s[k];
+    return V;
+}
+FV fv_of(DevBuf<float>* f) {
+    FV V;
+    float** d = &V.sxx_ii;
+    for (int k = 0; k < kNW; ++k) d[k] = f[k].get();
+    return V;
+}
+FF with_fields(FF F, const FV& V) {
+    float** d = &F.sxx_ii;
+    float* const* s = &V.sxx_ii;
+    for (int k = 0; k < kNW; ++k) d[k] = s[k];
+    return F;
+}
+Targets vel_targets(const FV& V) { return Targets{{V.vx_hi, V.vz_hi, V.vx_ih, V.vz_ih, V.sxx_ii, V.szz_ii, V.sxx_hh, V.szz_hh}}; }
+
+FM mediaf(owl_fsg2d* b) {
+    FM M;
+    for (int k = 0; k < 6; ++k) {
+        M.ci[k] = b->med[k].get();
+        M.ch[k] = b->med[6 + k].get();
+    }
+    M.qhx = b->med[12].get();
+    M.qhz = b->med[13].get();
+    return M;
+}
+FP pmlf(owl_fsg2d* b) {
+    FP P;
+    const float** d = &P.axii;
+    for (int k = 0; k < 16; ++k) d[k] = b->pmlc[k].get();
+    return P;
+}
+
+// Algorithmic bytes per sweep: the stress sweep reads four velocities, six
+// old stresses and twelve moduli and writes six stresses (112 B per point);
+// the velocity sweep reads six stresses, four old velocities and two
+// |dt| / rho and writes four velocities (64 B); in the damping frame each
+// grid adds four memory variables (r+w) and four coefficients.
+void sweep_bytesf(const owl_fsg2d* b, double& bs, double& bv) {
+    bs = bv = 0;
+    for (auto& S : b->sd) {
+        const double lx = S.nx + 2.0 * b->pml, lz = S.nz + 2.0 * b->pml;
+        const double frame = lx * lz - double(std::max(S.nx - 2, 0)) * double(std::max(S.nz - 2, 0));
+        bs += 112.0 * lx * lz + 2.0 * 48.0 * frame;
+        bv += 64.0 * lx * lz + 2.0 * 48.0 * frame;
+    }
+}
+
+void maybe_collectf(owl_fsg2d* b) {
+    if (b->prof.pending() > 20000) {
+        OWL_HIP(hipStreamSynchronize(b->stream));
+        b->prof.collect();
+    }
+}
+
+template <class Fn>
+void dispatch_Rf(int R, Fn&& f) {
+    switch (R) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 6: f(std::integral_constant<int, 6>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    }
+}
+
+// One time step of F (memory variables in F) into O (O may be F's own
+// fields): stresses, then velocities (update_wavefield).
+template <int R>
+void stepf(owl_fsg2d* b, const FF& F, const FV& O, float dt, double bs, double bv) {
+    if (64 * b->cy > 256) fail(OWL_ERR_ARG, "launch shape exceeds the sweeps' 256-thread blocks");
+    hipStream_t st = b->stream;
+    const dim3 blk(64, unsigned(b->cy)), grd(unsigned(b->tile_shot.size() / size_t(b->cy)));
+    const FM M = mediaf(b);
+    const FP P = pmlf(b);
+    int ev = b->prof.begin(st, 0, bs);
+    hipLaunchKernelGGL(k_fsg_stress<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M, P, b->coef,
+                       dt, b->cfg.dx, b->cfg.dz, b->pml, b->kc);
+    b->prof.end(st, ev);
+    ev = b->prof.begin(st, 1, bv);
+    hipLaunchKernelGGL(k_fsg_velocity<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M, P,
+                       b->coef, dt < 0 ? -1.0f : 1.0f, b->cfg.dx, b->cfg.dz, b->pml, b->kc);
+    b->prof.end(st, ev);
+}
+
+void stripsf(owl_fsg2d* b, const FV& V, int t, int save) {
+    long long mx = 1;
+    for (auto& S : b->sd) mx = std::max(mx, stripf_len(S.nx, S.nz, b->R));
+    const dim3 grd(unsigned(std::min<long long>(1024, (mx + 255) / 256)), unsigned(b->sd.size()));
+    const int ev = b->prof.begin(b->stream, 4, 0);
+    dispatch_Rf(b->R, [&](auto r) {
+        hipLaunchKernelGGL(k_fsg_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(),
+                           vel_targets(V), b->d_strips.get(), t, b->pml, save);
+    });
+    b->prof.end(b->stream, ev);
+}
+
+template <int R>
+void run_forwardf(owl_fsg2d* b, bool save_boundary) {
+    hipStream_t st = b->stream;
+    for (auto& f : b->fld) f.zero(st);
+    b->d_traces.zero(st);
+    if (save_boundary) b->d_strips.alloc(std::max<size_t>(b->strip_elems, 1));
+    const FF F = fieldsf(b->fld);
+    const FV V = out_of(F);
+    const unsigned nshot = unsigned(b->sd.size());
+    double bs, bv;
+    sweep_bytesf(b, bs, bv);
+    const unsigned rg = unsigned(std::max(1, (2 * b->max_nr + 255) / 256));
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    // step t: [boundary record] -> record(t) -> update -> source(t)
+    // (submodule_forward_modeling.f90:55-240)
+    for (int t = 1; t <= b->nt; ++t) {
+        if (save_boundary) stripsf(b, V, t, 1);
+        int ev = b->prof.begin(st, 2, 0);
+        hipLaunchKernelGGL(k_fsg_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), vel_targets(V),
+                           b->d_rec_beg.get(), b->d_rec.get(), b->d_traces.get(), b->nt, t);
+        b->prof.end(st, ev);
+        if (t == b->nt) break;  // the last update changes nothing that is recorded
+        stepf<R>(b, F, V, b->dt, bs, bv);
+        ev = b->prof.begin(st, 2, 0);
+        hipLaunchKernelGGL(k_fsg_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), vel_targets(V),
+                           b->d_inj_beg.get(), b->d_inj.get(), b->d_amp.get(), b->nt, t);
+        b->prof.end(st, ev);
+        maybe_collectf(b);
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    b->prof.collect();
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = double(b->nt - 1) * b->pts_full;
+    b->have_boundary = save_boundary;
+    b->fwd_state_valid = save_boundary;
+}
+
+// OWL's reverse pass (submodule_fwi_gradient.f90:150-405): t = nt .. hnt:
+// prev = state; state = update(state, -dt); at t = nt the final state;
+// boundary record t; adjoint = update(adjoint, -dt); adjoint sources;
+// imaging.  The forward field steps out of place so the previous state
+// stays in the other buffer; at t = nt the update of the zero field is the
+// zero field, so the final state is used as is with prev = 0.  OWL reads
+// the final state back with vz_hxiz named twice
+// (module_boundary_saving.f90:85-90): the saved vz_ih lands in vz_hi and
+// vz_ih keeps the zero of the reconstruction; restore_all = 0 keeps that
+// order, 1 restores every field.  OWL's loop continues the adjoint below
+// hnt without imaging; those steps do not touch the gradients.
+template <int R>
+void run_reversef(owl_fsg2d* b) {
+    hipStream_t st = b->stream;
+    if (!b->have_boundary || !b->fwd_state_valid)
+        fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
+    if (!b->d_adj.get() || b->d_adj.size() < b->trace_elems) fail(OWL_ERR_STATE, "no adjoint source");
+    for (auto& H : b->hs)
+        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one gradient batch must share hnt");
+    for (int k = kNW; k < kNF; ++k) b->fld[k].zero(st);  // the reverse run's memory variables start at zero
+    for (auto& f : b->fB) {
+        f.alloc(b->field_elems);
+        f.zero(st);
+    }
+    for (auto& f : b->aF) {
+        f.alloc(b->field_elems);
+        f.zero(st);
+    }
+    const size_t ng = std::max<size_t>(b->grad_elems, 1);
+    Acc7 G;
+    for (int k = 0; k < 7; ++k) {
+        b->d_g[k].alloc(ng);
+        b->d_g[k].zero(st);
+        G.g[k] = b->d_g[k].get();
+    }
+    Comp C;
+    for (int k = 0; k < 6; ++k) C.s[k] = b->comp[k].get();
+    const unsigned nshot = unsigned(b->sd.size());
+    const int hnt = std::max(b->hs[0].hnt, 1);
+    const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
+    double bs, bv;
+    sweep_bytesf(b, bs, bv);
+    FF X = fieldsf(b->fld);
+    FV Y = fv_of(b->fB);
+    const FF A = fieldsf(b->aF);
+    const FV adj = out_of(A);
+    if (!b->restore_all) {
+        // the final state as OWL reads it back
+        OWL_HIP(hipMemcpyAsync(X.vz_hi, X.vz_ih, b->field_elems * sizeof(float), hipMemcpyDeviceToDevice, st));
+        b->fld[9].zero(st);
+    }
+    // imaging: 30 wavefields, 6 compliances, seven FP64 accumulators r+w
+    const double image_bytes = (30.0 * 4 + 24.0 + 112.0) * double(b->grad_elems);
+    const unsigned ig = unsigned(std::max<size_t>(1, std::min<size_t>(2048, (b->grad_elems / nshot + 255) / 256)));
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    for (int t = b->nt; t >= hnt; --t) {
+        FV cur = out_of(X);
+        const FV prev = out_of(X);
+        if (t != b->nt) {
+            stepf<R>(b, X, Y, -b->dt, bs, bv);
+            cur = Y;
+        }
+        stripsf(b, cur, t, 0);
+        stepf<R>(b, A, adj, -b->dt, bs, bv);
+        int ev = b->prof.begin(st, 2, 0);
+        if (b->max_nadj > 0)
+            hipLaunchKernelGGL(k_fsg_inject_adjoint, dim3(unsigned((b->max_nadj + 255) / 256), nshot), dim3(256), 0,
+                               st, b->d_shots.get(), vel_targets(adj), b->d_adj_beg.get(), b->d_adj_taps.get(),
+                               b->d_adj.get(), b->nt, t);
+        b->prof.end(st, ev);
+        if (t % cc == 0) {
+            ev = b->prof.begin(st, 3, image_bytes);
+            hipLaunchKernelGGL(k_fsg_image, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(), cur, prev, adj, C, G,
+                               t == b->nt ? 1 : 0, b->pml, R);
+            b->prof.end(st, ev);
+        }
+        if (t != b->nt) {
+            const FV old = out_of(X);
+            X = with_fields(X, Y);
+            Y = old;
+        }
+        maybe_collectf(b);
+    }
+    OWL_HIP(hipGetLastError());
+    OWL_HIP(hipEventRecord(b->ev1, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    b->prof.collect();
+    float ms = 0.f;
+    OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_updates = 2.0 * double(b->nt - hnt) * b->pts_full;
+    b->fwd_state_valid = false;
+}
+
+void traces_to_hostf(const owl_fsg2d* b, const float* dev, float* out) {
+    std::vector<float> h(b->trace_elems);
+    OWL_HIP(hipMemcpy(h.data(), dev, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (auto& S : b->sd)
+        for (int c = 0; c < 2; ++c)
+            for (int r = 0; r < S.nr; ++r)
+                for (int t = 0; t < b->nt; ++t)
+                    out[size_t(S.trace0) + (size_t(c) * S.nr + r) * b->nt + t] =
+                        h[size_t(S.trace0) + size_t(c) * b->nt * S.nr + size_t(t) * S.nr + r];
+}
+
+void traces_to_devicef(owl_fsg2d* b, const float* in, DevBuf<float>& dev) {
+    std::vector<float> h(b->trace_elems);
+    for (auto& S : b->sd)
+        for (int c = 0; c < 2; ++c)
+            for (int r = 0; r < S.nr; ++r)
+                for (int t = 0; t < b->nt; ++t)
+                    h[size_t(S.trace0) + size_t(c) * b->nt * S.nr + size_t(t) * S.nr + r] =
+                        in[size_t(S.trace0) + (size_t(c) * S.nr + r) * b->nt + t];
+    dev.alloc(std::max<size_t>(h.size(), 1));
+    dev.upload(h.data(), h.size(), b->stream);
+    OWL_HIP(hipStreamSynchronize(b->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+owl_status owl_fsg2d_create(const owl_acoustic_config* cfg, const float* const* cij, const float* rho, int aniso,
+                            int nshot, const owl_shot* shots, owl_fsg2d** out) {
+    return guarded([&] {
+        if (!out) fail(OWL_ERR_ARG, "out is null");
+        *out = nullptr;
+        auto* b = new owl_fsg2d();
+        try {
+            planf(b, cfg, cij, rho, aniso, nshot, shots);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+void owl_fsg2d_destroy(owl_fsg2d* b) { delete b; }
+
+int owl_fsg2d_nt(const owl_fsg2d* b) { return b ? b->nt : 0; }
+
+owl_status owl_fsg2d_damping_ratios(const owl_fsg2d* b, int shot, float* left, float* right, float* top,
+                                    float* bottom) {
+    return guarded([&] {
+        if (!b || shot < 0 || shot >= int(b->hs.size())) fail(OWL_ERR_ARG, "bad shot index");
+        const ShotHostF& H = b->hs[size_t(shot)];
+        float* dst[4] = {left, right, top, bottom};
+        for (int k = 0; k < 4; ++k)
+            if (dst[k]) std::copy(H.ratio[k].begin(), H.ratio[k].end(), dst[k]);
+    });
+}
+
+owl_status owl_fsg2d_damp_ratio(const float* c, float rho, int dirx, float* ratio) {
+    return guarded([&] {
+        if (!c || !ratio) fail(OWL_ERR_ARG, "null argument");
+        *ratio = mpml_damp_ratio(c, rho, dirx != 0);
+    });
+}
+
+owl_status owl_fsg2d_set_restore_all(owl_fsg2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->restore_all = on ? 1 : 0;
+    });
+}
+
+owl_status owl_fsg2d_forward(owl_fsg2d* b, float* traces, int save_boundary) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        dispatch_Rf(b->R, [&](auto r) { run_forwardf<decltype(r)::value>(b, save_boundary != 0); });
+        if (traces) traces_to_hostf(b, b->d_traces.get(), traces);
+    });
+}
+
+owl_status owl_fsg2d_adjoint(owl_fsg2d* b, const float* adj, float* const* grads) {
+    return guarded([&] {
+        if (!b || !adj) fail(OWL_ERR_ARG, "null argument");
+        traces_to_devicef(b, adj, b->d_adj);
+        dispatch_Rf(b->R, [&](auto r) { run_reversef<decltype(r)::value>(b); });
+        if (!grads) return;
+        std::vector<double> g(b->grad_elems);
+        for (int k = 0; k < 7; ++k) {
+            if (!grads[k]) continue;
+            OWL_HIP(hipMemcpy(g.data(), b->d_g[k].get(), g.size() * sizeof(double), hipMemcpyDeviceToHost));
+            // per-shot window gradients (nz_s, nx_s), z fastest, NaN/Inf -> 0
+            for (const ShotF& S : b->sd)
+                for (int kz = 0; kz < S.nz; ++kz)
+                    for (int i = 0; i < S.nx; ++i) {
+                        const float v = float(g[size_t(S.g0) + i + size_t(kz) * S.nx]);
+                        grads[k][size_t(S.g0) + kz + size_t(i) * S.nz] = std::isfinite(v) ? v : 0.f;
+                    }
+        }
+    });
+}
+
+owl_status owl_fsg2d_gradient(owl_fsg2d* b, const float* obs, double* misfit, double* const* grads,
+                              float* traces_out) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (obs) {
+            traces_to_devicef(b, obs, b->d_obs);
+            b->have_obs = true;
+        }
+        if (!b->have_obs) fail(OWL_ERR_STATE, "no observed data");
+        dispatch_Rf(b->R, [&](auto r) { run_forwardf<decltype(r)::value>(b, true); });
+        const double fwd_ms = b->last_ms, fwd_up = b->last_updates;
+        hipStream_t st = b->stream;
+        const unsigned nshot = unsigned(b->sd.size());
+        b->d_adj.alloc(std::max<size_t>(b->trace_elems, 1));
+        b->d_misfit.alloc(2 * size_t(nshot));
+        b->d_scratch.alloc(std::max<size_t>(4 * size_t(b->nrec_total), 1));
+        hipLaunchKernelGGL(k_adjoint_source_2c<ShotF>, dim3(nshot, 2), dim3(256), 0, st, b->d_shots.get(),
+                           b->d_traces.get(), b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(),
+                           b->d_misfit.get(), b->nt, 1.0e-6f, b->d_scratch.get());
+        dispatch_Rf(b->R, [&](auto r) { run_reversef<decltype(r)::value>(b); });
+        const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
+        b->d_full.alloc(nfull);
+        const unsigned mg = unsigned(std::min<size_t>(8192, (nfull + 255) / 256));
+        std::vector<double> h(nfull);
+        for (int k = 0; k < 7; ++k) {
+            if (!grads || !grads[k]) continue;
+            hipLaunchKernelGGL(k_fsg_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), b->d_win.get(), int(nshot),
+                               b->d_g[k].get(), b->d_full.get(), b->cfg.nx, b->cfg.nz);
+            OWL_HIP(hipGetLastError());
+            OWL_HIP(hipMemcpyAsync(h.data(), b->d_full.get(), nfull * sizeof(double), hipMemcpyDeviceToHost, st));
+            OWL_HIP(hipStreamSynchronize(st));
+            for (size_t e = 0; e < nfull; ++e) grads[k][e] = std::isfinite(h[e]) ? h[e] : 0.0;
+        }
+        OWL_HIP(hipStreamSynchronize(st));
+        if (misfit) {
+            std::vector<double> m2(2 * size_t(nshot));
+            OWL_HIP(hipMemcpy(m2.data(), b->d_misfit.get(), m2.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (unsigned s = 0; s < nshot; ++s) misfit[s] = m2[2 * s] + m2[2 * s + 1];
+        }
+        if (traces_out) traces_to_hostf(b, b->d_traces.get(), traces_out);
+        b->last_ms += fwd_ms;
+        b->last_updates += fwd_up;
+    });
+}
+
+owl_status owl_fsg2d_set_launch(owl_fsg2d* b, int planes, int rows) {
+    return guarded([&] {
+        if (!b || planes < 1 || (rows != 1 && rows != 2 && rows != 4)) fail(OWL_ERR_ARG, "bad launch shape");
+        b->kc = planes;
+        b->cy = rows;
+        retilef(b);
+        b->d_shots.upload(b->sd, b->stream);
+        b->d_tile_shot.upload(b->tile_shot, b->stream);
+        OWL_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+owl_status owl_fsg2d_set_profiling(owl_fsg2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->prof.enable(on != 0);
+    });
+}
+
+owl_status owl_fsg2d_kernel_stats(const owl_fsg2d* b, int kind, double* total_ms, long long* launches,
+                                  double* bytes_total) {
+    return guarded([&] {
+        if (!b || kind < 0 || kind >= Profiler::kKinds) fail(OWL_ERR_ARG, "bad kernel kind");
+        if (total_ms) *total_ms = b->prof.ms(kind);
+        if (launches) *launches = b->prof.launches(kind);
+        if (bytes_total) *bytes_total = b->prof.bytes(kind);
+    });
+}
+
+owl_status owl_fsg2d_last_stats(const owl_fsg2d* b, double* ms, double* updates) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (ms) *ms = b->last_ms;
+        if (updates) *updates = b->last_updates;
+    });
+}
+
+}  // extern "C"

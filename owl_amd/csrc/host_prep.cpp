@@ -175,4 +175,91 @@ PmlAxis cfspml_axis(int n, int pml, float d, float dt, float pmlvp, float f0max)
     return P;
 }
 
+// ---------------------------------------------------------------------------
+// M-PML damping ratio of a general 2D anisotropic medium
+// (elastic_tti_2d/module_cfspml.f90:80-280, Meza-Fajardo and Papageorgiou
+// 2008 extended to anisotropy).  A = trace and B = minus the determinant of
+// the 2D Christoffel matrix; for a stable medium both squared eigenvalues
+// -(A rho +- sqrt((A^2 + 4B) rho^2)) / (2 rho^2) are negative reals, so the
+// eigenvalues are purely imaginary and the complex arithmetic of
+// mpml_eigenvalue_derivative reduces to the real arithmetic below, in the
+// same order.  Integer powers as gfortran expands them.
+// ---------------------------------------------------------------------------
+namespace {
+
+inline float p2(float x) { return x * x; }
+inline float p3(float x) { return (x * x) * x; }
+inline float p4(float x) {
+    const float y = x * x;
+    return y * y;
+}
+inline float chr_a(float c11, float c15, float c33, float c35, float c55, float k1, float k3) {
+    return (c11 + c55) * p2(k1) + 2 * (c15 + c35) * k1 * k3 + (c33 + c55) * p2(k3);
+}
+inline float chr_b(float c11, float c13, float c15, float c33, float c35, float c55, float k1, float k3) {
+    return (p2(c15) - c11 * c55) * p4(k1) + 2 * (c13 * c15 - c11 * c35) * p3(k1) * k3 +
+           (p2(c13) - c11 * c33 - 2 * c15 * c35 + 2 * c13 * c55) * p2(k1) * p2(k3) +
+           2 * (-(c15 * c33) + c13 * c35) * k1 * p3(k3) + (p2(c35) - c33 * c55) * p4(k3);
+}
+// real part of eigval**2 for mode 1 or 2 (mpml_eigenvalue, :80-112)
+float eigen_sq(const float* c, float rho, int mode, float k1, float k3) {
+    const float A = chr_a(c[0], c[2], c[3], c[4], c[5], k1, k3);
+    const float B = chr_b(c[0], c[1], c[2], c[3], c[4], c[5], k1, k3);
+    const float s = std::sqrt((p2(A) + 4 * B) * p2(rho));
+    const float negx = mode == 1 ? (A * rho + s) / p2(rho) : -((-(A * rho) + s) / p2(rho));
+    const float b = std::sqrt(negx) / std::sqrt(2.0f);
+    return -(b * b);
+}
+// real part of mpml_eigenvalue_derivative (:114-160) for scaled moduli c
+float eigen_deriv(const float* c, float rho, bool dirx, float e2, float ratio, float k1, float k3) {
+    const float c11 = c[0], c13 = c[1], c15 = c[2], c33 = c[3], c35 = c[4], c55 = c[5];
+    const float e4 = e2 * e2;
+    const float den = 2 * chr_b(c11, c13, c15, c33, c35, c55, k1, k3) -
+                      3 * e2 * chr_a(c11, c15, c33, c35, c55, k1, k3) * rho - 4 * e4 * p2(rho);
+    float num;
+    if (dirx) {
+        num = 2 * c15 * c35 * p2(k1) * p2(k3) + 3 * c15 * c33 * k1 * p3(k3) - 2 * p2(c35) * p4(k3) +
+              2 * c33 * c55 * p4(k3) - 2 * p2(c15) * p4(k1) * ratio + 2 * c15 * c35 * p2(k1) * p2(k3) * ratio +
+              c15 * c33 * k1 * p3(k3) * ratio - p2(c13) * p2(k1) * p2(k3) * (1 + ratio) -
+              c13 * k1 * k3 * (c15 * p2(k1) * (1 + 3 * ratio) + k3 * (2 * c55 * k1 * (1 + ratio) + c35 * k3 * (3 + ratio))) +
+              e2 * (k3 * (3 * c15 * k1 * (1 + ratio) + 3 * c35 * k1 * (1 + ratio) + c33 * k3 * (2 + ratio)) +
+                    c55 * (p2(k3) * (2 + ratio) + p2(k1) * (1 + 2 * ratio))) * rho +
+              2 * e4 * (1 + ratio) * p2(rho) +
+              c11 * p2(k1) * (2 * c55 * p2(k1) * ratio + c33 * p2(k3) * (1 + ratio) + c35 * k1 * k3 * (1 + 3 * ratio) +
+                              e2 * (1 + 2 * ratio) * rho);
+    } else {
+        num = -2 * p2(c15) * p4(k1) +
+              p2(k3) * (-2 * (p2(c35) - c33 * c55) * p2(k3) * ratio - p2(c13) * p2(k1) * (1 + ratio) -
+                        c13 * k1 * (2 * c55 * k1 * (1 + ratio) + c35 * k3 * (1 + 3 * ratio))) +
+              e2 * (k3 * (3 * c35 * k1 * (1 + ratio) + c33 * k3 * (1 + 2 * ratio)) +
+                    c55 * (p2(k1) * (2 + ratio) + p2(k3) * (1 + 2 * ratio))) * rho +
+              2 * e4 * (1 + ratio) * p2(rho) +
+              c15 * k1 * k3 * (-(c13 * p2(k1) * (3 + ratio)) + k3 * (2 * c35 * k1 * (1 + ratio) + c33 * k3 * (1 + 3 * ratio)) +
+                               3 * e2 * (1 + ratio) * rho) +
+              c11 * p2(k1) * (2 * c55 * p2(k1) + k3 * (c33 * k3 * (1 + ratio) + c35 * k1 * (3 + ratio)) +
+                              e2 * (2 + ratio) * rho);
+    }
+    return num / den;
+}
+
+}  // namespace
+
+float mpml_damp_ratio(const float* c, float rho, bool dirx) {
+    const float dtheta = float(kPi / 180.0), alpha = -0.01f, dratio = 0.002f, sc = 1.0e-9f;
+    const float d[6] = {c[0] * sc, c[1] * sc, c[2] * sc, c[3] * sc, c[4] * sc, c[5] * sc};
+    const float dr = rho * sc;
+    float ratio = 0.0f;
+    for (int i = 1; i <= 180; ++i) {
+        const float theta = (float(i) + 0.5f) * dtheta;
+        const float kx = std::sin(theta), kz = std::cos(theta);
+        const float e1 = eigen_sq(c, rho, 1, kx, kz), e2 = eigen_sq(c, rho, 2, kx, kz);
+        float dg = std::fmax(eigen_deriv(d, dr, dirx, e1, ratio, kx, kz), eigen_deriv(d, dr, dirx, e2, ratio, kx, kz));
+        while (dg >= alpha && ratio < 1.0f) {
+            ratio = ratio + dratio;
+            dg = std::fmax(eigen_deriv(d, dr, dirx, e1, ratio, kx, kz), eigen_deriv(d, dr, dirx, e2, ratio, kx, kz));
+        }
+    }
+    return ratio;
+}
+
 }  // namespace owl

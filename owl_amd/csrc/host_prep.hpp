@@ -62,6 +62,11 @@ PmlAxis cfspml_axis(int n, int pml, float d, float dt, float pmlvp, float f0max)
 
 // Integer-node / half-node PML masks as used by the update sweeps
 // (acoustic_iso_2d/module_wavefield.f90:58-65, 79-82, 96-99).
+// M-PML damping ratio of a 2D anisotropic medium at one boundary cell
+// (elastic_tti_2d/module_cfspml.f90:164-219): c = c11, c13, c15, c33, c35,
+// c55; dirx selects the x (left/right) or z (top/bottom) direction.
+float mpml_damp_ratio(const float* c, float rho, bool dirx);
+
 inline bool pml_mask_int(int i, int n) { return i <= 0 || i >= n + 1; }
 inline bool pml_mask_half(int m, int n) { return m <= 1 || m >= n + 1; }
 

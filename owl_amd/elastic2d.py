@@ -16,55 +16,61 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-class Elastic2DBatch:
-    """vp, vs, rho: (nz, nx).  shots: list of (Geometry, window) where window
-    is None (full model) or an object/dict with nxbeg, nxend, nzbeg, nzend
-    (1-based)."""
+def shot_array_2d(shots, nx, nz):
+    """owl_shot array for a list of (Geometry, window) pairs, window None
+    (full model) or an object/dict with nxbeg, nxend, nzbeg, nzend
+    (1-based).  Returns (array, objects to keep alive, nr per shot,
+    (nz_s, nx_s) per shot)."""
+    arr = (_lib.owl_shot * len(shots))()
+    keep, nr, windows = [], [], []
+    for k, (geom, win) in enumerate(shots):
+        srcs = (_lib.owl_source * geom.ns)()
+        for i, s in enumerate(geom.srcr):
+            srcs[i] = _lib.owl_source(s.x, 0.0, s.z, s.f0, s.amp, s.t0, s.polar, 0.0,
+                                      _lib.MECHANISMS[s.mechanism], _lib.WAVELETS[s.wavelet])
+        recs = (_lib.owl_receiver * geom.nr)()
+        for i, r in enumerate(geom.recr):
+            recs[i] = _lib.owl_receiver(r.x, 0.0, r.z, r.weight)
+        keep += [srcs, recs]
+        w = win if isinstance(win, dict) else ({} if win is None else
+                                               dict(nxbeg=win.nxbeg, nxend=win.nxend,
+                                                    nzbeg=win.nzbeg, nzend=win.nzend))
+        arr[k] = _lib.owl_shot(geom.id, geom.ns, C.cast(srcs, C.POINTER(_lib.owl_source)),
+                               geom.nr, C.cast(recs, C.POINTER(_lib.owl_receiver)),
+                               w.get("nxbeg", 0), w.get("nxend", 0), 1, 1,
+                               w.get("nzbeg", 0), w.get("nzend", 0))
+        nr.append(geom.nr)
+        windows.append(((w.get("nzend", 0) or nz) - (w.get("nzbeg", 0) or 1) + 1,
+                        (w.get("nxend", 0) or nx) - (w.get("nxbeg", 0) or 1) + 1))
+    return arr, keep, nr, windows
 
+
+class TwoComponentBatch:
+    """Shared host side of the 2D elastic batches: two-component (vx, vz)
+    traces, launch shape, profiling.  Subclasses set PREFIX (the C-ABI
+    function prefix) and create self._h."""
+
+    PREFIX = ""
     KERNELS = ("stress", "velocity", "sources_receivers", "imaging", "boundary_records")
 
-    def __init__(self, nx, nz, dx, dz, vp, vs, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
-                 f0factor=1.0, cc_step_interval=1):
+    def _fn(self, name):
+        return getattr(lib(), f"{self.PREFIX}_{name}")
+
+    def _setup(self, nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval):
         self.shape = (nz, nx)
         self.cfg = _lib.owl_acoustic_config(nx, 1, nz, dx, 1.0, dz, ox, 0.0, oz, dt, tmax, dt, tmax,
                                             pml, fdhalf, 0, f0factor, cc_step_interval)
-        self._m = [np.asfortranarray(a, dtype=np.float32) for a in (vp, vs, rho)]
-        for a in self._m:
-            if a.shape != self.shape:
-                raise ValueError(f"models must have shape {self.shape} (nz, nx)")
-        arr = (_lib.owl_shot * len(shots))()
-        self._keep = []
-        self.nr = []
-        self._windows = []
-        for k, (geom, win) in enumerate(shots):
-            srcs = (_lib.owl_source * geom.ns)()
-            for i, s in enumerate(geom.srcr):
-                srcs[i] = _lib.owl_source(s.x, 0.0, s.z, s.f0, s.amp, s.t0, s.polar, 0.0,
-                                          _lib.MECHANISMS[s.mechanism], _lib.WAVELETS[s.wavelet])
-            recs = (_lib.owl_receiver * geom.nr)()
-            for i, r in enumerate(geom.recr):
-                recs[i] = _lib.owl_receiver(r.x, 0.0, r.z, r.weight)
-            self._keep += [srcs, recs]
-            w = win if isinstance(win, dict) else ({} if win is None else
-                                                   dict(nxbeg=win.nxbeg, nxend=win.nxend,
-                                                        nzbeg=win.nzbeg, nzend=win.nzend))
-            arr[k] = _lib.owl_shot(geom.id, geom.ns, C.cast(srcs, C.POINTER(_lib.owl_source)),
-                                   geom.nr, C.cast(recs, C.POINTER(_lib.owl_receiver)),
-                                   w.get("nxbeg", 0), w.get("nxend", 0), 1, 1,
-                                   w.get("nzbeg", 0), w.get("nzend", 0))
-            self.nr.append(geom.nr)
-            self._windows.append(((w.get("nzend", 0) or nz) - (w.get("nzbeg", 0) or 1) + 1,
-                                  (w.get("nxend", 0) or nx) - (w.get("nxbeg", 0) or 1) + 1))
-        self._shots = arr
-        h = C.c_void_p()
-        check(lib().owl_elastic2d_create(C.byref(self.cfg), *[_ptr(a) for a in self._m], len(shots),
-                                         C.cast(arr, C.c_void_p), C.byref(h)))
-        self._h = h
-        self.nt = lib().owl_elastic2d_nt(h)
+        self._shots, self._keep, self.nr, self._windows = shot_array_2d(shots, nx, nz)
+
+    def _model(self, a):
+        a = np.asfortranarray(a, dtype=np.float32)
+        if a.shape != self.shape:
+            raise ValueError(f"models must have shape {self.shape} (nz, nx)")
+        return a
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().owl_elastic2d_destroy(self._h)
+            self._fn("destroy")(self._h)
             self._h = None
 
     def __del__(self):
@@ -74,7 +80,7 @@ class Elastic2DBatch:
             pass
 
     def set_launch(self, planes, rows=4):
-        check(lib().owl_elastic2d_set_launch(self._h, int(planes), int(rows)))
+        check(self._fn("set_launch")(self._h, int(planes), int(rows)))
 
     def _split(self, flat):
         out, o = [], 0
@@ -93,11 +99,53 @@ class Elastic2DBatch:
             parts.append(a.ravel())
         return np.ascontiguousarray(np.concatenate(parts))
 
+    def _window_arrays(self, flats):
+        out, o = [], 0
+        for (nz, nx) in self._windows:
+            n = nz * nx
+            out.append(tuple(a[o:o + n].reshape(nx, nz).T.copy() for a in flats))
+            o += n
+        return out
+
     def forward(self, want_traces=True, save_boundary=False):
         """Per shot (vx, vz) traces, each (nr, nt)."""
         flat = np.zeros(2 * sum(self.nr) * self.nt, np.float32) if want_traces else None
-        check(lib().owl_elastic2d_forward(self._h, None if flat is None else _ptr(flat), int(bool(save_boundary))))
+        check(self._fn("forward")(self._h, None if flat is None else _ptr(flat), int(bool(save_boundary))))
         return None if flat is None else self._split(flat)
+
+    def set_profiling(self, on=True):
+        check(self._fn("set_profiling")(self._h, int(bool(on))))
+
+    def kernel_stats(self):
+        out = {}
+        for k, name in enumerate(self.KERNELS):
+            ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
+            check(self._fn("kernel_stats")(self._h, k, C.byref(ms), C.byref(n), C.byref(by)))
+            out[name] = dict(total_ms=ms.value, launches=n.value, bytes_total=by.value)
+        return out
+
+    def last_stats(self):
+        ms, up = C.c_double(), C.c_double()
+        check(self._fn("last_stats")(self._h, C.byref(ms), C.byref(up)))
+        return ms.value, up.value
+
+
+class Elastic2DBatch(TwoComponentBatch):
+    """vp, vs, rho: (nz, nx).  shots: list of (Geometry, window) where window
+    is None (full model) or an object/dict with nxbeg, nxend, nzbeg, nzend
+    (1-based)."""
+
+    PREFIX = "owl_elastic2d"
+
+    def __init__(self, nx, nz, dx, dz, vp, vs, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
+                 f0factor=1.0, cc_step_interval=1):
+        self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval)
+        self._m = [self._model(a) for a in (vp, vs, rho)]
+        h = C.c_void_p()
+        check(lib().owl_elastic2d_create(C.byref(self.cfg), *[_ptr(a) for a in self._m], len(shots),
+                                         C.cast(self._shots, C.c_void_p), C.byref(h)))
+        self._h = h
+        self.nt = lib().owl_elastic2d_nt(h)
 
     def adjoint(self, adj):
         """OWL's reverse pass after forward(save_boundary=True).  adj: per
@@ -107,12 +155,7 @@ class Elastic2DBatch:
         ne = sum(a * b for a, b in self._windows)
         gv, gs, gr = (np.zeros(ne, np.float32) for _ in range(3))
         check(lib().owl_elastic2d_adjoint(self._h, _ptr(flat), _ptr(gv), _ptr(gs), _ptr(gr)))
-        out, o = [], 0
-        for (nz, nx) in self._windows:
-            n = nz * nx
-            out.append(tuple(a[o:o + n].reshape(nx, nz).T.copy() for a in (gv, gs, gr)))
-            o += n
-        return out
+        return self._window_arrays((gv, gs, gr))
 
     def gradient(self, obs=None):
         """Forward, L2 adjoint source per component, reverse pass and the sum
@@ -128,19 +171,3 @@ class Elastic2DBatch:
                                            _ptr(gs), _ptr(gr), _ptr(tr)))
         g = tuple(a.reshape(nx, nz).T.copy() for a in (gv, gs, gr))
         return (mis,) + g + (self._split(tr),)
-
-    def set_profiling(self, on=True):
-        check(lib().owl_elastic2d_set_profiling(self._h, int(bool(on))))
-
-    def kernel_stats(self):
-        out = {}
-        for k, name in enumerate(self.KERNELS):
-            ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
-            check(lib().owl_elastic2d_kernel_stats(self._h, k, C.byref(ms), C.byref(n), C.byref(by)))
-            out[name] = dict(total_ms=ms.value, launches=n.value, bytes_total=by.value)
-        return out
-
-    def last_stats(self):
-        ms, up = C.c_double(), C.c_double()
-        check(lib().owl_elastic2d_last_stats(self._h, C.byref(ms), C.byref(up)))
-        return ms.value, up.value

@@ -1,0 +1,89 @@
+"""2D general-anisotropic elastic solver on the Lebedev fully staggered grid
+over the C ABI (owl_fsg2d_*, include/owl_amd.h): a batch of shots with
+per-shot windows sharing one (nz, nx) model given as c11, c13, c15, c33, c35,
+c55 and rho; forward modelling as OWL's elastic_tti_2d compute_forward
+(src/solver/elastic_tti_2d/submodule_forward_modeling.f90:30-300) with force
+and explosion sources, two-component receivers and CFS-MPML damping ratios
+from the Christoffel scan (module_cfspml.f90:80-280), and OWL's gradient
+(src/solver/elastic_tti_2d/submodule_fwi_gradient.f90:40-727)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+from .elastic2d import TwoComponentBatch, _ptr
+
+CIJ = ("c11", "c13", "c15", "c33", "c35", "c55")
+GRADS = tuple("grad_" + k for k in CIJ) + ("grad_rho",)
+
+
+def damp_ratio(c, rho, direction):
+    """M-PML damping ratio of one medium (host computation, no GPU): c =
+    (c11, c13, c15, c33, c35, c55); direction "x" or "z"."""
+    cc = np.ascontiguousarray(c, dtype=np.float32)
+    if cc.shape != (6,):
+        raise ValueError("c must hold c11, c13, c15, c33, c35, c55")
+    out = C.c_float()
+    check(lib().owl_fsg2d_damp_ratio(_ptr(cc), float(rho), 1 if direction == "x" else 0, C.byref(out)))
+    return out.value
+
+
+class FSG2DBatch(TwoComponentBatch):
+    """cij: dict of c11, c13, c15, c33, c35, c55, each (nz, nx); rho (nz, nx).
+    aniso: True takes the M-PML damping ratios from the Christoffel scan,
+    False sets them to 0 (OWL's anisotropy_type 'iso').  shots: list of
+    (Geometry, window) as for Elastic2DBatch."""
+
+    PREFIX = "owl_fsg2d"
+
+    def __init__(self, nx, nz, dx, dz, cij, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
+                 f0factor=1.0, cc_step_interval=1, aniso=True):
+        self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval)
+        self._m = [self._model(cij[k]) for k in CIJ]
+        self._rho = self._model(rho)
+        ptrs = (C.c_void_p * 6)(*[a.ctypes.data for a in self._m])
+        h = C.c_void_p()
+        check(lib().owl_fsg2d_create(C.byref(self.cfg), ptrs, _ptr(self._rho), int(bool(aniso)), len(shots),
+                                     C.cast(self._shots, C.c_void_p), C.byref(h)))
+        self._h = h
+        self.nt = lib().owl_fsg2d_nt(h)
+
+    def damping_ratios(self, shot=0):
+        """(left, right, top, bottom) M-PML ratio profiles of one shot."""
+        nz, nx = self._windows[shot]
+        pml = self.cfg.pml
+        out = [np.zeros(nz + 2 * pml + 2, np.float32) for _ in range(2)] + \
+              [np.zeros(nx + 2 * pml + 2, np.float32) for _ in range(2)]
+        check(lib().owl_fsg2d_damping_ratios(self._h, int(shot), *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def set_restore_all(self, on=True):
+        """False (default): the final state is read back in OWL's order (the
+        saved vz of the (i, j-1/2) grid lands in the (i-1/2, j) grid's vz);
+        True: every field is restored."""
+        check(lib().owl_fsg2d_set_restore_all(self._h, int(bool(on))))
+
+    def adjoint(self, adj):
+        """OWL's reverse pass after forward(save_boundary=True).  adj: per
+        shot (ax, az), each (nr, nt).  Returns per shot a dict of the seven
+        gradients (grad_c11 .. grad_c55, grad_rho) on the shot's window,
+        each (nz_s, nx_s)."""
+        flat = self._flat(adj)
+        ne = sum(a * b for a, b in self._windows)
+        gs = [np.zeros(ne, np.float32) for _ in GRADS]
+        ptrs = (C.c_void_p * 7)(*[a.ctypes.data for a in gs])
+        check(lib().owl_fsg2d_adjoint(self._h, _ptr(flat), ptrs))
+        return [dict(zip(GRADS, w)) for w in self._window_arrays(gs)]
+
+    def gradient(self, obs=None):
+        """Forward, L2 adjoint source per component, reverse pass and the sum
+        over shots.  Returns (misfit per shot, dict of the seven gradients
+        as (nz, nx) float64, synthetic traces per shot)."""
+        flat = None if obs is None else self._flat(obs)
+        nz, nx = self.shape
+        mis = np.zeros(len(self.nr))
+        gs = [np.zeros(nz * nx) for _ in GRADS]
+        ptrs = (C.c_void_p * 7)(*[a.ctypes.data for a in gs])
+        tr = np.zeros(2 * sum(self.nr) * self.nt, np.float32)
+        check(lib().owl_fsg2d_gradient(self._h, None if flat is None else _ptr(flat), _ptr(mis), ptrs, _ptr(tr)))
+        return mis, {k: a.reshape(nx, nz).T.copy() for k, a in zip(GRADS, gs)}, self._split(tr)
