@@ -127,7 +127,7 @@ def cpu_baseline(vp, rho, vp0, rho0, shot, nt):
                 seconds=round(wall, 3), s_per_shot=round(wall, 3))
 
 
-def bench_3d(sync):
+def bench_3d(sync, steps=1500):
     """3D acoustic batched forward: 256^3, 10 m, R = 4, 1500 steps, 20-cell
     PML, 5-layer model, 16 shots with uneven 128..256 windows, 100 x 100
     receivers each.  dt = 0.9 ms: 1 ms exceeds the scheme's stable step
@@ -137,7 +137,7 @@ def bench_3d(sync):
     vp, rho = acoustic3d_layered_model()
     shots = acoustic3d_shots()
     dt = 0.0009
-    b = Acoustic3DBatch(256, 256, 256, 10.0, 10.0, 10.0, vp, rho, shots, dt, 1499 * dt, pml=20, fdhalf=4)
+    b = Acoustic3DBatch(256, 256, 256, 10.0, 10.0, 10.0, vp, rho, shots, dt, (steps - 1) * dt, pml=20, fdhalf=4)
     kc, var = b.autotune(3)
     b.set_profiling(True)
     sync()
@@ -149,11 +149,41 @@ def bench_3d(sync):
     ms, updates = b.last_stats()
     b.close()
     _, roof = roofline_of(stats)
-    return {"workload": "3D acoustic batched forward, 256^3, 16 shots (windows 128-256), 1500 steps, "
+    return {"workload": f"3D acoustic batched forward, 256^3, 16 shots (windows 128-256), {steps} steps, "
                         "dt 0.9 ms, R 4, pml 20",
             "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
             "launch_shape": {"planes": kc, "variant": var}, "roofline": roof,
             "kernels": kernel_table(stats, 1)}
+
+
+def bench_fsg(sync, steps):
+    """2D tilted-TI elastic on the Lebedev fully staggered grid, batched
+    forward: 801 x 401, 5 m, R = 4, dt 0.4 ms, 20-cell CFS-MPML with the
+    Christoffel-scan damping ratios, 64 shots with uneven 200..801-column
+    windows, vertical forces, two-component receivers every 10 m; no free
+    surface."""
+    from owl_amd import FSG2DBatch
+    from owl_amd.synthetic import fsg2d_shots, fsg2d_tti_model
+    cij, rho = fsg2d_tti_model()
+    shots = fsg2d_shots()
+    dt = 0.0004
+    t0 = time.perf_counter()
+    b = FSG2DBatch(801, 401, 5.0, 5.0, cij, rho, shots, dt, (steps - 1) * dt, pml=20, fdhalf=4)
+    plan_s = time.perf_counter() - t0
+    b.set_profiling(True)
+    sync()
+    t0 = time.perf_counter()
+    b.forward(want_traces=False)
+    sync()
+    wall = time.perf_counter() - t0
+    stats = b.kernel_stats()
+    ms, updates = b.last_stats()
+    b.close()
+    _, roof = roofline_of(stats)
+    return {"workload": f"2D tilted-TI FSG batched forward, 801x401, 5 m, dt 0.4 ms, R 4, pml 20, 64 shots "
+                        f"(windows 200-801), {steps} steps, no free surface",
+            "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
+            "plan_seconds": round(plan_s, 3), "roofline": roof, "kernels": kernel_table(stats, 1)}
 
 
 def main():
@@ -172,6 +202,11 @@ def main():
                     help="with --headline-only: that many time steps instead of 2000 (short counter runs "
                          "with the same launches)")
     ap.add_argument("--only-3d", action="store_true", help="only the 3D batched forward (for profiler runs)")
+    ap.add_argument("--steps-3d", type=int, default=1500,
+                    help="with --only-3d: time steps of the 3D forward (short counter runs)")
+    ap.add_argument("--no-fsg", action="store_true", help="skip the 2D FSG forward")
+    ap.add_argument("--only-fsg", action="store_true", help="only the 2D FSG forward (for profiler runs)")
+    ap.add_argument("--fsg-steps", type=int, default=5000, help="time steps of the 2D FSG forward")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device five times (profiler counter calibration)")
     args = ap.parse_args()
@@ -199,7 +234,10 @@ def main():
     sync = torch.cuda.synchronize
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
     if args.only_3d:
-        print(json.dumps({"acoustic3d_forward": bench_3d(sync)}))
+        print(json.dumps({"acoustic3d_forward": bench_3d(sync, args.steps_3d)}))
+        return
+    if args.only_fsg:
+        print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps)}))
         return
     log("building workload")
     vp, rho, vp0, rho0, shots = workload()
@@ -267,6 +305,11 @@ def main():
         log("3D batched forward")
         three_d = bench_3d(sync)
 
+    fsg = None
+    if not args.no_fsg:
+        log("2D FSG batched forward")
+        fsg = bench_fsg(sync, args.fsg_steps)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
@@ -305,6 +348,7 @@ def main():
         "timing": "value and ms_per_step: graph-replayed gradients; roofline and kernels: HIP events "
                   "of one eager gradient run right after the timed region",
         "acoustic3d_forward": three_d,
+        "fsg2d_forward": fsg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

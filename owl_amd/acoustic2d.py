@@ -145,9 +145,25 @@ class Acoustic2DBatch:
     def set_step_kernel(self, mode="auto", zrows=16):
         """Forward-step kernel of the exact-adjoint paths: "sweeps" (p then
         v), "fused" (one z-marching kernel per step) or "auto" (fused for 4
-        shots or more).  Results are bit-identical."""
+        shots or more).  Results are bit-identical.  Turns the z-row
+        autotuner off."""
         m = {"sweeps": 0, "fused": 1, "auto": 2}[mode]
         check(lib().owl_acoustic2d_set_step_kernel(self._h, m, int(zrows)))
+
+    def set_adjoint_rows(self, zrows):
+        """z rows per tile of the fused adjoint step (turns the autotuner off)."""
+        check(lib().owl_acoustic2d_set_adjoint_rows(self._h, int(zrows)))
+
+    def set_autotune(self, on=True):
+        """Time z-row candidates of both fused kernels at the next exact
+        gradient and keep the fastest (results do not change)."""
+        check(lib().owl_acoustic2d_set_autotune(self._h, int(bool(on))))
+
+    def fused_rows(self):
+        """(forward, adjoint) z rows per fused tile in use."""
+        z, a = C.c_int(), C.c_int()
+        check(lib().owl_acoustic2d_fused_rows(self._h, C.byref(z), C.byref(a)))
+        return z.value, a.value
 
     def kernel_stats(self):
         out = {}

@@ -42,6 +42,7 @@ struct Shot {
     int pitch;            // floats per padded row
     int nrow;             // padded rows
     long long off;        // element offset of node (lo, lo) in every field buffer
+    long long mdel;       // model element = field element + mdel (shots with one window share the model)
     int tile0, ntx;       // first update tile, tiles per row
     int prof;             // offset of the shot's PML profiles
     int nr;               // receivers
@@ -54,7 +55,11 @@ struct Shot {
     int edge0;            // offset of the shot's saved edge values (2*nz + 2*nx)
     int nxbeg, nzbeg;     // window origin in the full model (1-based)
     int ftile0, fntx;     // fused step: first column-strip tile, strips per row
+    int atile0;           // fused adjoint: first column-strip tile (its own z-chunk rows)
 };
+
+// Element of the model arrays (bk, rho) at field element q of shot S.
+__device__ __forceinline__ size_t mq(const Shot& S, size_t q) { return size_t((long long)q + S.mdel); }
 
 // One set of wavefields for a sweep.  Inputs and outputs may alias (in
 // place) or not (ping-pong); the CFS-PML memory variables are m[0] dx vx,
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_p(const Shot* __restrict__
             F.m_out[1][q] = mem;
             pdz = pdz + mem;
         }
-        F.p_out[q] = F.p_in[q] - dt * bk[q] * (pdx + pdz);
+        F.p_out[q] = F.p_in[q] - dt * bk[mq(S, q)] * (pdx + pdz);
     }
 }
 
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
                 F.m_out[2][q] = mem;
                 d = d + mem;
             }
-            const float reff = 0.5f * (rho[q] + rho[q - 1]);
+            const float reff = 0.5f * (rho[mq(S, q)] + rho[mq(S, q) - 1]);
             F.vx_out[q] = F.vx_in[q] - dt / reff * d;
         }
         if (!free_surface || j >= 2) {
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
                 F.m_out[3][q] = mem;
                 d = d + mem;
             }
-            const float reff = 0.5f * (rho[q] + rho[q - pitch]);
+            const float reff = 0.5f * (rho[mq(S, q)] + rho[mq(S, q) - pitch]);
             F.vz_out[q] = F.vz_in[q] - dt / reff * d;
         }
     }
@@ -239,11 +244,23 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
 // tile may already have written); arithmetic as k_update_p / k_update_v,
 // so the results are bit-identical to the two-sweep form.
 // ---------------------------------------------------------------------------
+// Load from base + a 32-bit byte offset: the batch's buffers stay below
+// 2 GiB (checked on the host), so the compiler can address them as a
+// scalar base plus one 32-bit VGPR.
+template <class T>
+__device__ __forceinline__ T ldo(const T* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void sto(T* base, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 template <int R>
 __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ shots,
                                                     const int* __restrict__ ftile_shot, int nftile,
                                                     FieldSet F, const float* __restrict__ bk,
-                                                    const float* __restrict__ rho,
+                                                    const float* __restrict__ qhx, const float* __restrict__ qhz,
                                                     const float* __restrict__ prof, Coef c, float dt,
                                                     float idx, float idz, int pml, int zc) {
     constexpr int W = 64 - (2 * R - 1);
@@ -279,41 +296,42 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
         axh_ = axh[col + pml - 1];
         bxh_ = bxh[col + pml - 1];
     }
-    const size_t pitch = size_t(S.pitch);
-    const size_t base = size_t(S.off) + size_t(col - lo);  // (col, lo)
+    // byte offsets: field element of (col, lo), one row, model minus field
+    const unsigned base = unsigned(S.off + (col - lo)) * 4u;
+    const unsigned rowb = unsigned(S.pitch) * 4u;
+    const unsigned mdb = unsigned(S.mdel) * 4u;
     // Inputs of one loop iteration: the p row r and the v row j = r - R + 1.
     struct Row {
         float vx[2 * R];  // vx_in at row r, columns col - R + 1 .. col + R
         float vzf;        // vz_in at row r + R (the front of the vz queue)
         float p, bk, m0, m1;
-        float rq, rl, ru, vxj, m2, m3;  // rho (j), rho (j, col - 1), rho (j - 1), vx_in (j)
+        float qx, qz, vxj, m2, m3;  // dt / rho at the velocity nodes of row j, vx_in (j)
     };
     const int rs = z0 - R, re = z1 + R - 1;
     auto load = [&](int r, Row& w) {
         const int vr = r + R;
-        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? F.vz_in[base + size_t(vr - lo) * pitch] : 0.0f;
+        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? ldo(F.vz_in, base + unsigned(vr - lo) * rowb) : 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const size_t q = base + size_t(r - lo) * pitch;
+            const unsigned q = base + unsigned(r - lo) * rowb;
 #pragma unroll
-            for (int k = 0; k < 2 * R; ++k) w.vx[k] = F.vx_in[q - (R - 1) + k];
-            w.p = F.p_in[q];
-            w.bk = bk[q];
-            w.m0 = xs_i ? F.m_in[0][q] : 0.0f;
-            w.m1 = (r <= 0 || r >= nz + 1) ? F.m_in[1][q] : 0.0f;
+            for (int k = 0; k < 2 * R; ++k) w.vx[k] = ldo(F.vx_in, q + unsigned(k - (R - 1)) * 4u);
+            w.p = ldo(F.p_in, q);
+            w.bk = ldo(bk, q + mdb);
+            w.m0 = xs_i ? ldo(F.m_in[0], q) : 0.0f;
+            w.m1 = (r <= 0 || r >= nz + 1) ? ldo(F.m_in[1], q) : 0.0f;
         }
         const int j = r - R + 1;
         if (own && j >= z0) {
-            const size_t q = base + size_t(j - lo) * pitch;
-            w.rq = rho[q];
-            w.rl = rho[q - 1];
-            w.ru = rho[q - pitch];
-            w.vxj = F.vx_in[q];
-            w.m2 = xs_h ? F.m_in[2][q] : 0.0f;
-            w.m3 = (j <= 1 || j >= nz + 1) ? F.m_in[3][q] : 0.0f;
+            const unsigned q = base + unsigned(j - lo) * rowb;
+            w.qx = ldo(qhx, q + mdb);
+            w.qz = ldo(qhz, q + mdb);
+            w.vxj = ldo(F.vx_in, q);
+            w.m2 = xs_h ? ldo(F.m_in[2], q) : 0.0f;
+            w.m3 = (j <= 1 || j >= nz + 1) ? ldo(F.m_in[3], q) : 0.0f;
         }
     };
     auto vz_row = [&](int row) -> float {
-        return (mcol && row >= lo && row <= nz + pml + R) ? F.vz_in[base + size_t(row - lo) * pitch] : 0.0f;
+        return (mcol && row >= lo && row <= nz + pml + R) ? ldo(F.vz_in, base + unsigned(row - lo) * rowb) : 0.0f;
     };
     // V[k] = vz_in at row r - R + 1 + k;  P[k] = new p at row r - 2R + 1 + k
     float V[2 * R], P[2 * R];
@@ -321,77 +339,82 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
     for (int k = 0; k < 2 * R - 1; ++k) V[k] = vz_row(rs - R + 1 + k);
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) P[k] = 0.0f;
-    Row cur, nxt;
-    load(rs, cur);
-    for (int r = rs; r <= re; ++r) {
-        if (r < re) load(r + 1, nxt);  // in flight while row r is computed
-        V[2 * R - 1] = cur.vzf;
-        // ---- p at (col, r)
+    // one row: p at (col, r), then v at row r - R + 1, from the inputs in w
+    auto row = [&](int r, const Row& w) {
+        V[2 * R - 1] = w.vzf;
         float pn = 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const size_t q = base + size_t(r - lo) * pitch;
-            float sx = c.c[0] * (cur.vx[R] - cur.vx[R - 1]);
+            const unsigned q = base + unsigned(r - lo) * rowb;
+            float sx = c.c[0] * (w.vx[R] - w.vx[R - 1]);
             float sz = c.c[0] * (V[R] - V[R - 1]);
 #pragma unroll
             for (int m = 2; m <= R; ++m) {
-                sx = sx + c.c[m - 1] * (cur.vx[R - 1 + m] - cur.vx[R - m]);
+                sx = sx + c.c[m - 1] * (w.vx[R - 1 + m] - w.vx[R - m]);
                 sz = sz + c.c[m - 1] * (V[R - 1 + m] - V[R - m]);
             }
             float pdx = idx * sx;
             float pdz = idz * sz;
             const bool wr = own && r >= z0 && r <= z1;
             if (xs_i) {
-                const float mem = ax * pdx + bx * cur.m0;
-                if (wr) F.m_out[0][q] = mem;
+                const float mem = ax * pdx + bx * w.m0;
+                if (wr) sto(F.m_out[0], q, mem);
                 pdx = pdx + mem;
             }
             if (r <= 0 || r >= nz + 1) {
-                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * cur.m1;
-                if (wr) F.m_out[1][q] = mem;
+                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * w.m1;
+                if (wr) sto(F.m_out[1], q, mem);
                 pdz = pdz + mem;
             }
-            pn = cur.p - dt * cur.bk * (pdx + pdz);
-            if (wr) F.p_out[q] = pn;
+            pn = w.p - dt * w.bk * (pdx + pdz);
+            if (wr) sto(F.p_out, q, pn);
         }
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) P[k] = P[k + 1];
         P[2 * R - 1] = pn;
-        // ---- v at row j = r - R + 1
         const int j = r - R + 1;
         if (j >= z0) {
             float px[2 * R];  // new p of row j at columns col - R .. col + R - 1
 #pragma unroll
             for (int k = 0; k < 2 * R; ++k) px[k] = __shfl(P[R], l - R + k, 64);
             if (own) {
-                const size_t q = base + size_t(j - lo) * pitch;
+                const unsigned q = base + unsigned(j - lo) * rowb;
                 float s = c.c[0] * (px[R] - px[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (px[R + m - 1] - px[R - m]);
                 float d = idx * s;
                 if (xs_h) {
-                    const float mem = axh_ * d + bxh_ * cur.m2;
-                    F.m_out[2][q] = mem;
+                    const float mem = axh_ * d + bxh_ * w.m2;
+                    sto(F.m_out[2], q, mem);
                     d = d + mem;
                 }
-                float reff = 0.5f * (cur.rq + cur.rl);
-                F.vx_out[q] = cur.vxj - dt / reff * d;
+                sto(F.vx_out, q, w.vxj - w.qx * d);
                 s = c.c[0] * (P[R] - P[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (P[R + m - 1] - P[R - m]);
                 d = idz * s;
                 if (j <= 1 || j >= nz + 1) {
-                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * cur.m3;
-                    F.m_out[3][q] = mem;
+                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * w.m3;
+                    sto(F.m_out[3], q, mem);
                     d = d + mem;
                 }
-                reff = 0.5f * (cur.rq + cur.ru);
-                F.vz_out[q] = V[0] - dt / reff * d;
+                sto(F.vz_out, q, V[0] - w.qz * d);
             }
         }
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) V[k] = V[k + 1];
-        cur = nxt;
+    };
+    // two row buffers in turn: the next row's inputs are in flight while a
+    // row is computed, with no copy between them
+    Row ra, rb;
+    load(rs, ra);
+    int r = rs;
+    for (; r + 1 <= re; r += 2) {
+        load(r + 1, rb);
+        row(r, ra);
+        if (r + 2 <= re) load(r + 2, ra);
+        row(r + 1, rb);
     }
+    if (r <= re) row(r, ra);
 }
 
 // One tap of a point source or receiver: field cell and two weights.
@@ -717,7 +740,7 @@ __global__ __launch_bounds__(kBX * kBY) void k_adj_p(const Shot* __restrict__ sh
         }
         const float ph = A.ph[q] - (idx * sx + idz * sz);
         A.ph[q] = ph;
-        const float K = bk[q];
+        const float K = bk[mq(S, q)];
         A.gK[q] += double((p_cur[q] - p_prev[q]) / K * ph);
         const float qv = -dt * K * ph;
         float dxv = qv, dzv = qv;
@@ -766,8 +789,8 @@ __global__ __launch_bounds__(kBX * kBY) void k_adj_v(const Shot* __restrict__ sh
         const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
         if (j > nz + pml) break;
         const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
-        const float rx = 0.5f * (rho[q] + rho[q - 1]);
-        const float rz = 0.5f * (rho[q] + rho[q - pitch]);
+        const float rx = 0.5f * (rho[mq(S, q)] + rho[mq(S, q) - 1]);
+        const float rz = 0.5f * (rho[mq(S, q)] + rho[mq(S, q) - pitch]);
         float vxh = A.vxh[q], vzh = A.vzh[q];
         A.gX[q] += double((vx_prev[q] - vx_cur[q]) / rx * vxh);
         A.gZ[q] += double((vz_prev[q] - vz_cur[q]) / rz * vzh);
@@ -826,7 +849,7 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
     const int tile = int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y);
     if (tile >= nftile) return;
     const Shot S = shots[ftile_shot[tile]];
-    const int lt = tile - S.ftile0;
+    const int lt = tile - S.atile0;
     const int tx = lt % S.fntx, tz = lt / S.fntx;
     const int l = int(threadIdx.x);
     const int nx = S.nx, nz = S.nz;
@@ -874,16 +897,16 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
             w.ph = io.ph[q];
             w.psx = xs_i ? io.psx[q] : 0.0f;
             w.psz = (r <= 0 || r >= nz + 1) ? io.psz[q] : 0.0f;
-            w.k = bk[q];
+            w.k = bk[mq(S, q)];
             w.pc = p_cur[q];
             w.pp = p_prev[q];
         }
         const int j = r - R + 1;
         if (own && j >= z0) {
             const size_t q = base + size_t(j - lo) * pitch;
-            w.rq = rho[q];
-            w.rl = rho[q - 1];
-            w.ru = rho[q - pitch];
+            w.rq = rho[mq(S, q)];
+            w.rl = rho[mq(S, q) - 1];
+            w.ru = rho[mq(S, q) - pitch];
             w.vxh = A.vxh[q];
             w.vzh = A.vzh[q];
             w.phx = xs_h ? A.phx[q] : 0.0f;
@@ -1012,7 +1035,7 @@ __global__ void k_src_correction(const Shot* __restrict__ shots, const int* __re
             if (t < tp.tbeg || t > tp.tend) continue;
             src = src + amp_tab[size_t(tp.row) * nt + (t - 1)] * tp.w1 * tp.w2;
         }
-        gK[cell] -= double(src / bk[cell] * ph[cell]);
+        gK[cell] -= double(src / bk[mq(S, size_t(cell))] * ph[cell]);
     }
 }
 
@@ -1148,6 +1171,7 @@ struct owl_acoustic2d {
     std::vector<Shot> sd;          // device descriptors (host copy)
     std::vector<int> tile_shot;
     size_t field_elems = 0;        // elements of one field buffer (all shots)
+    size_t model_elems = 0;        // elements of bk / rho (one padded grid per distinct window)
     size_t grad_elems = 0;         // sum of nx*nz
     size_t trace_elems = 0;        // sum of nt*nr
     size_t strip_elems = 0;        // sum of nt*strip_len
@@ -1156,6 +1180,7 @@ struct owl_acoustic2d {
     int max_ninj = 0, max_nadj = 0;
     int src_t0 = 1, src_t1 = 0;    // steps during which any source tap is active
     double pts_full = 0;           // sum over shots of (nx+2pml)(nz+2pml)
+    double pts_model = 0;          // the same over distinct windows (model reads)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0, last_updates = 0;
@@ -1176,6 +1201,7 @@ struct owl_acoustic2d {
     DevBuf<Shot> d_shots;
     DevBuf<int> d_tile_shot;
     DevBuf<float> d_bk, d_rho, d_den, d_prof;
+    DevBuf<float> d_qhx, d_qhz;       // dt / rho at the half nodes (model layout), for the fused kernels
     DevBuf<float> fA[3], fB[3], fM[4];  // forward field (ping-pong) and PML memory
     DevBuf<float> aF[3], aM[4];         // adjoint field and PML memory
     DevBuf<float> d_amp;                // source amplitude rows [row][nt]
@@ -1193,6 +1219,12 @@ struct owl_acoustic2d {
     int step_kernel = 2, fzc = 16, nftile = 0;
     std::vector<int> ftile_shot;
     DevBuf<int> d_ftile_shot;
+    // fused adjoint: z rows per tile and its tile list; the autotuner picks
+    // fzc and azc on the first exact gradient unless the caller set them
+    int azc = 8, natile = 0;
+    std::vector<int> atile_shot;
+    DevBuf<int> d_atile_shot;
+    bool autotune = true, tuned = false;
     DevBuf<float> adj[11];            // ph vxh vzh psx psz phx phz ex ez dx dz
     DevBuf<float> adj2[5];            // fused adjoint: second ph psx psz ex ez
     int adj_cur = 0;                  // which of the two holds the current ph psx psz ex ez
@@ -1245,7 +1277,7 @@ inline float full_at(const owl_acoustic2d* b, const std::vector<float>& m, const
 // acoustic_iso_2d/module_vars.f90:108-115), and rho*vp**3 for the imaging
 // condition (submodule_fwi_gradient.f90:218).
 void upload_model(owl_acoustic2d* b) {
-    std::vector<float> bk(b->field_elems, 0.f), rh(b->field_elems, 0.f), den(b->grad_elems, 0.f);
+    std::vector<float> bk(b->model_elems, 0.f), rh(b->model_elems, 0.f), den(b->grad_elems, 0.f);
     std::vector<float> vpw(b->grad_elems, 0.f), rhow(b->grad_elems, 0.f);
     const int lo = 1 - b->pml - b->R;
     for (size_t s = 0; s < b->hs.size(); ++s) {
@@ -1256,7 +1288,7 @@ void upload_model(owl_acoustic2d* b) {
             for (int ii = 0; ii < NXp; ++ii) {
                 const int i = lo + ii, j = lo + jj;
                 const float v = full_at(b, b->vp_full, H, i, j), r = full_at(b, b->rho_full, H, i, j);
-                const size_t q = cell_of(b, S, i, j);
+                const size_t q = size_t((long long)cell_of(b, S, i, j) + S.mdel);
                 rh[q] = r;
                 bk[q] = r * (v * v);
             }
@@ -1268,8 +1300,23 @@ void upload_model(owl_acoustic2d* b) {
                 rhow[size_t(S.g0) + size_t(i - 1) + size_t(j - 1) * H.nx] = r;
             }
     }
+    // dt over the 2-point densities at the velocity nodes, as the v-sweep forms
+    // them (0.5 (rho(i) + rho(i-1)), then the division), for the fused kernels
+    std::vector<float> qx(b->model_elems, 0.f), qz(b->model_elems, 0.f);
+    for (size_t s = 0; s < b->hs.size(); ++s) {
+        const Shot& S = b->sd[s];
+        const size_t m0 = size_t(S.off + S.mdel), pitch = size_t(S.pitch);
+        for (size_t jj = 1; jj < size_t(S.nrow); ++jj)
+            for (size_t ii = 1; ii < pitch; ++ii) {
+                const size_t q = m0 + ii + jj * pitch;
+                qx[q] = b->dt / (0.5f * (rh[q] + rh[q - 1]));
+                qz[q] = b->dt / (0.5f * (rh[q] + rh[q - pitch]));
+            }
+    }
     b->d_bk.upload(bk, b->stream);
     b->d_rho.upload(rh, b->stream);
+    b->d_qhx.upload(qx, b->stream);
+    b->d_qhz.upload(qz, b->stream);
     b->d_den.upload(den, b->stream);
     b->d_vpw.upload(vpw, b->stream);
     b->d_rhow.upload(rhow, b->stream);
@@ -1290,6 +1337,24 @@ void fused_tiling(owl_acoustic2d* b) {
     }
     b->nftile = int(b->ftile_shot.size());
     if (b->ftile_shot.empty()) b->ftile_shot.push_back(0);
+    b->atile_shot.clear();
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        Shot& S = b->sd[s];
+        S.atile0 = int(b->atile_shot.size());
+        const int nzc = (S.nz + 2 * b->pml + b->azc - 1) / b->azc;
+        b->atile_shot.insert(b->atile_shot.end(), size_t(S.fntx) * nzc, int(s));
+    }
+    b->natile = int(b->atile_shot.size());
+    if (b->atile_shot.empty()) b->atile_shot.push_back(0);
+}
+
+// Re-tile the fused kernels and upload the descriptors (synchronous).
+void upload_fused_tiling(owl_acoustic2d* b) {
+    fused_tiling(b);
+    b->d_shots.upload(b->sd, b->stream);
+    b->d_ftile_shot.upload(b->ftile_shot, b->stream);
+    b->d_atile_shot.upload(b->atile_shot, b->stream);
+    OWL_HIP(hipStreamSynchronize(b->stream));
 }
 
 void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, const float* rho,
@@ -1319,13 +1384,14 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     b->hs.assign(nshot, ShotHost{});
     b->sd.assign(nshot, Shot{});
     b->tile_shot.clear();
-    long long off = 0, strip = 0, trace = 0, g0 = 0;
+    long long off = 0, moff = 0, strip = 0, trace = 0, g0 = 0;
     int edge = 0, prof = 0;
     std::vector<float> profs, amp;
     std::vector<int> inj_beg{0}, adj_beg{0}, rec_beg{0};
     std::vector<Tap> inj_taps, adj_taps, rec_taps;
     std::vector<float> rec_w;
     b->pts_full = 0;
+    b->pts_model = 0;
     b->nrec_total = 0;   // every per-plan table is rebuilt from zero
     b->max_ninj = 0;
     b->max_nadj = 0;
@@ -1379,6 +1445,23 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         S.off = off;
         off += (long long)S.pitch * NZp;
         off = (off + 63) / 64 * 64;
+        // shots on the same window read one copy of the padded model
+        {
+            long long mo = -1;
+            for (int u = 0; u < s; ++u)
+                if (b->hs[u].nxbeg == H.nxbeg && b->hs[u].nx == H.nx && b->hs[u].nzbeg == H.nzbeg &&
+                    b->hs[u].nz == H.nz) {
+                    mo = b->sd[u].off + b->sd[u].mdel;
+                    break;
+                }
+            if (mo < 0) {
+                b->pts_model += double(H.nx + 2 * pml) * double(H.nz + 2 * pml);
+                mo = moff;
+                moff += (long long)S.pitch * NZp;
+                moff = (moff + 63) / 64 * 64;
+            }
+            S.mdel = mo - S.off;
+        }
         S.tile0 = int(b->tile_shot.size());
         S.ntx = (H.nx + 2 * pml + kBX - 1) / kBX;
         const int ntz = (H.nz + 2 * pml + kRows - 1) / kRows;
@@ -1510,6 +1593,7 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         b->max_nadj = std::max(b->max_nadj, S.nadj);
     }
     b->field_elems = size_t(off);
+    b->model_elems = size_t(moff);
     b->grad_elems = size_t(g0);
     b->trace_elems = size_t(trace);
     b->strip_elems = size_t(strip);
@@ -1529,6 +1613,7 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
             size_t(S.g0) + size_t(S.nx) * size_t(S.nz) <= b->grad_elems &&
             size_t(S.edge0) + size_t(2 * S.nx + 2 * S.nz) <= b->edge_elems &&
             size_t(S.off) + size_t(S.pitch) * size_t(S.nrow) <= b->field_elems &&
+            S.off + S.mdel >= 0 && size_t(S.off + S.mdel) + size_t(S.pitch) * size_t(S.nrow) <= b->model_elems &&
             size_t(S.prof) + size_t(4 * (S.nx + 2 * pml) + 4 * (S.nz + 2 * pml)) <= profs.size();
         if (!ok) fail(OWL_ERR_ARG, "internal layout check failed for shot " + std::to_string(s));
     }
@@ -1548,6 +1633,7 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     b->d_shots.upload(b->sd, st);
     b->d_tile_shot.upload(b->tile_shot, st);
     b->d_ftile_shot.upload(b->ftile_shot, st);
+    b->d_atile_shot.upload(b->atile_shot, st);
     b->d_prof.upload(profs, st);
     if (amp.empty()) amp.push_back(0.f);
     b->d_amp.upload(amp, st);
@@ -1624,9 +1710,10 @@ void prof_collect(owl_acoustic2d* b) {
 
 // Algorithmic (compulsory) bytes of one launch of the update kernels over
 // all shots for nsets field sets: each array element read or written once.
-//   p-sweep: vx, vz, p, bk read, p written (20 B/point) + 8 B per slab
-//            cell and damped direction (memory variable read + write);
-//   v-sweep: p, vx, vz, rho read, vx, vz written (24 B/point) + slabs.
+//   p-sweep: vx, vz, p read, p written (16 B/point) + bk (4 B per point of
+//            each distinct window) + 8 B per slab cell and damped direction
+//            (memory variable read + write);
+//   v-sweep: p, vx, vz read, vx, vz written (20 B/point) + rho + slabs.
 struct StepBytes { double p, v; };
 StepBytes step_bytes(const owl_acoustic2d* b, int nsets) {
     double bp = 0, bv = 0;
@@ -1636,10 +1723,10 @@ StepBytes step_bytes(const owl_acoustic2d* b, int nsets) {
         const double pts = lx * lz;
         const double slab_p = 2.0 * pml * lz + 2.0 * pml * lx;              // i<=0|i>=nx+1 ; j likewise
         const double slab_v = (2.0 * pml + 1) * lz + (2.0 * pml + 1) * lx;  // m<=1|m>=nx+1 ; likewise
-        bp += 20.0 * pts + 8.0 * slab_p;
-        bv += 24.0 * pts + 8.0 * slab_v;
+        bp += 16.0 * pts + 8.0 * slab_p;
+        bv += 20.0 * pts + 8.0 * slab_v;
     }
-    return StepBytes{bp * nsets, bv * nsets};
+    return StepBytes{(bp + 4.0 * b->pts_model) * nsets, (bv + 4.0 * b->pts_model) * nsets};
 }
 
 template <int R>
@@ -1814,29 +1901,35 @@ W7 work3(owl_acoustic2d* b) {
 // The other half of the w1/w3 ping-pong.
 W7 other_of(owl_acoustic2d* b, const W7& w) { return w.f[0] == b->fA[0].get() ? work3(b) : work1(b); }
 
-// Algorithmic bytes of one fused step over all shots: p, vx, vz, bk, rho
-// read, p, vx, vz written (32 B/point), plus 8 B per slab cell and damped
-// direction (memory variable read + write) for the p and the v parts.
+// Algorithmic bytes of one fused step over all shots: p, vx, vz read and
+// written (24 B/point), bk and rho read once per distinct window (shots on
+// one window share them), plus 8 B per slab cell and damped direction
+// (memory variable read + write) for the p and the v parts.
 double fused_bytes(const owl_acoustic2d* b) {
     double by = 0;
     for (auto& S : b->sd) {
         const double lx = S.nx + 2.0 * b->pml, lz = S.nz + 2.0 * b->pml;
         const double slab_p = 2.0 * b->pml * lz + 2.0 * b->pml * lx;
         const double slab_v = (2.0 * b->pml + 1) * lz + (2.0 * b->pml + 1) * lx;
-        by += 32.0 * lx * lz + 8.0 * (slab_p + slab_v);
+        by += 24.0 * lx * lz + 8.0 * (slab_p + slab_v);
     }
-    return by;
+    return by + 8.0 * b->pts_model;  // bk and rho once per distinct window
+}
+
+// The fused kernels address fields and model with 32-bit byte offsets.
+bool offsets32(const owl_acoustic2d* b) {
+    return b->field_elems < (size_t(1) << 29) && b->model_elems < (size_t(1) << 29);
 }
 
 // The fused adjoint step (k_adj_fused) runs wherever the fused forward step
 // may: batches of 4 shots or more by default, no free surface.
 bool adj_fused(const owl_acoustic2d* b) {
-    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
+    if (b->step_kernel == 0 || b->cfg.free_surface || !offsets32(b)) return false;
     return !(b->step_kernel == 2 && b->sd.size() < 4);
 }
 
 bool use_fused(const owl_acoustic2d* b, const W7& in, const W7& out) {
-    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
+    if (b->step_kernel == 0 || b->cfg.free_surface || !offsets32(b)) return false;
     if (b->step_kernel == 2 && b->sd.size() < 4) return false;
     for (int k = 0; k < 7; ++k)
         for (int m = 0; m < 7; ++m)
@@ -1865,7 +1958,7 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
     if (use_fused(b, in, out)) {
         const int ev = prof_begin(b, kKFused, b->profiling ? fused_bytes(b) : 0.0);
         hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
-                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
+                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_qhx.get(), b->d_qhz.get(),
                            b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
         prof_end(b, ev);
     } else {
@@ -2022,12 +2115,13 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
         float** in = b->adj_cur ? s1 : s0;
         float** out = b->adj_cur ? s0 : s1;
         const AdjIO io{in[0], in[1], in[2], in[3], in[4], out[0], out[1], out[2], out[3], out[4]};
-        // ex, ez, ph, bk, p_t, p_{t-1}, rho, vxh, vzh, 4 states read; ph, vxh, vzh, ex, ez
-        // written; gK, gX, gZ r+w (120 B per point, PML memories aside)
-        const int ev = prof_begin(b, 14, pts * 120.0);
-        hipLaunchKernelGGL(k_adj_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
-                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, io, A, b->d_bk.get(), b->d_rho.get(),
-                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc, cur[0], prev[0], cur[1],
+        // ex, ez, ph, p_t, p_{t-1}, vxh, vzh, 4 states read; ph, vxh, vzh, ex, ez
+        // written; gK, gX, gZ r+w (112 B per point, PML memories aside); bk
+        // and rho once per distinct window
+        const int ev = prof_begin(b, 14, pts * 112.0 + 8.0 * b->pts_model);
+        hipLaunchKernelGGL(k_adj_fused<R>, dim3(unsigned((b->natile + 3) / 4)), dim3(64, 4), 0, b->stream,
+                           b->d_shots.get(), b->d_atile_shot.get(), b->natile, io, A, b->d_bk.get(), b->d_rho.get(),
+                           b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->azc, cur[0], prev[0], cur[1],
                            prev[1], cur[2], prev[2]);
         prof_end(b, ev);
         b->adj_cur = 1 - b->adj_cur;
@@ -2039,7 +2133,7 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
         return;
     }
     // p-pass: ex, ez, ph, bk, p_t, p_{t-1}, psx/psz read; ph, dx, dz written; gK r+w
-    int ev = prof_begin(b, 10, pts * (6 * 4 + 3 * 4 + 16));
+    int ev = prof_begin(b, 10, pts * (5 * 4 + 3 * 4 + 16) + 4.0 * b->pts_model);
     hipLaunchKernelGGL(k_adj_p<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), A, b->d_bk.get(),
                        b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, cur[0], prev[0]);
     prof_end(b, ev);
@@ -2048,7 +2142,7 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
                            b->d_inj_beg.get(), b->d_inj_taps.get(), b->d_amp.get(), b->d_bk.get(), b->adj[0].get(),
                            b->gK.get(), b->nt, t);
     // v-pass: dx, dz, vxh, vzh, rho, 4 states read; vxh, vzh, ex, ez written; gX, gZ r+w
-    ev = prof_begin(b, 11, pts * (9 * 4 + 4 * 4 + 32));
+    ev = prof_begin(b, 11, pts * (8 * 4 + 4 * 4 + 32) + 4.0 * b->pts_model);
     hipLaunchKernelGGL(k_adj_v<R>, grd, blk, 0, b->stream, b->d_shots.get(), b->d_tile_shot.get(), A, b->d_rho.get(),
                        b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, cur[1], prev[1], cur[2], prev[2]);
     prof_end(b, ev);
@@ -2352,12 +2446,80 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
     (void)adj_steps;
 }
 
+// Autotuner of the fused kernels' z-chunk rows: times each candidate over
+// three launches (after one warm-up) on the batch's own buffers and keeps
+// the fastest for the forward step and for the adjoint step.  Results do
+// not depend on the choice.  The buffers it writes (work3, the adjoint
+// set) are cleared again before a gradient uses them.
+template <int R>
+void autotune_rows(owl_acoustic2d* b) {
+    hipStream_t st = b->stream;
+    adjoint_alloc_zero(b);
+    const W7 in = work1(b), out = work3(b), w2 = work2(b);
+    FieldSet F{};
+    F.p_in = in.f[0];
+    F.p_out = out.f[0];
+    F.vx_in = in.f[1];
+    F.vx_out = out.f[1];
+    F.vz_in = in.f[2];
+    F.vz_out = out.f[2];
+    for (int k = 0; k < 4; ++k) {
+        F.m_in[k] = in.f[3 + k];
+        F.m_out[k] = out.f[3 + k];
+    }
+    DevBuf<float>* a = b->adj;
+    DevBuf<float>* a2 = b->adj2;
+    const AdjIO io{a[0].get(), a[3].get(), a[4].get(), a[7].get(), a[8].get(),
+                   a2[0].get(), a2[1].get(), a2[2].get(), a2[3].get(), a2[4].get()};
+    const AdjSet A = adj_set(b);
+    auto timed = [&](auto&& launch) {
+        launch();
+        OWL_HIP(hipEventRecord(b->ev0, st));
+        for (int k = 0; k < 3; ++k) launch();
+        OWL_HIP(hipEventRecord(b->ev1, st));
+        OWL_HIP(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        return ms;
+    };
+    const int fwd_rows[] = {8, 12, 16, 24, 32}, adj_rows[] = {4, 6, 8, 12, 16};
+    int best_f = b->fzc, best_a = b->azc;
+    float tf = 1e30f, ta = 1e30f;
+    for (int z : fwd_rows) {
+        b->fzc = z;
+        upload_fused_tiling(b);
+        const float ms = timed([&] {
+            hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, st,
+                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_qhx.get(), b->d_qhz.get(),
+                               b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
+        });
+        if (ms < tf) tf = ms, best_f = z;
+    }
+    b->fzc = best_f;
+    for (int z : adj_rows) {
+        b->azc = z;
+        upload_fused_tiling(b);
+        const float ms = timed([&] {
+            hipLaunchKernelGGL(k_adj_fused<R>, dim3(unsigned((b->natile + 3) / 4)), dim3(64, 4), 0, st,
+                               b->d_shots.get(), b->d_atile_shot.get(), b->natile, io, A, b->d_bk.get(),
+                               b->d_rho.get(), b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->azc,
+                               in.f[0], w2.f[0], in.f[1], w2.f[1], in.f[2], w2.f[2]);
+        });
+        if (ms < ta) ta = ms, best_a = z;
+    }
+    b->azc = best_a;
+    upload_fused_tiling(b);
+    OWL_HIP(hipGetLastError());
+    b->tuned = true;
+}
+
 // Every device pointer and launch parameter an exact-adjoint call bakes
 // into its launches: a graph captured under one signature is replayed only
 // under the same one.
 std::vector<uintptr_t> exact_signature(owl_acoustic2d* b, int ckpt, int param, int fmt) {
     std::vector<uintptr_t> v{uintptr_t(ckpt), uintptr_t(param), uintptr_t(fmt), uintptr_t(b->R), uintptr_t(b->nt),
                              uintptr_t(b->step_kernel), uintptr_t(b->fzc), uintptr_t(b->nftile),
+                             uintptr_t(b->azc), uintptr_t(b->natile),
                              uintptr_t(b->tile_shot.size()), uintptr_t(b->field_elems)};
     auto add = [&](const void* p) { v.push_back(reinterpret_cast<uintptr_t>(p)); };
     for (int k = 0; k < 3; ++k) { add(b->fA[k].get()); add(b->fB[k].get()); add(b->fC[k].get()); }
@@ -2369,7 +2531,7 @@ std::vector<uintptr_t> exact_signature(owl_acoustic2d* b, int ckpt, int param, i
     for (auto* f : {&b->d_traces, &b->d_obs, &b->d_adj, &b->d_bk, &b->d_rho, &b->d_prof, &b->d_amp, &b->d_rec_w,
                     &b->d_vpw, &b->d_rhow})
         add(f->get());
-    add(b->d_shots.get()); add(b->d_tile_shot.get()); add(b->d_ftile_shot.get());
+    add(b->d_shots.get()); add(b->d_tile_shot.get()); add(b->d_ftile_shot.get()); add(b->d_atile_shot.get());
     add(b->d_inj_beg.get()); add(b->d_adj_beg.get()); add(b->d_rec_beg.get());
     add(b->d_inj_taps.get()); add(b->d_adj_taps.get()); add(b->d_rec_taps.get());
     return v;
@@ -2393,6 +2555,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     if (do_rev && !do_fwd &&
         (!b->have_store || b->store_policy != ckpt || b->store_param != param || b->store_fmt != fmt))
         fail(OWL_ERR_STATE, "backward needs a preceding forward_store with the same storage policy");
+    if (b->autotune && !b->tuned && adj_fused(b)) autotune_rows<R>(b);
     const bool graph_ok = phase == kBoth && b->use_graphs && !b->profiling;
     std::vector<uintptr_t> sig;
     if (graph_ok) sig = exact_signature(b, ckpt, param, fmt);
@@ -2742,13 +2905,38 @@ owl_status owl_acoustic2d_set_step_kernel(owl_acoustic2d* b, int mode, int zrows
     return guarded([&] {
         if (!b || mode < 0 || mode > 2 || zrows < 1) fail(OWL_ERR_ARG, "bad step kernel choice");
         b->step_kernel = mode;
+        b->autotune = false;
         if (zrows != b->fzc) {
             b->fzc = zrows;
-            fused_tiling(b);
-            b->d_shots.upload(b->sd, b->stream);
-            b->d_ftile_shot.upload(b->ftile_shot, b->stream);
-            OWL_HIP(hipStreamSynchronize(b->stream));
+            upload_fused_tiling(b);
         }
+    });
+}
+
+owl_status owl_acoustic2d_set_adjoint_rows(owl_acoustic2d* b, int zrows) {
+    return guarded([&] {
+        if (!b || zrows < 1) fail(OWL_ERR_ARG, "bad adjoint row count");
+        b->autotune = false;
+        if (zrows != b->azc) {
+            b->azc = zrows;
+            upload_fused_tiling(b);
+        }
+    });
+}
+
+owl_status owl_acoustic2d_set_autotune(owl_acoustic2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->autotune = on != 0;
+        b->tuned = false;
+    });
+}
+
+owl_status owl_acoustic2d_fused_rows(const owl_acoustic2d* b, int* zrows, int* adjoint_rows) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        if (zrows) *zrows = b->fzc;
+        if (adjoint_rows) *adjoint_rows = b->azc;
     });
 }
 

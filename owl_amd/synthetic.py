@@ -125,3 +125,62 @@ def acoustic3d_shots(n=256, d=10.0, seed=0, f0=8.0, zs=20.0, zr=20.0, nrec_side=
             out.append((g, win))
             sid += 1
     return out
+
+
+def thomsen_tilted_cij(vp, vs, rho, eps, delta, theta):
+    """2D elastic moduli (c11, c13, c15, c33, c35, c55) of a tilted TI medium
+    from Thomsen parameters: VTI moduli c33 = rho vp^2, c55 = rho vs^2,
+    c11 = c33 (1 + 2 eps), (c13 + c55)^2 = 2 delta c33 (c33 - c55) +
+    (c33 - c55)^2, rotated by the tilt theta (radians) with the Bond matrix
+    of the x-z plane.  Arrays broadcast; float32 dict out."""
+    vp, vs, rho, eps, delta, theta = np.broadcast_arrays(*(np.asarray(a, np.float64) for a in
+                                                           (vp, vs, rho, eps, delta, theta)))
+    c33 = rho * vp * vp
+    c55 = rho * vs * vs
+    c11 = c33 * (1 + 2 * eps)
+    c13 = np.sqrt(2 * delta * c33 * (c33 - c55) + (c33 - c55) ** 2) - c55
+    c, s = np.cos(theta), np.sin(theta)
+    M = np.array([[c * c, s * s, 2 * c * s], [s * s, c * c, -2 * c * s], [-c * s, c * s, c * c - s * s]])
+    z = np.zeros_like(c11)
+    C0 = np.array([[c11, c13, z], [c13, c33, z], [z, z, c55]])
+    Cr = np.einsum("ik...,kl...,jl...->ij...", M, C0, M)
+    keys = ("c11", "c13", "c15", "c33", "c35", "c55")
+    return {k: np.asarray(Cr[i, j], np.float32) for k, (i, j) in
+            zip(keys, ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)))}
+
+
+def fsg2d_tti_model(nx=801, nz=401, dz=5.0, seed=0):
+    """Tilted-TI model of the FSG workload: vp 1800 -> 4500 m/s with depth
+    plus a smooth random perturbation (+-5 %), vs = vp / 1.8, Gardner rho,
+    epsilon in U(0.1, 0.3) and delta in U(0, 0.15) as smooth random fields,
+    tilt a smooth random field in [0, 45] degrees.  Returns (cij dict, rho),
+    each (nz, nx)."""
+    z = np.arange(nz, dtype=np.float64) * dz
+    base = 1800.0 + 2700.0 * z / z[-1]
+    vp = base[:, None] * (1.0 + 0.05 * random_field((nz, nx), 10.0, seed))
+    u = lambda s: 0.5 * (1.0 + random_field((nz, nx), 15.0, s))  # noqa: E731  smooth, in [0, 1]
+    eps = 0.1 + 0.2 * u(seed + 1)
+    delta = 0.15 * u(seed + 2)
+    theta = np.deg2rad(45.0) * u(seed + 3)
+    rho = gardner(vp)
+    return thomsen_tilted_cij(vp, vp / 1.8, rho, eps, delta, theta), rho
+
+
+def fsg2d_shots(nx=801, dx=5.0, nshot=64, seed=0):
+    """64 surface shots (vertical force, 15 Hz Ricker, at 10 m depth) spread
+    along x, each with its own window of 200..801 columns around the source
+    (full depth) and receivers every 10 m at 10 m depth inside the window.
+    Returns [(Geometry, window dict)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nshot):
+        w = int(rng.integers(200, nx + 1))
+        xs = (k + 0.5) * (nx - 1) * dx / nshot
+        c = int(round(xs / dx)) + 1
+        b = min(max(c - w // 2, 1), nx - w + 1)
+        e = b + w - 1
+        x0 = (b - 1) * dx
+        recs = [Receiver(x=float(x), y=0.0, z=10.0) for x in np.arange(x0, (e - 1) * dx + 1e-3, 10.0)]
+        src = Source(x=float(xs), y=0.0, z=10.0, f0=15.0, mechanism="force", polar=0.0, amp=1e6)
+        out.append((Geometry(id=k + 1, srcr=[src], recr=recs), dict(nxbeg=b, nxend=e, nzbeg=0, nzend=0)))
+    return out

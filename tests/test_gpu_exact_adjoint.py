@@ -146,11 +146,12 @@ def test_autograd_backward_equals_gradient(oracle32, gpu):
     b.close()
 
 
-@pytest.mark.parametrize("R,zrows", [(4, 32), (2, 7), (8, 16), (5, 1)])
-def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows):
-    """The fused z-marching forward step against the two sweeps (p, then v):
-    five shots on uneven windows, traces, misfits and gradients bit-identical
-    for full FP32 and FP16 storage, uniform and binomial checkpoints."""
+@pytest.mark.parametrize("R,zrows,arows", [(4, 32, 8), (2, 7, 3), (8, 16, 16), (5, 1, 5)])
+def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows, arows):
+    """The fused z-marching forward and adjoint steps against the two sweeps
+    and the two-pass adjoint: five shots on uneven windows, traces, misfits
+    and gradients bit-identical for full FP32 and FP16 storage, uniform and
+    binomial checkpoints, at several z-row counts of either kernel."""
     from owl_amd.synthetic import gardner, random_field
     nz, nx, dx = 60, 140, 10.0
     vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 11))).astype(np.float32)
@@ -173,10 +174,12 @@ def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows):
              (OWL_CKPT_UNIFORM, 9, OWL_STORE_F32), (OWL_CKPT_BINOMIAL, 6, OWL_STORE_F32),
              (OWL_CKPT_BINOMIAL, 5, OWL_STORE_F16)]
     for ckpt, param, store in cases:
-        print(f"case: fused vs sweeps R={R} zrows={zrows} ckpt={ckpt} param={param} store={store}", flush=True)
+        print(f"case: fused vs sweeps R={R} zrows={zrows} arows={arows} ckpt={ckpt} param={param} store={store}",
+              flush=True)
         out = []
         for mode in ("sweeps", "fused"):
             b.set_step_kernel(mode, zrows)
+            b.set_adjoint_rows(arows)
             b.set_profiling(True)
             out.append(b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store))
             st = b.kernel_stats()
@@ -222,4 +225,37 @@ def test_graph_replays_are_bit_identical_to_eager(gpu):
             if k % 2:
                 b.forward()   # eager work between replays
     b.set_graphs(False)
+    b.close()
+
+
+def test_autotuned_rows_give_the_same_gradient(gpu):
+    """The z-row autotuner picks rows from its candidate sets, and the
+    autotuned batch's exact gradient is bit-identical to one with fixed rows."""
+    from owl_amd.synthetic import gardner, random_field
+    nz, nx, dx = 60, 140, 10.0
+    vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 12))).astype(np.float32)
+    rho = gardner(vp)
+    wins = [Window(1, 140, 1, 60, 0.0, 0.0), Window(11, 70, 1, 45, 100.0, 0.0),
+            Window(50, 131, 5, 60, 490.0, 40.0), Window(30, 62, 1, 33, 290.0, 0.0)]
+    geoms = []
+    for k, w in enumerate(wins):
+        x0 = (w.nxbeg - 1) * dx
+        rx = x0 + 5.0 + 10.0 * np.arange(w.nx - 1)
+        geoms.append(simple_geometry(k + 1, x0 + 0.5 * (w.nx - 1) * dx + 3.3, (w.nzbeg - 1) * dx + 80.0, rx,
+                                     np.full(len(rx), (w.nzbeg - 1) * dx + 30.0), f0=20.0))
+    print("case: autotuned fused rows", flush=True)
+    b = Acoustic2DBatch(nx, nz, dx, dx, vp, rho, list(zip(geoms, wins)), 0.001, 0.15, pml=10, fdhalf=4)
+    obs = b.forward()
+    b.set_model((vp * 0.98).astype(np.float32), rho)
+    b.set_graphs(False)
+    kw = dict(mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_BINOMIAL, ckpt_param=6, store=OWL_STORE_F32)
+    m0, gv0, gr0, _ = b.gradient(obs, **kw)
+    z, a = b.fused_rows()
+    assert z in (8, 12, 16, 24, 32) and a in (4, 6, 8, 12, 16)
+    b.set_step_kernel("fused", 16)
+    b.set_adjoint_rows(8)
+    m1, gv1, gr1, _ = b.gradient(obs, **kw)
+    assert np.array_equal(m0, m1)
+    assert np.abs(gv0).max() > 0
+    assert np.array_equal(gv0, gv1) and np.array_equal(gr0, gr1)
     b.close()
