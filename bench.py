@@ -127,7 +127,7 @@ def cpu_baseline(vp, rho, vp0, rho0, shot, nt):
                 seconds=round(wall, 3), s_per_shot=round(wall, 3))
 
 
-def bench_3d(sync, steps=1500):
+def bench_3d(sync, steps=1500, shape=None):
     """3D acoustic batched forward: 256^3, 10 m, R = 4, 1500 steps, 20-cell
     PML, 5-layer model, 16 shots with uneven 128..256 windows, 100 x 100
     receivers each.  dt = 0.9 ms: 1 ms exceeds the scheme's stable step
@@ -138,7 +138,11 @@ def bench_3d(sync, steps=1500):
     shots = acoustic3d_shots()
     dt = 0.0009
     b = Acoustic3DBatch(256, 256, 256, 10.0, 10.0, 10.0, vp, rho, shots, dt, (steps - 1) * dt, pml=20, fdhalf=4)
-    kc, var = b.autotune(3)
+    if shape:
+        kc, var = shape
+        b.set_launch(kc, var)
+    else:
+        kc, var = b.autotune(3)
     b.set_profiling(True)
     sync()
     t0 = time.perf_counter()
@@ -204,6 +208,8 @@ def main():
     ap.add_argument("--only-3d", action="store_true", help="only the 3D batched forward (for profiler runs)")
     ap.add_argument("--steps-3d", type=int, default=1500,
                     help="with --only-3d: time steps of the 3D forward (short counter runs)")
+    ap.add_argument("--shape-3d", default="", metavar="PLANES,VARIANT",
+                    help="with --only-3d: this launch shape instead of the autotuner's")
     ap.add_argument("--no-fsg", action="store_true", help="skip the 2D FSG forward")
     ap.add_argument("--only-fsg", action="store_true", help="only the 2D FSG forward (for profiler runs)")
     ap.add_argument("--fsg-steps", type=int, default=5000, help="time steps of the 2D FSG forward")
@@ -234,7 +240,8 @@ def main():
     sync = torch.cuda.synchronize
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
     if args.only_3d:
-        print(json.dumps({"acoustic3d_forward": bench_3d(sync, args.steps_3d)}))
+        shape = tuple(int(v) for v in args.shape_3d.split(",")) if args.shape_3d else None
+        print(json.dumps({"acoustic3d_forward": bench_3d(sync, args.steps_3d, shape)}))
         return
     if args.only_fsg:
         print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps)}))

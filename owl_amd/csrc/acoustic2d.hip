@@ -244,23 +244,11 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
 // tile may already have written); arithmetic as k_update_p / k_update_v,
 // so the results are bit-identical to the two-sweep form.
 // ---------------------------------------------------------------------------
-// Load from base + a 32-bit byte offset: the batch's buffers stay below
-// 2 GiB (checked on the host), so the compiler can address them as a
-// scalar base plus one 32-bit VGPR.
-template <class T>
-__device__ __forceinline__ T ldo(const T* base, unsigned off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
-}
-template <class T>
-__device__ __forceinline__ void sto(T* base, unsigned off, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
-}
-
 template <int R>
 __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ shots,
                                                     const int* __restrict__ ftile_shot, int nftile,
                                                     FieldSet F, const float* __restrict__ bk,
-                                                    const float* __restrict__ qhx, const float* __restrict__ qhz,
+                                                    const float* __restrict__ rho,
                                                     const float* __restrict__ prof, Coef c, float dt,
                                                     float idx, float idz, int pml, int zc) {
     constexpr int W = 64 - (2 * R - 1);
@@ -296,42 +284,41 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
         axh_ = axh[col + pml - 1];
         bxh_ = bxh[col + pml - 1];
     }
-    // byte offsets: field element of (col, lo), one row, model minus field
-    const unsigned base = unsigned(S.off + (col - lo)) * 4u;
-    const unsigned rowb = unsigned(S.pitch) * 4u;
-    const unsigned mdb = unsigned(S.mdel) * 4u;
+    const size_t pitch = size_t(S.pitch);
+    const size_t base = size_t(S.off) + size_t(col - lo);  // (col, lo)
     // Inputs of one loop iteration: the p row r and the v row j = r - R + 1.
     struct Row {
         float vx[2 * R];  // vx_in at row r, columns col - R + 1 .. col + R
         float vzf;        // vz_in at row r + R (the front of the vz queue)
         float p, bk, m0, m1;
-        float qx, qz, vxj, m2, m3;  // dt / rho at the velocity nodes of row j, vx_in (j)
+        float rq, rl, ru, vxj, m2, m3;  // rho (j), rho (j, col - 1), rho (j - 1), vx_in (j)
     };
     const int rs = z0 - R, re = z1 + R - 1;
     auto load = [&](int r, Row& w) {
         const int vr = r + R;
-        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? ldo(F.vz_in, base + unsigned(vr - lo) * rowb) : 0.0f;
+        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? F.vz_in[base + size_t(vr - lo) * pitch] : 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const unsigned q = base + unsigned(r - lo) * rowb;
+            const size_t q = base + size_t(r - lo) * pitch;
 #pragma unroll
-            for (int k = 0; k < 2 * R; ++k) w.vx[k] = ldo(F.vx_in, q + unsigned(k - (R - 1)) * 4u);
-            w.p = ldo(F.p_in, q);
-            w.bk = ldo(bk, q + mdb);
-            w.m0 = xs_i ? ldo(F.m_in[0], q) : 0.0f;
-            w.m1 = (r <= 0 || r >= nz + 1) ? ldo(F.m_in[1], q) : 0.0f;
+            for (int k = 0; k < 2 * R; ++k) w.vx[k] = F.vx_in[q - (R - 1) + k];
+            w.p = F.p_in[q];
+            w.bk = bk[mq(S, q)];
+            w.m0 = xs_i ? F.m_in[0][q] : 0.0f;
+            w.m1 = (r <= 0 || r >= nz + 1) ? F.m_in[1][q] : 0.0f;
         }
         const int j = r - R + 1;
         if (own && j >= z0) {
-            const unsigned q = base + unsigned(j - lo) * rowb;
-            w.qx = ldo(qhx, q + mdb);
-            w.qz = ldo(qhz, q + mdb);
-            w.vxj = ldo(F.vx_in, q);
-            w.m2 = xs_h ? ldo(F.m_in[2], q) : 0.0f;
-            w.m3 = (j <= 1 || j >= nz + 1) ? ldo(F.m_in[3], q) : 0.0f;
+            const size_t q = base + size_t(j - lo) * pitch;
+            w.rq = rho[mq(S, q)];
+            w.rl = rho[mq(S, q) - 1];
+            w.ru = rho[mq(S, q) - pitch];
+            w.vxj = F.vx_in[q];
+            w.m2 = xs_h ? F.m_in[2][q] : 0.0f;
+            w.m3 = (j <= 1 || j >= nz + 1) ? F.m_in[3][q] : 0.0f;
         }
     };
     auto vz_row = [&](int row) -> float {
-        return (mcol && row >= lo && row <= nz + pml + R) ? ldo(F.vz_in, base + unsigned(row - lo) * rowb) : 0.0f;
+        return (mcol && row >= lo && row <= nz + pml + R) ? F.vz_in[base + size_t(row - lo) * pitch] : 0.0f;
     };
     // V[k] = vz_in at row r - R + 1 + k;  P[k] = new p at row r - 2R + 1 + k
     float V[2 * R], P[2 * R];
@@ -339,82 +326,77 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
     for (int k = 0; k < 2 * R - 1; ++k) V[k] = vz_row(rs - R + 1 + k);
 #pragma unroll
     for (int k = 0; k < 2 * R; ++k) P[k] = 0.0f;
-    // one row: p at (col, r), then v at row r - R + 1, from the inputs in w
-    auto row = [&](int r, const Row& w) {
-        V[2 * R - 1] = w.vzf;
+    Row cur, nxt;
+    load(rs, cur);
+    for (int r = rs; r <= re; ++r) {
+        if (r < re) load(r + 1, nxt);  // in flight while row r is computed
+        V[2 * R - 1] = cur.vzf;
+        // ---- p at (col, r)
         float pn = 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const unsigned q = base + unsigned(r - lo) * rowb;
-            float sx = c.c[0] * (w.vx[R] - w.vx[R - 1]);
+            const size_t q = base + size_t(r - lo) * pitch;
+            float sx = c.c[0] * (cur.vx[R] - cur.vx[R - 1]);
             float sz = c.c[0] * (V[R] - V[R - 1]);
 #pragma unroll
             for (int m = 2; m <= R; ++m) {
-                sx = sx + c.c[m - 1] * (w.vx[R - 1 + m] - w.vx[R - m]);
+                sx = sx + c.c[m - 1] * (cur.vx[R - 1 + m] - cur.vx[R - m]);
                 sz = sz + c.c[m - 1] * (V[R - 1 + m] - V[R - m]);
             }
             float pdx = idx * sx;
             float pdz = idz * sz;
             const bool wr = own && r >= z0 && r <= z1;
             if (xs_i) {
-                const float mem = ax * pdx + bx * w.m0;
-                if (wr) sto(F.m_out[0], q, mem);
+                const float mem = ax * pdx + bx * cur.m0;
+                if (wr) F.m_out[0][q] = mem;
                 pdx = pdx + mem;
             }
             if (r <= 0 || r >= nz + 1) {
-                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * w.m1;
-                if (wr) sto(F.m_out[1], q, mem);
+                const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * cur.m1;
+                if (wr) F.m_out[1][q] = mem;
                 pdz = pdz + mem;
             }
-            pn = w.p - dt * w.bk * (pdx + pdz);
-            if (wr) sto(F.p_out, q, pn);
+            pn = cur.p - dt * cur.bk * (pdx + pdz);
+            if (wr) F.p_out[q] = pn;
         }
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) P[k] = P[k + 1];
         P[2 * R - 1] = pn;
+        // ---- v at row j = r - R + 1
         const int j = r - R + 1;
         if (j >= z0) {
             float px[2 * R];  // new p of row j at columns col - R .. col + R - 1
 #pragma unroll
             for (int k = 0; k < 2 * R; ++k) px[k] = __shfl(P[R], l - R + k, 64);
             if (own) {
-                const unsigned q = base + unsigned(j - lo) * rowb;
+                const size_t q = base + size_t(j - lo) * pitch;
                 float s = c.c[0] * (px[R] - px[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (px[R + m - 1] - px[R - m]);
                 float d = idx * s;
                 if (xs_h) {
-                    const float mem = axh_ * d + bxh_ * w.m2;
-                    sto(F.m_out[2], q, mem);
+                    const float mem = axh_ * d + bxh_ * cur.m2;
+                    F.m_out[2][q] = mem;
                     d = d + mem;
                 }
-                sto(F.vx_out, q, w.vxj - w.qx * d);
+                float reff = 0.5f * (cur.rq + cur.rl);
+                F.vx_out[q] = cur.vxj - dt / reff * d;
                 s = c.c[0] * (P[R] - P[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (P[R + m - 1] - P[R - m]);
                 d = idz * s;
                 if (j <= 1 || j >= nz + 1) {
-                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * w.m3;
-                    sto(F.m_out[3], q, mem);
+                    const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * cur.m3;
+                    F.m_out[3][q] = mem;
                     d = d + mem;
                 }
-                sto(F.vz_out, q, V[0] - w.qz * d);
+                reff = 0.5f * (cur.rq + cur.ru);
+                F.vz_out[q] = V[0] - dt / reff * d;
             }
         }
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) V[k] = V[k + 1];
-    };
-    // two row buffers in turn: the next row's inputs are in flight while a
-    // row is computed, with no copy between them
-    Row ra, rb;
-    load(rs, ra);
-    int r = rs;
-    for (; r + 1 <= re; r += 2) {
-        load(r + 1, rb);
-        row(r, ra);
-        if (r + 2 <= re) load(r + 2, ra);
-        row(r + 1, rb);
+        cur = nxt;
     }
-    if (r <= re) row(r, ra);
 }
 
 // One tap of a point source or receiver: field cell and two weights.
@@ -898,8 +880,10 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
             w.psx = xs_i ? io.psx[q] : 0.0f;
             w.psz = (r <= 0 || r >= nz + 1) ? io.psz[q] : 0.0f;
             w.k = bk[mq(S, q)];
-            w.pc = p_cur[q];
-            w.pp = p_prev[q];
+            if (own && r >= z0 && r <= z1) {  // the K term is imaged on the tile's own cells only
+                w.pc = p_cur[q];
+                w.pp = p_prev[q];
+            }
         }
         const int j = r - R + 1;
         if (own && j >= z0) {
@@ -1201,7 +1185,6 @@ struct owl_acoustic2d {
     DevBuf<Shot> d_shots;
     DevBuf<int> d_tile_shot;
     DevBuf<float> d_bk, d_rho, d_den, d_prof;
-    DevBuf<float> d_qhx, d_qhz;       // dt / rho at the half nodes (model layout), for the fused kernels
     DevBuf<float> fA[3], fB[3], fM[4];  // forward field (ping-pong) and PML memory
     DevBuf<float> aF[3], aM[4];         // adjoint field and PML memory
     DevBuf<float> d_amp;                // source amplitude rows [row][nt]
@@ -1300,23 +1283,8 @@ void upload_model(owl_acoustic2d* b) {
                 rhow[size_t(S.g0) + size_t(i - 1) + size_t(j - 1) * H.nx] = r;
             }
     }
-    // dt over the 2-point densities at the velocity nodes, as the v-sweep forms
-    // them (0.5 (rho(i) + rho(i-1)), then the division), for the fused kernels
-    std::vector<float> qx(b->model_elems, 0.f), qz(b->model_elems, 0.f);
-    for (size_t s = 0; s < b->hs.size(); ++s) {
-        const Shot& S = b->sd[s];
-        const size_t m0 = size_t(S.off + S.mdel), pitch = size_t(S.pitch);
-        for (size_t jj = 1; jj < size_t(S.nrow); ++jj)
-            for (size_t ii = 1; ii < pitch; ++ii) {
-                const size_t q = m0 + ii + jj * pitch;
-                qx[q] = b->dt / (0.5f * (rh[q] + rh[q - 1]));
-                qz[q] = b->dt / (0.5f * (rh[q] + rh[q - pitch]));
-            }
-    }
     b->d_bk.upload(bk, b->stream);
     b->d_rho.upload(rh, b->stream);
-    b->d_qhx.upload(qx, b->stream);
-    b->d_qhz.upload(qz, b->stream);
     b->d_den.upload(den, b->stream);
     b->d_vpw.upload(vpw, b->stream);
     b->d_rhow.upload(rhow, b->stream);
@@ -1916,20 +1884,15 @@ double fused_bytes(const owl_acoustic2d* b) {
     return by + 8.0 * b->pts_model;  // bk and rho once per distinct window
 }
 
-// The fused kernels address fields and model with 32-bit byte offsets.
-bool offsets32(const owl_acoustic2d* b) {
-    return b->field_elems < (size_t(1) << 29) && b->model_elems < (size_t(1) << 29);
-}
-
 // The fused adjoint step (k_adj_fused) runs wherever the fused forward step
 // may: batches of 4 shots or more by default, no free surface.
 bool adj_fused(const owl_acoustic2d* b) {
-    if (b->step_kernel == 0 || b->cfg.free_surface || !offsets32(b)) return false;
+    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
     return !(b->step_kernel == 2 && b->sd.size() < 4);
 }
 
 bool use_fused(const owl_acoustic2d* b, const W7& in, const W7& out) {
-    if (b->step_kernel == 0 || b->cfg.free_surface || !offsets32(b)) return false;
+    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
     if (b->step_kernel == 2 && b->sd.size() < 4) return false;
     for (int k = 0; k < 7; ++k)
         for (int m = 0; m < 7; ++m)
@@ -1958,7 +1921,7 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
     if (use_fused(b, in, out)) {
         const int ev = prof_begin(b, kKFused, b->profiling ? fused_bytes(b) : 0.0);
         hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, b->stream,
-                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_qhx.get(), b->d_qhz.get(),
+                           b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
                            b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
         prof_end(b, ev);
     } else {
@@ -2490,7 +2453,7 @@ void autotune_rows(owl_acoustic2d* b) {
         upload_fused_tiling(b);
         const float ms = timed([&] {
             hipLaunchKernelGGL(k_step_fused<R>, dim3(unsigned((b->nftile + 3) / 4)), dim3(64, 4), 0, st,
-                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_qhx.get(), b->d_qhz.get(),
+                               b->d_shots.get(), b->d_ftile_shot.get(), b->nftile, F, b->d_bk.get(), b->d_rho.get(),
                                b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->fzc);
         });
         if (ms < tf) tf = ms, best_f = z;
