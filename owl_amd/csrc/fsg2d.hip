@@ -106,22 +106,21 @@ __device__ __forceinline__ size_t atf(const ShotF& S, int lo, int i, int j) {
     return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.px);
 }
 
-// m = a d + b m; d = (d + m) / kappa with kappa = 1
-__device__ __forceinline__ float mpml(float* m, size_t q, float d, float a, float b) {
-    const float v = a * d + b * m[q];
-    m[q] = v;
-    return d + v;
+// Load and store at a 32-bit byte offset from a kernel-argument base: a
+// batch's buffers stay below 2 GiB (checked on the host), so every access of
+// a sweep shares one offset register against scalar bases.
+__device__ __forceinline__ float ldo(const float* base, unsigned off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void sto(float* base, unsigned off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
 }
 
-// pdxw_stencil(w, i + off, j) (macro_fd_stencil.f90): sum_m c_m (w(i+off+m)
-// - w(i+off-m+1)) in the macro's order
-template <int R>
-__device__ __forceinline__ float sten_x(const float* __restrict__ w, size_t q, int off, const Coef& c) {
-    const float* p = w + q + off;
-    float s = c.c[0] * (p[1] - p[0]);
-#pragma unroll
-    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (p[m] - p[1 - m]);
-    return s;
+// m = a d + b m; d = (d + m) / kappa with kappa = 1
+__device__ __forceinline__ float mpml(float* m, unsigned qb, float d, float a, float b) {
+    const float v = a * d + b * ldo(m, qb);
+    sto(m, qb, v);
+    return d + v;
 }
 
 // the z stencil from a queue: slot k holds row j + k - R
@@ -133,83 +132,110 @@ __device__ __forceinline__ float sten_q(const float* qv, int off, const Coef& c)
     return s;
 }
 
+// pdxw_stencil(w, i + off, j) (macro_fd_stencil.f90) from the lanes' own
+// values: xv[k] is the value at column
+// i - R + off + 1 + k, taken from lane l - R + off + 1 + k; returns
+// sum_m c_m (w(i+off+m) - w(i+off-m+1)) in the macro's order.
+template <int R>
+__device__ __forceinline__ float sten_lanes(float v, int l, int off, const Coef& c) {
+    float xv[2 * R];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) xv[k] = __shfl(v, l - R + off + 1 + k, 64);
+    float s = c.c[0] * (xv[R] - xv[R - 1]);
+#pragma unroll
+    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (xv[R - 1 + m] - xv[R - m]);
+    return s;
+}
+
 // ---------------------------------------------------------------------------
 // Stress sweep (module_wavefield.f90:87-186): the integer-grid stresses at
 // (i, j) from dx of vx_hi, vz_hi and dz of vx_ih, vz_ih; the corner stresses
 // at storage (i, j) from dx of vx_ih, vz_ih and dz of vx_hi, vz_hi, each
 // one node back.  Reads velocities and old stresses from F, writes O.
+// One wavefront per tile of 64 - 2R columns: its 64 lanes hold columns
+// i0 - R .. i0 + 63 - R, each with register queues of the four velocities
+// over rows j - R .. j + R; the x stencils take the neighbouring lanes'
+// current rows by shuffles, so lanes R .. 63 - R update and the others only
+// feed them.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ shots, const int* __restrict__ tile_shot,
                                                     FF F, FV O, FM M, FP P, Coef c, float dt, float dx, float dz,
                                                     int pml, int kc) {
+    constexpr int W = 64 - 2 * R;
     const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
     const ShotF S = shots[tile_shot[tile]];
     const int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
-    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    if (i > nx + pml) return;
+    const int l = int(threadIdx.x);
+    const int i = -pml + 1 + tx * W - R + l;
     const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
-    if (j0 > j1) return;  // padding tile beyond the shot's rows
-    const size_t sz = S.px;
-    size_t q = atf(S, lo, i, j0);
+    if (j0 > j1) return;  // padding tile beyond the shot's rows (whole wavefront)
+    const bool mcol = i <= nx + pml + R;                 // inside the padded grid
+    const bool own = l >= R && l <= 63 - R && i <= nx + pml;
+    const unsigned szb = unsigned(S.px) * 4u;
+    unsigned q = unsigned(atf(S, lo, mcol ? i : lo, j0)) * 4u;  // byte offset
     // queues over rows j-R .. j+R
     float qxh[2 * R + 1], qzh[2 * R + 1], qxi[2 * R + 1], qzi[2 * R + 1];  // vx_hi vz_hi vx_ih vz_ih
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
-        const size_t r = q + (m - R) * sz;
-        qxh[m] = F.vx_hi[r];
-        qzh[m] = F.vz_hi[r];
-        qxi[m] = F.vx_ih[r];
-        qzi[m] = F.vz_ih[r];
+        const unsigned r = q + unsigned(m - R) * szb;
+        qxh[m] = mcol ? ldo(F.vx_hi, r) : 0.0f;
+        qzh[m] = mcol ? ldo(F.vz_hi, r) : 0.0f;
+        qxi[m] = mcol ? ldo(F.vx_ih, r) : 0.0f;
+        qzi[m] = mcol ? ldo(F.vz_ih, r) : 0.0f;
     }
     const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
-    for (int j = j0; j <= j1; ++j, q += sz) {
+    for (int j = j0; j <= j1; ++j, q += szb) {
         {
-            const size_t r = q + R * sz;
-            qxh[2 * R] = F.vx_hi[r];
-            qzh[2 * R] = F.vz_hi[r];
-            qxi[2 * R] = F.vx_ih[r];
-            qzi[2 * R] = F.vz_ih[r];
+            const unsigned r = q + unsigned(R) * szb;
+            qxh[2 * R] = mcol ? ldo(F.vx_hi, r) : 0.0f;
+            qzh[2 * R] = mcol ? ldo(F.vz_hi, r) : 0.0f;
+            qxi[2 * R] = mcol ? ldo(F.vx_ih, r) : 0.0f;
+            qzi[2 * R] = mcol ? ldo(F.vz_ih, r) : 0.0f;
         }
-        {   // integer grid
-            float pdxvx = sten_x<R>(F.vx_hi, q, 0, c) / dx;
-            float pdxvz = sten_x<R>(F.vz_hi, q, 0, c) / dx;
-            float pdzvx = sten_q<R>(qxi, 0, c) / dz;
-            float pdzvz = sten_q<R>(qzi, 0, c) / dz;
-            if (xi || j <= 0 || j >= nz + 1) {
-                const float ax = P.axii[q], bx = P.bxii[q], az = P.azii[q], bz = P.bzii[q];
-                pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
-                pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
-                pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
-                pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
+        const float sxh = sten_lanes<R>(qxh[R], l, 0, c), szh = sten_lanes<R>(qzh[R], l, 0, c);
+        const float sxi = sten_lanes<R>(qxi[R], l, -1, c), szi = sten_lanes<R>(qzi[R], l, -1, c);
+        if (own) {
+            {   // integer grid
+                float pdxvx = sxh / dx;
+                float pdxvz = szh / dx;
+                float pdzvx = sten_q<R>(qxi, 0, c) / dz;
+                float pdzvz = sten_q<R>(qzi, 0, c) / dz;
+                if (xi || j <= 0 || j >= nz + 1) {
+                    const float ax = ldo(P.axii, q), bx = ldo(P.bxii, q), az = ldo(P.azii, q), bz = ldo(P.bzii, q);
+                    pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
+                    pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
+                    pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
+                    pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
+                }
+                const float c11 = ldo(M.ci[0], q), c13 = ldo(M.ci[1], q), c15 = ldo(M.ci[2], q);
+                const float c33 = ldo(M.ci[3], q), c35 = ldo(M.ci[4], q), c55 = ldo(M.ci[5], q);
+                const float sh = pdxvz + pdzvx;
+                sto(O.sxx_ii, q, ldo(F.sxx_ii, q) + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
+                sto(O.szz_ii, q, ldo(F.szz_ii, q) + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
+                sto(O.sxz_ii, q, ldo(F.sxz_ii, q) + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
             }
-            const float c11 = M.ci[0][q], c13 = M.ci[1][q], c15 = M.ci[2][q];
-            const float c33 = M.ci[3][q], c35 = M.ci[4][q], c55 = M.ci[5][q];
-            const float sh = pdxvz + pdzvx;
-            O.sxx_ii[q] = F.sxx_ii[q] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh);
-            O.szz_ii[q] = F.szz_ii[q] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh);
-            O.sxz_ii[q] = F.sxz_ii[q] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh);
-        }
-        {   // corner grid
-            float pdxvx = sten_x<R>(F.vx_ih, q, -1, c) / dx;
-            float pdxvz = sten_x<R>(F.vz_ih, q, -1, c) / dx;
-            float pdzvx = sten_q<R>(qxh, -1, c) / dz;
-            float pdzvz = sten_q<R>(qzh, -1, c) / dz;
-            if (xh || j <= 1 || j >= nz + 1) {
-                const float ax = P.axhh[q], bx = P.bxhh[q], az = P.azhh[q], bz = P.bzhh[q];
-                pdxvx = mpml(F.m_dxvx_hh, q, pdxvx, ax, bx);
-                pdxvz = mpml(F.m_dxvz_hh, q, pdxvz, ax, bx);
-                pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
-                pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
+            {   // corner grid
+                float pdxvx = sxi / dx;
+                float pdxvz = szi / dx;
+                float pdzvx = sten_q<R>(qxh, -1, c) / dz;
+                float pdzvz = sten_q<R>(qzh, -1, c) / dz;
+                if (xh || j <= 1 || j >= nz + 1) {
+                    const float ax = ldo(P.axhh, q), bx = ldo(P.bxhh, q), az = ldo(P.azhh, q), bz = ldo(P.bzhh, q);
+                    pdxvx = mpml(F.m_dxvx_hh, q, pdxvx, ax, bx);
+                    pdxvz = mpml(F.m_dxvz_hh, q, pdxvz, ax, bx);
+                    pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
+                    pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
+                }
+                const float c11 = ldo(M.ch[0], q), c13 = ldo(M.ch[1], q), c15 = ldo(M.ch[2], q);
+                const float c33 = ldo(M.ch[3], q), c35 = ldo(M.ch[4], q), c55 = ldo(M.ch[5], q);
+                const float sh = pdxvz + pdzvx;
+                sto(O.sxx_hh, q, ldo(F.sxx_hh, q) + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
+                sto(O.szz_hh, q, ldo(F.szz_hh, q) + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
+                sto(O.sxz_hh, q, ldo(F.sxz_hh, q) + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
             }
-            const float c11 = M.ch[0][q], c13 = M.ch[1][q], c15 = M.ch[2][q];
-            const float c33 = M.ch[3][q], c35 = M.ch[4][q], c55 = M.ch[5][q];
-            const float sh = pdxvz + pdzvx;
-            O.sxx_hh[q] = F.sxx_hh[q] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh);
-            O.szz_hh[q] = F.szz_hh[q] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh);
-            O.sxz_hh[q] = F.sxz_hh[q] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh);
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -226,72 +252,81 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
 // integer-grid stresses one node back and dz of the corner stresses;
 // (i, j - 1/2) from dx of the corner stresses and dz of the integer-grid
 // stresses one node back.  New stresses from O, old velocities from F.
-// dt / rho_h is sgn * |dt| / rho_h, exactly.
+// dt / rho_h is sgn * |dt| / rho_h, exactly.  Tiles as the stress sweep:
+// the x stencils of sxx and sxz come from the lanes by shuffles.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ shots,
                                                       const int* __restrict__ tile_shot, FF F, FV O, FM M, FP P,
                                                       Coef c, float sgn, float dx, float dz, int pml, int kc) {
+    constexpr int W = 64 - 2 * R;
     const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
     const ShotF S = shots[tile_shot[tile]];
     const int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
-    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    if (i > nx + pml) return;
+    const int l = int(threadIdx.x);
+    const int i = -pml + 1 + tx * W - R + l;
     const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
     if (j0 > j1) return;
-    const size_t sz = S.px;
-    size_t q = atf(S, lo, i, j0);
+    const bool mcol = i <= nx + pml + R;
+    const bool own = l >= R && l <= 63 - R && i <= nx + pml;
+    const unsigned szb = unsigned(S.px) * 4u;
+    unsigned q = unsigned(atf(S, lo, mcol ? i : lo, j0)) * 4u;  // byte offset
     float qah[2 * R + 1], qch[2 * R + 1], qai[2 * R + 1], qci[2 * R + 1];  // sxz_hh szz_hh sxz_ii szz_ii
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
-        const size_t r = q + (m - R) * sz;
-        qah[m] = O.sxz_hh[r];
-        qch[m] = O.szz_hh[r];
-        qai[m] = O.sxz_ii[r];
-        qci[m] = O.szz_ii[r];
+        const unsigned r = q + unsigned(m - R) * szb;
+        qah[m] = mcol ? ldo(O.sxz_hh, r) : 0.0f;
+        qch[m] = mcol ? ldo(O.szz_hh, r) : 0.0f;
+        qai[m] = mcol ? ldo(O.sxz_ii, r) : 0.0f;
+        qci[m] = mcol ? ldo(O.szz_ii, r) : 0.0f;
     }
     const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
-    for (int j = j0; j <= j1; ++j, q += sz) {
+    for (int j = j0; j <= j1; ++j, q += szb) {
         {
-            const size_t r = q + R * sz;
-            qah[2 * R] = O.sxz_hh[r];
-            qch[2 * R] = O.szz_hh[r];
-            qai[2 * R] = O.sxz_ii[r];
-            qci[2 * R] = O.szz_ii[r];
+            const unsigned r = q + unsigned(R) * szb;
+            qah[2 * R] = mcol ? ldo(O.sxz_hh, r) : 0.0f;
+            qch[2 * R] = mcol ? ldo(O.szz_hh, r) : 0.0f;
+            qai[2 * R] = mcol ? ldo(O.sxz_ii, r) : 0.0f;
+            qci[2 * R] = mcol ? ldo(O.szz_ii, r) : 0.0f;
         }
-        {   // (i - 1/2, j)
-            float pdxxx = sten_x<R>(O.sxx_ii, q, -1, c) / dx;
-            float pdxxz = sten_x<R>(O.sxz_ii, q, -1, c) / dx;
-            float pdzxz = sten_q<R>(qah, 0, c) / dz;
-            float pdzzz = sten_q<R>(qch, 0, c) / dz;
-            if (xh || j <= 0 || j >= nz + 1) {
-                const float ax = P.axhi[q], bx = P.bxhi[q], az = P.azhi[q], bz = P.bzhi[q];
-                pdxxx = mpml(F.m_dxxx_hi, q, pdxxx, ax, bx);
-                pdxxz = mpml(F.m_dxxz_hi, q, pdxxz, ax, bx);
-                pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
-                pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
+        const float xxi = mcol ? ldo(O.sxx_ii, q) : 0.0f, xxh = mcol ? ldo(O.sxx_hh, q) : 0.0f;
+        const float d_xxi = sten_lanes<R>(xxi, l, -1, c), d_xzi = sten_lanes<R>(qai[R], l, -1, c);
+        const float d_xxh = sten_lanes<R>(xxh, l, 0, c), d_xzh = sten_lanes<R>(qah[R], l, 0, c);
+        if (own) {
+            {   // (i - 1/2, j)
+                float pdxxx = d_xxi / dx;
+                float pdxxz = d_xzi / dx;
+                float pdzxz = sten_q<R>(qah, 0, c) / dz;
+                float pdzzz = sten_q<R>(qch, 0, c) / dz;
+                if (xh || j <= 0 || j >= nz + 1) {
+                    const float ax = ldo(P.axhi, q), bx = ldo(P.bxhi, q), az = ldo(P.azhi, q), bz = ldo(P.bzhi, q);
+                    pdxxx = mpml(F.m_dxxx_hi, q, pdxxx, ax, bx);
+                    pdxxz = mpml(F.m_dxxz_hi, q, pdxxz, ax, bx);
+                    pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
+                    pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
+                }
+                const float w = sgn * ldo(M.qhx, q);
+                sto(O.vx_hi, q, ldo(F.vx_hi, q) + w * (pdxxx + pdzxz));
+                sto(O.vz_hi, q, ldo(F.vz_hi, q) + w * (pdxxz + pdzzz));
             }
-            const float w = sgn * M.qhx[q];
-            O.vx_hi[q] = F.vx_hi[q] + w * (pdxxx + pdzxz);
-            O.vz_hi[q] = F.vz_hi[q] + w * (pdxxz + pdzzz);
-        }
-        {   // (i, j - 1/2)
-            float pdxxx = sten_x<R>(O.sxx_hh, q, 0, c) / dx;
-            float pdxxz = sten_x<R>(O.sxz_hh, q, 0, c) / dx;
-            float pdzxz = sten_q<R>(qai, -1, c) / dz;
-            float pdzzz = sten_q<R>(qci, -1, c) / dz;
-            if (xi || j <= 1 || j >= nz + 1) {
-                const float ax = P.axih[q], bx = P.bxih[q], az = P.azih[q], bz = P.bzih[q];
-                pdxxx = mpml(F.m_dxxx_ih, q, pdxxx, ax, bx);
-                pdxxz = mpml(F.m_dxxz_ih, q, pdxxz, ax, bx);
-                pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
-                pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
+            {   // (i, j - 1/2)
+                float pdxxx = d_xxh / dx;
+                float pdxxz = d_xzh / dx;
+                float pdzxz = sten_q<R>(qai, -1, c) / dz;
+                float pdzzz = sten_q<R>(qci, -1, c) / dz;
+                if (xi || j <= 1 || j >= nz + 1) {
+                    const float ax = ldo(P.axih, q), bx = ldo(P.bxih, q), az = ldo(P.azih, q), bz = ldo(P.bzih, q);
+                    pdxxx = mpml(F.m_dxxx_ih, q, pdxxx, ax, bx);
+                    pdxxz = mpml(F.m_dxxz_ih, q, pdxxz, ax, bx);
+                    pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
+                    pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
+                }
+                const float w = sgn * ldo(M.qhz, q);
+                sto(O.vx_ih, q, ldo(F.vx_ih, q) + w * (pdxxx + pdzxz));
+                sto(O.vz_ih, q, ldo(F.vz_ih, q) + w * (pdxxz + pdzzz));
             }
-            const float w = sgn * M.qhz[q];
-            O.vx_ih[q] = F.vx_ih[q] + w * (pdxxx + pdzxz);
-            O.vz_ih[q] = F.vz_ih[q] + w * (pdxxz + pdzzz);
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -628,7 +663,8 @@ void retilef(owl_fsg2d* b) {
     b->tile_shot.clear();
     for (size_t s = 0; s < b->sd.size(); ++s) {
         ShotF& S = b->sd[s];
-        S.ntx = (S.nx + 2 * b->pml + 63) / 64;
+        const int W = 64 - 2 * b->R;  // columns a tile updates (the sweeps' halo lanes feed shuffles)
+        S.ntx = (S.nx + 2 * b->pml + W - 1) / W;
         S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
         S.tile0 = int(b->tile_shot.size());
         b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nzc, int(s));
@@ -846,6 +882,8 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
         b->max_nr = std::max(b->max_nr, S.nr);
     }
     b->field_elems = size_t(off);
+    if (b->field_elems >= (size_t(1) << 29))
+        fail(OWL_ERR_ARG, "the shots' padded grids exceed 2 GiB per field; split the batch");
     b->trace_elems = size_t(trace);
     b->strip_elems = size_t(strip);
     b->grad_elems = size_t(grad);
