@@ -244,6 +244,21 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
 // tile may already have written); arithmetic as k_update_p / k_update_v,
 // so the results are bit-identical to the two-sweep form.
 // ---------------------------------------------------------------------------
+// Load and store at a 32-bit byte offset from a kernel-argument base: the
+// batch's buffers stay below 2 GiB (checked on the host), so the fused
+// kernels address every array with one offset register against scalar
+// bases instead of a 64-bit address pair per array.
+__device__ __forceinline__ float ldo(const float* base, unsigned off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void sto(float* base, unsigned off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
+}
+__device__ __forceinline__ void addo(double* base, unsigned off8, double v) {
+    double* p = reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off8);
+    *p += v;
+}
+
 template <int R>
 __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ shots,
                                                     const int* __restrict__ ftile_shot, int nftile,
@@ -284,8 +299,9 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
         axh_ = axh[col + pml - 1];
         bxh_ = bxh[col + pml - 1];
     }
-    const size_t pitch = size_t(S.pitch);
-    const size_t base = size_t(S.off) + size_t(col - lo);  // (col, lo)
+    const unsigned pitchb = unsigned(S.pitch) * 4u;
+    const unsigned base = unsigned(S.off + (col - lo)) * 4u;  // byte offset of (col, lo)
+    const unsigned mdb = unsigned(S.mdel) * 4u;               // model minus field, bytes
     // Inputs of one loop iteration: the p row r and the v row j = r - R + 1.
     struct Row {
         float vx[2 * R];  // vx_in at row r, columns col - R + 1 .. col + R
@@ -296,29 +312,29 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
     const int rs = z0 - R, re = z1 + R - 1;
     auto load = [&](int r, Row& w) {
         const int vr = r + R;
-        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? F.vz_in[base + size_t(vr - lo) * pitch] : 0.0f;
+        w.vzf = (mcol && vr >= lo && vr <= nz + pml + R) ? ldo(F.vz_in, base + unsigned(vr - lo) * pitchb) : 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const size_t q = base + size_t(r - lo) * pitch;
+            const unsigned q = base + unsigned(r - lo) * pitchb;
 #pragma unroll
-            for (int k = 0; k < 2 * R; ++k) w.vx[k] = F.vx_in[q - (R - 1) + k];
-            w.p = F.p_in[q];
-            w.bk = bk[mq(S, q)];
-            w.m0 = xs_i ? F.m_in[0][q] : 0.0f;
-            w.m1 = (r <= 0 || r >= nz + 1) ? F.m_in[1][q] : 0.0f;
+            for (int k = 0; k < 2 * R; ++k) w.vx[k] = ldo(F.vx_in, q - unsigned(R - 1) * 4u + unsigned(k) * 4u);
+            w.p = ldo(F.p_in, q);
+            w.bk = ldo(bk, q + mdb);
+            w.m0 = xs_i ? ldo(F.m_in[0], q) : 0.0f;
+            w.m1 = (r <= 0 || r >= nz + 1) ? ldo(F.m_in[1], q) : 0.0f;
         }
         const int j = r - R + 1;
         if (own && j >= z0) {
-            const size_t q = base + size_t(j - lo) * pitch;
-            w.rq = rho[mq(S, q)];
-            w.rl = rho[mq(S, q) - 1];
-            w.ru = rho[mq(S, q) - pitch];
-            w.vxj = F.vx_in[q];
-            w.m2 = xs_h ? F.m_in[2][q] : 0.0f;
-            w.m3 = (j <= 1 || j >= nz + 1) ? F.m_in[3][q] : 0.0f;
+            const unsigned q = base + unsigned(j - lo) * pitchb;
+            w.rq = ldo(rho, q + mdb);
+            w.rl = ldo(rho, q + mdb - 4u);
+            w.ru = ldo(rho, q + mdb - pitchb);
+            w.vxj = ldo(F.vx_in, q);
+            w.m2 = xs_h ? ldo(F.m_in[2], q) : 0.0f;
+            w.m3 = (j <= 1 || j >= nz + 1) ? ldo(F.m_in[3], q) : 0.0f;
         }
     };
     auto vz_row = [&](int row) -> float {
-        return (mcol && row >= lo && row <= nz + pml + R) ? F.vz_in[base + size_t(row - lo) * pitch] : 0.0f;
+        return (mcol && row >= lo && row <= nz + pml + R) ? ldo(F.vz_in, base + unsigned(row - lo) * pitchb) : 0.0f;
     };
     // V[k] = vz_in at row r - R + 1 + k;  P[k] = new p at row r - 2R + 1 + k
     float V[2 * R], P[2 * R];
@@ -334,7 +350,7 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
         // ---- p at (col, r)
         float pn = 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const size_t q = base + size_t(r - lo) * pitch;
+            const unsigned q = base + unsigned(r - lo) * pitchb;
             float sx = c.c[0] * (cur.vx[R] - cur.vx[R - 1]);
             float sz = c.c[0] * (V[R] - V[R - 1]);
 #pragma unroll
@@ -347,16 +363,16 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
             const bool wr = own && r >= z0 && r <= z1;
             if (xs_i) {
                 const float mem = ax * pdx + bx * cur.m0;
-                if (wr) F.m_out[0][q] = mem;
+                if (wr) sto(F.m_out[0], q, mem);
                 pdx = pdx + mem;
             }
             if (r <= 0 || r >= nz + 1) {
                 const float mem = azi[r + pml - 1] * pdz + bzi[r + pml - 1] * cur.m1;
-                if (wr) F.m_out[1][q] = mem;
+                if (wr) sto(F.m_out[1], q, mem);
                 pdz = pdz + mem;
             }
             pn = cur.p - dt * cur.bk * (pdx + pdz);
-            if (wr) F.p_out[q] = pn;
+            if (wr) sto(F.p_out, q, pn);
         }
 #pragma unroll
         for (int k = 0; k < 2 * R - 1; ++k) P[k] = P[k + 1];
@@ -368,29 +384,29 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
 #pragma unroll
             for (int k = 0; k < 2 * R; ++k) px[k] = __shfl(P[R], l - R + k, 64);
             if (own) {
-                const size_t q = base + size_t(j - lo) * pitch;
+                const unsigned q = base + unsigned(j - lo) * pitchb;
                 float s = c.c[0] * (px[R] - px[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (px[R + m - 1] - px[R - m]);
                 float d = idx * s;
                 if (xs_h) {
                     const float mem = axh_ * d + bxh_ * cur.m2;
-                    F.m_out[2][q] = mem;
+                    sto(F.m_out[2], q, mem);
                     d = d + mem;
                 }
                 float reff = 0.5f * (cur.rq + cur.rl);
-                F.vx_out[q] = cur.vxj - dt / reff * d;
+                sto(F.vx_out, q, cur.vxj - dt / reff * d);
                 s = c.c[0] * (P[R] - P[R - 1]);
 #pragma unroll
                 for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (P[R + m - 1] - P[R - m]);
                 d = idz * s;
                 if (j <= 1 || j >= nz + 1) {
                     const float mem = azh[j + pml - 1] * d + bzh[j + pml - 1] * cur.m3;
-                    F.m_out[3][q] = mem;
+                    sto(F.m_out[3], q, mem);
                     d = d + mem;
                 }
                 reff = 0.5f * (cur.rq + cur.ru);
-                F.vz_out[q] = V[0] - dt / reff * d;
+                sto(F.vz_out, q, V[0] - dt / reff * d);
             }
         }
 #pragma unroll
@@ -860,8 +876,9 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
         axh_ = axh[col + pml - 1];
         bxh_ = bxh[col + pml - 1];
     }
-    const size_t pitch = size_t(S.pitch);
-    const size_t base = size_t(S.off) + size_t(col - lo);
+    const unsigned pitchb = unsigned(S.pitch) * 4u;
+    const unsigned base = unsigned(S.off + (col - lo)) * 4u;  // byte offset of (col, lo)
+    const unsigned mdb = unsigned(S.mdel) * 4u;
     struct Row {
         float ex[2 * R];  // ex at row r, columns col - R + 1 .. col + R
         float ezf;        // ez at row r + R
@@ -871,38 +888,38 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
     const int rs = z0 - R, re = z1 + R - 1;
     auto load = [&](int r, Row& w) {
         const int er = r + R;
-        w.ezf = (mcol && er >= lo && er <= nz + pml + R) ? io.ez[base + size_t(er - lo) * pitch] : 0.0f;
+        w.ezf = (mcol && er >= lo && er <= nz + pml + R) ? ldo(io.ez, base + unsigned(er - lo) * pitchb) : 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const size_t q = base + size_t(r - lo) * pitch;
+            const unsigned q = base + unsigned(r - lo) * pitchb;
 #pragma unroll
-            for (int k = 0; k < 2 * R; ++k) w.ex[k] = io.ex[q - (R - 1) + k];
-            w.ph = io.ph[q];
-            w.psx = xs_i ? io.psx[q] : 0.0f;
-            w.psz = (r <= 0 || r >= nz + 1) ? io.psz[q] : 0.0f;
-            w.k = bk[mq(S, q)];
+            for (int k = 0; k < 2 * R; ++k) w.ex[k] = ldo(io.ex, q - unsigned(R - 1) * 4u + unsigned(k) * 4u);
+            w.ph = ldo(io.ph, q);
+            w.psx = xs_i ? ldo(io.psx, q) : 0.0f;
+            w.psz = (r <= 0 || r >= nz + 1) ? ldo(io.psz, q) : 0.0f;
+            w.k = ldo(bk, q + mdb);
             if (own && r >= z0 && r <= z1) {  // the K term is imaged on the tile's own cells only
-                w.pc = p_cur[q];
-                w.pp = p_prev[q];
+                w.pc = ldo(p_cur, q);
+                w.pp = ldo(p_prev, q);
             }
         }
         const int j = r - R + 1;
         if (own && j >= z0) {
-            const size_t q = base + size_t(j - lo) * pitch;
-            w.rq = rho[mq(S, q)];
-            w.rl = rho[mq(S, q) - 1];
-            w.ru = rho[mq(S, q) - pitch];
-            w.vxh = A.vxh[q];
-            w.vzh = A.vzh[q];
-            w.phx = xs_h ? A.phx[q] : 0.0f;
-            w.phz = (j <= 1 || j >= nz + 1) ? A.phz[q] : 0.0f;
-            w.xc = vx_cur[q];
-            w.xp = vx_prev[q];
-            w.zc_ = vz_cur[q];
-            w.zp = vz_prev[q];
+            const unsigned q = base + unsigned(j - lo) * pitchb;
+            w.rq = ldo(rho, q + mdb);
+            w.rl = ldo(rho, q + mdb - 4u);
+            w.ru = ldo(rho, q + mdb - pitchb);
+            w.vxh = ldo(A.vxh, q);
+            w.vzh = ldo(A.vzh, q);
+            w.phx = xs_h ? ldo(A.phx, q) : 0.0f;
+            w.phz = (j <= 1 || j >= nz + 1) ? ldo(A.phz, q) : 0.0f;
+            w.xc = ldo(vx_cur, q);
+            w.xp = ldo(vx_prev, q);
+            w.zc_ = ldo(vz_cur, q);
+            w.zp = ldo(vz_prev, q);
         }
     };
     auto ez_row = [&](int row) -> float {
-        return (mcol && row >= lo && row <= nz + pml + R) ? io.ez[base + size_t(row - lo) * pitch] : 0.0f;
+        return (mcol && row >= lo && row <= nz + pml + R) ? ldo(io.ez, base + unsigned(row - lo) * pitchb) : 0.0f;
     };
     // E[k] = ez at row r - R + 1 + k;  D[k] = this step's dz at row r - 2R + 1 + k
     float E[2 * R], D[2 * R];
@@ -921,7 +938,7 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
         // ---- p-pass at (col, r)
         float dxv = 0.0f, dzv = 0.0f;
         if (pcol && r >= -pml + 1 && r <= nz + pml) {
-            const size_t q = base + size_t(r - lo) * pitch;
+            const unsigned q = base + unsigned(r - lo) * pitchb;
             float sx = 0.0f, sz = 0.0f;
 #pragma unroll
             for (int m = 1; m <= R; ++m) {
@@ -931,8 +948,8 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
             const float ph = cur.ph - (idx * sx + idz * sz);
             const bool wr = own && r >= z0 && r <= z1;
             if (wr) {
-                io.pho[q] = ph;
-                A.gK[q] += double((cur.pc - cur.pp) / cur.k * ph);
+                sto(io.pho, q, ph);
+                addo(A.gK, 2u * q, double((cur.pc - cur.pp) / cur.k * ph));
             }
             const float qv = -dt * cur.k * ph;
             dxv = qv;
@@ -940,12 +957,12 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
             if (xs_i) {
                 const float sv = cur.psx + qv;
                 dxv = qv + ax * sv;
-                if (wr) io.psxo[q] = bx * sv;
+                if (wr) sto(io.psxo, q, bx * sv);
             }
             if (r <= 0 || r >= nz + 1) {
                 const float sv = cur.psz + qv;
                 dzv = qv + azi[r + pml - 1] * sv;
-                if (wr) io.pszo[q] = bzi[r + pml - 1] * sv;
+                if (wr) sto(io.pszo, q, bzi[r + pml - 1] * sv);
             }
         }
 #pragma unroll
@@ -961,12 +978,12 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
 #pragma unroll
             for (int k = 0; k < 2 * R; ++k) xs[k] = __shfl(dxq[0], l - R + k, 64);
             if (own) {
-                const size_t q = base + size_t(j - lo) * pitch;
+                const unsigned q = base + unsigned(j - lo) * pitchb;
                 const float rx = 0.5f * (cur.rq + cur.rl);
                 const float rz = 0.5f * (cur.rq + cur.ru);
                 float vxh = cur.vxh, vzh = cur.vzh;
-                A.gX[q] += double((cur.xp - cur.xc) / rx * vxh);
-                A.gZ[q] += double((cur.zp - cur.zc_) / rz * vzh);
+                addo(A.gX, 2u * q, double((cur.xp - cur.xc) / rx * vxh));
+                addo(A.gZ, 2u * q, double((cur.zp - cur.zc_) / rz * vzh));
                 float sx = 0.0f, sz = 0.0f;
 #pragma unroll
                 for (int m = 1; m <= R; ++m) {
@@ -975,24 +992,24 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
                 }
                 vxh = vxh - idx * sx;
                 vzh = vzh - idz * sz;
-                A.vxh[q] = vxh;
-                A.vzh[q] = vzh;
+                sto(A.vxh, q, vxh);
+                sto(A.vzh, q, vzh);
                 float rr = -(dt / rx) * vxh;
                 float e = rr;
                 if (xs_h) {
                     const float sv = cur.phx + rr;
                     e = rr + axh_ * sv;
-                    A.phx[q] = bxh_ * sv;
+                    sto(A.phx, q, bxh_ * sv);
                 }
-                io.exo[q] = e;
+                sto(io.exo, q, e);
                 rr = -(dt / rz) * vzh;
                 e = rr;
                 if (j <= 1 || j >= nz + 1) {
                     const float sv = cur.phz + rr;
                     e = rr + azh[j + pml - 1] * sv;
-                    A.phz[q] = bzh[j + pml - 1] * sv;
+                    sto(A.phz, q, bzh[j + pml - 1] * sv);
                 }
-                io.ezo[q] = e;
+                sto(io.ezo, q, e);
             }
         }
 #pragma unroll
@@ -1884,15 +1901,21 @@ double fused_bytes(const owl_acoustic2d* b) {
     return by + 8.0 * b->pts_model;  // bk and rho once per distinct window
 }
 
+// The fused kernels address fields and model with 32-bit byte offsets
+// (FP64 accumulators with twice the offset).
+bool offsets32(const owl_acoustic2d* b) {
+    return b->field_elems < (size_t(1) << 28) && b->model_elems < (size_t(1) << 28);
+}
+
 // The fused adjoint step (k_adj_fused) runs wherever the fused forward step
 // may: batches of 4 shots or more by default, no free surface.
 bool adj_fused(const owl_acoustic2d* b) {
-    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
+    if (b->step_kernel == 0 || b->cfg.free_surface || !offsets32(b)) return false;
     return !(b->step_kernel == 2 && b->sd.size() < 4);
 }
 
 bool use_fused(const owl_acoustic2d* b, const W7& in, const W7& out) {
-    if (b->step_kernel == 0 || b->cfg.free_surface) return false;
+    if (b->step_kernel == 0 || b->cfg.free_surface || !offsets32(b)) return false;
     if (b->step_kernel == 2 && b->sd.size() < 4) return false;
     for (int k = 0; k < 7; ++k)
         for (int m = 0; m < 7; ++m)
