@@ -160,6 +160,34 @@ def bench_3d(sync, steps=1500, shape=None):
             "kernels": kernel_table(stats, 1)}
 
 
+def bench_elastic3d(sync, steps):
+    """3D isotropic elastic batched forward: 200 x 200 x 150, 10 m, R = 4,
+    dt 0.8 ms, 20-cell CFS-MPML on five faces and a flat free surface,
+    vp a smooth random field in 2000..5000 m/s, vs = vp/1.73, Gardner rho,
+    8 vertical-force shots, 50 x 50 three-component receivers."""
+    from owl_amd import Elastic3DBatch
+    from owl_amd.synthetic import elastic3d_model, elastic3d_shots
+    vp, vs, rho = elastic3d_model()
+    shots = elastic3d_shots()
+    dt = 0.0008
+    b = Elastic3DBatch(200, 200, 150, 10.0, 10.0, 10.0, vp, vs, rho, shots, dt, (steps - 1) * dt, pml=20,
+                       fdhalf=4, free_surface=True)
+    b.set_profiling(True)
+    sync()
+    t0 = time.perf_counter()
+    b.forward(want_traces=False)
+    sync()
+    wall = time.perf_counter() - t0
+    stats = b.kernel_stats()
+    ms, updates = b.last_stats()
+    b.close()
+    _, roof = roofline_of(stats)
+    return {"workload": f"3D isotropic elastic batched forward, 200x200x150, 10 m, dt 0.8 ms, R 4, pml 20 on five "
+                        f"faces, flat free surface, 8 shots, 50x50 3-C receivers, {steps} steps",
+            "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms, "roofline": roof,
+            "kernels": kernel_table(stats, 1)}
+
+
 def bench_fsg(sync, steps):
     """2D tilted-TI elastic on the Lebedev fully staggered grid, batched
     forward: 801 x 401, 5 m, R = 4, dt 0.4 ms, 20-cell CFS-MPML with the
@@ -211,6 +239,9 @@ def main():
     ap.add_argument("--shape-3d", default="", metavar="PLANES,VARIANT",
                     help="with --only-3d: this launch shape instead of the autotuner's")
     ap.add_argument("--no-fsg", action="store_true", help="skip the 2D FSG forward")
+    ap.add_argument("--no-elastic3d", action="store_true", help="skip the 3D elastic forward")
+    ap.add_argument("--only-elastic3d", action="store_true", help="only the 3D elastic forward (for profiler runs)")
+    ap.add_argument("--elastic3d-steps", type=int, default=2500, help="time steps of the 3D elastic forward")
     ap.add_argument("--only-fsg", action="store_true", help="only the 2D FSG forward (for profiler runs)")
     ap.add_argument("--fsg-steps", type=int, default=5000, help="time steps of the 2D FSG forward")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -242,6 +273,9 @@ def main():
     if args.only_3d:
         shape = tuple(int(v) for v in args.shape_3d.split(",")) if args.shape_3d else None
         print(json.dumps({"acoustic3d_forward": bench_3d(sync, args.steps_3d, shape)}))
+        return
+    if args.only_elastic3d:
+        print(json.dumps({"elastic3d_forward": bench_elastic3d(sync, args.elastic3d_steps)}))
         return
     if args.only_fsg:
         print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps)}))
@@ -312,6 +346,11 @@ def main():
         log("3D batched forward")
         three_d = bench_3d(sync)
 
+    el3 = None
+    if not args.no_elastic3d:
+        log("3D elastic batched forward")
+        el3 = bench_elastic3d(sync, args.elastic3d_steps)
+
     fsg = None
     if not args.no_fsg:
         log("2D FSG batched forward")
@@ -355,6 +394,7 @@ def main():
         "timing": "value and ms_per_step: graph-replayed gradients; roofline and kernels: HIP events "
                   "of one eager gradient run right after the timed region",
         "acoustic3d_forward": three_d,
+        "elastic3d_forward": el3,
         "fsg2d_forward": fsg,
         "cpu_baseline": cpu,
     }

@@ -96,6 +96,50 @@ __device__ __forceinline__ float damp(float* w, size_t q, float v, float d, floa
     return (v + m) / 1.0f;
 }
 
+// Load and store at a 32-bit byte offset from a kernel-argument base (the
+// sweeps' fields stay below 2 GiB, checked on the host): every array of a
+// sweep is addressed by a few shared offset registers against scalar bases.
+__device__ __forceinline__ float ldo(const float* base, unsigned off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void sto(float* base, unsigned off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
+}
+__device__ __forceinline__ float damp_o(float* w, unsigned qb, float v, float d, float alpha, float adt) {
+    if (d == 0.0f) return v;
+    const float a_ = d / 1.0f;
+    const float b_ = d / 1.0f + alpha;
+    const float ca = -(2.0f * adt * a_ / (2.0f + adt * b_));
+    const float cb = (2.0f - adt * b_) / (2.0f + adt * b_);
+    const float m = ca * v + cb * ldo(w, qb);
+    sto(w, qb, m);
+    return (v + m) / 1.0f;
+}
+
+// x stencil sum_m c_m (w(i+off+m) - w(i+off-m+1)) from the lanes' values of
+// the current plane: lane l holds column i and reads columns
+// i - R + off + 1 .. i + R + off from lanes l - R + off + 1 .. l + R + off.
+template <int R>
+__device__ __forceinline__ float sten_lanes(float v, int l, int off, const Coef& c) {
+    float xv[2 * R];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) xv[k] = __shfl(v, l - R + off + 1 + k, 64);
+    float s = c.c[0] * (xv[R] - xv[R - 1]);
+#pragma unroll
+    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (xv[R - 1 + m] - xv[R - m]);
+    return s;
+}
+
+// y stencil sum_m c_m (w(j+off+m) - w(j+off-m+1)) by loads at byte offsets
+// yo[k] = q + (k - R) rows, k = 0 .. 2R.
+template <int R>
+__device__ __forceinline__ float sten_y(const float* w, const unsigned* yo, int off, const Coef& c) {
+    float s = c.c[0] * (ldo(w, yo[R + off + 1]) - ldo(w, yo[R + off]));
+#pragma unroll
+    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (ldo(w, yo[R + off + m]) - ldo(w, yo[R + off + 1 - m]));
+    return s;
+}
+
 struct Prof {
     const float *dix, *dhx, *aix, *ahx, *diy, *dhy, *aiy, *ahy, *diz, *dhz, *aiz, *ahz;
     int pml;
@@ -124,6 +168,7 @@ template <int R>
 __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots, const int* __restrict__ tile_shot,
                                                  EF F, EV O, EM M, const float* __restrict__ prof, Coef c, float dt,
                                                  float idx, float idy, float idz, int pml, int kc, int fs) {
+    constexpr int W = 64 - 2 * R;  // columns a wavefront updates; the other lanes feed its x stencils
     const int tile = blockIdx.x;
     const ShotE S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
@@ -132,108 +177,106 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
     const int ty = lt % S.nty, tzc = lt / S.nty;
     const int nx = S.nx, ny = S.ny, nz = S.nz;
     const int lo = 1 - pml - R;
-    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    const int l = int(threadIdx.x);
+    const int i = -pml + 1 + tx * W - R + l;
     const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
-    if (i > nx + pml || j > ny + pml) return;
+    if (j > ny + pml) return;  // whole wavefront
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 1) : kt;
     if (k0 > k1) return;
+    const bool mcol = i <= nx + pml + R;
+    const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const Prof P = prof_of(prof, S, pml);
     const float adt = fabsf(dt);
     const float* __restrict__ vx = F.vx;
     const float* __restrict__ vy = F.vy;
     const float* __restrict__ vz = F.vz;
-    const size_t sx = 1, sy = S.px, sz = S.pxy;
+    const unsigned syb = unsigned(S.px) * 4u, szb = unsigned(S.pxy) * 4u;
     const int ii = i + pml - 1, jj = j + pml - 1;
-    size_t q = ate(S, lo, i, j, k0);
+    unsigned q = unsigned(ate(S, lo, mcol ? i : lo, j, k0)) * 4u;
     // queues: plane k + m - R at slot m, m = 0 .. 2R
     float qx[2 * R + 1], qy[2 * R + 1], qz[2 * R + 1];
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
-        qx[m] = vx[q + (m - R) * sz];
-        qy[m] = vy[q + (m - R) * sz];
-        qz[m] = vz[q + (m - R) * sz];
+        const unsigned r = q + unsigned(m - R) * szb;
+        qx[m] = mcol ? ldo(vx, r) : 0.0f;
+        qy[m] = mcol ? ldo(vy, r) : 0.0f;
+        qz[m] = mcol ? ldo(vz, r) : 0.0f;
     }
     const bool xin = !(i <= 0 || i >= nx + 1), yin = !(j <= 0 || j >= ny + 1);
     const bool Xin = !(i <= 1 || i >= nx + 1), Yin = !(j <= 1 || j >= ny + 1);
-    for (int k = k0; k <= k1; ++k, q += sz) {
-        qx[2 * R] = vx[q + R * sz];
-        qy[2 * R] = vy[q + R * sz];
-        qz[2 * R] = vz[q + R * sz];
-        const int kk = k + pml - 1;
-        // ---- normal stresses
+    for (int k = k0; k <= k1; ++k, q += szb) {
         {
-            float a = c.c[0] * (vx[q + sx] - vx[q]);
-            float b = c.c[0] * (vy[q + sy] - vy[q]);
-#pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (vx[q + m * sx] - vx[q - (m - 1) * sx]);
-                b = b + c.c[m - 1] * (vy[q + m * sy] - vy[q - (m - 1) * sy]);
-            }
-            float pdxvx = idx * a, pdyvy = idy * b, pdzvz;
-            const float c11 = M.c11[q], c12 = M.c12[q];
-            if (fs && k == 1) {
-                if (!xin || !yin) {
-                    pdxvx = damp(F.m_dxvx, q, pdxvx, P.dix[ii], P.aix[ii], adt);
-                    pdyvy = damp(F.m_dyvy, q, pdyvy, P.diy[jj], P.aiy[jj], adt);
-                }
-                pdzvz = -(c12 / c11 * pdxvx) - c12 / c11 * pdyvy;
-            } else {
-                float s = c.c[0] * (qz[R + 1] - qz[R]);
-#pragma unroll
-                for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (qz[R + m] - qz[R - m + 1]);
-                pdzvz = idz * s;
-                if (!xin || !yin || (!fs && k <= 0) || k >= nz + 1) {
-                    pdxvx = damp(F.m_dxvx, q, pdxvx, P.dix[ii], P.aix[ii], adt);
-                    pdyvy = damp(F.m_dyvy, q, pdyvy, P.diy[jj], P.aiy[jj], adt);
-                    pdzvz = damp(F.m_dzvz, q, pdzvz, P.diz[kk], P.aiz[kk], adt);
-                }
-            }
-            O.sxx[q] = F.sxx[q] + dt * (c11 * pdxvx + c12 * pdyvy + c12 * pdzvz);
-            O.syy[q] = F.syy[q] + dt * (c12 * pdxvx + c11 * pdyvy + c12 * pdzvz);
-            O.szz[q] = F.szz[q] + dt * (c12 * pdxvx + c12 * pdyvy + c11 * pdzvz);
+            const unsigned r = q + unsigned(R) * szb;
+            qx[2 * R] = mcol ? ldo(vx, r) : 0.0f;
+            qy[2 * R] = mcol ? ldo(vy, r) : 0.0f;
+            qz[2 * R] = mcol ? ldo(vz, r) : 0.0f;
         }
-        // ---- shear stresses with a z derivative: syz, sxz (Z = k)
-        if (!fs || k >= 2) {
-            float a = c.c[0] * (qy[R] - qy[R - 1]);   // d-z vy
-            float b = c.c[0] * (vz[q] - vz[q - sy]);  // d-y vz
-            float e = c.c[0] * (qx[R] - qx[R - 1]);   // d-z vx
-            float f = c.c[0] * (vz[q] - vz[q - sx]);  // d-x vz
+        // x stencils of this plane from the lanes (all lanes take part)
+        const float sx_vx = sten_lanes<R>(qx[R], l, 0, c);    // d+x vx
+        const float sx_vz = sten_lanes<R>(qz[R], l, -1, c);   // d-x vz
+        const float sx_vy = sten_lanes<R>(qy[R], l, -1, c);   // d-x vy
+        if (own) {
+            const int kk = k + pml - 1;
+            unsigned yo[2 * R + 1];  // rows j - R .. j + R of this plane
 #pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (qy[R + m - 1] - qy[R - m]);
-                b = b + c.c[m - 1] * (vz[q + (m - 1) * sy] - vz[q - m * sy]);
-                e = e + c.c[m - 1] * (qx[R + m - 1] - qx[R - m]);
-                f = f + c.c[m - 1] * (vz[q + (m - 1) * sx] - vz[q - m * sx]);
-            }
-            float pdzvy = idz * a, pdyvz = idy * b, pdzvx = idz * e, pdxvz = idx * f;
-            const bool zpml = (!fs && k <= 1) || k >= nz + 1;
-            if (!xin || !Yin || zpml) {
-                pdyvz = damp(F.m_dyvz, q, pdyvz, P.dhy[jj], P.ahy[jj], adt);
-                pdzvy = damp(F.m_dzvy, q, pdzvy, P.dhz[kk], P.ahz[kk], adt);
-            }
-            O.syz[q] = F.syz[q] + dt * M.mu_yz[q] * (pdzvy + pdyvz);
-            if (!Xin || !yin || zpml) {
-                pdxvz = damp(F.m_dxvz, q, pdxvz, P.dhx[ii], P.ahx[ii], adt);
-                pdzvx = damp(F.m_dzvx, q, pdzvx, P.dhz[kk], P.ahz[kk], adt);
-            }
-            O.sxz[q] = F.sxz[q] + dt * M.mu_xz[q] * (pdzvx + pdxvz);
-        }
-        // ---- sxy (X, Y, k)
-        {
-            float a = c.c[0] * (vx[q] - vx[q - sy]);  // d-y vx
-            float b = c.c[0] * (vy[q] - vy[q - sx]);  // d-x vy
+            for (int m = 0; m <= 2 * R; ++m) yo[m] = q + unsigned(m - R) * syb;
+            // ---- normal stresses
+            {
+                float pdxvx = idx * sx_vx, pdyvy = idy * sten_y<R>(vy, yo, 0, c), pdzvz;
+                const float c11 = ldo(M.c11, q), c12 = ldo(M.c12, q);
+                if (fs && k == 1) {
+                    if (!xin || !yin) {
+                        pdxvx = damp_o(F.m_dxvx, q, pdxvx, P.dix[ii], P.aix[ii], adt);
+                        pdyvy = damp_o(F.m_dyvy, q, pdyvy, P.diy[jj], P.aiy[jj], adt);
+                    }
+                    pdzvz = -(c12 / c11 * pdxvx) - c12 / c11 * pdyvy;
+                } else {
+                    float s = c.c[0] * (qz[R + 1] - qz[R]);
 #pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (vx[q + (m - 1) * sy] - vx[q - m * sy]);
-                b = b + c.c[m - 1] * (vy[q + (m - 1) * sx] - vy[q - m * sx]);
+                    for (int m = 2; m <= R; ++m) s = s + c.c[m - 1] * (qz[R + m] - qz[R - m + 1]);
+                    pdzvz = idz * s;
+                    if (!xin || !yin || (!fs && k <= 0) || k >= nz + 1) {
+                        pdxvx = damp_o(F.m_dxvx, q, pdxvx, P.dix[ii], P.aix[ii], adt);
+                        pdyvy = damp_o(F.m_dyvy, q, pdyvy, P.diy[jj], P.aiy[jj], adt);
+                        pdzvz = damp_o(F.m_dzvz, q, pdzvz, P.diz[kk], P.aiz[kk], adt);
+                    }
+                }
+                sto(O.sxx, q, ldo(F.sxx, q) + dt * (c11 * pdxvx + c12 * pdyvy + c12 * pdzvz));
+                sto(O.syy, q, ldo(F.syy, q) + dt * (c12 * pdxvx + c11 * pdyvy + c12 * pdzvz));
+                sto(O.szz, q, ldo(F.szz, q) + dt * (c12 * pdxvx + c12 * pdyvy + c11 * pdzvz));
             }
-            float pdyvx = idy * a, pdxvy = idx * b;
-            if (!Xin || !Yin || (!fs && (k <= 0 || k >= nz + 1))) {
-                pdxvy = damp(F.m_dxvy, q, pdxvy, P.dhx[ii], P.ahx[ii], adt);
-                pdyvx = damp(F.m_dyvx, q, pdyvx, P.dhy[jj], P.ahy[jj], adt);
+            // ---- shear stresses with a z derivative: syz, sxz (Z = k)
+            if (!fs || k >= 2) {
+                float a = c.c[0] * (qy[R] - qy[R - 1]);   // d-z vy
+                float e = c.c[0] * (qx[R] - qx[R - 1]);   // d-z vx
+#pragma unroll
+                for (int m = 2; m <= R; ++m) {
+                    a = a + c.c[m - 1] * (qy[R + m - 1] - qy[R - m]);
+                    e = e + c.c[m - 1] * (qx[R + m - 1] - qx[R - m]);
+                }
+                float pdzvy = idz * a, pdyvz = idy * sten_y<R>(vz, yo, -1, c), pdzvx = idz * e, pdxvz = idx * sx_vz;
+                const bool zpml = (!fs && k <= 1) || k >= nz + 1;
+                if (!xin || !Yin || zpml) {
+                    pdyvz = damp_o(F.m_dyvz, q, pdyvz, P.dhy[jj], P.ahy[jj], adt);
+                    pdzvy = damp_o(F.m_dzvy, q, pdzvy, P.dhz[kk], P.ahz[kk], adt);
+                }
+                sto(O.syz, q, ldo(F.syz, q) + dt * ldo(M.mu_yz, q) * (pdzvy + pdyvz));
+                if (!Xin || !yin || zpml) {
+                    pdxvz = damp_o(F.m_dxvz, q, pdxvz, P.dhx[ii], P.ahx[ii], adt);
+                    pdzvx = damp_o(F.m_dzvx, q, pdzvx, P.dhz[kk], P.ahz[kk], adt);
+                }
+                sto(O.sxz, q, ldo(F.sxz, q) + dt * ldo(M.mu_xz, q) * (pdzvx + pdxvz));
             }
-            O.sxy[q] = F.sxy[q] + dt * M.mu_xy[q] * (pdyvx + pdxvy);
+            // ---- sxy (X, Y, k)
+            {
+                float pdyvx = idy * sten_y<R>(vx, yo, -1, c), pdxvy = idx * sx_vy;
+                if (!Xin || !Yin || (!fs && (k <= 0 || k >= nz + 1))) {
+                    pdxvy = damp_o(F.m_dxvy, q, pdxvy, P.dhx[ii], P.ahx[ii], adt);
+                    pdyvx = damp_o(F.m_dyvx, q, pdyvx, P.dhy[jj], P.ahy[jj], adt);
+                }
+                sto(O.sxy, q, ldo(F.sxy, q) + dt * ldo(M.mu_xy, q) * (pdyvx + pdxvy));
+            }
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -254,6 +297,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                                                    const int* __restrict__ tile_shot, EF F, EV O, EM M,
                                                    const float* __restrict__ prof, Coef c, float dt, float idx,
                                                    float idy, float idz, int pml, int kc, int fs) {
+    constexpr int W = 64 - 2 * R;
     const int tile = blockIdx.x;
     const ShotE S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
@@ -262,12 +306,15 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
     const int ty = lt % S.nty, tzc = lt / S.nty;
     const int nx = S.nx, ny = S.ny, nz = S.nz;
     const int lo = 1 - pml - R;
-    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
+    const int l = int(threadIdx.x);
+    const int i = -pml + 1 + tx * W - R + l;
     const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
-    if (i > nx + pml || j > ny + pml) return;
+    if (j > ny + pml) return;  // whole wavefront
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 1) : kt;
     if (k0 > k1) return;
+    const bool mcol = i <= nx + pml + R;
+    const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const Prof P = prof_of(prof, S, pml);
     const float adt = fabsf(dt);
     const float* __restrict__ sxx = O.sxx;
@@ -276,81 +323,77 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
     const float* __restrict__ syz = O.syz;
     const float* __restrict__ sxz = O.sxz;
     const float* __restrict__ sxy = O.sxy;
-    const size_t sx = 1, sy = S.px, sz = S.pxy;
+    const unsigned syb = unsigned(S.px) * 4u, szb = unsigned(S.pxy) * 4u;
     const int ii = i + pml - 1, jj = j + pml - 1;
-    size_t q = ate(S, lo, i, j, k0);
+    unsigned q = unsigned(ate(S, lo, mcol ? i : lo, j, k0)) * 4u;
     float qa[2 * R + 1], qb[2 * R + 1], qc[2 * R + 1];  // sxz, syz, szz
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
-        qa[m] = sxz[q + (m - R) * sz];
-        qb[m] = syz[q + (m - R) * sz];
-        qc[m] = szz[q + (m - R) * sz];
+        const unsigned r = q + unsigned(m - R) * szb;
+        qa[m] = mcol ? ldo(sxz, r) : 0.0f;
+        qb[m] = mcol ? ldo(syz, r) : 0.0f;
+        qc[m] = mcol ? ldo(szz, r) : 0.0f;
     }
     const bool xin = !(i <= 0 || i >= nx + 1), yin = !(j <= 0 || j >= ny + 1);
     const bool Xin = !(i <= 1 || i >= nx + 1), Yin = !(j <= 1 || j >= ny + 1);
-    for (int k = k0; k <= k1; ++k, q += sz) {
-        qa[2 * R] = sxz[q + R * sz];
-        qb[2 * R] = syz[q + R * sz];
-        qc[2 * R] = szz[q + R * sz];
-        const int kk = k + pml - 1;
-        const bool zin = !((!fs && k <= 0) || k >= nz + 1);
-        // ---- vx (X, j, k)
+    for (int k = k0; k <= k1; ++k, q += szb) {
         {
-            float a = c.c[0] * (sxx[q] - sxx[q - sx]);       // d-x sxx
-            float b = c.c[0] * (sxy[q + sy] - sxy[q]);       // d+y sxy
-            float e = c.c[0] * (qa[R + 1] - qa[R]);          // d+z sxz
-#pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (sxx[q + (m - 1) * sx] - sxx[q - m * sx]);
-                b = b + c.c[m - 1] * (sxy[q + m * sy] - sxy[q - (m - 1) * sy]);
-                e = e + c.c[m - 1] * (qa[R + m] - qa[R - m + 1]);
-            }
-            float pdxxx = idx * a, pdyxy = idy * b, pdzxz = idz * e;
-            if (!Xin || !yin || !zin) {
-                pdxxx = damp(F.m_dxxx, q, pdxxx, P.dhx[ii], P.ahx[ii], adt);
-                pdyxy = damp(F.m_dyxy, q, pdyxy, P.diy[jj], P.aiy[jj], adt);
-                pdzxz = damp(F.m_dzxz, q, pdzxz, P.diz[kk], P.aiz[kk], adt);
-            }
-            O.vx[q] = F.vx[q] + dt / M.rx[q] * (pdxxx + pdyxy + pdzxz);
+            const unsigned r = q + unsigned(R) * szb;
+            qa[2 * R] = mcol ? ldo(sxz, r) : 0.0f;
+            qb[2 * R] = mcol ? ldo(syz, r) : 0.0f;
+            qc[2 * R] = mcol ? ldo(szz, r) : 0.0f;
         }
-        // ---- vy (i, Y, k)
-        {
-            float a = c.c[0] * (sxy[q + sx] - sxy[q]);       // d+x sxy
-            float b = c.c[0] * (syy[q] - syy[q - sy]);       // d-y syy
-            float e = c.c[0] * (qb[R + 1] - qb[R]);          // d+z syz
+        // x stencils of this plane from the lanes (all lanes take part)
+        const float cxx = mcol ? ldo(sxx, q) : 0.0f, cxy = mcol ? ldo(sxy, q) : 0.0f;
+        const float sx_xx = sten_lanes<R>(cxx, l, -1, c);    // d-x sxx
+        const float sx_xy = sten_lanes<R>(cxy, l, 0, c);     // d+x sxy
+        const float sx_xz = sten_lanes<R>(qa[R], l, 0, c);   // d+x sxz
+        if (own) {
+            const int kk = k + pml - 1;
+            const bool zin = !((!fs && k <= 0) || k >= nz + 1);
+            unsigned yo[2 * R + 1];  // rows j - R .. j + R of this plane
 #pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (sxy[q + m * sx] - sxy[q - (m - 1) * sx]);
-                b = b + c.c[m - 1] * (syy[q + (m - 1) * sy] - syy[q - m * sy]);
-                e = e + c.c[m - 1] * (qb[R + m] - qb[R - m + 1]);
-            }
-            float pdxxy = idx * a, pdyyy = idy * b, pdzyz = idz * e;
-            if (!xin || !Yin || !zin) {
-                pdxxy = damp(F.m_dxxy, q, pdxxy, P.dix[ii], P.aix[ii], adt);
-                pdyyy = damp(F.m_dyyy, q, pdyyy, P.dhy[jj], P.ahy[jj], adt);
-                pdzyz = damp(F.m_dzyz, q, pdzyz, P.diz[kk], P.aiz[kk], adt);
-            }
-            O.vy[q] = F.vy[q] + dt / M.ry[q] * (pdxxy + pdyyy + pdzyz);
-        }
-        // ---- vz (i, j, Z = k)
-        if (!fs || k >= 2) {
-            float a = c.c[0] * (sxz[q + sx] - sxz[q]);       // d+x sxz
-            float b = c.c[0] * (syz[q + sy] - syz[q]);       // d+y syz
-            float e = c.c[0] * (qc[R] - qc[R - 1]);          // d-z szz
+            for (int m = 0; m <= 2 * R; ++m) yo[m] = q + unsigned(m - R) * syb;
+            // ---- vx (X, j, k)
+            {
+                float e = c.c[0] * (qa[R + 1] - qa[R]);          // d+z sxz
 #pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (sxz[q + m * sx] - sxz[q - (m - 1) * sx]);
-                b = b + c.c[m - 1] * (syz[q + m * sy] - syz[q - (m - 1) * sy]);
-                e = e + c.c[m - 1] * (qc[R + m - 1] - qc[R - m]);
+                for (int m = 2; m <= R; ++m) e = e + c.c[m - 1] * (qa[R + m] - qa[R - m + 1]);
+                float pdxxx = idx * sx_xx, pdyxy = idy * sten_y<R>(sxy, yo, 0, c), pdzxz = idz * e;
+                if (!Xin || !yin || !zin) {
+                    pdxxx = damp_o(F.m_dxxx, q, pdxxx, P.dhx[ii], P.ahx[ii], adt);
+                    pdyxy = damp_o(F.m_dyxy, q, pdyxy, P.diy[jj], P.aiy[jj], adt);
+                    pdzxz = damp_o(F.m_dzxz, q, pdzxz, P.diz[kk], P.aiz[kk], adt);
+                }
+                sto(O.vx, q, ldo(F.vx, q) + dt / ldo(M.rx, q) * (pdxxx + pdyxy + pdzxz));
             }
-            float pdxxz = idx * a, pdyyz = idy * b, pdzzz = idz * e;
-            const bool Zin = !((!fs && k <= 1) || k >= nz + 1);
-            if (!xin || !yin || !Zin) {
-                pdxxz = damp(F.m_dxxz, q, pdxxz, P.dix[ii], P.aix[ii], adt);
-                pdyyz = damp(F.m_dyyz, q, pdyyz, P.diy[jj], P.aiy[jj], adt);
-                pdzzz = damp(F.m_dzzz, q, pdzzz, P.dhz[kk], P.ahz[kk], adt);
+            // ---- vy (i, Y, k)
+            {
+                float e = c.c[0] * (qb[R + 1] - qb[R]);          // d+z syz
+#pragma unroll
+                for (int m = 2; m <= R; ++m) e = e + c.c[m - 1] * (qb[R + m] - qb[R - m + 1]);
+                float pdxxy = idx * sx_xy, pdyyy = idy * sten_y<R>(syy, yo, -1, c), pdzyz = idz * e;
+                if (!xin || !Yin || !zin) {
+                    pdxxy = damp_o(F.m_dxxy, q, pdxxy, P.dix[ii], P.aix[ii], adt);
+                    pdyyy = damp_o(F.m_dyyy, q, pdyyy, P.dhy[jj], P.ahy[jj], adt);
+                    pdzyz = damp_o(F.m_dzyz, q, pdzyz, P.diz[kk], P.aiz[kk], adt);
+                }
+                sto(O.vy, q, ldo(F.vy, q) + dt / ldo(M.ry, q) * (pdxxy + pdyyy + pdzyz));
             }
-            O.vz[q] = F.vz[q] + dt / M.rz[q] * (pdxxz + pdyyz + pdzzz);
+            // ---- vz (i, j, Z = k)
+            if (!fs || k >= 2) {
+                float e = c.c[0] * (qc[R] - qc[R - 1]);          // d-z szz
+#pragma unroll
+                for (int m = 2; m <= R; ++m) e = e + c.c[m - 1] * (qc[R + m - 1] - qc[R - m]);
+                float pdxxz = idx * sx_xz, pdyyz = idy * sten_y<R>(syz, yo, 0, c), pdzzz = idz * e;
+                const bool Zin = !((!fs && k <= 1) || k >= nz + 1);
+                if (!xin || !yin || !Zin) {
+                    pdxxz = damp_o(F.m_dxxz, q, pdxxz, P.dix[ii], P.aix[ii], adt);
+                    pdyyz = damp_o(F.m_dyyz, q, pdyyz, P.diy[jj], P.aiy[jj], adt);
+                    pdzzz = damp_o(F.m_dzzz, q, pdzzz, P.dhz[kk], P.ahz[kk], adt);
+                }
+                sto(O.vz, q, ldo(F.vz, q) + dt / ldo(M.rz, q) * (pdxxz + pdyyz + pdzzz));
+            }
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -887,7 +930,7 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
         off += S.pxy * S.nzp;
         off = (off + 63) / 64 * 64;
         S.nty = (H.ny + 2 * pml + b->cy - 1) / b->cy;
-        S.ntx = (H.nx + 2 * pml + 63) / 64;
+        S.ntx = (H.nx + 2 * pml + (64 - 2 * R) - 1) / (64 - 2 * R);  // sweeps update 64 - 2R columns per wavefront
         S.nzc = (H.nz + 2 * pml + b->kc - 1) / b->kc;
         S.tile0 = int(b->tile_shot.size());
         b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nty * S.nzc, s);
@@ -1024,6 +1067,8 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
         b->max_nr = std::max(b->max_nr, S.nr);
     }
     b->field_elems = size_t(off);
+    if (b->field_elems >= (size_t(1) << 29))
+        fail(OWL_ERR_ARG, "the shots' padded grids exceed 2 GiB per field; split the batch");
     b->trace_elems = size_t(trace);
     b->strip_elems = size_t(strip);
     b->grad_elems = size_t(grad);
@@ -1503,7 +1548,7 @@ owl_status owl_elastic3d_set_launch(owl_elastic3d* b, int planes, int rows) {
         for (size_t s = 0; s < b->sd.size(); ++s) {
             ShotE& S = b->sd[s];
             S.nty = (S.ny + 2 * b->pml + b->cy - 1) / b->cy;
-            S.ntx = (S.nx + 2 * b->pml + 63) / 64;
+            S.ntx = (S.nx + 2 * b->pml + (64 - 2 * b->R) - 1) / (64 - 2 * b->R);
             S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
             S.tile0 = lt;
             lt += S.ntx * S.nty * S.nzc;

@@ -184,3 +184,32 @@ def fsg2d_shots(nx=801, dx=5.0, nshot=64, seed=0):
         src = Source(x=float(xs), y=0.0, z=10.0, f0=15.0, mechanism="force", polar=0.0, amp=1e6)
         out.append((Geometry(id=k + 1, srcr=[src], recr=recs), dict(nxbeg=b, nxend=e, nzbeg=0, nzend=0)))
     return out
+
+
+def elastic3d_model(nx=200, ny=200, nz=150, seed=0):
+    """Isotropic elastic model of the 3D elastic workload: vp a smooth random
+    field in 2000..5000 m/s (Gaussian-smoothed N(0,1) with a 6-cell
+    correlation length, rescaled), vs = vp / 1.73, Gardner rho.  Returns
+    (vp, vs, rho), each (nz, ny, nx) float32."""
+    from scipy.ndimage import gaussian_filter
+    g = gaussian_filter(np.random.default_rng(seed).standard_normal((nz, ny, nx)), 6.0, mode="nearest")
+    g = (g - g.min()) / (g.max() - g.min())
+    vp = (2000.0 + 3000.0 * g).astype(np.float32)
+    return vp, (vp / np.float32(1.73)).astype(np.float32), gardner(vp)
+
+
+def elastic3d_shots(nx=200, ny=200, d=10.0, nshot=8, zs=20.0, zr=10.0, nrec_side=50, f0=10.0):
+    """Vertical-force shots on a 4 x 2 surface grid (10 Hz Ricker at zs) with
+    one 50 x 50 grid of three-component receivers at zr spanning the model;
+    full-model windows.  Returns [(Geometry, None)]."""
+    xs = np.linspace(0.2, 0.8, 4) * (nx - 1) * d
+    ys = np.linspace(0.3, 0.7, 2) * (ny - 1) * d
+    rx = np.linspace(0.0, (nx - 1) * d, nrec_side)
+    ry = np.linspace(0.0, (ny - 1) * d, nrec_side)
+    recs = [Receiver(x=float(x), y=float(y), z=zr) for y in ry for x in rx]
+    out = []
+    for k in range(nshot):
+        src = Source(x=float(xs[k % 4]), y=float(ys[k // 4 % 2]), z=zs, f0=f0, mechanism="force", polar=0.0,
+                     amp=1e6)
+        out.append((Geometry(id=k + 1, srcr=[src], recr=list(recs)), None))
+    return out
