@@ -230,6 +230,9 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the batched exact gradient (for profiler runs)")
     ap.add_argument("--eager", action="store_true", help="no HIP graphs (for counter runs)")
+    ap.add_argument("--rows", default="", metavar="FWD,ADJ",
+                    help="fixed z rows of the fused forward and adjoint steps instead of the autotuner's "
+                         "(counter runs without the tuner's trial launches)")
     ap.add_argument("--time-steps", type=int, default=0,
                     help="with --headline-only: that many time steps instead of 2000 (short counter runs "
                          "with the same launches)")
@@ -289,6 +292,10 @@ def main():
     b = make_batch(vp, rho, vp0, rho0, shots,
                    tmax=(args.time_steps - 1) * CFG["dt"] if args.headline_only and args.time_steps > 1 else None)
     b.set_graphs(not args.eager)
+    if args.rows:
+        zf, za = (int(v) for v in args.rows.split(","))
+        b.set_step_kernel("auto", zf)
+        b.set_adjoint_rows(za)
     nt = b.nt
     log("batched exact gradient, binomial 40 snapshots, FP32")
     wall, updates, stats, g32 = time_gradient(b, args.steps, args.warmup, sync, **exact_b40)

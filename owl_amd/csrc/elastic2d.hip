@@ -89,80 +89,109 @@ __device__ __forceinline__ size_t at2(const ShotE2& S, int lo, int i, int j) {
     return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.px);
 }
 
+// Load and store at a 32-bit byte offset from a kernel-argument base (the
+// batch's fields stay below 2 GiB, checked on the host): a sweep addresses
+// every array with a few shared offset registers against scalar bases.
+__device__ __forceinline__ float ldo(const float* base, unsigned off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void sto(float* base, unsigned off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 // Memory-variable update m = a d + b m, d = (d + m) / kappa with kappa = 1
 // (module_wavefield.f90:60-66).
-__device__ __forceinline__ float mpml(float* m, size_t q, float d, float a, float b) {
-    const float v = a * d + b * m[q];
-    m[q] = v;
+__device__ __forceinline__ float mpml(float* m, unsigned qb, float d, float a, float b) {
+    const float v = a * d + b * ldo(m, qb);
+    sto(m, qb, v);
     return d + v;
+}
+
+// x stencil sum_m c_m (w(i+off+m) - w(i+off-m+1)) from the lanes' values of
+// the current row: lane l holds column i and reads columns
+// i - R + off + 1 .. i + R + off from lanes l - R + off + 1 .. l + R + off.
+template <int R>
+__device__ __forceinline__ float sten_lanes(float v, int l, int off, const float* c) {
+    float xv[2 * R];
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) xv[k] = __shfl(v, l - R + off + 1 + k, 64);
+    float s = c[0] * (xv[R] - xv[R - 1]);
+#pragma unroll
+    for (int m = 2; m <= R; ++m) s = s + c[m - 1] * (xv[R - 1 + m] - xv[R - m]);
+    return s;
 }
 
 // ---------------------------------------------------------------------------
 // Stress sweep: sxx, szz at (i, j), sxz at storage (i, j) = position
 // (i - 1/2, j - 1/2) (module_wavefield.f90:52-93).  z stencils from queues
-// over rows j-R .. j+R of vx and vz.
+// over rows j-R .. j+R of vx and vz; x stencils from the other lanes' current
+// rows by shuffles, so a wavefront of 64 lanes updates the middle 64 - 2R
+// columns and the others only feed them.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ shots, const int* __restrict__ tile_shot,
                                                   EF2 F, EV2 O, EM2 M, const float* __restrict__ prof, Coef c, float dt,
                                                   float idx, float idz, int pml, int kc) {
+    constexpr int W = 64 - 2 * R;
     const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
     const ShotE2 S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
-    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    if (i > nx + pml) return;
+    const int l = int(threadIdx.x);
+    const int i = -pml + 1 + tx * W - R + l;
     const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
-    if (j0 > j1) return;  // padding tile beyond the shot's rows
+    if (j0 > j1) return;  // padding tile beyond the shot's rows (whole wavefront)
+    const bool mcol = i <= nx + pml + R;
+    const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const Prof2 P = prof_of(prof, S, pml);
     const float* __restrict__ vx = F.vx;
     const float* __restrict__ vz = F.vz;
-    const size_t sz = S.px;
+    const unsigned szb = unsigned(S.px) * 4u;
     const int ii = i + pml - 1;
-    size_t q = at2(S, lo, i, j0);
+    unsigned q = unsigned(at2(S, lo, mcol ? i : lo, j0)) * 4u;
     float qx[2 * R + 1], qz[2 * R + 1];  // rows j + m - R at slot m
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
-        qx[m] = vx[q + (m - R) * sz];
-        qz[m] = vz[q + (m - R) * sz];
+        const unsigned r = q + unsigned(m - R) * szb;
+        qx[m] = mcol ? ldo(vx, r) : 0.0f;
+        qz[m] = mcol ? ldo(vz, r) : 0.0f;
     }
     const bool xin = !(i <= 0 || i >= nx + 1), Xin = !(i <= 1 || i >= nx + 1);
-    for (int j = j0; j <= j1; ++j, q += sz) {
-        qx[2 * R] = vx[q + R * sz];
-        qz[2 * R] = vz[q + R * sz];
-        const int jj = j + pml - 1;
+    for (int j = j0; j <= j1; ++j, q += szb) {
         {
-            float a = c.c[0] * (vx[q + 1] - vx[q]);
-            float b = c.c[0] * (qz[R + 1] - qz[R]);
-#pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (vx[q + m] - vx[q - (m - 1)]);
-                b = b + c.c[m - 1] * (qz[R + m] - qz[R - m + 1]);
-            }
-            float pdxvx = idx * a, pdzvz = idz * b;
-            if (!xin || j <= 0 || j >= nz + 1) {
-                pdxvx = mpml(F.m_dxvx, q, pdxvx, P.axi[ii], P.bxi[ii]);
-                pdzvz = mpml(F.m_dzvz, q, pdzvz, P.azi[jj], P.bzi[jj]);
-            }
-            const float c11 = M.c11[q], c13 = M.c13[q];
-            O.sxx[q] = F.sxx[q] + dt * (c11 * pdxvx + c13 * pdzvz);
-            O.szz[q] = F.szz[q] + dt * (c13 * pdxvx + c11 * pdzvz);
+            const unsigned r = q + unsigned(R) * szb;
+            qx[2 * R] = mcol ? ldo(vx, r) : 0.0f;
+            qz[2 * R] = mcol ? ldo(vz, r) : 0.0f;
         }
-        {
-            float a = c.c[0] * (qx[R] - qx[R - 1]);        // d-z vx
-            float b = c.c[0] * (vz[q] - vz[q - 1]);        // d-x vz
+        const float sxvx = sten_lanes<R>(qx[R], l, 0, c.c);   // d+x vx
+        const float sxvz = sten_lanes<R>(qz[R], l, -1, c.c);  // d-x vz
+        if (own) {
+            const int jj = j + pml - 1;
+            {
+                float b = c.c[0] * (qz[R + 1] - qz[R]);
 #pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (qx[R + m - 1] - qx[R - m]);
-                b = b + c.c[m - 1] * (vz[q + (m - 1)] - vz[q - m]);
+                for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qz[R + m] - qz[R - m + 1]);
+                float pdxvx = idx * sxvx, pdzvz = idz * b;
+                if (!xin || j <= 0 || j >= nz + 1) {
+                    pdxvx = mpml(F.m_dxvx, q, pdxvx, P.axi[ii], P.bxi[ii]);
+                    pdzvz = mpml(F.m_dzvz, q, pdzvz, P.azi[jj], P.bzi[jj]);
+                }
+                const float c11 = ldo(M.c11, q), c13 = ldo(M.c13, q);
+                sto(O.sxx, q, ldo(F.sxx, q) + dt * (c11 * pdxvx + c13 * pdzvz));
+                sto(O.szz, q, ldo(F.szz, q) + dt * (c13 * pdxvx + c11 * pdzvz));
             }
-            float pdzvx = idz * a, pdxvz = idx * b;
-            if (!Xin || j <= 1 || j >= nz + 1) {
-                pdxvz = mpml(F.m_dxvz, q, pdxvz, P.axh[ii], P.bxh[ii]);
-                pdzvx = mpml(F.m_dzvx, q, pdzvx, P.azh[jj], P.bzh[jj]);
+            {
+                float a = c.c[0] * (qx[R] - qx[R - 1]);        // d-z vx
+#pragma unroll
+                for (int m = 2; m <= R; ++m) a = a + c.c[m - 1] * (qx[R + m - 1] - qx[R - m]);
+                float pdzvx = idz * a, pdxvz = idx * sxvz;
+                if (!Xin || j <= 1 || j >= nz + 1) {
+                    pdxvz = mpml(F.m_dxvz, q, pdxvz, P.axh[ii], P.bxh[ii]);
+                    pdzvx = mpml(F.m_dzvx, q, pdzvx, P.azh[jj], P.bzh[jj]);
+                }
+                sto(O.sxz, q, ldo(F.sxz, q) + dt * ldo(M.mu, q) * (pdzvx + pdxvz));
             }
-            O.sxz[q] = F.sxz[q] + dt * M.mu[q] * (pdzvx + pdxvz);
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -174,69 +203,74 @@ __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ sho
 
 // ---------------------------------------------------------------------------
 // Velocity sweep: vx at storage (i, j) = (i - 1/2, j), vz at (i, j - 1/2)
-// (module_wavefield.f90:95-134).  z stencils from queues of sxz and szz.
+// (module_wavefield.f90:95-134).  z stencils from queues of sxz and szz, x
+// stencils of sxx and sxz by shuffles as in the stress sweep.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ shots,
                                                     const int* __restrict__ tile_shot, EF2 F, EV2 O, EM2 M,
                                                     const float* __restrict__ prof, Coef c, float dt, float idx,
                                                     float idz, int pml, int kc) {
+    constexpr int W = 64 - 2 * R;
     const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
     const ShotE2 S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
-    const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    if (i > nx + pml) return;
+    const int l = int(threadIdx.x);
+    const int i = -pml + 1 + tx * W - R + l;
     const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
-    if (j0 > j1) return;  // padding tile beyond the shot's rows
+    if (j0 > j1) return;
+    const bool mcol = i <= nx + pml + R;
+    const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const Prof2 P = prof_of(prof, S, pml);
     const float* __restrict__ sxx = O.sxx;
     const float* __restrict__ szz = O.szz;
     const float* __restrict__ sxz = O.sxz;
-    const size_t sz = S.px;
+    const unsigned szb = unsigned(S.px) * 4u;
     const int ii = i + pml - 1;
-    size_t q = at2(S, lo, i, j0);
+    unsigned q = unsigned(at2(S, lo, mcol ? i : lo, j0)) * 4u;
     float qa[2 * R + 1], qc[2 * R + 1];  // sxz, szz
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
-        qa[m] = sxz[q + (m - R) * sz];
-        qc[m] = szz[q + (m - R) * sz];
+        const unsigned r = q + unsigned(m - R) * szb;
+        qa[m] = mcol ? ldo(sxz, r) : 0.0f;
+        qc[m] = mcol ? ldo(szz, r) : 0.0f;
     }
     const bool xin = !(i <= 0 || i >= nx + 1), Xin = !(i <= 1 || i >= nx + 1);
-    for (int j = j0; j <= j1; ++j, q += sz) {
-        qa[2 * R] = sxz[q + R * sz];
-        qc[2 * R] = szz[q + R * sz];
-        const int jj = j + pml - 1;
+    for (int j = j0; j <= j1; ++j, q += szb) {
         {
-            float a = c.c[0] * (sxx[q] - sxx[q - 1]);     // d-x sxx
-            float b = c.c[0] * (qa[R + 1] - qa[R]);       // d+z sxz
-#pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (sxx[q + (m - 1)] - sxx[q - m]);
-                b = b + c.c[m - 1] * (qa[R + m] - qa[R - m + 1]);
-            }
-            float pdxxx = idx * a, pdzxz = idz * b;
-            if (!Xin || j <= 0 || j >= nz + 1) {
-                pdxxx = mpml(F.m_dxxx, q, pdxxx, P.axh[ii], P.bxh[ii]);
-                pdzxz = mpml(F.m_dzxz, q, pdzxz, P.azi[jj], P.bzi[jj]);
-            }
-            O.vx[q] = F.vx[q] + dt / M.rx[q] * (pdxxx + pdzxz);
+            const unsigned r = q + unsigned(R) * szb;
+            qa[2 * R] = mcol ? ldo(sxz, r) : 0.0f;
+            qc[2 * R] = mcol ? ldo(szz, r) : 0.0f;
         }
-        {
-            float a = c.c[0] * (sxz[q + 1] - sxz[q]);     // d+x sxz
-            float b = c.c[0] * (qc[R] - qc[R - 1]);       // d-z szz
+        const float cxx = mcol ? ldo(sxx, q) : 0.0f;
+        const float sxxx = sten_lanes<R>(cxx, l, -1, c.c);    // d-x sxx
+        const float sxxz = sten_lanes<R>(qa[R], l, 0, c.c);   // d+x sxz
+        if (own) {
+            const int jj = j + pml - 1;
+            {
+                float b = c.c[0] * (qa[R + 1] - qa[R]);       // d+z sxz
 #pragma unroll
-            for (int m = 2; m <= R; ++m) {
-                a = a + c.c[m - 1] * (sxz[q + m] - sxz[q - (m - 1)]);
-                b = b + c.c[m - 1] * (qc[R + m - 1] - qc[R - m]);
+                for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qa[R + m] - qa[R - m + 1]);
+                float pdxxx = idx * sxxx, pdzxz = idz * b;
+                if (!Xin || j <= 0 || j >= nz + 1) {
+                    pdxxx = mpml(F.m_dxxx, q, pdxxx, P.axh[ii], P.bxh[ii]);
+                    pdzxz = mpml(F.m_dzxz, q, pdzxz, P.azi[jj], P.bzi[jj]);
+                }
+                sto(O.vx, q, ldo(F.vx, q) + dt / ldo(M.rx, q) * (pdxxx + pdzxz));
             }
-            float pdxxz = idx * a, pdzzz = idz * b;
-            if (!xin || j <= 1 || j >= nz + 1) {
-                pdxxz = mpml(F.m_dxxz, q, pdxxz, P.axi[ii], P.bxi[ii]);
-                pdzzz = mpml(F.m_dzzz, q, pdzzz, P.azh[jj], P.bzh[jj]);
+            {
+                float b = c.c[0] * (qc[R] - qc[R - 1]);       // d-z szz
+#pragma unroll
+                for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qc[R + m - 1] - qc[R - m]);
+                float pdxxz = idx * sxxz, pdzzz = idz * b;
+                if (!xin || j <= 1 || j >= nz + 1) {
+                    pdxxz = mpml(F.m_dxxz, q, pdxxz, P.axi[ii], P.bxi[ii]);
+                    pdzzz = mpml(F.m_dzzz, q, pdzzz, P.azh[jj], P.bzh[jj]);
+                }
+                sto(O.vz, q, ldo(F.vz, q) + dt / ldo(M.rz, q) * (pdxxz + pdzzz));
             }
-            O.vz[q] = F.vz[q] + dt / M.rz[q] * (pdxxz + pdzzz);
         }
 #pragma unroll
         for (int m = 0; m < 2 * R; ++m) {
@@ -523,7 +557,7 @@ void retile2(owl_elastic2d* b) {
     b->tile_shot.clear();
     for (size_t s = 0; s < b->sd.size(); ++s) {
         ShotE2& S = b->sd[s];
-        S.ntx = (S.nx + 2 * b->pml + 63) / 64;
+        S.ntx = (S.nx + 2 * b->pml + (64 - 2 * b->R) - 1) / (64 - 2 * b->R);  // sweeps update 64 - 2R columns
         S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
         S.tile0 = int(b->tile_shot.size());
         b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nzc, int(s));
@@ -716,6 +750,8 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         b->max_nr = std::max(b->max_nr, S.nr);
     }
     b->field_elems = size_t(off);
+    if (b->field_elems >= (size_t(1) << 29))
+        fail(OWL_ERR_ARG, "the shots' padded grids exceed 2 GiB per field; split the batch");
     b->trace_elems = size_t(trace);
     b->strip_elems = size_t(strip);
     b->grad_elems = size_t(grad);
