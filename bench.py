@@ -266,8 +266,16 @@ def main():
         c = torch.empty_like(a)
         for _ in range(5):
             c.copy_(a)
+        # the same bytes through torch's elementwise kernel (read n, write n)
+        # and a reduction (read n)
+        for _ in range(5):
+            torch.add(a, 1.0, out=c)
+        for _ in range(5):
+            a.sum()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5,
+                          "elementwise_add": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5},
+                          "sum": {"bytes_read": 4 * n, "launches": 5}}))
         return
     from owl_amd import (OWL_CKPT_BINOMIAL, OWL_CKPT_FULL, OWL_GRAD_EXACT, OWL_GRAD_OWL, OWL_STORE_F16,
                          OWL_STORE_F32)

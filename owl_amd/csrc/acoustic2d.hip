@@ -1239,6 +1239,9 @@ struct owl_acoustic2d {
     bool have_store = false;
     std::vector<std::vector<int>> binom_split;  // cached optimal splits [s][l]
     int binom_n = 0, binom_s = -1;
+    bool binom_carry = false;
+    long long binom_steps = 0;       // forward steps of the cached schedule
+    DevBuf<float> fK[6];             // carried states of the binomial schedule (2 x p, vx, vz)
     // HIP graph of the exact-adjoint gradient: captured on the second call
     // with the same schedule and buffers, replayed afterwards
     int use_graphs = 1;
@@ -2135,18 +2138,27 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
     inject_residual(b, t);
 }
 
-// Optimal binomial split table: cost[s][l] = fewest forward steps to run the
-// adjoint over l steps with s free snapshots (each adjoint step recomputes
-// its own forward step); split[s][l] = steps to advance before the next
-// snapshot.  cost[s][1] = 1, cost[0][l] = l(l+1)/2,
-// cost[s][l] = min_j j + cost[s-1][l-j] + cost[s][j].
-void binomial_table(owl_acoustic2d* b, int N, int S) {
-    if (b->binom_n == N && b->binom_s == S) return;
+// Optimal binomial split tables.  cost[s][l] = fewest forward steps to run
+// the adjoint over l steps from a stored u_a with s free snapshots;
+// split[s][l] = steps to advance before the next snapshot.  The adjoint
+// step t reads u_t and u_{t-1}.
+//  * carry = false: each adjoint step computes its own u_t from u_{t-1}:
+//    cost[s][1] = 1, cost[0][l] = l(l+1)/2.
+//  * carry = true: u_t is kept from the adjoint step before, where it was
+//    the older state, so only u_{t-1} is recomputed: cost[s][1] = 0,
+//    cost[0][l] = l(l-1)/2, plus one step for u_N.  This needs the two
+//    states to come out of one exact trajectory, so it is used with FP32
+//    snapshots only: with 16-bit ones u_t and u_{t-1} could be recomputed
+//    from different rounded snapshots, and the imaging terms use their
+//    difference.
+// Both: cost[s][l] = min_j j + cost[s-1][l-j] + cost[s][j].
+void binomial_table(owl_acoustic2d* b, int N, int S, bool carry) {
+    if (b->binom_n == N && b->binom_s == S && b->binom_carry == carry) return;
     std::vector<std::vector<long long>> cost(S + 1, std::vector<long long>(N + 1, 0));
     b->binom_split.assign(S + 1, std::vector<int>(N + 1, 1));
-    for (int l = 1; l <= N; ++l) cost[0][l] = (long long)l * (l + 1) / 2;
+    for (int l = 1; l <= N; ++l) cost[0][l] = carry ? (long long)l * (l - 1) / 2 : (long long)l * (l + 1) / 2;
     for (int s = 1; s <= S; ++s) {
-        cost[s][1] = 1;
+        cost[s][1] = carry ? 0 : 1;
         int jlo = 1;
         for (int l = 2; l <= N; ++l) {
             long long best = -1;
@@ -2164,28 +2176,51 @@ void binomial_table(owl_acoustic2d* b, int N, int S) {
     }
     b->binom_n = N;
     b->binom_s = S;
+    b->binom_carry = carry;
+    b->binom_steps = cost[S][N] + (carry ? 1 : 0);
 }
 
 struct RevCtx {
     bool first = true;           // still on the first forward descent (recording)
     bool direct = false;         // FP32 snapshots: steps read and write the slots directly
+    bool carry = false;          // u_t of each adjoint step is kept from the step before
+    bool carried = false;        // `cur` holds u_t of the next adjoint step
+    int nlog = 1;                // snapshots of the schedule
+    std::vector<int> phys;       // snapshot -> physical slot
+    int spare = -1;              // the physical slot outside `phys` (carry mode)
+    W7 cur{};                    // u_t of the next adjoint step (p, vx, vz)
+    int cur_phys = -1;           // its physical slot, or -1
     std::function<void()> on_first_adjoint;
 };
 
 // Where a forward state is: kW1 / kW3 = the working sets w1 / w3 (the
 // steps ping-pong between them), kZero = the zero initial state
-// (materialised by clearing w1), >= 0 = snapshot slot.
+// (materialised by clearing w1), >= 0 = snapshot.
 constexpr int kW1 = -2, kZero = -1, kW3 = -3;
 
-W7 slot_set(owl_acoustic2d* b, int slot) {
+W7 slot_set(owl_acoustic2d* b, int phys) {
     W7 w;
-    for (int k = 0; k < 7; ++k) w.f[k] = const_cast<float*>(slot_ptr(b, b->snaps, slot, k));
+    for (int k = 0; k < 7; ++k) w.f[k] = const_cast<float*>(slot_ptr(b, b->snaps, phys, k));
     return w;
 }
 
+// The two carried-state sets (p, vx, vz; memory variables not kept).
+W7 carry_set(owl_acoustic2d* b, int k) {
+    W7 w{};
+    for (int i = 0; i < 3; ++i) w.f[i] = b->fK[3 * k + i].get();
+    return w;
+}
+
+// The physical slot a store into snapshot `s` writes.  When that slot still
+// holds the carried state, the spare slot takes its place in the map.
+int store_slot(RevCtx& ctx, int s) {
+    if (ctx.phys[s] == ctx.cur_phys) std::swap(ctx.phys[s], ctx.spare);
+    return ctx.phys[s];
+}
+
 // The state at `loc` as a readable set (clearing w1 for the zero state).
-W7 state_at(owl_acoustic2d* b, int loc) {
-    if (loc >= 0) return slot_set(b, loc);
+W7 state_at(owl_acoustic2d* b, int loc, const RevCtx& ctx) {
+    if (loc >= 0) return slot_set(b, ctx.phys[loc]);
     if (loc == kW3) return work3(b);
     const W7 w1 = work1(b);
     if (loc == kZero) zero_set(b, w1);
@@ -2200,10 +2235,10 @@ int loc_of(owl_acoustic2d* b, const W7& w) { return w.f[0] == b->fA[0].get() ? k
 // otherwise the state is restored into w1 and saved by copy.  Returns
 // where u_to is.
 template <int R>
-int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record, bool direct) {
-    if (!direct) {
+int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record, RevCtx& ctx) {
+    if (!ctx.direct) {
         W7 cur = work1(b);
-        if (loc >= 0) slot_load(b, b->snaps, loc, cur.f);
+        if (loc >= 0) slot_load(b, b->snaps, ctx.phys[loc], cur.f);
         else if (loc == kZero) zero_set(b, cur);
         else if (loc == kW3) cur = work3(b);
         for (int t = from + 1; t <= to; ++t) {
@@ -2211,22 +2246,22 @@ int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record,
             fwd_step<R>(b, cur, out, t, record);
             cur = out;
         }
-        if (into >= 0) slot_save(b, b->snaps, into, cur.f);
+        if (into >= 0) slot_save(b, b->snaps, store_slot(ctx, into), cur.f);
         return loc_of(b, cur);
     }
     if (to == from) {
         if (into >= 0 && into != loc) {
-            const W7 src = state_at(b, loc);
-            slot_save(b, b->snaps, into, src.f);
+            const W7 src = state_at(b, loc, ctx);
+            slot_save(b, b->snaps, store_slot(ctx, into), src.f);
             return into;
         }
         return loc;
     }
-    W7 cur = state_at(b, loc);
+    W7 cur = state_at(b, loc, ctx);
     bool in_slot = loc >= 0;
     for (int t = from + 1; t <= to; ++t) {
         W7 out;
-        if (t == to && into >= 0) out = slot_set(b, into);
+        if (t == to && into >= 0) out = slot_set(b, store_slot(ctx, into));
         else out = in_slot ? work1(b) : other_of(b, cur);
         fwd_step<R>(b, cur, out, t, record);
         cur = out;
@@ -2235,13 +2270,24 @@ int advance(owl_acoustic2d* b, int from, int to, int loc, int into, bool record,
     return into >= 0 ? into : loc_of(b, cur);
 }
 
+// One forward step from the full state `in` to u_t, writing p, vx, vz into
+// `dst` and the memory variables into the working set `in` does not use.
+template <int R>
+W7 step_into(owl_acoustic2d* b, const W7& in, const W7& dst, int t, bool record) {
+    W7 out = in.f[3] == b->fM[0].get() ? work3(b) : work1(b);
+    for (int i = 0; i < 3; ++i) out.f[i] = dst.f[i];
+    fwd_step<R>(b, in, out, t, record);
+    return out;
+}
+
 // Reverse sweep over steps b..a+1: u_a is stored at `slot` (snapshot, or
 // kZero) and currently at `loc`; sfree free snapshots.
 template <int R>
 void rev(owl_acoustic2d* bt, int a, int b, int sfree, int slot, int loc, RevCtx& ctx) {
-    const W7 w2 = work2(bt);
+    // carry = false: u_{t-1} recomputed from u_a, then u_t by one step into w2
     auto last_step = [&](int t, int at) {
-        const W7 prev = state_at(bt, at);
+        const W7 w2 = work2(bt);
+        const W7 prev = state_at(bt, at, ctx);
         fwd_step<R>(bt, prev, w2, t, ctx.first);
         if (ctx.first) {
             ctx.first = false;
@@ -2249,20 +2295,63 @@ void rev(owl_acoustic2d* bt, int a, int b, int sfree, int slot, int loc, RevCtx&
         }
         adjoint_step<R>(bt, t, w2.f, prev.f);
     };
+    // carry = true: u_{t-1} recomputed from u_a (at `from`) into the carried
+    // set t-1 (or read in place when it is u_a itself); u_t is the state the
+    // adjoint step t+1 read as its older one
+    auto reverse_step = [&](int t, int from) {
+        int prev_phys = -1;
+        W7 prev;
+        if (t - 1 > a) {
+            const int at = advance<R>(bt, a, t - 2, from, -1, ctx.first, ctx);
+            prev = step_into<R>(bt, state_at(bt, at, ctx), carry_set(bt, (t - 1) & 1), t - 1, ctx.first);
+        } else if (from >= 0) {
+            prev_phys = ctx.phys[from];
+            prev = slot_set(bt, prev_phys);
+        } else if (from == kZero) {
+            prev = state_at(bt, kZero, ctx);  // u_0; t = 1 is the last adjoint step
+        } else {
+            // a working set: copy the fields out of the next recomputation's way
+            const W7 w = state_at(bt, from, ctx);
+            prev = w;
+            const W7 k = carry_set(bt, (t - 1) & 1);
+            for (int i = 0; i < 3; ++i) {
+                OWL_HIP(hipMemcpyAsync(k.f[i], w.f[i], bt->field_elems * sizeof(float), hipMemcpyDeviceToDevice,
+                                       bt->stream));
+                prev.f[i] = k.f[i];
+            }
+        }
+        if (!ctx.carried) {  // the first adjoint step: u_t from u_{t-1}
+            ctx.cur = step_into<R>(bt, prev, carry_set(bt, t & 1), t, ctx.first);
+            ctx.cur_phys = -1;
+            ctx.carried = true;
+            if (ctx.first) {
+                ctx.first = false;
+                ctx.on_first_adjoint();
+            }
+        }
+        adjoint_step<R>(bt, t, ctx.cur.f, prev.f);
+        ctx.cur = prev;
+        ctx.cur_phys = prev_phys;
+    };
     if (b == a + 1) {
-        last_step(b, ctx.direct ? loc : advance<R>(bt, a, a, loc, -1, ctx.first, false));
+        if (ctx.carry) reverse_step(b, loc);
+        else last_step(b, ctx.direct ? loc : advance<R>(bt, a, a, loc, -1, ctx.first, ctx));
         return;
     }
     if (sfree == 0) {
         for (int t = b; t >= a + 1; --t) {
-            const int at = advance<R>(bt, a, t - 1, t == b ? loc : slot, -1, ctx.first, ctx.direct);
-            last_step(t, at);
+            if (ctx.carry) {
+                reverse_step(t, t == b ? loc : slot);
+            } else {
+                const int at = advance<R>(bt, a, t - 1, t == b ? loc : slot, -1, ctx.first, ctx);
+                last_step(t, at);
+            }
         }
         return;
     }
     const int j = bt->binom_split[sfree][b - a];
-    const int mine = bt->snaps.nslot - sfree;
-    const int at = advance<R>(bt, a, a + j, loc, mine, ctx.first, ctx.direct);
+    const int mine = ctx.nlog - sfree;
+    const int at = advance<R>(bt, a, a + j, loc, mine, ctx.first, ctx);
     rev<R>(bt, a + j, b, sfree - 1, mine, at, ctx);
     rev<R>(bt, a, a + j, sfree, slot, slot, ctx);
 }
@@ -2391,18 +2480,28 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
         if (phase == kForwardOnly) {
             // the schedule starts from u_0 in the reverse call; here only the
             // traces are modelled
-            advance<R>(b, 0, N, kW1, -1, true, false);
+            RevCtx plain;
+            advance<R>(b, 0, N, kW1, -1, true, plain);
             return;
         }
-        binomial_table(b, N, S);
-        slots_alloc(b, b->snaps, std::max(S, 1), 7, fmt);
         RevCtx ctx;
         ctx.first = do_fwd;
         ctx.direct = fmt == OWL_STORE_F32;
-        // steps write FP32 snapshots over the update domain only: their outer
-        // halo must read as zero
-        if (ctx.direct)
-            dzero(b, b->snaps.f32.get(), size_t(std::max(S, 1)) * 7 * b->field_elems * sizeof(float));
+        ctx.carry = ctx.direct;
+        binomial_table(b, N, S, ctx.carry);
+        // carry mode: one physical slot more, for a carried state whose
+        // snapshot is stored over before the adjoint step reads it
+        ctx.nlog = std::max(S, 1);
+        const int nphys = ctx.nlog + (ctx.carry ? 1 : 0);
+        ctx.phys.resize(size_t(ctx.nlog));
+        for (int k = 0; k < ctx.nlog; ++k) ctx.phys[size_t(k)] = k;
+        ctx.spare = ctx.carry ? ctx.nlog : -1;
+        slots_alloc(b, b->snaps, nphys, 7, fmt);
+        // steps write FP32 snapshots and the carried sets over the update
+        // domain only: their outer halo must read as zero
+        if (ctx.direct) dzero(b, b->snaps.f32.get(), size_t(nphys) * 7 * b->field_elems * sizeof(float));
+        if (ctx.carry)
+            for (auto& k : b->fK) dzero(b, k.get(), b->field_elems * sizeof(float));
         ctx.on_first_adjoint = [&] {
             residual();
             adjoint_alloc_zero(b);
@@ -2512,6 +2611,7 @@ std::vector<uintptr_t> exact_signature(owl_acoustic2d* b, int ckpt, int param, i
     for (int k = 0; k < 4; ++k) { add(b->fM[k].get()); add(b->fM2[k].get()); add(b->fM3[k].get()); }
     for (auto& a : b->adj) add(a.get());
     for (auto& a : b->adj2) add(a.get());
+    for (auto& k : b->fK) add(k.get());
     for (auto* g : {&b->gK, &b->gX, &b->gZ, &b->d_gv, &b->d_gr, &b->d_misfit, &b->d_scratch}) add(g->get());
     for (auto* S : {&b->full, &b->snaps, &b->seg}) { add(S->f32.get()); add(S->h16.get()); add(S->amax.get()); }
     for (auto* f : {&b->d_traces, &b->d_obs, &b->d_adj, &b->d_bk, &b->d_rho, &b->d_prof, &b->d_amp, &b->d_rec_w,
@@ -2537,6 +2637,8 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     for (int k = 0; k < 4; ++k) b->fM2[k].alloc(b->field_elems);
     for (int k = 0; k < 3; ++k) b->fC[k].alloc(b->field_elems);
     for (int k = 0; k < 4; ++k) b->fM3[k].alloc(b->field_elems);
+    if (ckpt == OWL_CKPT_BINOMIAL && fmt == OWL_STORE_F32)
+        for (auto& k : b->fK) k.alloc(b->field_elems);
     const bool do_fwd = phase != kReverseOnly, do_rev = phase != kForwardOnly;
     if (do_rev && !do_fwd &&
         (!b->have_store || b->store_policy != ckpt || b->store_param != param || b->store_fmt != fmt))

@@ -62,6 +62,7 @@ def test_exact_gradient_matches_oracle(oracle32, gpu):
 
 @pytest.mark.parametrize("ckpt,param", [(OWL_CKPT_UNIFORM, 7), (OWL_CKPT_UNIFORM, 1),
                                         (OWL_CKPT_BINOMIAL, 4), (OWL_CKPT_BINOMIAL, 0),
+                                        (OWL_CKPT_BINOMIAL, 1), (OWL_CKPT_BINOMIAL, 2),
                                         (OWL_CKPT_BINOMIAL, 40)])
 def test_checkpointing_is_bit_identical_to_full_storage(oracle32, gpu, ckpt, param):
     vp, rho, geoms = _case(3)
@@ -127,17 +128,18 @@ def test_exact_gradient_finite_differences(oracle32, gpu):
     b.close()
 
 
-def test_autograd_backward_equals_gradient(oracle32, gpu):
+@pytest.mark.parametrize("ckpt,param", [(OWL_CKPT_UNIFORM, 9), (OWL_CKPT_BINOMIAL, 5)])
+def test_autograd_backward_equals_gradient(oracle32, gpu, ckpt, param):
     import torch
     from owl_amd.autograd import propagate
     vp, rho, geoms = _case(6)
     b = _batch(vp, rho, geoms)
     obs = _obs(oracle32, vp, rho, geoms, b.nt)
-    print("case: torch.autograd through the exact adjoint", flush=True)
+    print(f"case: torch.autograd through the exact adjoint, policy {ckpt} param {param}", flush=True)
     _, gv0, gr0, _ = b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=OWL_CKPT_FULL)
     tv = torch.tensor(vp, dtype=torch.float64, device="cuda:0", requires_grad=True)
     tr_ = torch.tensor(rho, dtype=torch.float64, device="cuda:0", requires_grad=True)
-    d = propagate(tv, tr_, b, OWL_CKPT_UNIFORM, 9, OWL_STORE_F32)
+    d = propagate(tv, tr_, b, ckpt, param, OWL_STORE_F32)
     o = torch.tensor(np.concatenate(obs, 0), device="cuda:0")
     loss = 0.5 * ((d.double() - o.double()) ** 2).sum()
     loss.backward()
