@@ -519,12 +519,11 @@ __global__ void k_inject_strips(const Shot* __restrict__ shots, float* vx, float
 
 // Adjoint source injection (submodule_fwi_gradient.f90:126-147): cells in CSR
 // order, contributions of each cell in receiver order.
-__global__ void k_inject_adjoint(const Shot* __restrict__ shots, float* pr,
-                                 const int* __restrict__ cell_beg, const Tap* __restrict__ taps,
-                                 const float* __restrict__ rec_w, const float* __restrict__ adj,
-                                 int t) {
-    const Shot S = shots[blockIdx.y];
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.nadj; c += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void inject_adjoint_cells(const Shot& S, int c0, int dc, float* pr,
+                                                     const int* __restrict__ cell_beg, const Tap* __restrict__ taps,
+                                                     const float* __restrict__ rec_w, const float* __restrict__ adj,
+                                                     int t) {
+    for (int c = c0; c < S.nadj; c += dc) {
         const int b = cell_beg[S.adj0 + c], e = cell_beg[S.adj0 + c + 1];
         const long long cell = taps[b].cell;
         float acc = pr[cell];
@@ -535,6 +534,14 @@ __global__ void k_inject_adjoint(const Shot* __restrict__ shots, float* pr,
         }
         pr[cell] = acc;
     }
+}
+
+__global__ void k_inject_adjoint(const Shot* __restrict__ shots, float* pr,
+                                 const int* __restrict__ cell_beg, const Tap* __restrict__ taps,
+                                 const float* __restrict__ rec_w, const float* __restrict__ adj,
+                                 int t) {
+    inject_adjoint_cells(shots[blockIdx.y], int(blockIdx.x * blockDim.x + threadIdx.x), int(gridDim.x * blockDim.x),
+                         pr, cell_beg, taps, rec_w, adj, t);
 }
 
 // Imaging condition of OWL's gradient (submodule_fwi_gradient.f90:201-254,
@@ -700,6 +707,7 @@ struct AdjSet {
     double* gK;
     double* gX;
     double* gZ;
+    double* gR;  // fused adjoint: neighbour-summed density term
 };
 
 template <int R>
@@ -930,7 +938,8 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
     float dxq[R];  // this step's dx of rows r - R + 1 .. r (own column)
 #pragma unroll
     for (int k = 0; k < R; ++k) dxq[k] = 0.0f;
-    Row cur, nxt;
+    double gpend = 0.0;  // gR of row j - 1 without the cz of row j
+    Row cur{}, nxt{};
     load(rs, cur);
     for (int r = rs; r <= re; ++r) {
         if (r < re) load(r + 1, nxt);
@@ -977,13 +986,26 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
             float xs[2 * R];  // dx of row j at columns col - R .. col + R - 1
 #pragma unroll
             for (int k = 0; k < 2 * R; ++k) xs[k] = __shfl(dxq[0], l - R + k, 64);
+            // density terms of row j: cx at (col - 1/2, j), cz at (col, j - 1/2);
+            // cxr = the next column's cx (zero unless its lane owns it)
+            const float rx = 0.5f * (cur.rq + cur.rl);
+            const float rz = 0.5f * (cur.rq + cur.ru);
+            const float cx = own ? (cur.xp - cur.xc) / rx * cur.vxh : 0.0f;
+            const float cxr = __shfl(cx, l + 1, 64);
             if (own) {
                 const unsigned q = base + unsigned(j - lo) * pitchb;
-                const float rx = 0.5f * (cur.rq + cur.rl);
-                const float rz = 0.5f * (cur.rq + cur.ru);
                 float vxh = cur.vxh, vzh = cur.vzh;
-                addo(A.gX, 2u * q, double((cur.xp - cur.xc) / rx * vxh));
-                addo(A.gZ, 2u * q, double((cur.zp - cur.zc_) / rz * vzh));
+                const float cz = (cur.zp - cur.zc_) / rz * vzh;
+                // gR(c) collects (cx(c) + cx(c+1) + cz(c) + cz(c+row)) / 2 where
+                // this lane owns both cells; a tile's first column and a chunk's
+                // first row also keep their own cx / cz in gX / gZ for the
+                // neighbour outside (see k_fold_exact)
+                const double s = 0.5 * double(cx) + 0.5 * double(cxr) + 0.5 * double(cz);
+                if (j > z0) addo(A.gR, 2u * (q - pitchb), gpend + 0.5 * double(cz));
+                else addo(A.gZ, 2u * q, double(cz));
+                gpend = s;
+                if (j == z1) addo(A.gR, 2u * q, gpend);
+                if (l == R) addo(A.gX, 2u * q, double(cx));
                 float sx = 0.0f, sz = 0.0f;
 #pragma unroll
                 for (int m = 1; m <= R; ++m) {
@@ -1021,11 +1043,10 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
 // The stored p_t includes the source injected at step t; the K gradient
 // needs the increment of the update alone: remove source/K * ph at the
 // source taps active at step t (one block per shot).
-__global__ void k_src_correction(const Shot* __restrict__ shots, const int* __restrict__ inj_cell_beg,
-                                 const Tap* __restrict__ inj_taps, const float* __restrict__ amp_tab,
-                                 const float* __restrict__ bk, const float* __restrict__ ph, double* gK,
-                                 int nt, int t) {
-    const Shot S = shots[blockIdx.x];
+__device__ __forceinline__ void src_correction_cells(const Shot& S, const int* __restrict__ inj_cell_beg,
+                                                     const Tap* __restrict__ inj_taps,
+                                                     const float* __restrict__ amp_tab, const float* __restrict__ bk,
+                                                     const float* ph, double* gK, int nt, int t) {
     for (int c = threadIdx.x; c < S.ninj; c += blockDim.x) {
         const int b = inj_cell_beg[S.inj0 + c], e = inj_cell_beg[S.inj0 + c + 1];
         const long long cell = inj_taps[b].cell;
@@ -1040,13 +1061,43 @@ __global__ void k_src_correction(const Shot* __restrict__ shots, const int* __re
     }
 }
 
+__global__ void k_src_correction(const Shot* __restrict__ shots, const int* __restrict__ inj_cell_beg,
+                                 const Tap* __restrict__ inj_taps, const float* __restrict__ amp_tab,
+                                 const float* __restrict__ bk, const float* __restrict__ ph, double* gK,
+                                 int nt, int t) {
+    src_correction_cells(shots[blockIdx.x], inj_cell_beg, inj_taps, amp_tab, bk, ph, gK, nt, t);
+}
+
+// The fused adjoint's tap work of reverse step t in one launch (one block
+// per shot): the source correction (reads ph at the source taps), then the
+// adjoint source injection into ph, in the order of the two kernels above.
+__global__ __launch_bounds__(256) void k_adjoint_taps(const Shot* __restrict__ shots,
+                                                      const int* __restrict__ inj_cell_beg,
+                                                      const Tap* __restrict__ inj_taps,
+                                                      const float* __restrict__ amp_tab,
+                                                      const float* __restrict__ bk, float* ph, double* gK, int nt,
+                                                      const int* __restrict__ adj_beg,
+                                                      const Tap* __restrict__ adj_taps,
+                                                      const float* __restrict__ rec_w, const float* __restrict__ adj,
+                                                      int t) {
+    const Shot S = shots[blockIdx.x];
+    src_correction_cells(S, inj_cell_beg, inj_taps, amp_tab, bk, ph, gK, nt, t);
+    __syncthreads();
+    inject_adjoint_cells(S, int(threadIdx.x), int(blockDim.x), ph, adj_beg, adj_taps, rec_w, adj, t);
+}
+
 // Fold the padded-grid gradients onto the window (edge replication
 // transposed) and change variables K = rho vp^2:
 //   d/dvp = 2 rho vp dK,  d/drho = drho + vp^2 dK,
-// with drho(c) = (gX(c) + gX(c+1) + gZ(c) + gZ(c+row)) / 2.
+// with drho(c) = (gX(c) + gX(c+1) + gZ(c) + gZ(c+row)) / 2 after the
+// two-pass adjoint.  After the fused adjoint (merged != 0) gR(c) already
+// holds those sums where one lane owned both cells, and gX / gZ hold only a
+// tile's first column and a chunk's first row: drho(c) = gR(c) +
+// (gX(c+1) + gZ(c+row)) / 2.
 template <int R>
 __global__ void k_fold_exact(const Shot* __restrict__ shots, const double* __restrict__ gK,
                              const double* __restrict__ gX, const double* __restrict__ gZ,
+                             const double* __restrict__ gR, int merged,
                              const float* __restrict__ vpw, const float* __restrict__ rhow, double* gv,
                              double* gr, int pml) {
     const Shot S = shots[blockIdx.y];
@@ -1062,9 +1113,10 @@ __global__ void k_fold_exact(const Shot* __restrict__ shots, const double* __res
             for (int ii = i0; ii <= i1; ++ii) {
                 const size_t q = size_t(S.off) + size_t(ii - lo) + size_t(jj - lo) * pitch;
                 k_acc += gK[q];
-                double gxr = gX[q] + ((ii + 1 <= nx + pml) ? gX[q + 1] : 0.0);
-                double gzr = gZ[q] + ((jj + 1 <= nz + pml) ? gZ[q + pitch] : 0.0);
-                r_acc += 0.5 * gxr + 0.5 * gzr;
+                const double gx1 = (ii + 1 <= nx + pml) ? gX[q + 1] : 0.0;
+                const double gz1 = (jj + 1 <= nz + pml) ? gZ[q + pitch] : 0.0;
+                if (merged) r_acc += gR[q] + (0.5 * gx1 + 0.5 * gz1);
+                else r_acc += 0.5 * (gX[q] + gx1) + 0.5 * (gZ[q] + gz1);
             }
         const double v = vpw[S.g0 + e], r = rhow[S.g0 + e];
         gv[S.g0 + e] = 2.0 * r * v * k_acc;
@@ -1228,7 +1280,7 @@ struct owl_acoustic2d {
     DevBuf<float> adj[11];            // ph vxh vzh psx psz phx phz ex ez dx dz
     DevBuf<float> adj2[5];            // fused adjoint: second ph psx psz ex ez
     int adj_cur = 0;                  // which of the two holds the current ph psx psz ex ez
-    DevBuf<double> gK, gX, gZ;
+    DevBuf<double> gK, gX, gZ, gR;
     struct Slots {                    // stored forward states
         int fmt = 0, nfield = 3, nslot = 0;
         DevBuf<float> f32;
@@ -2055,6 +2107,7 @@ AdjSet adj_set(owl_acoustic2d* b) {
     A.gK = b->gK.get();
     A.gX = b->gX.get();
     A.gZ = b->gZ.get();
+    A.gR = b->gR.get();
     return A;
 }
 
@@ -2063,7 +2116,7 @@ void adjoint_alloc_zero(owl_acoustic2d* b) {
         b->adj[k].alloc(b->field_elems);
         dzero(b, b->adj[k].get(), b->field_elems * sizeof(float));
     }
-    for (DevBuf<double>* g : {&b->gK, &b->gX, &b->gZ}) {
+    for (DevBuf<double>* g : {&b->gK, &b->gX, &b->gZ, &b->gR}) {
         g->alloc(b->field_elems);
         dzero(b, g->get(), b->field_elems * sizeof(double));
     }
@@ -2105,20 +2158,25 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
         float** out = b->adj_cur ? s0 : s1;
         const AdjIO io{in[0], in[1], in[2], in[3], in[4], out[0], out[1], out[2], out[3], out[4]};
         // ex, ez, ph, p_t, p_{t-1}, vxh, vzh, 4 states read; ph, vxh, vzh, ex, ez
-        // written; gK, gX, gZ r+w (112 B per point, PML memories aside); bk
-        // and rho once per distinct window
-        const int ev = prof_begin(b, 14, pts * 112.0 + 8.0 * b->pts_model);
+        // written; gK, gR r+w (96 B per point, PML memories aside); gX on a
+        // tile's first column and gZ on a chunk's first row r+w; bk and rho
+        // once per distinct window
+        const double wcol = 64 - (2 * R - 1);
+        const int ev = prof_begin(b, 14, pts * (96.0 + 16.0 / wcol + 16.0 / b->azc) + 8.0 * b->pts_model);
         hipLaunchKernelGGL(k_adj_fused<R>, dim3(unsigned((b->natile + 3) / 4)), dim3(64, 4), 0, b->stream,
                            b->d_shots.get(), b->d_atile_shot.get(), b->natile, io, A, b->d_bk.get(), b->d_rho.get(),
                            b->d_prof.get(), b->coef, b->dt, b->idx, b->idz, b->pml, b->azc, cur[0], prev[0], cur[1],
                            prev[1], cur[2], prev[2]);
         prof_end(b, ev);
         b->adj_cur = 1 - b->adj_cur;
-        if (b->max_ninj > 0)
-            hipLaunchKernelGGL(k_src_correction, dim3(unsigned(b->sd.size())), dim3(256), 0, b->stream,
+        if (b->max_ninj > 0 || b->max_nadj > 0) {
+            const int ev2 = prof_begin(b, kKAdj);
+            hipLaunchKernelGGL(k_adjoint_taps, dim3(unsigned(b->sd.size())), dim3(256), 0, b->stream,
                                b->d_shots.get(), b->d_inj_beg.get(), b->d_inj_taps.get(), b->d_amp.get(),
-                               b->d_bk.get(), cur_ph(b), b->gK.get(), b->nt, t);
-        inject_residual(b, t);
+                               b->d_bk.get(), cur_ph(b), b->gK.get(), b->nt, b->d_adj_beg.get(),
+                               b->d_adj_taps.get(), b->d_rec_w.get(), b->d_adj.get(), t);
+            prof_end(b, ev2);
+        }
         return;
     }
     // p-pass: ex, ez, ph, bk, p_t, p_{t-1}, psx/psz read; ph, dx, dz written; gK r+w
@@ -2525,7 +2583,8 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
     b->d_gr.alloc(b->grad_elems);
     const unsigned fg = unsigned(std::max<size_t>(1, std::min<size_t>(1024, (b->grad_elems / nshot + 255) / 256)));
     hipLaunchKernelGGL(k_fold_exact<R>, dim3(fg, nshot), dim3(256), 0, st, b->d_shots.get(), b->gK.get(), b->gX.get(),
-                       b->gZ.get(), b->d_vpw.get(), b->d_rhow.get(), b->d_gv.get(), b->d_gr.get(), b->pml);
+                       b->gZ.get(), b->gR.get(), int(adj_fused(b)), b->d_vpw.get(), b->d_rhow.get(), b->d_gv.get(),
+                       b->d_gr.get(), b->pml);
     }
     (void)fwd_steps;
     (void)adj_steps;
@@ -2612,7 +2671,7 @@ std::vector<uintptr_t> exact_signature(owl_acoustic2d* b, int ckpt, int param, i
     for (auto& a : b->adj) add(a.get());
     for (auto& a : b->adj2) add(a.get());
     for (auto& k : b->fK) add(k.get());
-    for (auto* g : {&b->gK, &b->gX, &b->gZ, &b->d_gv, &b->d_gr, &b->d_misfit, &b->d_scratch}) add(g->get());
+    for (auto* g : {&b->gK, &b->gX, &b->gZ, &b->gR, &b->d_gv, &b->d_gr, &b->d_misfit, &b->d_scratch}) add(g->get());
     for (auto* S : {&b->full, &b->snaps, &b->seg}) { add(S->f32.get()); add(S->h16.get()); add(S->amax.get()); }
     for (auto* f : {&b->d_traces, &b->d_obs, &b->d_adj, &b->d_bk, &b->d_rho, &b->d_prof, &b->d_amp, &b->d_rec_w,
                     &b->d_vpw, &b->d_rhow})

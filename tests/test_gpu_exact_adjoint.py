@@ -152,8 +152,11 @@ def test_autograd_backward_equals_gradient(oracle32, gpu, ckpt, param):
 def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows, arows):
     """The fused z-marching forward and adjoint steps against the two sweeps
     and the two-pass adjoint: five shots on uneven windows, traces, misfits
-    and gradients bit-identical for full FP32 and FP16 storage, uniform and
-    binomial checkpoints, at several z-row counts of either kernel."""
+    and the vp gradient bit-identical for full FP32 and FP16 storage, uniform
+    and binomial checkpoints, at several z-row counts of either kernel.  The
+    fused adjoint sums the four density terms of a cell per step before its
+    FP64 accumulation (the two-pass adjoint accumulates them apart), so the
+    rho gradient agrees to FP64 rounding: rel-L2 <= 1e-12."""
     from owl_amd.synthetic import gardner, random_field
     nz, nx, dx = 60, 140, 10.0
     vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 11))).astype(np.float32)
@@ -194,7 +197,7 @@ def test_fused_step_is_bit_identical_to_sweeps(gpu, R, zrows, arows):
             for x, y in zip(s0, s1):
                 assert np.array_equal(x, y)
             assert np.array_equal(gv0, gv1)
-            assert np.array_equal(gr0, gr1)
+            assert rel_l2(gr1, gr0) <= 1e-12, rel_l2(gr1, gr0)
     b.close()
 
 
@@ -232,7 +235,8 @@ def test_graph_replays_are_bit_identical_to_eager(gpu):
 
 def test_autotuned_rows_give_the_same_gradient(gpu):
     """The z-row autotuner picks rows from its candidate sets, and the
-    autotuned batch's exact gradient is bit-identical to one with fixed rows."""
+    autotuned batch's exact gradient equals one with fixed rows: the misfit
+    and the vp gradient bit for bit, the rho gradient to FP64 rounding."""
     from owl_amd.synthetic import gardner, random_field
     nz, nx, dx = 60, 140, 10.0
     vp = (2000.0 * (1 + 0.05 * random_field((nz, nx), 4.0, 12))).astype(np.float32)
@@ -259,5 +263,7 @@ def test_autotuned_rows_give_the_same_gradient(gpu):
     m1, gv1, gr1, _ = b.gradient(obs, **kw)
     assert np.array_equal(m0, m1)
     assert np.abs(gv0).max() > 0
-    assert np.array_equal(gv0, gv1) and np.array_equal(gr0, gr1)
+    # the density terms are split between accumulators by chunk, so the rho
+    # gradient depends on the rows at FP64 rounding only
+    assert np.array_equal(gv0, gv1) and rel_l2(gr1, gr0) <= 1e-12, rel_l2(gr1, gr0)
     b.close()
