@@ -34,6 +34,28 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
+# HBM bytes per launch of each workload's dominant kernel, measured on one
+# MI355X by two rocprofv3 counter passes over the current kernels:
+# 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE (KiB per dispatch), the factors that a
+# 1 GiB copy, elementwise add and reduction give.  Counters cannot run inside
+# a timed process, so the values are carried here; profiles/ holds the
+# commands and the raw counts.
+TRAFFIC = {
+    ("acoustic2d_gradient", "step_fused"): (2.0 * 148.5e3 + 112.9e3) * 1024,  # acoustic2d_fwi_gradient_32shots.md
+    ("acoustic3d_forward", "update_v"): (2.0 * 4.159e6 + 4.199e6) * 1024,    # acoustic3d_forward_256cube_16shots.md
+    ("acoustic3d_forward", "update_p"): (2.0 * 4.318e6 + 1.765e6) * 1024,
+    ("elastic3d_forward", "stress"): (2.0 * 5.952e6 + 2.686e6) * 1024,      # elastic3d_forward_200x200x150_8shots.md
+    ("elastic3d_forward", "velocity"): (2.0 * 4.972e6 + 1.602e6) * 1024,
+    ("fsg2d_forward", "stress"): (2.0 * 9.831e5 + 4.813e5) * 1024,          # fsg2d_forward_801x401_64shots.md
+    ("fsg2d_forward", "velocity"): (2.0 * 6.046e5 + 3.493e5) * 1024,
+}
+
+
+def with_traffic(workload, roof):
+    """The roofline with the measured HBM bytes per launch of its kernel."""
+    return dict(roof, traffic=TRAFFIC.get((workload, roof["kernel"])))
+
+
 CFG = dict(nx=401, nz=401, dx=10.0, dz=10.0, dt=0.001, tmax=1.999, pml=40, fdhalf=4)
 SNAPSHOTS = 40
 
@@ -156,7 +178,7 @@ def bench_3d(sync, steps=1500, shape=None):
     return {"workload": f"3D acoustic batched forward, 256^3, 16 shots (windows 128-256), {steps} steps, "
                         "dt 0.9 ms, R 4, pml 20",
             "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
-            "launch_shape": {"planes": kc, "variant": var}, "roofline": roof,
+            "launch_shape": {"planes": kc, "variant": var}, "roofline": with_traffic("acoustic3d_forward", roof),
             "kernels": kernel_table(stats, 1)}
 
 
@@ -184,7 +206,8 @@ def bench_elastic3d(sync, steps):
     _, roof = roofline_of(stats)
     return {"workload": f"3D isotropic elastic batched forward, 200x200x150, 10 m, dt 0.8 ms, R 4, pml 20 on five "
                         f"faces, flat free surface, 8 shots, 50x50 3-C receivers, {steps} steps",
-            "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms, "roofline": roof,
+            "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
+            "roofline": with_traffic("elastic3d_forward", roof),
             "kernels": kernel_table(stats, 1)}
 
 
@@ -215,7 +238,8 @@ def bench_fsg(sync, steps):
     return {"workload": f"2D tilted-TI FSG batched forward, 801x401, 5 m, dt 0.4 ms, R 4, pml 20, 64 shots "
                         f"(windows 200-801), {steps} steps, no free surface",
             "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
-            "plan_seconds": round(plan_s, 3), "roofline": roof, "kernels": kernel_table(stats, 1)}
+            "plan_seconds": round(plan_s, 3), "roofline": with_traffic("fsg2d_forward", roof),
+            "kernels": kernel_table(stats, 1)}
 
 
 def main():
@@ -404,7 +428,7 @@ def main():
         "owl_algorithm": {"batched_s_per_shot": wowl / nshot, "plain_s_per_shot": plain_owl_s,
                           "speedup": plain_owl_s / (wowl / nshot)},
         "autograd_s_per_shot": w_autograd / nshot,
-        "roofline": dict(roof, traffic=None),
+        "roofline": with_traffic("acoustic2d_gradient", roof),
         "kernels": kernel_table(stats, 1),
         "timing": "value and ms_per_step: graph-replayed gradients; roofline and kernels: HIP events "
                   "of one eager gradient run right after the timed region",

@@ -78,6 +78,14 @@ struct Tap {
     int field;      // 0 vx, 1 vy, 2 vz
 };
 
+// One receiver component's Kaiser-sinc box: the separable weights along x,
+// y, z (0 = tap left out) and the cell of offsets (-kNkw, -kNkw, -kNkw),
+// which may lie outside the grid where those weights are 0.
+struct RecBox {
+    long long cell0;
+    float w[3][2 * kNkw + 1];
+};
+
 __device__ __forceinline__ size_t ate(const ShotE& S, int lo, int i, int j, int k) {
     return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * size_t(S.px) + size_t(k - lo) * size_t(S.pxy);
 }
@@ -466,20 +474,37 @@ __global__ __launch_bounds__(1024) void ke_inject(const ShotE* __restrict__ shot
 }
 
 // Receivers of step t: vx at (hx, gy, gz), vy at (gx, hy, gz), vz at
-// (gx, gy, hz) (submodule_forward_modeling.f90:75-140).
+// (gx, gy, hz) (submodule_forward_modeling.f90:75-140).  The taps of a box
+// are summed z, y, x outermost to innermost, skipping zero weights, as OWL
+// lists them.
 __global__ __launch_bounds__(256) void ke_record(const ShotE* __restrict__ shots, EF F,
-                                                 const int* __restrict__ rec_beg, const Tap* __restrict__ rec,
+                                                 const RecBox* __restrict__ boxes,
                                                  const float* __restrict__ rec_w, float* traces, int nt, int t) {
+    constexpr int K = 2 * kNkw + 1;
     const ShotE S = shots[blockIdx.y];
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < 3 * S.nr; r += gridDim.x * blockDim.x) {
         const int comp = r / S.nr, ir = r % S.nr;
-        const int g = 3 * S.rec0 + comp * S.nr + ir;
+        const RecBox& B = boxes[3 * S.rec0 + comp * S.nr + ir];
         const float w = rec_w[S.rec0 + ir];
         const float* f = comp == 0 ? F.vx : (comp == 1 ? F.vy : F.vz);
+        float wx[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) wx[k] = B.w[0][k];
         float acc = 0.0f;
-        for (int k = rec_beg[g]; k < rec_beg[g + 1]; ++k) {
-            const Tap tp = rec[k];
-            acc = acc + f[tp.cell] * tp.w1 * tp.w2 * tp.w3 * w;
+        for (int kz = 0; kz < K; ++kz) {
+            const float w3 = B.w[2][kz];
+            if (w3 == 0.f) continue;
+            for (int ky = 0; ky < K; ++ky) {
+                const float w2 = B.w[1][ky];
+                if (w2 == 0.f) continue;
+                const float* row = f + (B.cell0 + (long long)ky * S.px + (long long)kz * S.pxy);
+#pragma unroll
+                for (int kx = 0; kx < K; ++kx) {
+                    const float w1 = wx[kx];
+                    if (w1 == 0.f) continue;
+                    acc = acc + row[kx] * w1 * w2 * w3 * w;
+                }
+            }
         }
         traces[S.trace0 + size_t(comp) * nt * S.nr + size_t(t - 1) * S.nr + ir] = acc;
     }
@@ -778,11 +803,12 @@ struct owl_elastic3d {
     Profiler prof;
     std::vector<float> vp_full, vs_full, rho_full;
     DevBuf<ShotE> d_shots;
-    DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg;
+    DevBuf<int> d_tile_shot, d_inj_beg;
     DevBuf<float> d_prof, d_amp, d_rec_w, d_traces;
     DevBuf<float> med[9];   // c11, c12, mu_yz, mu_xz, mu_xy, rho_x, rho_y, rho_z, c44 at the nodes
     DevBuf<float> fld[27];  // nine wavefields, eighteen memory variables
-    DevBuf<Tap> d_inj, d_rec;
+    DevBuf<Tap> d_inj;
+    DevBuf<RecBox> d_rec_box;
     // gradient: boundary records, the other half of the reverse ping-pong,
     // the adjoint field, adjoint-source CSR, accumulators
     DevBuf<float> d_strips, d_adj, d_obs;
@@ -874,7 +900,7 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
     std::vector<float> profs, amp, rec_w;
     std::vector<int> inj_beg{0};
     std::vector<Tap> inj;
-    std::vector<std::vector<Tap>> rec_taps;  // per (shot, component, receiver)
+    std::vector<RecBox> boxes;  // per (shot, component, receiver)
     std::vector<int> adj_beg{0};
     std::vector<Tap> adj_taps;
     b->max_nadj = 0;
@@ -1011,6 +1037,8 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
         // receivers: three components (outside the window: weight 0)
         S.rec0 = b->nrec_total;
         std::vector<std::vector<Tap>> comp(3 * S.nr);
+        const size_t box0 = boxes.size();
+        boxes.resize(box0 + 3 * size_t(S.nr), RecBox{});
         for (int r = 0; r < S.nr; ++r) {
             const owl_receiver& rc = H.rec[r];
             const bool valid = rc.x >= H.ox && rc.x <= xmax && rc.y >= H.oy && rc.y <= ymax && rc.z >= H.oz &&
@@ -1023,7 +1051,7 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
             const float* wx[3] = {L.hxw, L.ix, L.ix};
             const float* wy[3] = {L.iy, L.hyw, L.iy};
             const float* wz[3] = {L.iz, L.iz, L.hzw};
-            for (int c = 0; c < 3; ++c)
+            for (int c = 0; c < 3; ++c) {
                 for (int irz = -kNkw; irz <= kNkw; ++irz)
                     for (int iry = -kNkw; iry <= kNkw; ++iry)
                         for (int irx = -kNkw; irx <= kNkw; ++irx) {
@@ -1033,6 +1061,16 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
                                 Tap{(long long)cell_e(b, S, gx[c] + irx, gy[c] + iry, gz[c] + irz), w1, w2, w3, r, 0, 0,
                                     c});
                         }
+                RecBox& B = boxes[box0 + size_t(c) * S.nr + r];
+                const long long lo = 1 - b->pml - b->R;
+                B.cell0 = S.off + (gx[c] - kNkw - lo) + (gy[c] - kNkw - lo) * (long long)S.px +
+                          (gz[c] - kNkw - lo) * S.pxy;
+                for (int k = 0; k < 2 * kNkw + 1; ++k) {
+                    B.w[0][k] = wx[c][k];
+                    B.w[1][k] = wy[c][k];
+                    B.w[2][k] = wz[c][k];
+                }
+            }
         }
         // adjoint sources: the receivers' taps, below the free surface only
         // (submodule_fwi_gradient.f90:256-330), grouped by (component, cell)
@@ -1062,7 +1100,6 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
             if (!at.empty()) adj_beg.push_back(int(adj_taps.size()));
             b->max_nadj = std::max(b->max_nadj, S.nadj);
         }
-        for (auto& v : comp) rec_taps.push_back(std::move(v));
         b->nrec_total += S.nr;
         b->max_nr = std::max(b->max_nr, S.nr);
     }
@@ -1072,13 +1109,6 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
     b->trace_elems = size_t(trace);
     b->strip_elems = size_t(strip);
     b->grad_elems = size_t(grad);
-    // receiver CSR in (shot, component, receiver) order: entry 3*rec0 + c*nr + r
-    std::vector<int> rec_beg{0};
-    std::vector<Tap> rec;
-    for (auto& v : rec_taps) {
-        rec.insert(rec.end(), v.begin(), v.end());
-        rec_beg.push_back(int(rec.size()));
-    }
     if (!b->stream) {
         OWL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         OWL_HIP(hipEventCreate(&b->ev0));
@@ -1093,9 +1123,9 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
     b->d_inj_beg.upload(inj_beg, stm);
     if (inj.empty()) inj.push_back(Tap{});
     b->d_inj.upload(inj, stm);
-    b->d_rec_beg.upload(rec_beg, stm);
-    if (rec.empty()) rec.push_back(Tap{});
-    b->d_rec.upload(rec, stm);
+    // receiver boxes in (shot, component, receiver) order: entry 3*rec0 + c*nr + r
+    if (boxes.empty()) boxes.push_back(RecBox{});
+    b->d_rec_box.upload(boxes, stm);
     b->d_rec_w.upload(rec_w, stm);
     b->d_adj_beg.upload(adj_beg, stm);
     if (adj_taps.empty()) adj_taps.push_back(Tap{});
@@ -1253,8 +1283,8 @@ void run_forward_e(owl_elastic3d* b, bool save_boundary) {
         if (save_boundary) strips_e(b, R, F.vx, F.vy, F.vz, t, 1);
         if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 2);
         int ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(ke_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->d_rec_beg.get(),
-                           b->d_rec.get(), b->d_rec_w.get(), b->d_traces.get(), b->nt, t);
+        hipLaunchKernelGGL(ke_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->d_rec_box.get(),
+                           b->d_rec_w.get(), b->d_traces.get(), b->nt, t);
         b->prof.end(st, ev);
         if (t == b->nt) {
             // the last update changes nothing that is recorded; the state at
