@@ -267,7 +267,10 @@ __global__ __launch_bounds__(256) void k_step_fused(const Shot* __restrict__ sho
                                                     const float* __restrict__ prof, Coef c, float dt,
                                                     float idx, float idz, int pml, int zc) {
     constexpr int W = 64 - (2 * R - 1);
-    const int tile = int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y);
+    // one wavefront per tile (blockDim.x = 64): the tile, its shot and every
+    // row index are wavefront-uniform, so they live in scalar registers and
+    // the per-row PML coefficients come through scalar loads
+    const int tile = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y));
     if (tile >= nftile) return;
     const Shot S = shots[ftile_shot[tile]];
     const int lt = tile - S.ftile0;
@@ -852,7 +855,10 @@ __global__ __launch_bounds__(256) void k_adj_fused(const Shot* __restrict__ shot
                                                    const float* __restrict__ vx_cur, const float* __restrict__ vx_prev,
                                                    const float* __restrict__ vz_cur, const float* __restrict__ vz_prev) {
     constexpr int W = 64 - (2 * R - 1);
-    const int tile = int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y);
+    // one wavefront per tile (blockDim.x = 64): the tile, its shot and every
+    // row index are wavefront-uniform, so they live in scalar registers and
+    // the per-row PML coefficients come through scalar loads
+    const int tile = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y));
     if (tile >= nftile) return;
     const Shot S = shots[ftile_shot[tile]];
     const int lt = tile - S.atile0;
