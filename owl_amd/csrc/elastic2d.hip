@@ -133,7 +133,9 @@ __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ sho
                                                   EF2 F, EV2 O, EM2 M, const float* __restrict__ prof, Coef c, float dt,
                                                   float idx, float idz, int pml, int kc) {
     constexpr int W = 64 - 2 * R;
-    const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
+    // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
+    // are wavefront-uniform and live in scalar registers
+    const int tile = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y));
     const ShotE2 S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
@@ -212,7 +214,9 @@ __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ s
                                                     const float* __restrict__ prof, Coef c, float dt, float idx,
                                                     float idz, int pml, int kc) {
     constexpr int W = 64 - 2 * R;
-    const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
+    // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
+    // are wavefront-uniform and live in scalar registers
+    const int tile = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y));
     const ShotE2 S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;

@@ -187,7 +187,8 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
     const int lo = 1 - pml - R;
     const int l = int(threadIdx.x);
     const int i = -pml + 1 + tx * W - R + l;
-    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    // blockDim.x = 64: the row is wavefront-uniform (scalar registers)
+    const int j = -pml + 1 + ty * int(blockDim.y) + __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     if (j > ny + pml) return;  // whole wavefront
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 1) : kt;
@@ -316,7 +317,8 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
     const int lo = 1 - pml - R;
     const int l = int(threadIdx.x);
     const int i = -pml + 1 + tx * W - R + l;
-    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    // blockDim.x = 64: the row is wavefront-uniform (scalar registers)
+    const int j = -pml + 1 + ty * int(blockDim.y) + __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     if (j > ny + pml) return;  // whole wavefront
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 1) : kt;

@@ -163,7 +163,9 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
                                                     FF F, FV O, FM M, FP P, Coef c, float dt, float dx, float dz,
                                                     int pml, int kc) {
     constexpr int W = 64 - 2 * R;
-    const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
+    // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
+    // are wavefront-uniform and live in scalar registers
+    const int tile = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y));
     const ShotF S = shots[tile_shot[tile]];
     const int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
@@ -260,7 +262,9 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
                                                       const int* __restrict__ tile_shot, FF F, FV O, FM M, FP P,
                                                       Coef c, float sgn, float dx, float dz, int pml, int kc) {
     constexpr int W = 64 - 2 * R;
-    const int tile = blockIdx.x * int(blockDim.y) + int(threadIdx.y);
+    // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
+    // are wavefront-uniform and live in scalar registers
+    const int tile = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * int(blockDim.y) + int(threadIdx.y));
     const ShotF S = shots[tile_shot[tile]];
     const int lt = tile - S.tile0;
     const int tx = lt % S.ntx, tz = lt / S.ntx;
