@@ -116,7 +116,8 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_p(const Shot* __restrict__
     const size_t pitch = size_t(S.pitch);
 #pragma unroll
     for (int k = 0; k < kRows / kBY; ++k) {
-        const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
+        // kBX = 64: the row is wavefront-uniform (scalar registers)
+        const int j = -pml + 1 + tz * kRows + __builtin_amdgcn_readfirstlane(int(threadIdx.y)) + kBY * k;
         if (j > nz + pml) break;
         if (free_surface && j < 2) continue;
         const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
@@ -196,7 +197,8 @@ __global__ __launch_bounds__(kBX * kBY) void k_update_v(const Shot* __restrict__
     const size_t pitch = size_t(S.pitch);
 #pragma unroll
     for (int k = 0; k < kRows / kBY; ++k) {
-        const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
+        // kBX = 64: the row is wavefront-uniform (scalar registers)
+        const int j = -pml + 1 + tz * kRows + __builtin_amdgcn_readfirstlane(int(threadIdx.y)) + kBY * k;
         if (j > nz + pml) break;
         const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
         if (!free_surface || j >= 1) {
@@ -738,7 +740,8 @@ __global__ __launch_bounds__(kBX * kBY) void k_adj_p(const Shot* __restrict__ sh
     const size_t pitch = size_t(S.pitch);
 #pragma unroll
     for (int k = 0; k < kRows / kBY; ++k) {
-        const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
+        // kBX = 64: the row is wavefront-uniform (scalar registers)
+        const int j = -pml + 1 + tz * kRows + __builtin_amdgcn_readfirstlane(int(threadIdx.y)) + kBY * k;
         if (j > nz + pml) break;
         const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
         float sx = 0.0f, sz = 0.0f;
@@ -795,7 +798,8 @@ __global__ __launch_bounds__(kBX * kBY) void k_adj_v(const Shot* __restrict__ sh
     const size_t pitch = size_t(S.pitch);
 #pragma unroll
     for (int k = 0; k < kRows / kBY; ++k) {
-        const int j = -pml + 1 + tz * kRows + int(threadIdx.y) + kBY * k;
+        // kBX = 64: the row is wavefront-uniform (scalar registers)
+        const int j = -pml + 1 + tz * kRows + __builtin_amdgcn_readfirstlane(int(threadIdx.y)) + kBY * k;
         if (j > nz + pml) break;
         const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * pitch;
         const float rx = 0.5f * (rho[mq(S, q)] + rho[mq(S, q) - 1]);

@@ -369,7 +369,8 @@ __global__ __launch_bounds__(1024) void k3c_update_p(const Shot3* __restrict__ s
     const int nx = S.nx, ny = S.ny, nz = S.nz;
     const int lo = 1 - pml - R;
     const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    // blockDim.x = 64: the row is wavefront-uniform (scalar registers)
+    const int j = -pml + 1 + ty * int(blockDim.y) + __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     if (i > nx + pml || j > ny + pml) return;
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 2) : kt;
@@ -439,7 +440,8 @@ __global__ __launch_bounds__(1024) void k3c_update_v(const Shot3* __restrict__ s
     const int nx = S.nx, ny = S.ny, nz = S.nz;
     const int lo = 1 - pml - R;
     const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    // blockDim.x = 64: the row is wavefront-uniform (scalar registers)
+    const int j = -pml + 1 + ty * int(blockDim.y) + __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     if (i > nx + pml || j > ny + pml) return;
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 1) : kt;
@@ -522,7 +524,8 @@ __global__ __launch_bounds__(1024) void k3q_update_p(const Shot3* __restrict__ s
     const int nx = S.nx, ny = S.ny, nz = S.nz;
     const int lo = 1 - pml - R;
     const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    // blockDim.x = 64: the row is wavefront-uniform (scalar registers)
+    const int j = -pml + 1 + ty * int(blockDim.y) + __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     if (i > nx + pml || j > ny + pml) return;
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 2) : kt;
@@ -627,7 +630,8 @@ __global__ __launch_bounds__(1024) void k3q_update_v(const Shot3* __restrict__ s
     const int nx = S.nx, ny = S.ny, nz = S.nz;
     const int lo = 1 - pml - R;
     const int i = -pml + 1 + tx * 64 + int(threadIdx.x);
-    const int j = -pml + 1 + ty * int(blockDim.y) + int(threadIdx.y);
+    // blockDim.x = 64: the row is wavefront-uniform (scalar registers)
+    const int j = -pml + 1 + ty * int(blockDim.y) + __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     if (i > nx + pml || j > ny + pml) return;
     const int kt = -pml + 1 + tzc * kc, k1 = min(kt + kc - 1, nz + pml);
     const int k0 = fs ? max(kt, 1) : kt;
