@@ -189,14 +189,41 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
         qzi[m] = mcol ? ldo(F.vz_ih, r) : 0.0f;
     }
     const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
-    for (int j = j0; j <= j1; ++j, q += szb) {
-        {
-            const unsigned r = q + unsigned(R) * szb;
-            qxh[2 * R] = mcol ? ldo(F.vx_hi, r) : 0.0f;
-            qzh[2 * R] = mcol ? ldo(F.vz_hi, r) : 0.0f;
-            qxi[2 * R] = mcol ? ldo(F.vx_ih, r) : 0.0f;
-            qzi[2 * R] = mcol ? ldo(F.vz_ih, r) : 0.0f;
+    // The inputs of one row: the queues' fronts (row j + R) and, on the
+    // owned lanes, both grids' moduli and old stresses.  Row j + 1's are
+    // loaded while row j is computed.
+    struct Row {
+        float fxh, fzh, fxi, fzi;
+        float ci[6], ch[6], si[3], sh[3];
+    };
+    auto load = [&](unsigned qr, Row& w) {
+        const unsigned r = qr + unsigned(R) * szb;
+        w.fxh = mcol ? ldo(F.vx_hi, r) : 0.0f;
+        w.fzh = mcol ? ldo(F.vz_hi, r) : 0.0f;
+        w.fxi = mcol ? ldo(F.vx_ih, r) : 0.0f;
+        w.fzi = mcol ? ldo(F.vz_ih, r) : 0.0f;
+        if (own) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                w.ci[k] = ldo(M.ci[k], qr);
+                w.ch[k] = ldo(M.ch[k], qr);
+            }
+            w.si[0] = ldo(F.sxx_ii, qr);
+            w.si[1] = ldo(F.szz_ii, qr);
+            w.si[2] = ldo(F.sxz_ii, qr);
+            w.sh[0] = ldo(F.sxx_hh, qr);
+            w.sh[1] = ldo(F.szz_hh, qr);
+            w.sh[2] = ldo(F.sxz_hh, qr);
         }
+    };
+    Row cur{}, nxt{};
+    load(q, cur);
+    for (int j = j0; j <= j1; ++j, q += szb) {
+        if (j < j1) load(q + szb, nxt);
+        qxh[2 * R] = cur.fxh;
+        qzh[2 * R] = cur.fzh;
+        qxi[2 * R] = cur.fxi;
+        qzi[2 * R] = cur.fzi;
         const float sxh = sten_lanes<R>(qxh[R], l, 0, c), szh = sten_lanes<R>(qzh[R], l, 0, c);
         const float sxi = sten_lanes<R>(qxi[R], l, -1, c), szi = sten_lanes<R>(qzi[R], l, -1, c);
         if (own) {
@@ -212,12 +239,12 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
                     pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
                     pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
                 }
-                const float c11 = ldo(M.ci[0], q), c13 = ldo(M.ci[1], q), c15 = ldo(M.ci[2], q);
-                const float c33 = ldo(M.ci[3], q), c35 = ldo(M.ci[4], q), c55 = ldo(M.ci[5], q);
+                const float c11 = cur.ci[0], c13 = cur.ci[1], c15 = cur.ci[2];
+                const float c33 = cur.ci[3], c35 = cur.ci[4], c55 = cur.ci[5];
                 const float sh = pdxvz + pdzvx;
-                sto(O.sxx_ii, q, ldo(F.sxx_ii, q) + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
-                sto(O.szz_ii, q, ldo(F.szz_ii, q) + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
-                sto(O.sxz_ii, q, ldo(F.sxz_ii, q) + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
+                sto(O.sxx_ii, q, cur.si[0] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
+                sto(O.szz_ii, q, cur.si[1] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
+                sto(O.sxz_ii, q, cur.si[2] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
             }
             {   // corner grid
                 float pdxvx = sxi / dx;
@@ -231,12 +258,12 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
                     pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
                     pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
                 }
-                const float c11 = ldo(M.ch[0], q), c13 = ldo(M.ch[1], q), c15 = ldo(M.ch[2], q);
-                const float c33 = ldo(M.ch[3], q), c35 = ldo(M.ch[4], q), c55 = ldo(M.ch[5], q);
+                const float c11 = cur.ch[0], c13 = cur.ch[1], c15 = cur.ch[2];
+                const float c33 = cur.ch[3], c35 = cur.ch[4], c55 = cur.ch[5];
                 const float sh = pdxvz + pdzvx;
-                sto(O.sxx_hh, q, ldo(F.sxx_hh, q) + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
-                sto(O.szz_hh, q, ldo(F.szz_hh, q) + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
-                sto(O.sxz_hh, q, ldo(F.sxz_hh, q) + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
+                sto(O.sxx_hh, q, cur.sh[0] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
+                sto(O.szz_hh, q, cur.sh[1] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
+                sto(O.sxz_hh, q, cur.sh[2] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
             }
         }
 #pragma unroll
@@ -246,6 +273,7 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
             qxi[m] = qxi[m + 1];
             qzi[m] = qzi[m + 1];
         }
+        cur = nxt;
     }
 }
 
