@@ -315,15 +315,39 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
         qci[m] = mcol ? ldo(O.szz_ii, r) : 0.0f;
     }
     const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
-    for (int j = j0; j <= j1; ++j, q += szb) {
-        {
-            const unsigned r = q + unsigned(R) * szb;
-            qah[2 * R] = mcol ? ldo(O.sxz_hh, r) : 0.0f;
-            qch[2 * R] = mcol ? ldo(O.szz_hh, r) : 0.0f;
-            qai[2 * R] = mcol ? ldo(O.sxz_ii, r) : 0.0f;
-            qci[2 * R] = mcol ? ldo(O.szz_ii, r) : 0.0f;
+    // The inputs of one row: the queues' fronts (row j + R), both sxx, and on
+    // the owned lanes the buoyancies and old velocities.  Row j + 1's are
+    // loaded while row j is computed.
+    struct Row {
+        float fah, fch, fai, fci, xxi, xxh;
+        float qhx, qhz, vxhi, vzhi, vxih, vzih;
+    };
+    auto load = [&](unsigned qr, Row& w) {
+        const unsigned r = qr + unsigned(R) * szb;
+        w.fah = mcol ? ldo(O.sxz_hh, r) : 0.0f;
+        w.fch = mcol ? ldo(O.szz_hh, r) : 0.0f;
+        w.fai = mcol ? ldo(O.sxz_ii, r) : 0.0f;
+        w.fci = mcol ? ldo(O.szz_ii, r) : 0.0f;
+        w.xxi = mcol ? ldo(O.sxx_ii, qr) : 0.0f;
+        w.xxh = mcol ? ldo(O.sxx_hh, qr) : 0.0f;
+        if (own) {
+            w.qhx = ldo(M.qhx, qr);
+            w.qhz = ldo(M.qhz, qr);
+            w.vxhi = ldo(F.vx_hi, qr);
+            w.vzhi = ldo(F.vz_hi, qr);
+            w.vxih = ldo(F.vx_ih, qr);
+            w.vzih = ldo(F.vz_ih, qr);
         }
-        const float xxi = mcol ? ldo(O.sxx_ii, q) : 0.0f, xxh = mcol ? ldo(O.sxx_hh, q) : 0.0f;
+    };
+    Row cur{}, nxt{};
+    load(q, cur);
+    for (int j = j0; j <= j1; ++j, q += szb) {
+        if (j < j1) load(q + szb, nxt);
+        qah[2 * R] = cur.fah;
+        qch[2 * R] = cur.fch;
+        qai[2 * R] = cur.fai;
+        qci[2 * R] = cur.fci;
+        const float xxi = cur.xxi, xxh = cur.xxh;
         const float d_xxi = sten_lanes<R>(xxi, l, -1, c), d_xzi = sten_lanes<R>(qai[R], l, -1, c);
         const float d_xxh = sten_lanes<R>(xxh, l, 0, c), d_xzh = sten_lanes<R>(qah[R], l, 0, c);
         if (own) {
@@ -339,9 +363,9 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
                     pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
                     pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
                 }
-                const float w = sgn * ldo(M.qhx, q);
-                sto(O.vx_hi, q, ldo(F.vx_hi, q) + w * (pdxxx + pdzxz));
-                sto(O.vz_hi, q, ldo(F.vz_hi, q) + w * (pdxxz + pdzzz));
+                const float w = sgn * cur.qhx;
+                sto(O.vx_hi, q, cur.vxhi + w * (pdxxx + pdzxz));
+                sto(O.vz_hi, q, cur.vzhi + w * (pdxxz + pdzzz));
             }
             {   // (i, j - 1/2)
                 float pdxxx = d_xxh / dx;
@@ -355,9 +379,9 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
                     pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
                     pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
                 }
-                const float w = sgn * ldo(M.qhz, q);
-                sto(O.vx_ih, q, ldo(F.vx_ih, q) + w * (pdxxx + pdzxz));
-                sto(O.vz_ih, q, ldo(F.vz_ih, q) + w * (pdxxz + pdzzz));
+                const float w = sgn * cur.qhz;
+                sto(O.vx_ih, q, cur.vxih + w * (pdxxx + pdzxz));
+                sto(O.vz_ih, q, cur.vzih + w * (pdxxz + pdzzz));
             }
         }
 #pragma unroll
@@ -367,6 +391,7 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
             qai[m] = qai[m + 1];
             qci[m] = qci[m + 1];
         }
+        cur = nxt;
     }
 }
 
