@@ -214,13 +214,40 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
     }
     const bool xin = !(i <= 0 || i >= nx + 1), yin = !(j <= 0 || j >= ny + 1);
     const bool Xin = !(i <= 1 || i >= nx + 1), Yin = !(j <= 1 || j >= ny + 1);
-    for (int k = k0; k <= k1; ++k, q += szb) {
-        {
-            const unsigned r = q + unsigned(R) * szb;
-            qx[2 * R] = mcol ? ldo(vx, r) : 0.0f;
-            qy[2 * R] = mcol ? ldo(vy, r) : 0.0f;
-            qz[2 * R] = mcol ? ldo(vz, r) : 0.0f;
+    // The inputs of one plane: the queues' fronts (plane k + R) and, on the
+    // owned lanes, the moduli and old stresses.  Plane k + 1's are loaded
+    // while plane k is computed (the y stencils stay direct loads).
+    struct Plane {
+        float fx, fy, fz;
+        float c11, c12, myz, mxz, mxy;
+        float sxx, syy, szz, syz, sxz, sxy;
+    };
+    auto load = [&](unsigned qp, Plane& w) {
+        const unsigned r = qp + unsigned(R) * szb;
+        w.fx = mcol ? ldo(vx, r) : 0.0f;
+        w.fy = mcol ? ldo(vy, r) : 0.0f;
+        w.fz = mcol ? ldo(vz, r) : 0.0f;
+        if (own) {
+            w.c11 = ldo(M.c11, qp);
+            w.c12 = ldo(M.c12, qp);
+            w.myz = ldo(M.mu_yz, qp);
+            w.mxz = ldo(M.mu_xz, qp);
+            w.mxy = ldo(M.mu_xy, qp);
+            w.sxx = ldo(F.sxx, qp);
+            w.syy = ldo(F.syy, qp);
+            w.szz = ldo(F.szz, qp);
+            w.syz = ldo(F.syz, qp);
+            w.sxz = ldo(F.sxz, qp);
+            w.sxy = ldo(F.sxy, qp);
         }
+    };
+    Plane cur{}, nxt{};
+    load(q, cur);
+    for (int k = k0; k <= k1; ++k, q += szb) {
+        if (k < k1) load(q + szb, nxt);
+        qx[2 * R] = cur.fx;
+        qy[2 * R] = cur.fy;
+        qz[2 * R] = cur.fz;
         // x stencils of this plane from the lanes (all lanes take part)
         const float sx_vx = sten_lanes<R>(qx[R], l, 0, c);    // d+x vx
         const float sx_vz = sten_lanes<R>(qz[R], l, -1, c);   // d-x vz
@@ -233,7 +260,7 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
             // ---- normal stresses
             {
                 float pdxvx = idx * sx_vx, pdyvy = idy * sten_y<R>(vy, yo, 0, c), pdzvz;
-                const float c11 = ldo(M.c11, q), c12 = ldo(M.c12, q);
+                const float c11 = cur.c11, c12 = cur.c12;
                 if (fs && k == 1) {
                     if (!xin || !yin) {
                         pdxvx = damp_o(F.m_dxvx, q, pdxvx, P.dix[ii], P.aix[ii], adt);
@@ -251,9 +278,9 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
                         pdzvz = damp_o(F.m_dzvz, q, pdzvz, P.diz[kk], P.aiz[kk], adt);
                     }
                 }
-                sto(O.sxx, q, ldo(F.sxx, q) + dt * (c11 * pdxvx + c12 * pdyvy + c12 * pdzvz));
-                sto(O.syy, q, ldo(F.syy, q) + dt * (c12 * pdxvx + c11 * pdyvy + c12 * pdzvz));
-                sto(O.szz, q, ldo(F.szz, q) + dt * (c12 * pdxvx + c12 * pdyvy + c11 * pdzvz));
+                sto(O.sxx, q, cur.sxx + dt * (c11 * pdxvx + c12 * pdyvy + c12 * pdzvz));
+                sto(O.syy, q, cur.syy + dt * (c12 * pdxvx + c11 * pdyvy + c12 * pdzvz));
+                sto(O.szz, q, cur.szz + dt * (c12 * pdxvx + c12 * pdyvy + c11 * pdzvz));
             }
             // ---- shear stresses with a z derivative: syz, sxz (Z = k)
             if (!fs || k >= 2) {
@@ -270,12 +297,12 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
                     pdyvz = damp_o(F.m_dyvz, q, pdyvz, P.dhy[jj], P.ahy[jj], adt);
                     pdzvy = damp_o(F.m_dzvy, q, pdzvy, P.dhz[kk], P.ahz[kk], adt);
                 }
-                sto(O.syz, q, ldo(F.syz, q) + dt * ldo(M.mu_yz, q) * (pdzvy + pdyvz));
+                sto(O.syz, q, cur.syz + dt * cur.myz * (pdzvy + pdyvz));
                 if (!Xin || !yin || zpml) {
                     pdxvz = damp_o(F.m_dxvz, q, pdxvz, P.dhx[ii], P.ahx[ii], adt);
                     pdzvx = damp_o(F.m_dzvx, q, pdzvx, P.dhz[kk], P.ahz[kk], adt);
                 }
-                sto(O.sxz, q, ldo(F.sxz, q) + dt * ldo(M.mu_xz, q) * (pdzvx + pdxvz));
+                sto(O.sxz, q, cur.sxz + dt * cur.mxz * (pdzvx + pdxvz));
             }
             // ---- sxy (X, Y, k)
             {
@@ -284,7 +311,7 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
                     pdxvy = damp_o(F.m_dxvy, q, pdxvy, P.dhx[ii], P.ahx[ii], adt);
                     pdyvx = damp_o(F.m_dyvx, q, pdyvx, P.dhy[jj], P.ahy[jj], adt);
                 }
-                sto(O.sxy, q, ldo(F.sxy, q) + dt * ldo(M.mu_xy, q) * (pdyvx + pdxvy));
+                sto(O.sxy, q, cur.sxy + dt * cur.mxy * (pdyvx + pdxvy));
             }
         }
 #pragma unroll
@@ -293,6 +320,7 @@ __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots
             qy[m] = qy[m + 1];
             qz[m] = qz[m + 1];
         }
+        cur = nxt;
     }
 }
 
