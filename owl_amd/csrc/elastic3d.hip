@@ -374,15 +374,39 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
     }
     const bool xin = !(i <= 0 || i >= nx + 1), yin = !(j <= 0 || j >= ny + 1);
     const bool Xin = !(i <= 1 || i >= nx + 1), Yin = !(j <= 1 || j >= ny + 1);
-    for (int k = k0; k <= k1; ++k, q += szb) {
-        {
-            const unsigned r = q + unsigned(R) * szb;
-            qa[2 * R] = mcol ? ldo(sxz, r) : 0.0f;
-            qb[2 * R] = mcol ? ldo(syz, r) : 0.0f;
-            qc[2 * R] = mcol ? ldo(szz, r) : 0.0f;
+    // The inputs of one plane: the queues' fronts (plane k + R), sxx and sxy
+    // and, on the owned lanes, the densities and old velocities.  Plane
+    // k + 1's are loaded while plane k is computed (the y stencils stay
+    // direct loads).
+    struct Plane {
+        float fa, fb, fc, cxx, cxy;
+        float rx, ry, rz, vx, vy, vz;
+    };
+    auto load = [&](unsigned qp, Plane& w) {
+        const unsigned r = qp + unsigned(R) * szb;
+        w.fa = mcol ? ldo(sxz, r) : 0.0f;
+        w.fb = mcol ? ldo(syz, r) : 0.0f;
+        w.fc = mcol ? ldo(szz, r) : 0.0f;
+        w.cxx = mcol ? ldo(sxx, qp) : 0.0f;
+        w.cxy = mcol ? ldo(sxy, qp) : 0.0f;
+        if (own) {
+            w.rx = ldo(M.rx, qp);
+            w.ry = ldo(M.ry, qp);
+            w.rz = ldo(M.rz, qp);
+            w.vx = ldo(F.vx, qp);
+            w.vy = ldo(F.vy, qp);
+            w.vz = ldo(F.vz, qp);
         }
+    };
+    Plane cur{}, nxt{};
+    load(q, cur);
+    for (int k = k0; k <= k1; ++k, q += szb) {
+        if (k < k1) load(q + szb, nxt);
+        qa[2 * R] = cur.fa;
+        qb[2 * R] = cur.fb;
+        qc[2 * R] = cur.fc;
         // x stencils of this plane from the lanes (all lanes take part)
-        const float cxx = mcol ? ldo(sxx, q) : 0.0f, cxy = mcol ? ldo(sxy, q) : 0.0f;
+        const float cxx = cur.cxx, cxy = cur.cxy;
         const float sx_xx = sten_lanes<R>(cxx, l, -1, c);    // d-x sxx
         const float sx_xy = sten_lanes<R>(cxy, l, 0, c);     // d+x sxy
         const float sx_xz = sten_lanes<R>(qa[R], l, 0, c);   // d+x sxz
@@ -403,7 +427,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                     pdyxy = damp_o(F.m_dyxy, q, pdyxy, P.diy[jj], P.aiy[jj], adt);
                     pdzxz = damp_o(F.m_dzxz, q, pdzxz, P.diz[kk], P.aiz[kk], adt);
                 }
-                sto(O.vx, q, ldo(F.vx, q) + dt / ldo(M.rx, q) * (pdxxx + pdyxy + pdzxz));
+                sto(O.vx, q, cur.vx + dt / cur.rx * (pdxxx + pdyxy + pdzxz));
             }
             // ---- vy (i, Y, k)
             {
@@ -416,7 +440,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                     pdyyy = damp_o(F.m_dyyy, q, pdyyy, P.dhy[jj], P.ahy[jj], adt);
                     pdzyz = damp_o(F.m_dzyz, q, pdzyz, P.diz[kk], P.aiz[kk], adt);
                 }
-                sto(O.vy, q, ldo(F.vy, q) + dt / ldo(M.ry, q) * (pdxxy + pdyyy + pdzyz));
+                sto(O.vy, q, cur.vy + dt / cur.ry * (pdxxy + pdyyy + pdzyz));
             }
             // ---- vz (i, j, Z = k)
             if (!fs || k >= 2) {
@@ -430,7 +454,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
                     pdyyz = damp_o(F.m_dyyz, q, pdyyz, P.diy[jj], P.aiy[jj], adt);
                     pdzzz = damp_o(F.m_dzzz, q, pdzzz, P.dhz[kk], P.ahz[kk], adt);
                 }
-                sto(O.vz, q, ldo(F.vz, q) + dt / ldo(M.rz, q) * (pdxxz + pdyyz + pdzzz));
+                sto(O.vz, q, cur.vz + dt / cur.rz * (pdxxz + pdyyz + pdzzz));
             }
         }
 #pragma unroll
@@ -439,6 +463,7 @@ __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ sho
             qb[m] = qb[m + 1];
             qc[m] = qc[m + 1];
         }
+        cur = nxt;
     }
 }
 
