@@ -49,10 +49,10 @@ TRAFFIC = {
     ("acoustic2d_gradient", "adjoint_fused", 6): (2.0 * 433.9e3 + 306.3e3) * 1024,
     ("acoustic3d_forward", "update_v"): (2.0 * 4.159e6 + 4.199e6) * 1024,    # acoustic3d_forward_256cube_16shots.md
     ("acoustic3d_forward", "update_p"): (2.0 * 4.318e6 + 1.765e6) * 1024,
-    ("elastic3d_forward", "stress"): (2.0 * 5.954e6 + 2.686e6) * 1024,      # elastic3d_forward_200x200x150_8shots.md
-    ("elastic3d_forward", "velocity"): (2.0 * 4.974e6 + 1.602e6) * 1024,
-    ("fsg2d_forward", "stress"): (2.0 * 9.859e5 + 4.838e5) * 1024,          # fsg2d_forward_801x401_64shots.md
-    ("fsg2d_forward", "velocity"): (2.0 * 6.056e5 + 3.504e5) * 1024,
+    ("elastic3d_forward", "stress"): (2.0 * 6.017e6 + 2.686e6) * 1024,      # elastic3d_forward_200x200x150_8shots.md
+    ("elastic3d_forward", "velocity"): (2.0 * 5.066e6 + 1.602e6) * 1024,
+    ("fsg2d_forward", "stress"): (2.0 * 1.005e6 + 4.988e5) * 1024,          # fsg2d_forward_801x401_64shots.md
+    ("fsg2d_forward", "velocity"): (2.0 * 6.090e5 + 3.574e5) * 1024,
 }
 
 
