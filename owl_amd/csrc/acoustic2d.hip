@@ -1081,7 +1081,7 @@ __global__ void k_src_correction(const Shot* __restrict__ shots, const int* __re
 // The fused adjoint's tap work of reverse step t in one launch (one block
 // per shot): the source correction (reads ph at the source taps), then the
 // adjoint source injection into ph, in the order of the two kernels above.
-__global__ __launch_bounds__(256) void k_adjoint_taps(const Shot* __restrict__ shots,
+__global__ __launch_bounds__(512) void k_adjoint_taps(const Shot* __restrict__ shots,
                                                       const int* __restrict__ inj_cell_beg,
                                                       const Tap* __restrict__ inj_taps,
                                                       const float* __restrict__ amp_tab,
@@ -2181,7 +2181,7 @@ void adjoint_step(owl_acoustic2d* b, int t, const float* const* cur, const float
         b->adj_cur = 1 - b->adj_cur;
         if (b->max_ninj > 0 || b->max_nadj > 0) {
             const int ev2 = prof_begin(b, kKAdj);
-            hipLaunchKernelGGL(k_adjoint_taps, dim3(unsigned(b->sd.size())), dim3(256), 0, b->stream,
+            hipLaunchKernelGGL(k_adjoint_taps, dim3(unsigned(b->sd.size())), dim3(512), 0, b->stream,
                                b->d_shots.get(), b->d_inj_beg.get(), b->d_inj_taps.get(), b->d_amp.get(),
                                b->d_bk.get(), cur_ph(b), b->gK.get(), b->nt, b->d_adj_beg.get(),
                                b->d_adj_taps.get(), b->d_rec_w.get(), b->d_adj.get(), t);
