@@ -597,14 +597,14 @@ __global__ __launch_bounds__(256) void k_image_owl(const Shot* __restrict__ shot
 // L2 waveform misfit and adjoint source for one shot per block
 // (compute_adjsrc_waveform, module_inversion_adjoint_source.f90:371-383;
 // trace discard rule :1042-1054).  Traces are [nt][nr] per shot.
-__global__ __launch_bounds__(256) void k_adjoint_source(const Shot* __restrict__ shots,
+__global__ __launch_bounds__(512) void k_adjoint_source(const Shot* __restrict__ shots,
                                                         const float* __restrict__ syn,
                                                         const float* __restrict__ obs,
                                                         const float* __restrict__ rec_w,
                                                         float* adj, double* misfit, int nt,
                                                         float threshold, double* scratch) {
     const Shot S = shots[blockIdx.x];
-    __shared__ double red[256];
+    __shared__ double red[512];
     __shared__ double mo_s, ms_s;
     double* nobs = scratch + 2 * size_t(S.rec0);
     double* nsyn = nobs + S.nr;
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256) void k_adjoint_source(const Shot* __restrict__
     }
     red[threadIdx.x] = lmo;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 256; w > 0; w >>= 1) {
         if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(256) void k_adjoint_source(const Shot* __restrict__
     __syncthreads();
     red[threadIdx.x] = lms;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 256; w > 0; w >>= 1) {
         if (int(threadIdx.x) < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(256) void k_adjoint_source(const Shot* __restrict__
     }
     red[threadIdx.x] = sum + comp;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 256; w > 0; w >>= 1) {
         if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -2934,7 +2934,7 @@ owl_status owl_acoustic2d_gradient(owl_acoustic2d* b, const float* obs, int mode
         b->d_scratch.alloc(2 * size_t(std::max(1, b->nrec_total)));
         auto adjoint_source = [&] {
             const int ev = prof_begin(b, kKMisfit);
-            hipLaunchKernelGGL(k_adjoint_source, dim3(unsigned(b->sd.size())), dim3(256), 0, st, b->d_shots.get(),
+            hipLaunchKernelGGL(k_adjoint_source, dim3(unsigned(b->sd.size())), dim3(512), 0, st, b->d_shots.get(),
                                b->d_traces.get(), b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(),
                                b->d_misfit.get(), b->nt, 1.0e-6f, b->d_scratch.get());
             prof_end(b, ev);
