@@ -175,9 +175,15 @@ __device__ __forceinline__ Prof prof_of(const float* prof, const ShotE& S, int p
 template <int R>
 __global__ __launch_bounds__(256) void ke_stress(const ShotE* __restrict__ shots, const int* __restrict__ tile_shot,
                                                  EF F, EV O, EM M, const float* __restrict__ prof, Coef c, float dt,
-                                                 float idx, float idy, float idz, int pml, int kc, int fs) {
+                                                 float idx, float idy, float idz, int pml, int kc, int fs,
+                                                 int ntile) {
     constexpr int W = 64 - 2 * R;  // columns a wavefront updates; the other lanes feed its x stencils
-    const int tile = blockIdx.x;
+    // XCD-contiguous order: blocks are dealt to the 8 XCDs in turn, so block b
+    // takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous range
+    // of tiles, whose y neighbours then share its L2
+    const int per = (ntile + 7) / 8;
+    const int tile = int(blockIdx.x % 8u) * per + int(blockIdx.x / 8u);
+    if (tile >= ntile) return;
     const ShotE S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
     const int tx = lt % S.ntx;
@@ -333,9 +339,14 @@ template <int R>
 __global__ __launch_bounds__(256) void ke_velocity(const ShotE* __restrict__ shots,
                                                    const int* __restrict__ tile_shot, EF F, EV O, EM M,
                                                    const float* __restrict__ prof, Coef c, float dt, float idx,
-                                                   float idy, float idz, int pml, int kc, int fs) {
+                                                   float idy, float idz, int pml, int kc, int fs, int ntile) {
     constexpr int W = 64 - 2 * R;
-    const int tile = blockIdx.x;
+    // XCD-contiguous order: blocks are dealt to the 8 XCDs in turn, so block b
+    // takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous range
+    // of tiles, whose y neighbours then share its L2
+    const int per = (ntile + 7) / 8;
+    const int tile = int(blockIdx.x % 8u) * per + int(blockIdx.x / 8u);
+    if (tile >= ntile) return;
     const ShotE S = shots[tile_shot[tile]];
     int lt = tile - S.tile0;
     const int tx = lt % S.ntx;
@@ -1302,19 +1313,20 @@ void step_e(owl_elastic3d* b, const EF& F, const EV& O, float dt, double bs, dou
     hipStream_t st = b->stream;
     const unsigned nshot = unsigned(b->sd.size());
     if (64 * b->cy > 256) fail(OWL_ERR_ARG, "launch shape exceeds the sweeps' 256-thread blocks");
-    const dim3 blk(64, unsigned(b->cy)), grd(unsigned(b->tile_shot.size()));
+    const int ntile = int(b->tile_shot.size());
+    const dim3 blk(64, unsigned(b->cy)), grd(unsigned((ntile + 7) / 8 * 8));
     const EM M = media(b);
     if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 0);
     int ev = b->prof.begin(st, 0, bs);
     hipLaunchKernelGGL(ke_stress<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M, b->d_prof.get(),
-                       b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
+                       b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs, ntile);
     b->prof.end(st, ev);
     if (b->fs)
         hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), with_fields(F, O),
                            b->pml, 1);
     ev = b->prof.begin(st, 1, bv);
     hipLaunchKernelGGL(ke_velocity<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
-                       b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs);
+                       b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz, b->pml, b->kc, b->fs, ntile);
     b->prof.end(st, ev);
 }
 
