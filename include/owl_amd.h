@@ -33,6 +33,8 @@ typedef int owl_status;
 #define OWL_ERR_DEVICE (-3)    /* HIP runtime failure */
 #define OWL_ERR_NOMEM (-4)     /* device allocation failed */
 #define OWL_ERR_STATE (-5)     /* call out of order (e.g. adjoint before forward) */
+#define OWL_ERR_NONFINITE (-6) /* the wavefield became NaN or infinite (OWL stops with a
+                                  NaN check in verbose mode, module_wavefield.f90) */
 
 /* Source mechanisms and wavelets (module_source_receiver.f90:76-80, 1036-1061). */
 #define OWL_EXPLOSION 0
@@ -86,6 +88,11 @@ typedef struct owl_acoustic_config {
     int free_surface;        /* 2D only */
     float f0factor;          /* default 1.0 (module_source_receiver.f90:134) */
     int cc_step_interval;    /* imaging every k-th reverse step, default 1 */
+    float trace_discard_threshold; /* module_parameters.f90:306, 570; OWL's default 1.0e-6.
+                                      A trace is left out of the misfit and adjoint source when
+                                      its norm is below this fraction of the largest one
+                                      (module_inversion_adjoint_source.f90:1042-1054); 0 keeps
+                                      every trace with a non-zero norm */
 } owl_acoustic_config;
 
 /* Gradient algorithms. */
@@ -130,6 +137,29 @@ int owl_source_time_function(int wavelet, float f0, float dt, float f0factor, fl
  * nodes).  kappa is 1 everywhere. */
 owl_status owl_cfspml_axis(int n, int pml, float d, float dt, float pmlvp, float f0max, float *ai,
                            float *bi, float *ah, float *bh);
+
+/* Parameters of the per-shot aperture window (module_parameters.f90:90-91,
+ * 123-129, 419-420, 463-468; model extents module_model.f90:97-111).  2D:
+ * ny = 1, dy = 1, yn_adpy = 0 and every y coordinate 0. */
+typedef struct owl_window_params {
+    int nx, ny, nz;                 /* full model */
+    float dx, dy, dz;
+    float ox, oy, oz;
+    float xmin, xmax, ymin, ymax, zmin, zmax;   /* model extent (default ox .. ox+(nx-1)dx) */
+    int yn_adpx, yn_adpy, yn_adpz;  /* adaptive range along each axis (default 0) */
+    float adp_extrax, adp_extray, adp_extraz;   /* margin added around the shot (default 0) */
+    float offset_min, offset_max;   /* absolute-offset limits (default 0, float_huge) */
+} owl_window_params;
+
+/* set_adaptive_range (module_utility.f90:410-510) together with the
+ * absolute-offset rule of prepare_geometry (module_source_receiver.f90:
+ * 421-436), float32 as OWL computes them: zeroes the weight of receivers
+ * outside [offset_min, offset_max] and outside the window, and returns the
+ * window as 1-based inclusive node ranges window[6] = nxbeg, nxend, nybeg,
+ * nyend, nzbeg, nzend (for owl_shot) and, if range is not NULL, its
+ * coordinates range[6] = xbeg, xend, ybeg, yend, zbeg, zend. */
+owl_status owl_set_adaptive_range(const owl_window_params *p, int ns, const owl_source *src, int nr,
+                                  owl_receiver *rec, int *window, float *range);
 
 /* ---- 2D acoustic: batched shots on one GPU --------------------------------
  * Replaces the per-shot loops of forward_modeling
@@ -181,6 +211,13 @@ owl_status owl_acoustic2d_forward_store(owl_acoustic2d *b, float *traces, int ck
                                         int store);
 owl_status owl_acoustic2d_backward(owl_acoustic2d *b, const float *dtraces, double *grad_vp,
                                    double *grad_rho);
+/* Identifier of the forward states that the batch holds for
+ * owl_acoustic2d_backward: a new positive value after every
+ * owl_acoustic2d_forward_store, 0 when none are held (after set_model, a
+ * gradient call, or before any forward_store).  A caller that keeps several
+ * differentiable evaluations compares it before calling backward.  No OWL
+ * counterpart. */
+owl_status owl_acoustic2d_store_id(const owl_acoustic2d *b, long long *id);
 
 /* Adjoint pass alone, OWL's algorithm, on data from a preceding
  * owl_acoustic2d_forward(..., save_boundary = 1): adjoint source traces adj

@@ -99,6 +99,14 @@ MECH = {"explosion": 0, "force": 1}
 WAVELET = {"ricker": 0, "gaussian": 1, "gaussian_deriv": 2, "ricker_deriv": 3}
 
 
+class WindowPar(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
+        [(k, C.c_float) for k in ("dx", "dy", "dz", "ox", "oy", "oz", "xmin", "xmax", "ymin", "ymax",
+                                  "zmin", "zmax")] + \
+        [(k, C.c_int) for k in ("yn_adpx", "yn_adpy", "yn_adpz")] + \
+        [(k, C.c_float) for k in ("adp_extrax", "adp_extray", "adp_extraz", "offset_min", "offset_max")]
+
+
 def build():
     """Compile both oracle builds (float32 and float64) in place."""
     import subprocess
@@ -147,6 +155,40 @@ class Oracle:
 
     def _fn(self, name):
         return getattr(self.lib, self.p + name)
+
+    # -- per-shot aperture window (set_adaptive_range) ---------------------
+    def set_adaptive_range(self, par, sources, receivers):
+        """OWL's window and receiver weights: par holds the owl_window_params
+        fields (missing ones take OWL's defaults); sources/receivers are
+        dicts with x, y, z (and weight).  Returns (window[6], range[6],
+        weights)."""
+        F = np.float32
+        huge = float(np.finfo(np.float32).max)
+        nx, ny, nz = int(par["nx"]), int(par.get("ny", 1)), int(par["nz"])
+        dx, dy, dz = F(par["dx"]), F(par.get("dy", 1.0)), F(par["dz"])
+        ox, oy, oz = F(par.get("ox", 0.0)), F(par.get("oy", 0.0)), F(par.get("oz", 0.0))
+        vals = [nx, ny, nz, dx, dy, dz, ox, oy, oz,
+                par.get("xmin", ox), par.get("xmax", ox + F(nx - 1) * dx),
+                par.get("ymin", oy), par.get("ymax", oy + F(ny - 1) * dy),
+                par.get("zmin", oz), par.get("zmax", oz + F(nz - 1) * dz),
+                int(par.get("yn_adpx", 0)), int(par.get("yn_adpy", 0)), int(par.get("yn_adpz", 0)),
+                par.get("adp_extrax", 0.0), par.get("adp_extray", 0.0), par.get("adp_extraz", 0.0),
+                par.get("offset_min", 0.0), par.get("offset_max", huge)]
+        P = WindowPar(*vals)
+        col = lambda items, k, d=0.0: np.ascontiguousarray([it.get(k, d) for it in items], np.float32)  # noqa: E731
+        sx, sy, sz = col(sources, "x"), col(sources, "y"), col(sources, "z")
+        rx, ry, rz = col(receivers, "x"), col(receivers, "y"), col(receivers, "z")
+        w = col(receivers, "weight", 1.0)
+        win = np.zeros(6, np.int32)
+        rng = np.zeros(6, np.float32)
+        v = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        fn = self.lib.ora_set_adaptive_range
+        fn.restype = C.c_int
+        rc = fn(C.byref(P), len(sources), v(sx), v(sy), v(sz), len(receivers), v(rx), v(ry), v(rz), v(w),
+                v(win), v(rng))
+        if rc != 0:
+            raise ValueError("set_adaptive_range needs at least one source")
+        return win, rng, w
 
     # -- scalar / small helpers ------------------------------------------
     def fdcoefs(self, R):

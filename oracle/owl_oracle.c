@@ -1321,6 +1321,7 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
     real *psx = zalloc(n), *psz = zalloc(n), *phx = zalloc(n), *phz = zalloc(n);
     real *ex = zalloc(n), *ez = zalloc(n), *dxa = zalloc(n), *dza = zalloc(n);
     double *gK = (double *)calloc(n, sizeof(double)), *gR = (double *)calloc(n, sizeof(double));
+    real *gxa = zalloc(n), *gza = zalloc(n);
 #define INJECT_RT(tt)                                                                         \
     for (int ir = 0; ir < nr; ++ir) {                                                         \
         if (rec[ir].weight == 0.0f) continue;                                                 \
@@ -1336,7 +1337,10 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
     for (int t = nt - 1; t >= 1; --t) {
         const real *pc = PP + n * t, *pv = P + n * (t - 1);
         const real *vxc = VX + n * t, *vxp = VX + n * (t - 1), *vzc = VZ + n * t, *vzp = VZ + n * (t - 1);
-        /* (a) transpose of the v-update: e = -(dt/rho_eff) v^ (+ PML), phi^ update */
+        /* (a) transpose of the v-update: e = -(dt/rho_eff) v^ (+ PML), phi^ update.
+         * The density terms of a cell go to gx/gz first and are summed into
+         * gR by the pass after, so that rows can run in parallel. */
+#pragma omp parallel for schedule(static)
         for (int j = -pml + 1; j <= nz + pml; ++j)
             for (int i = -pml + 1; i <= nx + pml; ++i) {
                 size_t q = I2(*L, i, j);
@@ -1349,9 +1353,7 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
                     phx[q] = (real)M.bxh[a] * s;
                 } else ex[q] = r;
                 /* rho gradient from vx: (vx_{t-1} - vx_t)/rho_eff * v^x */
-                real gx = (vxp[q] - vxc[q]) / rx * vxh[q];
-                gR[q] += 0.5 * (double)gx;
-                gR[q - 1] += 0.5 * (double)gx;
+                gxa[q] = (vxp[q] - vxc[q]) / rx * vxh[q];
                 real rz = (real)0.5 * (M.rho[q] + M.rho[q - L->NX]);
                 r = -(dt / rz) * vzh[q];
                 if (j <= 1 || j >= nz + 1) {
@@ -1360,10 +1362,19 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
                     ez[q] = r + (real)M.azh[a] * s;
                     phz[q] = (real)M.bzh[a] * s;
                 } else ez[q] = r;
-                real gz = (vzp[q] - vzc[q]) / rz * vzh[q];
-                gR[q] += 0.5 * (double)gz;
-                gR[q - L->NX] += 0.5 * (double)gz;
+                gza[q] = (vzp[q] - vzc[q]) / rz * vzh[q];
             }
+        /* rho gradient: (v_{t-1} - v_t)/rho_eff * v^ of the faces at c and
+         * c+1 (x), c and c+row (z) of each cell; gx, gz are zero outside
+         * the update domain */
+#pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < n; ++q) {
+            double a = 0.5 * (double)gxa[q] + 0.5 * (double)gza[q];
+            if (q + 1 < n) a += 0.5 * (double)gxa[q + 1];
+            if (q + (size_t)L->NX < n) a += 0.5 * (double)gza[q + L->NX];
+            gR[q] += a;
+        }
+#pragma omp parallel for schedule(static)
         for (int j = -pml + 1; j <= nz + pml; ++j)
             for (int i = -pml + 1; i <= nx + pml; ++i) {
                 /* p^ += -D+^T... = -(idx * sum c (e(i+m) - e(i+1-m)) + z) */
@@ -1379,6 +1390,7 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
                 ph[q] = ph[q] - (M.idx * sx + M.idz * sz);
             }
         /* (b) transpose of the p-update: K gradient, d = -dt K p^ (+ PML), v^ update */
+#pragma omp parallel for schedule(static)
         for (int j = -pml + 1; j <= nz + pml; ++j)
             for (int i = -pml + 1; i <= nx + pml; ++i) {
                 size_t q = I2(*L, i, j);
@@ -1397,6 +1409,7 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
                     psz[q] = (real)M.bzi[a] * s;
                 } else dza[q] = qv;
             }
+#pragma omp parallel for schedule(static)
         for (int j = -pml + 1; j <= nz + pml; ++j)
             for (int i = -pml + 1; i <= nx + pml; ++i) {
                 size_t q = I2(*L, i, j);
@@ -1433,7 +1446,7 @@ int ORA_SYM(acoustic2d_gradient_exact)(const ora_grid2d *g, const float *vp, con
             grad_vp[(size_t)(j - 1) + (size_t)(i - 1) * nz] = (real)(2.0 * r * v * fK[o]);
             grad_rho[(size_t)(j - 1) + (size_t)(i - 1) * nz] = (real)(fR[o] + v * v * fK[o]);
         }
-    free(fK); free(fR); free(gK); free(gR);
+    free(fK); free(fR); free(gK); free(gR); free(gxa); free(gza);
     free(ph); free(vxh); free(vzh); free(psx); free(psz); free(phx); free(phz);
     free(ex); free(ez); free(dxa); free(dza);
     free(P); free(PP); free(VX); free(VZ); free(tr); free(res);

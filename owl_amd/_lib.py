@@ -15,6 +15,7 @@ OWL_ERR_CFL = -2
 OWL_ERR_DEVICE = -3
 OWL_ERR_NOMEM = -4
 OWL_ERR_STATE = -5
+OWL_ERR_NONFINITE = -6
 
 OWL_EXPLOSION, OWL_FORCE = 0, 1
 OWL_RICKER, OWL_GAUSSIAN, OWL_GAUSSIAN_DERIV, OWL_RICKER_DERIV = 0, 1, 2, 3
@@ -66,7 +67,12 @@ class owl_acoustic_config(C.Structure):
                 ("dt", C.c_float), ("tmax", C.c_float), ("data_dt", C.c_float),
                 ("data_tmax", C.c_float), ("pml", C.c_int), ("fdhalf", C.c_int),
                 ("free_surface", C.c_int), ("f0factor", C.c_float),
-                ("cc_step_interval", C.c_int)]
+                ("cc_step_interval", C.c_int), ("trace_discard_threshold", C.c_float)]
+
+    def __init__(self, *args, trace_discard_threshold=1.0e-6, **kw):
+        super().__init__(*args, **kw)
+        if len(args) < len(self._fields_):
+            self.trace_discard_threshold = trace_discard_threshold
 
 
 _P = C.c_void_p
@@ -76,6 +82,7 @@ _SIGNATURES = {
     "owl_dt_stable": (C.c_float, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
     "owl_locate_point": (C.c_int, [C.c_int] + [C.c_float] * 9 + [_P, _P]),
     "owl_source_time_function": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_float, _P, C.c_int, _P]),
+    "owl_set_adaptive_range": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P]),
     "owl_cfspml_axis": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                   _P, _P, _P, _P]),
     "owl_acoustic2d_create": (C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
@@ -98,6 +105,7 @@ _SIGNATURES = {
     "owl_acoustic2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_acoustic2d_forward_store": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int]),
     "owl_acoustic2d_backward": (C.c_int, [_P, _P, _P, _P]),
+    "owl_acoustic2d_store_id": (C.c_int, [_P, _P]),
     "owl_acoustic3d_create": (C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
     "owl_acoustic3d_destroy": (None, [_P]),
     "owl_acoustic3d_nt": (C.c_int, [_P]),

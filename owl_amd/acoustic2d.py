@@ -38,11 +38,12 @@ class Acoustic2DBatch:
     """
 
     def __init__(self, nx, nz, dx, dz, vp, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15,
-                 fdhalf=4, free_surface=False, f0factor=1.0, cc_step_interval=1):
+                 fdhalf=4, free_surface=False, f0factor=1.0, cc_step_interval=1,
+                 trace_discard_threshold=1.0e-6):
         self.nx, self.nz, self.dx, self.dz = int(nx), int(nz), float(dx), float(dz)
         self.cfg = _lib.owl_acoustic_config(
             nx, 1, nz, dx, 1.0, dz, ox, 0.0, oz, dt, tmax, dt, tmax, pml, fdhalf,
-            int(bool(free_surface)), f0factor, cc_step_interval)
+            int(bool(free_surface)), f0factor, cc_step_interval, trace_discard_threshold)
         self._vp = _model(vp, (nz, nx))
         self._rho = _model(rho, (nz, nx))
         self._keep = []
@@ -194,6 +195,12 @@ class Acoustic2DBatch:
         flat = np.zeros(sum(self.nr) * self.nt, np.float32)
         check(lib().owl_acoustic2d_forward_store(self._h, _ptr(flat), ckpt, ckpt_param, store))
         return self._split(flat)
+
+    def store_id(self):
+        """Identifier of the forward states held for backward (0: none)."""
+        v = C.c_longlong()
+        check(lib().owl_acoustic2d_store_id(self._h, C.byref(v)))
+        return v.value
 
     def backward(self, dtraces):
         """Exact adjoint after forward_store: dL/dvp, dL/drho as (nz, nx) float64

@@ -23,11 +23,12 @@ class Acoustic3DBatch:
     KERNELS = ("update_p", "update_v", "post_step", "image", "strips", "inject_adjoint")
 
     def __init__(self, nx, ny, nz, dx, dy, dz, vp, rho, shots, dt, tmax, ox=0.0, oy=0.0, oz=0.0,
-                 pml=15, fdhalf=4, f0factor=1.0, free_surface=False, cc_step_interval=1):
+                 pml=15, fdhalf=4, f0factor=1.0, free_surface=False, cc_step_interval=1,
+                 trace_discard_threshold=1.0e-6):
         self.shape = (nz, ny, nx)
         self.cfg = _lib.owl_acoustic_config(nx, ny, nz, dx, dy, dz, ox, oy, oz, dt, tmax, dt, tmax,
                                             pml, fdhalf, int(bool(free_surface)), f0factor,
-                                            int(cc_step_interval))
+                                            int(cc_step_interval), trace_discard_threshold)
         self._vp = np.asfortranarray(vp, dtype=np.float32)
         self._rho = np.asfortranarray(rho, dtype=np.float32)
         if self._vp.shape != self.shape or self._rho.shape != self.shape:

@@ -21,11 +21,16 @@ class AcousticPropagator2D(torch.autograd.Function):
                         rho.detach().to("cpu", torch.float32).numpy())
         tr = batch.forward_store(ckpt, ckpt_param, store)
         ctx.batch = batch
+        ctx.store_id = batch.store_id()
         ctx.meta = (vp.dtype, vp.device, rho.dtype, rho.device)
         return torch.from_numpy(np.concatenate(tr, axis=0)).to(vp.device)
 
     @staticmethod
     def backward(ctx, grad_traces):
+        if ctx.batch.store_id() != ctx.store_id:
+            raise RuntimeError("the batch no longer holds this evaluation's forward states (another "
+                               "propagate(), set_model() or gradient() call on the batch replaced them); "
+                               "call backward before reusing the batch")
         gv, gr = ctx.batch.backward(grad_traces.detach().to("cpu", torch.float32).numpy())
         vdt, vdev, rdt, rdev = ctx.meta
         return (torch.from_numpy(gv).to(vdev, vdt), torch.from_numpy(gr).to(rdev, rdt),

@@ -1145,13 +1145,21 @@ __global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) p[e] = v;
 }
 
+// Largest |x| of each shot's field (as float bits, for atomicMax), and a
+// flag raised when any value is NaN or infinite: a diverged run, whose
+// snapshots cannot be scaled.
 __global__ __launch_bounds__(256) void k_absmax(const Shot* __restrict__ shots, const float* __restrict__ f,
-                                                unsigned* amax) {
+                                                unsigned* amax, unsigned* nonfinite) {
     const Shot S = shots[blockIdx.y];
     const size_t n = size_t(S.pitch) * S.nrow;
     float m = 0.0f;
-    for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x)
-        m = fmaxf(m, fabsf(f[S.off + e]));
+    bool bad = false;
+    for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x) {
+        const float a = fabsf(f[S.off + e]);
+        bad |= !(a <= 3.4028235e38f);
+        m = fmaxf(m, a);
+    }
+    if (__syncthreads_or(bad) && threadIdx.x == 0) *nonfinite = 1u;
     __shared__ float red[256];
     red[threadIdx.x] = m;
     __syncthreads();
@@ -1299,6 +1307,8 @@ struct owl_acoustic2d {
     } full, snaps, seg;
     int store_policy = -1, store_param = 0, store_fmt = 0;  // what the last forward stored
     bool have_store = false;
+    long long store_id = 0, store_next = 0;  // identifies the stored states (0: none)
+    DevBuf<unsigned> d_nonfinite;    // raised by k_absmax when a 16-bit snapshot saw NaN/Inf
     std::vector<std::vector<int>> binom_split;  // cached optimal splits [s][l]
     int binom_n = 0, binom_s = -1;
     bool binom_carry = false;
@@ -1725,9 +1735,18 @@ Sets one_set(float* p, float* vx, float* vz, float* const* m) {
 enum KernelKind { kKP = 0, kKV = 1, kKPost = 2, kKImage = 3, kKStrips = 4, kKAdj = 5, kKMisfit = 6, kKMerge = 7,
                   kKP2 = 8, kKV2 = 9, kKFused = 13, kKinds = 16 };
 
+void prof_collect(owl_acoustic2d* b);
+constexpr size_t kMaxPendingPairs = 4096;
+
 // Begin/end a timed region around one launch when profiling is on.
 int prof_begin(owl_acoustic2d* b, int kind, double bytes = 0.0) {
     if (!b->profiling) return -1;
+    // keep at most kMaxPendingPairs event pairs in flight: fold the recorded
+    // ones into the totals first (profiling runs eagerly, never under capture)
+    if (b->ev_used.size() >= kMaxPendingPairs && !b->capturing) {
+        OWL_HIP(hipStreamSynchronize(b->stream));
+        prof_collect(b);
+    }
     if (b->ev_next + 2 > b->ev_pool.size()) {
         for (int k = 0; k < 256; ++k) {
             hipEvent_t e;
@@ -1824,10 +1843,6 @@ void run_forward(owl_acoustic2d* b, bool save) {
                            b->d_rec_taps.get(), b->d_rec_w.get(), b->d_traces.get(),
                            save ? b->d_strips.get() : nullptr, b->pml);
         prof_end(b, ev);
-        if (b->profiling && b->ev_used.size() > 20000) {
-            OWL_HIP(hipStreamSynchronize(st));
-            prof_collect(b);
-        }
     }
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
@@ -1907,10 +1922,6 @@ void run_reverse_owl(owl_acoustic2d* b) {
                                b->d_den.get(), b->d_gv.get(), b->d_gr.get(), t == b->nt ? 1 : 0, b->pml);
         prof_end(b, ev);
         std::swap(A, B);
-        if (b->profiling && b->ev_used.size() > 20000) {
-            OWL_HIP(hipStreamSynchronize(st));
-            prof_collect(b);
-        }
     }
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
@@ -2080,7 +2091,8 @@ void slot_save(owl_acoustic2d* b, owl_acoustic2d::Slots& S, int slot, float* con
         } else {
             unsigned* am = S.amax.get() + (size_t(slot) * S.nfield + k) * nshot;
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, b->stream, am, 0u, int(nshot));
-            hipLaunchKernelGGL(k_absmax, dim3(64, nshot), dim3(256), 0, b->stream, b->d_shots.get(), fields[k], am);
+            hipLaunchKernelGGL(k_absmax, dim3(64, nshot), dim3(256), 0, b->stream, b->d_shots.get(), fields[k], am,
+                               b->d_nonfinite.get());
             hipLaunchKernelGGL(k_pack16, dim3(256, nshot), dim3(256), 0, b->stream, b->d_shots.get(), fields[k],
                                S.h16.get() + o, am, S.fmt);
         }
@@ -2437,6 +2449,8 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
     const int N = b->nt - 1;  // forward steps
     const W7 w1 = work1(b), w2 = work2(b);
     const bool do_fwd = phase != kReverseOnly, do_rev = phase != kForwardOnly;
+    if (fmt != OWL_STORE_F32)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, b->d_nonfinite.get(), 0u, 1);
     if (do_fwd) {
         zero_set(b, w1);
         zero_set(b, w2);
@@ -2708,6 +2722,7 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     for (int k = 0; k < 4; ++k) b->fM3[k].alloc(b->field_elems);
     if (ckpt == OWL_CKPT_BINOMIAL && fmt == OWL_STORE_F32)
         for (auto& k : b->fK) k.alloc(b->field_elems);
+    b->d_nonfinite.alloc(1);
     const bool do_fwd = phase != kReverseOnly, do_rev = phase != kForwardOnly;
     if (do_rev && !do_fwd &&
         (!b->have_store || b->store_policy != ckpt || b->store_param != param || b->store_fmt != fmt))
@@ -2762,13 +2777,21 @@ void run_exact(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::funct
     b->last_updates = double(N) * (double(do_fwd) + double(do_rev)) * b->pts_full;
     b->fwd_state_valid = false;
     b->have_boundary = false;
+    b->have_store = false;
+    b->store_id = 0;
+    if (fmt != OWL_STORE_F32) {
+        unsigned nonfinite = 0;
+        OWL_HIP(hipMemcpy(&nonfinite, b->d_nonfinite.get(), sizeof(unsigned), hipMemcpyDeviceToHost));
+        if (nonfinite)
+            fail(OWL_ERR_NONFINITE, "the wavefield became NaN or infinite (the run diverged): 16-bit snapshots "
+                                    "cannot be scaled");
+    }
     if (phase == kForwardOnly) {
         b->have_store = true;
         b->store_policy = ckpt;
         b->store_param = param;
         b->store_fmt = fmt;
-    } else {
-        b->have_store = false;
+        b->store_id = ++b->store_next;
     }
 }
 
@@ -2870,6 +2893,16 @@ owl_status owl_acoustic2d_set_model(owl_acoustic2d* b, const float* vp, const fl
         plan(b, &cfg, vp, rho, int(shots.size()), shots.data());
         b->have_boundary = false;
         b->fwd_state_valid = false;
+        // stored forward states belong to the old model
+        b->have_store = false;
+        b->store_id = 0;
+    });
+}
+
+owl_status owl_acoustic2d_store_id(const owl_acoustic2d* b, long long* id) {
+    return guarded([&] {
+        if (!b || !id) fail(OWL_ERR_ARG, "null argument");
+        *id = b->have_store ? b->store_id : 0;
     });
 }
 
@@ -2936,7 +2969,7 @@ owl_status owl_acoustic2d_gradient(owl_acoustic2d* b, const float* obs, int mode
             const int ev = prof_begin(b, kKMisfit);
             hipLaunchKernelGGL(k_adjoint_source, dim3(unsigned(b->sd.size())), dim3(512), 0, st, b->d_shots.get(),
                                b->d_traces.get(), b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(),
-                               b->d_misfit.get(), b->nt, 1.0e-6f, b->d_scratch.get());
+                               b->d_misfit.get(), b->nt, b->cfg.trace_discard_threshold, b->d_scratch.get());
             prof_end(b, ev);
         };
         double total_ms = 0, total_up = 0;

@@ -2084,7 +2084,7 @@ owl_status owl_acoustic3d_gradient(owl_acoustic3d* b, const float* obs, double* 
         b->d_misfit.alloc(nshot);
         b->d_scratch.alloc(std::max<size_t>(2 * size_t(b->nrec_total), 1));
         hipLaunchKernelGGL(k3_adjoint_source, dim3(nshot), dim3(256), 0, st, b->d_shots.get(), b->d_traces.get(),
-                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, 1.0e-6f,
+                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, b->cfg.trace_discard_threshold,
                            b->d_scratch.get());
         dispatch_R3(b->R, [&](auto r) { run_reverse3<decltype(r)::value>(b); });
         const size_t nfull = size_t(b->cfg.nx) * b->cfg.ny * b->cfg.nz;

@@ -1149,7 +1149,7 @@ owl_status owl_elastic2d_gradient(owl_elastic2d* b, const float* obs, double* mi
         b->d_misfit.alloc(2 * size_t(nshot));
         b->d_scratch.alloc(std::max<size_t>(4 * size_t(b->nrec_total), 1));
         hipLaunchKernelGGL(k_adjoint_source_2c<ShotE2>, dim3(nshot, 2), dim3(256), 0, st, b->d_shots.get(), b->d_traces.get(),
-                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, 1.0e-6f,
+                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, b->cfg.trace_discard_threshold,
                            b->d_scratch.get());
         dispatch_R2e(b->R, [&](auto r) { run_reverse2<decltype(r)::value>(b); });
         const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;

@@ -1599,7 +1599,7 @@ owl_status owl_elastic3d_gradient(owl_elastic3d* b, const float* obs, double* mi
         b->d_misfit.alloc(3 * size_t(nshot));
         b->d_scratch.alloc(std::max<size_t>(6 * size_t(b->nrec_total), 1));
         hipLaunchKernelGGL(ke_adjoint_source, dim3(nshot, 3), dim3(256), 0, st, b->d_shots.get(), b->d_traces.get(),
-                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, 1.0e-6f,
+                           b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(), b->nt, b->cfg.trace_discard_threshold,
                            b->d_scratch.get());
         dispatch_Re(b->R, [&](auto r) { run_reverse_e<decltype(r)::value>(b); });
         const size_t nfull = size_t(b->cfg.nx) * b->cfg.ny * b->cfg.nz;

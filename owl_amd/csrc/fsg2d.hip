@@ -1450,7 +1450,7 @@ owl_status owl_fsg2d_gradient(owl_fsg2d* b, const float* obs, double* misfit, do
         b->d_scratch.alloc(std::max<size_t>(4 * size_t(b->nrec_total), 1));
         hipLaunchKernelGGL(k_adjoint_source_2c<ShotF>, dim3(nshot, 2), dim3(256), 0, st, b->d_shots.get(),
                            b->d_traces.get(), b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(),
-                           b->d_misfit.get(), b->nt, 1.0e-6f, b->d_scratch.get());
+                           b->d_misfit.get(), b->nt, b->cfg.trace_discard_threshold, b->d_scratch.get());
         dispatch_Rf(b->R, [&](auto r) { run_reversef<decltype(r)::value>(b); });
         const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
         b->d_full.alloc(nfull);

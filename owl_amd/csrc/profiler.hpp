@@ -24,8 +24,16 @@ public:
         next_ = 0;
     }
     bool on() const { return on_; }
+    // At most kMaxPending pairs stay in flight: beyond that the stream is
+    // synchronised and the recorded pairs are folded into the totals, so a
+    // long run holds a bounded number of events.
+    static constexpr size_t kMaxPending = 4096;
     int begin(hipStream_t s, int kind, double bytes) {
         if (!on_) return -1;
+        if (used_.size() >= kMaxPending) {
+            OWL_HIP(hipStreamSynchronize(s));
+            collect();
+        }
         if (next_ + 2 > pool_.size())
             for (int k = 0; k < 256; ++k) {
                 hipEvent_t e;

@@ -56,10 +56,12 @@ class TwoComponentBatch:
     def _fn(self, name):
         return getattr(lib(), f"{self.PREFIX}_{name}")
 
-    def _setup(self, nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval):
+    def _setup(self, nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval,
+               trace_discard_threshold=1.0e-6):
         self.shape = (nz, nx)
         self.cfg = _lib.owl_acoustic_config(nx, 1, nz, dx, 1.0, dz, ox, 0.0, oz, dt, tmax, dt, tmax,
-                                            pml, fdhalf, 0, f0factor, cc_step_interval)
+                                            pml, fdhalf, 0, f0factor, cc_step_interval,
+                                            trace_discard_threshold)
         self._shots, self._keep, self.nr, self._windows = shot_array_2d(shots, nx, nz)
 
     def _model(self, a):
@@ -138,8 +140,9 @@ class Elastic2DBatch(TwoComponentBatch):
     PREFIX = "owl_elastic2d"
 
     def __init__(self, nx, nz, dx, dz, vp, vs, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
-                 f0factor=1.0, cc_step_interval=1):
-        self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval)
+                 f0factor=1.0, cc_step_interval=1, trace_discard_threshold=1.0e-6):
+        self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval,
+                    trace_discard_threshold)
         self._m = [self._model(a) for a in (vp, vs, rho)]
         h = C.c_void_p()
         check(lib().owl_elastic2d_create(C.byref(self.cfg), *[_ptr(a) for a in self._m], len(shots),

@@ -24,10 +24,11 @@ class Elastic3DBatch:
     KERNELS = ("stress", "velocity", "sources_receivers", "imaging", "boundary_records")
 
     def __init__(self, nx, ny, nz, dx, dy, dz, vp, vs, rho, shots, dt, tmax, ox=0.0, oy=0.0, oz=0.0,
-                 pml=15, fdhalf=4, f0factor=1.0, free_surface=False):
+                 pml=15, fdhalf=4, f0factor=1.0, free_surface=False, trace_discard_threshold=1.0e-6):
         self.shape = (nz, ny, nx)
         self.cfg = _lib.owl_acoustic_config(nx, ny, nz, dx, dy, dz, ox, oy, oz, dt, tmax, dt, tmax,
-                                            pml, fdhalf, int(bool(free_surface)), f0factor, 1)
+                                            pml, fdhalf, int(bool(free_surface)), f0factor, 1,
+                                            trace_discard_threshold)
         self._m = [np.asfortranarray(a, dtype=np.float32) for a in (vp, vs, rho)]
         for a in self._m:
             if a.shape != self.shape:

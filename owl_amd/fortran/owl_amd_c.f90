@@ -48,6 +48,7 @@ module owl_amd_c
         integer(c_int) :: free_surface = 0
         real(c_float) :: f0factor = 1
         integer(c_int) :: cc_step_interval = 1
+        real(c_float) :: trace_discard_threshold = 1.0e-6
     end type
 
     interface

@@ -37,8 +37,9 @@ class FSG2DBatch(TwoComponentBatch):
     PREFIX = "owl_fsg2d"
 
     def __init__(self, nx, nz, dx, dz, cij, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
-                 f0factor=1.0, cc_step_interval=1, aniso=True):
-        self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval)
+                 f0factor=1.0, cc_step_interval=1, aniso=True, trace_discard_threshold=1.0e-6):
+        self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval,
+                    trace_discard_threshold)
         self._m = [self._model(cij[k]) for k in CIJ]
         self._rho = self._model(rho)
         ptrs = (C.c_void_p * 6)(*[a.ctypes.data for a in self._m])

@@ -2,11 +2,12 @@
 
 Source and Receiver mirror type(source) / type(receiver)
 (src/lib/module_source_receiver.f90:29-109); Geometry mirrors
-type(source_receiver_geometry) (:111-142).  set_adaptive_range restates
-src/lib/module_utility.f90:410-481 in float32 so the window indices are the
-reference's.
+type(source_receiver_geometry) (:111-142).  set_adaptive_range calls the
+library's owl_set_adaptive_range (src/lib/module_utility.f90:410-510 with
+the offset rule of module_source_receiver.f90:421-436, in float32) so the
+window indices and receiver weights are the reference's.
 """
-import math
+import ctypes as C
 from dataclasses import dataclass, field
 from typing import List
 
@@ -78,55 +79,47 @@ class Window:
         return self.nyend - self.nybeg + 1
 
 
-def _clip(v, lo, hi):
-    return max(lo, min(v, hi))
+FLOAT_HUGE = float(np.finfo(np.float32).max)
 
 
-def _axis(enabled, rec, src, w, extra, vmin, vmax, o, d, n):
-    """One axis of set_adaptive_range (module_utility.f90:423-437)."""
-    if enabled:
-        live = [f32(r) for r, ww in zip(rec, w) if ww != 0]
-        rmin = min(live) if live else f32(np.finfo(np.float32).max)
-        rmax = max(live) if live else f32(-np.finfo(np.float32).max)
-        beg = max(f32(vmin), f32(min(rmin, min(f32(s) for s in src)) - f32(extra)))
-        end = min(f32(vmax), f32(max(rmax, max(f32(s) for s in src)) + f32(extra)))
-        beg = f32(f32(math.floor((beg - f32(o)) / f32(d))) * f32(d) + f32(o))
-        end = f32(f32(math.ceil((end - f32(o)) / f32(d))) * f32(d) + f32(o))
-    else:
-        beg = f32(o)
-        end = f32(f32(o) + f32(n - 1) * f32(d))
-    nbeg = _clip(int((beg - f32(o)) / f32(d)) + 1, 1, n)
-    nend = _clip(int((end - f32(o)) / f32(d)) + 1, 1, n)
-    return beg, end, nbeg, nend
+class owl_window_params(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
+        [(k, C.c_float) for k in ("dx", "dy", "dz", "ox", "oy", "oz", "xmin", "xmax", "ymin", "ymax",
+                                  "zmin", "zmax")] + \
+        [(k, C.c_int) for k in ("yn_adpx", "yn_adpy", "yn_adpz")] + \
+        [(k, C.c_float) for k in ("adp_extrax", "adp_extray", "adp_extraz", "offset_min", "offset_max")]
 
 
 def set_adaptive_range(geom, nx, nz, dx, dz, ox=0.0, oz=0.0, yn_adpx=False, yn_adpz=False,
-                       adp_extrax=0.0, adp_extraz=0.0, offset_min=-np.inf, offset_max=np.inf,
-                       xmin=None, xmax=None, zmin=None, zmax=None):
-    """Per-shot window of OWL's set_adaptive_range; zeroes receiver weights
-    outside it (and outside [offset_min, offset_max]).  Returns a Window."""
-    xmin = ox if xmin is None else xmin
-    xmax = ox + (nx - 1) * dx if xmax is None else xmax
-    zmin = oz if zmin is None else zmin
-    zmax = oz + (nz - 1) * dz if zmax is None else zmax
-    sx = np.mean([s.x for s in geom.srcr], dtype=np.float32)
-    sz = np.mean([s.z for s in geom.srcr], dtype=np.float32)
-    for r in geom.recr:
-        aoff = math.sqrt((r.x - sx) ** 2 + (r.z - sz) ** 2)
-        if aoff < offset_min or aoff > offset_max:
-            r.weight = 0.0
-    w = [r.weight for r in geom.recr]
-    xb, xe, nxb, nxe = _axis(yn_adpx, [r.x for r in geom.recr], [s.x for s in geom.srcr], w,
-                             adp_extrax, xmin, xmax, ox, dx, nx)
-    if yn_adpx:
-        for r in geom.recr:
-            if f32(r.x) < xb or f32(r.x) > xe:
-                r.weight = 0.0
-    w = [r.weight for r in geom.recr]
-    zb, ze, nzb, nze = _axis(yn_adpz, [r.z for r in geom.recr], [s.z for s in geom.srcr], w,
-                             adp_extraz, zmin, zmax, oz, dz, nz)
-    if yn_adpz:
-        for r in geom.recr:
-            if f32(r.z) < zb or f32(r.z) > ze:
-                r.weight = 0.0
-    return Window(nxb, nxe, nzb, nze, float(xb), float(zb))
+                       adp_extrax=0.0, adp_extraz=0.0, offset_min=0.0, offset_max=FLOAT_HUGE,
+                       xmin=None, xmax=None, zmin=None, zmax=None, ny=1, dy=1.0, oy=0.0,
+                       yn_adpy=False, adp_extray=0.0, ymin=None, ymax=None):
+    """OWL's per-shot window (set_adaptive_range) for 2D (ny = 1) or 3D
+    models.  Zeroes the weight of receivers outside [offset_min, offset_max]
+    or outside the window, in place, and returns the Window (1-based node
+    ranges and the window's origin).  Defaults are OWL's parameter defaults
+    (module_parameters.f90:90-91, 123-129; extents module_model.f90:97-111)."""
+    from ._lib import check, lib
+    f = np.float32
+    ext = lambda v, o, n, d: f(o) + f(n - 1) * f(d) if v is None else v  # noqa: E731
+    P = owl_window_params(int(nx), int(ny), int(nz), dx, dy, dz, ox, oy, oz,
+                          ox if xmin is None else xmin, ext(xmax, ox, nx, dx),
+                          oy if ymin is None else ymin, ext(ymax, oy, ny, dy),
+                          oz if zmin is None else zmin, ext(zmax, oz, nz, dz),
+                          int(bool(yn_adpx)), int(bool(yn_adpy)), int(bool(yn_adpz)),
+                          adp_extrax, adp_extray, adp_extraz, offset_min, offset_max)
+    from ._lib import MECHANISMS, WAVELETS, owl_receiver, owl_source
+    srcs = (owl_source * geom.ns)()
+    for k, s in enumerate(geom.srcr):
+        srcs[k] = owl_source(s.x, s.y, s.z, s.f0, s.amp, s.t0, s.polar, s.azimuth, MECHANISMS[s.mechanism],
+                             WAVELETS[s.wavelet])
+    recs = (owl_receiver * max(1, geom.nr))()
+    for k, r in enumerate(geom.recr):
+        recs[k] = owl_receiver(r.x, r.y, r.z, r.weight)
+    win = (C.c_int * 6)()
+    rng = (C.c_float * 6)()
+    check(lib().owl_set_adaptive_range(C.byref(P), geom.ns, srcs, geom.nr, recs, win, rng))
+    for k, r in enumerate(geom.recr):
+        r.weight = float(recs[k].weight)
+    return Window(win[0], win[1], win[4], win[5], float(rng[0]), float(rng[4]), nybeg=win[2], nyend=win[3],
+                  ybeg=float(rng[2]))

@@ -4,7 +4,7 @@ forward).  Pure numpy, deterministic for a given seed.
 """
 import numpy as np
 
-from .geometry import Geometry, Receiver, Source
+from .geometry import Geometry, Receiver, Source, set_adaptive_range
 
 
 def gardner(vp):
@@ -95,10 +95,12 @@ def acoustic3d_layered_model(n=256, d=10.0, seed=0, undulation=50.0):
 
 def acoustic3d_shots(n=256, d=10.0, seed=0, f0=8.0, zs=20.0, zr=20.0, nrec_side=100,
                      wmin=128, wmax=256):
-    """16 shots on a 4x4 surface grid, each with its own window of wmin..wmax
-    cells per horizontal side (seeded, centred on the source, clipped to the
-    model) and a 100 x 100 receiver patch at zr spanning the window on grid
-    nodes.  Returns a list of (Geometry, window dict)."""
+    """16 shots on a 4x4 surface grid.  Each shot draws an aperture of
+    wmin..wmax cells per horizontal side (seeded, centred on the source,
+    clipped to the model) and lays a 100 x 100 receiver patch at zr across it
+    on grid nodes; its window is then OWL's set_adaptive_range with adaptive
+    x and y and a margin equal to the larger gap between the patch and the
+    aperture (full depth).  Returns a list of (Geometry, window dict)."""
     rng = np.random.default_rng(seed)
     out = []
     sid = 1
@@ -115,14 +117,18 @@ def acoustic3d_shots(n=256, d=10.0, seed=0, f0=8.0, zs=20.0, zr=20.0, nrec_side=
             recs = []
             stepx = max(1, (wx - 1) // (nrec_side - 1))
             stepy = max(1, (wy - 1) // (nrec_side - 1))
-            x0 = (win["nxbeg"] - 1 + (wx - 1 - stepx * (nrec_side - 1)) // 2) * d
-            y0 = (win["nybeg"] - 1 + (wy - 1 - stepy * (nrec_side - 1)) // 2) * d
+            gapx, gapy = wx - 1 - stepx * (nrec_side - 1), wy - 1 - stepy * (nrec_side - 1)
+            x0 = (win["nxbeg"] - 1 + gapx // 2) * d
+            y0 = (win["nybeg"] - 1 + gapy // 2) * d
             for ry in range(nrec_side):
                 for rx in range(nrec_side):
                     recs.append(Receiver(x=float(x0 + rx * stepx * d), y=float(y0 + ry * stepy * d),
                                          z=float(zr)))
             g = Geometry(id=sid, srcr=[Source(x=sx, y=sy, z=zs, f0=f0)], recr=recs)
-            out.append((g, win))
+            w = set_adaptive_range(g, n, n, d, d, ny=n, dy=d, yn_adpx=True, yn_adpy=True,
+                                   adp_extrax=(gapx - gapx // 2) * d, adp_extray=(gapy - gapy // 2) * d)
+            out.append((g, dict(nxbeg=w.nxbeg, nxend=w.nxend, nybeg=w.nybeg, nyend=w.nyend,
+                                nzbeg=w.nzbeg, nzend=w.nzend)))
             sid += 1
     return out
 
@@ -166,10 +172,11 @@ def fsg2d_tti_model(nx=801, nz=401, dz=5.0, seed=0):
     return thomsen_tilted_cij(vp, vp / 1.8, rho, eps, delta, theta), rho
 
 
-def fsg2d_shots(nx=801, dx=5.0, nshot=64, seed=0):
+def fsg2d_shots(nx=801, dx=5.0, nshot=64, seed=0, nz=401, dz=5.0):
     """64 surface shots (vertical force, 15 Hz Ricker, at 10 m depth) spread
-    along x, each with its own window of 200..801 columns around the source
-    (full depth) and receivers every 10 m at 10 m depth inside the window.
+    along x.  Each draws an aperture of 200..801 columns around the source and
+    places receivers every 10 m at 10 m depth across it; its window is OWL's
+    set_adaptive_range with adaptive x (no margin) over the full depth.
     Returns [(Geometry, window dict)]."""
     rng = np.random.default_rng(seed)
     out = []
@@ -182,7 +189,9 @@ def fsg2d_shots(nx=801, dx=5.0, nshot=64, seed=0):
         x0 = (b - 1) * dx
         recs = [Receiver(x=float(x), y=0.0, z=10.0) for x in np.arange(x0, (e - 1) * dx + 1e-3, 10.0)]
         src = Source(x=float(xs), y=0.0, z=10.0, f0=15.0, mechanism="force", polar=0.0, amp=1e6)
-        out.append((Geometry(id=k + 1, srcr=[src], recr=recs), dict(nxbeg=b, nxend=e, nzbeg=0, nzend=0)))
+        g = Geometry(id=k + 1, srcr=[src], recr=recs)
+        w = set_adaptive_range(g, nx, nz, dx, dz, yn_adpx=True)
+        out.append((g, dict(nxbeg=w.nxbeg, nxend=w.nxend, nzbeg=w.nzbeg, nzend=w.nzend)))
     return out
 
 
