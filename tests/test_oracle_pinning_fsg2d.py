@@ -6,6 +6,12 @@ OWL's elastic_tti_2d restated) against facts that do not come from it:
   standard-staggered elastic pin), for vertical and horizontal forces;
 * the float32 and float64 builds agree on a heterogeneous tilted-TI model
   with the M-PML damping ratios on, an oblique force and an explosion;
+* in a homogeneous tilted-TI medium (30 degree tilt) the qP moveout between
+  two receiver rings along seven directions equals the group velocity of
+  the 2D Christoffel equation (numpy, from the same c_ij) within 1 %, and
+  differs by 9-20 % from that of the mirror-image medium (c15, c35 ->
+  -c15, -c35), so a sign or placement error of the shear-coupling terms in
+  the update fails it;
 * the M-PML scan: its eigenvalues are those of the 2D Christoffel matrix
   (numpy eigvalsh), its x- and z-direction derivative expressions (written
   out separately in OWL) agree under the x <-> z transposition of the
@@ -98,6 +104,64 @@ def _christoffel(c, k1, k3, rho):
                   [0.0, c55 * k1 * k1 + 2 * c35 * k1 * k3 + c33 * k3 * k3]])
     G[1, 0] = G[0, 1]
     return np.sort(np.linalg.eigvalsh(G / rho))[::-1]
+
+
+def _qp_phase_velocity(c, rho, phi):
+    """qP phase velocity for phase direction (sin phi, cos phi) in (x, z):
+    the larger eigenvalue of the 2D Christoffel matrix, closed form."""
+    c11, c13, c15, c33, c35, c55 = c
+    n1, n3 = np.sin(phi), np.cos(phi)
+    g11 = c11 * n1 * n1 + 2 * c15 * n1 * n3 + c55 * n3 * n3
+    g13 = c15 * n1 * n1 + (c13 + c55) * n1 * n3 + c35 * n3 * n3
+    g33 = c55 * n1 * n1 + 2 * c35 * n1 * n3 + c33 * n3 * n3
+    tr, det = g11 + g33, g11 * g33 - g13 * g13
+    return np.sqrt(0.5 * (tr + np.sqrt(tr * tr - 4 * det)) / rho)
+
+
+def _qp_group_speed(c, rho, psi):
+    """qP group speed along the ray direction psi (from +z towards +x):
+    g = V n + dV/dphi n_perp over a fine sweep of phase angles, then
+    interpolated at psi (the qP wavefront is convex, one branch)."""
+    phi = np.linspace(-np.pi, np.pi, 200001)
+    v = _qp_phase_velocity(c, rho, phi)
+    dv = np.gradient(v, phi)
+    gx = v * np.sin(phi) + dv * np.cos(phi)
+    gz = v * np.cos(phi) - dv * np.sin(phi)
+    ang = np.arctan2(gx, gz)
+    o = np.argsort(ang)
+    return np.interp(psi, ang[o], np.hypot(gx, gz)[o])
+
+
+def test_tilted_ti_qp_moveout_matches_christoffel_group_velocity(oracle32):
+    n, d, rho, f0, dt, nt = 321, 5.0, 2000.0, 20.0, 0.4e-3, 800
+    m = tilted_ti(3000.0, 3000.0 / 1.8, rho, 0.25, 0.1, np.deg2rad(30.0))
+    c = [float(m[k]) for k in CIJ]
+    mirror = [c[0], c[1], -c[2], c[3], -c[4], c[5]]
+    assert abs(c[2]) > 0.1 * c[5] and abs(c[4]) > 0.1 * c[5]
+    cen = (n // 2) * d
+    psis = np.deg2rad([-75.0, -50.0, -25.0, 0.0, 25.0, 50.0, 75.0])
+    r1, r2 = 250.0, 600.0
+    rec = [dict(x=cen + r * np.sin(p), z=cen + r * np.cos(p)) for p in psis for r in (r1, r2)]
+    g = dict(nx=n, nz=n, dx=d, dz=d, dt=dt, nt=nt, pml=20, R=4, aniso=1)
+    full = lambda v: np.full((n, n), v, np.float32)  # noqa: E731
+    tx, tz = oracle32.fsg2d_forward(g, {k: full(v) for k, v in zip(CIJ, c)}, full(rho),
+                                    [dict(x=cen, z=cen, f0=f0, mechanism="explosion", amp=1e6)], rec)
+    t = np.arange(nt) * dt
+    worst_mirror = 0.0
+    for k, p in enumerate(psis):
+        vg = _qp_group_speed(c, rho, p)
+        # radial velocity at both radii, each windowed around its qP arrival
+        # (the qS arrivals come 0.07 s and later after it)
+        tr = [(tx[2 * k + q] * np.sin(p) + tz[2 * k + q] * np.cos(p))
+              * np.exp(-0.5 * ((t - r / vg - 1.0 / f0) * f0 / 0.8) ** 2) for q, r in enumerate((r1, r2))]
+        xc = np.correlate(tr[1], tr[0], mode="full")
+        j = int(np.argmax(xc))
+        y0, y1, y2 = xc[j - 1], xc[j], xc[j + 1]
+        lag = (j - (nt - 1) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)) * dt
+        measured = (r2 - r1) / lag
+        assert abs(measured / vg - 1) < 0.01, (np.rad2deg(p), measured, vg)
+        worst_mirror = max(worst_mirror, abs(measured / _qp_group_speed(mirror, rho, p) - 1))
+    assert worst_mirror > 0.08
 
 
 def test_mpml_eigenvalues_and_transposed_derivatives(oracle32):
