@@ -182,6 +182,20 @@ owl_status owl_acoustic2d_set_model(owl_acoustic2d *b, const float *vp, const fl
  * gradient needs (reconstruct = .true.). */
 owl_status owl_acoustic2d_forward(owl_acoustic2d *b, float *traces, int save_boundary);
 
+/* Forward modelling that allocates no device memory: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments as
+ * owl_acoustic2d_create followed by owl_acoustic2d_forward(..., traces, 0),
+ * and the same traces.  Every device buffer of the call is carved from a
+ * static device array of 32 MiB per GPU (held until the process ends): a
+ * run whose fields, model, tables and traces need more returns
+ * OWL_ERR_NOMEM.  Host-side planning uses ordinary host memory.  Calls on one
+ * GPU share the array and are serialised within a process; the array keeps
+ * the last call's contents, and every call clears what must start at zero. */
+owl_status owl_acoustic2d_forward_static(const owl_acoustic_config *cfg, const float *vp,
+                                         const float *rho, int nshot, const owl_shot *shots,
+                                         float *traces);
+
 /* FWI gradient over all shots.  obs: observed traces, laid out like the
  * forward traces.  Computes the L2 misfit and the adjoint source
  * syn - obs (compute_adjsrc_waveform,

@@ -90,6 +90,7 @@ _SIGNATURES = {
     "owl_acoustic2d_nt": (C.c_int, [_P]),
     "owl_acoustic2d_set_model": (C.c_int, [_P, _P, _P]),
     "owl_acoustic2d_forward": (C.c_int, [_P, _P, C.c_int]),
+    "owl_acoustic2d_forward_static": (C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
     "owl_acoustic2d_gradient": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
     "owl_acoustic2d_adjoint": (C.c_int, [_P, _P, _P, _P]),
     "owl_acoustic2d_shot_dims": (C.c_int, [_P, C.c_int, _P, _P, _P]),

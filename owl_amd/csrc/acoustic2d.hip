@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1321,9 +1322,12 @@ struct owl_acoustic2d {
     hipGraphExec_t gexec = nullptr;
     std::vector<uintptr_t> gsig, gready;
 
+    bool borrowed_stream = false;    // stream and ev0/ev1 belong to the static-scratch context
+
     ~owl_acoustic2d() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (borrowed_stream) return;
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -2911,6 +2915,76 @@ owl_status owl_acoustic2d_forward(owl_acoustic2d* b, float* traces, int save_bou
         if (!b) fail(OWL_ERR_ARG, "null batch");
         dispatch_R(b->R, [&](auto r) { run_forward<decltype(r)::value>(b, save_boundary != 0); });
         if (traces) download_traces(b, traces);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Forward modelling that allocates no device memory.  Every device buffer of
+// the call (fields, PML memory, model, tables, traces) is carved from the
+// file-scope array below, so a call fits if its plan needs at most
+// kStaticScratchBytes.  Each GPU holds its own copy of the array until the
+// process ends; its device address is looked up once per device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+struct StaticContext {
+    std::once_flag once;
+    hipError_t init_rc = hipSuccess;
+    void* ptr = nullptr;             // device address of this GPU's scratch
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;                   // one call at a time per GPU in this process
+};
+StaticContext g_static[kMaxDevices];
+
+StaticContext& static_context() {
+    int dev = 0;
+    OWL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(OWL_ERR_DEVICE, "device index out of range");
+    StaticContext& C = g_static[dev];
+    std::call_once(C.once, [&] {
+        // with this device current: its copy of the array, and a stream and
+        // two events the calls share
+        C.init_rc = hipGetSymbolAddress(&C.ptr, HIP_SYMBOL(scratch));
+        if (C.init_rc == hipSuccess) C.init_rc = hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking);
+        if (C.init_rc == hipSuccess) C.init_rc = hipEventCreate(&C.ev0);
+        if (C.init_rc == hipSuccess) C.init_rc = hipEventCreate(&C.ev1);
+    });
+    OWL_HIP(C.init_rc);
+    if (!C.ptr) fail(OWL_ERR_DEVICE, "no device address for the static scratch");
+    return C;
+}
+}  // namespace
+
+extern "C" {
+
+owl_status owl_acoustic2d_forward_static(const owl_acoustic_config* cfg, const float* vp, const float* rho,
+                                         int nshot, const owl_shot* shots, float* traces) {
+    return guarded([&] {
+        if (!traces) fail(OWL_ERR_ARG, "traces is null");
+        StaticContext& C = static_context();
+        std::lock_guard<std::mutex> lock(C.mu);
+        DeviceArena arena;
+        arena.base = static_cast<char*>(C.ptr);
+        arena.size = kStaticScratchBytes;
+        DeviceArenaScope scope(&arena);
+        owl_acoustic2d b;
+        b.stream = C.stream;
+        b.ev0 = C.ev0;
+        b.ev1 = C.ev1;
+        b.borrowed_stream = true;
+        b.use_graphs = 0;
+        plan(&b, cfg, vp, rho, nshot, shots);
+        dispatch_R(b.R, [&](auto r) { run_forward<decltype(r)::value>(&b, false); });
+        download_traces(&b, traces);
     });
 }
 

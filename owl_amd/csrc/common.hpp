@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -51,7 +52,26 @@ owl_status guarded(F&& f) {
     }
 }
 
-// Owning device allocation of n elements of T.
+// Device memory that DevBuf::alloc carves buffers from instead of calling
+// hipMalloc, while a DeviceArenaScope of the calling thread is active: a
+// bump allocator over memory the caller owns (256-byte aligned pieces, never
+// freed one by one).  Running past its end fails with OWL_ERR_NOMEM.
+struct DeviceArena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+};
+inline thread_local DeviceArena* tl_arena = nullptr;
+
+struct DeviceArenaScope {
+    explicit DeviceArenaScope(DeviceArena* a) : prev(tl_arena) { tl_arena = a; }
+    ~DeviceArenaScope() { tl_arena = prev; }
+    DeviceArenaScope(const DeviceArenaScope&) = delete;
+    DeviceArenaScope& operator=(const DeviceArenaScope&) = delete;
+    DeviceArena* prev;
+};
+
+// Device allocation of n elements of T: owned (hipMalloc/hipFree), or a
+// piece of the active DeviceArena (released with the arena).
 template <class T>
 class DevBuf {
 public:
@@ -60,9 +80,9 @@ public:
     ~DevBuf() { release(); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), owned_(o.owned_) { o.p_ = nullptr; o.n_ = 0; }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; owned_ = o.owned_; o.p_ = nullptr; o.n_ = 0; }
         return *this;
     }
     // (Re)allocate to n elements; keeps the buffer (and its contents) when the
@@ -71,11 +91,22 @@ public:
         if (p_ && n == n_) return;
         release();
         if (n == 0) return;
-        OWL_HIP(hipMalloc(&p_, n * sizeof(T)));
+        if (DeviceArena* a = tl_arena) {
+            const size_t at = (a->used + 255) / 256 * 256;
+            if (n > (a->size - std::min(at, a->size)) / sizeof(T))
+                fail(OWL_ERR_NOMEM, "the run needs more than the " + std::to_string(a->size >> 20) +
+                                        " MiB of static device scratch");
+            p_ = reinterpret_cast<T*>(a->base + at);
+            a->used = at + n * sizeof(T);
+            owned_ = false;
+        } else {
+            OWL_HIP(hipMalloc(&p_, n * sizeof(T)));
+            owned_ = true;
+        }
         n_ = n;
     }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_ && owned_) (void)hipFree(p_);
         p_ = nullptr;
         n_ = 0;
     }
@@ -96,6 +127,7 @@ public:
 private:
     T* p_ = nullptr;
     size_t n_ = 0;
+    bool owned_ = true;
 };
 
 }  // namespace owl
