@@ -461,6 +461,31 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
  * sources; no free surface (OWL's FSG free surface runs on its FLIT-built
  * topography-following mesh).  Traces: shot after shot, two blocks (vx, vz)
  * of [nr][nt], each the half-and-half mean of the two velocity grids. */
+/* Parameterisations of a 2D TTI medium (anisotropy_type of OWL's
+ * elastic_tti_2d solver, module_vars.f90:284-340). */
+#define OWL_ANISO_ISO 0      /* vp, vs, rho */
+#define OWL_ANISO_THOMSEN 1  /* vp, vs, rho, epsilon, delta, tilt theta (radians) */
+#define OWL_ANISO_AT 2       /* vp, vs, rho, epsilon, eta, tilt theta (Alkhalifah-Tsvankin) */
+
+/* c11, c13, c15, c33, c35, c55 of n points from the parameters (host only,
+ * elementwise, float32): 'iso' as in prepare_modeling
+ * (elastic_tti_2d/module_vars.f90:287-299); thomsen_to_cij2d and
+ * alkhalifah_tsvankin_to_cij2d (src/lib/module_anisotropy.f90:305-330,
+ * 399-418) with the tilt applied as cij_rotate_2d(..., -theta) (:184-240).
+ * p1, p2: epsilon and delta (THOMSEN) or epsilon and eta (AT); p1, p2,
+ * theta may be NULL for ISO.  cij[0..5]: outputs. */
+owl_status owl_tti2d_cij(int param, long long n, const float *vp, const float *vs, const float *rho,
+                         const float *p1, const float *p2, const float *theta, float *const *cij);
+/* Chain rule from the c_ij gradients of owl_fsg2d_adjoint/gradient to the
+ * parameter gradients (elastic_tti_2d/submodule_fwi_gradient.f90:484-560,
+ * derivative macros src/lib/macro_thomsen_2d.f90): with vp, vs rescaled to
+ * km/s as OWL does; grads[0..1] = vp, vs (ISO), grads[0..3] = vp, vs, p1,
+ * p2 (THOMSEN, AT).  The AT derivatives are the exact ones (OWL's macros
+ * write the exponent 3/2 as an integer division). */
+owl_status owl_tti2d_param_gradient(int param, long long n, const float *vp, const float *vs,
+                                    const float *rho, const float *p1, const float *p2,
+                                    const float *theta, const float *const *grad_cij, float *const *grads);
+
 typedef struct owl_fsg2d owl_fsg2d;
 owl_status owl_fsg2d_create(const owl_acoustic_config *cfg, const float *const *cij, const float *rho,
                             int aniso, int nshot, const owl_shot *shots, owl_fsg2d **out);

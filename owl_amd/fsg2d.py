@@ -28,6 +28,47 @@ def damp_ratio(c, rho, direction):
     return out.value
 
 
+PARAMS = {"iso": (0, ("vp", "vs")), "thomsen": (1, ("vp", "vs", "epsilon", "delta")),
+          "a-t": (2, ("vp", "vs", "epsilon", "eta"))}
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), shape))
+    return a
+
+
+def tti_cij(param, vp, vs, rho, p1=None, p2=None, theta=None):
+    """c11, c13, c15, c33, c35, c55 (dict of float32 arrays, the shape of
+    vp) from OWL's parameterisations (host computation, no GPU): param
+    "iso", "thomsen" (p1 = epsilon, p2 = delta) or "a-t" (p1 = epsilon,
+    p2 = eta); theta = tilt in radians."""
+    code, _ = PARAMS[param]
+    shape = np.shape(vp)
+    args = [_f32(a, shape) for a in (vp, vs, rho)]
+    extra = [_f32(0.0 if a is None else a, shape) for a in (p1, p2, theta)]
+    out = [np.zeros(shape, np.float32) for _ in CIJ]
+    ptrs = (C.c_void_p * 6)(*[a.ctypes.data for a in out])
+    check(lib().owl_tti2d_cij(code, int(np.prod(shape)), *[_ptr(a) for a in args + extra], ptrs))
+    return dict(zip(CIJ, out))
+
+
+def tti_param_gradient(param, grads, vp, vs, rho, p1=None, p2=None, theta=None):
+    """OWL's chain rule from the c_ij gradients (dict with grad_c11 ..
+    grad_c55, as FSG2DBatch.adjoint/gradient return them) to the
+    parameter gradients: dict grad_vp, grad_vs (+ grad_epsilon and
+    grad_delta or grad_eta), velocities in km/s as OWL rescales them."""
+    code, names = PARAMS[param]
+    shape = np.shape(vp)
+    args = [_f32(a, shape) for a in (vp, vs, rho)]
+    extra = [_f32(0.0 if a is None else a, shape) for a in (p1, p2, theta)]
+    g = [_f32(grads["grad_" + k], shape) for k in CIJ]
+    out = [np.zeros(shape, np.float32) for _ in names]
+    gp = (C.c_void_p * 6)(*[a.ctypes.data for a in g])
+    op = (C.c_void_p * 4)(*([a.ctypes.data for a in out] + [None] * (4 - len(out))))
+    check(lib().owl_tti2d_param_gradient(code, int(np.prod(shape)), *[_ptr(a) for a in args + extra], gp, op))
+    return {"grad_" + k: a for k, a in zip(names, out)}
+
+
 class FSG2DBatch(TwoComponentBatch):
     """cij: dict of c11, c13, c15, c33, c35, c55, each (nz, nx); rho (nz, nx).
     aniso: True takes the M-PML damping ratios from the Christoffel scan,
