@@ -20,7 +20,7 @@ class Acoustic3DBatch:
     """vp, rho: (nz, ny, nx).  shots: list of (Geometry, window) where window
     is None (full model) or a dict with nxbeg..nzend (1-based, inclusive)."""
 
-    KERNELS = ("update_p", "update_v", "post_step", "image", "strips", "inject_adjoint")
+    KERNELS = ("update_p", "update_v", "post_step", "image", "strips", "inject_adjoint", "step_fused")
 
     def __init__(self, nx, ny, nz, dx, dy, dz, vp, rho, shots, dt, tmax, ox=0.0, oy=0.0, oz=0.0,
                  pml=15, fdhalf=4, f0factor=1.0, free_surface=False, cc_step_interval=1,

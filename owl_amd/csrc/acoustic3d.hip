@@ -55,6 +55,7 @@ struct Shot3 {
     long long strip0;      // boundary records of the shot ([nt][strip length])
     long long edge0;       // saved interior-face velocities
     long long g0;          // window gradient (nx*ny*nz, x fastest)
+    int ftile0, fntx, fnty, fnzc;  // tiles of the fused step
 };
 
 // Wavefields of one sweep.  The p-sweep reads vx, vy, vz (stencils) and p
@@ -987,6 +988,248 @@ __global__ __launch_bounds__(kWX * kWY) void k3w_update_v(const Shot3* __restric
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Fused step (no free surface): the p-sweep and the v-sweep of one time step
+// in one z-marching kernel, out of place (reads set A, writes set B), so a
+// step reads p, vx, vy, vz, bk, rho once and writes p, vx, vy, vz once: 40 B
+// per point instead of the two sweeps' 24 + 32.
+//
+// A block of kFW wavefronts owns a tile of W = 65 - 2R columns by TY rows
+// and marches down a chunk of kc planes.  Its lanes compute the new pressure
+// on 64 columns (the tile and R - 1 + R halo columns) and TY + 2R - 1 rows,
+// R - 1 planes ahead of the velocities, which need the new pressure R - 1
+// planes below.  Each new-pressure plane goes to an LDS ring of R + 1
+// planes, where the velocity update reads its x and y stencils (one barrier
+// per plane); the z stencils stay in per-thread register queues.  Row slots
+// s < NO of wavefront w hold the owned rows Y0 + w + kFW*s, the other slots
+// the halo rows, so every wavefront owns the same number of rows.  The halo
+// pressure (and its PML memory, read only) is recomputed by both neighbouring
+// tiles; only the owner writes it.  FP32 in the sweeps' operation order:
+// results are bit-identical to k3c_update_p + k3c_update_v.
+// ---------------------------------------------------------------------------
+constexpr int kFW = 8, kFS = 4;
+
+template <int R>
+struct FusedShape {
+    static constexpr int NH = (2 * R - 1 + kFW - 1) / kFW;  // halo slots per wavefront
+    static constexpr int NO = kFS - NH;                      // owned slots per wavefront
+    static constexpr int TY = kFW * NO;                      // owned rows per tile
+    static constexpr int W = 65 - 2 * R;                     // owned columns per tile
+    static constexpr int NROW = TY + 2 * R;                  // ring rows (p rows Y0-R .. Y0+TY+R-1)
+};
+
+// Field sets of the fused step: inputs (A) and outputs (B) of p, vx, vy, vz
+// and the p-sweep's memory variables; the v-sweep's memory variables are
+// touched by their owners only and stay in place.
+struct F3F {
+    const float* p;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    const float* pm[3];
+    float* po;
+    float* vxo;
+    float* vyo;
+    float* vzo;
+    float* pmo[3];
+    float* vm[3];
+};
+
+template <int R>
+__global__ __launch_bounds__(64 * kFW) void k3f_step(const Shot3* __restrict__ shots,
+                                                     const int* __restrict__ ftile_shot, F3F F,
+                                                     const float* __restrict__ bk,
+                                                     const float* __restrict__ rho,
+                                                     const float* __restrict__ prof, Coef c, float dt, float idx,
+                                                     float idy, float idz, int pml, int kc, int ntiles, int remap) {
+    using FS = FusedShape<R>;
+    constexpr int NO = FS::NO, TY = FS::TY, W = FS::W, NROW = FS::NROW, NSL = R + 1;
+    __shared__ float ring[NSL][NROW][64];
+    const int tile = xcd_tile(ntiles, remap);
+    if (tile < 0) return;  // whole block
+    const Shot3 S = shots[ftile_shot[tile]];
+    int lt = tile - S.ftile0;
+    const int tx = lt % S.fntx;
+    lt /= S.fntx;
+    const int ty = lt % S.fnty, tzc = lt / S.fnty;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int lo = 1 - pml - R;
+    const int lane = int(threadIdx.x);
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
+    const int X0 = -pml + 1 + tx * W, Y0 = -pml + 1 + ty * TY;
+    const int x = X0 - R + lane;
+    const bool xin = x >= -pml + 1 && x <= nx + pml;
+    const bool xown = lane >= R && lane < R + W && xin;
+    const int K0 = -pml + 1 + tzc * kc, K1 = min(K0 + kc - 1, nz + pml);
+    const int lx = nx + 2 * pml, ly = ny + 2 * pml, lz = nz + 2 * pml;
+    const float* az_i = prof + S.prof + 4 * lx + 4 * ly;
+    const float* bz_i = az_i + lz;
+    const float* az_h = az_i + 2 * lz;
+    const float* bz_h = az_i + 3 * lz;
+    const bool xsp = xin && (x <= 0 || x >= nx + 1), xsv = xown && (x <= 1 || x >= nx + 1);
+    const float axi = xsp ? prof[S.prof + x + pml - 1] : 0.0f;
+    const float bxi = xsp ? prof[S.prof + lx + x + pml - 1] : 0.0f;
+    const float axh = xsv ? prof[S.prof + 2 * lx + x + pml - 1] : 0.0f;
+    const float bxh = xsv ? prof[S.prof + 3 * lx + x + pml - 1] : 0.0f;
+    const unsigned px = unsigned(S.px), pxy = unsigned(S.pxy);
+    // row of each slot and its position u in the ring (u = y - (Y0 - R))
+    int ys[kFS], us[kFS];
+    bool yin[kFS];
+    float ayi[kFS], byi[kFS];
+    bool ysp[kFS];
+#pragma unroll
+    for (int s2 = 0; s2 < kFS; ++s2) {
+        int u;
+        if (s2 < NO) {
+            u = R + w + kFW * s2;
+        } else {
+            const int h = w + kFW * (s2 - NO);
+            u = h < R ? h : (h < 2 * R - 1 ? TY + h : -1);  // -1: a spare slot, no row
+        }
+        us[s2] = u;
+        ys[s2] = Y0 - R + u;
+        yin[s2] = u >= 0 && ys[s2] >= -pml + 1 && ys[s2] <= ny + pml;
+        ysp[s2] = yin[s2] && (ys[s2] <= 0 || ys[s2] >= ny + 1);
+        ayi[s2] = ysp[s2] ? prof[S.prof + 4 * lx + ys[s2] + pml - 1] : 0.0f;
+        byi[s2] = ysp[s2] ? prof[S.prof + 4 * lx + ly + ys[s2] + pml - 1] : 0.0f;
+    }
+    // element offsets of (x, y, lo) per slot; meaningful only where the
+    // slot's row and the lane's column are inside the update region
+    // (batches are limited to 2^31 elements per field, so 32-bit offsets do)
+    unsigned base[kFS];
+#pragma unroll
+    for (int s2 = 0; s2 < kFS; ++s2)
+        base[s2] = unsigned(S.off) + unsigned(x - S.xb) + unsigned(max(ys[s2], lo) - lo) * unsigned(px);
+    const int kfirst = max(K0 - R, -pml + 1);  // first plane with a pressure update
+    float qz[kFS][2 * R];                      // vz(k-R+1 .. k+R) of each slot
+#pragma unroll
+    for (int s2 = 0; s2 < kFS; ++s2)
+#pragma unroll
+        for (int m = 0; m < 2 * R; ++m) qz[s2][m] = 0.0f;
+#pragma unroll
+    for (int s2 = 0; s2 < kFS; ++s2)
+        if (yin[s2] && xin) {
+#pragma unroll
+            for (int m = 0; m < 2 * R - 1; ++m) qz[s2][m] = F.vz[base[s2] + unsigned(kfirst - R + 1 + m - lo) * pxy];
+        }
+    float pq[NO][2 * R];  // new p(m-R .. m+R-1) of the owned rows, m = k-R+1
+    float rprev[NO];
+#pragma unroll
+    for (int s2 = 0; s2 < NO; ++s2) {
+#pragma unroll
+        for (int m = 0; m < 2 * R; ++m) pq[s2][m] = 0.0f;
+        rprev[s2] = (yin[s2] && xown) ? rho[base[s2] + unsigned(K0 - 1 - lo) * pxy] : 0.0f;
+    }
+    int slot = 0;  // ring slot of plane k
+    for (int k = K0 - R; k <= K1 + R - 1; ++k) {
+        // ---- new pressure at plane k on every slot
+        const bool kin = k >= kfirst && k <= nz + pml;
+        const bool kown = k >= K0 && k <= K1;
+        const bool zsp = kin && (k <= 0 || k >= nz + 1);
+        const float azi = zsp ? az_i[k + pml - 1] : 0.0f, bzi = zsp ? bz_i[k + pml - 1] : 0.0f;
+#pragma unroll
+        for (int s2 = 0; s2 < kFS; ++s2) {
+            float pn = 0.0f;
+            if (kin && yin[s2] && xin) {
+                const unsigned q = base[s2] + unsigned(k - lo) * pxy;
+                qz[s2][2 * R - 1] = F.vz[q + R * pxy];
+                const float* __restrict__ vx = F.vx;
+                const float* __restrict__ vy = F.vy;
+                float sx = c.c[0] * (vx[q + 1] - vx[q]);
+                float sy = c.c[0] * (vy[q + px] - vy[q]);
+                float sz = c.c[0] * (qz[s2][R] - qz[s2][R - 1]);
+#pragma unroll
+                for (int m = 2; m <= R; ++m) {
+                    sx = sx + c.c[m - 1] * (vx[q + m] - vx[q - (m - 1)]);
+                    sy = sy + c.c[m - 1] * (vy[q + m * px] - vy[q - (m - 1) * px]);
+                    sz = sz + c.c[m - 1] * (qz[s2][R - 1 + m] - qz[s2][R - m]);
+                }
+                float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+                const bool own = kown && xown && s2 < NO;
+                if (xsp) {
+                    const float mem = axi * dxv + bxi * F.pm[0][q];
+                    if (own) F.pmo[0][q] = mem;
+                    dxv = dxv + mem;
+                }
+                if (ysp[s2]) {
+                    const float mem = ayi[s2] * dyv + byi[s2] * F.pm[1][q];
+                    if (own) F.pmo[1][q] = mem;
+                    dyv = dyv + mem;
+                }
+                if (zsp) {
+                    const float mem = azi * dzv + bzi * F.pm[2][q];
+                    if (own) F.pmo[2][q] = mem;
+                    dzv = dzv + mem;
+                }
+                pn = F.p[q] - dt * bk[q] * (dxv + dyv + dzv);
+                if (own) F.po[q] = pn;
+#pragma unroll
+                for (int m = 0; m < 2 * R - 1; ++m) qz[s2][m] = qz[s2][m + 1];
+            }
+            if (us[s2] >= 0) ring[slot][us[s2]][lane] = pn;
+            if (s2 < NO) pq[s2][2 * R - 1] = pn;
+            __builtin_amdgcn_sched_barrier(0);  // one slot's loads in flight at a time
+        }
+        __syncthreads();
+        // ---- velocities at plane m = k - R + 1 on the owned rows
+        const int m = k - R + 1;
+        if (m >= K0) {
+            const int sm = slot >= R - 1 ? slot - (R - 1) : slot + 2;  // ring slot of plane m
+            const bool zsv = m <= 1 || m >= nz + 1;
+            const float azv = zsv ? az_h[m + pml - 1] : 0.0f, bzv = zsv ? bz_h[m + pml - 1] : 0.0f;
+#pragma unroll
+            for (int s2 = 0; s2 < NO; ++s2) {
+                if (!(yin[s2] && xown)) continue;
+                const int u = us[s2];
+                const float(*rg)[64] = ring[sm];
+                float sx = c.c[0] * (rg[u][lane] - rg[u][lane - 1]);
+                float sy = c.c[0] * (rg[u][lane] - rg[u - 1][lane]);
+                float sz = c.c[0] * (pq[s2][R] - pq[s2][R - 1]);
+#pragma unroll
+                for (int mm = 2; mm <= R; ++mm) {
+                    sx = sx + c.c[mm - 1] * (rg[u][lane + mm - 1] - rg[u][lane - mm]);
+                    sy = sy + c.c[mm - 1] * (rg[u + mm - 1][lane] - rg[u - mm][lane]);
+                    sz = sz + c.c[mm - 1] * (pq[s2][R - 1 + mm] - pq[s2][R - mm]);
+                }
+                float dxv = idx * sx, dyv = idy * sy, dzv = idz * sz;
+                const unsigned q = base[s2] + unsigned(m - lo) * pxy;
+                const float r0 = rho[q];
+                const float rx = 0.5f * (r0 + rho[q - 1]);
+                const float ry = 0.5f * (r0 + rho[q - px]);
+                const float rz = 0.5f * (r0 + rprev[s2]);
+                rprev[s2] = r0;
+                if (xsv) {
+                    const float mem = axh * dxv + bxh * F.vm[0][q];
+                    F.vm[0][q] = mem;
+                    dxv = dxv + mem;
+                }
+                const int y = ys[s2];
+                if (y <= 1 || y >= ny + 1) {
+                    const float mem = prof[S.prof + 4 * lx + 2 * ly + y + pml - 1] * dyv +
+                                      prof[S.prof + 4 * lx + 3 * ly + y + pml - 1] * F.vm[1][q];
+                    F.vm[1][q] = mem;
+                    dyv = dyv + mem;
+                }
+                if (zsv) {
+                    const float mem = azv * dzv + bzv * F.vm[2][q];
+                    F.vm[2][q] = mem;
+                    dzv = dzv + mem;
+                }
+                F.vxo[q] = F.vx[q] - dt / rx * dxv;
+                F.vyo[q] = F.vy[q] - dt / ry * dyv;
+                F.vzo[q] = F.vz[q] - dt / rz * dzv;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < NO; ++s2)
+#pragma unroll
+            for (int mm = 0; mm < 2 * R - 1; ++mm) pq[s2][mm] = pq[s2][mm + 1];
+        slot = slot == R ? 0 : slot + 1;
+    }
+}
+
 // Source injection of step t (add_source, module_wavefield.f90:322-444:
 // explosions into p, forces into vx, vy, vz) then receivers of step t+1;
 // one block per shot so that the samples see the injected source.  Taps are
@@ -1297,6 +1540,11 @@ struct owl_acoustic3d {
     int cy = 4;        // rows per block of the column variant
     int remap = 0;     // XCD-contiguous tile order
     int launch_code = 1;  // the public variant code of the current shape
+    bool fused = false;   // forward steps by the fused kernel (k3f_step; no free surface)
+    int fkc = 64;         // z planes per tile of the fused kernel
+    std::vector<int> ftile_shot;
+    DevBuf<int> d_ftile_shot;
+    DevBuf<float> mB[3];  // second set of the p-sweep memory variables (fused ping-pong)
     int fs = 0;        // free surface at z index 1
     float dt = 0, idx = 0, idy = 0, idz = 0;
     a3::Coef coef{};
@@ -1364,6 +1612,26 @@ int csr_by_cell(std::vector<Tap3>& st, std::vector<int>& beg, std::vector<Tap3>&
     return ncell;
 }
 
+// Owned columns and rows per tile of the fused step for radius R.
+void fused_shape(int R, int& W, int& TY) {
+    switch (R) {
+#define OWL_FS_CASE(RR)                  \
+    case RR:                             \
+        W = FusedShape<RR>::W;           \
+        TY = FusedShape<RR>::TY;         \
+        break;
+        OWL_FS_CASE(2)
+        OWL_FS_CASE(3)
+        OWL_FS_CASE(4)
+        OWL_FS_CASE(5)
+        OWL_FS_CASE(6)
+        OWL_FS_CASE(7)
+        OWL_FS_CASE(8)
+#undef OWL_FS_CASE
+        default: fail(OWL_ERR_ARG, "fdhalf must be 2..8");
+    }
+}
+
 // Tile table of the current launch shape: per shot x-y tiles of the
 // variant's block times z segments of kc planes, all shots in one grid.
 void retile(owl_acoustic3d* b) {
@@ -1379,8 +1647,23 @@ void retile(owl_acoustic3d* b) {
         S.tile0 = int(b->tile_shot.size());
         b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nty * S.nzc, int(s));
     }
+    b->ftile_shot.clear();
+    if (b->R <= 8) {
+        int W = 0, TY = 0;
+        fused_shape(b->R, W, TY);
+        for (size_t s = 0; s < b->sd.size(); ++s) {
+            Shot3& S = b->sd[s];
+            S.fntx = (S.nx + 2 * pml + W - 1) / W;
+            S.fnty = (S.ny + 2 * pml + TY - 1) / TY;
+            S.fnzc = (S.nz + 2 * pml + b->fkc - 1) / b->fkc;
+            S.ftile0 = int(b->ftile_shot.size());
+            b->ftile_shot.insert(b->ftile_shot.end(), size_t(S.fntx) * S.fnty * S.fnzc, int(s));
+        }
+    }
+    if (b->ftile_shot.empty()) b->ftile_shot.push_back(0);
     b->d_shots.upload(b->sd, b->stream);
     b->d_tile_shot.upload(b->tile_shot, b->stream);
+    b->d_ftile_shot.upload(b->ftile_shot, b->stream);
     OWL_HIP(hipStreamSynchronize(b->stream));
 }
 
@@ -1717,6 +2000,60 @@ void sweeps3(owl_acoustic3d* b, const F3& in_place_or_pair, float dt) {
     b->prof.end(st, ev);
 }
 
+// Algorithmic bytes of one fused step: p, vx, vy, vz read and written and
+// bk, rho read (40 B per point), 8 B per slab cell and damped direction for
+// each of the two sets of memory variables.
+double fused_bytes(const owl_acoustic3d* b) {
+    double bp, bv;
+    sweep_bytes(b, bp, bv);
+    double pts = 0;
+    for (auto& S : b->sd)
+        pts += (S.nx + 2.0 * b->pml) * (S.ny + 2.0 * b->pml) * (S.nz + 2.0 * b->pml);
+    return bp + bv - 16.0 * pts;
+}
+
+struct Set3 {
+    float* f[4];    // p, vx, vy, vz
+    float* pm[3];   // p-sweep memory variables
+};
+
+// One time step by the fused kernel from set `in` to set `out`.
+template <int R>
+void fused_step3(owl_acoustic3d* b, const Set3& in, const Set3& out, float dt) {
+    F3F F{};
+    F.p = in.f[0];
+    F.vx = in.f[1];
+    F.vy = in.f[2];
+    F.vz = in.f[3];
+    F.po = out.f[0];
+    F.vxo = out.f[1];
+    F.vyo = out.f[2];
+    F.vzo = out.f[3];
+    for (int k = 0; k < 3; ++k) {
+        F.pm[k] = in.pm[k];
+        F.pmo[k] = out.pm[k];
+        F.vm[k] = b->m[3 + k].get();
+    }
+    const int ntiles = int(b->ftile_shot.size());
+    const unsigned grid = unsigned((ntiles + kXcd - 1) / kXcd * kXcd);
+    const int ev = b->prof.begin(b->stream, 6, fused_bytes(b));
+    hipLaunchKernelGGL(k3f_step<R>, dim3(grid), dim3(64, kFW), 0, b->stream, b->d_shots.get(), b->d_ftile_shot.get(),
+                       F, b->d_bk.get(), b->d_rho.get(), b->d_prof.get(), b->coef, dt, b->idx, b->idy, b->idz,
+                       b->pml, b->fkc, ntiles, 1);
+    b->prof.end(b->stream, ev);
+}
+
+Set3 set_a(owl_acoustic3d* b) {
+    return Set3{{b->f[0].get(), b->f[1].get(), b->f[2].get(), b->f[3].get()},
+                {b->m[0].get(), b->m[1].get(), b->m[2].get()}};
+}
+Set3 set_b(owl_acoustic3d* b) {
+    for (auto& x : b->fB) x.alloc(b->field_elems);
+    for (auto& x : b->mB) x.alloc(b->field_elems);
+    return Set3{{b->fB[0].get(), b->fB[1].get(), b->fB[2].get(), b->fB[3].get()},
+                {b->mB[0].get(), b->mB[1].get(), b->mB[2].get()}};
+}
+
 void maybe_collect(owl_acoustic3d* b) {
     if (b->prof.pending() > 20000) {
         OWL_HIP(hipStreamSynchronize(b->stream));
@@ -1745,22 +2082,39 @@ void run_forward3(owl_acoustic3d* b, bool save_boundary) {
     float* mm[6];
     for (int k = 0; k < 6; ++k) mm[k] = b->m[k].get();
     const F3 F = f3_in_place(b->f[0].get(), b->f[1].get(), b->f[2].get(), b->f[3].get(), mm);
+    const bool fused = b->fused && !b->fs;
+    Set3 cur = set_a(b), nxt{};
+    if (fused) {
+        nxt = set_b(b);
+        for (auto& x : b->fB) x.zero(st);
+        for (auto& x : b->mB) x.zero(st);
+    }
     OWL_HIP(hipEventRecord(b->ev0, st));
     // step order (compute_forward): save strips(t) -> record -> update -> source;
     // the record of step 1 is the zero initial state.  The state at the start
     // of step nt is the final state the reverse pass starts from.
     for (int t = 1; t <= b->nt - 1; ++t) {
-        if (save_boundary) strips_launch(b, R, b->f[1].get(), b->f[2].get(), b->f[3].get(), t, 1);
-        sweeps3<R>(b, F, b->dt);
+        if (save_boundary) strips_launch(b, R, cur.f[1], cur.f[2], cur.f[3], t, 1);
+        if (fused) {
+            fused_step3<R>(b, cur, nxt, b->dt);
+            std::swap(cur, nxt);
+        } else {
+            sweeps3<R>(b, F, b->dt);
+        }
         const int ev = b->prof.begin(st, 2, 0);
         hipLaunchKernelGGL(k3_post_step, dim3(unsigned(b->sd.size())), dim3(1024), 0, st, b->d_shots.get(),
-                           b->f[0].get(), b->f[1].get(), b->f[2].get(), b->f[3].get(), b->d_inj_beg.get(),
+                           cur.f[0], cur.f[1], cur.f[2], cur.f[3], b->d_inj_beg.get(),
                            b->d_inj.get(), b->d_amp.get(), b->nt, t, b->d_rec_beg.get(), b->d_rec.get(),
                            b->d_rec_w.get(), b->d_traces.get());
         b->prof.end(st, ev);
         maybe_collect(b);
     }
-    if (save_boundary) strips_launch(b, R, b->f[1].get(), b->f[2].get(), b->f[3].get(), b->nt, 1);
+    if (save_boundary) strips_launch(b, R, cur.f[1], cur.f[2], cur.f[3], b->nt, 1);
+    // the final state lives in b->f for the reverse pass
+    if (fused && cur.f[0] != b->f[0].get()) {
+        for (int k = 0; k < 4; ++k) std::swap(b->f[k], b->fB[k]);
+        for (int k = 0; k < 3; ++k) std::swap(b->m[k], b->mB[k]);
+    }
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
@@ -1914,6 +2268,7 @@ template <int R>
 void autotune3(owl_acoustic3d* b, int steps) {
     static const int kVariants[] = {1, 2, 3, 5, 6, 8, 24, 9, 10, 11, 13, 0};
     static const int kPlanes[] = {32, 64, 128};
+    static const int kFusedPlanes[] = {32, 64, 96, 128, 192};
     hipStream_t st = b->stream;
     for (auto& x : b->f) x.zero(st);
     for (auto& x : b->m) x.zero(st);
@@ -1922,18 +2277,23 @@ void autotune3(owl_acoustic3d* b, int steps) {
     const F3 F = f3_in_place(b->f[0].get(), b->f[1].get(), b->f[2].get(), b->f[3].get(), mm);
     const bool prof_on = b->prof.on();
     b->prof.enable(false);
+    b->fused = false;
+    auto timed = [&](auto&& step) {
+        step();  // warm-up
+        OWL_HIP(hipEventRecord(b->ev0, st));
+        for (int t = 0; t < steps; ++t) step();
+        OWL_HIP(hipEventRecord(b->ev1, st));
+        OWL_HIP(hipStreamSynchronize(st));
+        float ms = 0.f;
+        OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        return ms;
+    };
     float best = -1.f;
     int best_v = b->launch_code, best_k = b->kc;
     for (int v : kVariants)
         for (int kc : kPlanes) {
             set_shape(b, kc, v);
-            sweeps3<R>(b, F, b->dt);  // warm-up
-            OWL_HIP(hipEventRecord(b->ev0, st));
-            for (int t = 0; t < steps; ++t) sweeps3<R>(b, F, b->dt);
-            OWL_HIP(hipEventRecord(b->ev1, st));
-            OWL_HIP(hipStreamSynchronize(st));
-            float ms = 0.f;
-            OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            const float ms = timed([&] { sweeps3<R>(b, F, b->dt); });
             if (best < 0 || ms < best) {
                 best = ms;
                 best_v = v;
@@ -1942,6 +2302,31 @@ void autotune3(owl_acoustic3d* b, int steps) {
         }
     set_shape(b, best_k, best_v);
     b->launch_code = best_v;
+    if (!b->fs && b->field_elems < (size_t(1) << 32)) {
+        // the fused step against the best pair of sweeps
+        const Set3 A = set_a(b), B = set_b(b);
+        for (auto& x : b->fB) x.zero(st);
+        for (auto& x : b->mB) x.zero(st);
+        float fbest = -1.f;
+        int fk = b->fkc;
+        for (int kc : kFusedPlanes) {
+            b->fkc = kc;
+            retile(b);
+            const float ms = timed([&] { fused_step3<R>(b, A, B, b->dt); });
+            if (fbest < 0 || ms < fbest) {
+                fbest = ms;
+                fk = kc;
+            }
+        }
+        b->fkc = fk;
+        retile(b);
+        if (fbest < best) {
+            b->fused = true;
+            b->launch_code = 32;
+        }
+        for (auto& x : b->fB) x.zero(st);
+        for (auto& x : b->mB) x.zero(st);
+    }
     for (auto& x : b->f) x.zero(st);
     for (auto& x : b->m) x.zero(st);
     OWL_HIP(hipStreamSynchronize(st));
@@ -1991,6 +2376,19 @@ namespace {
 // columns with 4, 8, 16 rows; +4 (or +16 for 8): XCD-contiguous tile order.
 void set_shape(owl_acoustic3d* b, int planes, int variant) {
     if (planes < 1 || variant < 0) fail(OWL_ERR_ARG, "bad launch shape");
+    if (variant == 32) {
+        // fused forward step; the reverse pass keeps the sweeps' shape
+        if (b->fs) fail(OWL_ERR_ARG, "the fused step does not cover the free surface");
+        if (b->field_elems >= (size_t(1) << 32))
+            fail(OWL_ERR_ARG, "the fused step addresses fields of fewer than 2^32 elements");
+        b->fused = true;
+        if (planes != b->fkc) {
+            b->fkc = planes;
+            retile(b);
+        }
+        return;
+    }
+    b->fused = false;
     int kind, cy = b->cy, rm;
     if (variant == 8 || variant == 24) {
         kind = 2;
@@ -2031,7 +2429,7 @@ owl_status owl_acoustic3d_autotune(owl_acoustic3d* b, int steps, int* planes, in
     return guarded([&] {
         if (!b || steps < 1) fail(OWL_ERR_ARG, "bad argument");
         dispatch_R3(b->R, [&](auto r) { autotune3<decltype(r)::value>(b, steps); });
-        if (planes) *planes = b->kc;
+        if (planes) *planes = b->fused ? b->fkc : b->kc;
         if (variant) *variant = b->launch_code;
     });
 }

@@ -60,7 +60,7 @@ def test_forward3d_matches_oracle(oracle32, gpu, R):
     print(f"case: 3D forward R={R}", flush=True)
     b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=R)
     outs = [b.forward()]
-    for kc, var in ((16, 0), (64, 8), (5, 24), (64, 9), (3, 15)):
+    for kc, var in ((16, 0), (64, 8), (5, 24), (64, 9), (3, 15), (16, 32), (5, 32), (64, 32)):
         b.set_launch(kc, var)
         outs.append(b.forward())
     for o in outs[1:]:
@@ -83,11 +83,46 @@ def test_forward3d_zchunk_invariance(gpu):
     print("case: 3D z-chunk invariance", flush=True)
     b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, [(geom, None)], 0.001, 0.1, pml=8)
     ref = b.forward()[0]
-    for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (1, 8), (13, 24), (64, 8), (1, 9), (13, 10), (64, 13), (64, 1)):
+    for kc, var in ((1, 1), (7, 0), (13, 1), (200, 0), (1, 8), (13, 24), (64, 8), (1, 9), (13, 10), (64, 13), (64, 1),
+                    (1, 32), (7, 32), (13, 32), (200, 32)):
         b.set_launch(kc, var)
         assert np.array_equal(b.forward()[0], ref), (kc, var)
     kc, var = b.autotune(1)
     assert np.array_equal(b.forward()[0], ref), ("autotuned", kc, var)
+    b.close()
+
+
+@pytest.mark.parametrize("tmax", [0.13, 0.131])
+def test_fused_step_force_sources_and_owl_gradient(gpu, tmax):
+    """The fused step (variant 32) against the sweeps (variant 8) on a
+    batch with an oblique force, an explosion and uneven windows: traces
+    bit-identical for an odd and an even number of steps (the fused step
+    ping-pongs between two field sets), and OWL's gradient from the fused
+    forward's final state and boundary records bit-identical too."""
+    nz, ny, nx, dx, dt, pml = 24, 30, 70, 10.0, 0.001, 10
+    vp, rho = _model(nz, ny, nx, seed=5)
+    geoms = [
+        (Geometry(id=1, srcr=[Source(x=323.0, y=141.7, z=95.0, f0=20.0, mechanism="force", polar=0.7,
+                                     azimuth=0.4)],
+                  recr=[Receiver(x=20.0 + 9.0 * k, y=60.5, z=30.0) for k in range(70)]), None),
+        (Geometry(id=2, srcr=[Source(x=410.0, y=150.0, z=80.0, f0=20.0)],
+                  recr=[Receiver(x=330.0 + 10.0 * k, y=140.0, z=40.0) for k in range(25)]),
+         dict(nxbeg=30, nxend=70, nybeg=5, nyend=30, nzbeg=1, nzend=20)),
+    ]
+    print(f"case: 3D fused step, tmax {tmax}", flush=True)
+    b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, dt, tmax, pml=pml, fdhalf=4)
+    b.set_launch(16, 8)
+    ref = b.forward(save_boundary=True)
+    adj = [(0.3 * t).astype(np.float32) for t in ref]
+    gref = b.adjoint(adj)
+    b.set_launch(9, 32)
+    got = b.forward(save_boundary=True)
+    for a, c in zip(got, ref):
+        assert np.abs(c).max() > 0
+        assert np.array_equal(a, c)
+    g = b.adjoint(adj)
+    for (a1, a2), (c1, c2) in zip(g, gref):
+        assert np.array_equal(a1, c1) and np.array_equal(a2, c2)
     b.close()
 
 
