@@ -212,6 +212,18 @@ owl_status owl_acoustic2d_forward_static(const owl_acoustic_config *cfg, const f
 owl_status owl_acoustic2d_gradient(owl_acoustic2d *b, const float *obs, int mode, int ckpt,
                                    int ckpt_param, int store, double *misfit, double *grad_vp,
                                    double *grad_rho, float *traces_out);
+/* energy_precond of OWL's gradient (on != 0): the OWL-algorithm gradient of
+ * each shot window divided by sqrt(source energy x receiver energy) of its
+ * imaging terms, each energy raised by 1e-3 of its largest value
+ * (acoustic_iso_2d/submodule_fwi_gradient.f90:173-184, 223-265).  Applies to
+ * OWL_GRAD_OWL and owl_acoustic2d_adjoint; default off. */
+owl_status owl_acoustic2d_set_energy_precond(owl_acoustic2d *b, int on);
+/* Misfit only: forward modelling of every shot and the per-shot L2 misfit
+ * with the trace-discard rule, no adjoint (compute_gradient_shots with
+ * yn_misfit_only = .true., src/inverse/2d/module_gradient.f90:93-388, as
+ * the step-size trials of module_inversion_step_size.f90:562-587 call it).
+ * obs may be NULL to keep the observed data already set. */
+owl_status owl_acoustic2d_misfit(owl_acoustic2d *b, const float *obs, double *misfit, float *traces_out);
 
 /* Differentiable path (exact adjoint): model every shot and keep the forward
  * states the adjoint needs under the given storage policy (OWL_CKPT_*,
@@ -461,6 +473,25 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
  * sources; no free surface (OWL's FSG free surface runs on its FLIT-built
  * topography-following mesh).  Traces: shot after shot, two blocks (vx, vz)
  * of [nr][nt], each the half-and-half mean of the two velocity grids. */
+/* ---- Misfits and adjoint sources -------------------------------------------
+ * misfit_type of OWL's FWI (module_inversion_adjoint_source.f90). */
+#define OWL_MISFIT_WAVEFORM 0  /* sum (obs - syn)^2; adjoint syn - obs (:371-383) */
+#define OWL_MISFIT_CORR 1      /* zero-lag normalised correlation (:391-413) */
+#define OWL_MISFIT_ENVELOPE 2  /* envelope difference (:424-452) */
+#define OWL_MISFIT_PHASE 3     /* instantaneous-phase difference (:460-481) */
+
+/* Misfit and adjoint source of the ntr traces of one shot component, on the
+ * GPU (compute_adjoint_source, module_inversion_adjoint_source.f90:985-1160,
+ * without resampling): obs, syn and adj are [ntr][nt]; weight (may be NULL)
+ * holds the receiver weights; threshold is trace_discard_threshold.  A trace
+ * with zero weight, an obs or syn norm below threshold x the largest of the
+ * shot, or a zero norm gets a zero adjoint and misfit.  misfit[ntr]: per-trace
+ * misfit in FP64 (OWL sums them into the shot's misfit).  Envelope and phase
+ * use the analytic signal of a length-nt FFT (FLIT's hilbert, envelope and
+ * instant_phase are not in the reference tree). */
+owl_status owl_adjoint_source(int type, int ntr, int nt, const float *obs, const float *syn,
+                              const float *weight, float threshold, float *adj, double *misfit);
+
 /* Parameterisations of a 2D TTI medium (anisotropy_type of OWL's
  * elastic_tti_2d solver, module_vars.f90:284-340). */
 #define OWL_ANISO_ISO 0      /* vp, vs, rho */
@@ -477,9 +508,9 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
 owl_status owl_tti2d_cij(int param, long long n, const float *vp, const float *vs, const float *rho,
                          const float *p1, const float *p2, const float *theta, float *const *cij);
 /* Chain rule from the c_ij gradients of owl_fsg2d_adjoint/gradient to the
- * parameter gradients (elastic_tti_2d/submodule_fwi_gradient.f90:484-560,
+ * parameter gradients (elastic_tti_2d/submodule_fwi_gradient.f90:476-560,
  * derivative macros src/lib/macro_thomsen_2d.f90): with vp, vs rescaled to
- * km/s as OWL does; grads[0..1] = vp, vs (ISO), grads[0..3] = vp, vs, p1,
+ * km/s and rho to g/cm^3 as OWL does; grads[0..1] = vp, vs (ISO), grads[0..3] = vp, vs, p1,
  * p2 (THOMSEN, AT).  The AT derivatives are the exact ones (OWL's macros
  * write the exponent 3/2 as an integer division). */
 owl_status owl_tti2d_param_gradient(int param, long long n, const float *vp, const float *vs,

@@ -281,6 +281,10 @@ class Oracle:
             raise RuntimeError(f"oracle acoustic2d_forward failed ({rc})")
         return tr
 
+    def set_energy_precond2(self, on):
+        """OWL's energy_precond for acoustic2d_gradient_owl."""
+        self._fn("set_energy_precond2")(int(bool(on)))
+
     def acoustic2d_gradient_owl(self, g, vp, rho, sources, receivers, adj):
         """OWL's (grad_vp, grad_rho) as (nz, nx) arrays, plus the forward traces."""
         G = self._grid2d(g)

@@ -628,9 +628,14 @@ int ORA_SYM(acoustic2d_forward)(const ora_grid2d *g, const float *vp, const floa
     return 0;
 }
 
+/* energy_precond of the 2D gradient below (submodule_fwi_gradient.f90:
+ * 173-184, 223-265); off by default. */
+static int g_energy_precond2 = 0;
+void ORA_SYM(set_energy_precond2)(int on) { g_energy_precond2 = on != 0; }
+
 /* OWL's gradient: compute_adjoint + compute_gradient,
- * acoustic_iso_2d/submodule_fwi_gradient.f90:31-269 (energy preconditioning
- * off).  Re-runs the forward with boundary saving, then the reverse loop
+ * acoustic_iso_2d/submodule_fwi_gradient.f90:31-269 (energy
+ * preconditioning as set by set_energy_precond2).  Re-runs the forward with boundary saving, then the reverse loop
  * t = nt .. hnt.  adj: [nr][nt] adjoint source traces (syn - obs).
  * grad_vp, grad_rho: (nz, nx) z-fast outputs = the transposed .grd arrays
  * (:187-195), NaN/Inf mapped to zero (FLIT return_normal).
@@ -665,6 +670,9 @@ int ORA_SYM(acoustic2d_gradient_owl)(const ora_grid2d *g, const float *vp, const
         ORA_SYM(locate_point)(2, rec[k].x, 0.0f, rec[k].z, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, &rp[k]);
     const int nx = g->nx, nz = g->nz;
     real *gv = zalloc((size_t)nx * nz), *gr = zalloc((size_t)nx * nz);
+    const int ep = g_energy_precond2;
+    real *esv = zalloc((size_t)nx * nz), *erv = zalloc((size_t)nx * nz);
+    real *esa = zalloc((size_t)nx * nz), *era = zalloc((size_t)nx * nz);
     const real mdt = -(real)g->dt;
     const int cc = g->cc_step_interval > 0 ? g->cc_step_interval : 1;
     for (int t = g->nt; t >= hnt; --t) {
@@ -705,9 +713,28 @@ int ORA_SYM(acoustic2d_gradient_owl)(const ora_grid2d *g, const float *vp, const
                     real sz = (F.vz[q] + F.vz[q + L->NX]) - (prev_vz[q] + prev_vz[q + L->NX]);
                     real rz = A.vz[q] + A.vz[q + L->NX];
                     gr[o] = gr[o] + sx * rx + sz * rz;
+                    if (ep) {
+                        esv[o] = esv[o] + sp_ * sp_ / (M.rho[q] * (v * v * v));
+                        erv[o] = erv[o] + rp_ * rp_ / (M.rho[q] * (v * v * v));
+                        esa[o] = esa[o] + (sx * sx + sz * sz);
+                        era[o] = era[o] + (rx * rx + rz * rz);
+                    }
                 }
         }
     }
+    if (ep) {
+        real m[4] = {0, 0, 0, 0};
+        real *e[4] = {esv, erv, esa, era};
+        for (int k = 0; k < 4; ++k)
+            for (size_t o = 0; o < (size_t)nx * nz; ++o)
+                if (e[k][o] > m[k]) m[k] = e[k][o];
+        for (size_t o = 0; o < (size_t)nx * nz; ++o) {
+            for (int k = 0; k < 4; ++k) e[k][o] = e[k][o] + (real)1.0e-3 * m[k];
+            gv[o] = gv[o] / sqrt(esv[o] * erv[o]);
+            gr[o] = gr[o] / sqrt(esa[o] * era[o]);
+        }
+    }
+    free(esv); free(erv); free(esa); free(era);
     for (int j = 1; j <= nz; ++j)
         for (int i = 1; i <= nx; ++i) {
             real a = gv[(size_t)(i - 1) + (size_t)(j - 1) * nx];

@@ -23,6 +23,9 @@ OWL_GRAD_OWL, OWL_GRAD_EXACT = 0, 1
 OWL_CKPT_FULL, OWL_CKPT_UNIFORM, OWL_CKPT_BINOMIAL = 0, 1, 2
 OWL_STORE_F32, OWL_STORE_F16, OWL_STORE_BF16 = 0, 1, 2
 OWL_ANISO_ISO, OWL_ANISO_THOMSEN, OWL_ANISO_AT = 0, 1, 2
+OWL_MISFIT_WAVEFORM, OWL_MISFIT_CORR, OWL_MISFIT_ENVELOPE, OWL_MISFIT_PHASE = 0, 1, 2, 3
+MISFITS = {"waveform": OWL_MISFIT_WAVEFORM, "corr": OWL_MISFIT_CORR, "envelope": OWL_MISFIT_ENVELOPE,
+           "phase": OWL_MISFIT_PHASE}
 
 MECHANISMS = {"explosion": OWL_EXPLOSION, "force": OWL_FORCE}
 WAVELETS = {"ricker": OWL_RICKER, "gaussian_deriv_deriv": OWL_RICKER, "gaussian_deriv2": OWL_RICKER,
@@ -92,7 +95,10 @@ _SIGNATURES = {
     "owl_acoustic2d_set_model": (C.c_int, [_P, _P, _P]),
     "owl_acoustic2d_forward": (C.c_int, [_P, _P, C.c_int]),
     "owl_acoustic2d_forward_static": (C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
+    "owl_acoustic2d_misfit": (C.c_int, [_P, _P, _P, _P]),
+    "owl_acoustic2d_set_energy_precond": (C.c_int, [_P, C.c_int]),
     "owl_tti2d_cij": (C.c_int, [C.c_int, C.c_longlong, _P, _P, _P, _P, _P, _P, _P]),
+    "owl_adjoint_source": (C.c_int, [C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_float, _P, _P]),
     "owl_tti2d_param_gradient": (C.c_int, [C.c_int, C.c_longlong, _P, _P, _P, _P, _P, _P, _P, _P]),
     "owl_acoustic2d_gradient": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
     "owl_acoustic2d_adjoint": (C.c_int, [_P, _P, _P, _P]),

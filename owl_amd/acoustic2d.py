@@ -190,6 +190,21 @@ class Acoustic2DBatch:
         return (mis, np.ascontiguousarray(gv), np.ascontiguousarray(gr),
                 self._split(syn) if syn is not None else None)
 
+    def set_energy_precond(self, on=True):
+        """OWL's energy_precond for the OWL-algorithm gradient."""
+        check(lib().owl_acoustic2d_set_energy_precond(self._h, int(bool(on))))
+
+    def misfit(self, obs=None, want_traces=False):
+        """Per-shot L2 misfit of the current model against the observed data
+        (forward only, no adjoint), and the synthetic traces if asked."""
+        flat = None if obs is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(a, np.float32).ravel() for a in obs]))
+        mis = np.zeros(self.nshot, np.float64)
+        syn = np.zeros(sum(self.nr) * self.nt, np.float32) if want_traces else None
+        check(lib().owl_acoustic2d_misfit(self._h, None if flat is None else _ptr(flat), _ptr(mis),
+                                          None if syn is None else _ptr(syn)))
+        return mis, (self._split(syn) if syn is not None else None)
+
     def forward_store(self, ckpt=_lib.OWL_CKPT_FULL, ckpt_param=0, store=_lib.OWL_STORE_F32):
         """Model every shot and keep the states the exact adjoint needs."""
         flat = np.zeros(sum(self.nr) * self.nt, np.float32)

@@ -57,6 +57,7 @@ struct Shot {
     int nxbeg, nzbeg;     // window origin in the full model (1-based)
     int ftile0, fntx;     // fused step: first column-strip tile, strips per row
     int atile0;           // fused adjoint: first column-strip tile (its own z-chunk rows)
+    int hnt;              // last reverse step OWL images for this shot (its first source's hnt)
 };
 
 // Element of the model arrays (bk, rho) at field element q of shot S.
@@ -568,8 +569,10 @@ __global__ __launch_bounds__(256) void k_image_owl(const Shot* __restrict__ shot
                                                    const float* __restrict__ vxr,
                                                    const float* __restrict__ vzr,
                                                    const float* __restrict__ den, double* gv,
-                                                   double* gr, int prev_zero, int pml) {
+                                                   double* gr, int prev_zero, int pml, double* energy,
+                                                   size_t ne, int t) {
     const Shot S = shots[blockIdx.y];
+    if (t < S.hnt) return;  // this shot's reverse loop has ended (whole block)
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const size_t pitch = size_t(S.pitch);
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nx * nz; e += gridDim.x * blockDim.x) {
@@ -592,6 +595,41 @@ __global__ __launch_bounds__(256) void k_image_owl(const Shot* __restrict__ shot
         const float rz = vzr[q] + vzr[q + pitch];
         gv[S.g0 + e] = gv[S.g0 + e] - double(inc_v);
         gr[S.g0 + e] = gr[S.g0 + e] + double(sx * rx) + double(sz * rz);
+        if (energy) {
+            // energy preconditioning (compute_gradient, :223-265): source and
+            // receiver energies of the two imaging terms
+            const size_t g = size_t(S.g0 + e);
+            energy[g] += double(sp * sp / den[S.g0 + e]);
+            energy[ne + g] += double(pr[q] * pr[q] / den[S.g0 + e]);
+            energy[2 * ne + g] += double(sx * sx + sz * sz);
+            energy[3 * ne + g] += double(rx * rx + rz * rz);
+        }
+    }
+}
+
+// Energy preconditioning of one shot window per block (compute_adjoint,
+// submodule_fwi_gradient.f90:173-184): each energy plus 1e-3 of its largest
+// value, the gradients divided by sqrt(source energy x receiver energy).
+__global__ __launch_bounds__(256) void k_energy_precond(const Shot* __restrict__ shots, double* gv, double* gr,
+                                                        const double* __restrict__ energy, size_t ne) {
+    const Shot S = shots[blockIdx.x];
+    const int n = S.nx * S.nz;
+    __shared__ double red[4][256];
+    double m[4] = {0, 0, 0, 0};
+    for (int e = threadIdx.x; e < n; e += blockDim.x)
+        for (int k = 0; k < 4; ++k) m[k] = fmax(m[k], energy[k * ne + S.g0 + e]);
+    for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = m[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double a0 = 1.0e-3 * red[0][0], a1 = 1.0e-3 * red[1][0], a2 = 1.0e-3 * red[2][0], a3 = 1.0e-3 * red[3][0];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const size_t g = size_t(S.g0 + e);
+        gv[g] = gv[g] / sqrt((energy[g] + a0) * (energy[ne + g] + a1));
+        gr[g] = gr[g] / sqrt((energy[2 * ne + g] + a2) * (energy[3 * ne + g] + a3));
     }
 }
 
@@ -1323,6 +1361,8 @@ struct owl_acoustic2d {
     std::vector<uintptr_t> gsig, gready;
 
     bool borrowed_stream = false;    // stream and ev0/ev1 belong to the static-scratch context
+    bool energy_precond = false;     // OWL's energy preconditioning of its gradient
+    DevBuf<double> d_energy;         // source/receiver energies of the vp and rho terms
 
     ~owl_acoustic2d() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1562,7 +1602,10 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
             const Located L = locate(2, sc.x, 0.f, sc.z, H.ox, 0.f, H.oz, cfg->dx, 1.f, cfg->dz);
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
-            if (k == 0) H.hnt = hnt;
+            if (k == 0) {
+                H.hnt = hnt;
+                S.hnt = std::max(hnt, 1);
+            }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1;
             const int nend = nbeg + int(stf.size()) - 1;
             b->src_t0 = std::min(b->src_t0, nbeg);
@@ -1882,14 +1925,17 @@ void run_reverse_owl(owl_acoustic2d* b) {
     b->d_gr.alloc(b->grad_elems);
     b->d_gv.zero(st);
     b->d_gr.zero(st);
+    if (b->energy_precond) {
+        b->d_energy.alloc(4 * b->grad_elems);
+        b->d_energy.zero(st);
+    }
     const unsigned nshot = unsigned(b->sd.size());
+    // OWL loops down to the first source's hnt of each shot; a batch whose
+    // shots disagree runs down to the smallest one and images each shot only
+    // while t >= its own hnt (the extra reverse steps of the other shots
+    // touch no gradient).
     int hnt = b->nt;
     for (auto& H : b->hs) hnt = std::min(hnt, std::max(H.hnt, 1));
-    // OWL loops down to the first source's hnt of each shot; a batch whose
-    // shots disagree runs each shot's imaging only while t >= its own hnt.
-    for (auto& H : b->hs)
-        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one batch must share hnt");
-    hnt = std::max(b->hs[0].hnt, 1);
     DevBuf<float>* A = b->fA;
     DevBuf<float>* B = b->fB;
     const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
@@ -1923,10 +1969,14 @@ void run_reverse_owl(owl_acoustic2d* b) {
             hipLaunchKernelGGL(k_image_owl<R>, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(),
                                B[0].get(), B[1].get(), B[2].get(), A[0].get(), A[1].get(), A[2].get(),
                                b->d_edge.get(), b->aF[0].get(), b->aF[1].get(), b->aF[2].get(),
-                               b->d_den.get(), b->d_gv.get(), b->d_gr.get(), t == b->nt ? 1 : 0, b->pml);
+                               b->d_den.get(), b->d_gv.get(), b->d_gr.get(), t == b->nt ? 1 : 0, b->pml,
+                               b->energy_precond ? b->d_energy.get() : nullptr, b->grad_elems, t);
         prof_end(b, ev);
         std::swap(A, B);
     }
+    if (b->energy_precond)
+        hipLaunchKernelGGL(k_energy_precond, dim3(nshot), dim3(256), 0, st, b->d_shots.get(), b->d_gv.get(),
+                           b->d_gr.get(), b->d_energy.get(), b->grad_elems);
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
@@ -3083,6 +3133,38 @@ owl_status owl_acoustic2d_gradient(owl_acoustic2d* b, const float* obs, int mode
         if (grad_vp) OWL_HIP(hipMemcpy(grad_vp, b->d_gfull_v.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
         if (grad_rho) OWL_HIP(hipMemcpy(grad_rho, b->d_gfull_r.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
         if (misfit) OWL_HIP(hipMemcpy(misfit, b->d_misfit.get(), b->sd.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (traces_out) download_traces(b, traces_out);
+    });
+}
+
+owl_status owl_acoustic2d_set_energy_precond(owl_acoustic2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->energy_precond = on != 0;
+    });
+}
+
+owl_status owl_acoustic2d_misfit(owl_acoustic2d* b, const float* obs, double* misfit, float* traces_out) {
+    return guarded([&] {
+        if (!b || !misfit) fail(OWL_ERR_ARG, "null argument");
+        hipStream_t st = b->stream;
+        if (obs) {
+            std::vector<float> h;
+            traces_to_dev(b, obs, h);
+            b->d_obs.upload(h, st);
+            b->have_obs = true;
+        }
+        if (!b->have_obs) fail(OWL_ERR_STATE, "no observed data: pass obs or call owl_acoustic2d_set_observed");
+        dispatch_R(b->R, [&](auto r) { run_forward<decltype(r)::value>(b, false); });
+        b->d_adj.alloc(b->trace_elems);
+        b->d_misfit.alloc(b->sd.size());
+        b->d_scratch.alloc(2 * size_t(std::max(1, b->nrec_total)));
+        hipLaunchKernelGGL(k_adjoint_source, dim3(unsigned(b->sd.size())), dim3(512), 0, st, b->d_shots.get(),
+                           b->d_traces.get(), b->d_obs.get(), b->d_rec_w.get(), b->d_adj.get(), b->d_misfit.get(),
+                           b->nt, b->cfg.trace_discard_threshold, b->d_scratch.get());
+        OWL_HIP(hipGetLastError());
+        OWL_HIP(hipMemcpyAsync(misfit, b->d_misfit.get(), b->sd.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        OWL_HIP(hipStreamSynchronize(st));
         if (traces_out) download_traces(b, traces_out);
     });
 }

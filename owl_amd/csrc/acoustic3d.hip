@@ -56,6 +56,7 @@ struct Shot3 {
     long long edge0;       // saved interior-face velocities
     long long g0;          // window gradient (nx*ny*nz, x fastest)
     int ftile0, fntx, fnty, fnzc;  // tiles of the fused step
+    int hnt;               // last reverse step OWL images for this shot (its first source's hnt)
 };
 
 // Wavefields of one sweep.  The p-sweep reads vx, vy, vz (stencils) and p
@@ -1395,8 +1396,9 @@ template <int R>
 __global__ __launch_bounds__(256) void k3_image_owl(const Shot3* __restrict__ shots, F3 cur, F3 prev,
                                                     const float* __restrict__ edge, F3 adj,
                                                     const float* __restrict__ den, double* gv, double* gr,
-                                                    int prev_zero, int pml) {
+                                                    int prev_zero, int pml, int t) {
     const Shot3 S = shots[blockIdx.y];
+    if (t < S.hnt) return;  // this shot's reverse loop has ended (whole block)
     const int nx = S.nx, ny = S.ny, nz = S.nz, lo = 1 - pml - R;
     const long long n = (long long)nx * ny * nz;
     const size_t px = S.px, pxy = S.pxy;
@@ -1794,7 +1796,10 @@ void plan3(owl_acoustic3d* b, const owl_acoustic_config* cfg, const float* vp, c
             const Located L = locate(3, sc.x, sc.y, sc.z, H.ox, H.oy, H.oz, cfg->dx, cfg->dy, cfg->dz);
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
-            if (si == 0) H.hnt = hnt;
+            if (si == 0) {
+                H.hnt = hnt;
+                S.hnt = std::max(hnt, 1);
+            }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
             std::vector<float> a(nt, 0.f);
             for (int t = std::max(nbeg, 1); t <= std::min(nend, nt); ++t) a[t - 1] = stf[t - nbeg] * sc.amp * cfg->dt;
@@ -2159,9 +2164,9 @@ void run_reverse3(owl_acoustic3d* b) {
     if (!b->have_boundary || !b->fwd_state_valid)
         fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
     if (!b->d_adj.get() || b->d_adj.size() < b->trace_elems) fail(OWL_ERR_STATE, "no adjoint source");
-    // OWL's loop ends at the first source's hnt (submodule_fwi_gradient.f90:81)
-    for (auto& H : b->hs)
-        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one gradient batch must share hnt");
+    // OWL's loop ends at the first source's hnt of the shot
+    // (submodule_fwi_gradient.f90:81); the batch runs down to the smallest one
+    // and images each shot only while t >= its own
     for (int k = 0; k < 4; ++k) {
         b->fB[k].alloc(b->field_elems);
         b->fB[k].zero(st);
@@ -2178,7 +2183,8 @@ void run_reverse3(owl_acoustic3d* b) {
     b->d_gv.zero(st);
     b->d_gr.zero(st);
     const unsigned nshot = unsigned(b->sd.size());
-    const int hnt = std::max(b->hs[0].hnt, 1);
+    int hnt = b->nt;
+    for (auto& H : b->hs) hnt = std::min(hnt, std::max(H.hnt, 1));
     const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
     DevBuf<float>* A = b->f;
     DevBuf<float>* Bf = b->fB;
@@ -2215,7 +2221,7 @@ void run_reverse3(owl_acoustic3d* b) {
             ev = b->prof.begin(st, 3, image_bytes);
             hipLaunchKernelGGL(k3_image_owl<R>, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(), cur, prv,
                                b->d_edge.get(), adj, b->d_den.get(), b->d_gv.get(), b->d_gr.get(),
-                               t == b->nt ? 1 : 0, b->pml);
+                               t == b->nt ? 1 : 0, b->pml, t);
             b->prof.end(st, ev);
         }
         std::swap(A, Bf);

@@ -197,8 +197,9 @@ owl_status owl_tti2d_param_gradient(int param, long long n, const float* vp, con
         if (param != OWL_ANISO_ISO && param != OWL_ANISO_THOMSEN && param != OWL_ANISO_AT)
             fail(OWL_ERR_ARG, "unknown parameterisation");
         for (long long q = 0; q < n; ++q) {
-            // OWL rescales the velocities to km/s before the chain rule
-            const double v = double(vp[q]) * 1.0e-3, s = double(vs[q]) * 1.0e-3, r = rho[q];
+            // OWL rescales the velocities to km/s and the density to g/cm^3
+            // before the chain rule (submodule_fwi_gradient.f90:476-480, 486-488)
+            const double v = double(vp[q]) * 1.0e-3, s = double(vs[q]) * 1.0e-3, r = double(rho[q]) * 1.0e-3;
             double g[6];
             for (int k = 0; k < 6; ++k) g[k] = grad_cij[k][q];
             if (param == OWL_ANISO_ISO) {

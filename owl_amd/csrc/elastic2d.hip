@@ -45,6 +45,7 @@ struct ShotE2 {
     long long trace0;      // two [nt][nr] blocks (vx, vz)
     long long strip0;      // boundary records: nt blocks
     long long g0;          // window gradient accumulators (nx*nz, x fastest)
+    int hnt;               // last reverse step OWL images for this shot (its first source's hnt)
 };
 
 // Wavefields and memory variables (named by derivative and field as in OWL).
@@ -396,8 +397,9 @@ __global__ void k2e_inject_adjoint(const ShotE2* __restrict__ shots, float* vx, 
 __global__ __launch_bounds__(256) void k2e_image(const ShotE2* __restrict__ shots, EV2 cur, EV2 prev, EV2 adj,
                                                  const float* __restrict__ c11, const float* __restrict__ c13,
                                                  const float* __restrict__ c55, double* gA, double* gB, double* gC,
-                                                 int prev_zero, int pml, int R) {
+                                                 int prev_zero, int pml, int R, int t) {
     const ShotE2 S = shots[blockIdx.y];
+    if (t < S.hnt) return;  // this shot's reverse loop has ended (whole block)
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const long long n = (long long)nx * nz;
     const size_t sx = 1, sz = S.px;
@@ -671,7 +673,10 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
             const Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
-            if (&sc == &H.src[0]) H.hnt = hnt;
+            if (&sc == &H.src[0]) {
+                H.hnt = hnt;
+                S.hnt = std::max(hnt, 1);
+            }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
             const float rhx = b->rho_full[model_index2(b, H, L.hx, L.gz)];
             const float rhz = b->rho_full[model_index2(b, H, L.gx, L.hz)];
@@ -966,8 +971,6 @@ void run_reverse2(owl_elastic2d* b) {
     if (!b->have_boundary || !b->fwd_state_valid)
         fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
     if (!b->d_adj.get() || b->d_adj.size() < b->trace_elems) fail(OWL_ERR_STATE, "no adjoint source");
-    for (auto& H : b->hs)
-        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one gradient batch must share hnt");
     for (int k = 5; k < 13; ++k) b->fld[k].zero(st);  // the reverse run's memory variables start at zero
     for (auto& f : b->fB) {
         f.alloc(b->field_elems);
@@ -983,7 +986,10 @@ void run_reverse2(owl_elastic2d* b) {
         g->zero(st);
     }
     const unsigned nshot = unsigned(b->sd.size());
-    const int hnt = std::max(b->hs[0].hnt, 1);
+    // each shot is imaged only while t >= its own hnt; the batch runs down to
+    // the smallest one
+    int hnt = b->nt;
+    for (auto& H : b->hs) hnt = std::min(hnt, std::max(H.hnt, 1));
     const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
     double bs, bv;
     sweep_bytes2(b, bs, bv);
@@ -1014,7 +1020,7 @@ void run_reverse2(owl_elastic2d* b) {
             ev = b->prof.begin(st, 3, image_bytes);
             hipLaunchKernelGGL(k2e_image, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(), cur, prev, adj,
                                b->med[0].get(), b->med[1].get(), b->med[5].get(), b->d_gA.get(), b->d_gB.get(),
-                               b->d_gC.get(), t == b->nt ? 1 : 0, b->pml, R);
+                               b->d_gC.get(), t == b->nt ? 1 : 0, b->pml, R, t);
             b->prof.end(st, ev);
         }
         if (t != b->nt) {

@@ -46,6 +46,7 @@ struct ShotE {
     long long strip0;      // boundary records: nt blocks of strip_len
     long long g0;          // window gradient accumulators (nx*ny*nz, x fastest)
     int nadj, adj0;        // adjoint-source cells (CSR)
+    int hnt;               // last reverse step OWL images for this shot (its first source's hnt)
 };
 
 // Wavefields and memory variables (named by derivative and field as in OWL).
@@ -689,8 +690,9 @@ __device__ __forceinline__ void compliances(float c11, float c12, float c44, flo
 __global__ __launch_bounds__(256) void ke_image(const ShotE* __restrict__ shots, EV cur, EV prev, EV adj,
                                                 const float* __restrict__ c11, const float* __restrict__ c12,
                                                 const float* __restrict__ c44, double* gA, double* gB, double* gC,
-                                                int prev_zero, int pml, int R) {
+                                                int prev_zero, int pml, int R, int t) {
     const ShotE S = shots[blockIdx.y];
+    if (t < S.hnt) return;  // this shot's reverse loop has ended (whole block)
     const int nx = S.nx, ny = S.ny, nz = S.nz, lo = 1 - pml - R;
     const long long n = (long long)nx * ny * nz;
     const size_t sx = 1, sy = S.px, sz = S.pxy;
@@ -1060,7 +1062,10 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
             const Located L = locate(3, sc.x, sc.y, sc.z, H.ox, H.oy, H.oz, cfg->dx, cfg->dy, cfg->dz);
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
-            if (&sc0 == &H.src[0]) H.hnt = hnt;
+            if (&sc0 == &H.src[0]) {
+                H.hnt = hnt;
+                S.hnt = std::max(hnt, 1);
+            }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
             const float rs = b->rho_full[model_index(b, H, L.gx, L.gy, L.gz)];
             const float sp = std::sin(sc.polar), cp = std::cos(sc.polar);
@@ -1409,8 +1414,6 @@ void run_reverse_e(owl_elastic3d* b) {
         fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
     if (!b->d_adj.get() || b->d_adj.size() < b->trace_elems) fail(OWL_ERR_STATE, "no adjoint source");
     // OWL's loop ends at the first source's hnt
-    for (auto& H : b->hs)
-        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one gradient batch must share hnt");
     for (int k = 9; k < 27; ++k) b->fld[k].zero(st);  // the reverse run's memory variables start at zero
     for (auto& f : b->fB) {
         f.alloc(b->field_elems);
@@ -1426,7 +1429,10 @@ void run_reverse_e(owl_elastic3d* b) {
         g->zero(st);
     }
     const unsigned nshot = unsigned(b->sd.size());
-    const int hnt = std::max(b->hs[0].hnt, 1);
+    // each shot is imaged only while t >= its own hnt; the batch runs down to
+    // the smallest one
+    int hnt = b->nt;
+    for (auto& H : b->hs) hnt = std::min(hnt, std::max(H.hnt, 1));
     const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
     double bs, bv;
     sweep_bytes(b, bs, bv);
@@ -1461,7 +1467,7 @@ void run_reverse_e(owl_elastic3d* b) {
             ev = b->prof.begin(st, 3, image_bytes);
             hipLaunchKernelGGL(ke_image, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(), cur, prev, adj,
                                b->med[0].get(), b->med[1].get(), b->med[8].get(), b->d_gA.get(), b->d_gB.get(),
-                               b->d_gC.get(), t == b->nt ? 1 : 0, b->pml, R);
+                               b->d_gC.get(), t == b->nt ? 1 : 0, b->pml, R, t);
             b->prof.end(st, ev);
         }
         if (t != b->nt) {

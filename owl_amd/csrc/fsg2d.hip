@@ -54,6 +54,7 @@ struct ShotF {
     long long trace0;      // two [nt][nr] blocks (vx, vz)
     long long strip0;      // boundary records: nt blocks
     long long g0;          // window gradient accumulators (nx*nz, x fastest)
+    int hnt;               // last reverse step OWL images for this shot (its first source's hnt)
 };
 
 // Wavefields (0..9) and CFS-PML memory variables (10..25), named as in OWL.
@@ -511,8 +512,9 @@ __device__ __forceinline__ float hsum(const float* a, size_t q, size_t sz) {
 }
 
 __global__ __launch_bounds__(256) void k_fsg_image(const ShotF* __restrict__ shots, FV cur, FV prev, FV adj, Comp C,
-                                                   Acc7 G, int prev_zero, int pml, int R) {
+                                                   Acc7 G, int prev_zero, int pml, int R, int t) {
     const ShotF S = shots[blockIdx.y];
+    if (t < S.hnt) return;  // this shot's reverse loop has ended (whole block)
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const long long n = (long long)nx * nz;
     const size_t sz = S.px;
@@ -842,7 +844,10 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
             const Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
-            if (&sc == &H.src[0]) H.hnt = hnt;
+            if (&sc == &H.src[0]) {
+                H.hnt = hnt;
+                S.hnt = std::max(hnt, 1);
+            }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
             auto taps = [&](int gx, int gz, const float* wx, const float* wz, int field, int row) {
                 for (int irz = -kNkw; irz <= kNkw; ++irz)
@@ -1253,8 +1258,6 @@ void run_reversef(owl_fsg2d* b) {
     if (!b->have_boundary || !b->fwd_state_valid)
         fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
     if (!b->d_adj.get() || b->d_adj.size() < b->trace_elems) fail(OWL_ERR_STATE, "no adjoint source");
-    for (auto& H : b->hs)
-        if (H.hnt != b->hs[0].hnt) fail(OWL_ERR_ARG, "shots of one gradient batch must share hnt");
     for (int k = kNW; k < kNF; ++k) b->fld[k].zero(st);  // the reverse run's memory variables start at zero
     for (auto& f : b->fB) {
         f.alloc(b->field_elems);
@@ -1274,7 +1277,10 @@ void run_reversef(owl_fsg2d* b) {
     Comp C;
     for (int k = 0; k < 6; ++k) C.s[k] = b->comp[k].get();
     const unsigned nshot = unsigned(b->sd.size());
-    const int hnt = std::max(b->hs[0].hnt, 1);
+    // each shot is imaged only while t >= its own hnt; the batch runs down to
+    // the smallest one
+    int hnt = b->nt;
+    for (auto& H : b->hs) hnt = std::min(hnt, std::max(H.hnt, 1));
     const int cc = b->cfg.cc_step_interval > 0 ? b->cfg.cc_step_interval : 1;
     double bs, bv;
     sweep_bytesf(b, bs, bv);
@@ -1309,7 +1315,7 @@ void run_reversef(owl_fsg2d* b) {
         if (t % cc == 0) {
             ev = b->prof.begin(st, 3, image_bytes);
             hipLaunchKernelGGL(k_fsg_image, dim3(ig, nshot), dim3(256), 0, st, b->d_shots.get(), cur, prev, adj, C, G,
-                               t == b->nt ? 1 : 0, b->pml, R);
+                               t == b->nt ? 1 : 0, b->pml, R, t);
             b->prof.end(st, ev);
         }
         if (t != b->nt) {

@@ -51,6 +51,16 @@ module owl_amd_c
         real(c_float) :: trace_discard_threshold = 1.0e-6
     end type
 
+    type, bind(c) :: owl_window_params_t
+        integer(c_int) :: nx = 1, ny = 1, nz = 1
+        real(c_float) :: dx = 1, dy = 1, dz = 1
+        real(c_float) :: ox = 0, oy = 0, oz = 0
+        real(c_float) :: xmin = 0, xmax = 0, ymin = 0, ymax = 0, zmin = 0, zmax = 0
+        integer(c_int) :: yn_adpx = 0, yn_adpy = 0, yn_adpz = 0
+        real(c_float) :: adp_extrax = 0, adp_extray = 0, adp_extraz = 0
+        real(c_float) :: offset_min = 0, offset_max = huge(1.0_c_float)
+    end type
+
     interface
 
         function owl_last_error() bind(c, name='owl_last_error') result(msg)
@@ -106,6 +116,125 @@ module owl_amd_c
             integer(c_int), value :: mode, ckpt, ckpt_param, store
             real(c_double), intent(out) :: misfit(*), grad_vp(*), grad_rho(*)
             real(c_float), intent(out) :: traces(*)
+            integer(c_int) :: st
+        end function
+
+        function owl_acoustic2d_set_energy_precond(b, on) &
+                bind(c, name='owl_acoustic2d_set_energy_precond') result(st)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: b
+            integer(c_int), value :: on
+            integer(c_int) :: st
+        end function
+
+        function owl_acoustic3d_create(cfg, vp, rho, nshot, shots, out) &
+                bind(c, name='owl_acoustic3d_create') result(st)
+            import :: c_int, c_float, c_ptr, owl_acoustic_config_t, owl_shot_t
+            type(owl_acoustic_config_t), intent(in) :: cfg
+            real(c_float), intent(in) :: vp(*), rho(*)
+            integer(c_int), value :: nshot
+            type(owl_shot_t), intent(in) :: shots(*)
+            type(c_ptr), intent(out) :: out
+            integer(c_int) :: st
+        end function
+
+        subroutine owl_acoustic3d_destroy(b) bind(c, name='owl_acoustic3d_destroy')
+            import :: c_ptr
+            type(c_ptr), value :: b
+        end subroutine
+
+        function owl_acoustic3d_nt(b) bind(c, name='owl_acoustic3d_nt') result(nt)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: b
+            integer(c_int) :: nt
+        end function
+
+        function owl_acoustic3d_forward(b, traces, save_boundary) &
+                bind(c, name='owl_acoustic3d_forward') result(st)
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: b
+            real(c_float), intent(out) :: traces(*)
+            integer(c_int), value :: save_boundary
+            integer(c_int) :: st
+        end function
+
+        function owl_acoustic3d_adjoint(b, adj, grad_vp, grad_rho) &
+                bind(c, name='owl_acoustic3d_adjoint') result(st)
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: b
+            real(c_float), intent(in) :: adj(*)
+            real(c_float), intent(out) :: grad_vp(*), grad_rho(*)
+            integer(c_int) :: st
+        end function
+
+        function owl_set_adaptive_range(p, ns, src, nr, rec, window, range) &
+                bind(c, name='owl_set_adaptive_range') result(st)
+            import :: c_int, c_float, owl_window_params_t, owl_source_t, owl_receiver_t
+            type(owl_window_params_t), intent(in) :: p
+            integer(c_int), value :: ns
+            type(owl_source_t), intent(in) :: src(*)
+            integer(c_int), value :: nr
+            type(owl_receiver_t), intent(inout) :: rec(*)
+            integer(c_int), intent(out) :: window(6)
+            real(c_float), intent(out) :: range(6)
+            integer(c_int) :: st
+        end function
+
+        function owl_fsg2d_create(cfg, cij, rho, aniso, nshot, shots, out) &
+                bind(c, name='owl_fsg2d_create') result(st)
+            import :: c_int, c_float, c_ptr, owl_acoustic_config_t, owl_shot_t
+            type(owl_acoustic_config_t), intent(in) :: cfg
+            type(c_ptr), intent(in) :: cij(6)
+            real(c_float), intent(in) :: rho(*)
+            integer(c_int), value :: aniso, nshot
+            type(owl_shot_t), intent(in) :: shots(*)
+            type(c_ptr), intent(out) :: out
+            integer(c_int) :: st
+        end function
+
+        subroutine owl_fsg2d_destroy(b) bind(c, name='owl_fsg2d_destroy')
+            import :: c_ptr
+            type(c_ptr), value :: b
+        end subroutine
+
+        function owl_fsg2d_nt(b) bind(c, name='owl_fsg2d_nt') result(nt)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: b
+            integer(c_int) :: nt
+        end function
+
+        function owl_fsg2d_forward(b, traces, save_boundary) bind(c, name='owl_fsg2d_forward') result(st)
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: b
+            real(c_float), intent(out) :: traces(*)
+            integer(c_int), value :: save_boundary
+            integer(c_int) :: st
+        end function
+
+        function owl_fsg2d_adjoint(b, adj, grads) bind(c, name='owl_fsg2d_adjoint') result(st)
+            import :: c_ptr, c_int, c_float
+            type(c_ptr), value :: b
+            real(c_float), intent(in) :: adj(*)
+            type(c_ptr), intent(in) :: grads(7)
+            integer(c_int) :: st
+        end function
+
+        function owl_tti2d_cij(param, n, vp, vs, rho, p1, p2, theta, cij) bind(c, name='owl_tti2d_cij') result(st)
+            import :: c_int, c_long_long, c_ptr
+            integer(c_int), value :: param
+            integer(c_long_long), value :: n
+            type(c_ptr), value :: vp, vs, rho, p1, p2, theta
+            type(c_ptr), intent(in) :: cij(6)
+            integer(c_int) :: st
+        end function
+
+        function owl_tti2d_param_gradient(param, n, vp, vs, rho, p1, p2, theta, grad_cij, grads) &
+                bind(c, name='owl_tti2d_param_gradient') result(st)
+            import :: c_int, c_long_long, c_ptr
+            integer(c_int), value :: param
+            integer(c_long_long), value :: n
+            type(c_ptr), value :: vp, vs, rho, p1, p2, theta
+            type(c_ptr), intent(in) :: grad_cij(6), grads(4)
             integer(c_int) :: st
         end function
 

@@ -16,7 +16,7 @@
 !
 program owl_fwd2d
 
-    use owl_acoustic_iso_2d
+    use acoustic_iso_2d
     use owl_amd_c
 
     implicit none
@@ -34,13 +34,16 @@ program owl_fwd2d
     read (u, '(a)') f_rho
     read (u, *) sx, sz, f0, amp
     read (u, *) nr
-    allocate (solver%srcr(1), solver%recr(nr))
-    solver%srcr(1)%x = sx
-    solver%srcr(1)%z = sz
-    solver%srcr(1)%f0 = f0
-    solver%srcr(1)%amp = amp
+    allocate (solver%gmtr%srcr(1), solver%gmtr%recr(nr))
+    solver%gmtr%id = 1
+    solver%gmtr%ns = 1
+    solver%gmtr%nr = nr
+    solver%gmtr%srcr(1)%x = sx
+    solver%gmtr%srcr(1)%z = sz
+    solver%gmtr%srcr(1)%f0 = f0
+    solver%gmtr%srcr(1)%amp = amp
     do ir = 1, nr
-        read (u, *) solver%recr(ir)%x, solver%recr(ir)%z
+        read (u, *) solver%gmtr%recr(ir)%x, solver%gmtr%recr(ir)%z
     end do
     read (u, '(a)') solver%dir_synthetic
     if (recon /= 0) then
@@ -58,7 +61,7 @@ program owl_fwd2d
 
     call solver%forward
     if (recon /= 0) call solver%adjoint
-    write (*, '(a,i0,a,i0)') 'owl_fwd2d: nt = ', size(solver%seis_p, 1), ', nr = ', size(solver%seis_p, 2)
+    write (*, '(a,i0,a,i0)') 'owl_fwd2d: nt = ', solver%seis_p%nt, ', nr = ', solver%seis_p%nr
 
 contains
 

@@ -56,7 +56,8 @@ def tti_param_gradient(param, grads, vp, vs, rho, p1=None, p2=None, theta=None):
     """OWL's chain rule from the c_ij gradients (dict with grad_c11 ..
     grad_c55, as FSG2DBatch.adjoint/gradient return them) to the
     parameter gradients: dict grad_vp, grad_vs (+ grad_epsilon and
-    grad_delta or grad_eta), velocities in km/s as OWL rescales them."""
+    grad_delta or grad_eta), with velocities in km/s and density in
+    g/cm^3 as OWL rescales them."""
     code, names = PARAMS[param]
     shape = np.shape(vp)
     args = [_f32(a, shape) for a in (vp, vs, rho)]

@@ -1,0 +1,34 @@
+"""Misfits and adjoint sources of OWL's FWI on the GPU over the C ABI
+(owl_adjoint_source, include/owl_amd.h): the L2 waveform, zero-lag
+correlation, envelope and instantaneous-phase misfits of
+src/inverse/module_inversion_adjoint_source.f90, batched over the traces of
+a shot, with OWL's trace-discard rule."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6):
+    """(adjoint source [ntr, nt] float32, per-trace misfit [ntr] float64) of
+    one shot component: kind is "waveform", "corr", "envelope" or "phase";
+    obs, syn [ntr, nt]; weight [ntr] receiver weights (None: all 1)."""
+    o = np.ascontiguousarray(obs, np.float32)
+    s = np.ascontiguousarray(syn, np.float32)
+    if o.shape != s.shape or o.ndim != 2:
+        raise ValueError("obs and syn must be [ntr, nt] of one shape")
+    ntr, nt = o.shape
+    w = None if weight is None else np.ascontiguousarray(weight, np.float32)
+    if w is not None and w.shape != (ntr,):
+        raise ValueError("weight must hold one value per trace")
+    adj = np.zeros_like(o)
+    mis = np.zeros(ntr, np.float64)
+    check(lib().owl_adjoint_source(_lib.MISFITS[kind], ntr, nt, _ptr(o), _ptr(s), None if w is None else _ptr(w),
+                                   float(threshold), _ptr(adj), _ptr(mis)))
+    return adj, mis

@@ -127,6 +127,32 @@ def test_owl_gradient_per_shot(oracle32, gpu):
     b.close()
 
 
+def test_owl_gradient_energy_precond(oracle32, gpu):
+    """OWL's energy_precond on the GPU against the float32 oracle, per shot
+    window (the energies accumulate in FP64 on the GPU, in FP32 in OWL)."""
+    nz, nx, dx, vp, rho, vp0, rho0, geoms = _grad_case(oracle32, seed=7)
+    dt, tmax, pml = 0.001, 0.3, 12
+    w = Window(1, nx, 1, nz, 0.0, 0.0)
+    print("case: OWL adjoint with energy preconditioning", flush=True)
+    b = Acoustic2DBatch(nx, nz, dx, dx, vp0, rho0, [(g, None) for g in geoms], dt, tmax, pml=pml)
+    b.set_energy_precond(True)
+    syn = b.forward(save_boundary=True)
+    adjs = [(0.5 * s).astype(np.float32) for s in syn]
+    grads = b.adjoint(adjs)
+    b.close()
+    oracle32.set_energy_precond2(True)
+    try:
+        for geom, adj, (gv, gr) in zip(geoms, adjs, grads):
+            grid = oracle_grid(w, dx, dx, dt, b.nt, pml, 4)
+            rv, rr, _ = oracle32.acoustic2d_gradient_owl(grid, vp0, rho0, oracle_sources(geom),
+                                                         oracle_receivers(geom, w, dx, dx), adj)
+            assert np.abs(rv).max() > 0
+            assert rel_l2(gv, rv) <= 1e-4, rel_l2(gv, rv)
+            assert rel_l2(gr, rr) <= 1e-4, rel_l2(gr, rr)
+    finally:
+        oracle32.set_energy_precond2(False)
+
+
 def test_owl_gradient_batch_sum(oracle32, gpu):
     nz, nx, dx, vp, rho, vp0, rho0, geoms = _grad_case(oracle32, seed=4)
     dt, tmax, pml = 0.001, 0.3, 12

@@ -1,0 +1,48 @@
+"""GPU misfits and adjoint sources (owl_adjoint_source through the C ABI)
+against the float64 oracle restatement of OWL's formulas: a shot of 48
+traces with two zero-weight and one near-silent receiver (the discard rule),
+for the waveform, correlation, envelope and phase misfits.  Tolerances
+(rel-L2 of the adjoint source, relative misfit): waveform 1e-6, correlation
+1e-5, envelope 1e-4 (single-precision FFTs), phase 2e-3 (single-precision
+phases next to the +-pi cut)."""
+import numpy as np
+import pytest
+
+import misfit_oracle as mo
+from helpers import rel_l2
+from owl_amd.misfit import adjoint_source
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"waveform": 1e-6, "corr": 1e-5, "envelope": 1e-4, "phase": 2e-3}
+
+
+def _shot(ntr=48, nt=700, seed=4):
+    rng = np.random.default_rng(seed)
+    t = np.arange(nt) * 1e-3
+    obs = np.zeros((ntr, nt))
+    syn = np.zeros((ntr, nt))
+    for r in range(ntr):
+        for k, (a, f) in enumerate(((1.0, 12.0), (0.5, 20.0))):
+            t0 = 0.15 + 0.004 * r + 0.2 * k
+            obs[r] += a * np.exp(-0.5 * ((t - t0) / 0.02) ** 2) * np.cos(2 * np.pi * f * (t - t0))
+            syn[r] += 0.9 * a * np.exp(-0.5 * ((t - t0 - 0.006) / 0.02) ** 2) * np.cos(2 * np.pi * f * (t - t0 - 0.006))
+    obs += 0.01 * rng.standard_normal(obs.shape)
+    syn += 0.01 * rng.standard_normal(syn.shape)
+    syn[7] *= 1e-9
+    w = np.ones(ntr, np.float32)
+    w[[3, 30]] = 0.0
+    return obs.astype(np.float32), syn.astype(np.float32), w
+
+
+@pytest.mark.parametrize("kind", ["waveform", "corr", "envelope", "phase"])
+def test_adjoint_sources_match_the_oracle(gpu, kind):
+    obs, syn, w = _shot()
+    print(f"case: adjoint source {kind}, 48 traces x 700 samples", flush=True)
+    adj, mis = adjoint_source(kind, obs, syn, w, 1e-6)
+    radj, rmis = mo.adjoint_source(kind, obs.astype(np.float64), syn.astype(np.float64), w, 1e-6)
+    for r in (3, 7, 30):
+        assert not adj[r].any() and mis[r] == 0
+    assert np.abs(radj).max() > 0
+    assert rel_l2(adj, radj) <= TOL[kind], rel_l2(adj, radj)
+    assert abs(mis.sum() - rmis.sum()) <= TOL[kind] * rmis.sum()

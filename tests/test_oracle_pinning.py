@@ -225,6 +225,39 @@ def test_owl_gradient_mirror_symmetry(oracle32):
     assert rel_l2(gr[:, ::-1], gr) < 1e-4
 
 
+def test_owl_gradient_energy_precond_invariances(oracle64):
+    """OWL's energy preconditioning divides each gradient by sqrt(source
+    energy x receiver energy) of its own imaging terms: the result is
+    unchanged when the adjoint source or the source amplitude is scaled (the
+    gradient scales like the square root of each energy, the 1e-3 floor
+    with it), keeps the sign of the unpreconditioned gradient everywhere,
+    and differs from it."""
+    nz, nx = 40, 50
+    z = np.arange(nz)[:, None] * 10.0
+    vp = np.broadcast_to(2000.0 + 2.0 * z, (nz, nx)).astype(np.float64)
+    rho = np.full((nz, nx), 1500.0, np.float32)
+    g = dict(nx=nx, nz=nz, dx=10.0, dz=10.0, dt=0.001, nt=300, pml=12, R=4)
+    rec = [dict(x=10.0 * k, z=30.0) for k in range(nx)]
+    src = [dict(x=230.0, z=120.0, f0=15.0)]
+    obs = oracle64.acoustic2d_forward(g, (vp * 1.03).astype(np.float64), rho, src, rec)
+    syn = oracle64.acoustic2d_forward(g, vp, rho, src, rec)
+    adj = (syn - obs).astype(np.float64)
+    off_v, off_r, _ = oracle64.acoustic2d_gradient_owl(g, vp, rho, src, rec, adj)
+    oracle64.set_energy_precond2(True)
+    try:
+        gv, gr, _ = oracle64.acoustic2d_gradient_owl(g, vp, rho, src, rec, adj)
+        gv7, gr7, _ = oracle64.acoustic2d_gradient_owl(g, vp, rho, src, rec, (7.0 * adj).astype(np.float64))
+        src3 = [dict(src[0], amp=3.0)]
+        gv3, gr3, _ = oracle64.acoustic2d_gradient_owl(g, vp, rho, src3, rec, adj)
+    finally:
+        oracle64.set_energy_precond2(False)
+    for a, b in ((gv7, gv), (gr7, gr), (gv3, gv), (gr3, gr)):
+        assert rel_l2(a, b) < 1e-10
+    for a, b in ((gv, off_v), (gr, off_r)):
+        assert np.all(np.sign(a) == np.sign(b))
+        assert rel_l2(a / np.abs(a).max(), b / np.abs(b).max()) > 0.05
+
+
 def test_owl_gradient_dot_product_consistency(oracle32):
     """OWL test 6 (test/test_jacobian_adjoint.py:55-170) with the reference's
     own configuration: 101 x 201 grid, vp 2500, rho 1, f0 15 Hz, dt 1 ms,

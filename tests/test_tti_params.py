@@ -124,10 +124,11 @@ def test_tilt_rotates_the_phase_velocity_surface():
 
 def _chain_fd(param, p1, p2):
     """sum_ij dC_ij/dp * g_ij by central differences of the float64
-    restatement, velocities in km/s as OWL rescales them."""
+    restatement, velocities in km/s and density in g/cm^3 as OWL rescales
+    them."""
     g = {("grad_" + k): RNG.standard_normal(N).astype(np.float32) for k in CIJ}
     G = np.array([g["grad_" + k] for k in CIJ], np.float64)
-    base = [VP * 1e-3, VS * 1e-3, RHO, p1, p2]
+    base = [VP * 1e-3, VS * 1e-3, RHO * 1e-3, p1, p2]
     base = [np.asarray(a, np.float64) for a in base]
     names = ["vp", "vs"] + (["epsilon", "delta"] if param == "thomsen" else ["epsilon", "eta"])
     idx = [0, 1, 3, 4]
@@ -153,7 +154,7 @@ def test_chain_rule_against_finite_differences():
                                        err_msg=f"{param} {k}")
     g = {("grad_" + k): RNG.standard_normal(N).astype(np.float32) for k in CIJ}
     iso = tti_param_gradient("iso", g, VP, VS, RHO)
-    v, s, r = VP * 1e-3, VS * 1e-3, RHO
+    v, s, r = VP * 1e-3, VS * 1e-3, RHO * 1e-3
     np.testing.assert_allclose(iso["grad_vp"], 2 * r * v * (g["grad_c11"] + g["grad_c13"] + g["grad_c33"]), rtol=1e-5)
     np.testing.assert_allclose(iso["grad_vs"], 2 * r * s * (g["grad_c55"] - 2 * g["grad_c13"]), rtol=1e-5)
 
