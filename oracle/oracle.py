@@ -87,6 +87,12 @@ class FSource2D(C.Structure):
 FSG_CIJ = ("c11", "c13", "c15", "c33", "c35", "c55")
 
 
+class FSurf2D(C.Structure):
+    _fields_ = [("ntopo", C.c_int), ("tx", C.c_void_p), ("tz", C.c_void_p), ("interp", C.c_int),
+                ("refine", C.c_float), ("dz_max", C.c_float), ("src_from_surface", C.c_int),
+                ("rec_from_surface", C.c_int)]
+
+
 class Point(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float),
                 ("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int),
@@ -619,6 +625,113 @@ class Oracle:
         fn(c.ctypes.data_as(C.c_void_p), float(rho), 1 if direction == "x" else 0, float(ratio), float(k1),
            float(k3), out.ctypes.data_as(C.c_void_p))
         return out
+
+    # ---- FSG with OWL's free surface on the refined, topography-following mesh
+    def _fsurf(self, g, surface):
+        """surface: dict with optional 'topo' ((x, elevation) arrays), 'interp'
+        ('linear' | 'cubic'), 'refine' (free_surface_dz_refine, 4), 'dz_max'
+        (the caller's dz_max, 1.5 dz), 'src_from_surface', 'rec_from_surface'."""
+        keep = []
+        topo = surface.get("topo")
+        if topo is not None:
+            tx = np.ascontiguousarray(topo[0], np.float32)
+            tz = np.ascontiguousarray(topo[1], np.float32)
+            keep += [tx, tz]
+            n, px, pz = len(tx), tx.ctypes.data, tz.ctypes.data
+        else:
+            n, px, pz = 0, None, None
+        fs = FSurf2D(n, px, pz, {"linear": 0, "cubic": 1}[surface.get("interp", "cubic")],
+                     float(surface.get("refine", 4.0)), float(surface.get("dz_max", 1.5 * g["dz"])),
+                     int(bool(surface.get("src_from_surface", False))),
+                     int(bool(surface.get("rec_from_surface", False))))
+        return fs, keep
+
+    def fsg2d_mesh(self, g, surface, c55, rho, f0min):
+        """OWL's refined free-surface mesh of one window: dict of the row
+        tables (j = -pml .. nz+pml+1), column tables (i = -pml .. nx+pml+1)
+        and scalars."""
+        G = self._fgrid2(g)
+        fs, keep = self._fsurf(g, surface)
+        c55 = self._model(c55, (G.nz, G.nx)).astype(np.float32)
+        rho = self._model(rho, (G.nz, G.nx)).astype(np.float32)
+        fn = self._fn("fsg2d_mesh")
+        fn.argtypes = [C.POINTER(FGrid2D), C.POINTER(FSurf2D), C.c_void_p, C.c_void_p, C.c_float, C.c_int,
+                       C.c_void_p, C.c_void_p] + [C.c_void_p] * 9
+        nz = C.c_int(0)
+        cap = 4096
+        while True:
+            rows = [np.zeros(cap, np.float32) for _ in range(5)]
+            cols = [np.zeros(G.nx + 2 * G.pml + 2, np.float32) for _ in range(4)]
+            scal = np.zeros(4, np.float32)
+            rc = fn(C.byref(G), C.byref(fs), c55.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p),
+                    float(f0min), cap, C.byref(nz), scal.ctypes.data_as(C.c_void_p),
+                    *[a.ctypes.data_as(C.c_void_p) for a in rows + cols])
+            if rc == -3:
+                cap = nz.value + 2 * G.pml + 2
+                continue
+            if rc != 0:
+                raise RuntimeError(f"oracle fsg2d_mesh failed ({rc})")
+            break
+        m = nz.value + 2 * G.pml + 2
+        out = dict(zip(("eta_i", "eta_h", "sc_i", "sc_h", "edz_i"), (r[:m].copy() for r in rows)))
+        out.update(zip(("topo_i", "slope_i", "topo_h", "slope_h"), cols))
+        out.update(nz=nz.value, dz=float(scal[0]), eta_max=float(scal[1]), depth_max=float(scal[2]),
+                   topo_max=float(scal[3]))
+        return out
+
+    def fsg2d_forward_fs(self, g, surface, cij, rho, sources, receivers):
+        """Velocity traces (tx, tz) of elastic_tti_2d compute_forward with
+        OWL's free surface (see _fsurf for `surface`)."""
+        G = self._fgrid2(g)
+        shape = (G.nz, G.nx)
+        ms, ptrs = self._cij_ptrs(cij, shape)
+        rho = self._model(rho, shape)
+        fs, keep2 = self._fsurf(g, surface)
+        arr, keep, f0max, _ = self._fsources2(g, sources)
+        f0 = np.array([s["f0"] for s in sources], np.float32)
+        rec = self._ereceivers2(receivers)
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("fsg2d_forward_fs")
+        fn.argtypes = [C.POINTER(FGrid2D), C.POINTER(FSurf2D), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                       C.c_void_p, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), C.byref(fs), C.cast(ptrs, C.c_void_p), rho.ctypes.data_as(C.c_void_p), len(sources),
+                C.cast(arr, C.c_void_p), f0.ctypes.data_as(C.c_void_p), f0max, len(receivers),
+                C.cast(rec, C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle fsg2d_forward_fs failed ({rc})")
+        return tr[0], tr[1]
+
+    def fsg2d_gradient_owl_fs(self, g, surface, cij, rho, sources, receivers, adj, restore_all=False):
+        """OWL's FSG gradient with the free surface, mapped back to the
+        regular window: dict of grad_c11 .. grad_c55, grad_rho as (nz, nx)
+        float32, plus the forward traces."""
+        G = self._fgrid2(g)
+        shape = (G.nz, G.nx)
+        ms, ptrs = self._cij_ptrs(cij, shape)
+        rho = self._model(rho, shape)
+        fs, keep2 = self._fsurf(g, surface)
+        arr, keep, f0max, hnt = self._fsources2(g, sources)
+        f0 = np.array([s["f0"] for s in sources], np.float32)
+        rec = self._ereceivers2(receivers)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (2, len(receivers), G.nt)
+        n = G.nx * G.nz
+        gr = [np.zeros(n, np.float32) for _ in range(7)]
+        gptr = (C.c_void_p * 7)(*[a.ctypes.data for a in gr])
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("fsg2d_gradient_owl_fs")
+        fn.argtypes = [C.POINTER(FGrid2D), C.POINTER(FSurf2D), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                       C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                       C.c_void_p]
+        rc = fn(C.byref(G), C.byref(fs), C.cast(ptrs, C.c_void_p), rho.ctypes.data_as(C.c_void_p), len(sources),
+                C.cast(arr, C.c_void_p), f0.ctypes.data_as(C.c_void_p), f0max, int(g.get("hnt", hnt)),
+                len(receivers), C.cast(rec, C.c_void_p), adj.ctypes.data_as(C.c_void_p), int(bool(restore_all)),
+                C.cast(gptr, C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle fsg2d_gradient_owl_fs failed ({rc})")
+        names = ["grad_" + k for k in FSG_CIJ] + ["grad_rho"]
+        out = {k: a.reshape(G.nx, G.nz).T.copy() for k, a in zip(names, gr)}
+        return out, (tr[0], tr[1])
 
     def fsg2d_gradient_owl(self, g, cij, rho, sources, receivers, adj, restore_all=False):
         """OWL's FSG gradient: dict of grad_c11 .. grad_c55 and grad_rho as

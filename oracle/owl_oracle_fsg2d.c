@@ -61,6 +61,7 @@ int ORA_SYM(fdcoefs)(int R, float *out);
 float ORA_SYM(alphamax)(float f0max);
 void ORA_SYM(locate_point)(int dims, float x, float y, float z, float ox, float oy, float oz, float dx, float dy,
                            float dz, ora_point *pt);
+void ORA_SYM(kaiser_sinc)(float x, float *h);
 
 typedef struct {
     int nx, nz;
@@ -212,9 +213,10 @@ typedef struct {
     real *sxx_ii, *szz_ii, *sxz_ii, *sxx_hh, *szz_hh, *sxz_hh, *vx_ih, *vz_ih, *vx_hi, *vz_hi;
     real *m_dxvx_hh, *m_dxvz_hh, *m_dxvx_ii, *m_dxvz_ii, *m_dxxx_hi, *m_dxxz_hi, *m_dxxx_ih, *m_dxxz_ih;
     real *m_dzvx_hh, *m_dzvz_hh, *m_dzvx_ii, *m_dzvz_ii, *m_dzxz_hi, *m_dzzz_hi, *m_dzxz_ih, *m_dzzz_ih;
+    real *m_dzxx_hi, *m_dzxx_ih; /* free-surface update only (memory_pdzxx_hxiz, _ixhz) */
 } ffld;
 
-#define NFF 26
+#define NFF 28
 static real *fz(size_t n) { return (real *)calloc(n, sizeof(real)); }
 static void ffld_alloc(ffld *F, const flay *L) {
     const size_t n = (size_t)L->NX * L->NZ;
@@ -244,8 +246,32 @@ static void damp_coef2(float pmlvp, float xd, float zd, float dx, float dz, int 
     *bz = (2.0f - adt * (alz + dampz / kz)) / (2.0f + adt * (alz + dampz / kz));
 }
 
-static void fmed_init(fmed *M, const ora_fgrid2d *g, const float *const *cij, const float *rho, float f0max) {
-    const int R = g->R, pml = g->pml, nx = g->nx, nz = g->nz;
+/* OWL's depth-refined free-surface mesh of one shot window
+ * (elastic_tti_2d/module_vars.f90:360-519).  Row tables over j = -pml ..
+ * nz+pml+1 (index j + pml), column tables over i = -pml .. nx+pml+1 (index
+ * i + pml); eta_i and eta_h are clipped to [-pml dz, eta_max] as OWL clips
+ * them before the time loop. */
+typedef struct fsmesh {
+    int nz;               /* refined rows */
+    int nx, nz0, pml, nci, nrj;
+    float dx, dz0, dz;    /* dz = eta_max / (nz0 - 1), the z spacing the update divides by */
+    float eta_max, depth_max, topo_max;
+    float *topo_i, *slope_i, *topo_h, *slope_h;
+    float *eta_i, *eta_h, *edz_i, *edz_h, *sc_i, *sc_h;
+    float *zz_i;          /* physical depth of the integer rows, [i + pml][j + pml], j fastest */
+} fsmesh;
+static void fsmesh_medium(const fsmesh *S, const ora_fgrid2d *g, const float *v, real *out, const flay *L);
+static float fsmesh_edz(const fsmesh *S, int j);
+
+/* S: the refined free-surface mesh (NULL for the regular mesh).  With S the
+ * medium is OWL's map_regular_to_irregular of the window and the z damping
+ * uses the local row spacing eta_dz_i(j) (module_cfspml.f90:395-480). */
+static void fmed_init(fmed *M, const ora_fgrid2d *g, const float *const *cij, const float *rho, float f0max,
+                      const fsmesh *S);
+static void fmed_init(fmed *M, const ora_fgrid2d *g, const float *const *cij, const float *rho, float f0max,
+                      const fsmesh *S) {
+    const int R = g->R, pml = g->pml, nx = g->nx, nz0 = g->nz;
+    const int nz = S ? S->nz : nz0;
     flay L = {1 - pml - R, nx + 2 * (pml + R), nz + 2 * (pml + R), nx, nz, pml, R};
     M->L = L;
     ORA_SYM(fdcoefs)(R, M->c);
@@ -254,19 +280,25 @@ static void fmed_init(fmed *M, const ora_fgrid2d *g, const float *const *cij, co
     const size_t n = (size_t)L.NX * L.NZ;
     real **cs[7] = {&M->c11, &M->c13, &M->c15, &M->c33, &M->c35, &M->c55, &M->rho};
     for (int k = 0; k < 7; ++k) *cs[k] = fz(n);
-    for (int j = L.lo; j < L.lo + L.NZ; ++j)
-        for (int i = L.lo; i < L.lo + L.NX; ++i) {
-            const int ic = i < 1 ? 1 : (i > nx ? nx : i), jc = j < 1 ? 1 : (j > nz ? nz : j);
-            const size_t m = (size_t)(jc - 1) + (size_t)(ic - 1) * nz, q = IF(L, i, j);
-            for (int k = 0; k < 6; ++k) (*cs[k])[q] = cij[k][m];
-            M->rho[q] = rho[m];
-        }
-    /* pmlvp (module_vars.f90:345-349): c11/rho and c33/rho on the four edge lines */
+    if (S) {
+        for (int k = 0; k < 6; ++k) fsmesh_medium(S, g, cij[k], *cs[k], &L);
+        fsmesh_medium(S, g, rho, M->rho, &L);
+    } else {
+        for (int j = L.lo; j < L.lo + L.NZ; ++j)
+            for (int i = L.lo; i < L.lo + L.NX; ++i) {
+                const int ic = i < 1 ? 1 : (i > nx ? nx : i), jc = j < 1 ? 1 : (j > nz ? nz : j);
+                const size_t m = (size_t)(jc - 1) + (size_t)(ic - 1) * nz, q = IF(L, i, j);
+                for (int k = 0; k < 6; ++k) (*cs[k])[q] = cij[k][m];
+                M->rho[q] = rho[m];
+            }
+    }
+    /* pmlvp (module_vars.f90:345-349): c11/rho and c33/rho on the four edge
+     * lines of the regular window, before any refinement */
     float pm = 0.0f;
     for (int i = 1; i <= nx; ++i)
-        for (int j = 1; j <= nz; ++j) {
-            if (!(i == 1 || i == nx || j == 1 || j == nz)) continue;
-            const size_t m = (size_t)(j - 1) + (size_t)(i - 1) * nz;
+        for (int j = 1; j <= nz0; ++j) {
+            if (!(i == 1 || i == nx || j == 1 || j == nz0)) continue;
+            const size_t m = (size_t)(j - 1) + (size_t)(i - 1) * nz0;
             const float a = cij[0][m] / rho[m], c = cij[3][m] / rho[m];
             if (a > pm) pm = a;
             if (c > pm) pm = c;
@@ -325,10 +357,11 @@ static void fmed_init(fmed *M, const ora_fgrid2d *g, const float *const *cij, co
             if (j <= 1) rz = top[i - m1];
             else if (j >= nz) rz = bottom[i - m1];
             const size_t q = IF(L, i, j);
-            damp_coef2(pmlvp, xi, zi, g->dx, g->dz, pml, adt, am, rx, rz, &M->xii.a[q], &M->xii.b[q], &M->zii.a[q], &M->zii.b[q]);
-            damp_coef2(pmlvp, xi, zh, g->dx, g->dz, pml, adt, am, rx, rz, &M->xih.a[q], &M->xih.b[q], &M->zih.a[q], &M->zih.b[q]);
-            damp_coef2(pmlvp, xh, zi, g->dx, g->dz, pml, adt, am, rx, rz, &M->xhi.a[q], &M->xhi.b[q], &M->zhi.a[q], &M->zhi.b[q]);
-            damp_coef2(pmlvp, xh, zh, g->dx, g->dz, pml, adt, am, rx, rz, &M->xhh.a[q], &M->xhh.b[q], &M->zhh.a[q], &M->zhh.b[q]);
+            const float dzj = S ? fsmesh_edz(S, j) : g->dz;
+            damp_coef2(pmlvp, xi, zi, g->dx, dzj, pml, adt, am, rx, rz, &M->xii.a[q], &M->xii.b[q], &M->zii.a[q], &M->zii.b[q]);
+            damp_coef2(pmlvp, xi, zh, g->dx, dzj, pml, adt, am, rx, rz, &M->xih.a[q], &M->xih.b[q], &M->zih.a[q], &M->zih.b[q]);
+            damp_coef2(pmlvp, xh, zi, g->dx, dzj, pml, adt, am, rx, rz, &M->xhi.a[q], &M->xhi.b[q], &M->zhi.a[q], &M->zhi.b[q]);
+            damp_coef2(pmlvp, xh, zh, g->dx, dzj, pml, adt, am, rx, rz, &M->xhh.a[q], &M->xhh.b[q], &M->zhh.a[q], &M->zhh.b[q]);
         }
     free(left); free(right); free(top); free(bottom);
 }
@@ -590,7 +623,7 @@ static int fforward(const ora_fgrid2d *g, const fmed *M, const ora_fsource2d *sr
 int ORA_SYM(fsg2d_forward)(const ora_fgrid2d *g, const float *const *cij, const float *rho, int ns,
                            const ora_fsource2d *src, float f0max, int nr, const ora_freceiver2d *rec, real *traces) {
     fmed M;
-    fmed_init(&M, g, cij, rho, f0max);
+    fmed_init(&M, g, cij, rho, f0max, NULL);
     const int rc = fforward(g, &M, src, ns, nr, rec, traces, NULL, NULL);
     fmed_free(&M);
     return rc;
@@ -601,7 +634,7 @@ int ORA_SYM(fsg2d_forward)(const ora_fgrid2d *g, const float *const *cij, const 
 int ORA_SYM(fsg2d_ratios)(const ora_fgrid2d *g, const float *const *cij, const float *rho, float *left, float *right,
                           float *top, float *bottom) {
     fmed M;
-    fmed_init(&M, g, cij, rho, 10.0f);
+    fmed_init(&M, g, cij, rho, 10.0f, NULL);
     const flay L = M.L;
     for (int j = -g->pml; j <= g->nz + g->pml + 1; ++j) {
         const size_t ql = IF(L, 1, j), qr = IF(L, g->nx, j);
@@ -634,7 +667,7 @@ int ORA_SYM(fsg2d_gradient_owl)(const ora_fgrid2d *g, const float *const *cij, c
                                 const ora_fsource2d *src, float f0max, int hnt, int nr, const ora_freceiver2d *rec,
                                 const real *adj, int restore_all, real *const *grad, real *traces_out) {
     fmed M;
-    fmed_init(&M, g, cij, rho, f0max);
+    fmed_init(&M, g, cij, rho, f0max, NULL);
     const flay L = M.L;
     const size_t n = (size_t)L.NX * L.NZ, sl = fstrip_len(&L);
     const int nt = g->nt;
@@ -746,5 +779,810 @@ int ORA_SYM(fsg2d_gradient_owl)(const ora_fgrid2d *g, const float *const *cij, c
     free(strips); free(fin); free(sp); free(rp);
     ffld_free(&F); ffld_free(&A); ffld_free(&P);
     fmed_free(&M);
+    return 0;
+}
+
+/* ======================================================================
+ * Free surface on OWL's depth-refined, topography-following mesh
+ *   module_vars.f90:360-519 (mesh), 169-249 (map_regular_to_irregular,
+ *   map_irregular_to_regular), 639-680 (source/receiver depths)
+ *   module_wavefield.f90:257-587 update_wavefield_free_surface, :593-736
+ *   add_source (rows >= 2)
+ *   module_cfspml.f90:395-480 damping with the local row spacing
+ *   submodule_forward_modeling.f90:66-75 velocity mirror before recording
+ *   submodule_fwi_gradient.f90:163, 300, 350-380, 456-470
+ *   src/lib/module_source_receiver.f90:556-760 nearest rows on the mesh
+ *
+ * FLIT helpers restated here (FLIT is not part of the OWL tree; these are
+ * assumptions, parity unpinned beyond the physical checks of the tests):
+ *   regspace(a, d, b): nint((b - a)/d) + 1 values a + k d;
+ *   deriv(x, 'center'): (x(i+1) - x(i-1))/2 inside, one-sided differences
+ *     at the two ends;
+ *   rescale(x, [r1, r2]): (x - min)/(max - min)*(r2 - r1) + r1;
+ *   ginterp 'linear': piecewise linear, y0 + (y1 - y0)*t on the segment
+ *     x0 <= v < x1, end segments extended;
+ *   ginterp 'cubic': natural cubic spline in double, end polynomials
+ *     extended;
+ *   pad_array: edge replication; float_small = FLT_EPSILON.
+ * ====================================================================== */
+
+typedef struct {
+    int ntopo;            /* topography control points; 0: flat at elevation 0 */
+    const float *tx, *tz; /* x and elevation (positive up) */
+    int interp;           /* topo_interp: 0 'linear', 1 'cubic' */
+    float refine;         /* free_surface_dz_refine */
+    float dz_max;         /* the caller's dz_max (module_model.f90:176-178) */
+    int src_from_surface, rec_from_surface;
+} ora_fsurf2d;
+
+static float *flit_regspace(float a, float d, float b, int *n) {
+    const int m = (int)lroundf((b - a) / d) + 1;
+    float *x = (float *)malloc(sizeof(float) * (size_t)(m > 0 ? m : 1));
+    for (int k = 0; k < m; ++k) x[k] = a + (float)k * d;
+    *n = m;
+    return x;
+}
+
+static void flit_deriv(const float *x, int n, float *d) {
+    if (n < 2) {
+        if (n == 1) d[0] = 0.0f;
+        return;
+    }
+    d[0] = x[1] - x[0];
+    d[n - 1] = x[n - 1] - x[n - 2];
+    for (int i = 1; i < n - 1; ++i) d[i] = 0.5f * (x[i + 1] - x[i - 1]);
+}
+
+static void flit_rescale(const float *x, int n, float r1, float r2, float *y) {
+    float lo = x[0], hi = x[0];
+    for (int i = 1; i < n; ++i) {
+        if (x[i] < lo) lo = x[i];
+        if (x[i] > hi) hi = x[i];
+    }
+    for (int i = 0; i < n; ++i) y[i] = (x[i] - lo) / (hi - lo) * (r2 - r1) + r1;
+}
+
+/* segment k with x[k] <= v < x[k+1] of an increasing x, clamped to the ends */
+static int seg_of(const float *x, int n, float v) {
+    if (v < x[1]) return 0;
+    if (v >= x[n - 2]) return n - 2;
+    int lo = 1, hi = n - 2; /* x[lo] <= v < x[hi] */
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (x[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+static void flit_ginterp(const float *x, const float *y, int n, const float *xi, int m, float *yi, int cubic) {
+    if (n == 1) {
+        for (int k = 0; k < m; ++k) yi[k] = y[0];
+        return;
+    }
+    if (!cubic || n < 3) {
+        for (int k = 0; k < m; ++k) {
+            const int s = seg_of(x, n, xi[k]);
+            const float t = (xi[k] - x[s]) / (x[s + 1] - x[s]);
+            yi[k] = y[s] + (y[s + 1] - y[s]) * t;
+        }
+        return;
+    }
+    /* natural spline: second derivatives by the tridiagonal system */
+    double *M = (double *)calloc((size_t)n, sizeof(double)), *cp = (double *)calloc((size_t)n, sizeof(double));
+    double *dp = (double *)calloc((size_t)n, sizeof(double));
+    for (int i = 1; i < n - 1; ++i) {
+        const double h0 = (double)x[i] - x[i - 1], h1 = (double)x[i + 1] - x[i];
+        const double a = h0, b = 2.0 * (h0 + h1), c = h1;
+        const double r = 6.0 * (((double)y[i + 1] - y[i]) / h1 - ((double)y[i] - y[i - 1]) / h0);
+        const double den = b - a * cp[i - 1];
+        cp[i] = c / den;
+        dp[i] = (r - a * dp[i - 1]) / den;
+    }
+    for (int i = n - 2; i >= 1; --i) M[i] = dp[i] - cp[i] * M[i + 1];
+    for (int k = 0; k < m; ++k) {
+        const int s = seg_of(x, n, xi[k]);
+        const double h = (double)x[s + 1] - x[s];
+        const double A = ((double)x[s + 1] - xi[k]) / h, B = ((double)xi[k] - x[s]) / h;
+        const double v = A * y[s] + B * y[s + 1] + ((A * A * A - A) * M[s] + (B * B * B - B) * M[s + 1]) * h * h / 6.0;
+        yi[k] = (float)v;
+    }
+    free(M);
+    free(cp);
+    free(dp);
+}
+
+static void fsmesh_free(fsmesh *S) {
+    float **p[11] = {&S->topo_i, &S->slope_i, &S->topo_h, &S->slope_h, &S->eta_i, &S->eta_h,
+                     &S->edz_i,  &S->edz_h,   &S->sc_i,   &S->sc_h,    &S->zz_i};
+    for (int k = 0; k < 11; ++k) {
+        free(*p[k]);
+        *p[k] = NULL;
+    }
+}
+
+static float fsmesh_edz(const fsmesh *S, int j) {
+    int jc = j < -S->pml ? -S->pml : j;
+    if (jc > S->nz + S->pml + 1) jc = S->nz + S->pml + 1;
+    return S->edz_i[jc + S->pml];
+}
+
+#define ZZ(S, i, j) ((S)->zz_i[(size_t)((i) + (S)->pml) * (size_t)(S)->nrj + (size_t)((j) + (S)->pml)])
+
+/* prepare_modeling's free-surface block (module_vars.f90:360-519).  c55 and
+ * rho: the window (nz0, nx) z fastest; f0min: the shot's smallest f0. */
+static int fsmesh_build(fsmesh *S, const ora_fgrid2d *g, const ora_fsurf2d *fs, const float *c55, const float *rho,
+                        float f0min) {
+    memset(S, 0, sizeof(*S));
+    const int nx = g->nx, nz0 = g->nz, pml = g->pml, nci = nx + 2 * pml + 2;
+    const float dx = g->dx, dz0 = g->dz, ox = g->ox;
+    if (nz0 < 3 || !(fs->refine > 0) || !(fs->dz_max > 0) || !(f0min > 0)) return -2;
+    S->nx = nx;
+    S->nz0 = nz0;
+    S->pml = pml;
+    S->nci = nci;
+    S->dx = dx;
+    S->dz0 = dz0;
+    int n1;
+    float *topox = flit_regspace(ox - (float)(pml + 1) * dx, dx, ox + (float)(nx + pml) * dx, &n1);
+    if (n1 != nci) {
+        free(topox);
+        return -2;
+    }
+    S->topo_i = (float *)calloc((size_t)nci, sizeof(float));
+    S->slope_i = (float *)calloc((size_t)nci, sizeof(float));
+    S->topo_h = (float *)calloc((size_t)nci, sizeof(float));
+    S->slope_h = (float *)calloc((size_t)nci, sizeof(float));
+    if (fs->ntopo > 0) {
+        flit_ginterp(fs->tx, fs->tz, fs->ntopo, topox, nci, S->topo_i, fs->interp);
+        flit_deriv(S->topo_i, nci, S->slope_i);
+        for (int k = 0; k < nci; ++k) S->slope_i[k] = S->slope_i[k] / dx;
+        float *xh = (float *)malloc(sizeof(float) * (size_t)nci);
+        for (int k = 0; k < nci; ++k) xh[k] = topox[k] - 0.5f * dx;
+        flit_ginterp(topox, S->topo_i, nci, xh, nci, S->topo_h, fs->interp);
+        flit_deriv(S->topo_h, nci, S->slope_h);
+        for (int k = 0; k < nci; ++k) S->slope_h[k] = S->slope_h[k] / dx;
+        free(xh);
+    }
+    free(topox);
+    float tmax = S->topo_i[0];
+    for (int k = 1; k < nci; ++k)
+        if (S->topo_i[k] > tmax) tmax = S->topo_i[k];
+    const float depth_max = (float)(nz0 - 1) * dz0 - tmax;
+    float eta_max = depth_max;
+    int bad = 0;
+    for (int k = 0; k < nci; ++k)
+        if ((S->topo_i[k] + depth_max) / eta_max < fmaxf(1.0f, fabsf(S->slope_i[k]))) bad = 1;
+    if (bad) {
+        float mn = INFINITY;
+        for (int k = 0; k < nci; ++k) {
+            const float v = (S->topo_i[k] + depth_max) / fmaxf(1.0f, fabsf(S->slope_i[k]));
+            if (v < mn) mn = v;
+        }
+        eta_max = 0.975f * mn;
+    }
+    const float dz = eta_max / (float)(nz0 - 1);
+    /* sqrt(mean(c55(:, 1)/rho(:, 1)))/minval(f0) over the padded first row */
+    float sm = 0.0f;
+    for (int i = -pml; i <= nx + pml + 1; ++i) {
+        const int ic = i < 1 ? 1 : (i > nx ? nx : i);
+        const size_t m = (size_t)(ic - 1) * nz0;
+        sm = sm + c55[m] / rho[m];
+    }
+    const float rl = sqrtf(sm / (float)nci) / f0min;
+    const int refine_nz = (int)ceilf(0.5f * rl / dz);
+    const float rr = fs->refine, cap = fs->dz_max * eta_max / (depth_max + tmax);
+    float dzm = fminf(cap, dz / rr);
+    if (!(dzm > 0) || refine_nz < 1) return -2;
+    int ne;
+    float *e0 = flit_regspace(0.0f, dzm, (float)(refine_nz - 1) * dz, &ne);
+    if (ne < 1) {
+        free(e0);
+        return -2;
+    }
+    int capn = ne + 64;
+    float *eta = (float *)malloc(sizeof(float) * (size_t)capn);
+    memcpy(eta, e0, sizeof(float) * (size_t)ne);
+    free(e0);
+    while (eta[ne - 1] < eta_max) {
+        dzm = fminf(dzm * 1.025f, cap);
+        if (ne == capn) {
+            capn *= 2;
+            eta = (float *)realloc(eta, sizeof(float) * (size_t)capn);
+        }
+        eta[ne] = eta[ne - 1] + dzm;
+        ++ne;
+        if (ne > 1000000) {
+            free(eta);
+            return -2;
+        }
+    }
+    const int nz = ne, nrj = nz + 2 * pml + 2;
+    S->nz = nz;
+    S->nrj = nrj;
+    S->dz = dz;
+    S->eta_max = eta_max;
+    S->depth_max = depth_max;
+    S->topo_max = tmax;
+    float *ei = (float *)calloc((size_t)nrj, sizeof(float)), *eh = (float *)calloc((size_t)nrj, sizeof(float));
+    for (int j = 1; j <= nz; ++j) ei[j + pml] = eta[j - 1];
+    eh[1 + pml] = -0.5f * dz / rr;
+    for (int j = 1; j <= nz - 1; ++j) eh[j + 1 + pml] = 0.5f * (eta[j - 1] + eta[j]);
+    for (int i = 1; i <= pml + 1; ++i) {
+        ei[1 - i + pml] = ei[1 + pml] - (float)i * dz / rr;
+        ei[nz + i + pml] = ei[nz + pml] + (float)i * dzm;
+        eh[1 - i + pml] = eh[1 + pml] - (float)i * dz / rr;
+        eh[nz + i + pml] = eh[nz + pml] + (float)i * dzm;
+    }
+    S->edz_i = (float *)calloc((size_t)nrj, sizeof(float));
+    S->edz_h = (float *)calloc((size_t)nrj, sizeof(float));
+    S->sc_i = (float *)calloc((size_t)nrj, sizeof(float));
+    S->sc_h = (float *)calloc((size_t)nrj, sizeof(float));
+    flit_deriv(ei, nrj, S->edz_i);
+    flit_deriv(eh, nrj, S->edz_h);
+    for (int k = 0; k < nrj; ++k) {
+        S->sc_i[k] = dz / S->edz_i[k];
+        S->sc_h[k] = dz / S->edz_h[k];
+    }
+    /* physical depths of the integer rows, per column */
+    S->zz_i = (float *)calloc((size_t)nci * (size_t)nrj, sizeof(float));
+    float *col = (float *)malloc(sizeof(float) * (size_t)nz);
+    for (int i = -pml; i <= nx + pml + 1; ++i) {
+        flit_rescale(eta, nz, -S->topo_i[i + pml], depth_max, col);
+        for (int j = 1; j <= nz; ++j) ZZ(S, i, j) = col[j - 1];
+    }
+    free(col);
+    float dsum = 0.0f;
+    for (int i = -pml; i <= nx + pml + 1; ++i) dsum = dsum + (ZZ(S, i, nz) - ZZ(S, i, nz - 1));
+    const float dzmean = dsum / (float)nci;
+    for (int i = -pml; i <= nx + pml + 1; ++i) {
+        for (int k = 1; k <= pml + 1; ++k) ZZ(S, i, nz + k) = ZZ(S, i, nz) + (float)k * dzmean;
+        const float d1 = ZZ(S, i, 2) - ZZ(S, i, 1);
+        for (int k = 1; k <= pml + 1; ++k) ZZ(S, i, 1 - k) = ZZ(S, i, 1) - (float)k * d1;
+    }
+    free(eta);
+    /* clip(eta, -pml dz, eta_max) */
+    const float clo = (float)(-pml) * dz;
+    for (int k = 0; k < nrj; ++k) {
+        ei[k] = fminf(fmaxf(ei[k], clo), eta_max);
+        eh[k] = fminf(fmaxf(eh[k], clo), eta_max);
+    }
+    S->eta_i = ei;
+    S->eta_h = eh;
+    return 0;
+}
+
+/* map_regular_to_irregular (module_vars.f90:169-203): each padded column
+ * of the window, at depths (k - 1) dz - max(topo) for k = -pml .. nz0+pml+1,
+ * linearly interpolated at the column's refined depths; written to the
+ * padded oracle layout with edge replication beyond the mesh. */
+static void fsmesh_medium(const fsmesh *S, const ora_fgrid2d *g, const float *v, real *out, const flay *L) {
+    const int nx = S->nx, nz0 = S->nz0, pml = S->pml, nz = S->nz, nr = nz0 + 2 * pml + 2;
+    float *zr = (float *)malloc(sizeof(float) * (size_t)nr), *vr = (float *)malloc(sizeof(float) * (size_t)nr);
+    float *zq = (float *)malloc(sizeof(float) * (size_t)S->nrj), *vq = (float *)malloc(sizeof(float) * (size_t)S->nrj);
+    float *mapped = (float *)malloc(sizeof(float) * (size_t)S->nci * (size_t)S->nrj);
+    for (int k = -pml; k <= nz0 + pml + 1; ++k) zr[k + pml] = (float)(k - 1) * g->dz - S->topo_max;
+    for (int i = -pml; i <= nx + pml + 1; ++i) {
+        const int ic = i < 1 ? 1 : (i > nx ? nx : i);
+        for (int k = -pml; k <= nz0 + pml + 1; ++k) {
+            const int kc = k < 1 ? 1 : (k > nz0 ? nz0 : k);
+            vr[k + pml] = v[(size_t)(kc - 1) + (size_t)(ic - 1) * nz0];
+        }
+        for (int j = -pml; j <= nz + pml + 1; ++j) zq[j + pml] = ZZ(S, i, j);
+        flit_ginterp(zr, vr, nr, zq, S->nrj, vq, 0);
+        memcpy(mapped + (size_t)(i + pml) * S->nrj, vq, sizeof(float) * (size_t)S->nrj);
+    }
+    for (int j = L->lo; j < L->lo + L->NZ; ++j)
+        for (int i = L->lo; i < L->lo + L->NX; ++i) {
+            const int ic = i < -pml ? -pml : (i > nx + pml + 1 ? nx + pml + 1 : i);
+            const int jc = j < -pml ? -pml : (j > nz + pml + 1 ? nz + pml + 1 : j);
+            out[IF(*L, i, j)] = (real)mapped[(size_t)(ic + pml) * S->nrj + (size_t)(jc + pml)];
+        }
+    free(zr); free(vr); free(zq); free(vq); free(mapped);
+}
+
+/* update_wavefield_free_surface (module_wavefield.f90:257-587) */
+static void fupdate_fs(const fmed *M, const fsmesh *S, ffld *F, real dt) {
+    const flay L = M->L;
+    const int R = L.R, pml = L.pml, nx = L.nx, nz = L.nz;
+    const size_t sz = (size_t)L.NX;
+    const float *c = M->c;
+    const real dx = M->dx, dz = (real)S->dz;
+    const real eta_max = (real)S->eta_max, depth_max = (real)S->depth_max;
+    /* particle velocities above the free surface are zero */
+    for (int j = 1; j <= R; ++j)
+        for (int i = L.lo; i < L.lo + L.NX; ++i) {
+            F->vx_hi[IF(L, i, 1 - j)] = 0;
+            F->vz_hi[IF(L, i, 1 - j)] = 0;
+            F->vx_ih[IF(L, i, 2 - j)] = 0;
+            F->vz_ih[IF(L, i, 2 - j)] = 0;
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= nz + pml; ++j)
+        for (int i = -pml + 1; i <= nx + pml; ++i) {
+            const size_t q = IF(L, i, j);
+            const real eta = (real)S->eta_i[j + pml];
+            const real ti = (real)S->topo_i[i + pml], si = (real)S->slope_i[i + pml];
+            const real tpaz = (eta_max - eta > 0 ? eta_max - eta : (real)0) / (ti + depth_max) * si;
+            const real tpcz = eta_max / (ti + depth_max);
+            const real c11 = M->c11[q], c13 = M->c13[q], c15 = M->c15[q], c33 = M->c33[q], c35 = M->c35[q], c55 = M->c55[q];
+            real pdxvx = dxs(F->vx_hi, q, 0, c, R) / dx, pdxvz = dxs(F->vz_hi, q, 0, c, R) / dx, pdzvx, pdzvz;
+            if (j == 1) {
+                if (i <= 0 || i >= nx + 1) {
+                    MP(F->m_dxvx_ii, q, pdxvx, M->xii.a, M->xii.b);
+                    MP(F->m_dxvz_ii, q, pdxvz, M->xii.a, M->xii.b);
+                }
+                /* z derivatives from sigma . n = 0 */
+                const real hx = si;
+                const real d11 = tpcz * (hx * hx * c11 + hx * (real)2.0f * c15 + c55);
+                const real d12 = tpcz * (hx * hx * c15 + hx * (c13 + c55) + c35);
+                const real d22 = tpcz * (hx * hx * c55 + hx * (real)2.0f * c35 + c33);
+                const real dd = d12 * d12 - d11 * d22;
+                const real f1 = (-(hx * c11) - c15) * pdxvx + (-(hx * c15) - c55) * pdxvz;
+                const real f2 = (-(hx * c15) - c13) * pdxvx + (-(hx * c55) - c35) * pdxvz;
+                pdzvx = (-(d22 * f1) + d12 * f2) / dd;
+                pdzvz = (-(d11 * f2) + d12 * f1) / dd;
+            } else {
+                pdzvx = dzs(F->vx_ih, q, 0, sz, c, R) / dz * (real)S->sc_i[j + pml];
+                pdzvz = dzs(F->vz_ih, q, 0, sz, c, R) / dz * (real)S->sc_i[j + pml];
+                if (i <= 0 || i >= nx + 1 || j >= nz + 1) {
+                    MP(F->m_dxvx_ii, q, pdxvx, M->xii.a, M->xii.b);
+                    MP(F->m_dxvz_ii, q, pdxvz, M->xii.a, M->xii.b);
+                    MP(F->m_dzvx_ii, q, pdzvx, M->zii.a, M->zii.b);
+                    MP(F->m_dzvz_ii, q, pdzvz, M->zii.a, M->zii.b);
+                }
+            }
+            const real pd1 = pdxvx + pdzvx * tpaz, pd2 = pdzvz * tpcz, pd3 = pdzvx * tpcz + pdxvz + pdzvz * tpaz;
+            F->sxx_ii[q] = F->sxx_ii[q] + dt * (c11 * pd1 + c13 * pd2 + c15 * pd3);
+            F->szz_ii[q] = F->szz_ii[q] + dt * (c13 * pd1 + c33 * pd2 + c35 * pd3);
+            F->sxz_ii[q] = F->sxz_ii[q] + dt * (c15 * pd1 + c35 * pd2 + c55 * pd3);
+        }
+#pragma omp parallel for schedule(static)
+    for (int n = 2; n <= nz + pml; ++n)
+        for (int m = -pml + 1; m <= nx + pml; ++m) {
+            const size_t q = IF(L, m, n);
+            const real eta = (real)S->eta_h[n + pml];
+            const real th = (real)S->topo_h[m + pml], sh = (real)S->slope_h[m + pml];
+            const real tpaz = (eta_max - eta > 0 ? eta_max - eta : (real)0) / (th + depth_max) * sh;
+            const real tpcz = eta_max / (th + depth_max);
+            const size_t q00 = q - 1 - sz, q10 = q - sz, q01 = q - 1, q11 = q;
+#define AV4(a) ((real)0.25f * ((a)[q00] + (a)[q10] + (a)[q01] + (a)[q11]))
+            const real c11 = AV4(M->c11), c13 = AV4(M->c13), c15 = AV4(M->c15), c33 = AV4(M->c33), c35 = AV4(M->c35),
+                       c55 = AV4(M->c55);
+#undef AV4
+            real pdxvx = dxs(F->vx_ih, q, -1, c, R) / dx, pdxvz = dxs(F->vz_ih, q, -1, c, R) / dx;
+            real pdzvx = dzs(F->vx_hi, q, -1, sz, c, R) / dz * (real)S->sc_h[n + pml];
+            real pdzvz = dzs(F->vz_hi, q, -1, sz, c, R) / dz * (real)S->sc_h[n + pml];
+            if (m <= 1 || m >= nx + 1 || n >= nz + 1) {
+                MP(F->m_dxvx_hh, q, pdxvx, M->xhh.a, M->xhh.b);
+                MP(F->m_dxvz_hh, q, pdxvz, M->xhh.a, M->xhh.b);
+                MP(F->m_dzvx_hh, q, pdzvx, M->zhh.a, M->zhh.b);
+                MP(F->m_dzvz_hh, q, pdzvz, M->zhh.a, M->zhh.b);
+            }
+            const real pd1 = pdxvx + pdzvx * tpaz, pd2 = pdzvz * tpcz, pd3 = pdzvx * tpcz + pdxvz + pdzvz * tpaz;
+            F->sxx_hh[q] = F->sxx_hh[q] + dt * (c11 * pd1 + c13 * pd2 + c15 * pd3);
+            F->szz_hh[q] = F->szz_hh[q] + dt * (c13 * pd1 + c33 * pd2 + c35 * pd3);
+            F->sxz_hh[q] = F->sxz_hh[q] + dt * (c15 * pd1 + c35 * pd2 + c55 * pd3);
+        }
+    /* odd mirror of the stresses about the free surface */
+    for (int j = 1; j <= R; ++j)
+        for (int i = L.lo; i < L.lo + L.NX; ++i) {
+            F->sxx_ii[IF(L, i, 1 - j)] = -F->sxx_ii[IF(L, i, 1 + j)];
+            F->szz_ii[IF(L, i, 1 - j)] = -F->szz_ii[IF(L, i, 1 + j)];
+            F->sxz_ii[IF(L, i, 1 - j)] = -F->sxz_ii[IF(L, i, 1 + j)];
+            F->sxx_hh[IF(L, i, 2 - j)] = -F->sxx_hh[IF(L, i, 1 + j)];
+            F->szz_hh[IF(L, i, 2 - j)] = -F->szz_hh[IF(L, i, 1 + j)];
+            F->sxz_hh[IF(L, i, 2 - j)] = -F->sxz_hh[IF(L, i, 1 + j)];
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= nz + pml; ++j)
+        for (int m = -pml + 1; m <= nx + pml; ++m) {
+            const size_t q = IF(L, m, j);
+            const real eta = (real)S->eta_i[j + pml];
+            const real th = (real)S->topo_h[m + pml], sh = (real)S->slope_h[m + pml];
+            const real tpaz = (eta_max - eta > 0 ? eta_max - eta : (real)0) / (th + depth_max) * sh;
+            const real tpcz = eta_max / (th + depth_max);
+            const real sc = (real)S->sc_i[j + pml];
+            real pdxxx = dxs(F->sxx_ii, q, -1, c, R) / dx, pdxxz = dxs(F->sxz_ii, q, -1, c, R) / dx;
+            real pdzxx = dzs(F->sxx_hh, q, 0, sz, c, R) / dz * sc;
+            real pdzxz = dzs(F->sxz_hh, q, 0, sz, c, R) / dz * sc;
+            real pdzzz = dzs(F->szz_hh, q, 0, sz, c, R) / dz * sc;
+            if (m <= 1 || m >= nx + 1 || j >= nz + 1) {
+                MP(F->m_dxxx_hi, q, pdxxx, M->xhi.a, M->xhi.b);
+                MP(F->m_dxxz_hi, q, pdxxz, M->xhi.a, M->xhi.b);
+                MP(F->m_dzxx_hi, q, pdzxx, M->zhi.a, M->zhi.b);
+                MP(F->m_dzxz_hi, q, pdzxz, M->zhi.a, M->zhi.b);
+                MP(F->m_dzzz_hi, q, pdzzz, M->zhi.a, M->zhi.b);
+            }
+            const real pd1 = pdxxx + tpaz * pdzxx + tpcz * pdzxz, pd2 = pdxxz + tpaz * pdzxz + tpcz * pdzzz;
+            const real rh = (real)0.5f * (M->rho[q - 1] + M->rho[q]);
+            F->vx_hi[q] = F->vx_hi[q] + dt / rh * pd1;
+            F->vz_hi[q] = F->vz_hi[q] + dt / rh * pd2;
+        }
+#pragma omp parallel for schedule(static)
+    for (int n = 2; n <= nz + pml; ++n)
+        for (int i = -pml + 1; i <= nx + pml; ++i) {
+            const size_t q = IF(L, i, n);
+            const real eta = (real)S->eta_h[n + pml];
+            const real ti = (real)S->topo_i[i + pml], si = (real)S->slope_i[i + pml];
+            const real tpaz = (eta_max - eta > 0 ? eta_max - eta : (real)0) / (ti + depth_max) * si;
+            const real tpcz = eta_max / (ti + depth_max);
+            const real sc = (real)S->sc_h[n + pml];
+            real pdxxx = dxs(F->sxx_hh, q, 0, c, R) / dx, pdxxz = dxs(F->sxz_hh, q, 0, c, R) / dx;
+            real pdzxx = dzs(F->sxx_ii, q, -1, sz, c, R) / dz * sc;
+            real pdzxz = dzs(F->sxz_ii, q, -1, sz, c, R) / dz * sc;
+            real pdzzz = dzs(F->szz_ii, q, -1, sz, c, R) / dz * sc;
+            if (i <= 0 || i >= nx + 1 || n >= nz + 1) {
+                MP(F->m_dxxx_ih, q, pdxxx, M->xih.a, M->xih.b);
+                MP(F->m_dxxz_ih, q, pdxxz, M->xih.a, M->xih.b);
+                MP(F->m_dzxx_ih, q, pdzxx, M->zih.a, M->zih.b);
+                MP(F->m_dzxz_ih, q, pdzxz, M->zih.a, M->zih.b);
+                MP(F->m_dzzz_ih, q, pdzzz, M->zih.a, M->zih.b);
+            }
+            const real pd1 = pdxxx + tpaz * pdzxx + tpcz * pdzxz, pd2 = pdxxz + tpaz * pdzxz + tpcz * pdzzz;
+            const real rh = (real)0.5f * (M->rho[q - sz] + M->rho[q]);
+            F->vx_ih[q] = F->vx_ih[q] + dt / rh * pd1;
+            F->vz_ih[q] = F->vz_ih[q] + dt / rh * pd2;
+        }
+}
+
+/* even mirror of the velocities before recording
+ * (submodule_forward_modeling.f90:66-75) */
+static void fmirror_v(const flay *L, ffld *F) {
+    for (int j = 1; j <= L->R; ++j)
+        for (int i = L->lo; i < L->lo + L->NX; ++i) {
+            F->vx_hi[IF(*L, i, 1 - j)] = F->vx_hi[IF(*L, i, 1 + j)];
+            F->vz_hi[IF(*L, i, 1 - j)] = F->vz_hi[IF(*L, i, 1 + j)];
+            F->vx_ih[IF(*L, i, 2 - j)] = F->vx_ih[IF(*L, i, 1 + j)];
+            F->vz_ih[IF(*L, i, 2 - j)] = F->vz_ih[IF(*L, i, 1 + j)];
+        }
+}
+
+/* Source and receiver depths on the mesh (module_vars.f90:639-680 and the
+ * depth-varying branch of prepare_geometry): the surface elevation under x
+ * by topo_interp, depth below the surface mapped to the virtual depth
+ * (z - stp)/(depth_max - stp) eta_max, nearest integer and half rows by
+ * the first minimum of |z - oz - eta|, Kaiser-sinc weights at the offset
+ * over the local row spacing.  A source above the first row moves to
+ * eta_dz_i(1) with its amplitude times 1.2. */
+static void locate_fs(const fsmesh *S, const ora_fgrid2d *g, const ora_fsurf2d *fs, float x, float z, int is_src,
+                      ora_point *P, float *amp_scale) {
+    const int pml = S->pml, nci = S->nci;
+    int n2;
+    float *topox = flit_regspace(0.0f, S->dx, (float)(S->nx + 2 * pml + 1) * S->dx, &n2);
+    float *surf = (float *)malloc(sizeof(float) * (size_t)nci);
+    for (int k = 0; k < nci; ++k) {
+        topox[k] = topox[k] - (float)(pml + 1) * S->dx;
+        surf[k] = ZZ(S, k - pml, 1);
+    }
+    float stp;
+    flit_ginterp(topox, surf, nci, &x, 1, &stp, fs->interp);
+    free(topox);
+    free(surf);
+    if (is_src ? fs->src_from_surface : fs->rec_from_surface) z = z + stp;
+    float zv = (z - stp) / (-stp + S->depth_max) * S->eta_max;
+    *amp_scale = 1.0f;
+    if (is_src && zv < S->edz_i[1 + pml]) {
+        zv = S->edz_i[1 + pml];
+        *amp_scale = 1.2f;
+    }
+    ORA_SYM(locate_point)(2, x, 0.0f, zv, g->ox, 0.0f, g->oz, g->dx, 1.0f, g->dz, P);
+    int gz = -pml, hz = -pml;
+    float bi = INFINITY, bh = INFINITY;
+    const float zs = zv + (zv == 0.0f ? 1.1920928955078125e-7f : 0.0f);
+    for (int j = -pml; j <= S->nz + pml + 1; ++j) {
+        const float di = fabsf(zv - g->oz - S->eta_i[j + pml]);
+        if (di < bi) {
+            bi = di;
+            gz = j;
+        }
+        const float dh = fabsf(zs - g->oz - S->eta_h[j + pml]);
+        if (dh < bh) {
+            bh = dh;
+            hz = j;
+        }
+    }
+    P->gz = gz;
+    P->hz = hz;
+    ORA_SYM(kaiser_sinc)((zv - g->oz - S->eta_i[gz + pml]) / S->edz_i[gz + pml], P->iz);
+    ORA_SYM(kaiser_sinc)((zv - g->oz - S->eta_h[hz + pml]) / S->edz_h[hz + pml], P->hzw);
+}
+
+/* add_source with a free surface: taps on rows >= 2 only; an explosion
+ * divides by dx dz_s with dz_s = eta_dz_i(gz)(topo_max + depth_max)/eta_max */
+static void fadd_source_fs(const fmed *M, const fsmesh *S, ffld *F, const ora_fsource2d *src, const ora_point *pts,
+                           const float *ascale, int ns, int t, real dt) {
+    const flay L = M->L;
+    for (int s = 0; s < ns; ++s) {
+        const ora_point *P = &pts[s];
+        const int nb = (int)roundf(src[s].t0 / (float)dt) + 1, ne = nb + src[s].nt_stf - 1;
+        if (t < nb || t > ne) continue;
+        real amp = src[s].stf[t - nb] * (real)(src[s].amp * ascale[s]) * dt;
+        if (src[s].mechanism == 1) {
+            const float dzs_ = S->edz_i[P->gz + S->pml] * (S->topo_max + S->depth_max) / S->eta_max;
+            amp = amp / (M->dx * (real)dzs_);
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    if (P->gz + irz >= 2) {
+                        const size_t q1 = IF(L, P->gx + irx, P->gz + irz);
+                        F->sxx_ii[q1] = F->sxx_ii[q1] - amp * (real)P->ix[irx + NKW] * (real)P->iz[irz + NKW];
+                        F->szz_ii[q1] = F->szz_ii[q1] - amp * (real)P->ix[irx + NKW] * (real)P->iz[irz + NKW];
+                    }
+                    if (P->hz + irz >= 2) {
+                        const size_t q2 = IF(L, P->hx + irx, P->hz + irz);
+                        F->sxx_hh[q2] = F->sxx_hh[q2] - amp * (real)P->hxw[irx + NKW] * (real)P->hzw[irz + NKW];
+                        F->szz_hh[q2] = F->szz_hh[q2] - amp * (real)P->hxw[irx + NKW] * (real)P->hzw[irz + NKW];
+                    }
+                }
+            continue;
+        }
+        amp = amp / M->rho[IF(L, P->gx, P->gz)];
+#if defined(ORA_F64)
+        const real sp = sin((double)src[s].polar), cp = cos((double)src[s].polar);
+#else
+        const real sp = sinf(src[s].polar), cp = cosf(src[s].polar);
+#endif
+        for (int irz = -NKW; irz <= NKW; ++irz)
+            for (int irx = -NKW; irx <= NKW; ++irx) {
+                if (P->gz + irz >= 2) {
+                    const size_t q1 = IF(L, P->hx + irx, P->gz + irz);
+                    F->vx_hi[q1] = F->vx_hi[q1] + sp * amp * (real)P->hxw[irx + NKW] * (real)P->iz[irz + NKW];
+                    F->vz_hi[q1] = F->vz_hi[q1] + cp * amp * (real)P->hxw[irx + NKW] * (real)P->iz[irz + NKW];
+                }
+                if (P->hz + irz >= 2) {
+                    const size_t q2 = IF(L, P->gx + irx, P->hz + irz);
+                    F->vx_ih[q2] = F->vx_ih[q2] + sp * amp * (real)P->ix[irx + NKW] * (real)P->hzw[irz + NKW];
+                    F->vz_ih[q2] = F->vz_ih[q2] + cp * amp * (real)P->ix[irx + NKW] * (real)P->hzw[irz + NKW];
+                }
+            }
+    }
+}
+
+typedef struct {
+    fsmesh S;
+    fmed M;
+    ora_point *sp, *rp;
+    float *ascale;
+} fsctx;
+
+static float min_f0(const float *f0, int ns) {
+    float m = f0[0];
+    for (int s = 1; s < ns; ++s)
+        if (f0[s] < m) m = f0[s];
+    return m;
+}
+
+static int fsctx_init(fsctx *X, const ora_fgrid2d *g, const ora_fsurf2d *fs, const float *const *cij, const float *rho,
+                      int ns, const ora_fsource2d *src, const float *f0, float f0max, int nr, const ora_freceiver2d *rec) {
+    memset(X, 0, sizeof(*X));
+    if (g->R < 2 || g->R > 8 || g->pml < g->R || ns < 1) return -1;
+    const int rc = fsmesh_build(&X->S, g, fs, cij[5], rho, min_f0(f0, ns));
+    if (rc) {
+        fsmesh_free(&X->S);
+        return rc;
+    }
+    fmed_init(&X->M, g, cij, rho, f0max, &X->S);
+    X->sp = (ora_point *)calloc((size_t)ns, sizeof(ora_point));
+    X->rp = (ora_point *)calloc(nr > 0 ? (size_t)nr : 1, sizeof(ora_point));
+    X->ascale = (float *)calloc((size_t)ns, sizeof(float));
+    float unused;
+    for (int s = 0; s < ns; ++s) locate_fs(&X->S, g, fs, src[s].x, src[s].z, 1, &X->sp[s], &X->ascale[s]);
+    for (int r = 0; r < nr; ++r) locate_fs(&X->S, g, fs, rec[r].x, rec[r].z, 0, &X->rp[r], &unused);
+    return 0;
+}
+
+static void fsctx_free(fsctx *X) {
+    fmed_free(&X->M);
+    fsmesh_free(&X->S);
+    free(X->sp);
+    free(X->rp);
+    free(X->ascale);
+}
+
+static void fforward_fs(const ora_fgrid2d *g, fsctx *X, const ora_fsource2d *src, int ns, int nr,
+                        const ora_freceiver2d *rec, real *tr, real *strips, real *fin) {
+    const flay L = X->M.L;
+    const size_t n = (size_t)L.NX * L.NZ, sl = fstrip_len(&L);
+    ffld F;
+    ffld_alloc(&F, &L);
+    memset(tr, 0, sizeof(real) * 2 * (size_t)nr * g->nt);
+    for (int t = 1; t <= g->nt; ++t) {
+        if (strips) fstrips(&L, &F, strips + (size_t)(t - 1) * sl, 1);
+        if (t == g->nt && fin) {
+            real **fw = &F.sxx_ii;
+            for (int k = 0; k < NSTATE; ++k) memcpy(fin + (size_t)k * n, fw[k], n * sizeof(real));
+        }
+        fmirror_v(&L, &F);
+        frecord(&L, &F, rec, X->rp, nr, g->nt, t, tr);
+        if (t == g->nt) break;
+        fupdate_fs(&X->M, &X->S, &F, (real)g->dt);
+        fadd_source_fs(&X->M, &X->S, &F, src, X->sp, X->ascale, ns, t, (real)g->dt);
+    }
+    ffld_free(&F);
+}
+
+/* The refined mesh of one window, for checking: row tables over j = -pml ..
+ * nz+pml+1 (cap entries available), column tables over i = -pml ..
+ * nx+pml+1; scal = dz, eta_max, depth_max, topo_max.  Returns -3 with *nz
+ * set when cap is too small. */
+int ORA_SYM(fsg2d_mesh)(const ora_fgrid2d *g, const ora_fsurf2d *fs, const float *c55, const float *rho, float f0min,
+                        int cap, int *nz, float *scal, float *eta_i, float *eta_h, float *sc_i, float *sc_h,
+                        float *edz_i, float *topo_i, float *slope_i, float *topo_h, float *slope_h) {
+    fsmesh S;
+    const int rc = fsmesh_build(&S, g, fs, c55, rho, f0min);
+    if (rc) {
+        fsmesh_free(&S);
+        return rc;
+    }
+    *nz = S.nz;
+    if (cap < S.nrj) {
+        fsmesh_free(&S);
+        return -3;
+    }
+    scal[0] = S.dz;
+    scal[1] = S.eta_max;
+    scal[2] = S.depth_max;
+    scal[3] = S.topo_max;
+    memcpy(eta_i, S.eta_i, sizeof(float) * (size_t)S.nrj);
+    memcpy(eta_h, S.eta_h, sizeof(float) * (size_t)S.nrj);
+    memcpy(sc_i, S.sc_i, sizeof(float) * (size_t)S.nrj);
+    memcpy(sc_h, S.sc_h, sizeof(float) * (size_t)S.nrj);
+    memcpy(edz_i, S.edz_i, sizeof(float) * (size_t)S.nrj);
+    memcpy(topo_i, S.topo_i, sizeof(float) * (size_t)S.nci);
+    memcpy(slope_i, S.slope_i, sizeof(float) * (size_t)S.nci);
+    memcpy(topo_h, S.topo_h, sizeof(float) * (size_t)S.nci);
+    memcpy(slope_h, S.slope_h, sizeof(float) * (size_t)S.nci);
+    fsmesh_free(&S);
+    return 0;
+}
+
+/* Forward traces with OWL's free surface; f0: the sources' f0 (the mesh
+ * refinement takes the smallest). */
+int ORA_SYM(fsg2d_forward_fs)(const ora_fgrid2d *g, const ora_fsurf2d *fs, const float *const *cij, const float *rho,
+                              int ns, const ora_fsource2d *src, const float *f0, float f0max, int nr,
+                              const ora_freceiver2d *rec, real *traces) {
+    fsctx X;
+    const int rc = fsctx_init(&X, g, fs, cij, rho, ns, src, f0, f0max, nr, rec);
+    if (rc) return rc;
+    fforward_fs(g, &X, src, ns, nr, rec, traces, NULL, NULL);
+    fsctx_free(&X);
+    return 0;
+}
+
+/* map_irregular_to_regular (module_vars.f90:208-249) of one refined-mesh
+ * gradient (i = 1..nx, j = 1..nz refined, x fastest) to the window (nz0, nx)
+ * z fastest: the column's cubic interpolant at (j - 1) dz0 - max(topo),
+ * zero above the free surface. */
+static void fs_to_regular(const fsmesh *S, const real *gin, float *out) {
+    const int nx = S->nx, nz = S->nz, nz0 = S->nz0;
+    float *x = (float *)malloc(sizeof(float) * (size_t)nz), *y = (float *)malloc(sizeof(float) * (size_t)nz);
+    float *xq = (float *)malloc(sizeof(float) * (size_t)nz0), *yq = (float *)malloc(sizeof(float) * (size_t)nz0);
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 1; j <= nz; ++j) {
+            x[j - 1] = ZZ(S, i, j);
+            y[j - 1] = (float)gin[(size_t)(i - 1) + (size_t)(j - 1) * nx];
+        }
+        for (int j = 1; j <= nz0; ++j) xq[j - 1] = (float)(j - 1) * S->dz0 - S->topo_max;
+        flit_ginterp(x, y, nz, xq, nz0, yq, 1);
+        for (int j = 1; j <= nz0; ++j) {
+            const float zz = (float)(j - 1) * S->dz0;
+            out[(size_t)(j - 1) + (size_t)(i - 1) * nz0] = zz < S->topo_max - S->topo_i[i + S->pml] ? 0.0f : yq[j - 1];
+        }
+    }
+    free(x); free(y); free(xq); free(yq);
+}
+
+/* OWL's FSG gradient with the free surface (as fsg2d_gradient_owl, on the
+ * refined mesh; adjoint sources on rows >= 2), mapped to the regular window.
+ * grad: 7 float arrays (nz0, nx) z fastest. */
+int ORA_SYM(fsg2d_gradient_owl_fs)(const ora_fgrid2d *g, const ora_fsurf2d *fs, const float *const *cij,
+                                   const float *rho, int ns, const ora_fsource2d *src, const float *f0, float f0max,
+                                   int hnt, int nr, const ora_freceiver2d *rec, const real *adj, int restore_all,
+                                   float *const *grad, real *traces_out) {
+    fsctx X;
+    int rc = fsctx_init(&X, g, fs, cij, rho, ns, src, f0, f0max, nr, rec);
+    if (rc) return rc;
+    const fmed *M = &X.M;
+    const flay L = M->L;
+    const size_t n = (size_t)L.NX * L.NZ, sl = fstrip_len(&L);
+    const int nt = g->nt;
+    real *strips = fz(sl * (size_t)nt), *fin = fz((size_t)NSTATE * n);
+    real *tr = traces_out ? traces_out : fz(2 * (size_t)nr * nt);
+    fforward_fs(g, &X, src, ns, nr, rec, tr, strips, fin);
+    ffld F, A, P;
+    ffld_alloc(&F, &L);
+    ffld_alloc(&A, &L);
+    ffld_alloc(&P, &L);
+    const int nx = g->nx, nz = L.nz;
+    const size_t ni = (size_t)nx * nz;
+    real *gc[7];
+    for (int c = 0; c < 7; ++c) gc[c] = fz(ni);
+    const real mdt = -(real)g->dt;
+    const int cc = g->cc_step_interval > 0 ? g->cc_step_interval : 1;
+    const size_t sx = 1, sz = (size_t)L.NX;
+    for (int t = nt; t >= hnt; --t) {
+        real **fw = &F.sxx_ii, **pw = &P.sxx_ii;
+        for (int k = 0; k < NSTATE; ++k) memcpy(pw[k], fw[k], n * sizeof(real));
+        fupdate_fs(M, &X.S, &F, mdt);
+        if (t == nt) {
+            for (int k = 0; k < NSTATE; ++k) {
+                if (!restore_all && (fw[k] == F.vz_hi || fw[k] == F.vz_ih)) continue;
+                memcpy(fw[k], fin + (size_t)k * n, n * sizeof(real));
+            }
+            if (!restore_all) memcpy(F.vz_hi, fin + (size_t)7 * n, n * sizeof(real));
+        }
+        fstrips(&L, &F, strips + (size_t)(t - 1) * sl, 0);
+        fupdate_fs(M, &X.S, &A, mdt);
+        for (int r = 0; r < nr; ++r) {
+            if (rec[r].weight == 0.0f) continue;
+            const ora_point *Q = &X.rp[r];
+            const float w = rec[r].weight;
+            const real ax = adj[(size_t)r * nt + (t - 1)], az = adj[((size_t)nr + r) * nt + (t - 1)];
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    if (Q->gz + irz < 2) continue;
+                    const real a1 = (real)(Q->hxw[irx + NKW] * Q->iz[irz + NKW] * w * 0.5f);
+                    const size_t q = IF(L, Q->hx + irx, Q->gz + irz);
+                    A.vx_hi[q] = A.vx_hi[q] + ax * a1;
+                    A.vz_hi[q] = A.vz_hi[q] + az * a1;
+                }
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    if (Q->hz + irz < 2) continue;
+                    const real a2 = (real)(Q->ix[irx + NKW] * Q->hzw[irz + NKW] * w * 0.5f);
+                    const size_t q = IF(L, Q->gx + irx, Q->hz + irz);
+                    A.vx_ih[q] = A.vx_ih[q] + ax * a2;
+                    A.vz_ih[q] = A.vz_ih[q] + az * a2;
+                }
+        }
+        if (t % cc != 0) continue;
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j <= nz; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = IF(L, i, j), o = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+                const real c11 = M->c11[q], c13 = M->c13[q], c15 = M->c15[q], c33 = M->c33[q], c35 = M->c35[q],
+                           c55 = M->c55[q];
+                const real t1 = c15 * c15 * c33 - 2 * c13 * c15 * c35 + c11 * (c35 * c35) + c13 * c13 * c55 - c11 * c33 * c55;
+                const real t2 = c15 * c15 * c33 - 2 * c13 * c15 * c35 + c13 * c13 * c55 + c11 * (c35 * c35 - c33 * c55);
+                const real s11 = (c35 * c35 - c33 * c55) / t1, s13 = (-(c15 * c35) + c13 * c55) / t2;
+                const real s15 = (c15 * c33 - c13 * c35) / t1, s33 = (c15 * c15 - c11 * c55) / t1;
+                const real s35 = (-(c13 * c15) + c11 * c35) / t2, s55 = (c13 * c13 - c11 * c33) / t1;
+#define H4(a) ((a)[q] + (a)[q + sx] + (a)[q + sz] + (a)[q + sx + sz])
+                const real txx = (F.sxx_ii[q] + (real)0.25f * H4(F.sxx_hh)) - (P.sxx_ii[q] + (real)0.25f * H4(P.sxx_hh));
+                const real tzz = (F.szz_ii[q] + (real)0.25f * H4(F.szz_hh)) - (P.szz_ii[q] + (real)0.25f * H4(P.szz_hh));
+                const real txz = (F.sxz_ii[q] + (real)0.25f * H4(F.sxz_hh)) - (P.sxz_ii[q] + (real)0.25f * H4(P.sxz_hh));
+                const real rxx = A.sxx_ii[q] + (real)0.25f * H4(A.sxx_hh);
+                const real rzz = A.szz_ii[q] + (real)0.25f * H4(A.szz_hh);
+                const real rxz = A.sxz_ii[q] + (real)0.25f * H4(A.sxz_hh);
+#undef H4
+                const real exx = s11 * txx + s13 * tzz + s15 * txz, ezz = s13 * txx + s33 * tzz + s35 * txz,
+                           exz = s15 * txx + s35 * tzz + s55 * txz;
+                const real fxx = s11 * rxx + s13 * rzz + s15 * rxz, fzz = s13 * rxx + s33 * rzz + s35 * rxz,
+                           fxz = s15 * rxx + s35 * rzz + s55 * rxz;
+                gc[0][o] = gc[0][o] - (fxx * exx);
+                gc[1][o] = gc[1][o] - (fxx * ezz + fzz * exx);
+                gc[2][o] = gc[2][o] - (fxx * exz + exx * fxz);
+                gc[3][o] = gc[3][o] - (fzz * ezz);
+                gc[4][o] = gc[4][o] - (fzz * exz + ezz * fxz);
+                gc[5][o] = gc[5][o] - (fxz * exz);
+                const real svx = ((real)0.5f * (F.vx_hi[q] + F.vx_hi[q + sx]) + (real)0.5f * (F.vx_ih[q] + F.vx_ih[q + sz])) -
+                                 ((real)0.5f * (P.vx_hi[q] + P.vx_hi[q + sx]) + (real)0.5f * (P.vx_ih[q] + P.vx_ih[q + sz]));
+                const real rvx = (real)0.5f * (A.vx_hi[q] + A.vx_hi[q + sx]) + (real)0.5f * (A.vx_ih[q] + A.vx_ih[q + sz]);
+                const real svz = ((real)0.5f * (F.vz_hi[q] + F.vz_hi[q + sx]) + (real)0.5f * (F.vz_ih[q] + F.vz_ih[q + sz])) -
+                                 ((real)0.5f * (P.vz_hi[q] + P.vz_hi[q + sx]) + (real)0.5f * (P.vz_ih[q] + P.vz_ih[q + sz]));
+                const real rvz = (real)0.5f * (A.vz_hi[q] + A.vz_hi[q + sx]) + (real)0.5f * (A.vz_ih[q] + A.vz_ih[q + sz]);
+                gc[6][o] = gc[6][o] + svx * rvx + svz * rvz;
+            }
+    }
+    for (int c = 0; c < 7; ++c) {
+        for (size_t e = 0; e < ni; ++e)
+            if (!isfinite(gc[c][e])) gc[c][e] = 0;
+        fs_to_regular(&X.S, gc[c], grad[c]);
+        free(gc[c]);
+    }
+    if (!traces_out) free(tr);
+    free(strips); free(fin);
+    ffld_free(&F); ffld_free(&A); ffld_free(&P);
+    fsctx_free(&X);
     return 0;
 }
