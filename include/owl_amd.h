@@ -559,6 +559,47 @@ owl_status owl_fsg2d_kernel_stats(const owl_fsg2d *b, int kind, double *total_ms
                                   double *bytes_total);
 owl_status owl_fsg2d_last_stats(const owl_fsg2d *b, double *ms, double *updates);
 
+/* OWL's free surface for the FSG solver (elastic_tti_2d: free_surface,
+ * file_topo, topo_interp, free_surface_dz_refine, dz_max,
+ * measure_source/receiver_depth_from_surface; interface.f90:39-68 and
+ * module_model.f90:176-178). */
+typedef struct owl_fsg_surface {
+    int ntopo;              /* topography control points; 0: flat at elevation 0 */
+    const float *topo_x;    /* x of each point (increasing) */
+    const float *topo_z;    /* elevation of each point, positive up (OWL's file_topo columns) */
+    int topo_interp;        /* 0 'linear', 1 'cubic' (OWL's default) */
+    float dz_refine;        /* free_surface_dz_refine, OWL's default 4.0 */
+    float dz_max;           /* largest z spacing; OWL's default 1.5 dz */
+    int src_depth_from_surface, rec_depth_from_surface;
+} owl_fsg_surface;
+
+/* As owl_fsg2d_create, with OWL's topography-following free surface on the
+ * depth-refined mesh (elastic_tti_2d/module_vars.f90:360-519, 532-567,
+ * 639-680; module_wavefield.f90:257-736; module_cfspml.f90:395-480): each
+ * shot window gets its own refined mesh, the medium mapped onto it, source
+ * and receiver rows on it, and the traction-free update at the surface.
+ * Gradients from owl_fsg2d_adjoint/gradient are mapped back to the regular
+ * window (map_irregular_to_regular, :208-249) and have the shapes above.
+ * surface = NULL is owl_fsg2d_create. */
+owl_status owl_fsg2d_create_surface(const owl_acoustic_config *cfg, const float *const *cij, const float *rho,
+                                    int aniso, int nshot, const owl_shot *shots, const owl_fsg_surface *surface,
+                                    owl_fsg2d **out);
+/* The refined mesh of one window, host only (the same construction as
+ * owl_fsg2d_create_surface): c55, rho (nz, nx) z fastest for the window of
+ * cfg; f0min the shot's smallest f0.  *nz: refined rows; scal: dz (the
+ * spacing the update divides by), eta_max, depth_max, topo_max.  Row tables
+ * over j = -pml .. nz+pml+1 (cap entries available; OWL_ERR_ARG with *nz
+ * set when too small): eta_i, eta_h (clipped), sc_i, sc_h (dz scaling),
+ * edz_i; column tables over i = -pml .. nx+pml+1: topo_i, slope_i, topo_h,
+ * slope_h. */
+/* Rows of one shot's mesh (the refined row count with the free surface,
+ * else the window's); 0 for a bad shot index. */
+int owl_fsg2d_rows(const owl_fsg2d *b, int shot);
+owl_status owl_fsg2d_surface_mesh(const owl_acoustic_config *cfg, const float *c55, const float *rho, float f0min,
+                                  const owl_fsg_surface *surface, int cap, int *nz, float *scal, float *eta_i,
+                                  float *eta_h, float *sc_i, float *sc_h, float *edz_i, float *topo_i,
+                                  float *slope_i, float *topo_h, float *slope_h);
+
 #ifdef __cplusplus
 }
 #endif

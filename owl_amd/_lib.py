@@ -150,6 +150,9 @@ _SIGNATURES = {
     "owl_fsg2d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_fsg2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_fsg2d_last_stats": (C.c_int, [_P, _P, _P]),
+    "owl_fsg2d_create_surface": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, _P, _P]),
+    "owl_fsg2d_surface_mesh": (C.c_int, [_P, _P, _P, C.c_float, _P, C.c_int, _P, _P] + [_P] * 9),
+    "owl_fsg2d_rows": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_create": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
     "owl_elastic3d_destroy": (None, [_P]),
     "owl_elastic3d_nt": (C.c_int, [_P]),

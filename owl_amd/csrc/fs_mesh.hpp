@@ -1,0 +1,98 @@
+// OWL's free-surface mesh for the 2D FSG solver: the depth-refined,
+// topography-following mesh of one shot window (elastic_tti_2d/
+// module_vars.f90:360-519), the window's medium on it
+// (map_regular_to_irregular, :169-203), source and receiver rows on it
+// (:639-680 and the depth-varying branch of prepare_geometry,
+// src/lib/module_source_receiver.f90:556-760), and the refined-mesh
+// gradient back on the window (map_irregular_to_regular, :208-249).
+//
+// OWL builds this with FLIT's regspace, deriv, rescale, ginterp and
+// pad_array, which are not part of the OWL tree.  They are taken as:
+//   regspace(a, d, b): nint((b - a)/d) + 1 values a + k d;
+//   deriv(x, 'center'): (x(i+1) - x(i-1))/2 inside, one-sided at the ends;
+//   rescale(x, [r1, r2]): (x - min)/(max - min)*(r2 - r1) + r1;
+//   ginterp 'linear': y0 + (y1 - y0) t on the segment x0 <= v < x1, end
+//     segments extended; 'cubic': natural cubic spline evaluated in double;
+//   pad_array: edge replication; float_small = FLT_EPSILON.
+// All in float32 in OWL's operation order.
+#pragma once
+#include <vector>
+
+namespace owl {
+namespace fsm {
+
+struct Surface {
+    std::vector<float> tx, tz;  // topography control points (x, elevation up); empty: flat at 0
+    int interp = 1;             // 0 linear, 1 cubic (OWL's topo_interp, default 'cubic')
+    float refine = 4.0f;        // free_surface_dz_refine
+    float dz_max = 0.0f;        // the caller's dz_max (OWL's default 1.5 dz)
+    bool src_from_surface = false, rec_from_surface = false;
+};
+
+// Mesh of one window: row tables over j = -pml .. nz+pml+1 at [j + pml],
+// column tables over i = -pml .. nx+pml+1 at [i + pml].  eta_i and eta_h
+// are clipped to [-pml dz, eta_max] as OWL clips them before stepping.
+struct Mesh {
+    int nx = 0, nz0 = 0, pml = 0, nz = 0;  // window columns, window rows, damping cells, refined rows
+    float dx = 0, dz0 = 0, ox = 0, oz = 0;
+    float dz = 0;  // eta_max / (nz0 - 1): the z spacing the update divides by
+    float eta_max = 0, depth_max = 0, topo_max = 0;
+    std::vector<float> topo_i, slope_i, topo_h, slope_h;
+    std::vector<float> eta_i, eta_h, edz_i, edz_h, sc_i, sc_h;
+    std::vector<float> zz;  // physical depth of integer row j in column i at [(i + pml) * nrj + j + pml]
+    int nci() const { return nx + 2 * pml + 2; }
+    int nrj() const { return nz + 2 * pml + 2; }
+    float zz_at(int i, int j) const { return zz[size_t(i + pml) * size_t(nrj()) + size_t(j + pml)]; }
+};
+
+// Builds the mesh.  c55, rho: the window (nz0, nx) z fastest; f0min: the
+// smallest source frequency of the shot.  Throws owl::Error on input that
+// cannot give a mesh.
+Mesh build(int nx, int nz0, float dx, float dz0, float ox, float oz, int pml, const Surface& s, const float* c55,
+           const float* rho, float f0min);
+
+// The window's values v (nz0, nx) z fastest on the refined mesh, over
+// i = -pml .. nx+pml+1, j = -pml .. nz+pml+1 at [(i + pml) * nrj + j + pml].
+std::vector<float> medium(const Mesh& m, const float* v);
+
+// A source or receiver on the mesh: virtual depth, nearest rows and
+// Kaiser-sinc weights in z; x as on the regular mesh.  amp_scale is 1.2 for
+// a source moved down to the first row below the surface, else 1.
+struct ZLoc {
+    float zv = 0;
+    int gz = 1, hz = 1;
+    float iz[9], hzw[9];
+    float amp_scale = 1.0f;
+};
+ZLoc locate_z(const Mesh& m, const Surface& s, float x, float z, bool source);
+
+// One refined-mesh gradient column set (nx, nz refined; x fastest) to the
+// window (nz0, nx) z fastest: the column's cubic interpolant at the regular
+// depths, zero above the free surface.
+void to_regular(const Mesh& m, const float* g, float* out);
+
+// Stable dt of the refined mesh (module_vars.f90:532-567): min over the
+// mesh of 1 / (sum|c| vmax dz0) with OWL's metric-scaled spacing.
+// vmax(i, j): the largest phase velocity at refined cell (i, j).
+template <class V>
+float dt_stable(const Mesh& m, float csum, V&& vmax);
+
+}  // namespace fsm
+}  // namespace owl
+
+// ---------------------------------------------------------------------------
+#include <cmath>
+
+template <class V>
+float owl::fsm::dt_stable(const Mesh& m, float csum, V&& vmax) {
+    float dts = INFINITY;
+    for (int j = 1; j <= m.nz + m.pml + 1; ++j)
+        for (int i = -m.pml + 1; i <= m.nx + m.pml; ++i) {
+            const float e = m.eta_i[size_t(j + m.pml)], ed = m.edz_i[size_t(j + m.pml)];
+            const float a = 1.0f / m.dx + std::fmax(0.0f, m.eta_max - e) / m.eta_max * m.slope_i[size_t(i + m.pml)] *
+                                              1.0f / ed;
+            const float dz0 = std::sqrt(a * a + (1.0f / ed) * (1.0f / ed));
+            dts = std::fmin(dts, 1.0f / (csum * vmax(i, j) * dz0));
+        }
+    return dts;
+}

@@ -7,7 +7,9 @@
 // own window, in every launch.  Moduli given as c11, c13, c15, c33, c35,
 // c55 (OWL's 'cij'), force and explosion sources, two-component velocity
 // receivers (half of each velocity grid), CFS-MPML with the damping ratios
-// of the Christoffel scan, no free surface.
+// of the Christoffel scan, and OWL's topography-following free surface on
+// its depth-refined mesh (update_wavefield_free_surface,
+// module_wavefield.f90:257-587; the mesh in fs_mesh.cpp).
 //
 // Grids, in storage index (i, j) of one padded array per field:
 //   ii  stresses at (i, j)                       sxx_ii szz_ii sxz_ii
@@ -22,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <string>
 #include <thread>
@@ -29,6 +32,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "fs_mesh.hpp"
 #include "host_prep.hpp"
 #include "owl_amd.h"
 #include "profiler.hpp"
@@ -55,16 +59,23 @@ struct ShotF {
     long long strip0;      // boundary records: nt blocks
     long long g0;          // window gradient accumulators (nx*nz, x fastest)
     int hnt;               // last reverse step OWL images for this shot (its first source's hnt)
+    // free surface: the z spacing the z derivatives divide by, eta_max,
+    // depth_max, and the offset of the shot's metric tables in the table
+    // buffer: per storage row (eta_i, eta_h, sc_i, sc_h), then per storage
+    // column (topo_i, slope_i, topo_h, slope_h)
+    float fdz, eta_max, depth_max;
+    long long met0;
 };
 
-// Wavefields (0..9) and CFS-PML memory variables (10..25), named as in OWL.
-constexpr int kNW = 10, kNM = 16, kNF = kNW + kNM;
+// Wavefields (0..9) and CFS-PML memory variables (10..27), named as in OWL.
+constexpr int kNW = 10, kNM = 18, kNF = kNW + kNM;
 struct FF {
     float *sxx_ii, *szz_ii, *sxz_ii, *sxx_hh, *szz_hh, *sxz_hh, *vx_hi, *vz_hi, *vx_ih, *vz_ih;
     float *m_dxvx_ii, *m_dxvz_ii, *m_dzvx_ii, *m_dzvz_ii;  // stress sweep, integer grid
     float *m_dxvx_hh, *m_dxvz_hh, *m_dzvx_hh, *m_dzvz_hh;  // stress sweep, corner grid
     float *m_dxxx_hi, *m_dxxz_hi, *m_dzxz_hi, *m_dzzz_hi;  // velocity sweep, (i-1/2, j)
     float *m_dxxx_ih, *m_dxxz_ih, *m_dzxz_ih, *m_dzzz_ih;  // velocity sweep, (i, j-1/2)
+    float *m_dzxx_hi, *m_dzxx_ih;                          // free surface only (memory_pdzxx_*)
 };
 
 // The ten wavefields a step writes.
@@ -159,10 +170,20 @@ __device__ __forceinline__ float sten_lanes(float v, int l, int off, const Coef&
 // current rows by shuffles, so lanes R .. 63 - R update and the others only
 // feed them.
 // ---------------------------------------------------------------------------
-template <int R>
+// Free-surface metric of one cell (update_wavefield_free_surface,
+// module_wavefield.f90:330-333): tpaz = max(0, eta_max - eta)/(topo +
+// depth_max) * slope.  A zero slope gives the slope itself, as the product
+// does, without the division.
+__device__ __forceinline__ float fs_tpaz(float eta_max, float eta, float den, float slope) {
+    if (slope == 0.0f) return slope;
+    const float a = eta_max - eta > 0.0f ? eta_max - eta : 0.0f;
+    return a / den * slope;
+}
+
+template <int R, bool FS>
 __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ shots, const int* __restrict__ tile_shot,
                                                     FF F, FV O, FM M, FP P, Coef c, float dt, float dx, float dz,
-                                                    int pml, int kc) {
+                                                    int pml, int kc, const float* __restrict__ met) {
     constexpr int W = 64 - 2 * R;
     // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
     // are wavefront-uniform and live in scalar registers
@@ -173,7 +194,8 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const int l = int(threadIdx.x);
     const int i = -pml + 1 + tx * W - R + l;
-    const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
+    // with the free surface the rows start at the surface (j = 1)
+    const int j0 = (FS ? 1 : -pml + 1) + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
     if (j0 > j1) return;  // padding tile beyond the shot's rows (whole wavefront)
     const bool mcol = i <= nx + pml + R;                 // inside the padded grid
     const bool own = l >= R && l <= 63 - R && i <= nx + pml;
@@ -190,6 +212,21 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
         qzi[m] = mcol ? ldo(F.vz_ih, r) : 0.0f;
     }
     const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
+    // free surface: the column's topography terms; integer columns take
+    // topo_i (ii grid), half columns topo_h (hh grid)
+    float den_i = 1.0f, den_h = 1.0f, sl_i = 0.0f, sl_h = 0.0f, tpc_i = 1.0f, tpc_h = 1.0f;
+    const float dzk = FS ? S.fdz : dz;
+    if constexpr (FS) {
+        if (own) {
+            const float* col = met + S.met0 + 4LL * S.nzp + 4LL * (i - lo);
+            den_i = col[0] + S.depth_max;
+            sl_i = col[1];
+            den_h = col[2] + S.depth_max;
+            sl_h = col[3];
+            tpc_i = S.eta_max / den_i;
+            tpc_h = S.eta_max / den_h;
+        }
+    }
     // The inputs of one row: the queues' fronts (row j + R) and, on the
     // owned lanes, both grids' moduli and old stresses.  Row j + 1's are
     // loaded while row j is computed.
@@ -229,42 +266,120 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
         const float sxi = sten_lanes<R>(qxi[R], l, -1, c), szi = sten_lanes<R>(qzi[R], l, -1, c);
         if (own) {
             {   // integer grid
-                float pdxvx = sxh / dx;
-                float pdxvz = szh / dx;
-                float pdzvx = sten_q<R>(qxi, 0, c) / dz;
-                float pdzvz = sten_q<R>(qzi, 0, c) / dz;
-                if (xi || j <= 0 || j >= nz + 1) {
-                    const float ax = ldo(P.axii, q), bx = ldo(P.bxii, q), az = ldo(P.azii, q), bz = ldo(P.bzii, q);
-                    pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
-                    pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
-                    pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
-                    pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
-                }
                 const float c11 = cur.ci[0], c13 = cur.ci[1], c15 = cur.ci[2];
                 const float c33 = cur.ci[3], c35 = cur.ci[4], c55 = cur.ci[5];
-                const float sh = pdxvz + pdzvx;
-                sto(O.sxx_ii, q, cur.si[0] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
-                sto(O.szz_ii, q, cur.si[1] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
-                sto(O.sxz_ii, q, cur.si[2] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
+                float pdxvx = sxh / dx;
+                float pdxvz = szh / dx;
+                if constexpr (FS) {
+                    const float* rowt = met + S.met0 + 4LL * (j - lo);
+                    const float tpaz = fs_tpaz(S.eta_max, rowt[0], den_i, sl_i), tpcz = tpc_i;
+                    float pdzvx, pdzvz;
+                    if (j == 1) {
+                        if (xi) {
+                            const float ax = ldo(P.axii, q), bx = ldo(P.bxii, q);
+                            pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
+                            pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
+                        }
+                        // z derivatives from the traction-free condition
+                        const float hx = sl_i;
+                        const float d11 = tpcz * (hx * hx * c11 + hx * 2.0f * c15 + c55);
+                        const float d12 = tpcz * (hx * hx * c15 + hx * (c13 + c55) + c35);
+                        const float d22 = tpcz * (hx * hx * c55 + hx * 2.0f * c35 + c33);
+                        const float dd = d12 * d12 - d11 * d22;
+                        const float f1 = (-(hx * c11) - c15) * pdxvx + (-(hx * c15) - c55) * pdxvz;
+                        const float f2 = (-(hx * c15) - c13) * pdxvx + (-(hx * c55) - c35) * pdxvz;
+                        pdzvx = (-(d22 * f1) + d12 * f2) / dd;
+                        pdzvz = (-(d11 * f2) + d12 * f1) / dd;
+                    } else {
+                        const float sc = rowt[2];
+                        pdzvx = sten_q<R>(qxi, 0, c) / dzk * sc;
+                        pdzvz = sten_q<R>(qzi, 0, c) / dzk * sc;
+                        if (xi || j >= nz + 1) {
+                            const float ax = ldo(P.axii, q), bx = ldo(P.bxii, q), az = ldo(P.azii, q),
+                                        bz = ldo(P.bzii, q);
+                            pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
+                            pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
+                            pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
+                            pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
+                        }
+                    }
+                    const float pd1 = pdxvx + pdzvx * tpaz, pd2 = pdzvz * tpcz;
+                    const float pd3 = pdzvx * tpcz + pdxvz + pdzvz * tpaz;
+                    const float vxx = cur.si[0] + dt * (c11 * pd1 + c13 * pd2 + c15 * pd3);
+                    const float vzz = cur.si[1] + dt * (c13 * pd1 + c33 * pd2 + c35 * pd3);
+                    const float vxz = cur.si[2] + dt * (c15 * pd1 + c35 * pd2 + c55 * pd3);
+                    sto(O.sxx_ii, q, vxx);
+                    sto(O.szz_ii, q, vzz);
+                    sto(O.sxz_ii, q, vxz);
+                    if (j >= 2 && j <= R + 1) {  // odd mirror: row 2 - j
+                        const unsigned qm = q - unsigned(2 * j - 2) * szb;
+                        sto(O.sxx_ii, qm, -vxx);
+                        sto(O.szz_ii, qm, -vzz);
+                        sto(O.sxz_ii, qm, -vxz);
+                    }
+                } else {
+                    float pdzvx = sten_q<R>(qxi, 0, c) / dz;
+                    float pdzvz = sten_q<R>(qzi, 0, c) / dz;
+                    if (xi || j <= 0 || j >= nz + 1) {
+                        const float ax = ldo(P.axii, q), bx = ldo(P.bxii, q), az = ldo(P.azii, q), bz = ldo(P.bzii, q);
+                        pdxvx = mpml(F.m_dxvx_ii, q, pdxvx, ax, bx);
+                        pdxvz = mpml(F.m_dxvz_ii, q, pdxvz, ax, bx);
+                        pdzvx = mpml(F.m_dzvx_ii, q, pdzvx, az, bz);
+                        pdzvz = mpml(F.m_dzvz_ii, q, pdzvz, az, bz);
+                    }
+                    const float sh = pdxvz + pdzvx;
+                    sto(O.sxx_ii, q, cur.si[0] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
+                    sto(O.szz_ii, q, cur.si[1] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
+                    sto(O.sxz_ii, q, cur.si[2] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
+                }
             }
-            {   // corner grid
+            if (!FS || j >= 2) {   // corner grid (with the free surface from row 2; row 1 is the mirror of row 2)
                 float pdxvx = sxi / dx;
                 float pdxvz = szi / dx;
-                float pdzvx = sten_q<R>(qxh, -1, c) / dz;
-                float pdzvz = sten_q<R>(qzh, -1, c) / dz;
-                if (xh || j <= 1 || j >= nz + 1) {
-                    const float ax = ldo(P.axhh, q), bx = ldo(P.bxhh, q), az = ldo(P.azhh, q), bz = ldo(P.bzhh, q);
-                    pdxvx = mpml(F.m_dxvx_hh, q, pdxvx, ax, bx);
-                    pdxvz = mpml(F.m_dxvz_hh, q, pdxvz, ax, bx);
-                    pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
-                    pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
-                }
                 const float c11 = cur.ch[0], c13 = cur.ch[1], c15 = cur.ch[2];
                 const float c33 = cur.ch[3], c35 = cur.ch[4], c55 = cur.ch[5];
-                const float sh = pdxvz + pdzvx;
-                sto(O.sxx_hh, q, cur.sh[0] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
-                sto(O.szz_hh, q, cur.sh[1] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
-                sto(O.sxz_hh, q, cur.sh[2] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
+                if constexpr (FS) {
+                    const float* rowt = met + S.met0 + 4LL * (j - lo);
+                    const float sc = rowt[3];
+                    float pdzvx = sten_q<R>(qxh, -1, c) / dzk * sc;
+                    float pdzvz = sten_q<R>(qzh, -1, c) / dzk * sc;
+                    if (xh || j >= nz + 1) {
+                        const float ax = ldo(P.axhh, q), bx = ldo(P.bxhh, q), az = ldo(P.azhh, q), bz = ldo(P.bzhh, q);
+                        pdxvx = mpml(F.m_dxvx_hh, q, pdxvx, ax, bx);
+                        pdxvz = mpml(F.m_dxvz_hh, q, pdxvz, ax, bx);
+                        pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
+                        pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
+                    }
+                    const float tpaz = fs_tpaz(S.eta_max, rowt[1], den_h, sl_h), tpcz = tpc_h;
+                    const float pd1 = pdxvx + pdzvx * tpaz, pd2 = pdzvz * tpcz;
+                    const float pd3 = pdzvx * tpcz + pdxvz + pdzvz * tpaz;
+                    const float vxx = cur.sh[0] + dt * (c11 * pd1 + c13 * pd2 + c15 * pd3);
+                    const float vzz = cur.sh[1] + dt * (c13 * pd1 + c33 * pd2 + c35 * pd3);
+                    const float vxz = cur.sh[2] + dt * (c15 * pd1 + c35 * pd2 + c55 * pd3);
+                    sto(O.sxx_hh, q, vxx);
+                    sto(O.szz_hh, q, vzz);
+                    sto(O.sxz_hh, q, vxz);
+                    if (j <= R + 1) {  // odd mirror: row 3 - j
+                        const unsigned qm = q - unsigned(2 * j - 3) * szb;
+                        sto(O.sxx_hh, qm, -vxx);
+                        sto(O.szz_hh, qm, -vzz);
+                        sto(O.sxz_hh, qm, -vxz);
+                    }
+                } else {
+                    float pdzvx = sten_q<R>(qxh, -1, c) / dz;
+                    float pdzvz = sten_q<R>(qzh, -1, c) / dz;
+                    if (xh || j <= 1 || j >= nz + 1) {
+                        const float ax = ldo(P.axhh, q), bx = ldo(P.bxhh, q), az = ldo(P.azhh, q), bz = ldo(P.bzhh, q);
+                        pdxvx = mpml(F.m_dxvx_hh, q, pdxvx, ax, bx);
+                        pdxvz = mpml(F.m_dxvz_hh, q, pdxvz, ax, bx);
+                        pdzvx = mpml(F.m_dzvx_hh, q, pdzvx, az, bz);
+                        pdzvz = mpml(F.m_dzvz_hh, q, pdzvz, az, bz);
+                    }
+                    const float sh = pdxvz + pdzvx;
+                    sto(O.sxx_hh, q, cur.sh[0] + dt * (c11 * pdxvx + c13 * pdzvz + c15 * sh));
+                    sto(O.szz_hh, q, cur.sh[1] + dt * (c13 * pdxvx + c33 * pdzvz + c35 * sh));
+                    sto(O.sxz_hh, q, cur.sh[2] + dt * (c15 * pdxvx + c35 * pdzvz + c55 * sh));
+                }
             }
         }
 #pragma unroll
@@ -286,10 +401,11 @@ __global__ __launch_bounds__(256) void k_fsg_stress(const ShotF* __restrict__ sh
 // dt / rho_h is sgn * |dt| / rho_h, exactly.  Tiles as the stress sweep:
 // the x stencils of sxx and sxz come from the lanes by shuffles.
 // ---------------------------------------------------------------------------
-template <int R>
+template <int R, bool FS>
 __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ shots,
                                                       const int* __restrict__ tile_shot, FF F, FV O, FM M, FP P,
-                                                      Coef c, float sgn, float dx, float dz, int pml, int kc) {
+                                                      Coef c, float sgn, float dx, float dz, int pml, int kc,
+                                                      const float* __restrict__ met) {
     constexpr int W = 64 - 2 * R;
     // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
     // are wavefront-uniform and live in scalar registers
@@ -300,13 +416,15 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const int l = int(threadIdx.x);
     const int i = -pml + 1 + tx * W - R + l;
-    const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
+    const int j0 = (FS ? 1 : -pml + 1) + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
     if (j0 > j1) return;
     const bool mcol = i <= nx + pml + R;
     const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const unsigned szb = unsigned(S.px) * 4u;
     unsigned q = unsigned(atf(S, lo, mcol ? i : lo, j0)) * 4u;  // byte offset
-    float qah[2 * R + 1], qch[2 * R + 1], qai[2 * R + 1], qci[2 * R + 1];  // sxz_hh szz_hh sxz_ii szz_ii
+    // sxz_hh szz_hh sxz_ii szz_ii; with the free surface also sxx_hh, sxx_ii
+    constexpr int NB = FS ? 2 * R + 1 : 1;
+    float qah[2 * R + 1], qch[2 * R + 1], qai[2 * R + 1], qci[2 * R + 1], qbh[NB], qbi[NB];
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m) {
         const unsigned r = q + unsigned(m - R) * szb;
@@ -314,8 +432,25 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
         qch[m] = mcol ? ldo(O.szz_hh, r) : 0.0f;
         qai[m] = mcol ? ldo(O.sxz_ii, r) : 0.0f;
         qci[m] = mcol ? ldo(O.szz_ii, r) : 0.0f;
+        if constexpr (FS) {
+            qbh[m] = mcol ? ldo(O.sxx_hh, r) : 0.0f;
+            qbi[m] = mcol ? ldo(O.sxx_ii, r) : 0.0f;
+        }
     }
     const bool xi = i <= 0 || i >= nx + 1, xh = i <= 1 || i >= nx + 1;
+    float den_i = 1.0f, den_h = 1.0f, sl_i = 0.0f, sl_h = 0.0f, tpc_i = 1.0f, tpc_h = 1.0f;
+    const float dzk = FS ? S.fdz : dz;
+    if constexpr (FS) {
+        if (own) {
+            const float* col = met + S.met0 + 4LL * S.nzp + 4LL * (i - lo);
+            den_i = col[0] + S.depth_max;
+            sl_i = col[1];
+            den_h = col[2] + S.depth_max;
+            sl_h = col[3];
+            tpc_i = S.eta_max / den_i;
+            tpc_h = S.eta_max / den_h;
+        }
+    }
     // The inputs of one row: the queues' fronts (row j + R), both sxx, and on
     // the owned lanes the buoyancies and old velocities.  Row j + 1's are
     // loaded while row j is computed.
@@ -329,8 +464,10 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
         w.fch = mcol ? ldo(O.szz_hh, r) : 0.0f;
         w.fai = mcol ? ldo(O.sxz_ii, r) : 0.0f;
         w.fci = mcol ? ldo(O.szz_ii, r) : 0.0f;
-        w.xxi = mcol ? ldo(O.sxx_ii, qr) : 0.0f;
-        w.xxh = mcol ? ldo(O.sxx_hh, qr) : 0.0f;
+        // sxx: the row itself, or with the free surface the queue fronts
+        const unsigned rx = FS ? r : qr;
+        w.xxi = mcol ? ldo(O.sxx_ii, rx) : 0.0f;
+        w.xxh = mcol ? ldo(O.sxx_hh, rx) : 0.0f;
         if (own) {
             w.qhx = ldo(M.qhx, qr);
             w.qhz = ldo(M.qhz, qr);
@@ -348,41 +485,95 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
         qch[2 * R] = cur.fch;
         qai[2 * R] = cur.fai;
         qci[2 * R] = cur.fci;
-        const float xxi = cur.xxi, xxh = cur.xxh;
+        float xxi = cur.xxi, xxh = cur.xxh;
+        if constexpr (FS) {
+            qbi[2 * R] = cur.xxi;
+            qbh[2 * R] = cur.xxh;
+            xxi = qbi[R];
+            xxh = qbh[R];
+        }
         const float d_xxi = sten_lanes<R>(xxi, l, -1, c), d_xzi = sten_lanes<R>(qai[R], l, -1, c);
         const float d_xxh = sten_lanes<R>(xxh, l, 0, c), d_xzh = sten_lanes<R>(qah[R], l, 0, c);
         if (own) {
-            {   // (i - 1/2, j)
-                float pdxxx = d_xxi / dx;
-                float pdxxz = d_xzi / dx;
-                float pdzxz = sten_q<R>(qah, 0, c) / dz;
-                float pdzzz = sten_q<R>(qch, 0, c) / dz;
-                if (xh || j <= 0 || j >= nz + 1) {
-                    const float ax = ldo(P.axhi, q), bx = ldo(P.bxhi, q), az = ldo(P.azhi, q), bz = ldo(P.bzhi, q);
-                    pdxxx = mpml(F.m_dxxx_hi, q, pdxxx, ax, bx);
-                    pdxxz = mpml(F.m_dxxz_hi, q, pdxxz, ax, bx);
-                    pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
-                    pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
+            if constexpr (FS) {
+                const float* rowt = met + S.met0 + 4LL * (j - lo);
+                {   // (i - 1/2, j), rows 1 .. nz+pml
+                    const float sc = rowt[2];
+                    float pdxxx = d_xxi / dx;
+                    float pdxxz = d_xzi / dx;
+                    float pdzxx = sten_q<R>(qbh, 0, c) / dzk * sc;
+                    float pdzxz = sten_q<R>(qah, 0, c) / dzk * sc;
+                    float pdzzz = sten_q<R>(qch, 0, c) / dzk * sc;
+                    if (xh || j >= nz + 1) {
+                        const float ax = ldo(P.axhi, q), bx = ldo(P.bxhi, q), az = ldo(P.azhi, q), bz = ldo(P.bzhi, q);
+                        pdxxx = mpml(F.m_dxxx_hi, q, pdxxx, ax, bx);
+                        pdxxz = mpml(F.m_dxxz_hi, q, pdxxz, ax, bx);
+                        pdzxx = mpml(F.m_dzxx_hi, q, pdzxx, az, bz);
+                        pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
+                        pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
+                    }
+                    const float tpaz = fs_tpaz(S.eta_max, rowt[0], den_h, sl_h), tpcz = tpc_h;
+                    const float pd1 = pdxxx + tpaz * pdzxx + tpcz * pdzxz;
+                    const float pd2 = pdxxz + tpaz * pdzxz + tpcz * pdzzz;
+                    const float w = sgn * cur.qhx;
+                    sto(O.vx_hi, q, cur.vxhi + w * pd1);
+                    sto(O.vz_hi, q, cur.vzhi + w * pd2);
                 }
-                const float w = sgn * cur.qhx;
-                sto(O.vx_hi, q, cur.vxhi + w * (pdxxx + pdzxz));
-                sto(O.vz_hi, q, cur.vzhi + w * (pdxxz + pdzzz));
-            }
-            {   // (i, j - 1/2)
-                float pdxxx = d_xxh / dx;
-                float pdxxz = d_xzh / dx;
-                float pdzxz = sten_q<R>(qai, -1, c) / dz;
-                float pdzzz = sten_q<R>(qci, -1, c) / dz;
-                if (xi || j <= 1 || j >= nz + 1) {
-                    const float ax = ldo(P.axih, q), bx = ldo(P.bxih, q), az = ldo(P.azih, q), bz = ldo(P.bzih, q);
-                    pdxxx = mpml(F.m_dxxx_ih, q, pdxxx, ax, bx);
-                    pdxxz = mpml(F.m_dxxz_ih, q, pdxxz, ax, bx);
-                    pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
-                    pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
+                if (j >= 2) {   // (i, j - 1/2), rows 2 .. nz+pml
+                    const float sc = rowt[3];
+                    float pdxxx = d_xxh / dx;
+                    float pdxxz = d_xzh / dx;
+                    float pdzxx = sten_q<R>(qbi, -1, c) / dzk * sc;
+                    float pdzxz = sten_q<R>(qai, -1, c) / dzk * sc;
+                    float pdzzz = sten_q<R>(qci, -1, c) / dzk * sc;
+                    if (xi || j >= nz + 1) {
+                        const float ax = ldo(P.axih, q), bx = ldo(P.bxih, q), az = ldo(P.azih, q), bz = ldo(P.bzih, q);
+                        pdxxx = mpml(F.m_dxxx_ih, q, pdxxx, ax, bx);
+                        pdxxz = mpml(F.m_dxxz_ih, q, pdxxz, ax, bx);
+                        pdzxx = mpml(F.m_dzxx_ih, q, pdzxx, az, bz);
+                        pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
+                        pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
+                    }
+                    const float tpaz = fs_tpaz(S.eta_max, rowt[1], den_i, sl_i), tpcz = tpc_i;
+                    const float pd1 = pdxxx + tpaz * pdzxx + tpcz * pdzxz;
+                    const float pd2 = pdxxz + tpaz * pdzxz + tpcz * pdzzz;
+                    const float w = sgn * cur.qhz;
+                    sto(O.vx_ih, q, cur.vxih + w * pd1);
+                    sto(O.vz_ih, q, cur.vzih + w * pd2);
                 }
-                const float w = sgn * cur.qhz;
-                sto(O.vx_ih, q, cur.vxih + w * (pdxxx + pdzxz));
-                sto(O.vz_ih, q, cur.vzih + w * (pdxxz + pdzzz));
+            } else {
+                {   // (i - 1/2, j)
+                    float pdxxx = d_xxi / dx;
+                    float pdxxz = d_xzi / dx;
+                    float pdzxz = sten_q<R>(qah, 0, c) / dz;
+                    float pdzzz = sten_q<R>(qch, 0, c) / dz;
+                    if (xh || j <= 0 || j >= nz + 1) {
+                        const float ax = ldo(P.axhi, q), bx = ldo(P.bxhi, q), az = ldo(P.azhi, q), bz = ldo(P.bzhi, q);
+                        pdxxx = mpml(F.m_dxxx_hi, q, pdxxx, ax, bx);
+                        pdxxz = mpml(F.m_dxxz_hi, q, pdxxz, ax, bx);
+                        pdzxz = mpml(F.m_dzxz_hi, q, pdzxz, az, bz);
+                        pdzzz = mpml(F.m_dzzz_hi, q, pdzzz, az, bz);
+                    }
+                    const float w = sgn * cur.qhx;
+                    sto(O.vx_hi, q, cur.vxhi + w * (pdxxx + pdzxz));
+                    sto(O.vz_hi, q, cur.vzhi + w * (pdxxz + pdzzz));
+                }
+                {   // (i, j - 1/2)
+                    float pdxxx = d_xxh / dx;
+                    float pdxxz = d_xzh / dx;
+                    float pdzxz = sten_q<R>(qai, -1, c) / dz;
+                    float pdzzz = sten_q<R>(qci, -1, c) / dz;
+                    if (xi || j <= 1 || j >= nz + 1) {
+                        const float ax = ldo(P.axih, q), bx = ldo(P.bxih, q), az = ldo(P.azih, q), bz = ldo(P.bzih, q);
+                        pdxxx = mpml(F.m_dxxx_ih, q, pdxxx, ax, bx);
+                        pdxxz = mpml(F.m_dxxz_ih, q, pdxxz, ax, bx);
+                        pdzxz = mpml(F.m_dzxz_ih, q, pdzxz, az, bz);
+                        pdzzz = mpml(F.m_dzzz_ih, q, pdzzz, az, bz);
+                    }
+                    const float w = sgn * cur.qhz;
+                    sto(O.vx_ih, q, cur.vxih + w * (pdxxx + pdzxz));
+                    sto(O.vz_ih, q, cur.vzih + w * (pdxxz + pdzzz));
+                }
             }
         }
 #pragma unroll
@@ -391,6 +582,10 @@ __global__ __launch_bounds__(256) void k_fsg_velocity(const ShotF* __restrict__ 
             qch[m] = qch[m + 1];
             qai[m] = qai[m + 1];
             qci[m] = qci[m + 1];
+            if constexpr (FS) {
+                qbh[m] = qbh[m + 1];
+                qbi[m] = qbi[m + 1];
+            }
         }
         cur = nxt;
     }
@@ -578,9 +773,14 @@ __global__ void k_fsg_merge(const ShotF* __restrict__ shots, const int* __restri
 struct ShotHostF {
     int id = 0, nx = 0, nz = 0, nxbeg = 1, nzbeg = 1, hnt = 1;
     float ox = 0, oz = 0;
+    long long out0 = 0;  // offset of the shot's window gradients in owl_fsg2d_adjoint's output
     std::vector<owl_source> src;
     std::vector<owl_receiver> rec;
     std::vector<float> ratio[4];  // left, right (j = -pml .. nz+pml+1), top, bottom (i = -pml .. nx+pml+1)
+    // free surface: the shot's mesh and, while planning, its medium on it
+    // (c11 .. c55, rho over i = -pml .. nx+pml+1, j = -pml .. nz+pml+1)
+    fsm::Mesh mesh;
+    std::vector<float> fmed[7];
 };
 
 }  // namespace fsg
@@ -593,6 +793,8 @@ struct owl_fsg2d {
     owl_acoustic_config cfg{};
     int R = 4, pml = 15, nt = 0, kc = 32, cy = 4, aniso = 1, restore_all = 0;
     float dt = 0;
+    bool fs = false;  // OWL's free surface on the refined mesh
+    fsm::Surface surf;
     fsg::Coef coef{};
     std::vector<ShotHostF> hs;
     std::vector<ShotF> sd;
@@ -609,6 +811,7 @@ struct owl_fsg2d {
     DevBuf<ShotF> d_shots;
     DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg, d_adj_beg, d_win;
     DevBuf<float> d_amp, d_rec_w, d_traces, d_strips, d_adj, d_obs;
+    DevBuf<float> d_met;  // free-surface metric tables
     DevBuf<float> med[14];  // ci[6], ch[6], qhx, qhz
     DevBuf<float> pmlc[16];
     DevBuf<float> comp[6];  // compliances at the integer nodes
@@ -668,27 +871,36 @@ void damping_ratios(owl_fsg2d* b) {
     std::vector<Job> jobs;
     const int pml = b->pml;
     for (auto& H : b->hs) {
-        H.ratio[0].assign(H.nz + 2 * pml + 2, 0.f);
-        H.ratio[1].assign(H.nz + 2 * pml + 2, 0.f);
+        const int nzr = b->fs ? H.mesh.nz : H.nz;
+        H.ratio[0].assign(nzr + 2 * pml + 2, 0.f);
+        H.ratio[1].assign(nzr + 2 * pml + 2, 0.f);
         H.ratio[2].assign(H.nx + 2 * pml + 2, 0.f);
         H.ratio[3].assign(H.nx + 2 * pml + 2, 0.f);
         if (!b->aniso) continue;
         auto add = [&](int i, int j, bool dirx, float* out) {
-            const size_t m = model_indexf(b, H, i, j);
             Job J;
-            for (int k = 0; k < 6; ++k) J.c[k] = b->cij_full[k][m];
-            J.rho = b->rho_full[m];
+            if (b->fs) {
+                // the refined mesh's medium (the ratios follow prepare_modeling)
+                const size_t m = size_t(i + pml) * size_t(H.mesh.nrj()) + size_t(j + pml);
+                for (int k = 0; k < 6; ++k) J.c[k] = H.fmed[k][m];
+                J.rho = H.fmed[6][m];
+            } else {
+                const size_t m = model_indexf(b, H, i, j);
+                for (int k = 0; k < 6; ++k) J.c[k] = b->cij_full[k][m];
+                J.rho = b->rho_full[m];
+            }
             J.dirx = dirx;
             J.out = out;
             jobs.push_back(J);
         };
-        for (int j = -pml; j <= H.nz + pml + 1; ++j) {
+        const int nzs = b->fs ? H.mesh.nz : H.nz;
+        for (int j = -pml; j <= nzs + pml + 1; ++j) {
             add(1, j, true, &H.ratio[0][j + pml]);
             add(H.nx, j, true, &H.ratio[1][j + pml]);
         }
         for (int i = -pml; i <= H.nx + pml + 1; ++i) {
             add(i, 1, false, &H.ratio[2][i + pml]);
-            add(i, H.nz, false, &H.ratio[3][i + pml]);
+            add(i, nzs, false, &H.ratio[3][i + pml]);
         }
     }
     if (jobs.empty()) return;
@@ -724,7 +936,9 @@ void retilef(owl_fsg2d* b) {
         ShotF& S = b->sd[s];
         const int W = 64 - 2 * b->R;  // columns a tile updates (the sweeps' halo lanes feed shuffles)
         S.ntx = (S.nx + 2 * b->pml + W - 1) / W;
-        S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
+        // rows -pml+1 .. nz+pml, or from the free surface 1 .. nz+pml
+        const int rows = b->fs ? S.nz + b->pml : S.nz + 2 * b->pml;
+        S.nzc = (rows + b->kc - 1) / b->kc;
         S.tile0 = int(b->tile_shot.size());
         b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nzc, int(s));
     }
@@ -747,6 +961,40 @@ float max_phase_velocity(const float* c, float rho) {
     return float(vmax);
 }
 
+// Stable dt of one free-surface shot (module_vars.f90:532-567): per cell of
+// the refined mesh the largest phase velocity of its medium over 36
+// directions, with OWL's metric-scaled spacing.  Rows are spread over host
+// threads; a cell with the previous cell's medium reuses its velocity.
+float surface_dt_stable(const owl_fsg2d* b, const ShotHostF& H) {
+    const fsm::Mesh& m = H.mesh;
+    const int pml = m.pml, nrj = m.nrj();
+    float csum = 0.0f;
+    for (int k = 0; k < b->R; ++k) csum = csum + std::fabs(b->coef.c[k]);
+    const int j0 = 1, j1 = m.nz + pml + 1, nrow = j1 - j0 + 1;
+    std::vector<float> vrow(size_t(nrow) * size_t(m.nx + 2 * pml), 0.f);
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nth; ++w)
+        pool.emplace_back([&, w] {
+            float last[7] = {NAN, NAN, NAN, NAN, NAN, NAN, NAN}, lv = 0.f;
+            for (int j = j0 + int(w); j <= j1; j += int(nth))
+                for (int i = -pml + 1; i <= m.nx + pml; ++i) {
+                    const size_t q = size_t(i + pml) * size_t(nrj) + size_t(j + pml);
+                    float c[7];
+                    for (int k = 0; k < 7; ++k) c[k] = H.fmed[k][q];
+                    if (std::memcmp(c, last, sizeof(c)) != 0) {
+                        lv = max_phase_velocity(c, c[6]);
+                        std::memcpy(last, c, sizeof(c));
+                    }
+                    vrow[size_t(j - j0) * size_t(m.nx + 2 * pml) + size_t(i + pml - 1)] = lv;
+                }
+        });
+    for (auto& t : pool) t.join();
+    return fsm::dt_stable(m, csum, [&](int i, int j) {
+        return vrow[size_t(j - j0) * size_t(m.nx + 2 * pml) + size_t(i + pml - 1)];
+    });
+}
+
 void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij, const float* rho, int aniso,
            int nshot, const owl_shot* shots) {
     if (!cfg || !cij || !rho || nshot < 1 || !shots) fail(OWL_ERR_ARG, "null argument or no shots");
@@ -756,7 +1004,8 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
     if (cfg->nx < 1 || cfg->nz < 1 || !(cfg->dx > 0) || !(cfg->dz > 0) || !(cfg->dt > 0))
         fail(OWL_ERR_ARG, "invalid grid or dt");
     if (cfg->pml < cfg->fdhalf || cfg->pml + cfg->fdhalf < kNkw) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
-    if (cfg->free_surface) fail(OWL_ERR_ARG, "the FSG solver here has no free surface");
+    b->fs = cfg->free_surface != 0 || b->fs;
+    if (b->fs && !(b->surf.dz_max > 0)) b->surf.dz_max = 1.5f * cfg->dz;  // module_model.f90:178
     if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
         fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
     b->cfg = *cfg;
@@ -777,7 +1026,7 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
     b->sd.assign(nshot, ShotF{});
     b->nrec_total = b->max_nr = b->max_nadj = 0;
     b->pts_full = 0;
-    long long off = 0, trace = 0, strip = 0, grad = 0;
+    long long off = 0, trace = 0, strip = 0, grad = 0, out = 0;
     std::vector<float> amp, rec_w;
     std::vector<int> inj_beg{0}, adj_beg{0};
     std::vector<Tap> inj, adj_taps;
@@ -801,35 +1050,60 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
             fail(OWL_ERR_ARG, "shot without sources or receivers");
         H.src.assign(in.sources, in.sources + in.ns);
         H.rec.assign(in.receivers, in.receivers + in.nr);
-        float vmax = 0.f;
-        for (int i = 1; i <= H.nx; ++i)
-            for (int j = 1; j <= H.nz; ++j) {
-                const size_t m = model_indexf(b, H, i, j);
-                if (vcell[m] < 0.f) {
-                    float c[6];
-                    for (int k = 0; k < 6; ++k) c[k] = b->cij_full[k][m];
-                    vcell[m] = max_phase_velocity(c, b->rho_full[m]);
-                }
-                vmax = std::max(vmax, vcell[m]);
+        float dts;
+        if (b->fs) {
+            // the window's medium, its refined mesh and the medium on it
+            std::vector<float> win[7];
+            for (int k = 0; k < 7; ++k) {
+                win[k].resize(size_t(H.nx) * H.nz);
+                for (int i = 1; i <= H.nx; ++i)
+                    for (int j = 1; j <= H.nz; ++j) {
+                        const size_t m = model_indexf(b, H, i, j);
+                        win[k][size_t(j - 1) + size_t(i - 1) * H.nz] = k < 6 ? b->cij_full[k][m] : b->rho_full[m];
+                    }
             }
-        const float dts = dt_stable(R, 2, vmax, cfg->dx, 1.0f, cfg->dz);
+            float f0min = H.src[0].f0;
+            for (auto& sc : H.src) f0min = std::min(f0min, sc.f0);
+            H.mesh = fsm::build(H.nx, H.nz, cfg->dx, cfg->dz, H.ox, H.oz, pml, b->surf, win[5].data(), win[6].data(),
+                                f0min);
+            for (int k = 0; k < 7; ++k) H.fmed[k] = fsm::medium(H.mesh, win[k].data());
+            dts = surface_dt_stable(b, H);
+        } else {
+            float vmax = 0.f;
+            for (int i = 1; i <= H.nx; ++i)
+                for (int j = 1; j <= H.nz; ++j) {
+                    const size_t m = model_indexf(b, H, i, j);
+                    if (vcell[m] < 0.f) {
+                        float c[6];
+                        for (int k = 0; k < 6; ++k) c[k] = b->cij_full[k][m];
+                        vcell[m] = max_phase_velocity(c, b->rho_full[m]);
+                    }
+                    vmax = std::max(vmax, vcell[m]);
+                }
+            dts = dt_stable(R, 2, vmax, cfg->dx, 1.0f, cfg->dz);
+        }
         if (cfg->dt > dts)
             fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " + std::to_string(dts));
         S.nx = H.nx;
-        S.nz = H.nz;
+        S.nz = b->fs ? H.mesh.nz : H.nz;
+        S.fdz = b->fs ? H.mesh.dz : cfg->dz;
+        S.eta_max = H.mesh.eta_max;
+        S.depth_max = H.mesh.depth_max;
         S.px = (H.nx + 2 * (pml + R) + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
-        S.nzp = H.nz + 2 * (pml + R);
+        S.nzp = S.nz + 2 * (pml + R);
         S.off = off;
         off += (long long)S.px * S.nzp;
         off = (off + 63) / 64 * 64;
-        b->pts_full += double(H.nx + 2 * pml) * double(H.nz + 2 * pml);
+        b->pts_full += double(H.nx + 2 * pml) * double(b->fs ? S.nz + pml : S.nz + 2 * pml);
         S.nr = int(H.rec.size());
         S.trace0 = trace;
         trace += 2LL * nt * S.nr;
         S.strip0 = strip;
-        strip += stripf_len(H.nx, H.nz, R) * nt;
+        strip += stripf_len(S.nx, S.nz, R) * nt;
         S.g0 = grad;
-        grad += (long long)H.nx * H.nz;
+        grad += (long long)S.nx * S.nz;
+        H.out0 = out;
+        out += (long long)H.nx * H.nz;
         // sources (add_source): a force adds sin/cos(polar) amp / rho(gx, gz)
         // to vx/vz on (hx, gz) with hx iz and on (gx, hz) with ix hz; an
         // explosion subtracts amp / (dx dz) from sxx, szz on (gx, gz) with
@@ -839,9 +1113,23 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
         for (const owl_source& sc : H.src) {
             if (sc.mechanism != OWL_FORCE && sc.mechanism != OWL_EXPLOSION)
                 fail(OWL_ERR_ARG, "the FSG solver takes force and explosion sources");
-            if (!(sc.x >= H.ox && sc.x <= xmax && sc.z >= H.oz && sc.z <= zmax))
+            Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            float zchk = sc.z, ascale = 1.0f, dz_s = cfg->dz;
+            if (b->fs) {
+                // depth on the refined mesh (module_vars.f90:639-680); a
+                // source above the first row moves to it with amp * 1.2
+                const fsm::ZLoc Z = fsm::locate_z(H.mesh, b->surf, sc.x, sc.z, true);
+                L.gz = Z.gz;
+                L.hz = Z.hz;
+                std::copy(Z.iz, Z.iz + 9, L.iz);
+                std::copy(Z.hzw, Z.hzw + 9, L.hzw);
+                zchk = Z.zv;
+                ascale = Z.amp_scale;
+                dz_s = H.mesh.edz_i[size_t(Z.gz + pml)] * (H.mesh.topo_max + H.mesh.depth_max) / H.mesh.eta_max;
+            }
+            if (!(sc.x >= H.ox && sc.x <= xmax && zchk >= H.oz && zchk <= zmax))
                 fail(OWL_ERR_ARG, "source outside its shot window");
-            const Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            const float samp = sc.amp * ascale;
             int hnt = 0;
             const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
             if (&sc == &H.src[0]) {
@@ -849,29 +1137,32 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
                 S.hnt = std::max(hnt, 1);
             }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
+            // with the free surface only rows >= 2 take the source (add_source)
             auto taps = [&](int gx, int gz, const float* wx, const float* wz, int field, int row) {
                 for (int irz = -kNkw; irz <= kNkw; ++irz)
                     for (int irx = -kNkw; irx <= kNkw; ++irx) {
                         const float w1 = wx[irx + kNkw], w2 = wz[irz + kNkw];
                         if (w1 == 0.f || w2 == 0.f) continue;
+                        if (b->fs && gz + irz < 2) continue;
                         st.push_back(Tap{(long long)cellf(b, S, gx + irx, gz + irz), w1, w2, row, nbeg, nend, field});
                     }
             };
             auto table = [&](float fac, bool divide_rho, float r) {
                 const int row = int(amp.size() / nt);
                 for (int t = 1; t <= nt; ++t) {
-                    float a = (t >= nbeg && t <= nend) ? stf[t - nbeg] * sc.amp * cfg->dt : 0.0f;
+                    float a = (t >= nbeg && t <= nend) ? stf[t - nbeg] * samp * cfg->dt : 0.0f;
                     if (divide_rho) {
                         a = a / r;
                         amp.push_back(fac * a);
                     } else {
-                        amp.push_back(-(a / (cfg->dx * cfg->dz)));
+                        amp.push_back(-(a / (cfg->dx * dz_s)));
                     }
                 }
                 return row;
             };
             if (sc.mechanism == OWL_FORCE) {
-                const float r = b->rho_full[model_indexf(b, H, L.gx, L.gz)];
+                const float r = b->fs ? H.fmed[6][size_t(L.gx + pml) * H.mesh.nrj() + size_t(L.gz + pml)]
+                                      : b->rho_full[model_indexf(b, H, L.gx, L.gz)];
                 const int rs = table(std::sin(sc.polar), true, r), rc = table(std::cos(sc.polar), true, r);
                 taps(L.hx, L.gz, L.hxw, L.iz, 0, rs);
                 taps(L.hx, L.gz, L.hxw, L.iz, 1, rc);
@@ -904,20 +1195,49 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
         std::vector<Tap> at;
         for (int r = 0; r < S.nr; ++r) {
             const owl_receiver& rc = H.rec[r];
-            std::vector<Tap> taps;
-            const bool valid = rc.x >= H.ox && rc.x <= xmax && rc.z >= H.oz && rc.z <= zmax;
+            std::vector<Tap> taps, ataps;
+            Located L = locate(2, rc.x, 0.0f, rc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            float zchk = rc.z;
+            if (b->fs) {
+                const fsm::ZLoc Z = fsm::locate_z(H.mesh, b->surf, rc.x, rc.z, false);
+                L.gz = Z.gz;
+                L.hz = Z.hz;
+                std::copy(Z.iz, Z.iz + 9, L.iz);
+                std::copy(Z.hzw, Z.hzw + 9, L.hzw);
+                zchk = Z.zv;
+            }
+            const bool valid = rc.x >= H.ox && rc.x <= xmax && zchk >= H.oz && zchk <= zmax;
             rec_w.push_back((valid && rc.weight != 0.f) ? rc.weight : 0.f);
             if (valid && rc.weight != 0.f) {
-                const Located L = locate(2, rc.x, 0.0f, rc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+                // With the free surface the velocities above it are read as
+                // their even mirror (submodule_forward_modeling.f90:66-75:
+                // vx_hi row 1-j = row 1+j, vx_ih row 2-j = row 1+j, j = 1..R),
+                // and the adjoint sources go to rows >= 2 only
+                // (submodule_fwi_gradient.f90:350-380).
+                auto rrow = [&](int row, bool half) {
+                    if (!b->fs) return row;
+                    if (!half && row >= 1 - R && row <= 0) return 2 - row;
+                    if (half && row >= 2 - R && row <= 1) return 3 - row;
+                    return row;
+                };
                 for (int irz = -kNkw; irz <= kNkw; ++irz)
                     for (int irx = -kNkw; irx <= kNkw; ++irx) {
                         const float a1 = L.hxw[irx + kNkw] * L.iz[irz + kNkw] * rc.weight * 0.5f;
                         const float a2 = L.ix[irx + kNkw] * L.hzw[irz + kNkw] * rc.weight * 0.5f;
-                        if (a1 != 0.f) taps.push_back(Tap{(long long)cellf(b, S, L.hx + irx, L.gz + irz), a1, 0.f, r, 0, 0, 0});
-                        if (a2 != 0.f) taps.push_back(Tap{(long long)cellf(b, S, L.gx + irx, L.hz + irz), a2, 0.f, r, 0, 0, 2});
+                        const int r1 = L.gz + irz, r2 = L.hz + irz;
+                        if (a1 != 0.f) {
+                            taps.push_back(Tap{(long long)cellf(b, S, L.hx + irx, rrow(r1, false)), a1, 0.f, r, 0, 0, 0});
+                            if (!b->fs || r1 >= 2)
+                                ataps.push_back(Tap{(long long)cellf(b, S, L.hx + irx, r1), a1, 0.f, r, 0, 0, 0});
+                        }
+                        if (a2 != 0.f) {
+                            taps.push_back(Tap{(long long)cellf(b, S, L.gx + irx, rrow(r2, true)), a2, 0.f, r, 0, 0, 2});
+                            if (!b->fs || r2 >= 2)
+                                ataps.push_back(Tap{(long long)cellf(b, S, L.gx + irx, r2), a2, 0.f, r, 0, 0, 2});
+                        }
                     }
             }
-            for (const Tap& tp : taps)
+            for (const Tap& tp : ataps)
                 for (int c = 0; c < 2; ++c) {
                     Tap u = tp;
                     u.field = tp.field + c;
@@ -997,12 +1317,25 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
         const ShotHostF& H = b->hs[s];
         const ShotF& S = b->sd[s];
         auto at = [&](int i, int j) { return size_t(S.off) + size_t(i - lo) + size_t(j - lo) * S.px; };
-        for (int j = lo; j < lo + S.nzp; ++j)
-            for (int i = lo; i < lo + S.px; ++i) {
-                const size_t m = model_indexf(b, H, i, j), q = at(i, j);
-                for (int k = 0; k < 6; ++k) ci[k][q] = b->cij_full[k][m];
-                rh[q] = b->rho_full[m];
-            }
+        if (b->fs) {
+            // the medium on the refined mesh, edge-replicated beyond it
+            const int nrj = H.mesh.nrj();
+            for (int j = lo; j < lo + S.nzp; ++j)
+                for (int i = lo; i < lo + S.px; ++i) {
+                    const int ic = std::min(std::max(i, -pml), H.nx + pml + 1);
+                    const int jc = std::min(std::max(j, -pml), S.nz + pml + 1);
+                    const size_t m = size_t(ic + pml) * size_t(nrj) + size_t(jc + pml), q = at(i, j);
+                    for (int k = 0; k < 6; ++k) ci[k][q] = H.fmed[k][m];
+                    rh[q] = H.fmed[6][m];
+                }
+        } else {
+            for (int j = lo; j < lo + S.nzp; ++j)
+                for (int i = lo; i < lo + S.px; ++i) {
+                    const size_t m = model_indexf(b, H, i, j), q = at(i, j);
+                    for (int k = 0; k < 6; ++k) ci[k][q] = b->cij_full[k][m];
+                    rh[q] = b->rho_full[m];
+                }
+        }
         for (int j = lo + 1; j < lo + S.nzp; ++j)
             for (int i = lo + 1; i < lo + S.nx + 2 * (pml + R); ++i) {
                 const size_t q = at(i, j), sz = S.px;
@@ -1038,7 +1371,7 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
             }
         for (auto& sc : H.src) f0max = std::max(f0max, sc.f0);
         const float pmlvp = std::sqrt(pm), am = float(double(1.0f * f0max) * M_PI);
-        const int nx = H.nx, nz = H.nz;
+        const int nx = H.nx, nz = S.nz;
         for (int j = -pml + 1; j <= nz + pml; ++j)
             for (int i = -pml + 1; i <= nx + pml; ++i) {
                 if (i > 1 && i < nx && j > 1 && j < nz) continue;
@@ -1067,7 +1400,9 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
                 else if (j >= nz)
                     rz = H.ratio[3][i + pml];
                 const size_t q = at(i, j);
-                const float dxf = cfg->dx, dzf = cfg->dz;
+                // with the free surface the z damping uses the local row
+                // spacing eta_dz_i(j) on all four grids (module_cfspml.f90:395-480)
+                const float dxf = cfg->dx, dzf = b->fs ? H.mesh.edz_i[size_t(j + pml)] : cfg->dz;
                 // grids: ii (xi, zi), hh (xh, zh), hi (xh, zi), ih (xi, zh)
                 damp_coef_mpml(pmlvp, xi, zi, dxf, dzf, pml, adt, am, rx, rz, &pc[0][q], &pc[1][q], &pc[2][q], &pc[3][q]);
                 damp_coef_mpml(pmlvp, xh, zh, dxf, dzf, pml, adt, am, rx, rz, &pc[4][q], &pc[5][q], &pc[6][q], &pc[7][q]);
@@ -1075,6 +1410,30 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
                 damp_coef_mpml(pmlvp, xi, zh, dxf, dzf, pml, adt, am, rx, rz, &pc[12][q], &pc[13][q], &pc[14][q],
                                &pc[15][q]);
             }
+    }
+    if (b->fs) {
+        // per shot: (eta_i, eta_h, sc_i, sc_h) per storage row, then
+        // (topo_i, slope_i, topo_h, slope_h) per storage column
+        std::vector<float> met;
+        for (size_t s = 0; s < b->hs.size(); ++s) {
+            ShotF& S = b->sd[s];
+            const fsm::Mesh& m = b->hs[s].mesh;
+            S.met0 = (long long)met.size();
+            for (int j = lo; j < lo + S.nzp; ++j) {
+                const size_t jc = size_t(std::min(std::max(j, -pml), m.nz + pml + 1) + pml);
+                met.insert(met.end(), {m.eta_i[jc], m.eta_h[jc], m.sc_i[jc], m.sc_h[jc]});
+            }
+            for (int i = lo; i < lo + S.px; ++i) {
+                const size_t ic = size_t(std::min(std::max(i, -pml), m.nx + pml + 1) + pml);
+                met.insert(met.end(), {m.topo_i[ic], m.slope_i[ic], m.topo_h[ic], m.slope_h[ic]});
+            }
+        }
+        b->d_met.upload(met, stm);
+        b->d_shots.upload(b->sd, stm);
+        for (auto& H : b->hs)
+            for (auto& v : H.fmed) std::vector<float>().swap(v);
+    } else {
+        b->d_met.upload(std::vector<float>(4, 0.f), stm);
     }
     for (int k = 0; k < 6; ++k) {
         b->med[k].upload(ci[k], stm);
@@ -1178,13 +1537,22 @@ void stepf(owl_fsg2d* b, const FF& F, const FV& O, float dt, double bs, double b
     const dim3 blk(64, unsigned(b->cy)), grd(unsigned(b->tile_shot.size() / size_t(b->cy)));
     const FM M = mediaf(b);
     const FP P = pmlf(b);
+    const float* met = b->d_met.get();
     int ev = b->prof.begin(st, 0, bs);
-    hipLaunchKernelGGL(k_fsg_stress<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M, P, b->coef,
-                       dt, b->cfg.dx, b->cfg.dz, b->pml, b->kc);
+    if (b->fs)
+        hipLaunchKernelGGL((k_fsg_stress<R, true>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
+                           P, b->coef, dt, b->cfg.dx, b->cfg.dz, b->pml, b->kc, met);
+    else
+        hipLaunchKernelGGL((k_fsg_stress<R, false>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O,
+                           M, P, b->coef, dt, b->cfg.dx, b->cfg.dz, b->pml, b->kc, met);
     b->prof.end(st, ev);
     ev = b->prof.begin(st, 1, bv);
-    hipLaunchKernelGGL(k_fsg_velocity<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M, P,
-                       b->coef, dt < 0 ? -1.0f : 1.0f, b->cfg.dx, b->cfg.dz, b->pml, b->kc);
+    if (b->fs)
+        hipLaunchKernelGGL((k_fsg_velocity<R, true>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O,
+                           M, P, b->coef, dt < 0 ? -1.0f : 1.0f, b->cfg.dx, b->cfg.dz, b->pml, b->kc, met);
+    else
+        hipLaunchKernelGGL((k_fsg_velocity<R, false>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O,
+                           M, P, b->coef, dt < 0 ? -1.0f : 1.0f, b->cfg.dx, b->cfg.dz, b->pml, b->kc, met);
     b->prof.end(st, ev);
 }
 
@@ -1360,9 +1728,85 @@ void traces_to_devicef(owl_fsg2d* b, const float* in, DevBuf<float>& dev) {
     OWL_HIP(hipStreamSynchronize(b->stream));
 }
 
+// The caller's surface description; zero dz_refine and dz_max take OWL's
+// defaults (4 and 1.5 dz).
+fsm::Surface surface_of(const owl_fsg_surface* sf, float dz) {
+    fsm::Surface S;
+    S.dz_max = 1.5f * dz;
+    if (!sf) return S;
+    if (sf->ntopo < 0 || (sf->ntopo > 0 && (!sf->topo_x || !sf->topo_z))) fail(OWL_ERR_ARG, "bad topography");
+    S.tx.assign(sf->topo_x, sf->topo_x + sf->ntopo);
+    S.tz.assign(sf->topo_z, sf->topo_z + sf->ntopo);
+    if (sf->topo_interp != 0 && sf->topo_interp != 1) fail(OWL_ERR_ARG, "topo_interp must be 0 (linear) or 1 (cubic)");
+    S.interp = sf->topo_interp;
+    if (sf->dz_refine < 0 || sf->dz_max < 0) fail(OWL_ERR_ARG, "dz_refine and dz_max must not be negative");
+    if (sf->dz_refine > 0) S.refine = sf->dz_refine;
+    if (sf->dz_max > 0) S.dz_max = sf->dz_max;
+    S.src_from_surface = sf->src_depth_from_surface != 0;
+    S.rec_from_surface = sf->rec_depth_from_surface != 0;
+    return S;
+}
+
+// One shot's refined-mesh accumulators (nx, nz refined, x fastest) on its
+// regular window (nz0, nx) z fastest, NaN/Inf -> 0 first.
+void shot_to_regular(const owl_fsg2d* b, size_t s, const double* g, float* out) {
+    const ShotF& S = b->sd[s];
+    std::vector<float> gf(size_t(S.nx) * S.nz);
+    for (size_t e = 0; e < gf.size(); ++e) {
+        const float v = float(g[size_t(S.g0) + e]);
+        gf[e] = std::isfinite(v) ? v : 0.f;
+    }
+    fsm::to_regular(b->hs[s].mesh, gf.data(), out);
+}
+
 }  // namespace
 
 extern "C" {
+
+owl_status owl_fsg2d_create_surface(const owl_acoustic_config* cfg, const float* const* cij, const float* rho,
+                                    int aniso, int nshot, const owl_shot* shots, const owl_fsg_surface* surface,
+                                    owl_fsg2d** out) {
+    return guarded([&] {
+        if (!out || !cfg) fail(OWL_ERR_ARG, "null argument");
+        *out = nullptr;
+        auto* b = new owl_fsg2d();
+        try {
+            b->fs = surface != nullptr || cfg->free_surface != 0;
+            b->surf = surface_of(surface, cfg->dz);
+            planf(b, cfg, cij, rho, aniso, nshot, shots);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+owl_status owl_fsg2d_surface_mesh(const owl_acoustic_config* cfg, const float* c55, const float* rho, float f0min,
+                                  const owl_fsg_surface* surface, int cap, int* nz, float* scal, float* eta_i,
+                                  float* eta_h, float* sc_i, float* sc_h, float* edz_i, float* topo_i,
+                                  float* slope_i, float* topo_h, float* slope_h) {
+    return guarded([&] {
+        if (!cfg || !c55 || !rho || !nz || !scal) fail(OWL_ERR_ARG, "null argument");
+        const fsm::Surface S = surface_of(surface, cfg->dz);
+        const fsm::Mesh m =
+            fsm::build(cfg->nx, cfg->nz, cfg->dx, cfg->dz, cfg->ox, cfg->oz, cfg->pml, S, c55, rho, f0min);
+        *nz = m.nz;
+        if (cap < m.nrj()) fail(OWL_ERR_ARG, "row tables need nz + 2 pml + 2 entries");
+        scal[0] = m.dz;
+        scal[1] = m.eta_max;
+        scal[2] = m.depth_max;
+        scal[3] = m.topo_max;
+        const std::vector<float>* rows[5] = {&m.eta_i, &m.eta_h, &m.sc_i, &m.sc_h, &m.edz_i};
+        float* rdst[5] = {eta_i, eta_h, sc_i, sc_h, edz_i};
+        for (int k = 0; k < 5; ++k)
+            if (rdst[k]) std::copy(rows[k]->begin(), rows[k]->end(), rdst[k]);
+        const std::vector<float>* cols[4] = {&m.topo_i, &m.slope_i, &m.topo_h, &m.slope_h};
+        float* cdst[4] = {topo_i, slope_i, topo_h, slope_h};
+        for (int k = 0; k < 4; ++k)
+            if (cdst[k]) std::copy(cols[k]->begin(), cols[k]->end(), cdst[k]);
+    });
+}
 
 owl_status owl_fsg2d_create(const owl_acoustic_config* cfg, const float* const* cij, const float* rho, int aniso,
                             int nshot, const owl_shot* shots, owl_fsg2d** out) {
@@ -1383,6 +1827,11 @@ owl_status owl_fsg2d_create(const owl_acoustic_config* cfg, const float* const* 
 void owl_fsg2d_destroy(owl_fsg2d* b) { delete b; }
 
 int owl_fsg2d_nt(const owl_fsg2d* b) { return b ? b->nt : 0; }
+
+int owl_fsg2d_rows(const owl_fsg2d* b, int shot) {
+    if (!b || shot < 0 || shot >= int(b->sd.size())) return 0;
+    return b->sd[size_t(shot)].nz;
+}
 
 owl_status owl_fsg2d_damping_ratios(const owl_fsg2d* b, int shot, float* left, float* right, float* top,
                                     float* bottom) {
@@ -1427,13 +1876,21 @@ owl_status owl_fsg2d_adjoint(owl_fsg2d* b, const float* adj, float* const* grads
         for (int k = 0; k < 7; ++k) {
             if (!grads[k]) continue;
             OWL_HIP(hipMemcpy(g.data(), b->d_g[k].get(), g.size() * sizeof(double), hipMemcpyDeviceToHost));
-            // per-shot window gradients (nz_s, nx_s), z fastest, NaN/Inf -> 0
-            for (const ShotF& S : b->sd)
+            // per-shot window gradients (nz_s, nx_s), z fastest, NaN/Inf -> 0;
+            // with the free surface mapped from the refined mesh
+            for (size_t s = 0; s < b->sd.size(); ++s) {
+                const ShotF& S = b->sd[s];
+                float* dst = grads[k] + b->hs[s].out0;
+                if (b->fs) {
+                    shot_to_regular(b, s, g.data(), dst);
+                    continue;
+                }
                 for (int kz = 0; kz < S.nz; ++kz)
                     for (int i = 0; i < S.nx; ++i) {
                         const float v = float(g[size_t(S.g0) + i + size_t(kz) * S.nx]);
-                        grads[k][size_t(S.g0) + kz + size_t(i) * S.nz] = std::isfinite(v) ? v : 0.f;
+                        dst[kz + size_t(i) * S.nz] = std::isfinite(v) ? v : 0.f;
                     }
+            }
         }
     });
 }
@@ -1462,7 +1919,26 @@ owl_status owl_fsg2d_gradient(owl_fsg2d* b, const float* obs, double* misfit, do
         b->d_full.alloc(nfull);
         const unsigned mg = unsigned(std::min<size_t>(8192, (nfull + 255) / 256));
         std::vector<double> h(nfull);
-        for (int k = 0; k < 7; ++k) {
+        if (b->fs) {
+            // map each shot's refined-mesh gradient to its window on the host,
+            // then sum the windows on the model (module_gradient.f90:366-388)
+            std::vector<double> g(b->grad_elems);
+            for (int k = 0; k < 7; ++k) {
+                if (!grads || !grads[k]) continue;
+                OWL_HIP(hipMemcpy(g.data(), b->d_g[k].get(), g.size() * sizeof(double), hipMemcpyDeviceToHost));
+                std::fill(grads[k], grads[k] + nfull, 0.0);
+                for (size_t s = 0; s < b->sd.size(); ++s) {
+                    const ShotHostF& H = b->hs[s];
+                    std::vector<float> w(size_t(H.nx) * H.nz);
+                    shot_to_regular(b, s, g.data(), w.data());
+                    for (int i = 0; i < H.nx; ++i)
+                        for (int kz = 0; kz < H.nz; ++kz)
+                            grads[k][size_t(H.nzbeg - 1 + kz) + size_t(H.nxbeg - 1 + i) * b->cfg.nz] +=
+                                double(w[size_t(kz) + size_t(i) * H.nz]);
+                }
+            }
+        }
+        for (int k = 0; k < 7 && !b->fs; ++k) {
             if (!grads || !grads[k]) continue;
             hipLaunchKernelGGL(k_fsg_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), b->d_win.get(), int(nshot),
                                b->d_g[k].get(), b->d_full.get(), b->cfg.nx, b->cfg.nz);

@@ -70,30 +70,97 @@ def tti_param_gradient(param, grads, vp, vs, rho, p1=None, p2=None, theta=None):
     return {"grad_" + k: a for k, a in zip(names, out)}
 
 
+class Surface(C.Structure):
+    """owl_fsg_surface: OWL's free surface (topography, refinement)."""
+    _fields_ = [("ntopo", C.c_int), ("topo_x", C.c_void_p), ("topo_z", C.c_void_p), ("topo_interp", C.c_int),
+                ("dz_refine", C.c_float), ("dz_max", C.c_float), ("src_depth_from_surface", C.c_int),
+                ("rec_depth_from_surface", C.c_int)]
+
+
+def _surface(surface):
+    """dict -> (Surface, arrays to keep alive).  Keys, all optional: topo =
+    (x, elevation) control points (elevation positive up; none: flat at
+    0), interp "linear" | "cubic" (OWL's default), refine
+    (free_surface_dz_refine, 4), dz_max (1.5 dz), src_from_surface,
+    rec_from_surface (depths measured from the surface)."""
+    keep = []
+    topo = surface.get("topo")
+    n, px, pz = 0, None, None
+    if topo is not None:
+        tx = np.ascontiguousarray(topo[0], np.float32)
+        tz = np.ascontiguousarray(topo[1], np.float32)
+        keep += [tx, tz]
+        n, px, pz = len(tx), tx.ctypes.data, tz.ctypes.data
+    sf = Surface(n, px, pz, {"linear": 0, "cubic": 1}[surface.get("interp", "cubic")],
+                 float(surface.get("refine", 0.0)), float(surface.get("dz_max", 0.0)),
+                 int(bool(surface.get("src_from_surface", False))), int(bool(surface.get("rec_from_surface", False))))
+    return sf, keep
+
+
+def surface_mesh(nx, nz, dx, dz, c55, rho, f0min, surface, pml=15, ox=0.0, oz=0.0):
+    """OWL's refined free-surface mesh of one window (host computation, no
+    GPU): dict of the row tables eta_i, eta_h (clipped), sc_i, sc_h, edz_i
+    (j = -pml .. nz+pml+1), the column tables topo_i, slope_i, topo_h,
+    slope_h (i = -pml .. nx+pml+1), nz (refined rows), dz, eta_max,
+    depth_max, topo_max.  c55, rho: (nz, nx)."""
+    from . import _lib
+    cfg = _lib.owl_acoustic_config(int(nx), 1, int(nz), float(dx), 1.0, float(dz), float(ox), 0.0, float(oz),
+                                   1.0, 1.0, 1.0, 1.0, int(pml), 4, 1, 1.0, 1, 1.0e-6)
+    c55 = np.ascontiguousarray(np.asarray(c55, np.float32).T)
+    rho = np.ascontiguousarray(np.asarray(rho, np.float32).T)
+    sf, keep = _surface(surface)
+    nzr = C.c_int(0)
+    cap = 8192
+    rows = [np.zeros(cap, np.float32) for _ in range(5)]
+    cols = [np.zeros(nx + 2 * pml + 2, np.float32) for _ in range(4)]
+    scal = np.zeros(4, np.float32)
+    check(lib().owl_fsg2d_surface_mesh(C.byref(cfg), _ptr(c55), _ptr(rho), float(f0min), C.byref(sf), cap,
+                                       C.byref(nzr), _ptr(scal), *[_ptr(a) for a in rows + cols]))
+    m = nzr.value + 2 * pml + 2
+    out = dict(zip(("eta_i", "eta_h", "sc_i", "sc_h", "edz_i"), (r[:m].copy() for r in rows)))
+    out.update(zip(("topo_i", "slope_i", "topo_h", "slope_h"), cols))
+    out.update(nz=nzr.value, dz=float(scal[0]), eta_max=float(scal[1]), depth_max=float(scal[2]),
+               topo_max=float(scal[3]))
+    return out
+
+
 class FSG2DBatch(TwoComponentBatch):
     """cij: dict of c11, c13, c15, c33, c35, c55, each (nz, nx); rho (nz, nx).
     aniso: True takes the M-PML damping ratios from the Christoffel scan,
     False sets them to 0 (OWL's anisotropy_type 'iso').  shots: list of
-    (Geometry, window) as for Elastic2DBatch."""
+    (Geometry, window) as for Elastic2DBatch.  surface: None, or a dict as
+    for surface_mesh for OWL's topography-following free surface on the
+    depth-refined mesh (gradients come back on the regular windows)."""
 
     PREFIX = "owl_fsg2d"
 
     def __init__(self, nx, nz, dx, dz, cij, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
-                 f0factor=1.0, cc_step_interval=1, aniso=True, trace_discard_threshold=1.0e-6):
+                 f0factor=1.0, cc_step_interval=1, aniso=True, trace_discard_threshold=1.0e-6, surface=None):
         self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval,
                     trace_discard_threshold)
         self._m = [self._model(cij[k]) for k in CIJ]
         self._rho = self._model(rho)
         ptrs = (C.c_void_p * 6)(*[a.ctypes.data for a in self._m])
         h = C.c_void_p()
-        check(lib().owl_fsg2d_create(C.byref(self.cfg), ptrs, _ptr(self._rho), int(bool(aniso)), len(shots),
-                                     C.cast(self._shots, C.c_void_p), C.byref(h)))
+        if surface is None:
+            check(lib().owl_fsg2d_create(C.byref(self.cfg), ptrs, _ptr(self._rho), int(bool(aniso)), len(shots),
+                                         C.cast(self._shots, C.c_void_p), C.byref(h)))
+        else:
+            sf, keep = _surface(surface)
+            check(lib().owl_fsg2d_create_surface(C.byref(self.cfg), ptrs, _ptr(self._rho), int(bool(aniso)),
+                                                 len(shots), C.cast(self._shots, C.c_void_p), C.byref(sf),
+                                                 C.byref(h)))
         self._h = h
         self.nt = lib().owl_fsg2d_nt(h)
+
+    def rows(self, shot=0):
+        """Rows of one shot's mesh (refined with the free surface)."""
+        return lib().owl_fsg2d_rows(self._h, int(shot))
 
     def damping_ratios(self, shot=0):
         """(left, right, top, bottom) M-PML ratio profiles of one shot."""
         nz, nx = self._windows[shot]
+        nz = self.rows(shot)
         pml = self.cfg.pml
         out = [np.zeros(nz + 2 * pml + 2, np.float32) for _ in range(2)] + \
               [np.zeros(nx + 2 * pml + 2, np.float32) for _ in range(2)]
