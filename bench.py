@@ -51,8 +51,8 @@ TRAFFIC = {
     ("acoustic3d_forward", "update_p"): (2.0 * 4.318e6 + 1.765e6) * 1024,
     ("elastic3d_forward", "stress"): (2.0 * 6.017e6 + 2.686e6) * 1024,      # elastic3d_forward_200x200x150_8shots.md
     ("elastic3d_forward", "velocity"): (2.0 * 5.066e6 + 1.602e6) * 1024,
-    ("fsg2d_forward", "stress"): (2.0 * 1.005e6 + 4.988e5) * 1024,          # fsg2d_forward_801x401_64shots.md
-    ("fsg2d_forward", "velocity"): (2.0 * 6.090e5 + 3.574e5) * 1024,
+    ("fsg2d_forward", "stress"): (2.0 * 7.750e5 + 3.767e5) * 1024,   # fsg2d_forward_free_surface_801x401_64shots.md
+    ("fsg2d_forward", "velocity"): (2.0 * 4.983e5 + 2.834e5) * 1024,
 }
 
 
@@ -296,20 +296,26 @@ def bench_elastic3d(sync, steps):
             "kernels": kernel_table(stats, 1)}
 
 
-def bench_fsg(sync, steps):
-    """2D tilted-TI elastic on the Lebedev fully staggered grid, batched
-    forward: 801 x 401, 5 m, R = 4, dt 0.4 ms, 20-cell CFS-MPML with the
-    Christoffel-scan damping ratios, 64 shots with uneven 200..801-column
-    windows, vertical forces, two-component receivers every 10 m; no free
-    surface."""
+def bench_fsg(sync, steps, gradient=True):
+    """2D tilted-TI elastic on the Lebedev fully staggered grid with OWL's
+    flat free surface on the depth-refined mesh (free_surface_dz_refine 4,
+    dz_max 1.5 dz), batched: 801 x 401, 5 m, R = 4, dt 0.4 ms, 20-cell
+    CFS-MPML with the Christoffel-scan damping ratios, 64 shots with uneven
+    200..801-column windows, vertical forces, two-component receivers every
+    10 m.  The forward is timed with per-kernel events; then OWL's gradient
+    (boundary-saving reconstruction, c11 .. c55 and rho, FP64 imaging,
+    mapped back to the regular windows and summed) over all shots, without
+    events."""
     from owl_amd import FSG2DBatch
     from owl_amd.synthetic import fsg2d_shots, fsg2d_tti_model
     cij, rho = fsg2d_tti_model()
     shots = fsg2d_shots()
     dt = 0.0004
     t0 = time.perf_counter()
-    b = FSG2DBatch(801, 401, 5.0, 5.0, cij, rho, shots, dt, (steps - 1) * dt, pml=20, fdhalf=4)
+    b = FSG2DBatch(801, 401, 5.0, 5.0, cij, rho, shots, dt, (steps - 1) * dt, pml=20, fdhalf=4,
+                   surface=dict(refine=4.0))
     plan_s = time.perf_counter() - t0
+    rows = [b.rows(s) for s in range(len(shots))]
     b.set_profiling(True)
     sync()
     t0 = time.perf_counter()
@@ -318,13 +324,32 @@ def bench_fsg(sync, steps):
     wall = time.perf_counter() - t0
     stats = b.kernel_stats()
     ms, updates = b.last_stats()
+    grad = None
+    if gradient:
+        b.set_profiling(False)
+        obs = [tuple(np.zeros((nr, b.nt), np.float32) for _ in range(2)) for nr in b.nr]
+        b.gradient(obs)  # untimed warm-up: buffers, strips
+        sync()
+        t0 = time.perf_counter()
+        mis, g, _ = b.gradient(obs)
+        sync()
+        gw = time.perf_counter() - t0
+        gms, gup = b.last_stats()
+        grad = {"seconds": gw, "s_per_shot": gw / len(shots), "device_ms": gms,
+                "grid_pt_updates_per_s": gup / gw,
+                "algorithm": "OWL's gradient: forward with boundary records in HBM, L2 adjoint source per "
+                             "component (observed traces zero), reverse reconstruction and adjoint, FP64 "
+                             "imaging of c11..c55 and rho on the refined meshes, mapped to the regular "
+                             "windows and summed on the host",
+                "finite": bool(all(np.isfinite(v).all() for v in g.values()))}
     b.close()
     _, roof = roofline_of(stats)
-    return {"workload": f"2D tilted-TI FSG batched forward, 801x401, 5 m, dt 0.4 ms, R 4, pml 20, 64 shots "
-                        f"(windows 200-801), {steps} steps, no free surface",
+    return {"workload": f"2D tilted-TI FSG batched forward with OWL's flat free surface on the depth-refined "
+                        f"mesh (refinement 4, dz_max 7.5 m; {min(rows)}-{max(rows)} refined rows per window), "
+                        f"801x401, 5 m, dt 0.4 ms, R 4, pml 20, 64 shots (windows 200-801), {steps} steps",
             "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
             "plan_seconds": round(plan_s, 3), "roofline": with_traffic("fsg2d_forward", roof),
-            "kernels": kernel_table(stats, 1)}
+            "kernels": kernel_table(stats, 1), "gradient": grad}
 
 
 def main():
@@ -352,6 +377,7 @@ def main():
     ap.add_argument("--shape-3d", default="", metavar="PLANES,VARIANT",
                     help="with --only-3d: this launch shape instead of the autotuner's")
     ap.add_argument("--no-fsg", action="store_true", help="skip the 2D FSG forward")
+    ap.add_argument("--no-fsg-gradient", action="store_true", help="skip the 2D FSG gradient")
     ap.add_argument("--no-elastic3d", action="store_true", help="skip the 3D elastic forward")
     ap.add_argument("--only-elastic3d", action="store_true", help="only the 3D elastic forward (for profiler runs)")
     ap.add_argument("--elastic3d-steps", type=int, default=2500, help="time steps of the 3D elastic forward")
@@ -399,7 +425,7 @@ def main():
         print(json.dumps({"elastic3d_forward": bench_elastic3d(sync, args.elastic3d_steps)}))
         return
     if args.only_fsg:
-        print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps)}))
+        print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient)}))
         return
     log("building workload")
     vp, rho, vp0, rho0, shots = workload()
@@ -490,8 +516,8 @@ def main():
 
     fsg = None
     if not args.no_fsg:
-        log("2D FSG batched forward")
-        fsg = bench_fsg(sync, args.fsg_steps)
+        log("2D FSG batched forward and gradient")
+        fsg = bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient)
 
     cpu = None
     if not args.no_cpu:

@@ -1500,8 +1500,9 @@ FP pmlf(owl_fsg2d* b) {
 void sweep_bytesf(const owl_fsg2d* b, double& bs, double& bv) {
     bs = bv = 0;
     for (auto& S : b->sd) {
-        const double lx = S.nx + 2.0 * b->pml, lz = S.nz + 2.0 * b->pml;
-        const double frame = lx * lz - double(std::max(S.nx - 2, 0)) * double(std::max(S.nz - 2, 0));
+        // with the free surface the sweeps start at the surface row
+        const double lx = S.nx + 2.0 * b->pml, lz = S.nz + (b->fs ? 1.0 : 2.0) * b->pml;
+        const double frame = lx * lz - double(std::max(S.nx - 2, 0)) * double(std::max(S.nz - (b->fs ? 1 : 2), 0));
         bs += 112.0 * lx * lz + 2.0 * 48.0 * frame;
         bv += 64.0 * lx * lz + 2.0 * 48.0 * frame;
     }

@@ -11,9 +11,12 @@
 ! dir_adjoint/shot_<id>_seismogram_{vx,vz}.su into seis_vxr, seis_vzr and
 ! writes dir_working/shot_<id>_grad_<m>.grd for the parameters of
 ! anisotropy_type (c11 .. c55; or vp, vs [, epsilon, delta | eta]) and rho.
-! Not provided (refused): the free surface with its refined, topography-
-! following mesh, energy_precond, moment-tensor and source-update options,
-! trace resampling.
+! free_surface runs OWL's topography-following free surface on the
+! depth-refined mesh (module_vars.f90:360-519): file_topo (two columns, x
+! and elevation), topo_interp, free_surface_dz_refine, dz_max (0: 1.5 dz)
+! and measure_source/receiver_depth_from_surface as OWL reads them.
+! Not provided (refused): energy_precond, moment-tensor and source-update
+! options, trace resampling, wavefield snapshots.
 !
 module elastic_tti_2d
 
@@ -117,14 +120,12 @@ contains
         type(owl_receiver_t), allocatable, target :: cr(:)
         real, allocatable, target :: rho(:, :), tr(:), p1(:, :), p2(:, :), th(:, :)
         type(c_ptr) :: cp(6)
+        type(owl_fsg_surface_t) :: surf
+        real(c_float), allocatable, target :: tx(:), tz(:)
         integer :: nt, nr, ir, k, pc
         integer(c_int) :: st
 
         call this%release()
-        if (this%free_surface) then
-            write (*, '(a)') ' <elastic_tti_2d> the free surface (refined topography-following mesh) is not provided'
-            error stop 1
-        end if
         if (this%data_dt /= this%dt .or. this%data_tmax /= this%tmax) then
             write (*, '(a)') ' <elastic_tti_2d> data_dt/data_tmax must equal dt/tmax (no trace resampling)'
             error stop 1
@@ -185,8 +186,33 @@ contains
         shot(1)%sources = c_loc(cs(1))
         shot(1)%nr = nr
         shot(1)%receivers = c_loc(cr(1))
-        st = owl_fsg2d_create(cfg, cp, rho, merge(0_c_int, 1_c_int, pc == 0), 1_c_int, shot, this%batch)
-        call check_status(st, 'owl_fsg2d_create')
+        if (this%free_surface) then
+            if (this%file_topo /= '') then
+                call load_topography(trim(this%file_topo), tx, tz)
+                surf%ntopo = size(tx)
+                surf%topo_x = c_loc(tx(1))
+                surf%topo_z = c_loc(tz(1))
+            end if
+            select case (trim(this%topo_interp))
+                case ('linear')
+                    surf%topo_interp = 0
+                case ('cubic')
+                    surf%topo_interp = 1
+                case default
+                    write (*, '(a)') ' <elastic_tti_2d> topo_interp '//trim(this%topo_interp)//' is not provided'
+                    error stop 1
+            end select
+            surf%dz_refine = this%free_surface_dz_refine
+            surf%dz_max = max(this%dz_max, 0.0)
+            surf%src_depth_from_surface = merge(1_c_int, 0_c_int, this%measure_source_depth_from_surface)
+            surf%rec_depth_from_surface = merge(1_c_int, 0_c_int, this%measure_receiver_depth_from_surface)
+            st = owl_fsg2d_create_surface(cfg, cp, rho, merge(0_c_int, 1_c_int, pc == 0), 1_c_int, shot, surf, &
+                this%batch)
+            call check_status(st, 'owl_fsg2d_create_surface')
+        else
+            st = owl_fsg2d_create(cfg, cp, rho, merge(0_c_int, 1_c_int, pc == 0), 1_c_int, shot, this%batch)
+            call check_status(st, 'owl_fsg2d_create')
+        end if
         nt = owl_fsg2d_nt(this%batch)
         allocate (tr(2*nt*nr))
         if (this%verbose) write (*, '(a)') ' <elastic_tti_2d> forward modelling of shot '//itoa(this%gmtr%id)
@@ -200,6 +226,31 @@ contains
             '_seismogram_vz.su', this%seis_vz)
         if (.not. this%reconstruct) call this%release()
 
+    end subroutine
+
+    ! file_topo: x and elevation per line (load(file_topo, ntp, 2, ascii))
+    subroutine load_topography(path, tx, tz)
+        character(len=*), intent(in) :: path
+        real(c_float), allocatable, intent(out) :: tx(:), tz(:)
+        integer :: u, n, k, ios
+        real :: a, b
+        open (newunit=u, file=path, status='old', action='read')
+        n = 0
+        do
+            read (u, *, iostat=ios) a, b
+            if (ios /= 0) exit
+            n = n + 1
+        end do
+        if (n < 1) then
+            write (*, '(a)') ' <elastic_tti_2d> no topography points in '//path
+            error stop 1
+        end if
+        allocate (tx(n), tz(n))
+        rewind (u)
+        do k = 1, n
+            read (u, *) tx(k), tz(k)
+        end do
+        close (u)
     end subroutine
 
     subroutine fill(g, tr, nt, nr, dt)

@@ -51,6 +51,19 @@ module owl_amd_c
         real(c_float) :: trace_discard_threshold = 1.0e-6
     end type
 
+    ! owl_fsg_surface: OWL's free surface (file_topo points, topo_interp,
+    ! free_surface_dz_refine, dz_max, depths measured from the surface)
+    type, bind(c) :: owl_fsg_surface_t
+        integer(c_int) :: ntopo = 0
+        type(c_ptr) :: topo_x = c_null_ptr
+        type(c_ptr) :: topo_z = c_null_ptr
+        integer(c_int) :: topo_interp = 1
+        real(c_float) :: dz_refine = 4.0
+        real(c_float) :: dz_max = 0.0
+        integer(c_int) :: src_depth_from_surface = 0
+        integer(c_int) :: rec_depth_from_surface = 0
+    end type
+
     type, bind(c) :: owl_window_params_t
         integer(c_int) :: nx = 1, ny = 1, nz = 1
         real(c_float) :: dx = 1, dy = 1, dz = 1
@@ -177,6 +190,19 @@ module owl_amd_c
             type(owl_receiver_t), intent(inout) :: rec(*)
             integer(c_int), intent(out) :: window(6)
             real(c_float), intent(out) :: range(6)
+            integer(c_int) :: st
+        end function
+
+        function owl_fsg2d_create_surface(cfg, cij, rho, aniso, nshot, shots, surface, out) &
+                bind(c, name='owl_fsg2d_create_surface') result(st)
+            import :: c_int, c_float, c_ptr, owl_acoustic_config_t, owl_shot_t, owl_fsg_surface_t
+            type(owl_acoustic_config_t), intent(in) :: cfg
+            type(c_ptr), intent(in) :: cij(6)
+            real(c_float), intent(in) :: rho(*)
+            integer(c_int), value :: aniso, nshot
+            type(owl_shot_t), intent(in) :: shots(*)
+            type(owl_fsg_surface_t), intent(in) :: surface
+            type(c_ptr), intent(out) :: out
             integer(c_int) :: st
         end function
 

@@ -7,7 +7,7 @@
 !                                       vp_file / rho_file  ((nz, ny, nx) float32)
 !                                       id sx sy sz f0 / nr / nr lines "rx ry rz"
 !                                       dir_synthetic / dir_adjoint / dir_working
-!   owl_solvers_3d_tti tti <job>   job: nx nz d dt tmax pml
+!   owl_solvers_3d_tti tti <job>   job: nx nz d dt tmax pml (optional last line: the free surface)
 !                                       anisotropy_type
 !                                       vp vs rho epsilon delta theta files ((nz, nx))
 !                                       id sx sz f0 / nr / nr lines "rx rz"
@@ -89,9 +89,10 @@ contains
     subroutine runtti(path)
         character(len=*), intent(in) :: path
         type(wave_solver_elastic_tti_2d) :: s
-        character(len=1024) :: f(6)
-        integer :: u, nr, ir, k
-        real :: d
+        character(len=1024) :: f(6), topo
+        character(len=2048) :: line
+        integer :: u, nr, ir, k, ios, fs1, fs2
+        real :: d, refine, dzmax
         open (newunit=u, file=path, status='old', action='read')
         read (u, *) s%nx, s%nz, d, s%dt, s%tmax, s%pml
         s%dx = d
@@ -112,6 +113,26 @@ contains
         read (u, '(a)') s%dir_synthetic
         read (u, '(a)') s%dir_adjoint
         read (u, '(a)') s%dir_working
+        ! optional: free surface  <file_topo or '-'>  <refine>  <dz_max>  <src, rec depths from surface 0|1>
+        ! (the path is the first blank-separated token; list-directed input
+        ! would stop at its slashes)
+        read (u, '(a)', iostat=ios) line
+        if (ios == 0 .and. len_trim(line) > 0) then
+            line = adjustl(line)
+            k = index(line, ' ')
+            topo = line(1:k - 1)
+            read (line(k:), *, iostat=ios) refine, dzmax, fs1, fs2
+        else
+            ios = 1
+        end if
+        if (ios == 0) then
+            s%free_surface = .true.
+            if (topo /= '-') s%file_topo = topo
+            s%free_surface_dz_refine = refine
+            s%dz_max = dzmax
+            s%measure_source_depth_from_surface = fs1 /= 0
+            s%measure_receiver_depth_from_surface = fs2 /= 0
+        end if
         close (u)
         allocate (s%vp(s%nz, s%nx), s%vs(s%nz, s%nx), s%rho(s%nz, s%nx))
         allocate (s%tieps(s%nz, s%nx), s%tidel(s%nz, s%nx), s%tithe(s%nz, s%nx))

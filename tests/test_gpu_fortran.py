@@ -197,6 +197,19 @@ def test_acoustic_iso_3d_drop_in(gpu, tmp_path):
 
 
 def test_elastic_tti_2d_drop_in_thomsen(gpu, tmp_path):
+    _tti_drop_in(tmp_path, surface=None)
+
+
+def test_elastic_tti_2d_drop_in_free_surface(gpu, tmp_path):
+    """The drop-in with free_surface, a file_topo and depths measured from
+    the surface equals the library batch with the same surface."""
+    xt = np.linspace(-300.0, 900.0, 25).astype(np.float32)
+    zt = (12.0 * np.sin(xt / 150.0) - 0.06 * xt).astype(np.float32)
+    _tti_drop_in(tmp_path, surface=dict(topo=(xt, zt), interp="cubic", refine=3.0, dz_max=14.0,
+                                        src_from_surface=True, rec_from_surface=True))
+
+
+def _tti_drop_in(tmp_path, surface):
     from owl_amd import FSG2DBatch
     from owl_amd.fsg2d import tti_cij, tti_param_gradient
     from owl_amd.geometry import Geometry, Receiver, Source
@@ -220,7 +233,10 @@ def test_elastic_tti_2d_drop_in_thomsen(gpu, tmp_path):
     geom = Geometry(id=2, srcr=[Source(x=213.0, z=181.0, f0=20.0, mechanism="force")],
                     recr=[Receiver(x=x, z=z) for x, z in rec])
     cij = tti_cij("thomsen", vp, vs, rho, eps, dlt, the)
-    b = FSG2DBatch(nx, nz, d, d, cij, rho, [(geom, None)], dt, tmax, pml=pml, fdhalf=4, aniso=True)
+    if surface is not None:
+        dt = 0.4e-3  # the refined rows near the surface
+    b = FSG2DBatch(nx, nz, d, d, cij, rho, [(geom, None)], dt, tmax, pml=pml, fdhalf=4, aniso=True,
+                   surface=surface)
     tx, tz = b.forward(save_boundary=True)[0]
     ax, az = (0.5 * tx).astype(np.float32), (0.5 * tz).astype(np.float32)
     g = b.adjoint([(ax, az)])[0]
@@ -229,11 +245,18 @@ def test_elastic_tti_2d_drop_in_thomsen(gpu, tmp_path):
     write_su(os.path.join(dirs["adj"], "shot_2_seismogram_vx.su"), ax, dt)
     write_su(os.path.join(dirs["adj"], "shot_2_seismogram_vz.su"), az, dt)
     job = str(tmp_path / "jobtti.txt")
+    extra = []
+    if surface is not None:
+        topo = str(tmp_path / "topo.txt")
+        with open(topo, "w") as f:
+            for x, z in zip(*surface["topo"]):
+                f.write(f"{float(x)!r} {float(z)!r}\n")
+        extra = [f"{topo} {surface['refine']!r} {surface['dz_max']!r} 1 1"]
     with open(job, "w") as f:
-        f.write("\n".join([f"{nx} {nz} {d} {dt} {tmax} {pml}", "thomsen"] + [files[k] for k in names]
+        f.write("\n".join([f"{nx} {nz} {d} {dt!r} {tmax} {pml}", "thomsen"] + [files[k] for k in names]
                           + ["2 213.0 181.0 20.0", f"{len(rec)}"] + [f"{x!r} {z!r}" for x, z in rec]
-                          + [dirs["syn"], dirs["adj"], dirs["work"]]) + "\n")
-    print("case: elastic_tti_2d drop-in forward + adjoint, Thomsen", flush=True)
+                          + [dirs["syn"], dirs["adj"], dirs["work"]] + extra) + "\n")
+    print(f"case: elastic_tti_2d drop-in forward + adjoint, Thomsen, surface {surface is not None}", flush=True)
     r = subprocess.run([SOLVERS, "tti", job], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     sx, _ = read_su(os.path.join(dirs["syn"], "shot_2_seismogram_vx.su"))
