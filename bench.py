@@ -456,19 +456,26 @@ def main():
     log("batched OWL-algorithm gradient")
     wowl, _, stowl, _ = time_gradient(b, 1, 1, sync, mode=OWL_GRAD_OWL)
 
-    log("torch.autograd gradient (binomial 40, FP32)")
+    log("torch.autograd gradient (binomial 40, FP32), device tensors and host tensors")
     from owl_amd.autograd import propagate
     b.set_model(vp, rho)
-    obs = torch.from_numpy(np.concatenate(b.forward(), axis=0))
-    tv = torch.tensor(vp0, requires_grad=True)
-    tr_ = torch.tensor(rho0, requires_grad=True)
-    sync()
-    t0 = time.perf_counter()
-    syn = propagate(tv, tr_, b, OWL_CKPT_BINOMIAL, SNAPSHOTS, OWL_STORE_F32)
-    loss = 0.5 * ((syn - obs) ** 2).sum()
-    loss.backward()
-    sync()
-    w_autograd = time.perf_counter() - t0
+    obs_h = torch.from_numpy(np.concatenate(b.forward(), axis=0))
+
+    def autograd_once(device):
+        obs = obs_h.to(device)
+        tv = torch.tensor(vp0, requires_grad=True, device=device)
+        tr_ = torch.tensor(rho0, requires_grad=True, device=device)
+        sync()
+        t0 = time.perf_counter()
+        syn = propagate(tv, tr_, b, OWL_CKPT_BINOMIAL, SNAPSHOTS, OWL_STORE_F32)
+        loss = 0.5 * ((syn - obs) ** 2).sum()
+        loss.backward()
+        sync()
+        return time.perf_counter() - t0
+
+    autograd_once("cuda")  # untimed warm-up
+    w_autograd = autograd_once("cuda")
+    w_autograd_host = autograd_once("cpu")
     b.close()
 
     # ---- plain per-shot: one shot per launch, full FP32 storage
@@ -556,6 +563,7 @@ def main():
         "owl_algorithm": {"batched_s_per_shot": wowl / nshot, "plain_s_per_shot": plain_owl_s,
                           "speedup": plain_owl_s / (wowl / nshot)},
         "autograd_s_per_shot": w_autograd / nshot,
+        "autograd_host_tensors_s_per_shot": w_autograd_host / nshot,
         "roofline": with_traffic("acoustic2d_gradient", roof,
                                  rows[0] if roof["kernel"] == "step_fused" else rows[1]),
         "kernels": kernel_table(stats, 1),

@@ -237,6 +237,22 @@ owl_status owl_acoustic2d_forward_store(owl_acoustic2d *b, float *traces, int ck
                                         int store);
 owl_status owl_acoustic2d_backward(owl_acoustic2d *b, const float *dtraces, double *grad_vp,
                                    double *grad_rho);
+/* The differentiable path with device-resident tensors (no OWL
+ * counterpart; for torch.autograd): the same calls with every array in
+ * device memory of the batch's GPU, so nothing is staged through the host.
+ * The caller completes its writes of vp, rho and dtraces before the call;
+ * each call returns after its work is complete.  set_model_device replaces
+ * the model as owl_acoustic2d_set_model does (vp, rho: (nz, nx) z fastest,
+ * or x fastest when x_fastest != 0), with the CFL check and the PML
+ * profiles of the new edge velocities, without planning the batch again;
+ * explosion sources only.  forward_store_device writes the traces in the
+ * trace layout; backward_device takes dL/dtraces in that layout and writes
+ * dL/dvp, dL/drho as float (nz, nx) (z fastest, or x fastest). */
+owl_status owl_acoustic2d_set_model_device(owl_acoustic2d *b, const float *vp, const float *rho, int x_fastest);
+owl_status owl_acoustic2d_forward_store_device(owl_acoustic2d *b, float *traces, int ckpt, int ckpt_param,
+                                               int store);
+owl_status owl_acoustic2d_backward_device(owl_acoustic2d *b, const float *dtraces, float *grad_vp,
+                                          float *grad_rho, int x_fastest);
 /* Identifier of the forward states that the batch holds for
  * owl_acoustic2d_backward: a new positive value after every
  * owl_acoustic2d_forward_store, 0 when none are held (after set_model, a

@@ -150,6 +150,9 @@ _SIGNATURES = {
     "owl_fsg2d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_fsg2d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_fsg2d_last_stats": (C.c_int, [_P, _P, _P]),
+    "owl_acoustic2d_set_model_device": (C.c_int, [_P, _P, _P, C.c_int]),
+    "owl_acoustic2d_forward_store_device": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int]),
+    "owl_acoustic2d_backward_device": (C.c_int, [_P, _P, _P, _P, C.c_int]),
     "owl_fsg2d_create_surface": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, _P, _P]),
     "owl_fsg2d_surface_mesh": (C.c_int, [_P, _P, _P, C.c_float, _P, C.c_int, _P, _P] + [_P] * 9),
     "owl_fsg2d_rows": (C.c_int, [_P, C.c_int]),

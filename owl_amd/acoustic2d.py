@@ -211,6 +211,24 @@ class Acoustic2DBatch:
         check(lib().owl_acoustic2d_forward_store(self._h, _ptr(flat), ckpt, ckpt_param, store))
         return self._split(flat)
 
+    def set_model_device(self, vp_ptr, rho_ptr, x_fastest=True):
+        """Replace the model from device memory (pointers to float32 (nz, nx)
+        arrays on the batch's GPU, x fastest by default, as a row-major
+        torch tensor is)."""
+        check(lib().owl_acoustic2d_set_model_device(self._h, C.c_void_p(vp_ptr), C.c_void_p(rho_ptr),
+                                                    int(bool(x_fastest))))
+
+    def forward_store_device(self, traces_ptr, ckpt=_lib.OWL_CKPT_FULL, ckpt_param=0, store=_lib.OWL_STORE_F32):
+        """forward_store with the traces written to device memory: a float32
+        (sum of receivers, nt) buffer on the batch's GPU."""
+        check(lib().owl_acoustic2d_forward_store_device(self._h, C.c_void_p(traces_ptr), ckpt, ckpt_param, store))
+
+    def backward_device(self, dtraces_ptr, grad_vp_ptr, grad_rho_ptr, x_fastest=True):
+        """backward with device buffers: dL/dtraces (sum of receivers, nt) in,
+        float32 (nz, nx) dL/dvp, dL/drho out (x fastest by default)."""
+        check(lib().owl_acoustic2d_backward_device(self._h, C.c_void_p(dtraces_ptr), C.c_void_p(grad_vp_ptr),
+                                                   C.c_void_p(grad_rho_ptr), int(bool(x_fastest))))
+
     def store_id(self):
         """Identifier of the forward states held for backward (0: none)."""
         v = C.c_longlong()

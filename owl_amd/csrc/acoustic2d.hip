@@ -1266,6 +1266,78 @@ struct ShotHost {
     float f0max = 0;
 };
 
+// ---------------------------------------------------------------------------
+// Differentiable path with device-resident tensors: the model from device
+// memory and the trace/gradient layout conversions, so nothing crosses PCIe.
+
+// Padded bk = rho vp^2 and rho of each shot's model copy, and rho vp^3, vp,
+// rho on the window (imaging arrays), from the full model in device memory
+// (element (ix, kz) at kz + ix nz, or at ix + kz nx when xf); per shot the
+// largest vp, the largest vp on the window's edge lines and the count of
+// non-positive vp or rho (stats[3 s .. 3 s + 2], positive floats compared
+// as their bit patterns).  One block row per shot; shots that share a
+// window write the same values into the shared copy.
+__global__ void k_model_from_device(const Shot* __restrict__ shots, const float* __restrict__ vp,
+                                    const float* __restrict__ rho, int nxf, int nzf, int xf, int pml, int R,
+                                    float* bk, float* rh, float* den, float* vpw, float* rhow, unsigned* stats) {
+    const Shot S = shots[blockIdx.y];
+    const int lo = 1 - pml - R, nxp = S.nx + 2 * (pml + R), nzp = S.nz + 2 * (pml + R);
+    const long long n = (long long)nxp * nzp;
+    unsigned vmax = 0, vedge = 0, bad = 0;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = lo + int(e % nxp), j = lo + int(e / nxp);
+        const int ic = min(max(i, 1), S.nx), jc = min(max(j, 1), S.nz);
+        const int ix = S.nxbeg - 1 + ic - 1, kz = S.nzbeg - 1 + jc - 1;
+        const size_t m = xf ? size_t(ix) + size_t(kz) * size_t(nxf) : size_t(kz) + size_t(ix) * size_t(nzf);
+        const float v = vp[m], r = rho[m];
+        const size_t q = size_t((long long)S.off + S.mdel + (i - lo) + (long long)(j - lo) * S.pitch);
+        rh[q] = r;
+        bk[q] = r * (v * v);
+        if (i >= 1 && i <= S.nx && j >= 1 && j <= S.nz) {
+            const size_t g = size_t(S.g0) + size_t(i - 1) + size_t(j - 1) * S.nx;
+            den[g] = r * (v * v * v);
+            vpw[g] = v;
+            rhow[g] = r;
+            if (!(v > 0.0f) || !(r > 0.0f)) {
+                ++bad;
+            } else {
+                vmax = max(vmax, __float_as_uint(v));
+                if (i == 1 || i == S.nx || j == 1 || j == S.nz) vedge = max(vedge, __float_as_uint(v));
+            }
+        }
+    }
+    atomicMax(&stats[3 * blockIdx.y], vmax);
+    atomicMax(&stats[3 * blockIdx.y + 1], vedge);
+    if (bad) atomicAdd(&stats[3 * blockIdx.y + 2], bad);
+}
+
+// Traces between the internal [t][r] blocks and the C ABI's [r][t] blocks.
+__global__ void k_traces_layout(const Shot* __restrict__ shots, const float* __restrict__ src, float* dst, int nt,
+                                int to_abi) {
+    const Shot S = shots[blockIdx.y];
+    const long long n = (long long)S.nr * nt;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int r = int(e / nt), t = int(e % nt);
+        const size_t a = size_t(S.trace0) + size_t(r) * nt + t, in = size_t(S.trace0) + size_t(t) * S.nr + r;
+        if (to_abi)
+            dst[a] = src[in];
+        else
+            dst[in] = src[a];
+    }
+}
+
+// The FP64 model gradient (z fastest) as float, z fastest or x fastest.
+__global__ void k_grad_to_float(const double* __restrict__ g, float* out, int nxf, int nzf, int xf) {
+    const long long n = (long long)nxf * nzf;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int kz = int(e % nzf), ix = int(e / nzf);
+        out[xf ? size_t(ix) + size_t(kz) * nxf : size_t(e)] = float(g[e]);
+    }
+}
+
 }  // namespace a2
 }  // namespace owl
 
@@ -1311,6 +1383,8 @@ struct owl_acoustic2d {
     DevBuf<Shot> d_shots;
     DevBuf<int> d_tile_shot;
     DevBuf<float> d_bk, d_rho, d_den, d_prof;
+    std::vector<float> h_prof;  // host copy of the PML profiles (rebuilt when a model update moves pmlvp)
+    DevBuf<unsigned> d_mstat;
     DevBuf<float> fA[3], fB[3], fM[4];  // forward field (ping-pong) and PML memory
     DevBuf<float> aF[3], aM[4];         // adjoint field and PML memory
     DevBuf<float> d_amp;                // source amplitude rows [row][nt]
@@ -1742,6 +1816,7 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     b->d_ftile_shot.upload(b->ftile_shot, st);
     b->d_atile_shot.upload(b->atile_shot, st);
     b->d_prof.upload(profs, st);
+    b->h_prof = profs;
     if (amp.empty()) amp.push_back(0.f);
     b->d_amp.upload(amp, st);
     b->d_inj_beg.upload(inj_beg, st);
@@ -2895,6 +2970,71 @@ void traces_to_dev(const owl_acoustic2d* b, const float* in, std::vector<float>&
     }
 }
 
+// The model from device memory (owl_acoustic2d_set_model_device): refill
+// the padded moduli and the imaging arrays on the device, check positivity
+// and the CFL bound per shot as plan() does, rebuild the PML profiles of the
+// shots whose edge velocity moved, keep the host copy of the model current,
+// and drop every state that belonged to the old model.
+void set_model_device(owl_acoustic2d* b, const float* vp, const float* rho, int xf) {
+    hipStream_t st = b->stream;
+    const unsigned nshot = unsigned(b->sd.size());
+    b->d_mstat.alloc(3 * size_t(nshot));
+    b->d_mstat.zero(st);
+    long long mx = 1;
+    for (const Shot& S : b->sd)
+        mx = std::max(mx, (long long)(S.nx + 2 * (b->pml + b->R)) * (S.nz + 2 * (b->pml + b->R)));
+    const dim3 grd(unsigned(std::min<long long>(1024, (mx + 255) / 256)), nshot);
+    hipLaunchKernelGGL(k_model_from_device, grd, dim3(256), 0, st, b->d_shots.get(), vp, rho, b->cfg.nx, b->cfg.nz,
+                       xf, b->pml, b->R, b->d_bk.get(), b->d_rho.get(), b->d_den.get(), b->d_vpw.get(),
+                       b->d_rhow.get(), b->d_mstat.get());
+    OWL_HIP(hipGetLastError());
+    std::vector<unsigned> stat(3 * size_t(nshot));
+    OWL_HIP(hipMemcpyAsync(stat.data(), b->d_mstat.get(), stat.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    // the host copy of the model follows the device one
+    const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
+    std::vector<float> hv(nfull), hr(nfull);
+    OWL_HIP(hipMemcpyAsync(hv.data(), vp, nfull * sizeof(float), hipMemcpyDeviceToHost, st));
+    OWL_HIP(hipMemcpyAsync(hr.data(), rho, nfull * sizeof(float), hipMemcpyDeviceToHost, st));
+    OWL_HIP(hipStreamSynchronize(st));
+    b->have_boundary = false;
+    b->fwd_state_valid = false;
+    b->have_store = false;
+    b->store_id = 0;
+    for (size_t e = 0; e < nfull; ++e) {
+        const size_t ix = e / size_t(b->cfg.nz), kz = e % size_t(b->cfg.nz);
+        const size_t m = xf ? ix + kz * size_t(b->cfg.nx) : e;
+        b->vp_full[e] = hv[m];
+        b->rho_full[e] = hr[m];
+    }
+    bool moved = false;
+    for (size_t s = 0; s < nshot; ++s) {
+        if (stat[3 * s + 2]) fail(OWL_ERR_ARG, "vp and rho must be positive");
+        float vmax, vedge;
+        std::memcpy(&vmax, &stat[3 * s], sizeof(float));
+        std::memcpy(&vedge, &stat[3 * s + 1], sizeof(float));
+        const float dts = dt_stable(b->R, 2, vmax, b->cfg.dx, 1.0f, b->cfg.dz);
+        if (b->cfg.dt > dts)
+            fail(OWL_ERR_CFL, "dt = " + std::to_string(b->cfg.dt) + " exceeds the stable dt = " + std::to_string(dts));
+        ShotHost& H = b->hs[s];
+        if (vedge != H.pmlvp) {
+            H.pmlvp = vedge;
+            const Shot& S = b->sd[s];
+            const PmlAxis px = cfspml_axis(H.nx, b->pml, b->cfg.dx, b->cfg.dt, vedge, H.f0max);
+            const PmlAxis pz = cfspml_axis(H.nz, b->pml, b->cfg.dz, b->cfg.dt, vedge, H.f0max);
+            size_t o = size_t(S.prof);
+            for (const auto* v : {&px.ai, &px.bi, &px.ah, &px.bh, &pz.ai, &pz.bi, &pz.ah, &pz.bh}) {
+                std::copy(v->begin(), v->end(), b->h_prof.begin() + o);
+                o += v->size();
+            }
+            moved = true;
+        }
+    }
+    if (moved) {
+        b->d_prof.upload(b->h_prof, st);
+        OWL_HIP(hipStreamSynchronize(st));
+    }
+}
+
 void download_traces(owl_acoustic2d* b, float* out) {
     std::vector<float> h(b->trace_elems);
     OWL_HIP(hipMemcpyAsync(h.data(), b->d_traces.get(), h.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream));
@@ -3198,6 +3338,68 @@ owl_status owl_acoustic2d_backward(owl_acoustic2d* b, const float* dtraces, doub
         const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
         if (grad_vp) OWL_HIP(hipMemcpy(grad_vp, b->d_gfull_v.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
         if (grad_rho) OWL_HIP(hipMemcpy(grad_rho, b->d_gfull_r.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+owl_status owl_acoustic2d_set_model_device(owl_acoustic2d* b, const float* vp, const float* rho, int x_fastest) {
+    return guarded([&] {
+        if (!b || !vp || !rho) fail(OWL_ERR_ARG, "null argument");
+        for (auto& H : b->hs)
+            for (auto& sc : H.src)
+                if (sc.mechanism != OWL_EXPLOSION)
+                    fail(OWL_ERR_ARG, "device model updates cover explosion sources only (a force's amplitude "
+                                      "depends on rho)");
+        set_model_device(b, vp, rho, x_fastest != 0);
+    });
+}
+
+owl_status owl_acoustic2d_forward_store_device(owl_acoustic2d* b, float* traces, int ckpt, int ckpt_param,
+                                               int store) {
+    return guarded([&] {
+        if (!b || !traces) fail(OWL_ERR_ARG, "null argument");
+        if (b->cfg.free_surface) fail(OWL_ERR_ARG, "the exact adjoint does not cover the free surface");
+        for (auto& H : b->hs)
+            for (auto& sc : H.src)
+                if (sc.mechanism != OWL_EXPLOSION) fail(OWL_ERR_ARG, "the exact adjoint covers explosion sources only");
+        dispatch_R(b->R, [&](auto r) {
+            run_exact<decltype(r)::value>(b, ckpt, ckpt_param, store, [] {}, kForwardOnly);
+        });
+        int mnr = 1;
+        for (const Shot& S : b->sd) mnr = std::max(mnr, S.nr);
+        const unsigned g = unsigned(std::min<long long>(256, (mnr * (long long)b->nt + 255) / 256));
+        hipLaunchKernelGGL(k_traces_layout, dim3(std::max(g, 1u), unsigned(b->sd.size())), dim3(256), 0, b->stream,
+                           b->d_shots.get(), b->d_traces.get(), traces, b->nt, 1);
+        OWL_HIP(hipGetLastError());
+        OWL_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+owl_status owl_acoustic2d_backward_device(owl_acoustic2d* b, const float* dtraces, float* grad_vp, float* grad_rho,
+                                          int x_fastest) {
+    return guarded([&] {
+        if (!b || !dtraces) fail(OWL_ERR_ARG, "null argument");
+        if (!b->have_store) fail(OWL_ERR_STATE, "backward needs a preceding owl_acoustic2d_forward_store");
+        hipStream_t st = b->stream;
+        b->d_adj.alloc(std::max<size_t>(b->trace_elems, 1));
+        int mnr = 1;
+        for (const Shot& S : b->sd) mnr = std::max(mnr, S.nr);
+        const unsigned g = unsigned(std::min<long long>(256, (mnr * (long long)b->nt + 255) / 256));
+        hipLaunchKernelGGL(k_traces_layout, dim3(std::max(g, 1u), unsigned(b->sd.size())), dim3(256), 0, st,
+                           b->d_shots.get(), dtraces, b->d_adj.get(), b->nt, 0);
+        dispatch_R(b->R, [&](auto r) {
+            run_exact<decltype(r)::value>(b, b->store_policy, b->store_param, b->store_fmt, [] {}, kReverseOnly);
+        });
+        merge_full(b);
+        const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
+        const unsigned mg = unsigned(std::min<size_t>(4096, (nfull + 255) / 256));
+        if (grad_vp)
+            hipLaunchKernelGGL(k_grad_to_float, dim3(mg), dim3(256), 0, st, b->d_gfull_v.get(), grad_vp, b->cfg.nx,
+                               b->cfg.nz, x_fastest != 0 ? 1 : 0);
+        if (grad_rho)
+            hipLaunchKernelGGL(k_grad_to_float, dim3(mg), dim3(256), 0, st, b->d_gfull_r.get(), grad_rho, b->cfg.nx,
+                               b->cfg.nz, x_fastest != 0 ? 1 : 0);
+        OWL_HIP(hipGetLastError());
+        OWL_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -267,3 +267,44 @@ def test_autotuned_rows_give_the_same_gradient(gpu):
     # gradient depends on the rows at FP64 rounding only
     assert np.array_equal(gv0, gv1) and rel_l2(gr1, gr0) <= 1e-12, rel_l2(gr1, gr0)
     b.close()
+
+
+def test_device_path_equals_host_path(gpu):
+    """The device entry points (model, traces and gradients in device memory)
+    give the host path's traces and float gradients bit for bit, for both
+    model layouts, and a second model whose edge velocities move the PML
+    profiles equals a batch planned on that model."""
+    import torch
+    vp, rho, geoms = _case(8)
+    vp2 = (vp * 1.07).astype(np.float32)  # moves every window's edge velocity
+    dtr = None
+    for k, (v, r) in enumerate(((vp, rho), (vp2, rho))):
+        print(f"case: device vs host differentiable path, model {k}", flush=True)
+        host = _batch(v, r, geoms)
+        dev = _batch(vp, rho, geoms)  # planned on the first model, updated on the device
+        tr_h = np.concatenate(host.forward_store(OWL_CKPT_BINOMIAL, 5, OWL_STORE_F32), 0)
+        if dtr is None:
+            dtr = np.random.default_rng(3).standard_normal(tr_h.shape).astype(np.float32)
+        gv_h, gr_h = host.backward(dtr)
+        for xf in (True, False):
+            tv = torch.tensor(v if xf else np.ascontiguousarray(v.T), device="cuda:0")
+            tr_ = torch.tensor(r if xf else np.ascontiguousarray(r.T), device="cuda:0")
+            out = torch.empty(tr_h.shape, dtype=torch.float32, device="cuda:0")
+            torch.cuda.synchronize()
+            dev.set_model_device(tv.data_ptr(), tr_.data_ptr(), x_fastest=xf)
+            dev.forward_store_device(out.data_ptr(), OWL_CKPT_BINOMIAL, 5, OWL_STORE_F32)
+            assert np.array_equal(out.cpu().numpy(), tr_h)
+            g = torch.tensor(dtr, device="cuda:0")
+            shape = (NZ, NX) if xf else (NX, NZ)
+            gv = torch.empty(shape, dtype=torch.float32, device="cuda:0")
+            gr = torch.empty(shape, dtype=torch.float32, device="cuda:0")
+            torch.cuda.synchronize()
+            dev.backward_device(g.data_ptr(), gv.data_ptr(), gr.data_ptr(), x_fastest=xf)
+            gvn, grn = gv.cpu().numpy(), gr.cpu().numpy()
+            if not xf:
+                gvn, grn = gvn.T, grn.T
+            assert np.abs(gv_h).max() > 0
+            assert np.array_equal(gvn, gv_h.astype(np.float32))
+            assert np.array_equal(grn, gr_h.astype(np.float32))
+        host.close()
+        dev.close()
