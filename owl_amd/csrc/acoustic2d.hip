@@ -1431,6 +1431,9 @@ struct owl_acoustic2d {
     // with the same schedule and buffers, replayed afterwards
     int use_graphs = 1;
     bool capturing = false;
+    // the reverse call of a split binomial gradient replays the schedule's
+    // first descent on the host only: its results are still in the buffers
+    bool skip_gpu = false;
     hipGraphExec_t gexec = nullptr;
     std::vector<uintptr_t> gsig, gready;
 
@@ -1942,6 +1945,8 @@ void launch_step(owl_acoustic2d* b, const Sets& sets, int nsets, float dt) {
 template <int R>
 void run_forward(owl_acoustic2d* b, bool save) {
     hipStream_t st = b->stream;
+    b->have_store = false;  // the working sets below hold a stored exact-adjoint state
+    b->store_id = 0;
     for (int k = 0; k < 3; ++k) b->fA[k].zero(st);
     for (int k = 0; k < 4; ++k) b->fM[k].zero(st);
     b->d_traces.zero(st);
@@ -1982,6 +1987,8 @@ void run_forward(owl_acoustic2d* b, bool save) {
 template <int R>
 void run_reverse_owl(owl_acoustic2d* b) {
     hipStream_t st = b->stream;
+    b->have_store = false;
+    b->store_id = 0;
     if (!b->have_boundary || !b->fwd_state_valid)
         fail(OWL_ERR_STATE, "the adjoint needs a preceding forward run with boundary saving");
     for (int k = 0; k < 3; ++k) {
@@ -2133,6 +2140,7 @@ bool use_fused(const owl_acoustic2d* b, const W7& in, const W7& out) {
 // recording).
 template <int R>
 void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record) {
+    if (b->skip_gpu) return;
     Sets sets{};
     FieldSet& F = sets.s[0];
     F.p_in = in.f[0];
@@ -2169,7 +2177,7 @@ void fwd_step(owl_acoustic2d* b, const W7& in, const W7& out, int t, bool record
 // Clear device memory on the batch stream: a memset, or a fill kernel while
 // a graph is being captured (memset nodes are kept out of graphs).
 void dzero(owl_acoustic2d* b, void* p, size_t bytes) {
-    if (!p || !bytes) return;
+    if (!p || !bytes || b->skip_gpu) return;
     if (b->capturing) {
         const size_t n = bytes / 4;
         const unsigned g = unsigned(std::min<size_t>(4096, (n + 255) / 256));
@@ -2210,6 +2218,7 @@ void slots_alloc(owl_acoustic2d* b, owl_acoustic2d::Slots& S, int nslot, int nfi
 }
 
 void slot_save(owl_acoustic2d* b, owl_acoustic2d::Slots& S, int slot, float* const* fields) {
+    if (b->skip_gpu) return;
     const size_t fe = b->field_elems;
     const unsigned nshot = unsigned(b->sd.size());
     const int ev = prof_begin(b, 12);
@@ -2230,6 +2239,7 @@ void slot_save(owl_acoustic2d* b, owl_acoustic2d::Slots& S, int slot, float* con
 }
 
 void slot_load(owl_acoustic2d* b, owl_acoustic2d::Slots& S, int slot, float* const* fields) {
+    if (b->skip_gpu) return;
     const size_t fe = b->field_elems;
     const unsigned nshot = unsigned(b->sd.size());
     const int ev = prof_begin(b, 12);
@@ -2524,8 +2534,9 @@ void rev(owl_acoustic2d* bt, int a, int b, int sfree, int slot, int loc, RevCtx&
             prev = w;
             const W7 k = carry_set(bt, (t - 1) & 1);
             for (int i = 0; i < 3; ++i) {
-                OWL_HIP(hipMemcpyAsync(k.f[i], w.f[i], bt->field_elems * sizeof(float), hipMemcpyDeviceToDevice,
-                                       bt->stream));
+                if (!bt->skip_gpu)
+                    OWL_HIP(hipMemcpyAsync(k.f[i], w.f[i], bt->field_elems * sizeof(float),
+                                           hipMemcpyDeviceToDevice, bt->stream));
                 prev.f[i] = k.f[i];
             }
         }
@@ -2586,8 +2597,10 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
         dzero(b, b->d_traces.get(), b->d_traces.size() * sizeof(float));
     }
     // w3 is a step target: its halo must read as zero, its memory variables
-    // start from zero wherever a schedule first writes them
-    zero_set(b, work3(b));
+    // start from zero wherever a schedule first writes them (a resumed
+    // binomial schedule finds it as its forward call left it)
+    const bool resume = phase == kReverseOnly && ckpt == OWL_CKPT_BINOMIAL;
+    if (!resume) zero_set(b, work3(b));
     double fwd_steps = 0, adj_steps = 0;
     if (ckpt == OWL_CKPT_FULL) {
         if (do_fwd) {
@@ -2688,15 +2701,8 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
         adj_steps += N;
     } else if (ckpt == OWL_CKPT_BINOMIAL) {
         const int S = std::max(0, std::min(param, N));
-        if (phase == kForwardOnly) {
-            // the schedule starts from u_0 in the reverse call; here only the
-            // traces are modelled
-            RevCtx plain;
-            advance<R>(b, 0, N, kW1, -1, true, plain);
-            return;
-        }
         RevCtx ctx;
-        ctx.first = do_fwd;
+        ctx.first = true;
         ctx.direct = fmt == OWL_STORE_F32;
         ctx.carry = ctx.direct;
         binomial_table(b, N, S, ctx.carry);
@@ -2710,20 +2716,40 @@ void exact_body(owl_acoustic2d* b, int ckpt, int param, int fmt, const std::func
         slots_alloc(b, b->snaps, nphys, 7, fmt);
         // steps write FP32 snapshots and the carried sets over the update
         // domain only: their outer halo must read as zero
-        if (ctx.direct) dzero(b, b->snaps.f32.get(), size_t(nphys) * 7 * b->field_elems * sizeof(float));
-        if (ctx.carry)
-            for (auto& k : b->fK) dzero(b, k.get(), b->field_elems * sizeof(float));
+        if (do_fwd) {
+            if (ctx.direct) dzero(b, b->snaps.f32.get(), size_t(nphys) * 7 * b->field_elems * sizeof(float));
+            if (ctx.carry)
+                for (auto& k : b->fK) dzero(b, k.get(), b->field_elems * sizeof(float));
+        }
+        // The schedule's first descent models the traces and records the
+        // checkpoints.  A forward-only call stops where the first adjoint
+        // step would start; the reverse-only call replays the same descent
+        // on the host without GPU work (its checkpoints, carried states and
+        // working sets are still in the buffers) and goes on from there, so
+        // the split call does the work of the joint one.
+        struct StopAtFirstAdjoint {};
+        if (phase == kForwardOnly) {
+            ctx.on_first_adjoint = [] { throw StopAtFirstAdjoint{}; };
+            try {
+                rev<R>(b, 0, N, S, kZero, kW1, ctx);
+            } catch (const StopAtFirstAdjoint&) {
+            }
+            return;
+        }
         ctx.on_first_adjoint = [&] {
+            b->skip_gpu = false;
             residual();
             adjoint_alloc_zero(b);
             inject_residual(b, b->nt);
         };
-        if (!do_fwd) {
-            zero_set(b, w1);
-            zero_set(b, w2);   // out-of-place target: its halo must read as zero
-            ctx.on_first_adjoint();
+        b->skip_gpu = !do_fwd;
+        try {
+            rev<R>(b, 0, N, S, kZero, kW1, ctx);
+        } catch (...) {
+            b->skip_gpu = false;
+            throw;
         }
-        rev<R>(b, 0, N, S, kZero, kW1, ctx);
+        b->skip_gpu = false;
         adj_steps += N;
         fwd_steps += 0;  // counted by the schedule; reported through the updates figure below
     } else {

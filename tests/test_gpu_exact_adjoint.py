@@ -308,3 +308,25 @@ def test_device_path_equals_host_path(gpu):
             assert np.array_equal(grn, gr_h.astype(np.float32))
         host.close()
         dev.close()
+
+
+@pytest.mark.parametrize("ckpt,param,store", [(OWL_CKPT_BINOMIAL, 5, OWL_STORE_F32),
+                                              (OWL_CKPT_BINOMIAL, 7, OWL_STORE_F16),
+                                              (OWL_CKPT_UNIFORM, 9, OWL_STORE_F32)])
+def test_split_calls_equal_the_joint_gradient(oracle32, gpu, ckpt, param, store):
+    """forward_store then backward(syn - obs) gives the joint exact gradient
+    call's vp and rho gradients bit for bit: the split binomial schedule
+    resumes where its forward call stopped instead of starting again."""
+    vp, rho, geoms = _case(9)
+    b = _batch(vp, rho, geoms)
+    obs = _obs(oracle32, vp, rho, geoms, b.nt)
+    print(f"case: split vs joint exact gradient, ckpt {ckpt} param {param} store {store}", flush=True)
+    _, gv0, gr0, syn0 = b.gradient(obs, mode=OWL_GRAD_EXACT, ckpt=ckpt, ckpt_param=param, store=store)
+    syn = b.forward_store(ckpt, param, store)
+    for x, y in zip(syn, syn0):
+        assert np.array_equal(x, y)
+    gv, gr = b.backward([s - o for s, o in zip(syn, obs)])
+    assert np.abs(gv0).max() > 0
+    assert np.array_equal(gv, gv0)
+    assert np.array_equal(gr, gr0)
+    b.close()
