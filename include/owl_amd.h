@@ -371,6 +371,13 @@ owl_status owl_acoustic3d_gradient(owl_acoustic3d *b, const float *obs, double *
  * XCD-contiguous order; 9, 10, 11: software-pipelined columns with 4, 8, 16
  * rows per block, 13-15 the same with the XCD-contiguous order). */
 owl_status owl_acoustic3d_set_launch(owl_acoustic3d *b, int planes, int variant);
+/* HIP graph of the forward time loop (no OWL counterpart): with on != 0
+ * (default), forward modelling without boundary records and with profiling
+ * off runs the whole time loop as one captured graph, captured by the first
+ * such call and replayed while the launch shape and buffers stay the same;
+ * results are bit-identical to eager launches.  on = 0 drops the graph and
+ * launches eagerly. */
+owl_status owl_acoustic3d_set_graphs(owl_acoustic3d *b, int on);
 /* Launch-shape autotuner (no OWL counterpart): times `steps` sweeps of the
  * batch under each candidate (variant, planes) and keeps the fastest;
  * writes the chosen shape (either pointer may be NULL).  Leaves the wave

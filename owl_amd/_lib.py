@@ -123,6 +123,7 @@ _SIGNATURES = {
     "owl_acoustic3d_adjoint": (C.c_int, [_P, _P, _P, _P]),
     "owl_acoustic3d_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "owl_acoustic3d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_acoustic3d_set_graphs": (C.c_int, [_P, C.c_int]),
     "owl_acoustic3d_autotune": (C.c_int, [_P, C.c_int, _P, _P]),
     "owl_acoustic3d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_acoustic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),

@@ -141,6 +141,11 @@ class Acoustic3DBatch:
     def set_profiling(self, on=True):
         check(lib().owl_acoustic3d_set_profiling(self._h, int(bool(on))))
 
+    def set_graphs(self, on=True):
+        """Forward modelling through one captured HIP graph of the time loop
+        (default on; bit-identical to eager launches)."""
+        check(lib().owl_acoustic3d_set_graphs(self._h, int(bool(on))))
+
     def kernel_stats(self):
         out = {}
         for k, name in enumerate(self.KERNELS):

@@ -1571,7 +1571,15 @@ struct owl_acoustic3d {
     DevBuf<double> d_gv, d_gr, d_gfull_v, d_gfull_r, d_misfit, d_scratch;
     DevBuf<Tap3> d_inj, d_rec, d_adj_taps;
     bool have_boundary = false, fwd_state_valid = false, have_obs = false;
+    // HIP graph of the forward time loop (modelling without boundary
+    // records, profiling off): captured by the first such call, replayed by
+    // the next ones while the launch shape and buffers stay the same
+    int use_graphs = 1;
+    bool capturing = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gsig;
     ~owl_acoustic3d() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -2068,6 +2076,25 @@ void maybe_collect(owl_acoustic3d* b) {
 
 void strips_launch(owl_acoustic3d* b, int R, float* vx, float* vy, float* vz, int t, int save);
 
+// Every launch parameter and device pointer the forward loop bakes into its
+// launches: a graph captured under one signature is replayed only under the
+// same one.
+std::vector<uintptr_t> forward_signature(owl_acoustic3d* b) {
+    std::vector<uintptr_t> v{uintptr_t(b->nt), uintptr_t(b->R), uintptr_t(b->fused), uintptr_t(b->fkc),
+                             uintptr_t(b->kc), uintptr_t(b->variant), uintptr_t(b->cy), uintptr_t(b->remap),
+                             uintptr_t(b->tile_shot.size()), uintptr_t(b->ftile_shot.size()),
+                             uintptr_t(b->sd.size())};
+    auto add = [&](const void* p) { v.push_back(reinterpret_cast<uintptr_t>(p)); };
+    for (auto& x : b->f) add(x.get());
+    for (auto& x : b->m) add(x.get());
+    for (auto& x : b->fB) add(x.get());
+    for (auto& x : b->mB) add(x.get());
+    for (auto* x : {&b->d_traces, &b->d_bk, &b->d_rho, &b->d_prof, &b->d_amp, &b->d_rec_w}) add(x->get());
+    add(b->d_shots.get()); add(b->d_tile_shot.get()); add(b->d_ftile_shot.get());
+    add(b->d_inj_beg.get()); add(b->d_inj.get()); add(b->d_rec_beg.get()); add(b->d_rec.get());
+    return v;
+}
+
 template <int R>
 void run_forward3(owl_acoustic3d* b, bool save_boundary) {
     hipStream_t st = b->stream;
@@ -2094,29 +2121,68 @@ void run_forward3(owl_acoustic3d* b, bool save_boundary) {
         for (auto& x : b->fB) x.zero(st);
         for (auto& x : b->mB) x.zero(st);
     }
-    OWL_HIP(hipEventRecord(b->ev0, st));
     // step order (compute_forward): save strips(t) -> record -> update -> source;
     // the record of step 1 is the zero initial state.  The state at the start
     // of step nt is the final state the reverse pass starts from.
-    for (int t = 1; t <= b->nt - 1; ++t) {
-        if (save_boundary) strips_launch(b, R, cur.f[1], cur.f[2], cur.f[3], t, 1);
-        if (fused) {
-            fused_step3<R>(b, cur, nxt, b->dt);
-            std::swap(cur, nxt);
-        } else {
-            sweeps3<R>(b, F, b->dt);
+    auto loop = [&] {
+        for (int t = 1; t <= b->nt - 1; ++t) {
+            if (save_boundary) strips_launch(b, R, cur.f[1], cur.f[2], cur.f[3], t, 1);
+            if (fused) {
+                fused_step3<R>(b, cur, nxt, b->dt);
+                std::swap(cur, nxt);
+            } else {
+                sweeps3<R>(b, F, b->dt);
+            }
+            const int ev = b->prof.begin(st, 2, 0);
+            hipLaunchKernelGGL(k3_post_step, dim3(unsigned(b->sd.size())), dim3(1024), 0, st, b->d_shots.get(),
+                               cur.f[0], cur.f[1], cur.f[2], cur.f[3], b->d_inj_beg.get(),
+                               b->d_inj.get(), b->d_amp.get(), b->nt, t, b->d_rec_beg.get(), b->d_rec.get(),
+                               b->d_rec_w.get(), b->d_traces.get());
+            b->prof.end(st, ev);
+            if (!b->capturing) maybe_collect(b);
         }
-        const int ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(k3_post_step, dim3(unsigned(b->sd.size())), dim3(1024), 0, st, b->d_shots.get(),
-                           cur.f[0], cur.f[1], cur.f[2], cur.f[3], b->d_inj_beg.get(),
-                           b->d_inj.get(), b->d_amp.get(), b->nt, t, b->d_rec_beg.get(), b->d_rec.get(),
-                           b->d_rec_w.get(), b->d_traces.get());
-        b->prof.end(st, ev);
-        maybe_collect(b);
+    };
+    const bool graph_ok = b->use_graphs && !b->prof.on() && !save_boundary;
+    std::vector<uintptr_t> sig;
+    if (graph_ok) sig = forward_signature(b);
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    if (graph_ok && b->gexec && sig == b->gsig) {
+        OWL_HIP(hipGraphLaunch(b->gexec, st));
+    } else if (graph_ok) {
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            b->gsig.clear();
+        }
+        hipGraph_t g = nullptr;
+        b->capturing = true;
+        OWL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            loop();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            b->capturing = false;
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        b->capturing = false;
+        if (e != hipSuccess || !g) fail(OWL_ERR_DEVICE, "graph capture of the forward loop failed");
+        const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b->gexec = nullptr;
+            fail(OWL_ERR_DEVICE, "graph instantiation of the forward loop failed");
+        }
+        b->gsig = sig;
+        OWL_HIP(hipGraphLaunch(b->gexec, st));
+    } else {
+        loop();
     }
     if (save_boundary) strips_launch(b, R, cur.f[1], cur.f[2], cur.f[3], b->nt, 1);
-    // the final state lives in b->f for the reverse pass
-    if (fused && cur.f[0] != b->f[0].get()) {
+    // the final state lives in b->f for the reverse pass (modelling alone
+    // keeps the buffers where they are, so a captured loop stays valid)
+    if (save_boundary && fused && cur.f[0] != b->f[0].get()) {
         for (int k = 0; k < 4; ++k) std::swap(b->f[k], b->fB[k]);
         for (int k = 0; k < 3; ++k) std::swap(b->m[k], b->mB[k]);
     }
@@ -2422,6 +2488,18 @@ void set_shape(owl_acoustic3d* b, int planes, int variant) {
 }  // namespace
 
 extern "C" {
+
+owl_status owl_acoustic3d_set_graphs(owl_acoustic3d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->use_graphs = on != 0;
+        if (b->gexec) {
+            OWL_HIP(hipGraphExecDestroy(b->gexec));
+            b->gexec = nullptr;
+        }
+        b->gsig.clear();
+    });
+}
 
 owl_status owl_acoustic3d_set_launch(owl_acoustic3d* b, int planes, int variant) {
     return guarded([&] {

@@ -209,3 +209,41 @@ def test_owl_gradient3d_batch_sum_and_misfit(oracle32, gpu):
     assert rel_l2(gv, ref_v) < 1e-4
     assert rel_l2(gr, ref_r) < 1e-4
     b.close()
+
+
+@pytest.mark.parametrize("kc,var", [(16, 8), (9, 32), (13, 1)])
+def test_forward3d_graph_replays_are_bit_identical(gpu, kc, var):
+    """The graph-captured forward loop (capture, then replays) gives the eager
+    traces bit for bit, with eager work (a profiled forward, a forward with
+    boundary records, which moves the fused ping-pong buffers) between the
+    replays; a new launch shape takes a new graph."""
+    nz, ny, nx, dx = 30, 22, 40, 10.0
+    vp, rho = _model(nz, ny, nx, seed=12)
+    geoms = [(Geometry(id=1, srcr=[Source(x=185.0, y=105.0, z=150.0, f0=20.0)],
+                       recr=[Receiver(x=20.0 + 9.0 * k, y=100.0, z=40.0) for k in range(35)]), None),
+             (Geometry(id=2, srcr=[Source(x=300.0, y=120.0, z=90.0, f0=20.0, mechanism="force", polar=0.5)],
+                       recr=[Receiver(x=220.0 + 10.0 * k, y=110.0, z=30.0) for k in range(15)]),
+              dict(nxbeg=15, nxend=40, nybeg=3, nyend=22, nzbeg=1, nzend=25))]
+    print(f"case: 3D forward graph replays, shape ({kc}, {var})", flush=True)
+    b = Acoustic3DBatch(nx, ny, nz, dx, dx, dx, vp, rho, geoms, 0.001, 0.121, pml=8, fdhalf=4)
+    b.set_launch(kc, var)
+    b.set_graphs(False)
+    ref = b.forward()
+    b.set_graphs(True)
+    for k in range(4):
+        got = b.forward()
+        for a, c in zip(got, ref):
+            assert np.abs(c).max() > 0
+            assert np.array_equal(a, c), k
+        if k == 1:
+            b.set_profiling(True)
+            for a, c in zip(b.forward(), ref):
+                assert np.array_equal(a, c)
+            b.set_profiling(False)
+        if k == 2:
+            for a, c in zip(b.forward(save_boundary=True), ref):
+                assert np.array_equal(a, c)
+    b.set_launch(7, 1)
+    for a, c in zip(b.forward(), ref):
+        assert np.array_equal(a, c)
+    b.close()
