@@ -82,7 +82,7 @@ typedef struct owl_acoustic_config {
     float dx, dy, dz;
     float ox, oy, oz;
     float dt, tmax;          /* nt = nint(tmax/dt + 1) */
-    float data_dt, data_tmax;/* must equal dt, tmax: OWL's cubic resampling is not provided */
+    float data_dt, data_tmax;/* must equal dt, tmax: resample afterwards with owl_resample_traces */
     int pml;                 /* npml, default 15 (module_model.f90:174) */
     int fdhalf;              /* R: 2..8; OWL's -D_fdorder<2R>_ */
     int free_surface;        /* 2D only */
@@ -160,6 +160,21 @@ typedef struct owl_window_params {
  * coordinates range[6] = xbeg, xend, ybeg, yend, zbeg, zend. */
 owl_status owl_set_adaptive_range(const owl_window_params *p, int ns, const owl_source *src, int nr,
                                   owl_receiver *rec, int *window, float *range);
+
+/* Trace resampling of su%resamp (src/lib/module_su.f90:425-468), which the
+ * solvers call when data_dt/data_tmax differ from dt/tmax
+ * (acoustic_iso_2d/submodule_forward_modeling.f90:124-125) and on the
+ * adjoint source (submodule_fwi_gradient.f90:46): in[nr][nt] sampled at
+ * k dt to out[nr][nnt] at oot + k ddt by FLIT's interp 'cubic' (taken as
+ * the natural cubic spline, end cubics extended); all-zero traces stay
+ * zero.  Host code; in and out may not alias. */
+owl_status owl_resample_traces(int nr, int nt, float dt, const float *in, int nnt, float ddt, float oot,
+                               float *out);
+
+/* FLIT's ginterp(x, y, xi, 'linear' | 'cubic') on increasing x, as
+ * load_geometry calls it on the topography to drop sources and receivers
+ * above the free surface (module_source_receiver.f90:302-316). */
+owl_status owl_ginterp(int n, const float *x, const float *y, int m, const float *xi, float *yi, int cubic);
 
 /* ---- 2D acoustic: batched shots on one GPU --------------------------------
  * Replaces the per-shot loops of forward_modeling

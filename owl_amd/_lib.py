@@ -87,6 +87,8 @@ _SIGNATURES = {
     "owl_locate_point": (C.c_int, [C.c_int] + [C.c_float] * 9 + [_P, _P]),
     "owl_source_time_function": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_float, _P, C.c_int, _P]),
     "owl_set_adaptive_range": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P]),
+    "owl_resample_traces": (C.c_int, [C.c_int, C.c_int, C.c_float, _P, C.c_int, C.c_float, C.c_float, _P]),
+    "owl_ginterp": (C.c_int, [C.c_int, _P, _P, C.c_int, _P, _P, C.c_int]),
     "owl_cfspml_axis": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                   _P, _P, _P, _P]),
     "owl_acoustic2d_create": (C.c_int, [_P, _P, _P, C.c_int, _P, _P]),

@@ -1547,7 +1547,7 @@ void plan(owl_acoustic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         fail(OWL_ERR_ARG, "invalid grid or dt");
     if (cfg->pml < cfg->fdhalf) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
     if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
-        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (resample the traces with owl_resample_traces)");
     b->cfg = *cfg;
     b->R = cfg->fdhalf;
     b->pml = cfg->pml;

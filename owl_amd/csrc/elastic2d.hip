@@ -582,7 +582,7 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     if (cfg->pml < cfg->fdhalf || cfg->pml + cfg->fdhalf < kNkw) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
     if (cfg->free_surface) fail(OWL_ERR_ARG, "the 2D elastic solver here has no free surface");
     if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
-        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (resample the traces with owl_resample_traces)");
     b->cfg = *cfg;
     b->R = cfg->fdhalf;
     b->pml = cfg->pml;

@@ -941,7 +941,7 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
         fail(OWL_ERR_ARG, "invalid grid or dt");
     if (cfg->pml < cfg->fdhalf || cfg->pml + cfg->fdhalf < kNkw) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
     if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
-        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (resample the traces with owl_resample_traces)");
     b->cfg = *cfg;
     b->R = cfg->fdhalf;
     b->pml = cfg->pml;

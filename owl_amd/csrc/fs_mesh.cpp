@@ -44,6 +44,17 @@ int segment(const float* x, int n, float v) {
     return lo;
 }
 
+void rescale(const float* x, int n, float r1, float r2, float* y) {
+    float lo = x[0], hi = x[0];
+    for (int i = 1; i < n; ++i) {
+        lo = std::min(lo, x[i]);
+        hi = std::max(hi, x[i]);
+    }
+    for (int i = 0; i < n; ++i) y[i] = (x[i] - lo) / (hi - lo) * (r2 - r1) + r1;
+}
+
+}  // namespace
+
 void ginterp(const float* x, const float* y, int n, const float* xi, int m, float* yi, bool cubic) {
     if (n == 1) {
         std::fill(yi, yi + m, y[0]);
@@ -78,17 +89,6 @@ void ginterp(const float* x, const float* y, int n, const float* xi, int m, floa
         yi[k] = float(v);
     }
 }
-
-void rescale(const float* x, int n, float r1, float r2, float* y) {
-    float lo = x[0], hi = x[0];
-    for (int i = 1; i < n; ++i) {
-        lo = std::min(lo, x[i]);
-        hi = std::max(hi, x[i]);
-    }
-    for (int i = 0; i < n; ++i) y[i] = (x[i] - lo) / (hi - lo) * (r2 - r1) + r1;
-}
-
-}  // namespace
 
 Mesh build(int nx, int nz0, float dx, float dz0, float ox, float oz, int pml, const Surface& s, const float* c55,
            const float* rho, float f0min) {

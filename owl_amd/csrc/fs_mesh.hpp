@@ -45,6 +45,10 @@ struct Mesh {
     float zz_at(int i, int j) const { return zz[size_t(i + pml) * size_t(nrj()) + size_t(j + pml)]; }
 };
 
+// FLIT's ginterp on increasing x: 'linear' (cubic = false) or the natural
+// cubic spline evaluated in double (cubic = true), end segments extended.
+void ginterp(const float* x, const float* y, int n, const float* xi, int m, float* yi, bool cubic);
+
 // Builds the mesh.  c55, rho: the window (nz0, nx) z fastest; f0min: the
 // smallest source frequency of the shot.  Throws owl::Error on input that
 // cannot give a mesh.

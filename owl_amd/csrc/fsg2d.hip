@@ -1007,7 +1007,7 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
     b->fs = cfg->free_surface != 0 || b->fs;
     if (b->fs && !(b->surf.dz_max > 0)) b->surf.dz_max = 1.5f * cfg->dz;  // module_model.f90:178
     if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
-        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (trace resampling is not provided)");
+        fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (resample the traces with owl_resample_traces)");
     b->cfg = *cfg;
     b->R = cfg->fdhalf;
     b->pml = cfg->pml;

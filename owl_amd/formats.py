@@ -96,3 +96,20 @@ def shot_file(directory, shot_id, kind, name):
     if kind == "seismogram":
         return os.path.join(directory, f"shot_{shot_id}_seismogram_{name}.su")
     return os.path.join(directory, f"shot_{shot_id}_grad_{name}.grd")
+
+
+def resample(traces, dt, nnt, ddt, oot=0.0):
+    """su%resamp (module_su.f90:425-468) through owl_resample_traces:
+    traces (nr, nt) sampled at k dt to (nr, nnt) at oot + k ddt, FLIT's
+    interp 'cubic' per trace, all-zero traces kept zero.  Returns the input
+    unchanged when nnt, ddt and oot already match, as OWL does."""
+    from . import _lib
+    tr = np.ascontiguousarray(traces, dtype=np.float32)
+    if tr.ndim != 2:
+        raise ValueError("traces must be (nr, nt)")
+    nr, nt = tr.shape
+    if nt == nnt and np.float32(dt) == np.float32(ddt) and oot == 0:
+        return tr.copy()
+    out = np.empty((nr, int(nnt)), np.float32)
+    _lib.check(_lib.lib().owl_resample_traces(nr, nt, dt, tr.ctypes.data, int(nnt), ddt, oot, out.ctypes.data))
+    return out

@@ -13,15 +13,16 @@
 ! adjoint (compute_adjoint, submodule_fwi_gradient.f90:31-199) reads
 ! dir_adjoint/shot_<id>_seismogram_p.su into seis_pr and writes
 ! dir_working/shot_<id>_grad_{vp,rho}.grd, with energy_precond applied as
-! OWL does.  Not provided: wavefield snapshots (snaps is accepted and
-! reported), and trace resampling (data_dt, data_tmax must equal dt, tmax).
+! OWL does.  Traces are resampled to data_dt, data_tmax after the forward
+! and the adjoint source back to dt, tmax (su%resamp, owl_resample_traces).
+! Not provided: wavefield snapshots (snaps is accepted and reported).
 !
 module acoustic_iso_2d
 
     use iso_c_binding
     use owl_amd_c
     use owl_geometry, only: source_receiver_geometry, su
-    use owl_files, only: write_su_gather, read_su_gather, write_grd_file, itoa
+    use owl_files, only: write_su_gather, read_su_gather, write_grd_file, itoa, resamp_gather
 
     implicit none
 
@@ -62,9 +63,24 @@ module acoustic_iso_2d
 
     private
     public :: wave_solver_acoustic_iso_2d
-    public :: to_c_sources, to_c_receivers, check_status
+    public :: to_c_sources, to_c_receivers, check_status, data_samples
 
 contains
+
+    ! data_nt = nint(data_tmax/data_dt + 1) (module_vars.f90:162-164); a
+    ! data_dt left at 0 means dt
+    subroutine data_samples(dt, tmax, data_dt, data_tmax, ddt, dnt)
+        real, intent(in) :: dt, tmax, data_dt, data_tmax
+        real, intent(out) :: ddt
+        integer, intent(out) :: dnt
+        if (data_dt > 0) then
+            ddt = data_dt
+            dnt = nint(data_tmax/data_dt + 1)
+        else
+            ddt = dt
+            dnt = nint(tmax/dt + 1)
+        end if
+    end subroutine
 
     subroutine check_status(st, what)
         integer(c_int), intent(in) :: st
@@ -146,14 +162,11 @@ contains
         type(owl_source_t), allocatable, target :: cs(:)
         type(owl_receiver_t), allocatable, target :: cr(:)
         real, allocatable :: rho(:, :), tr(:)
-        integer :: nt, nr, ir
+        integer :: nt, nr, ir, dnt
+        real :: ddt
         integer(c_int) :: st
 
         call this%release()
-        if (this%data_dt /= this%dt .or. this%data_tmax /= this%tmax) then
-            write (*, '(a)') ' <acoustic_iso_2d> data_dt/data_tmax must equal dt/tmax (no trace resampling)'
-            error stop 1
-        end if
         if (allocated(this%snaps)) then
             if (size(this%snaps) > 0 .and. this%verbose) &
                 write (*, '(a)') ' <acoustic_iso_2d> wavefield snapshots are not written'
@@ -210,6 +223,9 @@ contains
         do ir = 1, nr
             this%seis_p%trace(ir)%data = tr((ir - 1)*nt + 1:ir*nt)
         end do
+        ! resample if necessary (submodule_forward_modeling.f90:124-125)
+        call data_samples(this%dt, this%tmax, this%data_dt, this%data_tmax, ddt, dnt)
+        call resamp_gather(this%seis_p, dnt, ddt)
         call write_su_gather(trim(this%dir_synthetic)//'/shot_'//itoa(this%gmtr%id)//'_seismogram_p.su', &
             this%seis_p)
 
@@ -232,6 +248,8 @@ contains
         call read_su_gather(trim(this%dir_adjoint)//'/shot_'//itoa(this%gmtr%id)//'_seismogram_p.su', &
             this%seis_pr)
         nt = owl_acoustic2d_nt(this%batch)
+        ! back on the solver's time axis (submodule_fwi_gradient.f90:46)
+        call resamp_gather(this%seis_pr, nt, this%dt)
         if (this%seis_pr%nt /= nt .or. this%seis_pr%nr /= this%seis_p%nr) then
             write (*, '(a)') ' <acoustic_iso_2d> the adjoint source does not match the forward traces'
             error stop 1

@@ -6,16 +6,17 @@
 ! .f90:31-183) fills seis_p and writes dir_synthetic/shot_<id>_seismogram_p.su;
 ! adjoint (submodule_fwi_gradient.f90:33-274) reads
 ! dir_adjoint/shot_<id>_seismogram_p.su into seis_pr and writes
-! dir_working/shot_<id>_grad_{vp,rho}.grd as (nz, ny, nx) grids.  Not
-! provided: energy_precond (refused), snapshots, trace resampling.
+! dir_working/shot_<id>_grad_{vp,rho}.grd as (nz, ny, nx) grids.  Traces
+! are resampled to data_dt, data_tmax and the adjoint source back to dt, tmax
+! (su%resamp).  Not provided: energy_precond (refused), snapshots.
 !
 module acoustic_iso_3d
 
     use iso_c_binding
     use owl_amd_c
     use owl_geometry, only: source_receiver_geometry, su
-    use owl_files, only: write_su_gather, read_su_gather, write_grd_file, itoa
-    use acoustic_iso_2d, only: to_c_sources, to_c_receivers, check_status
+    use owl_files, only: write_su_gather, read_su_gather, write_grd_file, itoa, resamp_gather
+    use acoustic_iso_2d, only: to_c_sources, to_c_receivers, check_status, data_samples
 
     implicit none
 
@@ -73,14 +74,11 @@ contains
         type(owl_source_t), allocatable, target :: cs(:)
         type(owl_receiver_t), allocatable, target :: cr(:)
         real, allocatable :: rho(:, :, :), tr(:)
-        integer :: nt, nr, ir
+        integer :: nt, nr, ir, dnt
+        real :: ddt
         integer(c_int) :: st
 
         call this%release()
-        if (this%data_dt /= this%dt .or. this%data_tmax /= this%tmax) then
-            write (*, '(a)') ' <acoustic_iso_3d> data_dt/data_tmax must equal dt/tmax (no trace resampling)'
-            error stop 1
-        end if
         cfg%nx = this%nx
         cfg%ny = this%ny
         cfg%nz = this%nz
@@ -131,6 +129,9 @@ contains
         do ir = 1, nr
             this%seis_p%trace(ir)%data = tr((ir - 1)*nt + 1:ir*nt)
         end do
+        ! resample if necessary (submodule_forward_modeling.f90:173)
+        call data_samples(this%dt, this%tmax, this%data_dt, this%data_tmax, ddt, dnt)
+        call resamp_gather(this%seis_p, dnt, ddt)
         call write_su_gather(trim(this%dir_synthetic)//'/shot_'//itoa(this%gmtr%id)//'_seismogram_p.su', &
             this%seis_p)
         if (.not. this%reconstruct) call this%release()
@@ -156,6 +157,8 @@ contains
         call read_su_gather(trim(this%dir_adjoint)//'/shot_'//itoa(this%gmtr%id)//'_seismogram_p.su', &
             this%seis_pr)
         nt = owl_acoustic3d_nt(this%batch)
+        ! back on the solver's time axis (submodule_fwi_gradient.f90:51)
+        call resamp_gather(this%seis_pr, nt, this%dt)
         if (this%seis_pr%nt /= nt .or. this%seis_pr%nr /= this%seis_p%nr) then
             write (*, '(a)') ' <acoustic_iso_3d> the adjoint source does not match the forward traces'
             error stop 1

@@ -6,25 +6,26 @@
 ! window.  The medium is converted from anisotropy_type 'iso', 'thomsen',
 ! 'a-t' or 'cij' as prepare_modeling does (module_vars.f90:284-340).
 ! forward (submodule_forward_modeling.f90:30-300) fills seis_vx, seis_vz and
-! writes dir_synthetic/shot_<id>_seismogram_{vx,vz}.su (compx, compz).
+! writes dir_synthetic/shot_<id>_seismogram_{x,z}.su (compx, compz).
 ! adjoint (submodule_fwi_gradient.f90:40-560) reads the adjoint sources
-! dir_adjoint/shot_<id>_seismogram_{vx,vz}.su into seis_vxr, seis_vzr and
+! dir_adjoint/shot_<id>_seismogram_{x,z}.su into seis_vxr, seis_vzr and
 ! writes dir_working/shot_<id>_grad_<m>.grd for the parameters of
 ! anisotropy_type (c11 .. c55; or vp, vs [, epsilon, delta | eta]) and rho.
 ! free_surface runs OWL's topography-following free surface on the
 ! depth-refined mesh (module_vars.f90:360-519): file_topo (two columns, x
 ! and elevation), topo_interp, free_surface_dz_refine, dz_max (0: 1.5 dz)
 ! and measure_source/receiver_depth_from_surface as OWL reads them.
-! Not provided (refused): energy_precond, moment-tensor and source-update
-! options, trace resampling, wavefield snapshots.
+! Traces are resampled to data_dt, data_tmax and the adjoint sources back
+! to dt, tmax (su%resamp).  Not provided (refused): energy_precond,
+! moment-tensor and source-update options, wavefield snapshots.
 !
 module elastic_tti_2d
 
     use iso_c_binding
     use owl_amd_c
     use owl_geometry, only: source_receiver_geometry, su
-    use owl_files, only: write_su_gather, read_su_gather, write_grd_file, itoa
-    use acoustic_iso_2d, only: to_c_sources, to_c_receivers, check_status
+    use owl_files, only: write_su_gather, read_su_gather, write_grd_file, itoa, resamp_gather
+    use acoustic_iso_2d, only: to_c_sources, to_c_receivers, check_status, data_samples
 
     implicit none
 
@@ -122,14 +123,11 @@ contains
         type(c_ptr) :: cp(6)
         type(owl_fsg_surface_t) :: surf
         real(c_float), allocatable, target :: tx(:), tz(:)
-        integer :: nt, nr, ir, k, pc
+        integer :: nt, nr, ir, k, pc, dnt
+        real :: ddt
         integer(c_int) :: st
 
         call this%release()
-        if (this%data_dt /= this%dt .or. this%data_tmax /= this%tmax) then
-            write (*, '(a)') ' <elastic_tti_2d> data_dt/data_tmax must equal dt/tmax (no trace resampling)'
-            error stop 1
-        end if
         cfg%nx = this%nx
         cfg%nz = this%nz
         cfg%dx = this%dx
@@ -220,10 +218,14 @@ contains
         call check_status(st, 'owl_fsg2d_forward')
         call fill(this%seis_vx, tr(1:nt*nr), nt, nr, this%dt)
         call fill(this%seis_vz, tr(nt*nr + 1:2*nt*nr), nt, nr, this%dt)
+        ! resample if necessary (submodule_forward_modeling.f90:286-291)
+        call data_samples(this%dt, this%tmax, this%data_dt, this%data_tmax, ddt, dnt)
+        call resamp_gather(this%seis_vx, dnt, ddt)
+        call resamp_gather(this%seis_vz, dnt, ddt)
         if (this%compx) call write_su_gather(trim(this%dir_synthetic)//'/shot_'//itoa(this%gmtr%id)// &
-            '_seismogram_vx.su', this%seis_vx)
+            '_seismogram_x.su', this%seis_vx)
         if (this%compz) call write_su_gather(trim(this%dir_synthetic)//'/shot_'//itoa(this%gmtr%id)// &
-            '_seismogram_vz.su', this%seis_vz)
+            '_seismogram_z.su', this%seis_vz)
         if (.not. this%reconstruct) call this%release()
 
     end subroutine
@@ -251,6 +253,15 @@ contains
             read (u, *) tx(k), tz(k)
         end do
         close (u)
+    end subroutine
+
+    subroutine check_adjoint(g, nt, nr)
+        type(su), intent(in) :: g
+        integer, intent(in) :: nt, nr
+        if (g%nt /= nt .or. g%nr /= nr) then
+            write (*, '(a)') ' <elastic_tti_2d> the adjoint source does not match the forward traces'
+            error stop 1
+        end if
     end subroutine
 
     subroutine fill(g, tr, nt, nr, dt)
@@ -295,13 +306,17 @@ contains
         adj = 0.0
         pre = trim(this%dir_adjoint)//'/shot_'//itoa(this%gmtr%id)
         if (this%compx) then
-            call read_su_gather(pre//'_seismogram_vx.su', this%seis_vxr)
+            call read_su_gather(pre//'_seismogram_x.su', this%seis_vxr)
+            call resamp_gather(this%seis_vxr, nt, this%dt)
+            call check_adjoint(this%seis_vxr, nt, nr)
             do ir = 1, nr
                 adj((ir - 1)*nt + 1:ir*nt) = this%seis_vxr%trace(ir)%data
             end do
         end if
         if (this%compz) then
-            call read_su_gather(pre//'_seismogram_vz.su', this%seis_vzr)
+            call read_su_gather(pre//'_seismogram_z.su', this%seis_vzr)
+            call resamp_gather(this%seis_vzr, nt, this%dt)
+            call check_adjoint(this%seis_vzr, nt, nr)
             do ir = 1, nr
                 adj(nt*nr + (ir - 1)*nt + 1:nt*nr + ir*nt) = this%seis_vzr%trace(ir)%data
             end do

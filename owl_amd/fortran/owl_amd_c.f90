@@ -14,6 +14,8 @@ module owl_amd_c
     integer(c_int), parameter :: OWL_GRAD_OWL = 0, OWL_GRAD_EXACT = 1
     integer(c_int), parameter :: OWL_CKPT_FULL = 0, OWL_CKPT_UNIFORM = 1, OWL_CKPT_BINOMIAL = 2
     integer(c_int), parameter :: OWL_STORE_F32 = 0, OWL_STORE_F16 = 1, OWL_STORE_BF16 = 2
+    integer(c_int), parameter :: OWL_MISFIT_WAVEFORM = 0, OWL_MISFIT_CORR = 1, OWL_MISFIT_ENVELOPE = 2, &
+        OWL_MISFIT_PHASE = 3
 
     type, bind(c) :: owl_source_t
         real(c_float) :: x = 0, y = 0, z = 0
@@ -190,6 +192,35 @@ module owl_amd_c
             type(owl_receiver_t), intent(inout) :: rec(*)
             integer(c_int), intent(out) :: window(6)
             real(c_float), intent(out) :: range(6)
+            integer(c_int) :: st
+        end function
+
+        function owl_adjoint_source(mtype, ntr, nt, obs, syn, weight, threshold, adj, misfit) &
+                bind(c, name='owl_adjoint_source') result(st)
+            import :: c_int, c_float, c_double
+            integer(c_int), value :: mtype, ntr, nt
+            real(c_float), intent(in) :: obs(*), syn(*), weight(*)
+            real(c_float), value :: threshold
+            real(c_float), intent(out) :: adj(*)
+            real(c_double), intent(out) :: misfit(*)
+            integer(c_int) :: st
+        end function
+
+        function owl_resample_traces(nr, nt, dt, tin, nnt, ddt, oot, tout) &
+                bind(c, name='owl_resample_traces') result(st)
+            import :: c_int, c_float
+            integer(c_int), value :: nr, nt, nnt
+            real(c_float), value :: dt, ddt, oot
+            real(c_float), intent(in) :: tin(*)
+            real(c_float), intent(out) :: tout(*)
+            integer(c_int) :: st
+        end function
+
+        function owl_ginterp(n, x, y, m, xi, yi, cubic) bind(c, name='owl_ginterp') result(st)
+            import :: c_int, c_float
+            integer(c_int), value :: n, m, cubic
+            real(c_float), intent(in) :: x(*), y(*), xi(*)
+            real(c_float), intent(out) :: yi(*)
             integer(c_int) :: st
         end function
 

@@ -7,12 +7,14 @@
 !
 module owl_files
 
+    use iso_c_binding, only: c_int
     use owl_geometry, only: su
+    use owl_amd_c, only: owl_resample_traces, owl_error_message, OWL_OK
 
     implicit none
 
     private
-    public :: write_su_gather, read_su_gather, write_grd_file, itoa
+    public :: write_su_gather, read_su_gather, write_grd_file, itoa, resamp_gather, read_grd_2d
 
     interface write_grd_file
         module procedure :: write_grd_2d
@@ -58,6 +60,33 @@ contains
             write (u) g%trace(ir)%data(1:g%nt)
         end do
         close (u)
+    end subroutine
+
+    ! su%resamp (src/lib/module_su.f90:425-468) through owl_resample_traces:
+    ! each trace to nnt samples at ddt (FLIT's interp 'cubic'); a gather that
+    ! already has nnt samples at ddt is left as it is.
+    subroutine resamp_gather(g, nnt, ddt)
+        type(su), intent(inout) :: g
+        integer, intent(in) :: nnt
+        real, intent(in) :: ddt
+        real, allocatable :: a(:), b(:)
+        integer :: ir
+        integer(c_int) :: st
+        if (g%nt == nnt .and. g%dt == ddt) return
+        allocate (a(max(1, g%nr*g%nt)), b(max(1, g%nr*nnt)))
+        do ir = 1, g%nr
+            a((ir - 1)*g%nt + 1:ir*g%nt) = g%trace(ir)%data(1:g%nt)
+        end do
+        st = owl_resample_traces(g%nr, g%nt, g%dt, a, nnt, ddt, 0.0, b)
+        if (st /= OWL_OK) then
+            write (*, '(a)') ' <owl_amd> owl_resample_traces failed: '//owl_error_message()
+            error stop 1
+        end if
+        do ir = 1, g%nr
+            g%trace(ir)%data = b((ir - 1)*nnt + 1:ir*nnt)
+        end do
+        g%nt = nnt
+        g%dt = ddt
     end subroutine
 
     subroutine read_su_gather(path, g)
@@ -118,6 +147,16 @@ contains
         open (newunit=u, file=path, access='stream', form='unformatted', status='replace')
         call grd_header(u, shape(a), [d1, d2], [o1, o2])
         write (u, pos=4001) a
+        close (u)
+    end subroutine
+
+    ! the float32 body of a 2D grid file into a, whose shape the caller knows
+    subroutine read_grd_2d(path, a)
+        character(len=*), intent(in) :: path
+        real, intent(out) :: a(:, :)
+        integer :: u
+        open (newunit=u, file=path, access='stream', form='unformatted', status='old', action='read')
+        read (u, pos=4001) a
         close (u)
     end subroutine
 

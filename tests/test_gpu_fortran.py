@@ -242,8 +242,8 @@ def _tti_drop_in(tmp_path, surface):
     g = b.adjoint([(ax, az)])[0]
     b.close()
     pg = tti_param_gradient("thomsen", g, vp, vs, rho, eps, dlt, the)
-    write_su(os.path.join(dirs["adj"], "shot_2_seismogram_vx.su"), ax, dt)
-    write_su(os.path.join(dirs["adj"], "shot_2_seismogram_vz.su"), az, dt)
+    write_su(os.path.join(dirs["adj"], "shot_2_seismogram_x.su"), ax, dt)
+    write_su(os.path.join(dirs["adj"], "shot_2_seismogram_z.su"), az, dt)
     job = str(tmp_path / "jobtti.txt")
     extra = []
     if surface is not None:
@@ -259,8 +259,8 @@ def _tti_drop_in(tmp_path, surface):
     print(f"case: elastic_tti_2d drop-in forward + adjoint, Thomsen, surface {surface is not None}", flush=True)
     r = subprocess.run([SOLVERS, "tti", job], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    sx, _ = read_su(os.path.join(dirs["syn"], "shot_2_seismogram_vx.su"))
-    sz, _ = read_su(os.path.join(dirs["syn"], "shot_2_seismogram_vz.su"))
+    sx, _ = read_su(os.path.join(dirs["syn"], "shot_2_seismogram_x.su"))
+    sz, _ = read_su(os.path.join(dirs["syn"], "shot_2_seismogram_z.su"))
     assert np.abs(tz).max() > 0 and np.array_equal(sx, tx) and np.array_equal(sz, tz)
     for k in ("vp", "vs", "epsilon", "delta"):
         a = read_grd(os.path.join(dirs["work"], f"shot_2_grad_{k}.grd"))
