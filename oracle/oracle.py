@@ -66,7 +66,7 @@ class EGrid2D(C.Structure):
 
 class ESource2D(C.Structure):
     _fields_ = [("x", C.c_float), ("z", C.c_float), ("amp", C.c_float), ("t0", C.c_float),
-                ("polar", C.c_float), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
+                ("polar", C.c_float), ("mechanism", C.c_int), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
 
 
 class EReceiver2D(C.Structure):
@@ -230,6 +230,16 @@ class Oracle:
                                 buf.ctypes.data_as(C.c_void_p), C.byref(hnt))
         return buf, hnt.value
 
+    def stf_time_derivative(self, stf, dt, nt):
+        """The explosion wavelet of the elastic solvers: deriv(stf)/dt
+        (time_integration = -1), extended to nt samples when its tail is
+        large."""
+        src = np.ascontiguousarray(stf, dtype=self.real)
+        out = np.zeros(max(len(src), int(nt)), self.real)
+        n = self._fn("stf_time_derivative")(src.ctypes.data_as(C.c_void_p), len(src), C.c_float(dt), int(nt),
+                                             out.ctypes.data_as(C.c_void_p))
+        return out[:n].copy()
+
     def cfspml_axis(self, n, pml, d, dt, pmlvp, f0max):
         arrs = [np.zeros(n + 2 * pml, np.float32) for _ in range(6)]
         self._fn("cfspml_axis")(n, pml, d, dt, pmlvp, f0max,
@@ -354,16 +364,19 @@ class Oracle:
                       int(g["pml"]), int(g["R"]), int(bool(g.get("free_surface", False))),
                       int(g.get("cc_step_interval", 1)))
 
-    def _sources3(self, g, sources):
+    def _sources3(self, g, sources, elastic=False):
+        """Elastic explosions take the time derivative of the wavelet."""
         keep = []
         arr = (Source3D * max(1, len(sources)))()
         f0max = 0.0
         hnt = 0
         for k, s in enumerate(sources):
             stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            mech = s.get("mechanism", "explosion")
+            if elastic and mech == "explosion":
+                stf = self.stf_time_derivative(stf, g["dt"], g["nt"])
             stf = np.ascontiguousarray(stf, dtype=self.real)
             keep.append(stf)
-            mech = s.get("mechanism", "explosion")
             arr[k] = Source3D(s["x"], s["y"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0),
                               s.get("polar", 0.0), s.get("azimuth", 0.0),
                               MECH[mech] if isinstance(mech, str) else int(mech), len(stf),
@@ -432,7 +445,7 @@ class Oracle:
         vp = self._model(vp, shape)
         vs = self._model(vs, shape)
         rho = self._model(rho, shape)
-        arr, keep, f0max, _ = self._sources3(g, [dict(s, mechanism="force") for s in sources])
+        arr, keep, f0max, _ = self._sources3(g, [dict(s, mechanism=s.get("mechanism", "force")) for s in sources], elastic=True)
         rec = self._receivers3(receivers)
         tr = np.zeros((3, len(receivers), G.nt), self.real)
         self._fn("elastic3d_forward").argtypes = [C.POINTER(EGrid3D), C.c_void_p, C.c_void_p, C.c_void_p,
@@ -454,7 +467,8 @@ class Oracle:
         vp = self._model(vp, shape)
         vs = self._model(vs, shape)
         rho = self._model(rho, shape)
-        arr, keep, f0max, hnt = self._sources3(g, [dict(s, mechanism="force") for s in sources])
+        arr, keep, f0max, hnt = self._sources3(g, [dict(s, mechanism=s.get("mechanism", "force")) for s in sources],
+                                                elastic=True)
         rec = self._receivers3(receivers)
         adj = np.ascontiguousarray(adj, dtype=self.real)
         assert adj.shape == (3, len(receivers), G.nt)
@@ -487,10 +501,13 @@ class Oracle:
         f0max, hnt = 0.0, 0
         for k, s in enumerate(sources):
             stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            mech = MECH[s.get("mechanism", "force")]
+            if mech == MECH["explosion"]:
+                stf = self.stf_time_derivative(stf, g["dt"], g["nt"])
             stf = np.ascontiguousarray(stf, dtype=self.real)
             keep.append(stf)
             arr[k] = ESource2D(s["x"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0), s.get("polar", 0.0),
-                               len(stf), stf.ctypes.data)
+                               mech, len(stf), stf.ctypes.data)
             f0max = max(f0max, s["f0"])
             if k == 0:
                 hnt = h
@@ -559,9 +576,11 @@ class Oracle:
         f0max, hnt = 0.0, 0
         for k, s in enumerate(sources):
             stf, h = self.stf(s["f0"], g["dt"], s.get("f0factor", 1.0), s.get("wavelet", "ricker"))
+            mech = {"force": 0, "explosion": 1}[s.get("mechanism", "force")]
+            if mech == 1:
+                stf = self.stf_time_derivative(stf, g["dt"], g["nt"])
             stf = np.ascontiguousarray(stf, dtype=self.real)
             keep.append(stf)
-            mech = {"force": 0, "explosion": 1}[s.get("mechanism", "force")]
             arr[k] = FSource2D(s["x"], s["z"], s.get("amp", 1.0), s.get("t0", 0.0), s.get("polar", 0.0), mech,
                                len(stf), stf.ctypes.data)
             f0max = max(f0max, s["f0"])

@@ -219,6 +219,38 @@ int ORA_SYM(prepare_stf)(int wavelet, float f0, float dt, float f0factor, real *
 }
 
 /* ------------------------------------------------------------------------ */
+/* Stress-drop (explosion) wavelet of the elastic solvers: time_integration */
+/* = -1 (elastic_vhtiort_2d/module_vars.f90:476, elastic_tti_2d/            */
+/* module_vars.f90:689) makes prepare_stf take deriv(stf)/dt                */
+/* (module_source_receiver.f90:1111-1124).  FLIT's deriv taken as centred   */
+/* inside and one-sided at the ends; if the last sample keeps more than a   */
+/* tenth of the peak, the wavelet runs to nt samples with its last value    */
+/* (pad_array, edge replication).  out holds max(n, nt) values; returns the */
+/* new length.                                                              */
+/* ------------------------------------------------------------------------ */
+int ORA_SYM(stf_time_derivative)(const real *stf, int n, float dt, int nt, real *out) {
+    real peak = 0;
+    for (int i = 0; i < n; ++i) {
+        real d;
+        if (n < 2) d = 0;
+        else if (i == 0) d = stf[1] - stf[0];
+        else if (i == n - 1) d = stf[n - 1] - stf[n - 2];
+        else d = (real)0.5 * (stf[i + 1] - stf[i - 1]);
+        out[i] = d / (real)dt;
+        real a = out[i] < 0 ? -out[i] : out[i];
+        if (a > peak) peak = a;
+    }
+    if (n > 0 && peak > 0) {
+        real last = out[n - 1] < 0 ? -out[n - 1] : out[n - 1];
+        if (last / peak > (real)0.1f && nt > n) {
+            for (int i = n; i < nt; ++i) out[i] = out[n - 1];
+            return nt;
+        }
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
 /* CFL: prepare_modeling, acoustic_iso_2d/module_vars.f90:121-126 and       */
 /* acoustic_iso_3d/module_vars.f90:174-181.  Computed in float32.           */
 /* ------------------------------------------------------------------------ */

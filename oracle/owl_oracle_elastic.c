@@ -70,7 +70,7 @@ typedef struct {
 
 typedef struct {
     float x, y, z, amp, t0, polar, azimuth; /* radians */
-    int mechanism;                          /* 1 force (only) */
+    int mechanism;                          /* 0 explosion, 1 force */
     int nt_stf;
     const real *stf;
 } ora_esource3d;
@@ -152,6 +152,7 @@ typedef struct {
     eprof px, py, pz;
     float alphamax;
     real idx, idy, idz;
+    float dx, dy, dz;
     float cf[8];
 } emed;
 
@@ -184,6 +185,7 @@ static void emed_init(emed *M, const ora_egrid3d *g, const float *vp, const floa
     L.sy = (size_t)L.NX;
     L.sz = (size_t)L.NX * L.NY;
     M->L = L;
+    M->dx = g->dx; M->dy = g->dy; M->dz = g->dz;
     const size_t n = (size_t)L.NX * L.NY * L.NZ;
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     M->c11 = ezalloc(n); M->c12 = ezalloc(n); M->c44 = ezalloc(n); M->rho = ezalloc(n);
@@ -420,7 +422,10 @@ static void eupdate(const emed *M, efld *F, real dt) {
             }
 }
 
-/* force sources (add_source, module_wavefield.f90:1115-1175) */
+/* add_source (module_wavefield.f90:1115-1228): a force adds to vx, vy, vz
+ * with amp / rho(gx, gy, gz); an explosion subtracts amp / (dx dy dz) from
+ * sxx, syy, szz on (gx, gy, gz) with ix iy iz (its stf is the time
+ * derivative, time_integration = -1); amp = stf amp dt. */
 static void eadd_source(const emed *M, efld *F, const ora_esource3d *src, const ora_point *pts, int ns, int t,
                         real dt) {
     const elay L = M->L;
@@ -430,6 +435,21 @@ static void eadd_source(const emed *M, efld *F, const ora_esource3d *src, const 
         if (t < nbeg || t > nend) continue;
         real amp = src[s].stf[t - nbeg] * (real)src[s].amp * dt;
         const ora_point *P = &pts[s];
+        if (src[s].mechanism == 0) {
+            amp = amp / ((real)M->dx * (real)M->dy * (real)M->dz);
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int iry = -NKW; iry <= NKW; ++iry)
+                    for (int irx = -NKW; irx <= NKW; ++irx)
+                        if (!fs || P->gz + irz >= 2) {
+                            const size_t q = IE(L, P->gx + irx, P->gy + iry, P->gz + irz);
+                            const real w = amp * (real)P->ix[irx + NKW] * (real)P->iy[iry + NKW] *
+                                           (real)P->iz[irz + NKW];
+                            F->sxx[q] = F->sxx[q] - w;
+                            F->syy[q] = F->syy[q] - w;
+                            F->szz[q] = F->szz[q] - w;
+                        }
+            continue;
+        }
         amp = amp / M->rho[IE(L, P->gx, P->gy, P->gz)];
 #if defined(ORA_F64)
         const real sp = sin((double)src[s].polar), cp = cos((double)src[s].polar);
@@ -473,7 +493,7 @@ int ORA_SYM(elastic3d_forward)(const ora_egrid3d *g, const float *vp, const floa
                                real *traces) {
     if (g->R < 2 || g->R > 8 || g->pml < g->R) return -1;
     for (int s = 0; s < ns; ++s)
-        if (src[s].mechanism != 1) return -2;
+        if (src[s].mechanism != 0 && src[s].mechanism != 1) return -2;
     emed M;
     emed_init(&M, g, vp, vs, rho, f0max);
     const elay L = M.L;
@@ -621,7 +641,7 @@ int ORA_SYM(elastic3d_gradient_owl)(const ora_egrid3d *g, const float *vp, const
                                     real *grad_rho, real *traces_out) {
     if (g->R < 2 || g->R > 8 || g->pml < g->R || ns < 1) return -1;
     for (int s = 0; s < ns; ++s)
-        if (src[s].mechanism != 1) return -2;
+        if (src[s].mechanism != 0 && src[s].mechanism != 1) return -2;
     emed M;
     emed_init(&M, g, vp, vs, rho, f0max);
     const elay L = M.L;

@@ -63,6 +63,7 @@ typedef struct {
 
 typedef struct {
     float x, z, amp, t0, polar; /* polar in radians */
+    int mechanism;              /* 0 explosion, 1 force */
     int nt_stf;
     const real *stf;
 } ora_esource2d;
@@ -123,6 +124,7 @@ typedef struct {
     lay2 L;
     float c[8];
     real idx, idz;
+    float dx, dz;
     real *c11, *c13, *c33, *c55, *mu, *rx, *rz, *rho;
     prof2 px, pz;
 } emed2;
@@ -152,6 +154,8 @@ static void emed2_init(emed2 *M, const ora_egrid2d *g, const float *vp, const fl
     const int R = g->R, pml = g->pml, nx = g->nx, nz = g->nz;
     lay2 L = {1 - pml - R, nx + 2 * (pml + R), nz + 2 * (pml + R), nx, nz, pml, R};
     M->L = L;
+    M->dx = g->dx;
+    M->dz = g->dz;
     ORA_SYM(fdcoefs)(R, M->c);
     M->idx = (real)(float)(1.0 / (double)g->dx);
     M->idz = (real)(float)(1.0 / (double)g->dz);
@@ -292,9 +296,11 @@ static void eupdate2(const emed2 *M, efld2 *F, real dt) {
         }
 }
 
-/* add_source, force (module_wavefield.f90): vx at (hx, gz) += sin(polar)
- * amp / rho(hx, gz) * hx_w * iz_w; vz at (gx, hz) += cos(polar) amp /
- * rho(gx, hz) * ix_w * hz_w; amp = stf amp dt. */
+/* add_source (module_wavefield.f90:294-403): a force adds to vx at (hx, gz)
+ * sin(polar) amp / rho(hx, gz) * hx_w * iz_w and to vz at (gx, hz)
+ * cos(polar) amp / rho(gx, hz) * ix_w * hz_w; an explosion subtracts
+ * amp / (dx dz) * ix_w * iz_w from sxx and szz at (gx, gz) (its stf is the
+ * time derivative, time_integration = -1); amp = stf amp dt. */
 static void eadd_source2(const emed2 *M, efld2 *F, const ora_esource2d *src, const ora_point *pts, int ns, int t,
                          real dt) {
     const lay2 L = M->L;
@@ -304,6 +310,16 @@ static void eadd_source2(const emed2 *M, efld2 *F, const ora_esource2d *src, con
         const int ne = nb + src[s].nt_stf - 1;
         if (t < nb || t > ne) continue;
         const real amp = src[s].stf[t - nb] * (real)src[s].amp * dt;
+        if (src[s].mechanism == 0) {
+            const real ae = amp / ((real)M->dx * (real)M->dz);
+            for (int irz = -NKW; irz <= NKW; ++irz)
+                for (int irx = -NKW; irx <= NKW; ++irx) {
+                    const size_t q = I2(L, P->gx + irx, P->gz + irz);
+                    F->sxx[q] = F->sxx[q] - ae * (real)P->ix[irx + NKW] * (real)P->iz[irz + NKW];
+                    F->szz[q] = F->szz[q] - ae * (real)P->ix[irx + NKW] * (real)P->iz[irz + NKW];
+                }
+            continue;
+        }
 #if defined(ORA_F64)
         const real sp = sin((double)src[s].polar), cp = cos((double)src[s].polar);
 #else

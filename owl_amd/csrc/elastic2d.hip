@@ -72,7 +72,7 @@ struct Tap {
     float w1, w2;
     int row;
     int tbeg, tend;
-    int field;  // 0 vx, 1 vz
+    int field;  // 0 vx, 1 vz, 2 sxx, 3 szz
 };
 
 struct Prof2 {
@@ -285,16 +285,19 @@ __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ s
     }
 }
 
-// Force injection of step t (add_source): one block per shot, taps grouped
-// by (field, cell) in source order.
+// Source injection of step t (add_source): forces into vx, vz, explosions
+// into sxx, szz; one block per shot, taps grouped by (field, cell) in
+// source order.
 __global__ __launch_bounds__(1024) void k2e_inject(const ShotE2* __restrict__ shots, float* vx, float* vz,
-                                                   const int* __restrict__ inj_beg, const Tap* __restrict__ inj,
-                                                   const float* __restrict__ amp_tab, int nt, int t) {
+                                                   float* sxx, float* szz, const int* __restrict__ inj_beg,
+                                                   const Tap* __restrict__ inj, const float* __restrict__ amp_tab,
+                                                   int nt, int t) {
     const ShotE2 S = shots[blockIdx.x];
     for (int c = threadIdx.x; c < S.ninj; c += blockDim.x) {
         const int b = inj_beg[S.inj0 + c], e = inj_beg[S.inj0 + c + 1];
         const long long cell = inj[b].cell;
-        float* f = inj[b].field == 0 ? vx : vz;
+        const int fd = inj[b].field;
+        float* f = fd == 0 ? vx : fd == 1 ? vz : fd == 2 ? sxx : szz;
         float acc = f[cell];
         for (int k = b; k < e; ++k) {
             const Tap tp = inj[k];
@@ -662,22 +665,42 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         strip += strip2_len(H.nx, H.nz, R) * nt;
         S.g0 = grad;
         grad += (long long)H.nx * H.nz;
-        // force sources: vx at (hx, gz) += sin(polar) amp / rho(hx, gz) w,
-        // vz at (gx, hz) += cos(polar) amp / rho(gx, hz) w, amp = stf amp dt
+        // sources (add_source, module_wavefield.f90:294-403): a force adds
+        // sin(polar) amp / rho(hx, gz) w to vx at (hx, gz) and cos(polar)
+        // amp / rho(gx, hz) w to vz at (gx, hz); an explosion subtracts
+        // amp / (dx dz) w from sxx and szz at (gx, gz), its wavelet the time
+        // derivative (time_integration = -1); amp = stf amp dt
         const float xmax = float(H.nx - 1) * cfg->dx + H.ox, zmax = float(H.nz - 1) * cfg->dz + H.oz;
         std::vector<Tap> st;
         for (const owl_source& sc : H.src) {
-            if (sc.mechanism != OWL_FORCE) fail(OWL_ERR_ARG, "the elastic solver here takes force sources");
+            if (sc.mechanism != OWL_FORCE && sc.mechanism != OWL_EXPLOSION)
+                fail(OWL_ERR_ARG, "the elastic solver takes force and explosion sources");
             if (!(sc.x >= H.ox && sc.x <= xmax && sc.z >= H.oz && sc.z <= zmax))
                 fail(OWL_ERR_ARG, "source outside its shot window");
             const Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
             int hnt = 0;
-            const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            if (sc.mechanism == OWL_EXPLOSION) stf = stf_time_derivative(stf, cfg->dt, nt);
             if (&sc == &H.src[0]) {
                 H.hnt = hnt;
                 S.hnt = std::max(hnt, 1);
             }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
+            if (sc.mechanism == OWL_EXPLOSION) {
+                const int row = int(amp.size() / nt);
+                for (int t = 1; t <= nt; ++t) {
+                    const float a = (t >= nbeg && t <= nend) ? stf[t - nbeg] * sc.amp * cfg->dt : 0.0f;
+                    amp.push_back(-(a / (cfg->dx * cfg->dz)));
+                }
+                for (int c = 2; c < 4; ++c)
+                    for (int irz = -kNkw; irz <= kNkw; ++irz)
+                        for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                            const float w1 = L.ix[irx + kNkw], w2 = L.iz[irz + kNkw];
+                            if (w1 == 0.f || w2 == 0.f) continue;
+                            st.push_back(Tap{(long long)cell2(b, S, L.gx + irx, L.gz + irz), w1, w2, row, nbeg, nend, c});
+                        }
+                continue;
+            }
             const float rhx = b->rho_full[model_index2(b, H, L.hx, L.gz)];
             const float rhz = b->rho_full[model_index2(b, H, L.gx, L.hz)];
             const float sp = std::sin(sc.polar), cp = std::cos(sc.polar);
@@ -940,7 +963,7 @@ void run_forward2(owl_elastic2d* b, bool save_boundary) {
         if (t == b->nt) break;  // the last update changes nothing that is recorded
         step2<R>(b, F, out_of(F), b->dt, bs, bv);
         ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(k2e_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), F.vx, F.vz,
+        hipLaunchKernelGGL(k2e_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), F.vx, F.vz, F.sxx, F.szz,
                            b->d_inj_beg.get(), b->d_inj.get(), b->d_amp.get(), b->nt, t);
         b->prof.end(st, ev);
         maybe_collect(b);

@@ -76,7 +76,7 @@ struct Tap {
     float w1, w2, w3;
     int row;        // amplitude row (source) or receiver index
     int tbeg, tend;
-    int field;      // 0 vx, 1 vy, 2 vz
+    int field;      // 0 vx, 1 vy, 2 vz, 3 sxx, 4 syy, 5 szz
 };
 
 // One receiver component's Kaiser-sinc box: the separable weights along x,
@@ -519,8 +519,9 @@ __global__ void ke_surface(const ShotE* __restrict__ shots, EF F, int pml, int m
     }
 }
 
-// Force injection of step t (add_source, module_wavefield.f90:1115-1175);
-// one block per shot, taps grouped by (field, cell) in source order.
+// Source injection of step t (add_source, module_wavefield.f90:1115-1228):
+// forces into vx, vy, vz, explosions into sxx, syy, szz; one block per shot,
+// taps grouped by (field, cell) in source order.
 __global__ __launch_bounds__(1024) void ke_inject(const ShotE* __restrict__ shots, EF F,
                                                   const int* __restrict__ inj_beg, const Tap* __restrict__ inj,
                                                   const float* __restrict__ amp_tab, int nt, int t) {
@@ -529,7 +530,7 @@ __global__ __launch_bounds__(1024) void ke_inject(const ShotE* __restrict__ shot
         const int b = inj_beg[S.inj0 + c], e = inj_beg[S.inj0 + c + 1];
         const long long cell = inj[b].cell;
         const int fld = inj[b].field;
-        float* f = fld == 0 ? F.vx : (fld == 1 ? F.vy : F.vz);
+        float* f = fld == 0 ? F.vx : fld == 1 ? F.vy : fld == 2 ? F.vz : fld == 3 ? F.sxx : fld == 4 ? F.syy : F.szz;
         float acc = f[cell];
         for (int k = b; k < e; ++k) {
             const Tap tp = inj[k];
@@ -1045,14 +1046,18 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
         }
         S.g0 = grad;
         grad += (long long)H.nx * H.ny * H.nz;
-        // force sources; above the first node below a free surface they move
-        // to z = dz with amplitude x 1.2 (module_vars.f90:650-655)
+        // sources; above the first node below a free surface they move to
+        // z = dz with amplitude x 1.2 (module_vars.f90:650-655).  A force adds
+        // to vx, vy, vz with amp / rho(gx, gy, gz); an explosion subtracts
+        // amp / (dx dy dz) w from sxx, syy, szz at (gx, gy, gz), its wavelet
+        // the time derivative (time_integration = -1, :664); amp = stf amp dt
         const float xmax = float(H.nx - 1) * cfg->dx + H.ox, ymax = float(H.ny - 1) * cfg->dy + H.oy,
                     zmax = float(H.nz - 1) * cfg->dz + H.oz;
         std::vector<Tap> st;
         for (const owl_source& sc0 : H.src) {
             owl_source sc = sc0;
-            if (sc.mechanism != OWL_FORCE) fail(OWL_ERR_ARG, "the elastic solver here takes force sources");
+            if (sc.mechanism != OWL_FORCE && sc.mechanism != OWL_EXPLOSION)
+                fail(OWL_ERR_ARG, "the elastic solver takes force and explosion sources");
             if (b->fs && sc.z < cfg->dz) {
                 sc.z = cfg->dz;
                 sc.amp = sc.amp * 1.2f;
@@ -1061,12 +1066,31 @@ void plan_e(owl_elastic3d* b, const owl_acoustic_config* cfg, const float* vp, c
                 fail(OWL_ERR_ARG, "source outside its shot window");
             const Located L = locate(3, sc.x, sc.y, sc.z, H.ox, H.oy, H.oz, cfg->dx, cfg->dy, cfg->dz);
             int hnt = 0;
-            const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            if (sc.mechanism == OWL_EXPLOSION) stf = stf_time_derivative(stf, cfg->dt, nt);
             if (&sc0 == &H.src[0]) {
                 H.hnt = hnt;
                 S.hnt = std::max(hnt, 1);
             }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
+            if (sc.mechanism == OWL_EXPLOSION) {
+                const int row = int(amp.size() / nt);
+                for (int t = 1; t <= nt; ++t) {
+                    const float a = (t >= nbeg && t <= nend) ? stf[t - nbeg] * sc.amp * cfg->dt : 0.0f;
+                    amp.push_back(-(a / (cfg->dx * cfg->dy * cfg->dz)));
+                }
+                for (int c = 3; c < 6; ++c)
+                    for (int irz = -kNkw; irz <= kNkw; ++irz)
+                        for (int iry = -kNkw; iry <= kNkw; ++iry)
+                            for (int irx = -kNkw; irx <= kNkw; ++irx) {
+                                const float w1 = L.ix[irx + kNkw], w2 = L.iy[iry + kNkw], w3 = L.iz[irz + kNkw];
+                                if (w1 == 0.f || w2 == 0.f || w3 == 0.f) continue;
+                                if (b->fs && L.gz + irz < 2) continue;
+                                st.push_back(Tap{(long long)cell_e(b, S, L.gx + irx, L.gy + iry, L.gz + irz), w1, w2,
+                                                 w3, row, nbeg, nend, c});
+                            }
+                continue;
+            }
             const float rs = b->rho_full[model_index(b, H, L.gx, L.gy, L.gz)];
             const float sp = std::sin(sc.polar), cp = std::cos(sc.polar);
             const float sa = std::sin(sc.azimuth), ca = std::cos(sc.azimuth);

@@ -1131,7 +1131,8 @@ void planf(owl_fsg2d* b, const owl_acoustic_config* cfg, const float* const* cij
                 fail(OWL_ERR_ARG, "source outside its shot window");
             const float samp = sc.amp * ascale;
             int hnt = 0;
-            const std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
+            if (sc.mechanism == OWL_EXPLOSION) stf = stf_time_derivative(stf, cfg->dt, nt);
             if (&sc == &H.src[0]) {
                 H.hnt = hnt;
                 S.hnt = std::max(hnt, 1);

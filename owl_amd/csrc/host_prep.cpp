@@ -121,6 +121,23 @@ std::vector<float> source_time_function(int wavelet, float f0, float dt, float f
     return w;
 }
 
+std::vector<float> stf_time_derivative(const std::vector<float>& stf, float dt, int nt) {
+    const size_t n = stf.size();
+    std::vector<float> d(n, 0.0f);
+    if (n >= 2) {
+        d[0] = stf[1] - stf[0];
+        d[n - 1] = stf[n - 1] - stf[n - 2];
+        for (size_t i = 1; i + 1 < n; ++i) d[i] = 0.5f * (stf[i + 1] - stf[i - 1]);
+    }
+    float peak = 0.0f;
+    for (float& v : d) {
+        v = v / dt;
+        peak = std::fabs(v) > peak ? std::fabs(v) : peak;
+    }
+    if (n > 0 && peak > 0.0f && std::fabs(d[n - 1]) / peak > 0.1f && size_t(nt) > n) d.resize(size_t(nt), d[n - 1]);
+    return d;
+}
+
 float dt_stable(int R, int dims, float vmax, float dx, float dy, float dz) {
     float csum = 0.0f;
     for (int m = 0; m < R; ++m) csum = csum + std::fabs(kCoef[R][m]);

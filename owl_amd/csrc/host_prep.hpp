@@ -45,6 +45,15 @@ int stf_length(float f0, float dt, float f0factor);
 // Normalised source time function and hnt = nint(nt_stf/2.0) (:1034-1097).
 std::vector<float> source_time_function(int wavelet, float f0, float dt, float f0factor, int* hnt);
 
+// The wavelet of a stress-drop (explosion) source in the elastic solvers:
+// OWL sets time_integration = -1 (elastic_vhtiort_2d/module_vars.f90:476,
+// elastic_tti_2d/module_vars.f90:689) and prepare_stf then takes
+// deriv(stf)/dt (module_source_receiver.f90:1111-1124).  FLIT's deriv is
+// taken as centred inside, one-sided at the ends; when the last sample keeps
+// more than a tenth of the peak the wavelet is extended to nt samples with
+// its last value (pad_array, edge replication).
+std::vector<float> stf_time_derivative(const std::vector<float>& stf, float dt, int nt);
+
 // dt_stable = 1/(sum|c| * vmax * sqrt(sum 1/d^2)) (acoustic_iso_2d/module_vars.f90:121-125).
 float dt_stable(int R, int dims, float vmax, float dx, float dy, float dz);
 // nt = nint(tmax/dt + 1) (acoustic_iso_2d/module_vars.f90:134).

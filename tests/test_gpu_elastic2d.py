@@ -1,7 +1,7 @@
 """GPU parity of the 2D isotropic elastic path (C ABI -> HIP) against the
 oracle: two-component velocity traces bit-identical to the float32 oracle
 (same operation order, no FMA contraction) per shot for a batch with uneven
-windows and oblique force sources, for R = 2, 4, 8, independent of the
+windows, oblique force sources and an explosion, for R = 2, 4, 8, independent of the
 launch shape; OWL's gradient per shot window within 1e-4 relative L2 of the
 float32 oracle (the imaging accumulates in FP64 here), the merged gradient
 equal to the sum of the windows, and the misfit equal to the traces' L2
@@ -40,7 +40,7 @@ def _oracle_args(geom, g, dx):
     for r in geom.recr:
         ok = g["ox"] <= f(r.x) <= x1 and g["oz"] <= f(r.z) <= z1
         recs.append(dict(x=r.x, z=r.z, weight=r.weight if ok else 0.0))
-    srcs = [dict(x=s.x, z=s.z, f0=s.f0, amp=s.amp, polar=s.polar) for s in geom.srcr]
+    srcs = [dict(x=s.x, z=s.z, f0=s.f0, amp=s.amp, polar=s.polar, mechanism=s.mechanism) for s in geom.srcr]
     return srcs, recs
 
 
@@ -48,7 +48,8 @@ def _geoms():
     return [
         (Geometry(id=1, srcr=[Source(x=223.0, y=0.0, z=151.7, f0=20.0, mechanism="force", polar=0.5, amp=1e4)],
                   recr=[Receiver(x=20.0 + 9.0 * k, y=0.0, z=12.0 + 2.0 * k) for k in range(40)]), None),
-        (Geometry(id=2, srcr=[Source(x=350.0, y=0.0, z=45.0, f0=20.0, mechanism="force", polar=0.0, amp=1e4)],
+        (Geometry(id=2, srcr=[Source(x=350.0, y=0.0, z=45.0, f0=20.0, mechanism="force", polar=0.0, amp=1e4),
+                              Source(x=402.5, y=0.0, z=231.0, f0=16.0, mechanism="explosion", amp=1e6)],
                   recr=[Receiver(x=200.0 + 10.0 * k, y=0.0, z=20.0) for k in range(25)]),
          dict(nxbeg=15, nxend=60, nzbeg=1, nzend=40)),
         (Geometry(id=3, srcr=[Source(x=120.0, y=0.0, z=300.0, f0=20.0, mechanism="force", polar=2.0, amp=1e4)],

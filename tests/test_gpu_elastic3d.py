@@ -1,7 +1,7 @@
 """GPU parity of the 3D isotropic elastic path (C ABI -> HIP) against the
 oracle: three-component velocity traces bit-identical to the float32 oracle
 (same operation order, no FMA contraction) per shot, for a batch with
-uneven windows, oblique force sources, with and without a free surface, for
+uneven windows, oblique force sources and an explosion, with and without a free surface, for
 R = 2, 4, 8, independent of the launch shape."""
 import numpy as np
 import pytest
@@ -33,7 +33,8 @@ def _oracle(o, vp, vs, rho, win, geom, dx, dt, nt, pml, R, fs):
     for r in geom.recr:
         ok = g["ox"] <= f(r.x) <= x1 and g["oy"] <= f(r.y) <= y1 and g["oz"] <= f(r.z) <= z1
         recs.append(dict(x=r.x, y=r.y, z=r.z, weight=r.weight if ok else 0.0))
-    srcs = [dict(x=s.x, y=s.y, z=s.z, f0=s.f0, amp=s.amp, polar=s.polar, azimuth=s.azimuth) for s in geom.srcr]
+    srcs = [dict(x=s.x, y=s.y, z=s.z, f0=s.f0, amp=s.amp, polar=s.polar, azimuth=s.azimuth, mechanism=s.mechanism)
+            for s in geom.srcr]
     return o.elastic3d_forward(g, vp[sl], vs[sl], rho[sl], srcs, recs)
 
 
@@ -42,7 +43,8 @@ def _geoms():
         (Geometry(id=1, srcr=[Source(x=123.0, y=101.7, z=45.0, f0=20.0, mechanism="force", polar=0.5,
                                      azimuth=0.8, amp=1e4)],
                   recr=[Receiver(x=20.0 + 9.0 * k, y=60.5, z=12.0 + 2.0 * k) for k in range(22)]), None),
-        (Geometry(id=2, srcr=[Source(x=150.0, y=110.0, z=5.0, f0=20.0, mechanism="force", polar=0.0, amp=1e4)],
+        (Geometry(id=2, srcr=[Source(x=150.0, y=110.0, z=5.0, f0=20.0, mechanism="force", polar=0.0, amp=1e4),
+                              Source(x=171.3, y=128.0, z=82.5, f0=18.0, mechanism="explosion", amp=1e6)],
                   recr=[Receiver(x=100.0 + 10.0 * k, y=100.0, z=0.0) for k in range(12)]),
          dict(nxbeg=5, nxend=26, nybeg=3, nyend=22, nzbeg=1, nzend=20)),
     ]
@@ -90,7 +92,8 @@ def _oracle_grad(o, vp, vs, rho, win, geom, dx, dt, nt, pml, R, fs, adj):
     for r in geom.recr:
         ok = g["ox"] <= f(r.x) <= x1 and g["oy"] <= f(r.y) <= y1 and g["oz"] <= f(r.z) <= z1
         recs.append(dict(x=r.x, y=r.y, z=r.z, weight=r.weight if ok else 0.0))
-    srcs = [dict(x=s.x, y=s.y, z=s.z, f0=s.f0, amp=s.amp, polar=s.polar, azimuth=s.azimuth) for s in geom.srcr]
+    srcs = [dict(x=s.x, y=s.y, z=s.z, f0=s.f0, amp=s.amp, polar=s.polar, azimuth=s.azimuth, mechanism=s.mechanism)
+            for s in geom.srcr]
     return o.elastic3d_gradient_owl(g, vp[sl], vs[sl], rho[sl], srcs, recs, np.stack(adj))
 
 

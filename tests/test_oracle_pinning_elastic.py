@@ -6,6 +6,11 @@ elastic_vhtiort_3d restated) against facts that do not come from it:
   Aki & Richards eq. 4.23), for all three velocity components at four
   geometries, with the injection volume dx dy dz and a half-sample shift
   of the staggered velocity grid;
+* an explosion (stress drop on sxx, syy, szz with the time derivative of
+  the wavelet, OWL's time_integration = -1) against the closed-form P wave
+  of an isotropic point moment M(t), v_r = (M''/(a r) + M'/r^2) /
+  (4 pi rho a^2) at t - r/a, with M the moment the stress drops add up to,
+  1.5 samples late: within 2.5 % (measured 1.1 %);
 * the Rayleigh-wave speed along a free surface (root of the Rayleigh
   equation), from the cross-correlation moveout of vertical velocities at
   the surface: the free-surface mirrors and the traction condition set it;
@@ -73,6 +78,27 @@ def test_point_force_matches_stokes_solution(oracle32):
         num = np.concatenate([tx[k], ty[k], tz[k]])
         err = rel_l2(num, np.concatenate(ref))
         assert err < 0.02, (k, err)
+
+
+def test_explosion_matches_closed_form(oracle32):
+    from test_oracle_pinning_elastic2d import injected_moment_spectrum
+    n, d, vp, vs, rho, f0, dt, amp = 61, 10.0, 2000.0, 1155.0, 1000.0, 10.0, 1e-3, 1e6
+    g = dict(nx=n, ny=n, nz=n, dx=d, dy=d, dz=d, dt=dt, nt=400, pml=12, R=4)
+    c = (n // 2) * d
+    geo = [(150.0, 100.0, 40.0), (0.0, 0.0, 200.0), (-120.0, 60.0, -140.0), (170.0, 0.0, 0.0)]
+    m = lambda v: np.full((n, n, n), v, np.float32)  # noqa: E731
+    tx, ty, tz = oracle32.elastic3d_forward(g, m(vp), m(vs), m(rho),
+                                            [dict(x=c, y=c, z=c, f0=f0, amp=amp, mechanism="explosion")],
+                                            [dict(x=c + a, y=c + b, z=c + e) for a, b, e in geo])
+    npad = 8 * g["nt"]
+    S, w = injected_moment_spectrum(dt, f0, npad)
+    iw = 1j * w
+    for k, xr in enumerate(geo):
+        r = float(np.linalg.norm(xr))
+        V = (iw ** 2 / (vp * r) + iw / r ** 2) * np.exp(-1j * w * (r / vp + 1.5e-3)) * amp * S / (4 * np.pi * rho * vp ** 2)
+        ref = [np.fft.irfft(V * gi, n=npad)[:g["nt"]] / dt for gi in np.asarray(xr) / r]
+        err = rel_l2(np.concatenate([tx[k], ty[k], tz[k]]), np.concatenate(ref))
+        assert err < 0.025, (k, err)
 
 
 def _rayleigh_speed(vp, vs):

@@ -4,6 +4,10 @@ OWL's elastic_tti_2d restated) against facts that do not come from it:
 * an isotropic medium on the FSG reproduces the closed-form 2D
   elastodynamic Green's function of a line force (the same reference as the
   standard-staggered elastic pin), for vertical and horizontal forces;
+* an explosion (stress drop on both grids with the time derivative of the
+  wavelet, OWL's time_integration = -1) in an isotropic medium gives the
+  closed-form P wave of an isotropic line moment, as the standard-staggered
+  pin states it (within 1.5 %);
 * the float32 and float64 builds agree on a heterogeneous tilted-TI model
   with the M-PML damping ratios on, an oblique force and an explosion;
 * in a homogeneous tilted-TI medium (30 degree tilt) the qP moveout between
@@ -75,6 +79,22 @@ def test_isotropic_fsg_matches_green_function(oracle32):
             ref = _green2d_velocity(xr, axis, 1e3, d * d, vp, vs, rho, g["nt"], dt, f0, 0.5e-3)
             err = rel_l2(np.concatenate([tx[k], tz[k]]), np.concatenate(ref))
             assert err < 0.01, (polar, k, err)
+
+
+def test_isotropic_fsg_explosion_matches_p_wave(oracle32):
+    from test_oracle_pinning_elastic2d import explosion_p_wave_2d
+    n, d, vp, vs, rho, f0, dt, amp = 201, 10.0, 2000.0, 1155.0, 1000.0, 10.0, 1e-3, 1e6
+    g = dict(nx=n, nz=n, dx=d, dz=d, dt=dt, nt=500, pml=15, R=4, aniso=0)
+    c = (n // 2) * d
+    full = lambda v: np.full((n, n), v, np.float32)  # noqa: E731
+    geo = [(150.0, 100.0), (0.0, 200.0), (-120.0, -140.0), (170.0, 0.0)]
+    tx, tz = oracle32.fsg2d_forward(g, iso_cij(full(vp), full(vs), full(rho)), full(rho),
+                                    [dict(x=c, z=c, f0=f0, amp=amp, mechanism="explosion")],
+                                    [dict(x=c + a, z=c + e) for a, e in geo])
+    for k, xr in enumerate(geo):
+        ref = explosion_p_wave_2d(xr, amp, vp, rho, g["nt"], dt, f0, 1.5e-3)
+        err = rel_l2(np.concatenate([tx[k], tz[k]]), np.concatenate(ref))
+        assert err < 0.015, (k, err)
 
 
 def _tti_model(nz, nx, seed=3):
