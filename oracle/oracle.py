@@ -64,6 +64,10 @@ class EGrid2D(C.Structure):
                 ("pml", C.c_int), ("R", C.c_int), ("cc_step_interval", C.c_int)]
 
 
+class ESurf2D(C.Structure):
+    _fields_ = [("refine", C.c_float), ("dz_max", C.c_float), ("f0min", C.c_float)]
+
+
 class ESource2D(C.Structure):
     _fields_ = [("x", C.c_float), ("z", C.c_float), ("amp", C.c_float), ("t0", C.c_float),
                 ("polar", C.c_float), ("mechanism", C.c_int), ("nt_stf", C.c_int), ("stf", C.c_void_p)]
@@ -537,6 +541,77 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle elastic2d_forward failed ({rc})")
         return tr[0], tr[1]
+
+    def _esurf2(self, g, surface, sources):
+        dz = float(g.get("dz", g["dx"]))
+        return ESurf2D(float(surface.get("refine", 4.0)), float(surface.get("dz_max", 1.5 * dz)),
+                       float(min(s["f0"] for s in sources)))
+
+    def elastic2d_forward_fs(self, g, surface, vp, vs, rho, sources, receivers):
+        """elastic_vhtiort_2d compute_forward with free_surface on OWL's
+        refined mesh (surface: refine, dz_max): (tx, tz), each (nr, nt)."""
+        G = self._egrid2(g)
+        shape = (G.nz, G.nx)
+        vp, vs, rho = (self._model(a, shape) for a in (vp, vs, rho))
+        arr, keep, f0max, _ = self._esources2(g, sources)
+        rec = self._ereceivers2(receivers)
+        S = self._esurf2(g, surface, sources)
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("elastic2d_forward_fs")
+        fn.argtypes = [C.POINTER(EGrid2D), C.POINTER(ESurf2D), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                       C.c_void_p, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), C.byref(S), vp.ctypes.data_as(C.c_void_p), vs.ctypes.data_as(C.c_void_p),
+                rho.ctypes.data_as(C.c_void_p), len(sources), C.cast(arr, C.c_void_p), f0max, len(receivers),
+                C.cast(rec, C.c_void_p), tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle elastic2d_forward_fs failed ({rc})")
+        return tr[0], tr[1]
+
+    def elastic2d_gradient_owl_fs(self, g, surface, vp, vs, rho, sources, receivers, adj):
+        """OWL's 2D elastic gradient with the free surface: (grad_vp, grad_vs,
+        grad_rho) on the window's regular rows, plus the traces."""
+        G = self._egrid2(g)
+        shape = (G.nz, G.nx)
+        vp, vs, rho = (self._model(a, shape) for a in (vp, vs, rho))
+        arr, keep, f0max, hnt = self._esources2(g, sources)
+        rec = self._ereceivers2(receivers)
+        S = self._esurf2(g, surface, sources)
+        adj = np.ascontiguousarray(adj, dtype=self.real)
+        assert adj.shape == (2, len(receivers), G.nt)
+        n = G.nx * G.nz
+        gv, gs, gr = (np.zeros(n, self.real) for _ in range(3))
+        tr = np.zeros((2, len(receivers), G.nt), self.real)
+        fn = self._fn("elastic2d_gradient_owl_fs")
+        fn.argtypes = [C.POINTER(EGrid2D), C.POINTER(ESurf2D), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                       C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_void_p]
+        rc = fn(C.byref(G), C.byref(S), vp.ctypes.data_as(C.c_void_p), vs.ctypes.data_as(C.c_void_p),
+                rho.ctypes.data_as(C.c_void_p), len(sources), C.cast(arr, C.c_void_p), f0max,
+                int(g.get("hnt", hnt)), len(receivers), C.cast(rec, C.c_void_p), adj.ctypes.data_as(C.c_void_p),
+                gv.ctypes.data_as(C.c_void_p), gs.ctypes.data_as(C.c_void_p), gr.ctypes.data_as(C.c_void_p),
+                tr.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle elastic2d_gradient_owl_fs failed ({rc})")
+        out = tuple(a.reshape(G.nx, G.nz).T.copy() for a in (gv, gs, gr))
+        return out + ((tr[0], tr[1]),)
+
+    def elastic2d_mesh(self, g, surface, vs, rho, f0min, cap=100000):
+        """OWL's refined mesh of a window: dict nz, zi, zh, dzi, dzh over rows
+        -pml .. nz+pml+1."""
+        G = self._egrid2(g)
+        vs, rho = (self._model(a, (G.nz, G.nx)) for a in (vs, rho))
+        S = ESurf2D(float(surface.get("refine", 4.0)), float(surface.get("dz_max", 1.5 * g.get("dz", g["dx"]))),
+                    float(f0min))
+        bufs = [np.zeros(cap, np.float32) for _ in range(4)]
+        fn = self._fn("elastic2d_mesh")
+        fn.argtypes = [C.POINTER(EGrid2D), C.POINTER(ESurf2D), C.c_void_p, C.c_void_p, C.c_float, C.c_int] + \
+            [C.c_void_p] * 4
+        nz = fn(C.byref(G), C.byref(S), vs.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p), f0min, cap,
+                *[b.ctypes.data_as(C.c_void_p) for b in bufs])
+        if nz < 0:
+            raise RuntimeError(f"oracle elastic2d_mesh failed ({nz})")
+        n = nz + 2 * G.pml + 2
+        return dict(nz=nz, **{k: b[:n].copy() for k, b in zip(("zi", "zh", "dzi", "dzh"), bufs)})
 
     def elastic2d_gradient_owl(self, g, vp, vs, rho, sources, receivers, adj):
         """OWL's isotropic 2D elastic gradient: (grad_vp, grad_vs, grad_rho) as

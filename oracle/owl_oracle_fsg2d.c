@@ -892,6 +892,14 @@ static void flit_ginterp(const float *x, const float *y, int n, const float *xi,
     free(dp);
 }
 
+/* the same FLIT restatements for the 2D standard-staggered free surface
+ * (owl_oracle_elastic2d.c) */
+float *ORA_SYM(flit_regspace)(float a, float d, float b, int *n) { return flit_regspace(a, d, b, n); }
+void ORA_SYM(flit_deriv)(const float *x, int n, float *d) { flit_deriv(x, n, d); }
+void ORA_SYM(flit_ginterp)(const float *x, const float *y, int n, const float *xi, int m, float *yi, int cubic) {
+    flit_ginterp(x, y, n, xi, m, yi, cubic);
+}
+
 static void fsmesh_free(fsmesh *S) {
     float **p[11] = {&S->topo_i, &S->slope_i, &S->topo_h, &S->slope_h, &S->eta_i, &S->eta_h,
                      &S->edz_i,  &S->edz_h,   &S->sc_i,   &S->sc_h,    &S->zz_i};
