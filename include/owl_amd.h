@@ -461,13 +461,34 @@ owl_status owl_elastic3d_last_stats(const owl_elastic3d *b, double *ms, double *
  * boundary records module_boundary_saving.f90:53-160), for which_medium =
  * 'elastic-iso' in src/forward/2d/module_modeling.f90 and
  * src/inverse/2d/module_gradient.f90.  vp, vs, rho: (nz, nx) z fastest; cfg
- * as for the acoustic solvers with ny = 1 and no free surface (OWL's 2D
- * elastic free surface runs on its depth-refined mesh).  Force sources
- * (polar).  Traces: shot after shot, two blocks (vx, vz) of [nr][nt]. */
+ * as for the acoustic solvers with ny = 1; free_surface = 1 runs OWL's
+ * free surface on its depth-refined mesh (owl_elastic2d_create_surface).
+ * Force (polar) and explosion sources.  Traces: shot after shot, two blocks
+ * (vx, vz) of [nr][nt]. */
 typedef struct owl_elastic2d owl_elastic2d;
 owl_status owl_elastic2d_create(const owl_acoustic_config *cfg, const float *vp, const float *vs,
                                 const float *rho, int nshot, const owl_shot *shots,
                                 owl_elastic2d **out);
+/* OWL's 2D elastic free surface (elastic_vhtiort_2d with free_surface:
+ * module_vars.f90:310-371, module_wavefield.f90:143-289, module_cfspml.f90:
+ * 398-480): each shot steps on its depth-refined mesh (rows dz/dz_refine
+ * apart over half a Rayleigh wavelength, then growing by 2.5 % per row up
+ * to dz_max) with its medium mapped onto it; traction-free surface row,
+ * odd stress mirrors, even velocity images for the receivers, sources and
+ * adjoint sources on rows >= 2, sources above the first row moved to it
+ * with amplitude x 1.2, gradients mapped back to the regular window.  The
+ * surface is flat at z = oz of each window. */
+typedef struct owl_elastic_surface {
+    float dz_refine;  /* free_surface_dz_refine, OWL's default 4.0 (<= 0: 4.0) */
+    float dz_max;     /* largest z spacing; OWL's default 1.5 dz (<= 0: 1.5 dz) */
+} owl_elastic_surface;
+/* As owl_elastic2d_create with that free surface (cfg->free_surface = 1 in
+ * owl_elastic2d_create takes OWL's defaults). */
+owl_status owl_elastic2d_create_surface(const owl_acoustic_config *cfg, const float *vp, const float *vs,
+                                        const float *rho, int nshot, const owl_shot *shots,
+                                        const owl_elastic_surface *surface, owl_elastic2d **out);
+/* Rows shot `shot` steps (its window's nz, or the refined mesh's). */
+int owl_elastic2d_rows(const owl_elastic2d *b, int shot);
 void owl_elastic2d_destroy(owl_elastic2d *b);
 int owl_elastic2d_nt(const owl_elastic2d *b);
 /* save_boundary != 0 keeps the boundary records of every step and the final

@@ -64,6 +64,10 @@ class owl_shot(C.Structure):
                 ("nzbeg", C.c_int), ("nzend", C.c_int)]
 
 
+class owl_elastic_surface(C.Structure):
+    _fields_ = [("dz_refine", C.c_float), ("dz_max", C.c_float)]
+
+
 class owl_acoustic_config(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
@@ -131,6 +135,8 @@ _SIGNATURES = {
     "owl_acoustic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_acoustic3d_last_stats": (C.c_int, [_P, _P, _P]),
     "owl_elastic2d_create": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
+    "owl_elastic2d_create_surface": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P, _P]),
+    "owl_elastic2d_rows": (C.c_int, [_P, C.c_int]),
     "owl_elastic2d_destroy": (None, [_P]),
     "owl_elastic2d_nt": (C.c_int, [_P]),
     "owl_elastic2d_forward": (C.c_int, [_P, _P, C.c_int]),

@@ -3,9 +3,12 @@
 // anisotropy_type = 'iso' (module_wavefield.f90:42-134,
 // module_cfspml.f90:285-578, submodule_forward_modeling.f90:31-180,
 // module_boundary_saving.f90:53-160, submodule_fwi_gradient.f90) for a batch
-// of shots, each on its own window, in every launch.  Force sources,
-// two-component velocity receivers, CFS-MPML boundaries (zero damping
-// ratios for isotropic media, so per-axis profiles), no free surface.
+// of shots, each on its own window, in every launch.  Force and explosion
+// sources, two-component velocity receivers, CFS-MPML boundaries (zero
+// damping ratios for isotropic media, so per-axis profiles), and OWL's free
+// surface on its depth-refined mesh (update_wavefield_free_surface,
+// module_wavefield.f90:143-289; module_vars.f90:310-371) as an
+// instantiation of the two sweeps with per-row dz scalings.
 //
 // Layout: per shot a padded grid, x fastest, rows padded to a multiple of 64
 // floats.  Each sweep tiles x with 64-wide thread rows and marches along z
@@ -19,6 +22,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "fs_mesh.hpp"
 #include "host_prep.hpp"
 #include "owl_amd.h"
 #include "profiler.hpp"
@@ -46,6 +50,7 @@ struct ShotE2 {
     long long strip0;      // boundary records: nt blocks
     long long g0;          // window gradient accumulators (nx*nz, x fastest)
     int hnt;               // last reverse step OWL images for this shot (its first source's hnt)
+    int sc0;               // free surface: dz/dz_i, dz/dz_h per row j at [sc0 + j + pml], [sc0 + nrj + j + pml]
 };
 
 // Wavefields and memory variables (named by derivative and field as in OWL).
@@ -129,10 +134,11 @@ __device__ __forceinline__ float sten_lanes(float v, int l, int off, const float
 // rows by shuffles, so a wavefront of 64 lanes updates the middle 64 - 2R
 // columns and the others only feed them.
 // ---------------------------------------------------------------------------
-template <int R>
+template <int R, bool FS>
 __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ shots, const int* __restrict__ tile_shot,
-                                                  EF2 F, EV2 O, EM2 M, const float* __restrict__ prof, Coef c, float dt,
-                                                  float idx, float idz, int pml, int kc) {
+                                                  EF2 F, EV2 O, EM2 M, const float* __restrict__ prof,
+                                                  const float* __restrict__ scal, Coef c, float dt, float idx,
+                                                  float idz, int pml, int kc) {
     constexpr int W = 64 - 2 * R;
     // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
     // are wavefront-uniform and live in scalar registers
@@ -143,11 +149,14 @@ __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ sho
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const int l = int(threadIdx.x);
     const int i = -pml + 1 + tx * W - R + l;
-    const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
+    // with the free surface the rows start at the surface row 1
+    const int j0 = (FS ? 1 : -pml + 1) + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
     if (j0 > j1) return;  // padding tile beyond the shot's rows (whole wavefront)
     const bool mcol = i <= nx + pml + R;
     const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const Prof2 P = prof_of(prof, S, pml);
+    const float* __restrict__ sci = scal + S.sc0 + pml;
+    const float* __restrict__ sch = sci + (nz + 2 * pml + 2);
     const float* __restrict__ vx = F.vx;
     const float* __restrict__ vz = F.vz;
     const unsigned szb = unsigned(S.px) * 4u;
@@ -172,24 +181,31 @@ __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ sho
         if (own) {
             const int jj = j + pml - 1;
             {
-                float b = c.c[0] * (qz[R + 1] - qz[R]);
-#pragma unroll
-                for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qz[R + m] - qz[R - m + 1]);
-                float pdxvx = idx * sxvx, pdzvz = idz * b;
-                if (!xin || j <= 0 || j >= nz + 1) {
-                    pdxvx = mpml(F.m_dxvx, q, pdxvx, P.axi[ii], P.bxi[ii]);
-                    pdzvz = mpml(F.m_dzvz, q, pdzvz, P.azi[jj], P.bzi[jj]);
-                }
                 const float c11 = ldo(M.c11, q), c13 = ldo(M.c13, q);
+                float pdxvx = idx * sxvx, pdzvz;
+                if (FS && j == 1) {
+                    // the surface row: dz vz from the traction condition, x damping only
+                    if (!xin) pdxvx = mpml(F.m_dxvx, q, pdxvx, P.axi[ii], P.bxi[ii]);
+                    pdzvz = -(c13 / c11 * pdxvx);
+                } else {
+                    float b = c.c[0] * (qz[R + 1] - qz[R]);
+#pragma unroll
+                    for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qz[R + m] - qz[R - m + 1]);
+                    pdzvz = FS ? idz * b * sci[j] : idz * b;
+                    if (!xin || (!FS && j <= 0) || j >= nz + 1) {
+                        pdxvx = mpml(F.m_dxvx, q, pdxvx, P.axi[ii], P.bxi[ii]);
+                        pdzvz = mpml(F.m_dzvz, q, pdzvz, P.azi[jj], P.bzi[jj]);
+                    }
+                }
                 sto(O.sxx, q, ldo(F.sxx, q) + dt * (c11 * pdxvx + c13 * pdzvz));
                 sto(O.szz, q, ldo(F.szz, q) + dt * (c13 * pdxvx + c11 * pdzvz));
             }
-            {
+            if (!FS || j >= 2) {
                 float a = c.c[0] * (qx[R] - qx[R - 1]);        // d-z vx
 #pragma unroll
                 for (int m = 2; m <= R; ++m) a = a + c.c[m - 1] * (qx[R + m - 1] - qx[R - m]);
-                float pdzvx = idz * a, pdxvz = idx * sxvz;
-                if (!Xin || j <= 1 || j >= nz + 1) {
+                float pdzvx = FS ? idz * a * sch[j] : idz * a, pdxvz = idx * sxvz;
+                if (!Xin || (!FS && j <= 1) || j >= nz + 1) {
                     pdxvz = mpml(F.m_dxvz, q, pdxvz, P.axh[ii], P.bxh[ii]);
                     pdzvx = mpml(F.m_dzvx, q, pdzvx, P.azh[jj], P.bzh[jj]);
                 }
@@ -209,11 +225,11 @@ __global__ __launch_bounds__(256) void k2e_stress(const ShotE2* __restrict__ sho
 // (module_wavefield.f90:95-134).  z stencils from queues of sxz and szz, x
 // stencils of sxx and sxz by shuffles as in the stress sweep.
 // ---------------------------------------------------------------------------
-template <int R>
+template <int R, bool FS>
 __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ shots,
                                                     const int* __restrict__ tile_shot, EF2 F, EV2 O, EM2 M,
-                                                    const float* __restrict__ prof, Coef c, float dt, float idx,
-                                                    float idz, int pml, int kc) {
+                                                    const float* __restrict__ prof, const float* __restrict__ scal,
+                                                    Coef c, float dt, float idx, float idz, int pml, int kc) {
     constexpr int W = 64 - 2 * R;
     // one wavefront per tile (blockDim.x = 64): tile, shot and row indices
     // are wavefront-uniform and live in scalar registers
@@ -224,11 +240,13 @@ __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ s
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const int l = int(threadIdx.x);
     const int i = -pml + 1 + tx * W - R + l;
-    const int j0 = -pml + 1 + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
+    const int j0 = (FS ? 1 : -pml + 1) + tz * kc, j1 = min(j0 + kc - 1, nz + pml);
     if (j0 > j1) return;
     const bool mcol = i <= nx + pml + R;
     const bool own = l >= R && l <= 63 - R && i <= nx + pml;
     const Prof2 P = prof_of(prof, S, pml);
+    const float* __restrict__ sci = scal + S.sc0 + pml;
+    const float* __restrict__ sch = sci + (nz + 2 * pml + 2);
     const float* __restrict__ sxx = O.sxx;
     const float* __restrict__ szz = O.szz;
     const float* __restrict__ sxz = O.sxz;
@@ -258,19 +276,19 @@ __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ s
                 float b = c.c[0] * (qa[R + 1] - qa[R]);       // d+z sxz
 #pragma unroll
                 for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qa[R + m] - qa[R - m + 1]);
-                float pdxxx = idx * sxxx, pdzxz = idz * b;
-                if (!Xin || j <= 0 || j >= nz + 1) {
+                float pdxxx = idx * sxxx, pdzxz = FS ? idz * b * sci[j] : idz * b;
+                if (!Xin || (!FS && j <= 0) || j >= nz + 1) {
                     pdxxx = mpml(F.m_dxxx, q, pdxxx, P.axh[ii], P.bxh[ii]);
                     pdzxz = mpml(F.m_dzxz, q, pdzxz, P.azi[jj], P.bzi[jj]);
                 }
                 sto(O.vx, q, ldo(F.vx, q) + dt / ldo(M.rx, q) * (pdxxx + pdzxz));
             }
-            {
+            if (!FS || j >= 2) {
                 float b = c.c[0] * (qc[R] - qc[R - 1]);       // d-z szz
 #pragma unroll
                 for (int m = 2; m <= R; ++m) b = b + c.c[m - 1] * (qc[R + m - 1] - qc[R - m]);
-                float pdxxz = idx * sxxz, pdzzz = idz * b;
-                if (!xin || j <= 1 || j >= nz + 1) {
+                float pdxxz = idx * sxxz, pdzzz = FS ? idz * b * sch[j] : idz * b;
+                if (!xin || (!FS && j <= 1) || j >= nz + 1) {
                     pdxxz = mpml(F.m_dxxz, q, pdxxz, P.axi[ii], P.bxi[ii]);
                     pdzzz = mpml(F.m_dzzz, q, pdzzz, P.azh[jj], P.bzh[jj]);
                 }
@@ -282,6 +300,24 @@ __global__ __launch_bounds__(256) void k2e_velocity(const ShotE2* __restrict__ s
             qa[m] = qa[m + 1];
             qc[m] = qc[m + 1];
         }
+    }
+}
+
+// Free surface, after the stress sweep (module_wavefield.f90:223-245): the
+// odd stress mirrors szz(1-j) = -szz(1+j), sxx(1-j) = -sxx(1+j),
+// sxz(2-j) = -sxz(1+j), j = 1 .. R, and szz(1) = 0, over every stored
+// column.  The velocities above the surface stay zero throughout (no sweep,
+// source or record writes them), which is what OWL's update zeroes them to.
+__global__ void k2e_fs_mirror(const ShotE2* __restrict__ shots, float* sxx, float* szz, float* sxz, int pml, int R) {
+    const ShotE2 S = shots[blockIdx.y];
+    const int lo = 1 - pml - R, w = S.nx + 2 * (pml + R);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < w * R; e += gridDim.x * blockDim.x) {
+        const int i = lo + e % w, j = 1 + e / w;
+        const size_t up = at2(S, lo, i, 1 + j);
+        szz[at2(S, lo, i, 1 - j)] = -szz[up];
+        sxx[at2(S, lo, i, 1 - j)] = -sxx[up];
+        sxz[at2(S, lo, i, 2 - j)] = -sxz[up];
+        if (j == 1) szz[at2(S, lo, i, 1)] = 0.0f;
     }
 }
 
@@ -485,6 +521,10 @@ using namespace owl::e2;
 
 struct owl_elastic2d {
     owl_acoustic_config cfg{};
+    bool fs = false;                       // OWL's free surface on the depth-refined mesh
+    float fs_refine = 4.0f, fs_dzmax = 0.0f;
+    std::vector<owl::fsm::SsgMesh> mesh;   // per shot (free surface)
+    std::vector<std::vector<float>> rho_map;  // per shot: rho on the refined rows, column i (1..nx) at [(i-1) nrj + j + pml]
     int R = 4, pml = 15, nt = 0, kc = 32, cy = 4;
     float dt = 0, idx = 0, idz = 0;
     e2::Coef coef{};
@@ -502,7 +542,7 @@ struct owl_elastic2d {
     std::vector<float> vp_full, vs_full, rho_full;
     DevBuf<ShotE2> d_shots;
     DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg, d_adj_beg, d_win;
-    DevBuf<float> d_prof, d_amp, d_rec_w, d_traces, d_strips, d_adj, d_obs;
+    DevBuf<float> d_prof, d_amp, d_rec_w, d_traces, d_strips, d_adj, d_obs, d_sc;
     DevBuf<float> med[6];   // c11, c13, mu (at sxz), rho_x, rho_z, c55 at the nodes
     DevBuf<float> fld[13];  // five wavefields, eight memory variables
     DevBuf<float> fB[5];    // the other half of the reverse ping-pong
@@ -534,16 +574,18 @@ inline size_t model_index2(const owl_elastic2d* b, const ShotHostE2& H, int i, i
 // kmax = 1, for the distances |i-1|/pml, |i-1-0.5|/pml (i <= 1) and
 // |i-n|/pml, |i-n-0.5|/pml (i >= n) formed in double (:466-575); zero
 // elsewhere.  Appends ai, bi, ah, bh of n + 2 pml entries.
+void damp_ab(float pmlvp, float nd, float d, int pml, float adt, float alphamax, float& a, float& b) {
+    const float R0 = 1.0e-5f, kx = 1.0f;
+    const float damp = pmlvp * (2 + 1.0f) * std::log(1.0f / R0) / (2.0f * float(pml) * d) * (nd * nd);
+    const float alpha = std::max(alphamax * (1.0f - nd), 0.25f * alphamax);
+    a = -2.0f * adt * damp / kx / (2.0f + adt * (alpha + damp / kx));
+    b = (2.0f - adt * (alpha + damp / kx)) / (2.0f + adt * (alpha + damp / kx));
+}
+
 void damp_axis(int n, int pml, float d, float pmlvp, float adt, float alphamax, std::vector<float>& out) {
     const int len = n + 2 * pml;
     std::vector<float> ai(len, 0.f), bi(len, 0.f), ah(len, 0.f), bh(len, 0.f);
-    auto ab = [&](float nd, float& a, float& b) {
-        const float R0 = 1.0e-5f, kx = 1.0f;
-        const float damp = pmlvp * (2 + 1.0f) * std::log(1.0f / R0) / (2.0f * float(pml) * d) * (nd * nd);
-        const float alpha = std::max(alphamax * (1.0f - nd), 0.25f * alphamax);
-        a = -2.0f * adt * damp / kx / (2.0f + adt * (alpha + damp / kx));
-        b = (2.0f - adt * (alpha + damp / kx)) / (2.0f + adt * (alpha + damp / kx));
-    };
+    auto ab = [&](float nd, float& a, float& b) { damp_ab(pmlvp, nd, d, pml, adt, alphamax, a, b); };
     for (int i = -pml + 1; i <= n + pml; ++i) {
         float ndi, ndh;
         if (i <= 1) {
@@ -562,12 +604,37 @@ void damp_axis(int n, int pml, float d, float pmlvp, float adt, float alphamax, 
     for (auto* v : {&ai, &bi, &ah, &bh}) out.insert(out.end(), v->begin(), v->end());
 }
 
+// The z profiles on the refined mesh: row i takes d = dz_i(i) for every
+// grid (module_cfspml.f90:446-473); dzi at [i + pml].
+void damp_axis_rows(int n, int pml, const std::vector<float>& dzi, float pmlvp, float adt, float alphamax,
+                    std::vector<float>& out) {
+    const int len = n + 2 * pml;
+    std::vector<float> ai(len, 0.f), bi(len, 0.f), ah(len, 0.f), bh(len, 0.f);
+    for (int i = -pml + 1; i <= n + pml; ++i) {
+        float ndi, ndh;
+        if (i <= 1) {
+            ndi = float(std::fabs((i - 1 + 0.0) / pml));
+            ndh = float(std::fabs((i - 1 - 0.5) / pml));
+        } else if (i >= n) {
+            ndi = float(std::fabs((i - n + 0.0) / pml));
+            ndh = float(std::fabs((i - n - 0.5) / pml));
+        } else {
+            continue;
+        }
+        const float d = dzi[size_t(i + pml)];
+        const int q = i + pml - 1;
+        damp_ab(pmlvp, ndi, d, pml, adt, alphamax, ai[q], bi[q]);
+        damp_ab(pmlvp, ndh, d, pml, adt, alphamax, ah[q], bh[q]);
+    }
+    for (auto* v : {&ai, &bi, &ah, &bh}) out.insert(out.end(), v->begin(), v->end());
+}
+
 void retile2(owl_elastic2d* b) {
     b->tile_shot.clear();
     for (size_t s = 0; s < b->sd.size(); ++s) {
         ShotE2& S = b->sd[s];
         S.ntx = (S.nx + 2 * b->pml + (64 - 2 * b->R) - 1) / (64 - 2 * b->R);  // sweeps update 64 - 2R columns
-        S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
+        S.nzc = (S.nz + (b->fs ? 1 : 2) * b->pml + b->kc - 1) / b->kc;  // free surface: rows 1 .. nz+pml
         S.tile0 = int(b->tile_shot.size());
         b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nzc, int(s));
     }
@@ -577,13 +644,15 @@ void retile2(owl_elastic2d* b) {
 }
 
 void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, const float* vs, const float* rho,
-           int nshot, const owl_shot* shots) {
+           int nshot, const owl_shot* shots, const owl_elastic_surface* surf) {
     if (!cfg || !vp || !vs || !rho || nshot < 1 || !shots) fail(OWL_ERR_ARG, "null argument or no shots");
     if (!fd_coefficients(cfg->fdhalf, b->coef.c)) fail(OWL_ERR_ARG, "fdhalf must be 2..8");
     if (cfg->nx < 1 || cfg->nz < 1 || !(cfg->dx > 0) || !(cfg->dz > 0) || !(cfg->dt > 0))
         fail(OWL_ERR_ARG, "invalid grid or dt");
     if (cfg->pml < cfg->fdhalf || cfg->pml + cfg->fdhalf < kNkw) fail(OWL_ERR_ARG, "pml must be >= fdhalf");
-    if (cfg->free_surface) fail(OWL_ERR_ARG, "the 2D elastic solver here has no free surface");
+    b->fs = cfg->free_surface != 0 || surf != nullptr;
+    b->fs_refine = surf && surf->dz_refine > 0 ? surf->dz_refine : 4.0f;
+    b->fs_dzmax = surf && surf->dz_max > 0 ? surf->dz_max : 1.5f * cfg->dz;
     if (cfg->data_dt != cfg->dt || cfg->data_tmax != cfg->tmax)
         fail(OWL_ERR_ARG, "data_dt/data_tmax must equal dt/tmax (resample the traces with owl_resample_traces)");
     b->cfg = *cfg;
@@ -604,9 +673,15 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     b->sd.assign(nshot, ShotE2{});
     b->nrec_total = b->max_nr = b->max_nadj = 0;
     b->pts_full = 0;
+    b->mesh.assign(b->fs ? size_t(nshot) : 0, fsm::SsgMesh{});
+    b->rho_map.assign(b->fs ? size_t(nshot) : 0, std::vector<float>{});
+    // free surface: per shot the window's c11 (= c33), c13, c55 and rho
+    // mapped onto the refined rows, column i at [(i + pml) * nrj + j + pml]
+    // for i, j = -pml .. n+pml+1
+    std::vector<std::vector<float>> mapped(b->fs ? size_t(nshot) * 4 : 0);
     long long off = 0, trace = 0, strip = 0, grad = 0;
     int prof = 0;
-    std::vector<float> profs, amp, rec_w;
+    std::vector<float> profs, amp, rec_w, scal;
     std::vector<int> inj_beg{0}, adj_beg{0};
     std::vector<Tap> inj, adj_taps;
     std::vector<std::vector<Tap>> rec_taps;  // per (shot, component, receiver)
@@ -641,30 +716,103 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
                 if (i == 1 || i == H.nx || j == 1 || j == H.nz) pm = std::max(pm, c33 / r);
             }
         const float pmlvp = std::sqrt(pm);
-        const float dts = dt_stable(R, 2, vmax, cfg->dx, 1.0f, cfg->dz);
-        if (cfg->dt > dts)
-            fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " + std::to_string(dts));
-        for (auto& sc : H.src) f0max = std::max(f0max, sc.f0);
+        float f0min = INFINITY;
+        for (auto& sc : H.src) {
+            f0max = std::max(f0max, sc.f0);
+            f0min = std::min(f0min, sc.f0);
+        }
+        int nzr = H.nz;  // rows the shot steps (refined with the free surface)
+        if (b->fs) {
+            std::vector<float> wc(size_t(H.nx) * H.nz), wr(size_t(H.nx) * H.nz), wc13(wc.size()), wc55(wc.size());
+            for (int i = 1; i <= H.nx; ++i)
+                for (int j = 1; j <= H.nz; ++j) {
+                    const size_t m = model_index2(b, H, i, j), q = size_t(j - 1) + size_t(i - 1) * H.nz;
+                    const float r = b->rho_full[m], p = b->vp_full[m], v = b->vs_full[m];
+                    const float c33 = r * (p * p), c55 = r * (v * v);
+                    wc[q] = c33;
+                    wc13[q] = c33 - 2 * c55;
+                    wc55[q] = c55;
+                    wr[q] = r;
+                }
+            fsm::SsgMesh& Mh = b->mesh[size_t(s)];
+            Mh = fsm::build_ssg(H.nx, H.nz, cfg->dz, pml, b->fs_refine, b->fs_dzmax, wc55.data(), wr.data(), f0min);
+            nzr = Mh.nz;
+            const int nrj = nzr + 2 * pml + 2, nci = H.nx + 2 * pml + 2;
+            const std::vector<float>* src4[4] = {&wc, &wc13, &wc55, &wr};
+            for (int k = 0; k < 4; ++k) {
+                auto& out = mapped[size_t(s) * 4 + k];
+                out.resize(size_t(nci) * nrj);
+                for (int i = -pml; i <= H.nx + pml + 1; ++i)
+                    fsm::map_column_ssg(Mh, H.nx, H.nz, src4[k]->data(), i, out.data() + size_t(i + pml) * nrj);
+            }
+            // stability on the refined mesh (module_vars.f90:383-408)
+            float csum = 0.0f;
+            for (int m = 0; m < R; ++m) csum = csum + std::fabs(b->coef.c[m]);
+            float dts = INFINITY;
+            const auto& mc = mapped[size_t(s) * 4], &mr = mapped[size_t(s) * 4 + 3];
+            for (int j = 1; j <= nzr; ++j) {
+                const float dmin = std::min(Mh.dzi[size_t(j + pml)], Mh.dzh[size_t(j + pml)]);
+                for (int i = 1; i <= H.nx; ++i) {
+                    const size_t q = size_t(i + pml) * nrj + size_t(j + pml);
+                    const float maxv = std::sqrt(mc[q] / mr[q]);  // c33 = c11 here
+                    dts = std::min(dts, 1.0f / (csum * maxv *
+                                               std::sqrt(1.0f / (cfg->dx * cfg->dx) + 1.0f / (dmin * dmin))));
+                }
+            }
+            if (cfg->dt > dts)
+                fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " +
+                                      std::to_string(dts) + " of the free-surface mesh");
+            auto& rm = b->rho_map[size_t(s)];
+            rm.resize(size_t(H.nx) * nrj);
+            for (int i = 1; i <= H.nx; ++i)
+                std::copy(mr.begin() + long(size_t(i + pml) * nrj), mr.begin() + long(size_t(i + pml + 1) * nrj),
+                          rm.begin() + long(size_t(i - 1) * nrj));
+            S.sc0 = int(scal.size());
+            scal.insert(scal.end(), Mh.sci.begin(), Mh.sci.end());
+            scal.insert(scal.end(), Mh.sch.begin(), Mh.sch.end());
+        } else {
+            const float dts = dt_stable(R, 2, vmax, cfg->dx, 1.0f, cfg->dz);
+            if (cfg->dt > dts)
+                fail(OWL_ERR_CFL, "dt = " + std::to_string(cfg->dt) + " exceeds the stable dt = " + std::to_string(dts));
+            S.sc0 = 0;
+        }
         S.nx = H.nx;
-        S.nz = H.nz;
+        S.nz = nzr;
         S.px = (H.nx + 2 * (pml + R) + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
-        S.nzp = H.nz + 2 * (pml + R);
+        S.nzp = nzr + 2 * (pml + R);
         S.off = off;
         off += (long long)S.px * S.nzp;
         off = (off + 63) / 64 * 64;
-        b->pts_full += double(H.nx + 2 * pml) * double(H.nz + 2 * pml);
+        b->pts_full += double(H.nx + 2 * pml) * double(nzr + (b->fs ? 1 : 2) * pml);
         S.prof = prof;
         const float amax = float(double(1.0f * f0max) * M_PI), adt = std::fabs(cfg->dt);
         damp_axis(H.nx, pml, cfg->dx, pmlvp, adt, amax, profs);
-        damp_axis(H.nz, pml, cfg->dz, pmlvp, adt, amax, profs);
-        prof += 4 * (H.nx + 2 * pml) + 4 * (H.nz + 2 * pml);
+        if (b->fs)
+            damp_axis_rows(nzr, pml, b->mesh[size_t(s)].dzi, pmlvp, adt, amax, profs);
+        else
+            damp_axis(H.nz, pml, cfg->dz, pmlvp, adt, amax, profs);
+        prof += 4 * (H.nx + 2 * pml) + 4 * (nzr + 2 * pml);
         S.nr = int(H.rec.size());
         S.trace0 = trace;
         trace += 2LL * nt * S.nr;
         S.strip0 = strip;
-        strip += strip2_len(H.nx, H.nz, R) * nt;
+        strip += strip2_len(H.nx, nzr, R) * nt;
         S.g0 = grad;
-        grad += (long long)H.nx * H.nz;
+        grad += (long long)H.nx * nzr;
+        const fsm::SsgMesh* Ms = b->fs ? &b->mesh[size_t(s)] : nullptr;
+        // a point's location: x on the regular mesh; z on the regular or the
+        // refined rows
+        auto place = [&](float x, float z) {
+            Located L = locate(2, x, 0.0f, z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            if (Ms) fsm::locate_ssg(*Ms, z, H.oz, L.gz, L.hz, L.iz, L.hzw);
+            return L;
+        };
+        // the mapped density at refined node (i, j) (free surface)
+        auto rho_at = [&](int i, int j) {
+            const int nrj = nzr + 2 * pml + 2;
+            const int ic = std::min(std::max(i, -pml), H.nx + pml + 1), jc = std::min(std::max(j, -pml), nzr + pml + 1);
+            return mapped[size_t(s) * 4 + 3][size_t(ic + pml) * nrj + size_t(jc + pml)];
+        };
         // sources (add_source, module_wavefield.f90:294-403): a force adds
         // sin(polar) amp / rho(hx, gz) w to vx at (hx, gz) and cos(polar)
         // amp / rho(gx, hz) w to vz at (gx, hz); an explosion subtracts
@@ -672,37 +820,47 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         // derivative (time_integration = -1); amp = stf amp dt
         const float xmax = float(H.nx - 1) * cfg->dx + H.ox, zmax = float(H.nz - 1) * cfg->dz + H.oz;
         std::vector<Tap> st;
-        for (const owl_source& sc : H.src) {
+        for (const owl_source& sc0 : H.src) {
+            owl_source sc = sc0;
             if (sc.mechanism != OWL_FORCE && sc.mechanism != OWL_EXPLOSION)
                 fail(OWL_ERR_ARG, "the elastic solver takes force and explosion sources");
+            // with the free surface a source above dz_i(1) moves to it with
+            // its amplitude x 1.2 (module_vars.f90:459-466)
+            if (Ms && sc.z < Ms->dzi[size_t(1 + pml)]) {
+                sc.z = Ms->dzi[size_t(1 + pml)];
+                sc.amp = sc.amp * 1.2f;
+            }
             if (!(sc.x >= H.ox && sc.x <= xmax && sc.z >= H.oz && sc.z <= zmax))
                 fail(OWL_ERR_ARG, "source outside its shot window");
-            const Located L = locate(2, sc.x, 0.0f, sc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            const Located L = place(sc.x, sc.z);
             int hnt = 0;
             std::vector<float> stf = source_time_function(sc.wavelet, sc.f0, cfg->dt, f0factor, &hnt);
             if (sc.mechanism == OWL_EXPLOSION) stf = stf_time_derivative(stf, cfg->dt, nt);
-            if (&sc == &H.src[0]) {
+            if (&sc0 == &H.src[0]) {
                 H.hnt = hnt;
                 S.hnt = std::max(hnt, 1);
             }
             const int nbeg = fortran_nint(sc.t0 / cfg->dt) + 1, nend = nbeg + int(stf.size()) - 1;
             if (sc.mechanism == OWL_EXPLOSION) {
+                // the stress drop divides by the local spacing dz_i(gz) on the refined mesh
+                const float dzs = Ms ? Ms->dzi[size_t(L.gz + pml)] : cfg->dz;
                 const int row = int(amp.size() / nt);
                 for (int t = 1; t <= nt; ++t) {
                     const float a = (t >= nbeg && t <= nend) ? stf[t - nbeg] * sc.amp * cfg->dt : 0.0f;
-                    amp.push_back(-(a / (cfg->dx * cfg->dz)));
+                    amp.push_back(-(a / (cfg->dx * dzs)));
                 }
                 for (int c = 2; c < 4; ++c)
                     for (int irz = -kNkw; irz <= kNkw; ++irz)
                         for (int irx = -kNkw; irx <= kNkw; ++irx) {
                             const float w1 = L.ix[irx + kNkw], w2 = L.iz[irz + kNkw];
                             if (w1 == 0.f || w2 == 0.f) continue;
+                            if (Ms && L.gz + irz < 2) continue;  // rows >= 2 take a source
                             st.push_back(Tap{(long long)cell2(b, S, L.gx + irx, L.gz + irz), w1, w2, row, nbeg, nend, c});
                         }
                 continue;
             }
-            const float rhx = b->rho_full[model_index2(b, H, L.hx, L.gz)];
-            const float rhz = b->rho_full[model_index2(b, H, L.gx, L.hz)];
+            const float rhx = Ms ? rho_at(L.hx, L.gz) : b->rho_full[model_index2(b, H, L.hx, L.gz)];
+            const float rhz = Ms ? rho_at(L.gx, L.hz) : b->rho_full[model_index2(b, H, L.gx, L.hz)];
             const float sp = std::sin(sc.polar), cp = std::cos(sc.polar);
             const int gx[2] = {L.hx, L.gx}, gz[2] = {L.gz, L.hz};
             const float* wx[2] = {L.hxw, L.ix};
@@ -718,6 +876,7 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
                     for (int irx = -kNkw; irx <= kNkw; ++irx) {
                         const float w1 = wx[c][irx + kNkw], w2 = wz[c][irz + kNkw];
                         if (w1 == 0.f || w2 == 0.f) continue;
+                        if (Ms && gz[c] + irz < 2) continue;
                         st.push_back(Tap{(long long)cell2(b, S, gx[c] + irx, gz[c] + irz), w1, w2, row, nbeg, nend, c});
                     }
             }
@@ -736,15 +895,19 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
         }
         if (!st.empty()) inj_beg.push_back(int(inj.size()));
         // receivers: vx at (hx, gz), vz at (gx, hz) (outside the window: weight 0)
+        // With the free surface a recorded tap above it reads its even image
+        // (vx(1-j) = vx(1+j), vz(2-j) = vz(1+j), submodule_forward_modeling
+        // .f90:67-75), and the adjoint sources skip the rows above 2
+        // (submodule_fwi_gradient.f90:223, 244).
         S.rec0 = b->nrec_total;
-        std::vector<std::vector<Tap>> comp(2 * S.nr);
+        std::vector<std::vector<Tap>> comp(2 * S.nr), acomp(2 * S.nr);
         for (int r = 0; r < S.nr; ++r) {
             const owl_receiver& rc = H.rec[r];
             const bool valid = rc.x >= H.ox && rc.x <= xmax && rc.z >= H.oz && rc.z <= zmax;
             const float w = (valid && rc.weight != 0.f) ? rc.weight : 0.f;
             rec_w.push_back(w);
             if (w == 0.f) continue;
-            const Located L = locate(2, rc.x, 0.0f, rc.z, H.ox, 0.0f, H.oz, cfg->dx, 1.0f, cfg->dz);
+            const Located L = place(rc.x, rc.z);
             const int gx[2] = {L.hx, L.gx}, gz[2] = {L.gz, L.hz};
             const float* wx[2] = {L.hxw, L.ix};
             const float* wz[2] = {L.iz, L.hzw};
@@ -753,15 +916,24 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
                     for (int irx = -kNkw; irx <= kNkw; ++irx) {
                         const float w1 = wx[c][irx + kNkw], w2 = wz[c][irz + kNkw];
                         if (w1 == 0.f || w2 == 0.f) continue;
+                        int row = gz[c] + irz;
+                        const Tap ta{(long long)cell2(b, S, gx[c] + irx, row), w1, w2, r, 0, 0, c};
+                        if (Ms) {
+                            if (row >= 2) acomp[size_t(c) * S.nr + r].push_back(ta);
+                            if (c == 0 && row < 1 && row >= 1 - R) row = 2 - row;  // rows above 1-R stay zero
+                            if (c == 1 && row < 2 && row >= 2 - R) row = 3 - row;
+                        } else {
+                            acomp[size_t(c) * S.nr + r].push_back(ta);
+                        }
                         comp[size_t(c) * S.nr + r].push_back(
-                            Tap{(long long)cell2(b, S, gx[c] + irx, gz[c] + irz), w1, w2, r, 0, 0, c});
+                            Tap{(long long)cell2(b, S, gx[c] + irx, row), w1, w2, r, 0, 0, c});
                     }
         }
         {
             std::vector<Tap> at;
             for (int c = 0; c < 2; ++c)
                 for (int r = 0; r < S.nr; ++r)
-                    for (const Tap& tp : comp[size_t(c) * S.nr + r]) at.push_back(tp);
+                    for (const Tap& tp : acomp[size_t(c) * S.nr + r]) at.push_back(tp);
             std::stable_sort(at.begin(), at.end(), [](const Tap& x, const Tap& y) {
                 return x.field != y.field ? x.field < y.field : x.cell < y.cell;
             });
@@ -827,18 +999,32 @@ void plan2(owl_elastic2d* b, const owl_acoustic_config* cfg, const float* vp, co
     for (size_t s = 0; s < b->hs.size(); ++s) {
         const ShotHostE2& H = b->hs[s];
         const ShotE2& S = b->sd[s];
+        const int nrj = S.nz + 2 * pml + 2;
         for (int j = lo; j < lo + S.nzp; ++j)
             for (int i = lo; i < lo + S.px; ++i) {
+                const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * S.px;
+                if (b->fs) {
+                    // the mapped columns, edge values beyond the padded range
+                    const int ic = std::min(std::max(i, -pml), H.nx + pml + 1);
+                    const int jc = std::min(std::max(j, -pml), S.nz + pml + 1);
+                    const size_t k = size_t(ic + pml) * nrj + size_t(jc + pml);
+                    c11[q] = mapped[s * 4][k];
+                    c13[q] = mapped[s * 4 + 1][k];
+                    c55[q] = mapped[s * 4 + 2][k];
+                    rh[q] = mapped[s * 4 + 3][k];
+                    continue;
+                }
                 const size_t m = model_index2(b, H, i, j);
                 const float r = b->rho_full[m], p = b->vp_full[m], v = b->vs_full[m];
                 const float a = r * (p * p), mu = r * (v * v);
-                const size_t q = size_t(S.off) + size_t(i - lo) + size_t(j - lo) * S.px;
                 c11[q] = a;
                 c13[q] = a - 2 * mu;
                 c55[q] = mu;
                 rh[q] = r;
             }
     }
+    if (scal.empty()) scal.push_back(1.0f);
+    b->d_sc.upload(scal, stm);
     std::vector<float> mxz(n, 0.f), rx(n, 1.f), rz(n, 1.f);
     for (size_t s = 0; s < b->sd.size(); ++s) {
         const ShotE2& S = b->sd[s];
@@ -882,7 +1068,7 @@ EM2 media2(owl_elastic2d* b) {
 void sweep_bytes2(const owl_elastic2d* b, double& bs, double& bv) {
     bs = bv = 0;
     for (auto& S : b->sd) {
-        const double lx = S.nx + 2.0 * b->pml, lz = S.nz + 2.0 * b->pml;
+        const double lx = S.nx + 2.0 * b->pml, lz = S.nz + (b->fs ? 1.0 : 2.0) * b->pml;
         const double slab = 2.0 * b->pml * (lx + lz);
         bs += 44.0 * lx * lz + 8.0 * 2.0 * slab;
         bv += 36.0 * lx * lz + 8.0 * 2.0 * slab;
@@ -919,12 +1105,28 @@ void step2(owl_elastic2d* b, const EF2& F, const EV2& O, float dt, double bs, do
     const dim3 blk(64, unsigned(b->cy)), grd(unsigned(b->tile_shot.size() / size_t(b->cy)));
     const EM2 M = media2(b);
     int ev = b->prof.begin(st, 0, bs);
-    hipLaunchKernelGGL(k2e_stress<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
-                       b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, b->kc);
+    if (b->fs)
+        hipLaunchKernelGGL((k2e_stress<R, true>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
+                           b->d_prof.get(), b->d_sc.get(), b->coef, dt, b->idx, b->idz, b->pml, b->kc);
+    else
+        hipLaunchKernelGGL((k2e_stress<R, false>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
+                           b->d_prof.get(), b->d_sc.get(), b->coef, dt, b->idx, b->idz, b->pml, b->kc);
     b->prof.end(st, ev);
+    if (b->fs) {
+        ev = b->prof.begin(st, 5, 0);
+        int w = 1;
+        for (auto& S : b->sd) w = std::max(w, (S.nx + 2 * (b->pml + b->R)) * b->R);
+        hipLaunchKernelGGL(k2e_fs_mirror, dim3(unsigned((w + 255) / 256), unsigned(b->sd.size())), dim3(256), 0, st,
+                           b->d_shots.get(), O.sxx, O.szz, O.sxz, b->pml, b->R);
+        b->prof.end(st, ev);
+    }
     ev = b->prof.begin(st, 1, bv);
-    hipLaunchKernelGGL(k2e_velocity<R>, grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O, M,
-                       b->d_prof.get(), b->coef, dt, b->idx, b->idz, b->pml, b->kc);
+    if (b->fs)
+        hipLaunchKernelGGL((k2e_velocity<R, true>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O,
+                           M, b->d_prof.get(), b->d_sc.get(), b->coef, dt, b->idx, b->idz, b->pml, b->kc);
+    else
+        hipLaunchKernelGGL((k2e_velocity<R, false>), grd, blk, 0, st, b->d_shots.get(), b->d_tile_shot.get(), F, O,
+                           M, b->d_prof.get(), b->d_sc.get(), b->coef, dt, b->idx, b->idz, b->pml, b->kc);
     b->prof.end(st, ev);
 }
 
@@ -1091,12 +1293,53 @@ void traces_to_device2(owl_elastic2d* b, const float* in, DevBuf<float>& dev) {
 
 // Chain rule (submodule_fwi_gradient.f90:330-360) with vp, vs in km/s and
 // rho in g/cm^3: grad_vp = 2 rho vp A, grad_vs = 2 rho vs B, grad_rho = C.
-inline void chain_rule2(float vp, float vs, float rho, double A, double B, double C, double& gv, double& gs,
-                        double& gr) {
-    const double r = double(rho * 1.0e-3f), p = double(vp * 1.0e-3f), v = double(vs * 1.0e-3f);
+inline void chain_rule2r(float vp, float vs, double r, double A, double B, double C, double& gv, double& gs,
+                         double& gr) {
+    const double p = double(vp * 1.0e-3f), v = double(vs * 1.0e-3f);
     gv = 2.0 * r * p * A;
     gs = 2.0 * r * v * B;
     gr = C;
+}
+inline void chain_rule2(float vp, float vs, float rho, double A, double B, double C, double& gv, double& gs,
+                        double& gr) {
+    chain_rule2r(vp, vs, double(rho * 1.0e-3f), A, B, C, gv, gs, gr);
+}
+
+// Free surface: shot s's accumulators on its refined rows to the window's
+// rows (map_irregular_to_regular, submodule_fwi_gradient.f90:324-338): A,
+// B, C column by column through z_i(1 .. nz), and rho x 1e-3 through all
+// refined rows -pml .. nz+pml+1; then the chain rule with the window's vp,
+// vs.  Outputs (nz_w, nx_w) z fastest.
+void window_grads_fs(const owl_elastic2d* b, size_t s, const double* gA, const double* gB, const double* gC,
+                     std::vector<double>& gv, std::vector<double>& gs, std::vector<double>& gr) {
+    const ShotE2& S = b->sd[s];
+    const ShotHostE2& H = b->hs[s];
+    const fsm::SsgMesh& M = b->mesh[s];
+    const int nx = H.nx, nz0 = H.nz, nzr = S.nz, pml = b->pml, nrj = nzr + 2 * pml + 2;
+    gv.assign(size_t(nx) * nz0, 0.0);
+    gs.assign(gv.size(), 0.0);
+    gr.assign(gv.size(), 0.0);
+    const size_t nr_ = static_cast<size_t>(nzr), n0 = static_cast<size_t>(nz0), nj = static_cast<size_t>(nrj);
+    std::vector<double> ca(nr_), cb(nr_), cc(nr_), ma(n0), mb(n0), mc(n0), rr(nj), mr(n0);
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 1; j <= nzr; ++j) {
+            const size_t o = size_t(S.g0) + size_t(i - 1) + size_t(j - 1) * nx;
+            ca[size_t(j - 1)] = gA[o];
+            cb[size_t(j - 1)] = gB[o];
+            cc[size_t(j - 1)] = gC[o];
+        }
+        fsm::column_to_regular_ssg(M, nz0, ca.data(), ma.data());
+        fsm::column_to_regular_ssg(M, nz0, cb.data(), mb.data());
+        fsm::column_to_regular_ssg(M, nz0, cc.data(), mc.data());
+        const std::vector<float>& rm = b->rho_map[s];
+        for (int k = 0; k < nrj; ++k) rr[size_t(k)] = double(rm[size_t(i - 1) * nrj + k] * 1.0e-3f);
+        fsm::spline_to_regular(M.zi.data(), rr.data(), nrj, M.dz, nz0, mr.data());
+        for (int k = 1; k <= nz0; ++k) {
+            const size_t m = model_index2(b, H, i, k), o = size_t(k - 1) + size_t(i - 1) * nz0;
+            chain_rule2r(b->vp_full[m], b->vs_full[m], double(float(mr[size_t(k - 1)])), ma[size_t(k - 1)],
+                         mb[size_t(k - 1)], mc[size_t(k - 1)], gv[o], gs[o], gr[o]);
+        }
+    }
 }
 
 }  // namespace
@@ -1105,18 +1348,29 @@ extern "C" {
 
 owl_status owl_elastic2d_create(const owl_acoustic_config* cfg, const float* vp, const float* vs, const float* rho,
                                 int nshot, const owl_shot* shots, owl_elastic2d** out) {
+    return owl_elastic2d_create_surface(cfg, vp, vs, rho, nshot, shots, nullptr, out);
+}
+
+owl_status owl_elastic2d_create_surface(const owl_acoustic_config* cfg, const float* vp, const float* vs,
+                                        const float* rho, int nshot, const owl_shot* shots,
+                                        const owl_elastic_surface* surface, owl_elastic2d** out) {
     return guarded([&] {
         if (!out) fail(OWL_ERR_ARG, "out is null");
         *out = nullptr;
         auto* b = new owl_elastic2d();
         try {
-            plan2(b, cfg, vp, vs, rho, nshot, shots);
+            plan2(b, cfg, vp, vs, rho, nshot, shots, surface);
         } catch (...) {
             delete b;
             throw;
         }
         *out = b;
     });
+}
+
+int owl_elastic2d_rows(const owl_elastic2d* b, int shot) {
+    if (!b || shot < 0 || size_t(shot) >= b->sd.size()) return -1;
+    return b->sd[size_t(shot)].nz;
 }
 
 void owl_elastic2d_destroy(owl_elastic2d* b) { delete b; }
@@ -1142,9 +1396,23 @@ owl_status owl_elastic2d_adjoint(owl_elastic2d* b, const float* adj, float* grad
         OWL_HIP(hipMemcpy(gB.data(), b->d_gB.get(), gB.size() * sizeof(double), hipMemcpyDeviceToHost));
         OWL_HIP(hipMemcpy(gC.data(), b->d_gC.get(), gC.size() * sizeof(double), hipMemcpyDeviceToHost));
         // per-shot window gradients (nz_s, nx_s), z fastest, NaN/Inf -> 0
+        size_t wo = 0;
         for (size_t s = 0; s < b->sd.size(); ++s) {
             const ShotE2& S = b->sd[s];
             const ShotHostE2& H = b->hs[s];
+            if (b->fs) {
+                std::vector<double> gv, gs, gr;
+                window_grads_fs(b, s, gA.data(), gB.data(), gC.data(), gv, gs, gr);
+                for (size_t e = 0; e < gv.size(); ++e) {
+                    const float a = float(gv[e]), c = float(gs[e]), f = float(gr[e]);
+                    if (grad_vp) grad_vp[wo + e] = std::isfinite(a) ? a : 0.f;
+                    if (grad_vs) grad_vs[wo + e] = std::isfinite(c) ? c : 0.f;
+                    if (grad_rho) grad_rho[wo + e] = std::isfinite(f) ? f : 0.f;
+                }
+                wo += gv.size();
+                continue;
+            }
+            wo += size_t(S.nx) * S.nz;
             for (int k = 0; k < S.nz; ++k)
                 for (int i = 0; i < S.nx; ++i) {
                     const size_t src = size_t(S.g0) + i + size_t(k) * S.nx;
@@ -1182,11 +1450,13 @@ owl_status owl_elastic2d_gradient(owl_elastic2d* b, const float* obs, double* mi
                            b->d_scratch.get());
         dispatch_R2e(b->R, [&](auto r) { run_reverse2<decltype(r)::value>(b); });
         const size_t nfull = size_t(b->cfg.nx) * b->cfg.nz;
-        for (auto* g : {&b->d_fA, &b->d_fB, &b->d_fC}) g->alloc(nfull);
-        const unsigned mg = unsigned(std::min<size_t>(8192, (nfull + 255) / 256));
-        hipLaunchKernelGGL(k2e_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), b->d_win.get(), int(nshot),
-                           b->d_gA.get(), b->d_gB.get(), b->d_gC.get(), b->d_fA.get(), b->d_fB.get(), b->d_fC.get(),
-                           b->cfg.nx, b->cfg.nz);
+        if (!b->fs) {
+            for (auto* g : {&b->d_fA, &b->d_fB, &b->d_fC}) g->alloc(nfull);
+            const unsigned mg = unsigned(std::min<size_t>(8192, (nfull + 255) / 256));
+            hipLaunchKernelGGL(k2e_merge, dim3(mg), dim3(256), 0, st, b->d_shots.get(), b->d_win.get(), int(nshot),
+                               b->d_gA.get(), b->d_gB.get(), b->d_gC.get(), b->d_fA.get(), b->d_fB.get(),
+                               b->d_fC.get(), b->cfg.nx, b->cfg.nz);
+        }
         OWL_HIP(hipGetLastError());
         OWL_HIP(hipStreamSynchronize(st));
         if (misfit) {
@@ -1194,7 +1464,32 @@ owl_status owl_elastic2d_gradient(owl_elastic2d* b, const float* obs, double* mi
             OWL_HIP(hipMemcpy(m2.data(), b->d_misfit.get(), m2.size() * sizeof(double), hipMemcpyDeviceToHost));
             for (unsigned s = 0; s < nshot; ++s) misfit[s] = m2[2 * s] + m2[2 * s + 1];
         }
-        if (grad_vp || grad_vs || grad_rho) {
+        if (b->fs && (grad_vp || grad_vs || grad_rho)) {
+            // each shot mapped back to its window and through the chain rule
+            // with its own mapped density, then summed on the model
+            std::vector<double> gA(b->grad_elems), gB(b->grad_elems), gC(b->grad_elems);
+            OWL_HIP(hipMemcpy(gA.data(), b->d_gA.get(), gA.size() * sizeof(double), hipMemcpyDeviceToHost));
+            OWL_HIP(hipMemcpy(gB.data(), b->d_gB.get(), gB.size() * sizeof(double), hipMemcpyDeviceToHost));
+            OWL_HIP(hipMemcpy(gC.data(), b->d_gC.get(), gC.size() * sizeof(double), hipMemcpyDeviceToHost));
+            std::vector<double> sv(nfull, 0.0), ss(nfull, 0.0), sr(nfull, 0.0);
+            for (size_t s = 0; s < b->sd.size(); ++s) {
+                const ShotHostE2& H = b->hs[s];
+                std::vector<double> gv, gs, gr;
+                window_grads_fs(b, s, gA.data(), gB.data(), gC.data(), gv, gs, gr);
+                for (int i = 1; i <= H.nx; ++i)
+                    for (int k = 1; k <= H.nz; ++k) {
+                        const size_t m = model_index2(b, H, i, k), o = size_t(k - 1) + size_t(i - 1) * H.nz;
+                        sv[m] += gv[o];
+                        ss[m] += gs[o];
+                        sr[m] += gr[o];
+                    }
+            }
+            for (size_t e = 0; e < nfull; ++e) {
+                if (grad_vp) grad_vp[e] = std::isfinite(sv[e]) ? sv[e] : 0.0;
+                if (grad_vs) grad_vs[e] = std::isfinite(ss[e]) ? ss[e] : 0.0;
+                if (grad_rho) grad_rho[e] = std::isfinite(sr[e]) ? sr[e] : 0.0;
+            }
+        } else if (grad_vp || grad_vs || grad_rho) {
             std::vector<double> fA(nfull), fB(nfull), fC(nfull);
             OWL_HIP(hipMemcpy(fA.data(), b->d_fA.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));
             OWL_HIP(hipMemcpy(fB.data(), b->d_fB.get(), nfull * sizeof(double), hipMemcpyDeviceToHost));

@@ -281,5 +281,116 @@ void to_regular(const Mesh& m, const float* g, float* out) {
     }
 }
 
+SsgMesh build_ssg(int nx, int nz0, float dz, int pml, float refine, float dz_max, const float* c55, const float* rho,
+                  float f0min) {
+    if (nx < 1 || nz0 < 3) fail(OWL_ERR_ARG, "the free surface needs at least 3 rows");
+    if (!(refine > 0)) fail(OWL_ERR_ARG, "free_surface_dz_refine must be positive");
+    if (!(dz_max > 0)) fail(OWL_ERR_ARG, "dz_max must be positive");
+    if (!(f0min > 0)) fail(OWL_ERR_ARG, "source f0 must be positive");
+    float sm = 0.0f;
+    for (int i = -pml; i <= nx + pml + 1; ++i) {
+        const int ic = std::min(std::max(i, 1), nx);
+        const size_t q = size_t(ic - 1) * size_t(nz0);
+        sm = sm + c55[q] / rho[q];
+    }
+    const float rl = std::sqrt(sm / float(nx + 2 * pml + 2)) / f0min;
+    const int rnz = int(std::ceil(0.5f * rl / dz));
+    SsgMesh m;
+    m.rr = refine;
+    float dzm = dz / refine;
+    std::vector<float> z = regspace(0.0f, dzm, float(rnz - 1) * dz);
+    if (z.empty()) fail(OWL_ERR_ARG, "empty refined layer");
+    while (z.back() < float(nz0 - 1) * dz) {
+        dzm = std::fmin(dzm * 1.025f, dz_max);
+        z.push_back(z.back() + dzm);
+        if (z.size() > 10000000) fail(OWL_ERR_ARG, "the free-surface mesh does not reach the window's depth");
+    }
+    const int nz = int(z.size()), nrj = nz + 2 * pml + 2;
+    std::vector<float> zh(static_cast<size_t>(nz));
+    zh[0] = -0.5f * dz / refine;
+    for (int i = 0; i + 1 < nz; ++i) zh[size_t(i + 1)] = 0.5f * (z[size_t(i)] + z[size_t(i + 1)]);
+    const std::vector<float> di = deriv(z), dh = deriv(zh);
+    m.nz = nz;
+    m.pml = pml;
+    m.dz = dz;
+    m.dzm = dzm;
+    for (auto* v : {&m.zi, &m.zh, &m.dzi, &m.dzh, &m.sci, &m.sch}) v->assign(size_t(nrj), 0.0f);
+    for (int j = -pml; j <= nz + pml + 1; ++j) {
+        const int jc = std::min(std::max(j, 1), nz);
+        const size_t k = size_t(j + pml), c = size_t(jc - 1);
+        m.dzi[k] = di[c];
+        m.dzh[k] = dh[c];
+        m.sci[k] = dz / di[c];
+        m.sch[k] = dz / dh[c];
+        if (j >= 1 && j <= nz) {
+            m.zi[k] = z[c];
+            m.zh[k] = zh[c];
+        }
+    }
+    for (int i = 1; i <= pml + 1; ++i) {
+        m.zi[size_t(1 - i + pml)] = m.zi[size_t(1 + pml)] - float(i) * dz / refine;
+        m.zi[size_t(nz + i + pml)] = m.zi[size_t(nz + pml)] + float(i) * dzm;
+        m.zh[size_t(1 - i + pml)] = m.zh[size_t(1 + pml)] - float(i) * dz / refine;
+        m.zh[size_t(nz + i + pml)] = m.zh[size_t(nz + pml)] + float(i) * dzm;
+    }
+    return m;
+}
+
+void map_column_ssg(const SsgMesh& m, int nx, int nz0, const float* v, int i, float* out) {
+    const int pml = m.pml, nr = nz0 + 2 * pml + 2, nq = m.nz + 2 * pml + 2;
+    std::vector<float> zr(static_cast<size_t>(nr)), vr(static_cast<size_t>(nr));
+    const int ic = std::min(std::max(i, 1), nx);
+    for (int k = -pml; k <= nz0 + pml + 1; ++k) {
+        const int kc = std::min(std::max(k, 1), nz0);
+        zr[size_t(k + pml)] = float(k - 1) * m.dz;
+        vr[size_t(k + pml)] = v[size_t(kc - 1) + size_t(ic - 1) * size_t(nz0)];
+    }
+    ginterp(zr.data(), vr.data(), nr, m.zi.data(), nq, out, false);
+}
+
+void locate_ssg(const SsgMesh& m, float z, float oz, int& gz, int& hz, float* iz, float* hzw) {
+    const int pml = m.pml;
+    const float zs = z + (z == 0.0f ? FLT_EPSILON : 0.0f);
+    float bi = INFINITY, bh = INFINITY;
+    for (int j = -pml; j <= m.nz + pml + 1; ++j) {
+        const float di = std::fabs(z - oz - m.zi[size_t(j + pml)]);
+        if (di < bi) {
+            bi = di;
+            gz = j;
+        }
+        const float dh = std::fabs(zs - oz - m.zh[size_t(j + pml)]);
+        if (dh < bh) {
+            bh = dh;
+            hz = j;
+        }
+    }
+    kaiser_sinc((z - oz - m.zi[size_t(gz + pml)]) / m.dzi[size_t(gz + pml)], iz);
+    kaiser_sinc((z - oz - m.zh[size_t(hz + pml)]) / m.dzh[size_t(hz + pml)], hzw);
+}
+
+void column_to_regular_ssg(const SsgMesh& m, int nz0, const double* g, double* out) {
+    spline_to_regular(m.zi.data() + m.pml + 1, g, m.nz, m.dz, nz0, out);
+}
+
+void spline_to_regular(const float* x, const double* g, int n, float dz, int nz0, double* out) {
+    if (n < 3) fail(OWL_ERR_ARG, "too few refined rows");
+    std::vector<double> M(size_t(n), 0.0), cp(size_t(n), 0.0), dp(size_t(n), 0.0);
+    for (int i = 1; i < n - 1; ++i) {
+        const double h0 = double(x[i]) - x[i - 1], h1 = double(x[i + 1]) - x[i];
+        const double r = 6.0 * ((g[i + 1] - g[i]) / h1 - (g[i] - g[i - 1]) / h0);
+        const double den = 2.0 * (h0 + h1) - h0 * cp[size_t(i - 1)];
+        cp[size_t(i)] = h1 / den;
+        dp[size_t(i)] = (r - h0 * dp[size_t(i - 1)]) / den;
+    }
+    for (int i = n - 2; i >= 1; --i) M[size_t(i)] = dp[size_t(i)] - cp[size_t(i)] * M[size_t(i + 1)];
+    for (int k = 0; k < nz0; ++k) {
+        const float v = float(k) * dz;
+        const int s = segment(x, n, v);
+        const double h = double(x[s + 1]) - x[s];
+        const double A = (double(x[s + 1]) - v) / h, B = (double(v) - x[s]) / h;
+        out[k] = A * g[s] + B * g[s + 1] + ((A * A * A - A) * M[size_t(s)] + (B * B * B - B) * M[size_t(s + 1)]) * h * h / 6.0;
+    }
+}
+
 }  // namespace fsm
 }  // namespace owl

@@ -75,6 +75,44 @@ ZLoc locate_z(const Mesh& m, const Surface& s, float x, float z, bool source);
 // depths, zero above the free surface.
 void to_regular(const Mesh& m, const float* g, float* out);
 
+// The depth-refined mesh of OWL's 2D standard-staggered solver with a free
+// surface (elastic_vhtiort_2d/module_vars.f90:310-371): rows dz/refine
+// apart over half a Rayleigh wavelength (sqrt of the mean c55/rho of the
+// padded first row over the smallest f0), then spacings min(1.025 dz_prev,
+// dz_max) until (nz0 - 1) dz; half rows at the midpoints, the first at
+// -dz/(2 refine); dz_i = deriv(z_i), dz_h = deriv(z_h) and the scalings dz/dz_i,
+// dz/dz_h on the unpadded rows, then edge-padded; z_i, z_h extended by
+// dz/refine above and by the last spacing below.  Tables over j = -pml ..
+// nz+pml+1 at [j + pml].  c55, rho: the window (nz0, nx) z fastest.
+struct SsgMesh {
+    int nz = 0, pml = 0;
+    float dz = 0, rr = 0, dzm = 0;
+    std::vector<float> zi, zh, dzi, dzh, sci, sch;
+};
+SsgMesh build_ssg(int nx, int nz0, float dz, int pml, float refine, float dz_max, const float* c55, const float* rho,
+                  float f0min);
+
+// Column i of the window's values v (nz0, nx) z fastest, padded by edge
+// replication over rows k = -pml .. nz0+pml+1 at (k - 1) dz, mapped onto
+// z_i of the refined mesh by FLIT's linear ginterp (map_regular_to_irregular,
+// module_vars.f90:139-170): out[j + pml], j = -pml .. nz+pml+1.
+void map_column_ssg(const SsgMesh& m, int nx, int nz0, const float* v, int i, float* out);
+
+// A point's refined rows (prepare_geometry's depth-varying branch,
+// src/lib/module_source_receiver.f90:596-680): gz, hz the first rows nearest
+// to z - oz on z_i and to (z + float_small at z = 0) - oz on z_h, weights
+// Kaiser-sinc of the offsets over dz_i(gz), dz_h(hz).
+void locate_ssg(const SsgMesh& m, float z, float oz, int& gz, int& hz, float* iz, float* hzw);
+
+// One window column of refined-row values g[j - 1], j = 1 .. nz, to the
+// regular rows 1 .. nz0 at (j - 1) dz by the natural cubic spline through
+// z_i(1 .. nz) (map_irregular_to_regular, module_vars.f90:175-203).
+void column_to_regular_ssg(const SsgMesh& m, int nz0, const double* g, double* out);
+
+// The natural cubic spline through (x[k], g[k]), k < n, at (k - 1) dz for
+// k = 1 .. nz0.
+void spline_to_regular(const float* x, const double* g, int n, float dz, int nz0, double* out);
+
 // Stable dt of the refined mesh (module_vars.f90:532-567): min over the
 // mesh of 1 / (sum|c| vmax dz0) with OWL's metric-scaled spacing.
 // vmax(i, j): the largest phase velocity at refined cell (i, j).

@@ -2,7 +2,8 @@
 include/owl_amd.h): a batch of shots with per-shot windows sharing one
 (nz, nx) model, forward modelling as OWL's elastic_vhtiort_2d
 compute_forward (src/solver/elastic_vhtiort_2d/submodule_forward_modeling.f90:31-180)
-for isotropic media with force sources and two-component receivers, and
+for isotropic media with force and explosion sources, two-component
+receivers and, optionally, OWL's free surface on its depth-refined mesh, and
 OWL's gradient (src/solver/elastic_vhtiort_2d/submodule_fwi_gradient.f90)."""
 import ctypes as C
 
@@ -135,20 +136,33 @@ class TwoComponentBatch:
 class Elastic2DBatch(TwoComponentBatch):
     """vp, vs, rho: (nz, nx).  shots: list of (Geometry, window) where window
     is None (full model) or an object/dict with nxbeg, nxend, nzbeg, nzend
-    (1-based)."""
+    (1-based).  surface: None, or a dict with refine (OWL's
+    free_surface_dz_refine, default 4) and dz_max (default 1.5 dz) for OWL's
+    free surface on its depth-refined mesh (flat, at the top of each
+    window)."""
 
     PREFIX = "owl_elastic2d"
+    KERNELS = TwoComponentBatch.KERNELS + ("free_surface_mirror",)
 
     def __init__(self, nx, nz, dx, dz, vp, vs, rho, shots, dt, tmax, ox=0.0, oz=0.0, pml=15, fdhalf=4,
-                 f0factor=1.0, cc_step_interval=1, trace_discard_threshold=1.0e-6):
+                 f0factor=1.0, cc_step_interval=1, trace_discard_threshold=1.0e-6, surface=None):
         self._setup(nx, nz, dx, dz, shots, dt, tmax, ox, oz, pml, fdhalf, f0factor, cc_step_interval,
                     trace_discard_threshold)
         self._m = [self._model(a) for a in (vp, vs, rho)]
         h = C.c_void_p()
-        check(lib().owl_elastic2d_create(C.byref(self.cfg), *[_ptr(a) for a in self._m], len(shots),
-                                         C.cast(self._shots, C.c_void_p), C.byref(h)))
+        surf = None
+        if surface is not None:
+            surf = _lib.owl_elastic_surface(float(surface.get("refine", 4.0)),
+                                            float(surface.get("dz_max", 1.5 * dz)))
+        check(lib().owl_elastic2d_create_surface(C.byref(self.cfg), *[_ptr(a) for a in self._m], len(shots),
+                                                 C.cast(self._shots, C.c_void_p),
+                                                 None if surf is None else C.byref(surf), C.byref(h)))
         self._h = h
         self.nt = lib().owl_elastic2d_nt(h)
+
+    def rows(self, shot):
+        """Rows shot `shot` steps: its window's, or its refined mesh's."""
+        return int(lib().owl_elastic2d_rows(self._h, int(shot)))
 
     def adjoint(self, adj):
         """OWL's reverse pass after forward(save_boundary=True).  adj: per
