@@ -7,18 +7,22 @@ the GPU.  Each runs the benchmark batch itself, all shots and all steps:
 * 3D elastic: 200 x 200 x 150, 10 m, R = 4, dt 0.8 ms, 2500 steps, flat free
   surface, 8 vertical-force shots, 50 x 50 three-component receivers;
 * 2D tilted-TI FSG: 801 x 401, 5 m, R = 4, dt 0.4 ms, 5000 steps, 64 shots
-  with uneven windows (200-799 columns), no free surface.
+  with uneven windows (200-799 columns), without and with the flat free
+  surface on OWL's depth-refined mesh (refinement 4, the benchmark's).
 
 Checks: a trace sample at step t depends only on steps up to t, so the
 float32 oracle run for the first n steps of a shot must equal the first n
 samples of the GPU's full-length traces bit for bit (n sized so that the
-oracle takes seconds; the smallest FSG window is checked over all 5000
-steps); and every shot modelled alone must give the same traces, bit for
-bit, as inside the full batch.
+oracle takes seconds; the smallest FSG window without the surface is checked
+over all 5000 steps); and every shot modelled alone must give the same
+traces, bit for bit, as inside the full batch.  The same prefixes from the
+float64 oracle state the FP32 scheme's error against an FP64 build of the
+same scheme at each workload (printed, and bounded by 1e-4 relative L2).
 """
 import numpy as np
 import pytest
 
+from helpers import rel_l2
 from owl_amd import Acoustic3DBatch, Elastic3DBatch, FSG2DBatch
 from owl_amd.synthetic import (acoustic3d_layered_model, acoustic3d_shots, elastic3d_model, elastic3d_shots,
                                fsg2d_shots, fsg2d_tti_model)
@@ -37,7 +41,7 @@ def _area(win, axes="xy"):
     return a
 
 
-def test_c2_acoustic3d_full_size(oracle32, gpu):
+def test_c2_acoustic3d_full_size(oracle32, oracle64, gpu):
     vp, rho = acoustic3d_layered_model()
     shots = acoustic3d_shots()
     dt, nt = 0.0009, 1500
@@ -61,9 +65,13 @@ def test_c2_acoustic3d_full_size(oracle32, gpu):
     ref = _oracle3(oracle32, vp, rho, shots[k][1], shots[k][0], 10.0, dt, n, 20, 4)
     assert np.abs(ref).max() > 0
     assert np.array_equal(tr[k][:, :n], ref), f"max diff {np.abs(tr[k][:, :n] - ref).max()}"
+    ref64 = _oracle3(oracle64, vp, rho, shots[k][1], shots[k][0], 10.0, dt, n, 20, 4)
+    e64 = rel_l2(tr[k][:, :n], ref64)
+    print(f"C2 shot {k}, first {n} steps: FP32 GPU vs FP64 oracle rel-L2 {e64:.3e}", flush=True)
+    assert e64 < 1e-4
 
 
-def test_c3_elastic3d_full_size(oracle32, gpu):
+def test_c3_elastic3d_full_size(oracle32, oracle64, gpu):
     vp, vs, rho = elastic3d_model()
     shots = elastic3d_shots()
     dt, nt = 0.0008, 2500
@@ -88,9 +96,13 @@ def test_c3_elastic3d_full_size(oracle32, gpu):
     for comp, (a, c) in enumerate(zip(tr[0], ref)):
         assert np.abs(c).max() > 0
         assert np.array_equal(a[:, :n], c), f"component {comp}: max diff {np.abs(a[:, :n] - c).max()}"
+    ref64 = _oracle_e3(oracle64, vp, vs, rho, None, shots[0][0], 10.0, dt, n, 20, 4, True)
+    e64 = rel_l2(np.concatenate([a[:, :n] for a in tr[0]]), np.concatenate(ref64))
+    print(f"C3 shot 0, first {n} steps: FP32 GPU vs FP64 oracle rel-L2 {e64:.3e}", flush=True)
+    assert e64 < 1e-4
 
 
-def test_c4_fsg2d_full_size(oracle32, gpu):
+def test_c4_fsg2d_full_size(oracle32, oracle64, gpu):
     cij, rho = fsg2d_tti_model()
     shots = fsg2d_shots()
     dt, nt = 0.0004, 5000
@@ -117,3 +129,40 @@ def test_c4_fsg2d_full_size(oracle32, gpu):
         for comp, (a, c) in enumerate(zip(tr[k], ref)):
             assert np.abs(c).max() > 0
             assert np.array_equal(a[:, :n], c), f"shot {k} comp {comp}: max diff {np.abs(a[:, :n] - c).max()}"
+        if k == small:
+            ref64 = oracle64.fsg2d_forward(g, _sub(cij, sl), np.ascontiguousarray(rho[sl]), srcs, recs)
+            e64 = rel_l2(np.concatenate([a[:, :n] for a in tr[k]]), np.concatenate(ref64))
+            print(f"C4 shot {k}, {n} steps: FP32 GPU vs FP64 oracle rel-L2 {e64:.3e}", flush=True)
+            assert e64 < 1e-4
+
+
+def test_c4_fsg2d_free_surface_full_size(oracle32, oracle64, gpu):
+    """The configs' FSG workload with its flat free surface on the refined
+    mesh (the benchmark's batch): the smallest window's first 1500 steps
+    bit-identical to the float32 oracle and stated against the float64 one;
+    the widest window's first 300 steps bit-identical."""
+    cij, rho = fsg2d_tti_model()
+    shots = fsg2d_shots()
+    dt, nt = 0.0004, 5000
+    surf = dict(refine=4.0)
+    print("case: FSG 801x401 with free surface, 64 shots, 5000 steps", flush=True)
+    b = FSG2DBatch(801, 401, 5.0, 5.0, cij, rho, shots, dt, (nt - 1) * dt, pml=20, fdhalf=4, surface=surf)
+    tr = b.forward()
+    b.close()
+    widths = [w["nxend"] - w["nxbeg"] + 1 for _, w in shots]
+    small, large = int(np.argmin(widths)), int(np.argmax(widths))
+    for k, n in ((small, 1500), (large, 300)):
+        geom, win = shots[k]
+        print(f"case: oracle with free surface, shot {k} (width {widths[k]}), first {n} steps", flush=True)
+        g, sl = _window_grid(rho, win, 5.0, dt, n, 20, 4)
+        g["aniso"] = 1
+        srcs, recs = _oracle_args(geom, g, 5.0)
+        ref = oracle32.fsg2d_forward_fs(g, surf, _sub(cij, sl), np.ascontiguousarray(rho[sl]), srcs, recs)
+        for comp, (a, c) in enumerate(zip(tr[k], ref)):
+            assert np.abs(c).max() > 0
+            assert np.array_equal(a[:, :n], c), f"shot {k} comp {comp}: max diff {np.abs(a[:, :n] - c).max()}"
+        if k == small:
+            ref64 = oracle64.fsg2d_forward_fs(g, surf, _sub(cij, sl), np.ascontiguousarray(rho[sl]), srcs, recs)
+            e64 = rel_l2(np.concatenate([a[:, :n] for a in tr[k]]), np.concatenate(ref64))
+            print(f"C4 free surface, shot {k}, {n} steps: FP32 GPU vs FP64 oracle rel-L2 {e64:.3e}", flush=True)
+            assert e64 < 1e-4
