@@ -49,10 +49,13 @@ TRAFFIC = {
     ("acoustic2d_gradient", "adjoint_fused", 6): (2.0 * 433.9e3 + 306.3e3) * 1024,
     ("acoustic3d_forward", "update_v"): (2.0 * 4.159e6 + 4.199e6) * 1024,    # acoustic3d_forward_256cube_16shots.md
     ("acoustic3d_forward", "update_p"): (2.0 * 4.318e6 + 1.765e6) * 1024,
-    ("elastic3d_forward", "stress"): (2.0 * 6.017e6 + 2.686e6) * 1024,      # elastic3d_forward_200x200x150_8shots.md
-    ("elastic3d_forward", "velocity"): (2.0 * 5.066e6 + 1.602e6) * 1024,
-    ("fsg2d_forward", "stress"): (2.0 * 7.750e5 + 3.767e5) * 1024,   # fsg2d_forward_free_surface_801x401_64shots.md
-    ("fsg2d_forward", "velocity"): (2.0 * 4.983e5 + 2.834e5) * 1024,
+    # (z planes, y rows per tile): elastic3d_forward_200x200x150_8shots.md
+    ("elastic3d_forward", "stress", (64, 4)): (2.0 * 6.017e6 + 2.686e6) * 1024,
+    ("elastic3d_forward", "velocity", (64, 4)): (2.0 * 5.066e6 + 1.602e6) * 1024,
+    # (rows per tile column, tiles per block) of the FSG sweeps:
+    # fsg2d_forward_free_surface_801x401_64shots.md
+    ("fsg2d_forward", "stress", (32, 4)): (2.0 * 7.750e5 + 3.767e5) * 1024,
+    ("fsg2d_forward", "velocity", (32, 4)): (2.0 * 4.983e5 + 2.834e5) * 1024,
 }
 
 
@@ -279,11 +282,14 @@ def bench_3d(sync, steps=1500, shape=None):
             "kernels": kernel_table(stats, 1)}
 
 
-def bench_elastic3d(sync, steps):
+def bench_elastic3d(sync, steps, gradient_shots=4, shape=None):
     """3D isotropic elastic batched forward: 200 x 200 x 150, 10 m, R = 4,
     dt 0.8 ms, 20-cell CFS-MPML on five faces and a flat free surface,
     vp a smooth random field in 2000..5000 m/s, vs = vp/1.73, Gardner rho,
-    8 vertical-force shots, 50 x 50 three-component receivers."""
+    8 vertical-force shots, 50 x 50 three-component receivers.  Then OWL's
+    gradient of vp, vs, rho (boundary records in HBM, reverse reconstruction
+    and adjoint, FP64 imaging) over the first `gradient_shots` shots, whose
+    boundary records of all steps (26 GB a shot) fit in HBM together."""
     from owl_amd import Elastic3DBatch
     from owl_amd.synthetic import elastic3d_model, elastic3d_shots
     vp, vs, rho = elastic3d_model()
@@ -291,6 +297,10 @@ def bench_elastic3d(sync, steps):
     dt = 0.0008
     b = Elastic3DBatch(200, 200, 150, 10.0, 10.0, 10.0, vp, vs, rho, shots, dt, (steps - 1) * dt, pml=20,
                        fdhalf=4, free_surface=True)
+    if shape:
+        b.set_launch(*shape)
+    else:
+        shape = b.autotune(3)
     b.set_profiling(True)
     sync()
     t0 = time.perf_counter()
@@ -300,21 +310,44 @@ def bench_elastic3d(sync, steps):
     stats = b.kernel_stats()
     ms, updates = b.last_stats()
     b.close()
+    grad = None
+    if gradient_shots:
+        sub = shots[:gradient_shots]
+        bg = Elastic3DBatch(200, 200, 150, 10.0, 10.0, 10.0, vp, vs, rho, sub, dt, (steps - 1) * dt, pml=20,
+                            fdhalf=4, free_surface=True)
+        bg.set_launch(*shape)
+        obs = [tuple(np.zeros((nr, bg.nt), np.float32) for _ in range(3)) for nr in bg.nr]
+        sync()
+        t0 = time.perf_counter()
+        _, gv, gs, gr, _ = bg.gradient(obs)
+        sync()
+        gw = time.perf_counter() - t0
+        gms, gup = bg.last_stats()
+        bg.close()
+        grad = {"shots": len(sub), "seconds": gw, "s_per_shot": gw / len(sub), "device_ms": gms,
+                "grid_pt_updates_per_s": gup / gw,
+                "algorithm": "OWL's gradient of vp, vs, rho: forward with boundary records in HBM, L2 adjoint "
+                             "source per component (observed traces zero), reverse reconstruction and adjoint, "
+                             "FP64 imaging, merged on the model; first call (allocations included)",
+                "finite": bool(np.isfinite(gv).all() and np.isfinite(gs).all() and np.isfinite(gr).all())}
     _, roof = roofline_of(stats)
     return {"workload": f"3D isotropic elastic batched forward, 200x200x150, 10 m, dt 0.8 ms, R 4, pml 20 on five "
                         f"faces, flat free surface, 8 shots, 50x50 3-C receivers, {steps} steps",
             "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
-            "roofline": with_traffic("elastic3d_forward", roof),
-            "kernels": kernel_table(stats, 1)}
+            "launch_shape": list(shape), "roofline": with_traffic("elastic3d_forward", roof, tuple(shape)),
+            "kernels": kernel_table(stats, 1), "gradient": grad}
 
 
-def bench_fsg(sync, steps, gradient=True):
+def bench_fsg(sync, steps, gradient=True, shape=None):
     """2D tilted-TI elastic on the Lebedev fully staggered grid with OWL's
     flat free surface on the depth-refined mesh (free_surface_dz_refine 4,
     dz_max 1.5 dz), batched: 801 x 401, 5 m, R = 4, dt 0.4 ms, 20-cell
     CFS-MPML with the Christoffel-scan damping ratios, 64 shots with uneven
     200..801-column windows, vertical forces, two-component receivers every
-    10 m.  The forward is timed with per-kernel events; then OWL's gradient
+    10 m.  The sweeps' launch shape is the autotuner's (or `shape`).  The
+    forward runs once eagerly with per-kernel events (the kernel table and
+    roofline), then as one captured HIP graph of the time loop: captured by
+    an untimed run, the replay is timed.  Then OWL's gradient
     (boundary-saving reconstruction, c11 .. c55 and rho, FP64 imaging,
     mapped back to the regular windows and summed) over all shots, without
     events."""
@@ -328,17 +361,28 @@ def bench_fsg(sync, steps, gradient=True):
                    surface=dict(refine=4.0))
     plan_s = time.perf_counter() - t0
     rows = [b.rows(s) for s in range(len(shots))]
+    if shape:
+        b.set_launch(*shape)
+    else:
+        shape = b.autotune(10)
     b.set_profiling(True)
     sync()
     t0 = time.perf_counter()
     b.forward(want_traces=False)
     sync()
-    wall = time.perf_counter() - t0
+    ev_wall = time.perf_counter() - t0
     stats = b.kernel_stats()
+    ev_ms, _ = b.last_stats()
+    b.set_profiling(False)
+    b.forward(want_traces=False)  # captures the time loop
+    sync()
+    t0 = time.perf_counter()
+    b.forward(want_traces=False)
+    sync()
+    wall = time.perf_counter() - t0
     ms, updates = b.last_stats()
     grad = None
     if gradient:
-        b.set_profiling(False)
         obs = [tuple(np.zeros((nr, b.nt), np.float32) for _ in range(2)) for nr in b.nr]
         b.gradient(obs)  # untimed warm-up: buffers, strips
         sync()
@@ -359,8 +403,11 @@ def bench_fsg(sync, steps, gradient=True):
     return {"workload": f"2D tilted-TI FSG batched forward with OWL's flat free surface on the depth-refined "
                         f"mesh (refinement 4, dz_max 7.5 m; {min(rows)}-{max(rows)} refined rows per window), "
                         f"801x401, 5 m, dt 0.4 ms, R 4, pml 20, 64 shots (windows 200-801), {steps} steps",
+            "launch_shape": list(shape),
+            "timing": "one captured HIP graph of the time loop, replayed (captured by an untimed run)",
             "grid_pt_updates_per_s": updates / wall, "seconds": wall, "device_ms": ms,
-            "plan_seconds": round(plan_s, 3), "roofline": with_traffic("fsg2d_forward", roof),
+            "evented_eager": {"seconds": ev_wall, "device_ms": ev_ms},
+            "plan_seconds": round(plan_s, 3), "roofline": with_traffic("fsg2d_forward", roof, tuple(shape)),
             "kernels": kernel_table(stats, 1), "gradient": grad}
 
 
@@ -393,8 +440,14 @@ def main():
     ap.add_argument("--no-elastic3d", action="store_true", help="skip the 3D elastic forward")
     ap.add_argument("--only-elastic3d", action="store_true", help="only the 3D elastic forward (for profiler runs)")
     ap.add_argument("--elastic3d-steps", type=int, default=2500, help="time steps of the 3D elastic forward")
+    ap.add_argument("--elastic3d-shape", default="", metavar="PLANES,ROWS",
+                    help="this launch shape of the 3D elastic sweeps instead of the autotuner's (counter runs)")
+    ap.add_argument("--elastic3d-gradient-shots", type=int, default=4,
+                    help="shots of the timed 3D elastic gradient (0 skips it)")
     ap.add_argument("--only-fsg", action="store_true", help="only the 2D FSG forward (for profiler runs)")
     ap.add_argument("--fsg-steps", type=int, default=5000, help="time steps of the 2D FSG forward")
+    ap.add_argument("--fsg-shape", default="", metavar="PLANES,ROWS",
+                    help="this launch shape of the FSG sweeps instead of the autotuner's (counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device five times (profiler counter calibration)")
     args = ap.parse_args()
@@ -429,15 +482,18 @@ def main():
                          OWL_STORE_F32)
     sync = torch.cuda.synchronize
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
+    fsg_shape = tuple(int(v) for v in args.fsg_shape.split(",")) if args.fsg_shape else None
+    el3_shape = tuple(int(v) for v in args.elastic3d_shape.split(",")) if args.elastic3d_shape else None
     if args.only_3d:
         shape = tuple(int(v) for v in args.shape_3d.split(",")) if args.shape_3d else None
         print(json.dumps({"acoustic3d_forward": bench_3d(sync, args.steps_3d, shape)}))
         return
     if args.only_elastic3d:
-        print(json.dumps({"elastic3d_forward": bench_elastic3d(sync, args.elastic3d_steps)}))
+        print(json.dumps({"elastic3d_forward": bench_elastic3d(sync, args.elastic3d_steps, 0, el3_shape)}))
         return
     if args.only_fsg:
-        print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient)}))
+        print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient,
+                                                     fsg_shape)}))
         return
     log("building workload")
     vp, rho, vp0, rho0, shots = workload()
@@ -531,12 +587,12 @@ def main():
     el3 = None
     if not args.no_elastic3d:
         log("3D elastic batched forward")
-        el3 = bench_elastic3d(sync, args.elastic3d_steps)
+        el3 = bench_elastic3d(sync, args.elastic3d_steps, args.elastic3d_gradient_shots, el3_shape)
 
     fsg = None
     if not args.no_fsg:
         log("2D FSG batched forward and gradient")
-        fsg = bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient)
+        fsg = bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient, fsg_shape)
 
     cpu = None
     if not args.no_cpu:

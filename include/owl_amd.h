@@ -446,6 +446,11 @@ owl_status owl_elastic3d_gradient(owl_elastic3d *b, const float *obs, double *mi
 /* Launch shape (no OWL counterpart; results do not depend on it): z planes
  * per block and thread rows per block of 64 threads (1, 2 or 4). */
 owl_status owl_elastic3d_set_launch(owl_elastic3d *b, int planes, int rows);
+/* Times `steps` steps of the stress and velocity sweeps under each launch
+ * shape (8 .. 256 z planes, 1, 2 or 4 y rows per tile) on this batch,
+ * keeps the fastest and returns it (planes, rows may be NULL).  No
+ * reference counterpart. */
+owl_status owl_elastic3d_autotune(owl_elastic3d *b, int steps, int *planes, int *rows);
 /* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
  * sweep, 2 sources and receivers, 3 imaging, 4 boundary records. */
 owl_status owl_elastic3d_set_profiling(owl_elastic3d *b, int on);
@@ -510,6 +515,10 @@ owl_status owl_elastic2d_gradient(owl_elastic2d *b, const float *obs, double *mi
 /* Launch shape (no OWL counterpart; results do not depend on it): z rows
  * per tile and 64-thread tiles per block (1, 2 or 4). */
 owl_status owl_elastic2d_set_launch(owl_elastic2d *b, int planes, int rows);
+/* As owl_fsg2d_autotune and owl_fsg2d_set_graphs, for this batch's stress
+ * and velocity sweeps and forward loop.  No reference counterpart. */
+owl_status owl_elastic2d_autotune(owl_elastic2d *b, int steps, int *planes, int *rows);
+owl_status owl_elastic2d_set_graphs(owl_elastic2d *b, int on);
 /* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
  * sweep, 2 sources and receivers, 3 imaging, 4 boundary records. */
 owl_status owl_elastic2d_set_profiling(owl_elastic2d *b, int on);
@@ -611,6 +620,15 @@ owl_status owl_fsg2d_gradient(owl_fsg2d *b, const float *obs, double *misfit, do
                               float *traces_out);
 /* Launch shape (no OWL counterpart; results do not depend on it). */
 owl_status owl_fsg2d_set_launch(owl_fsg2d *b, int planes, int rows);
+/* Times `steps` steps of the two sweeps under each launch shape (rows per
+ * tile column 4 .. 128, 1, 2 or 4 tiles per block) on this batch, keeps the
+ * fastest and returns it (planes, rows may be NULL).  No reference
+ * counterpart: OWL runs on the CPU. */
+owl_status owl_fsg2d_autotune(owl_fsg2d *b, int steps, int *planes, int *rows);
+/* on != 0 (the default): modelling without profiling or boundary records
+ * replays the forward time loop as one captured HIP graph, re-captured when
+ * a launch argument changes; 0 launches eagerly. */
+owl_status owl_fsg2d_set_graphs(owl_fsg2d *b, int on);
 /* Timing; kinds: 0 stress sweep, 1 velocity sweep, 2 sources and
  * receivers, 3 imaging, 4 boundary records. */
 owl_status owl_fsg2d_set_profiling(owl_fsg2d *b, int on);

@@ -1574,12 +1574,8 @@ struct owl_acoustic3d {
     // HIP graph of the forward time loop (modelling without boundary
     // records, profiling off): captured by the first such call, replayed by
     // the next ones while the launch shape and buffers stay the same
-    int use_graphs = 1;
-    bool capturing = false;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<uintptr_t> gsig;
+    LoopGraph graph;
     ~owl_acoustic3d() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -2139,46 +2135,15 @@ void run_forward3(owl_acoustic3d* b, bool save_boundary) {
                                b->d_inj.get(), b->d_amp.get(), b->nt, t, b->d_rec_beg.get(), b->d_rec.get(),
                                b->d_rec_w.get(), b->d_traces.get());
             b->prof.end(st, ev);
-            if (!b->capturing) maybe_collect(b);
+            if (!b->graph.capturing) maybe_collect(b);
         }
     };
-    const bool graph_ok = b->use_graphs && !b->prof.on() && !save_boundary;
-    std::vector<uintptr_t> sig;
-    if (graph_ok) sig = forward_signature(b);
+    const bool graph_ok = b->graph.enabled && !b->prof.on() && !save_boundary;
     OWL_HIP(hipEventRecord(b->ev0, st));
-    if (graph_ok && b->gexec && sig == b->gsig) {
-        OWL_HIP(hipGraphLaunch(b->gexec, st));
-    } else if (graph_ok) {
-        if (b->gexec) {
-            (void)hipGraphExecDestroy(b->gexec);
-            b->gexec = nullptr;
-            b->gsig.clear();
-        }
-        hipGraph_t g = nullptr;
-        b->capturing = true;
-        OWL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        try {
-            loop();
-        } catch (...) {
-            (void)hipStreamEndCapture(st, &g);
-            if (g) (void)hipGraphDestroy(g);
-            b->capturing = false;
-            throw;
-        }
-        const hipError_t e = hipStreamEndCapture(st, &g);
-        b->capturing = false;
-        if (e != hipSuccess || !g) fail(OWL_ERR_DEVICE, "graph capture of the forward loop failed");
-        const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ei != hipSuccess) {
-            b->gexec = nullptr;
-            fail(OWL_ERR_DEVICE, "graph instantiation of the forward loop failed");
-        }
-        b->gsig = sig;
-        OWL_HIP(hipGraphLaunch(b->gexec, st));
-    } else {
+    if (graph_ok)
+        b->graph.run(st, forward_signature(b), loop);
+    else
         loop();
-    }
     if (save_boundary) strips_launch(b, R, cur.f[1], cur.f[2], cur.f[3], b->nt, 1);
     // the final state lives in b->f for the reverse pass (modelling alone
     // keeps the buffers where they are, so a captured loop stays valid)
@@ -2492,12 +2457,8 @@ extern "C" {
 owl_status owl_acoustic3d_set_graphs(owl_acoustic3d* b, int on) {
     return guarded([&] {
         if (!b) fail(OWL_ERR_ARG, "null batch");
-        b->use_graphs = on != 0;
-        if (b->gexec) {
-            OWL_HIP(hipGraphExecDestroy(b->gexec));
-            b->gexec = nullptr;
-        }
-        b->gsig.clear();
+        b->graph.enabled = on != 0;
+        b->graph.reset();
     });
 }
 

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -128,6 +130,65 @@ private:
     T* p_ = nullptr;
     size_t n_ = 0;
     bool owned_ = true;
+};
+
+inline uint32_t float_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof u);
+    return u;
+}
+
+// A time loop captured once as a HIP graph and replayed while its
+// signature (every device address and launch parameter its kernels take)
+// is unchanged; a changed signature re-captures.  The loop must not
+// synchronise while `capturing` is set.
+class LoopGraph {
+public:
+    bool enabled = true;
+    bool capturing = false;
+    LoopGraph() = default;
+    LoopGraph(const LoopGraph&) = delete;
+    LoopGraph& operator=(const LoopGraph&) = delete;
+    ~LoopGraph() { reset(); }
+    void reset() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        exec_ = nullptr;
+        sig_.clear();
+    }
+    template <class Fn>
+    void run(hipStream_t st, const std::vector<uintptr_t>& sig, Fn&& loop) {
+        if (exec_ && sig == sig_) {
+            OWL_HIP(hipGraphLaunch(exec_, st));
+            return;
+        }
+        reset();
+        hipGraph_t g = nullptr;
+        capturing = true;
+        OWL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            loop();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            capturing = false;
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        capturing = false;
+        if (e != hipSuccess || !g) fail(OWL_ERR_DEVICE, "graph capture of the time loop failed");
+        const hipError_t ei = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            exec_ = nullptr;
+            fail(OWL_ERR_DEVICE, "graph instantiation of the time loop failed");
+        }
+        sig_ = sig;
+        OWL_HIP(hipGraphLaunch(exec_, st));
+    }
+
+private:
+    hipGraphExec_t exec_ = nullptr;
+    std::vector<uintptr_t> sig_;
 };
 
 }  // namespace owl

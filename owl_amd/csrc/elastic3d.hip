@@ -1548,6 +1548,24 @@ inline void chain_rule(float vp, float vs, float rho, double A, double B, double
     gr = C;
 }
 
+// Tiles of the current launch shape (kc z planes, cy y rows per tile) and
+// their shots, uploaded on the batch's stream.
+void retile_e(owl_elastic3d* b) {
+    int lt = 0;
+    b->tile_shot.clear();
+    for (size_t s = 0; s < b->sd.size(); ++s) {
+        ShotE& S = b->sd[s];
+        S.nty = (S.ny + 2 * b->pml + b->cy - 1) / b->cy;
+        S.ntx = (S.nx + 2 * b->pml + (64 - 2 * b->R) - 1) / (64 - 2 * b->R);
+        S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
+        S.tile0 = lt;
+        lt += S.ntx * S.nty * S.nzc;
+        b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nty * S.nzc, int(s));
+    }
+    b->d_shots.upload(b->sd, b->stream);
+    b->d_tile_shot.upload(b->tile_shot, b->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1670,20 +1688,59 @@ owl_status owl_elastic3d_set_launch(owl_elastic3d* b, int planes, int rows) {
         if (!b || planes < 1 || (rows != 1 && rows != 2 && rows != 4)) fail(OWL_ERR_ARG, "bad launch shape");
         b->kc = planes;
         b->cy = rows;
-        int lt = 0;
-        b->tile_shot.clear();
-        for (size_t s = 0; s < b->sd.size(); ++s) {
-            ShotE& S = b->sd[s];
-            S.nty = (S.ny + 2 * b->pml + b->cy - 1) / b->cy;
-            S.ntx = (S.nx + 2 * b->pml + (64 - 2 * b->R) - 1) / (64 - 2 * b->R);
-            S.nzc = (S.nz + 2 * b->pml + b->kc - 1) / b->kc;
-            S.tile0 = lt;
-            lt += S.ntx * S.nty * S.nzc;
-            b->tile_shot.insert(b->tile_shot.end(), size_t(S.ntx) * S.nty * S.nzc, int(s));
-        }
-        b->d_shots.upload(b->sd, b->stream);
-        b->d_tile_shot.upload(b->tile_shot, b->stream);
+        retile_e(b);
         OWL_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+// Launch-shape autotuner: times `steps` steps of the sweeps on the batch
+// under each candidate (z planes per tile, y rows per tile) and keeps the
+// fastest; the fields are zeroed afterwards (every forward starts from
+// zero).  Stencil timing does not depend on the field values.
+owl_status owl_elastic3d_autotune(owl_elastic3d* b, int steps, int* planes, int* rows) {
+    return guarded([&] {
+        if (!b || steps < 1) fail(OWL_ERR_ARG, "null batch or no steps");
+        static const int kPlanes[] = {8, 12, 16, 32, 64, 128, 256};
+        static const int kRows[] = {1, 2, 4};
+        hipStream_t st = b->stream;
+        const bool prof_on = b->prof.on();
+        b->prof.enable(false);
+        for (auto& f : b->fld) f.zero(st);
+        const EF F = fields(b);
+        double bs, bv;
+        sweep_bytes(b, bs, bv);
+        float best = -1.f;
+        int bk = b->kc, by = b->cy;
+        for (int kc : kPlanes)
+            for (int cy : kRows) {
+                b->kc = kc;
+                b->cy = cy;
+                retile_e(b);
+                dispatch_Re(b->R, [&](auto r) {
+                    step_e<decltype(r)::value>(b, F, out_of(F), b->dt, bs, bv);  // warm-up
+                    OWL_HIP(hipEventRecord(b->ev0, st));
+                    for (int t = 0; t < steps; ++t) step_e<decltype(r)::value>(b, F, out_of(F), b->dt, bs, bv);
+                    OWL_HIP(hipEventRecord(b->ev1, st));
+                });
+                OWL_HIP(hipStreamSynchronize(st));
+                float ms = 0.f;
+                OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (best < 0 || ms < best) {
+                    best = ms;
+                    bk = kc;
+                    by = cy;
+                }
+            }
+        b->kc = bk;
+        b->cy = by;
+        retile_e(b);
+        for (auto& f : b->fld) f.zero(st);
+        OWL_HIP(hipStreamSynchronize(st));
+        b->prof.enable(prof_on);
+        b->fwd_state_valid = false;
+        b->have_boundary = false;
+        if (planes) *planes = bk;
+        if (rows) *rows = by;
     });
 }
 

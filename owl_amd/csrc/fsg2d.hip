@@ -820,6 +820,9 @@ struct owl_fsg2d {
     DevBuf<float> aF[kNF];  // adjoint field
     DevBuf<Tap> d_inj, d_rec, d_adj_taps;
     DevBuf<double> d_g[7], d_full, d_misfit, d_scratch;
+    // the forward time loop as one HIP graph (modelling without profiling
+    // or boundary records), re-captured when a launch argument changes
+    LoopGraph graph;
     ~owl_fsg2d() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1570,6 +1573,22 @@ void stripsf(owl_fsg2d* b, const FV& V, int t, int save) {
     b->prof.end(b->stream, ev);
 }
 
+// Every device address and launch parameter the forward loop passes to its
+// kernels: a captured loop is replayed only while these are unchanged.
+std::vector<uintptr_t> forward_signaturef(const owl_fsg2d* b) {
+    std::vector<uintptr_t> v{uintptr_t(b->R), uintptr_t(b->nt), uintptr_t(b->kc), uintptr_t(b->cy),
+                             uintptr_t(b->tile_shot.size()), uintptr_t(b->max_nr), uintptr_t(b->fs),
+                             uintptr_t(float_bits(b->dt)),
+                             uintptr_t(b->d_shots.get()), uintptr_t(b->d_tile_shot.get()),
+                             uintptr_t(b->d_rec_beg.get()), uintptr_t(b->d_rec.get()), uintptr_t(b->d_traces.get()),
+                             uintptr_t(b->d_inj_beg.get()), uintptr_t(b->d_inj.get()), uintptr_t(b->d_amp.get()),
+                             uintptr_t(b->d_met.get())};
+    for (auto& f : b->fld) v.push_back(uintptr_t(f.get()));
+    for (auto& f : b->med) v.push_back(uintptr_t(f.get()));
+    for (auto& f : b->pmlc) v.push_back(uintptr_t(f.get()));
+    return v;
+}
+
 template <int R>
 void run_forwardf(owl_fsg2d* b, bool save_boundary) {
     hipStream_t st = b->stream;
@@ -1582,23 +1601,30 @@ void run_forwardf(owl_fsg2d* b, bool save_boundary) {
     double bs, bv;
     sweep_bytesf(b, bs, bv);
     const unsigned rg = unsigned(std::max(1, (2 * b->max_nr + 255) / 256));
-    OWL_HIP(hipEventRecord(b->ev0, st));
     // step t: [boundary record] -> record(t) -> update -> source(t)
     // (submodule_forward_modeling.f90:55-240)
-    for (int t = 1; t <= b->nt; ++t) {
-        if (save_boundary) stripsf(b, V, t, 1);
-        int ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(k_fsg_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), vel_targets(V),
-                           b->d_rec_beg.get(), b->d_rec.get(), b->d_traces.get(), b->nt, t);
-        b->prof.end(st, ev);
-        if (t == b->nt) break;  // the last update changes nothing that is recorded
-        stepf<R>(b, F, V, b->dt, bs, bv);
-        ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(k_fsg_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), vel_targets(V),
-                           b->d_inj_beg.get(), b->d_inj.get(), b->d_amp.get(), b->nt, t);
-        b->prof.end(st, ev);
-        maybe_collectf(b);
-    }
+    auto loop = [&] {
+        for (int t = 1; t <= b->nt; ++t) {
+            if (save_boundary) stripsf(b, V, t, 1);
+            int ev = b->prof.begin(st, 2, 0);
+            hipLaunchKernelGGL(k_fsg_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), vel_targets(V),
+                               b->d_rec_beg.get(), b->d_rec.get(), b->d_traces.get(), b->nt, t);
+            b->prof.end(st, ev);
+            if (t == b->nt) break;  // the last update changes nothing that is recorded
+            stepf<R>(b, F, V, b->dt, bs, bv);
+            ev = b->prof.begin(st, 2, 0);
+            hipLaunchKernelGGL(k_fsg_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), vel_targets(V),
+                               b->d_inj_beg.get(), b->d_inj.get(), b->d_amp.get(), b->nt, t);
+            b->prof.end(st, ev);
+            if (!b->graph.capturing) maybe_collectf(b);
+        }
+    };
+    const bool graph_ok = b->graph.enabled && !b->prof.on() && !save_boundary;
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    if (graph_ok)
+        b->graph.run(st, forward_signaturef(b), loop);
+    else
+        loop();
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
@@ -1958,6 +1984,69 @@ owl_status owl_fsg2d_gradient(owl_fsg2d* b, const float* obs, double* misfit, do
         if (traces_out) traces_to_hostf(b, b->d_traces.get(), traces_out);
         b->last_ms += fwd_ms;
         b->last_updates += fwd_up;
+    });
+}
+
+owl_status owl_fsg2d_set_graphs(owl_fsg2d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->graph.enabled = on != 0;
+        b->graph.reset();
+    });
+}
+
+// Launch-shape autotuner: times `steps` steps of the two sweeps on the
+// batch under each candidate (rows per tile column, tiles per block) and
+// keeps the fastest; the fields are zeroed afterwards (every forward starts
+// from zero).  Stencil timing does not depend on the field values.
+owl_status owl_fsg2d_autotune(owl_fsg2d* b, int steps, int* planes, int* rows) {
+    return guarded([&] {
+        if (!b || steps < 1) fail(OWL_ERR_ARG, "null batch or no steps");
+        static const int kPlanes[] = {4, 8, 12, 16, 32, 64, 128};
+        static const int kRows[] = {1, 2, 4};
+        hipStream_t st = b->stream;
+        const bool prof_on = b->prof.on();
+        b->prof.enable(false);
+        for (auto& f : b->fld) f.zero(st);
+        const FF F = fieldsf(b->fld);
+        const FV V = out_of(F);
+        double bs, bv;
+        sweep_bytesf(b, bs, bv);
+        float best = -1.f;
+        int bk = b->kc, by = b->cy;
+        auto shape = [&](int kc, int cy) {
+            b->kc = kc;
+            b->cy = cy;
+            retilef(b);
+            b->d_shots.upload(b->sd, st);
+            b->d_tile_shot.upload(b->tile_shot, st);
+        };
+        for (int kc : kPlanes)
+            for (int cy : kRows) {
+                shape(kc, cy);
+                dispatch_Rf(b->R, [&](auto r) {
+                    stepf<decltype(r)::value>(b, F, V, b->dt, bs, bv);  // warm-up
+                    OWL_HIP(hipEventRecord(b->ev0, st));
+                    for (int t = 0; t < steps; ++t) stepf<decltype(r)::value>(b, F, V, b->dt, bs, bv);
+                    OWL_HIP(hipEventRecord(b->ev1, st));
+                });
+                OWL_HIP(hipStreamSynchronize(st));
+                float ms = 0.f;
+                OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (best < 0 || ms < best) {
+                    best = ms;
+                    bk = kc;
+                    by = cy;
+                }
+            }
+        shape(bk, by);
+        for (auto& f : b->fld) f.zero(st);
+        OWL_HIP(hipStreamSynchronize(st));
+        b->prof.enable(prof_on);
+        b->fwd_state_valid = false;
+        b->have_boundary = false;
+        if (planes) *planes = bk;
+        if (rows) *rows = by;
     });
 }
 

@@ -85,6 +85,18 @@ class TwoComponentBatch:
     def set_launch(self, planes, rows=4):
         check(self._fn("set_launch")(self._h, int(planes), int(rows)))
 
+    def autotune(self, steps=20):
+        """Times `steps` steps of the sweeps under each launch shape and keeps
+        the fastest; returns (planes, rows)."""
+        p, r = C.c_int(), C.c_int()
+        check(self._fn("autotune")(self._h, int(steps), C.byref(p), C.byref(r)))
+        return p.value, r.value
+
+    def set_graphs(self, on=True):
+        """True (default): forward modelling without boundary records or
+        profiling replays its time loop as one captured HIP graph."""
+        check(self._fn("set_graphs")(self._h, int(bool(on))))
+
     def _split(self, flat):
         out, o = [], 0
         for n in self.nr:

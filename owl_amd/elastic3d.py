@@ -75,6 +75,13 @@ class Elastic3DBatch:
     def set_launch(self, planes, rows=4):
         check(lib().owl_elastic3d_set_launch(self._h, int(planes), int(rows)))
 
+    def autotune(self, steps=3):
+        """Times `steps` steps of the sweeps under each launch shape and keeps
+        the fastest; returns (planes, rows)."""
+        p, r = C.c_int(), C.c_int()
+        check(lib().owl_elastic3d_autotune(self._h, int(steps), C.byref(p), C.byref(r)))
+        return p.value, r.value
+
     def _split(self, flat):
         out, o = [], 0
         for n in self.nr:

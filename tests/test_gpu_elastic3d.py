@@ -64,6 +64,9 @@ def test_elastic3d_matches_oracle(oracle32, gpu, R, fs):
     outs.append(b.forward())
     with pytest.raises(Exception):
         b.set_launch(64, 8)
+    planes, rows = b.autotune(2)
+    assert planes in (8, 12, 16, 32, 64, 128, 256) and rows in (1, 2, 4)
+    outs.append(b.forward())
     for o in outs[1:]:
         for a, c in zip(o, outs[0]):
             for x, y in zip(a, c):
