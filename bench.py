@@ -287,10 +287,11 @@ def bench_elastic3d(sync, steps, gradient_shots=4, shape=None):
     dt 0.8 ms, 20-cell CFS-MPML on five faces and a flat free surface,
     vp a smooth random field in 2000..5000 m/s, vs = vp/1.73, Gardner rho,
     8 vertical-force shots, 50 x 50 three-component receivers.  Then OWL's
-    gradient of vp, vs, rho (boundary records in HBM, reverse reconstruction
-    and adjoint, FP64 imaging) over the first `gradient_shots` shots, whose
-    boundary records of all steps (26 GB a shot) fit in HBM together."""
-    from owl_amd import Elastic3DBatch
+    gradient of vp, vs, rho over the first `gradient_shots` shots (all 8 by
+    default): boundary records of every step in HBM as BF16 with one scale
+    per shot and step (13 GB a shot), reverse reconstruction and adjoint,
+    FP64 imaging."""
+    from owl_amd import OWL_STORE_BF16, Elastic3DBatch
     from owl_amd.synthetic import elastic3d_model, elastic3d_shots
     vp, vs, rho = elastic3d_model()
     shots = elastic3d_shots()
@@ -316,6 +317,7 @@ def bench_elastic3d(sync, steps, gradient_shots=4, shape=None):
         bg = Elastic3DBatch(200, 200, 150, 10.0, 10.0, 10.0, vp, vs, rho, sub, dt, (steps - 1) * dt, pml=20,
                             fdhalf=4, free_surface=True)
         bg.set_launch(*shape)
+        bg.set_record_storage(OWL_STORE_BF16)
         obs = [tuple(np.zeros((nr, bg.nt), np.float32) for _ in range(3)) for nr in bg.nr]
         sync()
         t0 = time.perf_counter()
@@ -326,9 +328,11 @@ def bench_elastic3d(sync, steps, gradient_shots=4, shape=None):
         bg.close()
         grad = {"shots": len(sub), "seconds": gw, "s_per_shot": gw / len(sub), "device_ms": gms,
                 "grid_pt_updates_per_s": gup / gw,
-                "algorithm": "OWL's gradient of vp, vs, rho: forward with boundary records in HBM, L2 adjoint "
-                             "source per component (observed traces zero), reverse reconstruction and adjoint, "
-                             "FP64 imaging, merged on the model; first call (allocations included)",
+                "algorithm": "OWL's gradient of vp, vs, rho: forward with boundary records in HBM (BF16, one "
+                             "scale per shot and step), L2 adjoint source per component (observed traces zero), "
+                             "reverse reconstruction and adjoint, FP64 imaging, merged on the model; first call "
+                             "(allocations included)",
+                "records": "bf16",
                 "finite": bool(np.isfinite(gv).all() and np.isfinite(gs).all() and np.isfinite(gr).all())}
     _, roof = roofline_of(stats)
     return {"workload": f"3D isotropic elastic batched forward, 200x200x150, 10 m, dt 0.8 ms, R 4, pml 20 on five "
@@ -348,10 +352,10 @@ def bench_fsg(sync, steps, gradient=True, shape=None):
     forward runs once eagerly with per-kernel events (the kernel table and
     roofline), then as one captured HIP graph of the time loop: captured by
     an untimed run, the replay is timed.  Then OWL's gradient
-    (boundary-saving reconstruction, c11 .. c55 and rho, FP64 imaging,
-    mapped back to the regular windows and summed) over all shots, without
-    events."""
-    from owl_amd import FSG2DBatch
+    (boundary records in HBM as FP16 with one scale per shot and step,
+    reconstruction, c11 .. c55 and rho, FP64 imaging, mapped back to the
+    regular windows and summed) over all shots, without events."""
+    from owl_amd import OWL_STORE_F16, FSG2DBatch
     from owl_amd.synthetic import fsg2d_shots, fsg2d_tti_model
     cij, rho = fsg2d_tti_model()
     shots = fsg2d_shots()
@@ -383,6 +387,7 @@ def bench_fsg(sync, steps, gradient=True, shape=None):
     ms, updates = b.last_stats()
     grad = None
     if gradient:
+        b.set_record_storage(OWL_STORE_F16)
         obs = [tuple(np.zeros((nr, b.nt), np.float32) for _ in range(2)) for nr in b.nr]
         b.gradient(obs)  # untimed warm-up: buffers, strips
         sync()
@@ -393,10 +398,11 @@ def bench_fsg(sync, steps, gradient=True, shape=None):
         gms, gup = b.last_stats()
         grad = {"seconds": gw, "s_per_shot": gw / len(shots), "device_ms": gms,
                 "grid_pt_updates_per_s": gup / gw,
-                "algorithm": "OWL's gradient: forward with boundary records in HBM, L2 adjoint source per "
-                             "component (observed traces zero), reverse reconstruction and adjoint, FP64 "
-                             "imaging of c11..c55 and rho on the refined meshes, mapped to the regular "
-                             "windows and summed on the host",
+                "algorithm": "OWL's gradient: forward with boundary records in HBM (FP16, one scale per shot "
+                             "and step), L2 adjoint source per component (observed traces zero), reverse "
+                             "reconstruction and adjoint, FP64 imaging of c11..c55 and rho on the refined "
+                             "meshes, mapped to the regular windows and summed on the host",
+                "records": "fp16",
                 "finite": bool(all(np.isfinite(v).all() for v in g.values()))}
     b.close()
     _, roof = roofline_of(stats)
@@ -442,7 +448,7 @@ def main():
     ap.add_argument("--elastic3d-steps", type=int, default=2500, help="time steps of the 3D elastic forward")
     ap.add_argument("--elastic3d-shape", default="", metavar="PLANES,ROWS",
                     help="this launch shape of the 3D elastic sweeps instead of the autotuner's (counter runs)")
-    ap.add_argument("--elastic3d-gradient-shots", type=int, default=4,
+    ap.add_argument("--elastic3d-gradient-shots", type=int, default=8,
                     help="shots of the timed 3D elastic gradient (0 skips it)")
     ap.add_argument("--only-fsg", action="store_true", help="only the 2D FSG forward (for profiler runs)")
     ap.add_argument("--fsg-steps", type=int, default=5000, help="time steps of the 2D FSG forward")

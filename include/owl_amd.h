@@ -451,6 +451,8 @@ owl_status owl_elastic3d_set_launch(owl_elastic3d *b, int planes, int rows);
  * keeps the fastest and returns it (planes, rows may be NULL).  No
  * reference counterpart. */
 owl_status owl_elastic3d_autotune(owl_elastic3d *b, int steps, int *planes, int *rows);
+/* As owl_fsg2d_set_record_storage. */
+owl_status owl_elastic3d_set_record_storage(owl_elastic3d *b, int kind);
 /* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
  * sweep, 2 sources and receivers, 3 imaging, 4 boundary records. */
 owl_status owl_elastic3d_set_profiling(owl_elastic3d *b, int on);
@@ -519,6 +521,8 @@ owl_status owl_elastic2d_set_launch(owl_elastic2d *b, int planes, int rows);
  * and velocity sweeps and forward loop.  No reference counterpart. */
 owl_status owl_elastic2d_autotune(owl_elastic2d *b, int steps, int *planes, int *rows);
 owl_status owl_elastic2d_set_graphs(owl_elastic2d *b, int on);
+/* As owl_fsg2d_set_record_storage. */
+owl_status owl_elastic2d_set_record_storage(owl_elastic2d *b, int kind);
 /* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity
  * sweep, 2 sources and receivers, 3 imaging, 4 boundary records. */
 owl_status owl_elastic2d_set_profiling(owl_elastic2d *b, int on);
@@ -588,6 +592,14 @@ owl_status owl_tti2d_param_gradient(int param, long long n, const float *vp, con
 typedef struct owl_fsg2d owl_fsg2d;
 owl_status owl_fsg2d_create(const owl_acoustic_config *cfg, const float *const *cij, const float *rho,
                             int aniso, int nshot, const owl_shot *shots, owl_fsg2d **out);
+/* A new medium (c11 .. c55 and rho as for owl_fsg2d_create) for the same
+ * shots: the batch is planned again (free-surface meshes, damping ratios,
+ * stability check, sources' rho) and keeps its stream, launch shape and
+ * storage settings; boundary records of an earlier forward are dropped.
+ * OWL's iterations compute each gradient on the current model
+ * (src/inverse/2d/module_gradient.f90:57-417).  A failed call (OWL_ERR_CFL,
+ * OWL_ERR_ARG) leaves the batch unusable: destroy it. */
+owl_status owl_fsg2d_set_model(owl_fsg2d *b, const float *const *cij, const float *rho);
 void owl_fsg2d_destroy(owl_fsg2d *b);
 int owl_fsg2d_nt(const owl_fsg2d *b);
 /* The damping-ratio profiles of one shot (compute_damping_profile_ratio,
@@ -629,6 +641,12 @@ owl_status owl_fsg2d_autotune(owl_fsg2d *b, int steps, int *planes, int *rows);
  * replays the forward time loop as one captured HIP graph, re-captured when
  * a launch argument changes; 0 launches eagerly. */
 owl_status owl_fsg2d_set_graphs(owl_fsg2d *b, int on);
+/* Storage of the boundary records that OWL's gradient saves each forward
+ * step (elastic_tti_2d/module_boundary_saving.f90:55-200): OWL_STORE_F32 (OWL's, the default),
+ * OWL_STORE_F16 or OWL_STORE_BF16 with one scale per (shot, step) record,
+ * halving their HBM; takes effect at the next forward with boundary
+ * records. */
+owl_status owl_fsg2d_set_record_storage(owl_fsg2d *b, int kind);
 /* Timing; kinds: 0 stress sweep, 1 velocity sweep, 2 sources and
  * receivers, 3 imaging, 4 boundary records. */
 owl_status owl_fsg2d_set_profiling(owl_fsg2d *b, int on);

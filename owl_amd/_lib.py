@@ -143,6 +143,7 @@ _SIGNATURES = {
     "owl_elastic2d_adjoint": (C.c_int, [_P, _P, _P, _P, _P]),
     "owl_elastic2d_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "owl_elastic2d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_elastic2d_set_record_storage": (C.c_int, [_P, C.c_int]),
     "owl_elastic2d_autotune": (C.c_int, [_P, C.c_int, _P, _P]),
     "owl_elastic2d_set_graphs": (C.c_int, [_P, C.c_int]),
     "owl_elastic2d_set_profiling": (C.c_int, [_P, C.c_int]),
@@ -158,6 +159,8 @@ _SIGNATURES = {
     "owl_fsg2d_adjoint": (C.c_int, [_P, _P, _P]),
     "owl_fsg2d_gradient": (C.c_int, [_P, _P, _P, _P, _P]),
     "owl_fsg2d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_fsg2d_set_model": (C.c_int, [_P, _P, _P]),
+    "owl_fsg2d_set_record_storage": (C.c_int, [_P, C.c_int]),
     "owl_fsg2d_autotune": (C.c_int, [_P, C.c_int, _P, _P]),
     "owl_fsg2d_set_graphs": (C.c_int, [_P, C.c_int]),
     "owl_fsg2d_set_profiling": (C.c_int, [_P, C.c_int]),
@@ -176,6 +179,7 @@ _SIGNATURES = {
     "owl_elastic3d_adjoint": (C.c_int, [_P, _P, _P, _P, _P]),
     "owl_elastic3d_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "owl_elastic3d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "owl_elastic3d_set_record_storage": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_autotune": (C.c_int, [_P, C.c_int, _P, _P]),
     "owl_elastic3d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),

@@ -26,6 +26,7 @@
 #include "host_prep.hpp"
 #include "owl_amd.h"
 #include "profiler.hpp"
+#include "record_store.hpp"
 #include "trace_misfit.hpp"
 
 namespace owl {
@@ -373,13 +374,16 @@ __host__ __device__ inline long long strip2_len(int nx, int nz, int R) {
 }
 
 template <int R>
-__global__ void k2e_strips(const ShotE2* __restrict__ shots, float* vx, float* vz, float* strips, int t, int pml,
-                           int save) {
+__global__ void k2e_strips(const ShotE2* __restrict__ shots, float* vx, float* vz, RecStore st, int t, int pml,
+                           int mode) {
     const ShotE2 S = shots[blockIdx.y];
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const long long ex = nx + 2 * R, ez = nz + 2 * R;
     const long long a = R * ez, b = ex * R, one = 2 * a + 2 * b, len = 2 * one;
-    float* rec = strips + S.strip0 + (long long)(t - 1) * len;
+    const long long rec0 = S.strip0 + (long long)(t - 1) * len;
+    const int rid = int(blockIdx.y) * st.nt + (t - 1);
+    const float sc = mode == kRecAmax ? 1.0f : rec_scale(st, rid), inv = 1.0f / sc;
+    float m = 0.0f;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
          e += (long long)gridDim.x * blockDim.x) {
         const int comp = int(e / one);
@@ -399,11 +403,14 @@ __global__ void k2e_strips(const ShotE2* __restrict__ shots, float* vx, float* v
         }
         float* f = comp == 0 ? vx : vz;
         const size_t q = at2(S, lo, i, j);
-        if (save)
-            rec[e] = f[q];
+        if (mode == kRecAmax)
+            m = fmaxf(m, fabsf(f[q]));
+        else if (mode == kRecSave)
+            rec_put(st, rec0 + e, f[q], inv);
         else
-            f[q] = rec[e];
+            f[q] = rec_get(st, rec0 + e, sc);
     }
+    if (mode == kRecAmax) rec_amax(st, rid, m);
 }
 
 // Adjoint sources into the adjoint velocities: the receivers' taps,
@@ -543,6 +550,8 @@ struct owl_elastic2d {
     DevBuf<ShotE2> d_shots;
     DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg, d_adj_beg, d_win;
     DevBuf<float> d_prof, d_amp, d_rec_w, d_traces, d_strips, d_adj, d_obs, d_sc;
+    DevBuf<unsigned> d_rscale;  // per (shot, step) boundary record: largest magnitude (16-bit storage)
+    int rec_store = OWL_STORE_F32;
     DevBuf<float> med[6];   // c11, c13, mu (at sxz), rho_x, rho_z, c55 at the nodes
     DevBuf<float> fld[13];  // five wavefields, eight memory variables
     DevBuf<float> fB[5];    // the other half of the reverse ping-pong
@@ -1137,10 +1146,14 @@ void strips2(owl_elastic2d* b, float* vx, float* vz, int t, int save) {
     long long mx = 1;
     for (auto& S : b->sd) mx = std::max(mx, strip2_len(S.nx, S.nz, b->R));
     const dim3 grd(unsigned(std::min<long long>(1024, (mx + 255) / 256)), unsigned(b->sd.size()));
+    const RecStore st{b->d_strips.get(), b->d_rscale.get(), b->rec_store, b->nt};
     const int ev = b->prof.begin(b->stream, 4, 0);
     dispatch_R2e(b->R, [&](auto r) {
+        if (save && b->rec_store != OWL_STORE_F32)
+            hipLaunchKernelGGL(k2e_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(), vx,
+                               vz, st, t, b->pml, int(kRecAmax));
         hipLaunchKernelGGL(k2e_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(), vx, vz,
-                           b->d_strips.get(), t, b->pml, save);
+                           st, t, b->pml, save ? int(kRecSave) : int(kRecRestore));
     });
     b->prof.end(b->stream, ev);
 }
@@ -1166,7 +1179,7 @@ void run_forward2(owl_elastic2d* b, bool save_boundary) {
     hipStream_t st = b->stream;
     for (auto& f : b->fld) f.zero(st);
     b->d_traces.zero(st);
-    if (save_boundary) b->d_strips.alloc(std::max<size_t>(b->strip_elems, 1));
+    if (save_boundary) alloc_records(b->d_strips, b->d_rscale, b->strip_elems, b->rec_store, b->sd.size(), b->nt, st);
     const EF2 F = fields2(b->fld);
     const unsigned nshot = unsigned(b->sd.size());
     double bs, bv;
@@ -1606,6 +1619,19 @@ owl_status owl_elastic2d_set_launch(owl_elastic2d* b, int planes, int rows) {
         b->d_shots.upload(b->sd, b->stream);
         b->d_tile_shot.upload(b->tile_shot, b->stream);
         OWL_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+owl_status owl_elastic2d_set_record_storage(owl_elastic2d* b, int kind) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        check_record_kind(kind);
+        if (kind != b->rec_store) {
+            b->rec_store = kind;
+            b->d_strips.release();
+            b->have_boundary = false;
+            b->fwd_state_valid = false;
+        }
     });
 }
 

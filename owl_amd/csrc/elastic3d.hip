@@ -21,6 +21,7 @@
 #include "host_prep.hpp"
 #include "owl_amd.h"
 #include "profiler.hpp"
+#include "record_store.hpp"
 
 namespace owl {
 namespace e3 {
@@ -604,13 +605,16 @@ __host__ __device__ inline EStrip estrip_geo(int nx, int ny, int nz, int R) {
 
 // save != 0: record of step t <- (vx, vy, vz); else the reverse.
 template <int R>
-__global__ void ke_strips(const ShotE* __restrict__ shots, float* vx, float* vy, float* vz, float* strips, int t,
-                          int pml, int save) {
+__global__ void ke_strips(const ShotE* __restrict__ shots, float* vx, float* vy, float* vz, RecStore st, int t,
+                          int pml, int mode) {
     const ShotE S = shots[blockIdx.y];
     const EStrip g = estrip_geo(S.nx, S.ny, S.nz, R);
     const long long one = 2 * (g.a + g.b + g.c), len = 3 * one;
     const int lo = 1 - pml - R, nx = S.nx, ny = S.ny, nz = S.nz;
-    float* rec = strips + S.strip0 + (long long)(t - 1) * len;
+    const long long rec0 = S.strip0 + (long long)(t - 1) * len;
+    const int rid = int(blockIdx.y) * st.nt + (t - 1);
+    const float sc = mode == kRecAmax ? 1.0f : rec_scale(st, rid), inv = 1.0f / sc;
+    float m = 0.0f;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
          e += (long long)gridDim.x * blockDim.x) {
         const int comp = int(e / one);
@@ -638,11 +642,14 @@ __global__ void ke_strips(const ShotE* __restrict__ shots, float* vx, float* vy,
         }
         float* f = comp == 0 ? vx : (comp == 1 ? vy : vz);
         const size_t q = ate(S, lo, i, j, k);
-        if (save)
-            rec[e] = f[q];
+        if (mode == kRecAmax)
+            m = fmaxf(m, fabsf(f[q]));
+        else if (mode == kRecSave)
+            rec_put(st, rec0 + e, f[q], inv);
         else
-            f[q] = rec[e];
+            f[q] = rec_get(st, rec0 + e, sc);
     }
+    if (mode == kRecAmax) rec_amax(st, rid, m);
 }
 
 // Adjoint sources into the adjoint velocities (submodule_fwi_gradient.f90:
@@ -881,6 +888,8 @@ struct owl_elastic3d {
     // gradient: boundary records, the other half of the reverse ping-pong,
     // the adjoint field, adjoint-source CSR, accumulators
     DevBuf<float> d_strips, d_adj, d_obs;
+    DevBuf<unsigned> d_rscale;  // per (shot, step) boundary record: largest magnitude (16-bit storage)
+    int rec_store = OWL_STORE_F32;
     DevBuf<float> fB[9];
     DevBuf<float> aF[27];
     DevBuf<int> d_adj_beg, d_win;
@@ -1366,7 +1375,7 @@ void run_forward_e(owl_elastic3d* b, bool save_boundary) {
     hipStream_t st = b->stream;
     for (auto& f : b->fld) f.zero(st);
     b->d_traces.zero(st);
-    if (save_boundary) b->d_strips.alloc(std::max<size_t>(b->strip_elems, 1));
+    if (save_boundary) alloc_records(b->d_strips, b->d_rscale, b->strip_elems, b->rec_store, b->sd.size(), b->nt, st);
     const EF F = fields(b);
     const unsigned nshot = unsigned(b->sd.size());
     double bs, bv;
@@ -1416,10 +1425,14 @@ void strips_e(owl_elastic3d* b, int R, float* vx, float* vy, float* vz, int t, i
         mx = std::max<size_t>(mx, size_t(6 * (g.a + g.b + g.c)));
     }
     const dim3 grd(unsigned(std::min<size_t>(4096, (mx + 255) / 256)), unsigned(b->sd.size()));
+    const RecStore st{b->d_strips.get(), b->d_rscale.get(), b->rec_store, b->nt};
     const int ev = b->prof.begin(b->stream, 4, 0);
     dispatch_Re(R, [&](auto r) {
+        if (save && b->rec_store != OWL_STORE_F32)
+            hipLaunchKernelGGL(ke_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(), vx, vy,
+                               vz, st, t, b->pml, int(kRecAmax));
         hipLaunchKernelGGL(ke_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(), vx, vy, vz,
-                           b->d_strips.get(), t, b->pml, save);
+                           st, t, b->pml, save ? int(kRecSave) : int(kRecRestore));
     });
     b->prof.end(b->stream, ev);
 }
@@ -1741,6 +1754,19 @@ owl_status owl_elastic3d_autotune(owl_elastic3d* b, int steps, int* planes, int*
         b->have_boundary = false;
         if (planes) *planes = bk;
         if (rows) *rows = by;
+    });
+}
+
+owl_status owl_elastic3d_set_record_storage(owl_elastic3d* b, int kind) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        check_record_kind(kind);
+        if (kind != b->rec_store) {
+            b->rec_store = kind;
+            b->d_strips.release();
+            b->have_boundary = false;
+            b->fwd_state_valid = false;
+        }
     });
 }
 

@@ -36,6 +36,7 @@
 #include "host_prep.hpp"
 #include "owl_amd.h"
 #include "profiler.hpp"
+#include "record_store.hpp"
 #include "trace_misfit.hpp"
 
 namespace owl {
@@ -640,12 +641,15 @@ __host__ __device__ inline long long stripf_len(int nx, int nz, int R) {
 }
 
 template <int R>
-__global__ void k_fsg_strips(const ShotF* __restrict__ shots, Targets T, float* strips, int t, int pml, int save) {
+__global__ void k_fsg_strips(const ShotF* __restrict__ shots, Targets T, RecStore st, int t, int pml, int mode) {
     const ShotF S = shots[blockIdx.y];
     const int nx = S.nx, nz = S.nz, lo = 1 - pml - R;
     const long long ex = nx + 2 * R, ez = nz + 2 * R;
     const long long a = R * ez, b = ex * R, one = 2 * a + 2 * b, len = 4 * one;
-    float* rec = strips + S.strip0 + (long long)(t - 1) * len;
+    const long long rec0 = S.strip0 + (long long)(t - 1) * len;
+    const int rid = int(blockIdx.y) * st.nt + (t - 1);
+    const float sc = mode == kRecAmax ? 1.0f : rec_scale(st, rid), inv = 1.0f / sc;
+    float m = 0.0f;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
          e += (long long)gridDim.x * blockDim.x) {
         const int comp = int(e / one);
@@ -665,11 +669,14 @@ __global__ void k_fsg_strips(const ShotF* __restrict__ shots, Targets T, float* 
         }
         float* f = T.f[comp];
         const size_t q = atf(S, lo, i, j);
-        if (save)
-            rec[e] = f[q];
+        if (mode == kRecAmax)
+            m = fmaxf(m, fabsf(f[q]));
+        else if (mode == kRecSave)
+            rec_put(st, rec0 + e, f[q], inv);
         else
-            f[q] = rec[e];
+            f[q] = rec_get(st, rec0 + e, sc);
     }
+    if (mode == kRecAmax) rec_amax(st, rid, m);
 }
 
 // Adjoint sources into both adjoint velocity grids (submodule_fwi_gradient.
@@ -811,6 +818,8 @@ struct owl_fsg2d {
     DevBuf<ShotF> d_shots;
     DevBuf<int> d_tile_shot, d_inj_beg, d_rec_beg, d_adj_beg, d_win;
     DevBuf<float> d_amp, d_rec_w, d_traces, d_strips, d_adj, d_obs;
+    DevBuf<unsigned> d_rscale;  // per (shot, step) boundary record: largest magnitude (16-bit storage)
+    int rec_store = OWL_STORE_F32;
     DevBuf<float> d_met;  // free-surface metric tables
     DevBuf<float> med[14];  // ci[6], ch[6], qhx, qhz
     DevBuf<float> pmlc[16];
@@ -1565,10 +1574,14 @@ void stripsf(owl_fsg2d* b, const FV& V, int t, int save) {
     long long mx = 1;
     for (auto& S : b->sd) mx = std::max(mx, stripf_len(S.nx, S.nz, b->R));
     const dim3 grd(unsigned(std::min<long long>(1024, (mx + 255) / 256)), unsigned(b->sd.size()));
+    const RecStore st{b->d_strips.get(), b->d_rscale.get(), b->rec_store, b->nt};
     const int ev = b->prof.begin(b->stream, 4, 0);
     dispatch_Rf(b->R, [&](auto r) {
+        if (save && b->rec_store != OWL_STORE_F32)
+            hipLaunchKernelGGL(k_fsg_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(),
+                               vel_targets(V), st, t, b->pml, int(kRecAmax));
         hipLaunchKernelGGL(k_fsg_strips<decltype(r)::value>, grd, dim3(256), 0, b->stream, b->d_shots.get(),
-                           vel_targets(V), b->d_strips.get(), t, b->pml, save);
+                           vel_targets(V), st, t, b->pml, save ? int(kRecSave) : int(kRecRestore));
     });
     b->prof.end(b->stream, ev);
 }
@@ -1594,7 +1607,7 @@ void run_forwardf(owl_fsg2d* b, bool save_boundary) {
     hipStream_t st = b->stream;
     for (auto& f : b->fld) f.zero(st);
     b->d_traces.zero(st);
-    if (save_boundary) b->d_strips.alloc(std::max<size_t>(b->strip_elems, 1));
+    if (save_boundary) alloc_records(b->d_strips, b->d_rscale, b->strip_elems, b->rec_store, b->sd.size(), b->nt, st);
     const FF F = fieldsf(b->fld);
     const FV V = out_of(F);
     const unsigned nshot = unsigned(b->sd.size());
@@ -1852,6 +1865,23 @@ owl_status owl_fsg2d_create(const owl_acoustic_config* cfg, const float* const* 
     });
 }
 
+owl_status owl_fsg2d_set_model(owl_fsg2d* b, const float* const* cij, const float* rho) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        // the shots as the batch was created with (planning replaces hs)
+        const std::vector<ShotHostF> old = b->hs;
+        std::vector<owl_shot> shots;
+        for (const ShotHostF& H : old)
+            shots.push_back(owl_shot{H.id, int(H.src.size()), H.src.data(), int(H.rec.size()), H.rec.data(),
+                                     H.nxbeg, H.nxbeg + H.nx - 1, 0, 0, H.nzbeg, H.nzbeg + H.nz - 1});
+        const owl_acoustic_config cfg = b->cfg;
+        b->have_boundary = false;
+        b->fwd_state_valid = false;
+        planf(b, &cfg, cij, rho, b->aniso, int(shots.size()), shots.data());
+        OWL_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
 void owl_fsg2d_destroy(owl_fsg2d* b) { delete b; }
 
 int owl_fsg2d_nt(const owl_fsg2d* b) { return b ? b->nt : 0; }
@@ -2059,6 +2089,19 @@ owl_status owl_fsg2d_set_launch(owl_fsg2d* b, int planes, int rows) {
         b->d_shots.upload(b->sd, b->stream);
         b->d_tile_shot.upload(b->tile_shot, b->stream);
         OWL_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+owl_status owl_fsg2d_set_record_storage(owl_fsg2d* b, int kind) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        check_record_kind(kind);
+        if (kind != b->rec_store) {
+            b->rec_store = kind;
+            b->d_strips.release();
+            b->have_boundary = false;
+            b->fwd_state_valid = false;
+        }
     });
 }
 

@@ -92,6 +92,11 @@ class TwoComponentBatch:
         check(self._fn("autotune")(self._h, int(steps), C.byref(p), C.byref(r)))
         return p.value, r.value
 
+    def set_record_storage(self, store):
+        """Storage of the boundary records of OWL's gradient: OWL_STORE_F32
+        (default), OWL_STORE_F16 or OWL_STORE_BF16 (scaled per record)."""
+        check(self._fn("set_record_storage")(self._h, int(store)))
+
     def set_graphs(self, on=True):
         """True (default): forward modelling without boundary records or
         profiling replays its time loop as one captured HIP graph."""

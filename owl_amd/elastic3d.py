@@ -75,6 +75,11 @@ class Elastic3DBatch:
     def set_launch(self, planes, rows=4):
         check(lib().owl_elastic3d_set_launch(self._h, int(planes), int(rows)))
 
+    def set_record_storage(self, store):
+        """Storage of the boundary records of OWL's gradient: OWL_STORE_F32
+        (default), OWL_STORE_F16 or OWL_STORE_BF16 (scaled per record)."""
+        check(lib().owl_elastic3d_set_record_storage(self._h, int(store)))
+
     def autotune(self, steps=3):
         """Times `steps` steps of the sweeps under each launch shape and keeps
         the fastest; returns (planes, rows)."""

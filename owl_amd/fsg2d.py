@@ -32,6 +32,13 @@ PARAMS = {"iso": (0, ("vp", "vs")), "thomsen": (1, ("vp", "vs", "epsilon", "delt
           "a-t": (2, ("vp", "vs", "epsilon", "eta"))}
 
 
+def _begin(win):
+    if win is None:
+        return 1, 1
+    g = win.get if isinstance(win, dict) else (lambda k, d=0: getattr(win, k, d))
+    return g("nzbeg", 0) or 1, g("nxbeg", 0) or 1
+
+
 def _f32(a, shape):
     a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), shape))
     return a
@@ -152,6 +159,15 @@ class FSG2DBatch(TwoComponentBatch):
                                                  C.byref(h)))
         self._h = h
         self.nt = lib().owl_fsg2d_nt(h)
+        self.window_begins = [_begin(win) for _, win in shots]  # (nzbeg, nxbeg) per shot
+
+    def set_model(self, cij, rho):
+        """A new medium for the same shots (owl_fsg2d_set_model: the batch is
+        planned again; launch shape and storage settings stay)."""
+        self._m = [self._model(cij[k]) for k in CIJ]
+        self._rho = self._model(rho)
+        ptrs = (C.c_void_p * 6)(*[a.ctypes.data for a in self._m])
+        check(lib().owl_fsg2d_set_model(self._h, ptrs, _ptr(self._rho)))
 
     def rows(self, shot=0):
         """Rows of one shot's mesh (refined with the free surface)."""
