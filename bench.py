@@ -52,10 +52,16 @@ TRAFFIC = {
     # (z planes, y rows per tile): elastic3d_forward_200x200x150_8shots.md
     ("elastic3d_forward", "stress", (64, 4)): (2.0 * 6.017e6 + 2.686e6) * 1024,
     ("elastic3d_forward", "velocity", (64, 4)): (2.0 * 5.066e6 + 1.602e6) * 1024,
+    ("elastic3d_forward", "stress", (8, 4)): (2.0 * 3.730e6 + 2.581e6) * 1024,
+    ("elastic3d_forward", "velocity", (8, 4)): (2.0 * 2.977e6 + 1.501e6) * 1024,
     # (rows per tile column, tiles per block) of the FSG sweeps:
     # fsg2d_forward_free_surface_801x401_64shots.md
     ("fsg2d_forward", "stress", (32, 4)): (2.0 * 7.750e5 + 3.767e5) * 1024,
     ("fsg2d_forward", "velocity", (32, 4)): (2.0 * 4.983e5 + 2.834e5) * 1024,
+    ("fsg2d_forward", "stress", (8, 2)): (2.0 * 9.090e5 + 3.793e5) * 1024,
+    ("fsg2d_forward", "velocity", (8, 2)): (2.0 * 6.567e5 + 2.862e5) * 1024,
+    ("fsg2d_forward", "stress", (4, 2)): (2.0 * 9.996e5 + 3.790e5) * 1024,
+    ("fsg2d_forward", "velocity", (4, 2)): (2.0 * 8.005e5 + 2.863e5) * 1024,
 }
 
 
