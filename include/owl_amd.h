@@ -551,6 +551,23 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
 #define OWL_MISFIT_CORR 1      /* zero-lag normalised correlation (:391-413) */
 #define OWL_MISFIT_ENVELOPE 2  /* envelope difference (:424-452) */
 #define OWL_MISFIT_PHASE 3     /* instantaneous-phase difference (:460-481) */
+#define OWL_MISFIT_ADAPTIVE 4  /* adaptive waveform inversion, Warner & Guasch (:490-537) */
+
+/* Lag penalties of the adaptive misfit (xcorr_penalty, :321-363). */
+#define OWL_PENALTY_LINEAR 0
+#define OWL_PENALTY_POWER 1
+#define OWL_PENALTY_GAUSSIAN 2
+#define OWL_PENALTY_EXP 3
+
+/* Parameters of the adaptive misfit (module_parameters.f90:262-265,
+ * 551-557). */
+typedef struct owl_misfit_params {
+    float dt;            /* trace sampling (s) */
+    float tlag_max;      /* largest penalised lag (s); OWL's default 0.1 tmax */
+    int penalty;         /* OWL_PENALTY_*; OWL's default linear */
+    float penalty_power; /* default 1 */
+    float deconv_eps;    /* Wiener-filter stabilisation relative to max |D|^2, default 0.1 */
+} owl_misfit_params;
 
 /* Misfit and adjoint source of the ntr traces of one shot component, on the
  * GPU (compute_adjoint_source, module_inversion_adjoint_source.f90:985-1160,
@@ -563,6 +580,17 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
  * instant_phase are not in the reference tree). */
 owl_status owl_adjoint_source(int type, int ntr, int nt, const float *obs, const float *syn,
                               const float *weight, float threshold, float *adj, double *misfit);
+/* As owl_adjoint_source, with the parameters that OWL_MISFIT_ADAPTIVE needs
+ * (p may be NULL for the other types).  The adaptive misfit per trace is
+ * 0.5 sum (w xc)^2 / sum xc^2 of the Wiener matching filter xc of syn onto
+ * obs (both tapered over nint(0.025 nt) samples at each end and padded to the
+ * 2-3-5 length >= 2 nt), w the lag penalty; its adjoint source is the
+ * gradient of that misfit (tapered at the end).  FLIT's taper, fft and
+ * rescale are taken as a Hann half-window, the unnormalised forward and
+ * 1/n inverse DFT, and the linear [min, max] -> [0, 1] map. */
+owl_status owl_adjoint_source_ex(int type, const owl_misfit_params *p, int ntr, int nt, const float *obs,
+                                 const float *syn, const float *weight, float threshold, float *adj,
+                                 double *misfit);
 
 /* Parameterisations of a 2D TTI medium (anisotropy_type of OWL's
  * elastic_tti_2d solver, module_vars.f90:284-340). */

@@ -1,14 +1,21 @@
 """Oracle (test infrastructure only): numpy restatement of OWL's misfits and
 adjoint sources (src/inverse/module_inversion_adjoint_source.f90:
 compute_adjsrc_waveform :371-383, compute_adjsrc_corr :391-413,
-compute_adjsrc_envelope :424-452, compute_adjsrc_phase :460-481) and the
+compute_adjsrc_envelope :424-452, compute_adjsrc_phase :460-481,
+compute_adjsrc_adaptive :490-537 with xcorr_penalty :321-363) and the
 trace-discard rule of compute_adjoint_source (:1042-1054), in float64.
 
 FLIT's hilbert, envelope and instant_phase are not in the reference tree;
 they are taken as the textbook analytic signal of a length-nt FFT
 (z = x + i H(x)), |z| and atan2(Im z, Re z).  Pinned in
 tests/test_misfit_oracle.py by closed forms (H(cos) = sin) and by central
-differences of each misfit (the stabilisation terms set to 0).
+differences of each misfit (the stabilisation terms set to 0).  For the
+adaptive misfit FLIT's taper, fft/ifft, next_power_235 and rescale are
+taken as a Hann half-window, numpy's FFT conventions, the smallest 2-3-5
+length and the linear [min, max] -> [0, 1] map; pinned by the closed form
+of a shifted broadband trace (misfit = penalty(shift)^2 / 2) and by central
+differences (the adjoint is the misfit's gradient outside the start taper,
+and the gradient over the taper ramp there).
 """
 import numpy as np
 
@@ -60,12 +67,90 @@ def phase(obs, syn, eps=5.0e-2):
     return mis, -(phiu - phid) * zu.imag / env + hilbert((phiu - phid) * syn / env)
 
 
+def next_power_235(n):
+    """Smallest m >= n whose only prime factors are 2, 3 and 5 (FLIT's
+    next_power_235)."""
+    m = max(int(n), 1)
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def hann_ramp(n):
+    """Rising half of a Hann window over n samples, 0 at the first: FLIT's
+    taper default, taken as this (parity unpinned for the shape; the tests
+    pin the adjoint against the misfit whatever the shape)."""
+    k = np.arange(n, dtype=np.float64)
+    return 0.5 - 0.5 * np.cos(np.pi * k / max(n, 1))
+
+
+def taper1(x, a, b):
+    y = np.array(x, np.float64)
+    if a:
+        y[:a] *= hann_ramp(a)
+    if b:
+        y[len(y) - b:] *= hann_ramp(b)[::-1]
+    return y
+
+
+def xcorr_penalty(dt, maxlag, tlag_max, method="linear", power=1.0):
+    """xcorr_penalty (module_inversion_adjoint_source.f90:321-363) over lags
+    -maxlag .. maxlag; FLIT's rescale(w, [0, 1]) taken as the linear map of
+    [min, max] onto [0, 1]."""
+    i = np.arange(-maxlag, maxlag + 1, dtype=np.float64)
+    scaling = 1.0 / np.sqrt(2.0 * np.log(100.0))
+    if method == "power":
+        return np.abs(i * dt / tlag_max) ** abs(power)
+    if method in ("gaussian", "exp"):
+        w = (np.exp(-(i * dt) ** 2 / (scaling * tlag_max ** 2)) if method == "gaussian"
+             else np.exp(-np.abs(i * dt) / (scaling * tlag_max)))
+        w = (w - w.min()) / (w.max() - w.min())
+        return 1.0 - w ** abs(power)
+    return np.abs(i * dt / tlag_max)
+
+
+def adaptive(obs, syn, dt, tlag_max, method="linear", power=1.0, eps=0.1):
+    """compute_adjsrc_adaptive (module_inversion_adjoint_source.f90:490-537):
+    the Wiener matching filter of syn onto obs on a 2-3-5 FFT length of at
+    least 2 nt, weighted by the lag penalty; (misfit, adjoint source)."""
+    obs = np.asarray(obs, np.float64)
+    syn = np.asarray(syn, np.float64)
+    n = len(obs)
+    maxlag = int(np.ceil(tlag_max / dt))
+    nnt = next_power_235(2 * n)
+    tl = int(np.floor(0.025 * n + 0.5))
+    if maxlag + 1 > nnt:
+        raise ValueError("tlag_max longer than the padded trace")
+    u = np.zeros(nnt)
+    d = np.zeros(nnt)
+    u[:n] = taper1(syn, tl, tl)
+    d[:n] = taper1(obs, tl, tl)
+    U, D = np.fft.fft(u), np.fft.fft(d)
+    den = (D * np.conj(D)).real
+    den = den + eps * den.max()
+    xc = np.fft.ifft(U * np.conj(D) / den).real
+    p = xcorr_penalty(dt, maxlag, tlag_max, method, power)
+    w = np.zeros(nnt)
+    w[:maxlag + 1] = p[maxlag:]
+    if maxlag:
+        w[nnt - maxlag:] = p[:maxlag]
+    sxc = np.sum(xc ** 2)
+    mis = 0.5 * np.sum((w * xc) ** 2) / sxc
+    a = np.fft.ifft(np.fft.fft((w ** 2 - 2.0 * mis) * xc / sxc) * D / den).real
+    return float(mis), taper1(a[:n], 0, tl)
+
+
 KINDS = {"waveform": waveform, "corr": corr, "envelope": envelope, "phase": phase}
 
 
-def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6):
+def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6, **params):
     """Per shot component [ntr, nt]: (adjoint source, per-trace misfit) with
-    the discard rule."""
+    the discard rule.  "adaptive" takes dt, tlag_max, method, power, eps."""
     obs = np.asarray(obs, np.float64)
     syn = np.asarray(syn, np.float64)
     no, ns = np.linalg.norm(obs, axis=1), np.linalg.norm(syn, axis=1)
@@ -74,5 +159,8 @@ def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6):
     adj = np.zeros_like(obs)
     mis = np.zeros(len(obs))
     for r in np.nonzero(keep)[0]:
-        mis[r], adj[r] = KINDS[kind](obs[r], syn[r])
+        if kind == "adaptive":
+            mis[r], adj[r] = adaptive(obs[r], syn[r], **params)
+        else:
+            mis[r], adj[r] = KINDS[kind](obs[r], syn[r])
     return adj, mis

@@ -2,8 +2,9 @@
 // batched over the traces of a shot on the GPU
 // (src/inverse/module_inversion_adjoint_source.f90: compute_adjsrc_waveform
 // :371-383, compute_adjsrc_corr :391-413, compute_adjsrc_envelope :424-452,
-// compute_adjsrc_phase :460-481; the per-shot driver and its trace-discard
-// rule compute_adjoint_source :985-1160).
+// compute_adjsrc_phase :460-481, compute_adjsrc_adaptive :490-537 with
+// xcorr_penalty :321-363; the per-shot driver and its trace-discard rule
+// compute_adjoint_source :985-1160).
 //
 // Layout: traces receiver-major [ntr][nt], as the SU traces hold them.  One
 // block per trace for the reductions (FP64, Neumaier-compensated like the
@@ -14,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -234,6 +236,87 @@ __global__ void k_combine(const float* __restrict__ r1, const Cplx* __restrict__
         adj[i] = r1[i] + sign * z2[i].y;
 }
 
+// Adaptive misfit (compute_adjsrc_adaptive).  Per trace: syn and obs
+// tapered (ramp over tl samples at each end) and zero-padded to nnt, as
+// complex rows U, D.
+__global__ void k_awi_load(const float* __restrict__ obs, const float* __restrict__ syn,
+                           const float* __restrict__ ramp, Cplx* U, Cplx* D, int nt, int nnt, int tl) {
+    const size_t tr = blockIdx.y;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nnt; k += gridDim.x * blockDim.x) {
+        float u = 0.0f, d = 0.0f;
+        if (k < nt) {
+            float r = 1.0f;
+            if (k < tl) r *= ramp[k];
+            if (k >= nt - tl) r *= ramp[nt - 1 - k];
+            u = syn[tr * nt + k] * r;
+            d = obs[tr * nt + k] * r;
+        }
+        U[tr * nnt + k] = Cplx{u, 0.0f};
+        D[tr * nnt + k] = Cplx{d, 0.0f};
+    }
+}
+
+// Spectra -> U = U conj(D) / den / nnt (the matching filter's spectrum with
+// the inverse FFT's 1/nnt) and D = D / den / nnt (kept for the adjoint),
+// den = |D|^2 + eps max |D|^2.  One block per trace.
+__global__ __launch_bounds__(kThreads) void k_awi_filter(Cplx* U, Cplx* D, int nnt, float eps) {
+    __shared__ float sh[kThreads];
+    Cplx* u = U + size_t(blockIdx.x) * nnt;
+    Cplx* d = D + size_t(blockIdx.x) * nnt;
+    float m = 0.0f;
+    for (int f = threadIdx.x; f < nnt; f += blockDim.x) m = fmaxf(m, d[f].x * d[f].x + d[f].y * d[f].y);
+    const float e = eps * block_max(m, sh);
+    const float inv = 1.0f / float(nnt);
+    for (int f = threadIdx.x; f < nnt; f += blockDim.x) {
+        const Cplx a = u[f], b = d[f];
+        const float den = (b.x * b.x + b.y * b.y) + e;
+        u[f] = Cplx{(a.x * b.x + a.y * b.y) / den * inv, (a.y * b.x - a.x * b.y) / den * inv};
+        d[f] = Cplx{b.x / den * inv, b.y / den * inv};
+    }
+}
+
+// From the filter xc (real part of U): misfit 0.5 sum (w xc)^2 / sum xc^2
+// (FP64) and U <- (w^2 - 2 misfit) xc / sum xc^2, the misfit's derivative
+// by xc.  One block per trace.
+__global__ __launch_bounds__(kThreads) void k_awi_misfit(Cplx* U, const float* __restrict__ w,
+                                                         const int* __restrict__ keep, int nnt, double* mis) {
+    __shared__ double sh[kThreads];
+    Cplx* u = U + size_t(blockIdx.x) * nnt;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nnt; k += blockDim.x) {
+        const double x = u[k].x, wx = double(w[k]) * x;
+        a += x * x;
+        b += wx * wx;
+    }
+    const double sxc = block_sum(a, sh), swx = block_sum(b, sh);
+    const bool ok = keep[blockIdx.x] && sxc > 0.0;
+    const double f = ok ? 0.5 * swx / sxc : 0.0;
+    for (int k = threadIdx.x; k < nnt; k += blockDim.x) {
+        const double wk = w[k];
+        u[k] = Cplx{ok ? float((wk * wk - 2.0 * f) * double(u[k].x) / sxc) : 0.0f, 0.0f};
+    }
+    if (threadIdx.x == 0) mis[blockIdx.x] = f;
+}
+
+// U <- U . D (D holds D / den / nnt).
+__global__ void k_awi_back(Cplx* U, const Cplx* __restrict__ D, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const Cplx a = U[i], b = D[i];
+        U[i] = Cplx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    }
+}
+
+// adj = the first nt samples of Re(U), tapered at the end.
+__global__ void k_awi_out(const Cplx* __restrict__ U, const float* __restrict__ ramp, const int* __restrict__ keep,
+                          float* adj, int nt, int nnt, int tl) {
+    const size_t tr = blockIdx.y;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nt; k += gridDim.x * blockDim.x) {
+        float a = keep[tr] ? U[tr * nnt + k].x : 0.0f;
+        if (k >= nt - tl) a *= ramp[nt - 1 - k];
+        adj[tr * nt + k] = a;
+    }
+}
+
 void fft_check(hipfftResult r, const char* what) {
     if (r != HIPFFT_SUCCESS) fail(OWL_ERR_DEVICE, std::string(what) + " failed (hipFFT " + std::to_string(int(r)) + ")");
 }
@@ -260,6 +343,58 @@ struct Analytic {
     }
 };
 
+// Host set-up of the adaptive misfit: padded length, taper, lag penalty.
+struct AwiShape {
+    int nnt = 0, tl = 0, maxlag = 0;
+    float eps = 0.0f;
+    std::vector<float> ramp, w;  // rising Hann half-window of tl samples; penalty per padded lag
+};
+
+int next_power_235(int n) {
+    for (int m = std::max(n, 1);; ++m) {
+        int k = m;
+        for (int p : {2, 3, 5})
+            while (k % p == 0) k /= p;
+        if (k == 1) return m;
+    }
+}
+
+AwiShape awi_shape(const owl_misfit_params* p, int nt) {
+    if (!p) fail(OWL_ERR_ARG, "the adaptive misfit needs its parameters");
+    if (!(p->dt > 0) || !(p->tlag_max > 0) || !(p->deconv_eps >= 0))
+        fail(OWL_ERR_ARG, "adaptive misfit: dt and tlag_max must be positive, deconv_eps non-negative");
+    if (p->penalty < OWL_PENALTY_LINEAR || p->penalty > OWL_PENALTY_EXP) fail(OWL_ERR_ARG, "unknown lag penalty");
+    AwiShape a;
+    a.maxlag = int(std::ceil(p->tlag_max / p->dt));  // ceiling(tlag_max/dt) in float32
+    a.nnt = next_power_235(2 * nt);
+    a.tl = int(std::floor(0.025f * float(nt) + 0.5f));
+    a.eps = p->deconv_eps;
+    if (a.maxlag + 1 > a.nnt) fail(OWL_ERR_ARG, "tlag_max exceeds the padded trace length");
+    a.ramp.resize(size_t(a.tl));
+    for (int k = 0; k < a.tl; ++k) a.ramp[size_t(k)] = float(0.5 - 0.5 * std::cos(M_PI * k / a.tl));
+    // xcorr_penalty over lags -maxlag .. maxlag, laid onto the padded lags
+    const int L = 2 * a.maxlag + 1;
+    std::vector<double> pen(static_cast<size_t>(L));
+    const double dt = p->dt, T = p->tlag_max, pw = std::fabs(double(p->penalty_power));
+    const double scaling = 1.0 / std::sqrt(2.0 * std::log(100.0));
+    for (int i = -a.maxlag; i <= a.maxlag; ++i) {
+        const double x = std::fabs(i * dt / T);
+        double v = x;
+        if (p->penalty == OWL_PENALTY_POWER) v = std::pow(x, pw);
+        if (p->penalty == OWL_PENALTY_GAUSSIAN) v = std::exp(-(i * dt) * (i * dt) / (scaling * T * T));
+        if (p->penalty == OWL_PENALTY_EXP) v = std::exp(-std::fabs(i * dt) / (scaling * T));
+        pen[size_t(i + a.maxlag)] = v;
+    }
+    if (p->penalty == OWL_PENALTY_GAUSSIAN || p->penalty == OWL_PENALTY_EXP) {
+        const double lo = *std::min_element(pen.begin(), pen.end()), hi = *std::max_element(pen.begin(), pen.end());
+        for (double& v : pen) v = 1.0 - std::pow(hi > lo ? (v - lo) / (hi - lo) : 0.0, pw);
+    }
+    a.w.assign(size_t(a.nnt), 0.0f);
+    for (int k = 0; k <= a.maxlag; ++k) a.w[size_t(k)] = float(pen[size_t(k + a.maxlag)]);
+    for (int k = 1; k <= a.maxlag; ++k) a.w[size_t(a.nnt - k)] = float(pen[size_t(a.maxlag - k)]);
+    return a;
+}
+
 }  // namespace mf
 }  // namespace owl
 
@@ -270,11 +405,19 @@ extern "C" {
 
 owl_status owl_adjoint_source(int type, int ntr, int nt, const float* obs, const float* syn, const float* weight,
                               float threshold, float* adj, double* misfit) {
+    return owl_adjoint_source_ex(type, nullptr, ntr, nt, obs, syn, weight, threshold, adj, misfit);
+}
+
+owl_status owl_adjoint_source_ex(int type, const owl_misfit_params* prm, int ntr, int nt, const float* obs,
+                                 const float* syn, const float* weight, float threshold, float* adj,
+                                 double* misfit) {
     return guarded([&] {
         if (ntr < 0 || nt < 1 || (ntr > 0 && (!obs || !syn || !adj || !misfit))) fail(OWL_ERR_ARG, "null argument");
         if (type != OWL_MISFIT_WAVEFORM && type != OWL_MISFIT_CORR && type != OWL_MISFIT_ENVELOPE &&
-            type != OWL_MISFIT_PHASE)
+            type != OWL_MISFIT_PHASE && type != OWL_MISFIT_ADAPTIVE)
             fail(OWL_ERR_ARG, "unknown misfit type");
+        AwiShape aw{};
+        if (type == OWL_MISFIT_ADAPTIVE) aw = awi_shape(prm, nt);
         if (ntr == 0) return;
         const size_t n = size_t(ntr) * nt;
         hipStream_t st = nullptr;
@@ -314,6 +457,36 @@ owl_status owl_adjoint_source(int type, int ntr, int nt, const float* obs, const
         } else if (type == OWL_MISFIT_CORR) {
             hipLaunchKernelGGL(k_corr, dim3(ntr), dim3(kThreads), 0, st, d_obs.get(), d_syn.get(), d_keep.get(), nt,
                                d_adj.get(), d_mis.get());
+        } else if (type == OWL_MISFIT_ADAPTIVE) {
+            const size_t m = size_t(ntr) * size_t(aw.nnt);
+            DevBuf<Cplx> U(m), D(m);
+            DevBuf<float> d_ramp(std::max<size_t>(aw.ramp.size(), 1)), d_w(size_t(aw.nnt));
+            if (!aw.ramp.empty()) d_ramp.upload(aw.ramp.data(), aw.ramp.size(), st);
+            d_w.upload(aw.w.data(), aw.w.size(), st);
+            hipfftHandle plan{};
+            fft_check(hipfftPlan1d(&plan, aw.nnt, HIPFFT_C2C, ntr), "hipfftPlan1d");
+            struct PlanGuard {
+                hipfftHandle p;
+                ~PlanGuard() { (void)hipfftDestroy(p); }
+            } pg{plan};
+            fft_check(hipfftSetStream(plan, st), "hipfftSetStream");
+            auto* uc = reinterpret_cast<hipfftComplex*>(U.get());
+            auto* dc = reinterpret_cast<hipfftComplex*>(D.get());
+            const dim3 rows(unsigned(std::min(64, (aw.nnt + 255) / 256)), unsigned(ntr));
+            hipLaunchKernelGGL(k_awi_load, rows, dim3(256), 0, st, d_obs.get(), d_syn.get(), d_ramp.get(), U.get(),
+                               D.get(), nt, aw.nnt, aw.tl);
+            fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_FORWARD), "forward FFT");
+            fft_check(hipfftExecC2C(plan, dc, dc, HIPFFT_FORWARD), "forward FFT");
+            hipLaunchKernelGGL(k_awi_filter, dim3(ntr), dim3(kThreads), 0, st, U.get(), D.get(), aw.nnt, aw.eps);
+            fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_BACKWARD), "inverse FFT");
+            hipLaunchKernelGGL(k_awi_misfit, dim3(ntr), dim3(kThreads), 0, st, U.get(), d_w.get(), d_keep.get(),
+                               aw.nnt, d_mis.get());
+            fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_FORWARD), "forward FFT");
+            const unsigned g = unsigned(std::min<size_t>((m + 255) / 256, 65535));
+            hipLaunchKernelGGL(k_awi_back, dim3(g), dim3(256), 0, st, U.get(), D.get(), m);
+            fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_BACKWARD), "inverse FFT");
+            hipLaunchKernelGGL(k_awi_out, dim3(unsigned(std::min(64, (nt + 255) / 256)), unsigned(ntr)), dim3(256), 0,
+                               st, U.get(), d_ramp.get(), d_keep.get(), d_adj.get(), nt, aw.nnt, aw.tl);
         } else {
             DevBuf<Cplx> zo(n), zs(n), z2(n);
             DevBuf<float> r1(n), r2(n);

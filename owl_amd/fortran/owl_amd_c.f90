@@ -15,7 +15,15 @@ module owl_amd_c
     integer(c_int), parameter :: OWL_CKPT_FULL = 0, OWL_CKPT_UNIFORM = 1, OWL_CKPT_BINOMIAL = 2
     integer(c_int), parameter :: OWL_STORE_F32 = 0, OWL_STORE_F16 = 1, OWL_STORE_BF16 = 2
     integer(c_int), parameter :: OWL_MISFIT_WAVEFORM = 0, OWL_MISFIT_CORR = 1, OWL_MISFIT_ENVELOPE = 2, &
-        OWL_MISFIT_PHASE = 3
+        OWL_MISFIT_PHASE = 3, OWL_MISFIT_ADAPTIVE = 4
+    integer(c_int), parameter :: OWL_PENALTY_LINEAR = 0, OWL_PENALTY_POWER = 1, OWL_PENALTY_GAUSSIAN = 2, &
+        OWL_PENALTY_EXP = 3
+
+    type, bind(c) :: owl_misfit_params_t
+        real(c_float) :: dt, tlag_max
+        integer(c_int) :: penalty
+        real(c_float) :: penalty_power, deconv_eps
+    end type
 
     type, bind(c) :: owl_source_t
         real(c_float) :: x = 0, y = 0, z = 0
@@ -199,6 +207,18 @@ module owl_amd_c
                 bind(c, name='owl_adjoint_source') result(st)
             import :: c_int, c_float, c_double
             integer(c_int), value :: mtype, ntr, nt
+            real(c_float), intent(in) :: obs(*), syn(*), weight(*)
+            real(c_float), value :: threshold
+            real(c_float), intent(out) :: adj(*)
+            real(c_double), intent(out) :: misfit(*)
+            integer(c_int) :: st
+        end function
+
+        function owl_adjoint_source_ex(mtype, prm, ntr, nt, obs, syn, weight, threshold, adj, misfit) &
+                bind(c, name='owl_adjoint_source_ex') result(st)
+            import :: c_int, c_float, c_double, owl_misfit_params_t
+            integer(c_int), value :: mtype, ntr, nt
+            type(owl_misfit_params_t), intent(in) :: prm
             real(c_float), intent(in) :: obs(*), syn(*), weight(*)
             real(c_float), value :: threshold
             real(c_float), intent(out) :: adj(*)

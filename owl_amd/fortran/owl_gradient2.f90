@@ -71,6 +71,8 @@ contains
                 mtype = OWL_MISFIT_ENVELOPE
             case ('phase')
                 mtype = OWL_MISFIT_PHASE
+            case ('adaptive')
+                mtype = OWL_MISFIT_ADAPTIVE
             case default
                 call stop_with('Error: misfit_type = '//tidy(misfit_type)//' is not provided by owl_gradient2')
         end select
@@ -198,6 +200,7 @@ contains
         character(len=:), allocatable :: name
         integer :: ir, nt, nr
         integer(c_int) :: st
+        type(owl_misfit_params_t) :: prm
         name = '/shot_'//itoa(g%id)//'_seismogram_'//d//'.su'
         call read_su_gather(tidy(dir_synthetic)//name, syn)
         call read_su_gather(tidy(dir_record)//name, obs)
@@ -212,7 +215,23 @@ contains
             s((ir - 1)*nt + 1:ir*nt) = syn%trace(ir)%data(1:nt)
             w(ir) = g%recr(ir)%weight
         end do
-        st = owl_adjoint_source(int(mtype, c_int), int(nr, c_int), int(nt, c_int), o, s, w, &
+        ! the adaptive misfit's lags are in the traces' own sampling
+        ! (compute_adjoint_source, module_inversion_adjoint_source.f90:1131)
+        prm%dt = real(syn%dt, c_float)
+        prm%tlag_max = real(tlag_max, c_float)
+        select case (tidy(penalty_method))
+            case ('power')
+                prm%penalty = OWL_PENALTY_POWER
+            case ('gaussian')
+                prm%penalty = OWL_PENALTY_GAUSSIAN
+            case ('exp')
+                prm%penalty = OWL_PENALTY_EXP
+            case default
+                prm%penalty = OWL_PENALTY_LINEAR
+        end select
+        prm%penalty_power = real(penalty_power, c_float)
+        prm%deconv_eps = real(deconv_eps, c_float)
+        st = owl_adjoint_source_ex(int(mtype, c_int), prm, int(nr, c_int), int(nt, c_int), o, s, w, &
             real(trace_discard_threshold, c_float), a, m)
         if (st /= OWL_OK) call stop_with('<owl_adjoint_source> '//owl_error_message())
         misfit = misfit + sum(m)

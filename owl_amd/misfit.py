@@ -1,6 +1,7 @@
 """Misfits and adjoint sources of OWL's FWI on the GPU over the C ABI
-(owl_adjoint_source, include/owl_amd.h): the L2 waveform, zero-lag
-correlation, envelope and instantaneous-phase misfits of
+(owl_adjoint_source, owl_adjoint_source_ex, include/owl_amd.h): the L2
+waveform, zero-lag correlation, envelope, instantaneous-phase and adaptive
+(AWI) misfits of
 src/inverse/module_inversion_adjoint_source.f90, batched over the traces of
 a shot, with OWL's trace-discard rule."""
 import ctypes as C
@@ -15,10 +16,13 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6):
+def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6, dt=None, tlag_max=None, penalty="linear",
+                   penalty_power=1.0, deconv_eps=0.1):
     """(adjoint source [ntr, nt] float32, per-trace misfit [ntr] float64) of
-    one shot component: kind is "waveform", "corr", "envelope" or "phase";
-    obs, syn [ntr, nt]; weight [ntr] receiver weights (None: all 1)."""
+    one shot component: kind is "waveform", "corr", "envelope", "phase" or
+    "adaptive"; obs, syn [ntr, nt]; weight [ntr] receiver weights (None: all
+    1).  "adaptive" needs dt; tlag_max defaults to OWL's 0.1 tmax =
+    0.1 (nt - 1) dt; penalty is "linear", "power", "gaussian" or "exp"."""
     o = np.ascontiguousarray(obs, np.float32)
     s = np.ascontiguousarray(syn, np.float32)
     if o.shape != s.shape or o.ndim != 2:
@@ -29,6 +33,13 @@ def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6):
         raise ValueError("weight must hold one value per trace")
     adj = np.zeros_like(o)
     mis = np.zeros(ntr, np.float64)
-    check(lib().owl_adjoint_source(_lib.MISFITS[kind], ntr, nt, _ptr(o), _ptr(s), None if w is None else _ptr(w),
-                                   float(threshold), _ptr(adj), _ptr(mis)))
+    prm = None
+    if kind == "adaptive":
+        if dt is None:
+            raise ValueError("the adaptive misfit needs dt")
+        prm = _lib.owl_misfit_params(float(dt), float(0.1 * (nt - 1) * dt if tlag_max is None else tlag_max),
+                                     _lib.PENALTIES[penalty], float(penalty_power), float(deconv_eps))
+    check(lib().owl_adjoint_source_ex(_lib.MISFITS[kind], None if prm is None else C.byref(prm), ntr, nt, _ptr(o),
+                                      _ptr(s), None if w is None else _ptr(w), float(threshold), _ptr(adj),
+                                      _ptr(mis)))
     return adj, mis

@@ -123,11 +123,18 @@ def test_owl_modeling2_elastic_tti_free_surface(gpu, tmp_path):
     assert np.array_equal(sx, tx) and np.array_equal(sz, tz)
 
 
-def test_owl_gradient2_acoustic(gpu, tmp_path):
+MISFIT_CASES = {"waveform": ({"misfit_type": "waveform"}, {}),
+                "adaptive": ({"misfit_type": "adaptive", "tlag_max": 0.04, "penalty_method": "power",
+                              "penalty_power": 2.0, "deconv_eps": 0.05},
+                             dict(dt=DT, tlag_max=0.04, penalty="power", penalty_power=2.0, deconv_eps=0.05))}
+
+
+@pytest.mark.parametrize("kind", list(MISFIT_CASES))
+def test_owl_gradient2_acoustic(gpu, tmp_path, kind):
     vp, rho = _model(5)
     vt = (vp * (1.0 + 0.03 * random_field((NZ, NX), 6.0, 9))).astype(np.float32)
-    p, shots, geoms, wins = _acoustic_project(str(tmp_path), vp, rho,
-                                              {"model_update": "vp, rho", "misfit_type": "waveform"})
+    fpar, kw = MISFIT_CASES[kind]
+    p, shots, geoms, wins = _acoustic_project(str(tmp_path), vp, rho, dict({"model_update": "vp, rho"}, **fpar))
     bt = Acoustic2DBatch(NX, NZ, DX, DX, vt, rho, list(zip(geoms, wins)), DT, TMAX, pml=PML)
     obs = bt.forward()
     bt.close()
@@ -138,7 +145,7 @@ def test_owl_gradient2_acoustic(gpu, tmp_path):
     syn = b.forward(save_boundary=True)
     adjs, misfit = [], 0.0
     for g, o, s in zip(geoms, obs, syn):
-        a, m = adjoint_source("waveform", o, s, np.array([r.weight for r in g.recr], np.float32))
+        a, m = adjoint_source(kind, o, s, np.array([r.weight for r in g.recr], np.float32), **kw)
         adjs.append(a)
         misfit += m.sum()
     grads = b.adjoint(adjs)
@@ -148,7 +155,7 @@ def test_owl_gradient2_acoustic(gpu, tmp_path):
         sl = (slice(w.nzbeg - 1, w.nzend), slice(w.nxbeg - 1, w.nxend))
         full[0][sl] += gv
         full[1][sl] += gr
-    _run(GRADIENT, p, "owl_gradient2 acoustic-iso, 3 shots, waveform misfit")
+    _run(GRADIENT, p, f"owl_gradient2 acoustic-iso, 3 shots, {kind} misfit")
     for k, name in enumerate(("vp", "rho")):
         got = np.fromfile(str(tmp_path / "work" / "iteration_1" / "model" / f"grad_{name}.bin"),
                           np.float32).reshape(NX, NZ).T

@@ -1,7 +1,7 @@
 """GPU misfits and adjoint sources (owl_adjoint_source through the C ABI)
 against the float64 oracle restatement of OWL's formulas: a shot of 48
 traces with two zero-weight and one near-silent receiver (the discard rule),
-for the waveform, correlation, envelope and phase misfits.  Tolerances
+for the waveform, correlation, envelope, phase and adaptive misfits.  Tolerances
 (rel-L2 of the adjoint source, relative misfit): waveform 1e-6, correlation
 1e-5, envelope 1e-4 (single-precision FFTs), phase 2e-3 (single-precision
 phases next to the +-pi cut)."""
@@ -46,3 +46,30 @@ def test_adjoint_sources_match_the_oracle(gpu, kind):
     assert np.abs(radj).max() > 0
     assert rel_l2(adj, radj) <= TOL[kind], rel_l2(adj, radj)
     assert abs(mis.sum() - rmis.sum()) <= TOL[kind] * rmis.sum()
+
+
+@pytest.mark.parametrize("penalty, power", [("linear", 1.0), ("power", 2.0), ("gaussian", 1.0), ("exp", 2.0)])
+def test_adaptive_matches_the_oracle(gpu, penalty, power):
+    """The adaptive (AWI) misfit: batched single-precision FFTs of the
+    2-3-5 padded length (700 -> 1440) against the float64 oracle, within
+    1e-4 (adjoint rel-L2 and summed misfit), with the discard rule."""
+    obs, syn, w = _shot()
+    dt = 1e-3
+    tlag = 0.1 * (obs.shape[1] - 1) * dt
+    print(f"case: adaptive misfit, penalty {penalty}^{power}", flush=True)
+    adj, mis = adjoint_source("adaptive", obs, syn, w, 1e-6, dt=dt, penalty=penalty, penalty_power=power)
+    radj, rmis = mo.adjoint_source("adaptive", obs.astype(np.float64), syn.astype(np.float64), w, 1e-6, dt=dt,
+                                   tlag_max=float(np.float32(tlag)), method=penalty, power=power, eps=0.1)
+    for r in (3, 7, 30):
+        assert not adj[r].any() and mis[r] == 0
+    assert np.abs(radj).max() > 0 and rmis.sum() > 0
+    assert rel_l2(adj, radj) <= 1e-4, rel_l2(adj, radj)
+    assert abs(mis.sum() - rmis.sum()) <= 1e-4 * rmis.sum()
+
+
+def test_adaptive_rejects_bad_parameters(gpu):
+    obs, syn, w = _shot(nt=100)
+    with pytest.raises(Exception):
+        adjoint_source("adaptive", obs, syn, w, dt=1e-3, tlag_max=10.0)  # longer than the padded trace
+    with pytest.raises(Exception):
+        adjoint_source("adaptive", obs, syn, w, dt=-1e-3)

@@ -71,3 +71,46 @@ def test_discard_rule():
     adj, mis = mo.adjoint_source("waveform", obs, syn, weight=[1, 0, 1, 1], threshold=1e-6)
     assert not adj[1].any() and not adj[2].any() and mis[1] == 0 and mis[2] == 0
     assert adj[0].any() and adj[3].any()
+
+
+def _broadband(n, seed, lo, hi):
+    x = np.zeros(n)
+    x[lo:hi] = np.random.default_rng(seed).standard_normal(hi - lo)
+    return x
+
+
+@pytest.mark.parametrize("method, power", [("linear", 1.0), ("power", 2.0)])
+def test_adaptive_misfit_of_a_shifted_trace_is_the_penalty_at_the_shift(method, power):
+    """A broadband trace and its copy shifted by k samples (away from the
+    tapers): the matching filter is a spike at lag k (wrapped for k < 0), so
+    the misfit is penalty(k)^2 / 2 = (|k| dt / tlag_max)^(2 power) / 2."""
+    n, dt = 400, 0.004
+    T = 0.1 * n * dt
+    x = _broadband(n, 3, 60, 340)
+    for k in (0, 3, -5, 9, -17):
+        f, _ = mo.adaptive(x, np.roll(x, k), dt, T, method=method, power=power, eps=1e-10)
+        assert f == pytest.approx(0.5 * (abs(k) * dt / T) ** (2 * power), abs=1e-12, rel=1e-9), k
+
+
+def test_adaptive_adjoint_is_the_misfit_gradient():
+    """OWL's AWI misfit carries the 1/2, so its adjoint is the full gradient
+    with respect to syn: exactly outside the start taper (OWL tapers only the
+    end of the adjoint), and the gradient over the taper ramp inside it."""
+    n, dt = 300, 0.004
+    T = 0.1 * n * dt
+    obs = _broadband(n, 4, 40, 260)
+    syn = np.roll(obs, 4) + 0.3 * np.roll(obs, -2) + 0.05 * _broadband(n, 5, 0, n)
+    for method in ("linear", "gaussian", "exp"):
+        f = lambda o, s: mo.adaptive(o, s, dt, T, method=method)  # noqa: E731
+        adj = f(obs, syn)[1]
+        g = 2.0 * _fd_half_grad(f, obs, syn)
+        tl = int(np.floor(0.025 * n + 0.5))
+        assert np.linalg.norm(adj[tl:] - g[tl:]) < 1e-6 * np.linalg.norm(g[tl:]), method
+        np.testing.assert_allclose(adj[:tl] * mo.hann_ramp(tl), g[:tl], atol=1e-8 * np.abs(g).max())
+
+
+def test_next_power_235():
+    assert [mo.next_power_235(n) for n in (1, 7, 11, 13, 97, 401, 800, 4001)] == [1, 8, 12, 15, 100, 405, 800, 4050]
+    for n in range(1, 400):
+        m = mo.next_power_235(n)
+        assert all(mo.next_power_235(j) == m for j in range(n, m + 1))
