@@ -552,6 +552,7 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
 #define OWL_MISFIT_ENVELOPE 2  /* envelope difference (:424-452) */
 #define OWL_MISFIT_PHASE 3     /* instantaneous-phase difference (:460-481) */
 #define OWL_MISFIT_ADAPTIVE 4  /* adaptive waveform inversion, Warner & Guasch (:490-537) */
+#define OWL_MISFIT_LOCAL_ADAPTIVE 5 /* localised AWI, Yong et al. (:545-625) */
 
 /* Lag penalties of the adaptive misfit (xcorr_penalty, :321-363). */
 #define OWL_PENALTY_LINEAR 0
@@ -567,6 +568,7 @@ typedef struct owl_misfit_params {
     int penalty;         /* OWL_PENALTY_*; OWL's default linear */
     float penalty_power; /* default 1 */
     float deconv_eps;    /* Wiener-filter stabilisation relative to max |D|^2, default 0.1 */
+    float lawi_sigma;    /* local adaptive: Gabor window standard deviation (s), default 0.25 */
 } owl_misfit_params;
 
 /* Misfit and adjoint source of the ntr traces of one shot component, on the
@@ -580,14 +582,17 @@ typedef struct owl_misfit_params {
  * instant_phase are not in the reference tree). */
 owl_status owl_adjoint_source(int type, int ntr, int nt, const float *obs, const float *syn,
                               const float *weight, float threshold, float *adj, double *misfit);
-/* As owl_adjoint_source, with the parameters that OWL_MISFIT_ADAPTIVE needs
- * (p may be NULL for the other types).  The adaptive misfit per trace is
+/* As owl_adjoint_source, with the parameters that OWL_MISFIT_ADAPTIVE and
+ * OWL_MISFIT_LOCAL_ADAPTIVE need (p may be NULL for the other types).  The adaptive misfit per trace is
  * 0.5 sum (w xc)^2 / sum xc^2 of the Wiener matching filter xc of syn onto
  * obs (both tapered over nint(0.025 nt) samples at each end and padded to the
  * 2-3-5 length >= 2 nt), w the lag penalty; its adjoint source is the
  * gradient of that misfit (tapered at the end).  FLIT's taper, fft and
  * rescale are taken as a Hann half-window, the unnormalised forward and
- * 1/n inverse DFT, and the linear [min, max] -> [0, 1] map. */
+ * 1/n inverse DFT, and the linear [min, max] -> [0, 1] map.  The local
+ * adaptive misfit sums v_i^2 over one matching filter per sample i under the
+ * Gabor window exp(-(k - i)^2 / (2 (lawi_sigma/dt)^2)) (FLIT's gaussian,
+ * taken unnormalised), v_i = sum (w xc_i)^2 / (sum xc_i^2 + eta). */
 owl_status owl_adjoint_source_ex(int type, const owl_misfit_params *p, int ntr, int nt, const float *obs,
                                  const float *syn, const float *weight, float threshold, float *adj,
                                  double *misfit);

@@ -145,6 +145,56 @@ def adaptive(obs, syn, dt, tlag_max, method="linear", power=1.0, eps=0.1):
     return float(mis), taper1(a[:n], 0, tl)
 
 
+def local_adaptive(obs, syn, dt, tlag_max, method="linear", power=1.0, eps=0.1, sigma=0.25, eta=None):
+    """compute_adjsrc_local_adaptive (module_inversion_adjoint_source.f90:
+    545-625): one matching filter per sample i under the Gabor window
+    g_i(k) = exp(-(k - i)^2 / (2 (sigma/dt)^2)) (FLIT's gaussian, taken
+    unnormalised), misfit sum_i v_i^2 with v_i = sum (w xc_i)^2 /
+    (sum xc_i^2 + eta); adjoint sum_i g_i IFFT(FFT(v_i (w^2 - v_i) xc_i /
+    (sum xc_i^2 + eta)) D_i / den_i) / sum_i g_i^2.  eps and eta come from
+    the whole traces; float_small replaces a zero.  eta may be given (the
+    pins hold it fixed: OWL's adjoint does not differentiate it)."""
+    obs = np.asarray(obs, np.float64)
+    syn = np.asarray(syn, np.float64)
+    n = len(obs)
+    maxlag = int(np.ceil(tlag_max / dt))
+    nnt = next_power_235(2 * n)
+    tl = int(np.floor(0.025 * n + 0.5))
+    u = np.zeros(nnt)
+    d = np.zeros(nnt)
+    u[:n] = taper1(syn, tl, tl)
+    d[:n] = taper1(obs, tl, tl)
+    U0, D0 = np.fft.fft(u), np.fft.fft(d)
+    small = float(np.finfo(np.float32).eps)
+    e = eps * np.max(np.abs(D0) ** 2)
+    e = e if e != 0 else small
+    if eta is None:
+        xc0 = np.fft.ifft(U0 * np.conj(D0) / (np.abs(D0) ** 2 + e)).real
+        eta = eps * np.max(xc0 ** 2)
+        eta = eta if eta != 0 else small
+    p = xcorr_penalty(dt, maxlag, tlag_max, method, power)
+    w = np.zeros(nnt)
+    w[:maxlag + 1] = p[maxlag:]
+    if maxlag:
+        w[nnt - maxlag:] = p[:maxlag]
+    a = np.arange(1, nnt + 1, dtype=np.float64)
+    sg = sigma / dt
+    mis = 0.0
+    adj = np.zeros(nnt)
+    s = np.zeros(nnt)
+    for i in range(1, n + 1):
+        g = np.exp(-(a - i) ** 2 / (2.0 * sg * sg))
+        uu, dd = np.fft.fft(u * g), np.fft.fft(d * g)
+        den = np.abs(dd) ** 2 + e
+        xc = np.fft.ifft(uu * np.conj(dd) / den).real
+        sx = np.sum(xc ** 2) + eta
+        vt = np.sum((w * xc) ** 2) / sx
+        mis += vt ** 2
+        adj += np.fft.ifft(np.fft.fft(vt * (w ** 2 - vt) * xc / sx) * dd / den).real * g
+        s += g ** 2
+    return float(mis), taper1(adj[:n] / s[:n], 0, tl)
+
+
 KINDS = {"waveform": waveform, "corr": corr, "envelope": envelope, "phase": phase}
 
 
@@ -161,6 +211,8 @@ def adjoint_source(kind, obs, syn, weight=None, threshold=1.0e-6, **params):
     for r in np.nonzero(keep)[0]:
         if kind == "adaptive":
             mis[r], adj[r] = adaptive(obs[r], syn[r], **params)
+        elif kind == "local-adaptive":
+            mis[r], adj[r] = local_adaptive(obs[r], syn[r], **params)
         else:
             mis[r], adj[r] = KINDS[kind](obs[r], syn[r])
     return adj, mis

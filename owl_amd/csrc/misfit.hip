@@ -298,6 +298,109 @@ __global__ __launch_bounds__(kThreads) void k_awi_misfit(Cplx* U, const float* _
     if (threadIdx.x == 0) mis[blockIdx.x] = f;
 }
 
+// Local adaptive misfit (compute_adjsrc_local_adaptive).  eps and eta of
+// each trace from the whole-trace filter: eps = deconv_eps max |D0|^2 is
+// taken by k_lawi_eps from the spectra, eta = deconv_eps max xc0^2 by
+// k_lawi_eta from the filter (FLT_EPSILON for a zero).
+__global__ __launch_bounds__(kThreads) void k_lawi_eps(const Cplx* __restrict__ D, int nnt, float rel, float* e) {
+    __shared__ float sh[kThreads];
+    const Cplx* d = D + size_t(blockIdx.x) * nnt;
+    float m = 0.0f;
+    for (int f = threadIdx.x; f < nnt; f += blockDim.x) m = fmaxf(m, d[f].x * d[f].x + d[f].y * d[f].y);
+    const float v = rel * block_max(m, sh);
+    if (threadIdx.x == 0) e[blockIdx.x] = v != 0.0f ? v : 1.1920929e-7f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lawi_eta(const Cplx* __restrict__ X, int nnt, float rel, float* eta) {
+    __shared__ float sh[kThreads];
+    const Cplx* x = X + size_t(blockIdx.x) * nnt;
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < nnt; k += blockDim.x) m = fmaxf(m, x[k].x * x[k].x);
+    const float v = rel * block_max(m, sh);
+    if (threadIdx.x == 0) eta[blockIdx.x] = v != 0.0f ? v : 1.1920929e-7f;
+}
+
+// Windowed copies of the tapered traces (row j = trace * nc + window c,
+// window centre i = i0 + c + 1 on the 1-based sample axis); windows past the
+// last sample are zero rows.
+__global__ void k_lawi_load(const Cplx* __restrict__ U0, const Cplx* __restrict__ D0, Cplx* U, Cplx* D, int nt,
+                            int nnt, int nc, int i0, float inv2s2) {
+    const int row = blockIdx.y, tr = row / nc, c = row % nc, i = i0 + c + 1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nnt; k += gridDim.x * blockDim.x) {
+        const float dk = float(k + 1 - i);
+        const float g = i <= nt ? expf(-dk * dk * inv2s2) : 0.0f;
+        U[size_t(row) * nnt + k] = Cplx{U0[size_t(tr) * nnt + k].x * g, 0.0f};
+        D[size_t(row) * nnt + k] = Cplx{D0[size_t(tr) * nnt + k].x * g, 0.0f};
+    }
+}
+
+// As k_awi_filter with the trace's eps.
+__global__ __launch_bounds__(kThreads) void k_lawi_filter(Cplx* U, Cplx* D, int nnt, int nc,
+                                                          const float* __restrict__ eps) {
+    Cplx* u = U + size_t(blockIdx.x) * nnt;
+    Cplx* d = D + size_t(blockIdx.x) * nnt;
+    const float e = eps[blockIdx.x / nc], inv = 1.0f / float(nnt);
+    for (int f = threadIdx.x; f < nnt; f += blockDim.x) {
+        const Cplx a = u[f], b = d[f];
+        const float den = (b.x * b.x + b.y * b.y) + e;
+        u[f] = Cplx{(a.x * b.x + a.y * b.y) / den * inv, (a.y * b.x - a.x * b.y) / den * inv};
+        d[f] = Cplx{b.x / den * inv, b.y / den * inv};
+    }
+}
+
+// One window: v = sum (w xc)^2 / (sum xc^2 + eta), v^2 into vt2[row], and
+// U <- v (w^2 - v) xc / (sum xc^2 + eta).
+__global__ __launch_bounds__(kThreads) void k_lawi_misfit(Cplx* U, const float* __restrict__ w, int nnt, int nc,
+                                                          const float* __restrict__ eta, double* vt2) {
+    __shared__ double sh[kThreads];
+    Cplx* u = U + size_t(blockIdx.x) * nnt;
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nnt; k += blockDim.x) {
+        const double x = u[k].x, wx = double(w[k]) * x;
+        a += x * x;
+        b += wx * wx;
+    }
+    const double sx = block_sum(a, sh) + double(eta[blockIdx.x / nc]), v = block_sum(b, sh) / sx;
+    for (int k = threadIdx.x; k < nnt; k += blockDim.x) {
+        const double wk = w[k];
+        u[k] = Cplx{float(v * (wk * wk - v) * double(u[k].x) / sx), 0.0f};
+    }
+    if (threadIdx.x == 0) vt2[blockIdx.x] = v * v;
+}
+
+// acc[tr][k] += sum over the chunk's windows of a_c(k) g_c(k), in window
+// order; mis[tr] += the windows' v^2 (thread 0 of the first block).
+__global__ void k_lawi_accum(const Cplx* __restrict__ A, const double* __restrict__ vt2, double* acc, double* mis,
+                             int nt, int nnt, int nc, int i0, float inv2s2) {
+    const int tr = blockIdx.y;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nt; k += gridDim.x * blockDim.x) {
+        double s = acc[size_t(tr) * nt + k];
+        for (int c = 0; c < nc && i0 + c < nt; ++c) {
+            const float dk = float(k + 1 - (i0 + c + 1));
+            s += double(A[(size_t(tr) * nc + c) * nnt + k].x) * double(expf(-dk * dk * inv2s2));
+        }
+        acc[size_t(tr) * nt + k] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double m = mis[tr];
+        for (int c = 0; c < nc && i0 + c < nt; ++c) m += vt2[size_t(tr) * nc + c];
+        mis[tr] = m;
+    }
+}
+
+// adj = acc / sum_i g_i^2, tapered at the end; the misfit of kept traces.
+__global__ void k_lawi_out(const double* __restrict__ acc, const double* __restrict__ gsum,
+                           const double* __restrict__ mis_acc, const float* __restrict__ ramp,
+                           const int* __restrict__ keep, float* adj, double* mis, int nt, int tl) {
+    const int tr = blockIdx.y;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nt; k += gridDim.x * blockDim.x) {
+        float a = keep[tr] ? float(acc[size_t(tr) * nt + k] / gsum[k]) : 0.0f;
+        if (k >= nt - tl) a *= ramp[nt - 1 - k];
+        adj[size_t(tr) * nt + k] = a;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) mis[tr] = keep[tr] ? mis_acc[tr] : 0.0;
+}
+
 // U <- U . D (D holds D / den / nnt).
 __global__ void k_awi_back(Cplx* U, const Cplx* __restrict__ D, size_t n) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
@@ -395,6 +498,93 @@ AwiShape awi_shape(const owl_misfit_params* p, int nt) {
     return a;
 }
 
+// A batched C2C plan of n-point transforms, destroyed with the object.
+struct FftPlan {
+    hipfftHandle p{};
+    FftPlan(int n, int batch, hipStream_t st) {
+        fft_check(hipfftPlan1d(&p, n, HIPFFT_C2C, batch), "hipfftPlan1d");
+        fft_check(hipfftSetStream(p, st), "hipfftSetStream");
+    }
+    ~FftPlan() { (void)hipfftDestroy(p); }
+    FftPlan(const FftPlan&) = delete;
+    FftPlan& operator=(const FftPlan&) = delete;
+    void run(Cplx* z, int dir) const {
+        auto* zc = reinterpret_cast<hipfftComplex*>(z);
+        fft_check(hipfftExecC2C(p, zc, zc, dir), dir == HIPFFT_FORWARD ? "forward FFT" : "inverse FFT");
+    }
+};
+
+// compute_adjsrc_local_adaptive for every trace: the whole-trace filter for
+// eps and eta, then the per-sample windows in chunks of nc windows of every
+// trace (about 1 GiB of spectra per chunk).
+void local_adaptive(const AwiShape& aw, const owl_misfit_params& p, int ntr, int nt, const float* obs,
+                    const float* syn, const int* keep, float* adj, double* mis, hipStream_t st) {
+    const int nnt = aw.nnt;
+    const size_t whole = size_t(ntr) * size_t(nnt);
+    DevBuf<Cplx> U0(whole), D0(whole);
+    DevBuf<float> d_ramp(std::max<size_t>(aw.ramp.size(), 1)), d_w(static_cast<size_t>(nnt)),
+        d_eps(static_cast<size_t>(ntr)), d_eta(static_cast<size_t>(ntr));
+    if (!aw.ramp.empty()) d_ramp.upload(aw.ramp.data(), aw.ramp.size(), st);
+    d_w.upload(aw.w.data(), aw.w.size(), st);
+    const float sg = p.lawi_sigma / p.dt, inv2s2 = 1.0f / (2.0f * sg * sg);
+    {
+        // the tapered traces (kept real in U0, D0) and the whole-trace filter
+        DevBuf<Cplx> Uf(whole), Df(whole);
+        const dim3 rows(unsigned(std::min(64, (nnt + 255) / 256)), unsigned(ntr));
+        hipLaunchKernelGGL(k_awi_load, rows, dim3(256), 0, st, obs, syn, d_ramp.get(), U0.get(), D0.get(), nt, nnt,
+                           aw.tl);
+        OWL_HIP(hipMemcpyAsync(Uf.get(), U0.get(), whole * sizeof(Cplx), hipMemcpyDeviceToDevice, st));
+        OWL_HIP(hipMemcpyAsync(Df.get(), D0.get(), whole * sizeof(Cplx), hipMemcpyDeviceToDevice, st));
+        FftPlan plan(nnt, ntr, st);
+        plan.run(Uf.get(), HIPFFT_FORWARD);
+        plan.run(Df.get(), HIPFFT_FORWARD);
+        hipLaunchKernelGGL(k_lawi_eps, dim3(ntr), dim3(kThreads), 0, st, Df.get(), nnt, aw.eps, d_eps.get());
+        hipLaunchKernelGGL(k_lawi_filter, dim3(ntr), dim3(kThreads), 0, st, Uf.get(), Df.get(), nnt, 1,
+                           d_eps.get());
+        plan.run(Uf.get(), HIPFFT_BACKWARD);
+        hipLaunchKernelGGL(k_lawi_eta, dim3(ntr), dim3(kThreads), 0, st, Uf.get(), nnt, aw.eps, d_eta.get());
+    }
+    // sum_i g_i(k)^2 over the windows i = 1 .. nt (host, double)
+    std::vector<double> gsum(size_t(nt), 0.0);
+    for (int k = 0; k < nt; ++k)
+        for (int i = 1; i <= nt; ++i) {
+            const float dk = float(k + 1 - i);
+            const double g = std::exp(double(-dk * dk * inv2s2));
+            gsum[size_t(k)] += g * g;
+        }
+    DevBuf<double> d_gsum(static_cast<size_t>(nt)), acc(static_cast<size_t>(ntr) * nt),
+        macc(static_cast<size_t>(ntr));
+    d_gsum.upload(gsum.data(), gsum.size(), st);
+    acc.zero(st);
+    macc.zero(st);
+    const size_t budget = size_t(1) << 30;
+    const int nc = int(std::max<size_t>(1, std::min<size_t>(size_t(nt), budget / (whole * 2 * sizeof(Cplx)))));
+    const size_t m = whole * size_t(nc);
+    DevBuf<Cplx> U(m), D(m);
+    DevBuf<double> vt2(size_t(ntr) * nc);
+    FftPlan plan(nnt, ntr * nc, st);
+    const unsigned gx = unsigned(std::min(64, (nnt + 255) / 256)), gb = unsigned(std::min<size_t>((m + 255) / 256, 65535));
+    const dim3 trows(unsigned(std::min(64, (nt + 255) / 256)), unsigned(ntr));
+    for (int i0 = 0; i0 < nt; i0 += nc) {
+        hipLaunchKernelGGL(k_lawi_load, dim3(gx, unsigned(ntr * nc)), dim3(256), 0, st, U0.get(), D0.get(), U.get(),
+                           D.get(), nt, nnt, nc, i0, inv2s2);
+        plan.run(U.get(), HIPFFT_FORWARD);
+        plan.run(D.get(), HIPFFT_FORWARD);
+        hipLaunchKernelGGL(k_lawi_filter, dim3(unsigned(ntr * nc)), dim3(kThreads), 0, st, U.get(), D.get(), nnt, nc,
+                           d_eps.get());
+        plan.run(U.get(), HIPFFT_BACKWARD);
+        hipLaunchKernelGGL(k_lawi_misfit, dim3(unsigned(ntr * nc)), dim3(kThreads), 0, st, U.get(), d_w.get(), nnt,
+                           nc, d_eta.get(), vt2.get());
+        plan.run(U.get(), HIPFFT_FORWARD);
+        hipLaunchKernelGGL(k_awi_back, dim3(gb), dim3(256), 0, st, U.get(), D.get(), m);
+        plan.run(U.get(), HIPFFT_BACKWARD);
+        hipLaunchKernelGGL(k_lawi_accum, trows, dim3(256), 0, st, U.get(), vt2.get(), acc.get(), macc.get(), nt, nnt,
+                           nc, i0, inv2s2);
+    }
+    hipLaunchKernelGGL(k_lawi_out, trows, dim3(256), 0, st, acc.get(), d_gsum.get(), macc.get(), d_ramp.get(), keep,
+                       adj, mis, nt, aw.tl);
+}
+
 }  // namespace mf
 }  // namespace owl
 
@@ -414,10 +604,12 @@ owl_status owl_adjoint_source_ex(int type, const owl_misfit_params* prm, int ntr
     return guarded([&] {
         if (ntr < 0 || nt < 1 || (ntr > 0 && (!obs || !syn || !adj || !misfit))) fail(OWL_ERR_ARG, "null argument");
         if (type != OWL_MISFIT_WAVEFORM && type != OWL_MISFIT_CORR && type != OWL_MISFIT_ENVELOPE &&
-            type != OWL_MISFIT_PHASE && type != OWL_MISFIT_ADAPTIVE)
+            type != OWL_MISFIT_PHASE && type != OWL_MISFIT_ADAPTIVE && type != OWL_MISFIT_LOCAL_ADAPTIVE)
             fail(OWL_ERR_ARG, "unknown misfit type");
         AwiShape aw{};
-        if (type == OWL_MISFIT_ADAPTIVE) aw = awi_shape(prm, nt);
+        if (type == OWL_MISFIT_ADAPTIVE || type == OWL_MISFIT_LOCAL_ADAPTIVE) aw = awi_shape(prm, nt);
+        if (type == OWL_MISFIT_LOCAL_ADAPTIVE && !(prm->lawi_sigma > 0))
+            fail(OWL_ERR_ARG, "local adaptive misfit: lawi_sigma must be positive");
         if (ntr == 0) return;
         const size_t n = size_t(ntr) * nt;
         hipStream_t st = nullptr;
@@ -487,6 +679,8 @@ owl_status owl_adjoint_source_ex(int type, const owl_misfit_params* prm, int ntr
             fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_BACKWARD), "inverse FFT");
             hipLaunchKernelGGL(k_awi_out, dim3(unsigned(std::min(64, (nt + 255) / 256)), unsigned(ntr)), dim3(256), 0,
                                st, U.get(), d_ramp.get(), d_keep.get(), d_adj.get(), nt, aw.nnt, aw.tl);
+        } else if (type == OWL_MISFIT_LOCAL_ADAPTIVE) {
+            local_adaptive(aw, *prm, ntr, nt, d_obs.get(), d_syn.get(), d_keep.get(), d_adj.get(), d_mis.get(), st);
         } else {
             DevBuf<Cplx> zo(n), zs(n), z2(n);
             DevBuf<float> r1(n), r2(n);

@@ -126,7 +126,9 @@ def test_owl_modeling2_elastic_tti_free_surface(gpu, tmp_path):
 MISFIT_CASES = {"waveform": ({"misfit_type": "waveform"}, {}),
                 "adaptive": ({"misfit_type": "adaptive", "tlag_max": 0.04, "penalty_method": "power",
                               "penalty_power": 2.0, "deconv_eps": 0.05},
-                             dict(dt=DT, tlag_max=0.04, penalty="power", penalty_power=2.0, deconv_eps=0.05))}
+                             dict(dt=DT, tlag_max=0.04, penalty="power", penalty_power=2.0, deconv_eps=0.05)),
+                "local-adaptive": ({"misfit_type": "local-adaptive", "tlag_max": 0.04, "lawi_sigma": 0.03},
+                                   dict(dt=DT, tlag_max=0.04, lawi_sigma=0.03))}
 
 
 @pytest.mark.parametrize("kind", list(MISFIT_CASES))

@@ -73,3 +73,22 @@ def test_adaptive_rejects_bad_parameters(gpu):
         adjoint_source("adaptive", obs, syn, w, dt=1e-3, tlag_max=10.0)  # longer than the padded trace
     with pytest.raises(Exception):
         adjoint_source("adaptive", obs, syn, w, dt=-1e-3)
+
+
+@pytest.mark.parametrize("penalty", ["linear", "gaussian"])
+def test_local_adaptive_matches_the_oracle(gpu, penalty):
+    """The local AWI misfit: one matching filter per sample (48 traces x 700
+    windows of the 1440-point padded length, chunked), against the float64
+    oracle within 2e-4 (adjoint rel-L2, summed misfit)."""
+    obs, syn, w = _shot()
+    dt = 1e-3
+    tlag = 0.1 * (obs.shape[1] - 1) * dt
+    print(f"case: local adaptive misfit, penalty {penalty}", flush=True)
+    adj, mis = adjoint_source("local-adaptive", obs, syn, w, 1e-6, dt=dt, penalty=penalty, lawi_sigma=0.05)
+    radj, rmis = mo.adjoint_source("local-adaptive", obs.astype(np.float64), syn.astype(np.float64), w, 1e-6, dt=dt,
+                                   tlag_max=float(np.float32(tlag)), method=penalty, eps=0.1, sigma=0.05)
+    for r in (3, 7, 30):
+        assert not adj[r].any() and mis[r] == 0
+    assert np.abs(radj).max() > 0 and rmis.sum() > 0
+    assert rel_l2(adj, radj) <= 2e-4, rel_l2(adj, radj)
+    assert abs(mis.sum() - rmis.sum()) <= 2e-4 * rmis.sum()

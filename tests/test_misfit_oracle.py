@@ -114,3 +114,25 @@ def test_next_power_235():
     for n in range(1, 400):
         m = mo.next_power_235(n)
         assert all(mo.next_power_235(j) == m for j in range(n, m + 1))
+
+
+def test_local_adaptive_adjoint_is_a_quarter_of_the_misfit_gradient_over_the_window_weights():
+    """OWL's local AWI adjoint divides the windowed back-projections by
+    sum_i g_i^2 and carries 1/4 of d(v^2)/dxc: adj * sum_i g_i^2 = 1/4
+    d misfit / d syn (eta held fixed, as OWL's adjoint does not
+    differentiate it), exactly outside the start taper and over the ramp
+    inside it."""
+    n, dt, sigma = 120, 0.004, 0.05
+    T = 0.1 * n * dt
+    obs = _broadband(n, 4, 15, 105)
+    syn = np.roll(obs, 3) + 0.3 * np.roll(obs, -2) + 0.05 * _broadband(n, 6, 0, n)
+    f = lambda o, s: mo.local_adaptive(o, s, dt, T, sigma=sigma, eta=0.02)  # noqa: E731
+    adj = f(obs, syn)[1]
+    g = 2.0 * _fd_half_grad(f, obs, syn)
+    a = np.arange(1, mo.next_power_235(2 * n) + 1)
+    gs = sum(np.exp(-(a - i) ** 2 / (2 * (sigma / dt) ** 2)) ** 2 for i in range(1, n + 1))[:n]
+    tl = int(np.floor(0.025 * n + 0.5))
+    assert np.linalg.norm(adj[tl:] * gs[tl:] - 0.25 * g[tl:]) < 1e-6 * np.linalg.norm(0.25 * g[tl:])
+    np.testing.assert_allclose(adj[:tl] * gs[:tl] * mo.hann_ramp(tl), 0.25 * g[:tl], atol=1e-8 * np.abs(g).max())
+    # equal traces: every local filter is a spike at lag 0, where the linear penalty is 0
+    assert mo.local_adaptive(obs, obs, dt, T, sigma=sigma, eps=1e-12)[0] < 1e-12
