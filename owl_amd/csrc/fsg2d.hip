@@ -1646,6 +1646,8 @@ void run_forwardf(owl_fsg2d* b, bool save_boundary) {
     OWL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
     b->last_updates = double(b->nt - 1) * b->pts_full;
+    b->have_boundary = b->fwd_state_valid = false;  // until the records are known to be usable
+    if (save_boundary) check_records_finite(b->d_rscale, b->rec_store, b->sd.size() * size_t(b->nt));
     b->have_boundary = save_boundary;
     b->fwd_state_valid = save_boundary;
 }

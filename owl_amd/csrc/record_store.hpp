@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <vector>
 
 #include "common.hpp"
 #include "owl_amd.h"
@@ -74,6 +75,19 @@ inline void alloc_records(DevBuf<float>& strips, DevBuf<unsigned>& amax, size_t 
         amax.alloc(std::max<size_t>(nshot * size_t(nt), 1));
         amax.zero(st);
     }
+}
+
+// After a forward that kept 16-bit records: a non-finite largest magnitude
+// (its bits at or above those of +inf) means the run diverged, and the
+// scaled records would hold nothing usable.
+inline void check_records_finite(const DevBuf<unsigned>& amax, int kind, size_t n) {
+    if (kind == OWL_STORE_F32 || n == 0) return;
+    std::vector<unsigned> h(n);
+    OWL_HIP(hipMemcpy(h.data(), amax.get(), n * sizeof(unsigned), hipMemcpyDeviceToHost));
+    for (unsigned v : h)
+        if (v >= 0x7f800000u)
+            fail(OWL_ERR_NONFINITE, "the wavefield became NaN or infinite (the run diverged): 16-bit boundary "
+                                    "records cannot hold it");
 }
 
 inline void check_record_kind(int kind) {

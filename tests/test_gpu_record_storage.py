@@ -103,3 +103,24 @@ def test_16bit_records(oracle64, gpu, solver):
         for x, y in zip(gs, hs):
             assert np.array_equal(x, y)
     b.close()
+
+
+def test_diverged_run_is_refused_with_16bit_records(gpu):
+    """A source amplitude that overflows float32 drives the wavefield to
+    inf/NaN (arithmetic only; no index depends on field values): with 16-bit
+    records the forward reports OWL_ERR_NONFINITE instead of storing
+    unusable records, and the reverse pass then refuses to run."""
+    from owl_amd import Geometry, Receiver, Source
+    from test_oracle_pinning_fsg2d import _tti_model
+    nz, nx = 30, 40
+    cij, rho = _tti_model(nz, nx, seed=2)
+    geom = Geometry(id=1, srcr=[Source(x=200.0, y=0.0, z=150.0, f0=20.0, mechanism="force", amp=3e38)],
+                    recr=[Receiver(x=20.0 + 10.0 * k, y=0.0, z=50.0) for k in range(30)])
+    print("case: FSG forward with an overflowing source and FP16 records", flush=True)
+    b = FSG2DBatch(nx, nz, 10.0, 10.0, cij, rho, [(geom, None)], 0.8e-3, 0.1, pml=10, fdhalf=4)
+    b.set_record_storage(OWL_STORE_F16)
+    with pytest.raises(Exception, match="NaN or infinite"):
+        b.forward(save_boundary=True)
+    with pytest.raises(Exception):
+        b.adjoint([tuple(np.zeros((30, b.nt), np.float32) for _ in range(2))])
+    b.close()
