@@ -553,6 +553,7 @@ owl_status owl_elastic2d_last_stats(const owl_elastic2d *b, double *ms, double *
 #define OWL_MISFIT_PHASE 3     /* instantaneous-phase difference (:460-481) */
 #define OWL_MISFIT_ADAPTIVE 4  /* adaptive waveform inversion, Warner & Guasch (:490-537) */
 #define OWL_MISFIT_LOCAL_ADAPTIVE 5 /* localised AWI, Yong et al. (:545-625) */
+#define OWL_MISFIT_ADAPTIVE_SPACETIME 6 /* AWI over groups of neighbouring traces (:629-701, 1171-1215) */
 
 /* Lag penalties of the adaptive misfit (xcorr_penalty, :321-363). */
 #define OWL_PENALTY_LINEAR 0
@@ -569,6 +570,7 @@ typedef struct owl_misfit_params {
     float penalty_power; /* default 1 */
     float deconv_eps;    /* Wiener-filter stabilisation relative to max |D|^2, default 0.1 */
     float lawi_sigma;    /* local adaptive: Gabor window standard deviation (s), default 0.25 */
+    int adaptive_hw;     /* space-time adaptive: half width of a trace group, default 3 */
 } owl_misfit_params;
 
 /* Misfit and adjoint source of the ntr traces of one shot component, on the
@@ -592,7 +594,12 @@ owl_status owl_adjoint_source(int type, int ntr, int nt, const float *obs, const
  * 1/n inverse DFT, and the linear [min, max] -> [0, 1] map.  The local
  * adaptive misfit sums v_i^2 over one matching filter per sample i under the
  * Gabor window exp(-(k - i)^2 / (2 (lawi_sigma/dt)^2)) (FLIT's gaussian,
- * taken unnormalised), v_i = sum (w xc_i)^2 / (sum xc_i^2 + eta). */
+ * taken unnormalised), v_i = sum (w xc_i)^2 / (sum xc_i^2 + eta).  The
+ * space-time adaptive misfit of trace i averages the filters of traces
+ * i - adaptive_hw .. i + adaptive_hw whose obs spectrum is not zero (no
+ * traces beyond the gather), pads to the 2-3-5 length >= 4 nt and
+ * back-projects through trace i's spectrum; as OWL's driver, it applies the
+ * receiver weights but not the discard threshold. */
 owl_status owl_adjoint_source_ex(int type, const owl_misfit_params *p, int ntr, int nt, const float *obs,
                                  const float *syn, const float *weight, float threshold, float *adj,
                                  double *misfit);

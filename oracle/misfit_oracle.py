@@ -195,6 +195,54 @@ def local_adaptive(obs, syn, dt, tlag_max, method="linear", power=1.0, eps=0.1, 
     return float(mis), taper1(adj[:n] / s[:n], 0, tl)
 
 
+def adaptive_spacetime(obs, syn, dt, tlag_max, method="linear", power=1.0, eps=0.1, hw=3, weight=None):
+    """compute_adjsrc_adaptive_spacetime (module_inversion_adjoint_source.f90:
+    629-701) for every trace of a gather [ntr, n] with its group of traces
+    i - hw .. i + hw (zero traces beyond the gather: FLIT's pad taken as zero
+    padding) and the driver's weight rule (:1171-1215): the matching filters
+    of the group's traces with a non-zero obs spectrum, averaged; misfit
+    0.5 sum (w xc)^2 / sum xc^2; adjoint through the centre trace's spectrum
+    only, without the average's 1/l.  Returns (adjoint [ntr, n], misfit
+    [ntr])."""
+    obs = np.asarray(obs, np.float64)
+    syn = np.asarray(syn, np.float64)
+    ntr, n = obs.shape
+    maxlag = int(np.ceil(tlag_max / dt))
+    nnt = next_power_235(4 * n)
+    tl = int(np.floor(0.025 * n + 0.5))
+    U = np.zeros((ntr, nnt), complex)
+    D = np.zeros((ntr, nnt), complex)
+    for j in range(ntr):
+        u = np.zeros(nnt)
+        d = np.zeros(nnt)
+        u[:n] = taper1(syn[j], tl, tl)
+        d[:n] = taper1(obs[j], tl, tl)
+        U[j], D[j] = np.fft.fft(u), np.fft.fft(d)
+    e = eps * np.max(np.abs(D) ** 2, axis=1)
+    xcs = np.zeros((ntr, nnt))
+    for j in range(ntr):
+        if e[j] > 0:
+            xcs[j] = np.fft.ifft(U[j] * np.conj(D[j]) / (np.abs(D[j]) ** 2 + e[j])).real
+    p = xcorr_penalty(dt, maxlag, tlag_max, method, power)
+    w = np.zeros(nnt)
+    w[:maxlag + 1] = p[maxlag:]
+    if maxlag:
+        w[nnt - maxlag:] = p[:maxlag]
+    wt = np.ones(ntr) if weight is None else np.asarray(weight)
+    adj = np.zeros((ntr, n))
+    mis = np.zeros(ntr)
+    for i in range(ntr):
+        grp = [j for j in range(i - hw, i + hw + 1) if 0 <= j < ntr and e[j] > 0]
+        if wt[i] == 0 or not grp or e[i] == 0:
+            continue
+        xc = xcs[grp].sum(axis=0) / len(grp)
+        sxc = np.sum(xc ** 2)
+        mis[i] = 0.5 * np.sum((w * xc) ** 2) / sxc
+        a = np.fft.ifft(np.fft.fft((w ** 2 - 2 * mis[i]) * xc / sxc) * D[i] / (np.abs(D[i]) ** 2 + e[i])).real
+        adj[i] = taper1(a[:n], 0, tl)
+    return adj, mis
+
+
 KINDS = {"waveform": waveform, "corr": corr, "envelope": envelope, "phase": phase}
 
 

@@ -24,9 +24,10 @@ OWL_CKPT_FULL, OWL_CKPT_UNIFORM, OWL_CKPT_BINOMIAL = 0, 1, 2
 OWL_STORE_F32, OWL_STORE_F16, OWL_STORE_BF16 = 0, 1, 2
 OWL_ANISO_ISO, OWL_ANISO_THOMSEN, OWL_ANISO_AT = 0, 1, 2
 OWL_MISFIT_WAVEFORM, OWL_MISFIT_CORR, OWL_MISFIT_ENVELOPE, OWL_MISFIT_PHASE = 0, 1, 2, 3
-OWL_MISFIT_ADAPTIVE, OWL_MISFIT_LOCAL_ADAPTIVE = 4, 5
+OWL_MISFIT_ADAPTIVE, OWL_MISFIT_LOCAL_ADAPTIVE, OWL_MISFIT_ADAPTIVE_SPACETIME = 4, 5, 6
 MISFITS = {"waveform": OWL_MISFIT_WAVEFORM, "corr": OWL_MISFIT_CORR, "envelope": OWL_MISFIT_ENVELOPE,
-           "phase": OWL_MISFIT_PHASE, "adaptive": OWL_MISFIT_ADAPTIVE, "local-adaptive": OWL_MISFIT_LOCAL_ADAPTIVE}
+           "phase": OWL_MISFIT_PHASE, "adaptive": OWL_MISFIT_ADAPTIVE, "local-adaptive": OWL_MISFIT_LOCAL_ADAPTIVE,
+           "adaptive-spacetime": OWL_MISFIT_ADAPTIVE_SPACETIME}
 PENALTIES = {"linear": 0, "power": 1, "gaussian": 2, "exp": 3}
 
 MECHANISMS = {"explosion": OWL_EXPLOSION, "force": OWL_FORCE}
@@ -51,7 +52,7 @@ class CflError(OwlError):
 
 class owl_misfit_params(C.Structure):
     _fields_ = [("dt", C.c_float), ("tlag_max", C.c_float), ("penalty", C.c_int), ("penalty_power", C.c_float),
-                ("deconv_eps", C.c_float), ("lawi_sigma", C.c_float)]
+                ("deconv_eps", C.c_float), ("lawi_sigma", C.c_float), ("adaptive_hw", C.c_int)]
 
 
 class owl_source(C.Structure):

@@ -259,7 +259,7 @@ __global__ void k_awi_load(const float* __restrict__ obs, const float* __restric
 // Spectra -> U = U conj(D) / den / nnt (the matching filter's spectrum with
 // the inverse FFT's 1/nnt) and D = D / den / nnt (kept for the adjoint),
 // den = |D|^2 + eps max |D|^2.  One block per trace.
-__global__ __launch_bounds__(kThreads) void k_awi_filter(Cplx* U, Cplx* D, int nnt, float eps) {
+__global__ __launch_bounds__(kThreads) void k_awi_filter(Cplx* U, Cplx* D, int nnt, float eps, int* valid) {
     __shared__ float sh[kThreads];
     Cplx* u = U + size_t(blockIdx.x) * nnt;
     Cplx* d = D + size_t(blockIdx.x) * nnt;
@@ -267,12 +267,33 @@ __global__ __launch_bounds__(kThreads) void k_awi_filter(Cplx* U, Cplx* D, int n
     for (int f = threadIdx.x; f < nnt; f += blockDim.x) m = fmaxf(m, d[f].x * d[f].x + d[f].y * d[f].y);
     const float e = eps * block_max(m, sh);
     const float inv = 1.0f / float(nnt);
+    // an all-zero obs spectrum (e = 0) gives zero rows and no filter
     for (int f = threadIdx.x; f < nnt; f += blockDim.x) {
         const Cplx a = u[f], b = d[f];
         const float den = (b.x * b.x + b.y * b.y) + e;
-        u[f] = Cplx{(a.x * b.x + a.y * b.y) / den * inv, (a.y * b.x - a.x * b.y) / den * inv};
-        d[f] = Cplx{b.x / den * inv, b.y / den * inv};
+        u[f] = e > 0.0f ? Cplx{(a.x * b.x + a.y * b.y) / den * inv, (a.y * b.x - a.x * b.y) / den * inv}
+                        : Cplx{0.0f, 0.0f};
+        d[f] = e > 0.0f ? Cplx{b.x / den * inv, b.y / den * inv} : Cplx{0.0f, 0.0f};
     }
+    if (valid && threadIdx.x == 0) valid[blockIdx.x] = e > 0.0f;
+}
+
+// Space-time adaptive misfit: trace i's filter is the mean of the valid
+// filters of traces i - hw .. i + hw (rows of X), in trace order; keep[i] =
+// weight != 0, trace i valid and a non-empty group.
+__global__ void k_stawi_group(const Cplx* __restrict__ X, const int* __restrict__ valid,
+                              const float* __restrict__ weight, Cplx* G, int* keep, int ntr, int nnt, int hw) {
+    const int i = blockIdx.y;
+    int l = 0;
+    for (int j = max(0, i - hw); j <= min(ntr - 1, i + hw); ++j) l += valid[j];
+    const float inv = l > 0 ? 1.0f / float(l) : 0.0f;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nnt; k += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int j = max(0, i - hw); j <= min(ntr - 1, i + hw); ++j)
+            if (valid[j]) s += X[size_t(j) * nnt + k].x;
+        G[size_t(i) * nnt + k] = Cplx{s * inv, 0.0f};
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) keep[i] = weight[i] != 0.0f && valid[i] && l > 0;
 }
 
 // From the filter xc (real part of U): misfit 0.5 sum (w xc)^2 / sum xc^2
@@ -462,14 +483,14 @@ int next_power_235(int n) {
     }
 }
 
-AwiShape awi_shape(const owl_misfit_params* p, int nt) {
+AwiShape awi_shape(const owl_misfit_params* p, int nt, int pad = 2) {
     if (!p) fail(OWL_ERR_ARG, "the adaptive misfit needs its parameters");
     if (!(p->dt > 0) || !(p->tlag_max > 0) || !(p->deconv_eps >= 0))
         fail(OWL_ERR_ARG, "adaptive misfit: dt and tlag_max must be positive, deconv_eps non-negative");
     if (p->penalty < OWL_PENALTY_LINEAR || p->penalty > OWL_PENALTY_EXP) fail(OWL_ERR_ARG, "unknown lag penalty");
     AwiShape a;
     a.maxlag = int(std::ceil(p->tlag_max / p->dt));  // ceiling(tlag_max/dt) in float32
-    a.nnt = next_power_235(2 * nt);
+    a.nnt = next_power_235(pad * nt);
     a.tl = int(std::floor(0.025f * float(nt) + 0.5f));
     a.eps = p->deconv_eps;
     if (a.maxlag + 1 > a.nnt) fail(OWL_ERR_ARG, "tlag_max exceeds the padded trace length");
@@ -513,6 +534,39 @@ struct FftPlan {
         fft_check(hipfftExecC2C(p, zc, zc, dir), dir == HIPFFT_FORWARD ? "forward FFT" : "inverse FFT");
     }
 };
+
+// compute_adjsrc_adaptive_spacetime for every trace of the gather, with the
+// driver's weight rule (module_inversion_adjoint_source.f90:1171-1215).
+void adaptive_spacetime(const AwiShape& aw, const owl_misfit_params& p, int ntr, int nt, const float* weight,
+                        const float* obs, const float* syn, float* adj, double* mis, hipStream_t st) {
+    const int nnt = aw.nnt;
+    const size_t m = size_t(ntr) * size_t(nnt);
+    DevBuf<Cplx> U(m), D(m), G(m);
+    DevBuf<float> d_ramp(std::max<size_t>(aw.ramp.size(), 1)), d_w(static_cast<size_t>(nnt)),
+        d_wt(static_cast<size_t>(ntr));
+    DevBuf<int> valid(static_cast<size_t>(ntr)), keep(static_cast<size_t>(ntr));
+    if (!aw.ramp.empty()) d_ramp.upload(aw.ramp.data(), aw.ramp.size(), st);
+    d_w.upload(aw.w.data(), aw.w.size(), st);
+    std::vector<float> wt(size_t(ntr), 1.0f);
+    if (weight) wt.assign(weight, weight + ntr);
+    d_wt.upload(wt.data(), wt.size(), st);
+    FftPlan plan(nnt, ntr, st);
+    const dim3 rows(unsigned(std::min(64, (nnt + 255) / 256)), unsigned(ntr));
+    hipLaunchKernelGGL(k_awi_load, rows, dim3(256), 0, st, obs, syn, d_ramp.get(), U.get(), D.get(), nt, nnt, aw.tl);
+    plan.run(U.get(), HIPFFT_FORWARD);
+    plan.run(D.get(), HIPFFT_FORWARD);
+    hipLaunchKernelGGL(k_awi_filter, dim3(ntr), dim3(kThreads), 0, st, U.get(), D.get(), nnt, aw.eps, valid.get());
+    plan.run(U.get(), HIPFFT_BACKWARD);
+    hipLaunchKernelGGL(k_stawi_group, rows, dim3(256), 0, st, U.get(), valid.get(), d_wt.get(), G.get(), keep.get(),
+                       ntr, nnt, p.adaptive_hw);
+    hipLaunchKernelGGL(k_awi_misfit, dim3(ntr), dim3(kThreads), 0, st, G.get(), d_w.get(), keep.get(), nnt, mis);
+    plan.run(G.get(), HIPFFT_FORWARD);
+    const unsigned g = unsigned(std::min<size_t>((m + 255) / 256, 65535));
+    hipLaunchKernelGGL(k_awi_back, dim3(g), dim3(256), 0, st, G.get(), D.get(), m);
+    plan.run(G.get(), HIPFFT_BACKWARD);
+    hipLaunchKernelGGL(k_awi_out, dim3(unsigned(std::min(64, (nt + 255) / 256)), unsigned(ntr)), dim3(256), 0, st,
+                       G.get(), d_ramp.get(), keep.get(), adj, nt, nnt, aw.tl);
+}
 
 // compute_adjsrc_local_adaptive for every trace: the whole-trace filter for
 // eps and eta, then the per-sample windows in chunks of nc windows of every
@@ -604,10 +658,15 @@ owl_status owl_adjoint_source_ex(int type, const owl_misfit_params* prm, int ntr
     return guarded([&] {
         if (ntr < 0 || nt < 1 || (ntr > 0 && (!obs || !syn || !adj || !misfit))) fail(OWL_ERR_ARG, "null argument");
         if (type != OWL_MISFIT_WAVEFORM && type != OWL_MISFIT_CORR && type != OWL_MISFIT_ENVELOPE &&
-            type != OWL_MISFIT_PHASE && type != OWL_MISFIT_ADAPTIVE && type != OWL_MISFIT_LOCAL_ADAPTIVE)
+            type != OWL_MISFIT_PHASE && type != OWL_MISFIT_ADAPTIVE && type != OWL_MISFIT_LOCAL_ADAPTIVE &&
+            type != OWL_MISFIT_ADAPTIVE_SPACETIME)
             fail(OWL_ERR_ARG, "unknown misfit type");
         AwiShape aw{};
         if (type == OWL_MISFIT_ADAPTIVE || type == OWL_MISFIT_LOCAL_ADAPTIVE) aw = awi_shape(prm, nt);
+        if (type == OWL_MISFIT_ADAPTIVE_SPACETIME) {
+            aw = awi_shape(prm, nt, 4);
+            if (prm->adaptive_hw < 0) fail(OWL_ERR_ARG, "space-time adaptive misfit: adaptive_hw must be >= 0");
+        }
         if (type == OWL_MISFIT_LOCAL_ADAPTIVE && !(prm->lawi_sigma > 0))
             fail(OWL_ERR_ARG, "local adaptive misfit: lawi_sigma must be positive");
         if (ntr == 0) return;
@@ -669,7 +728,8 @@ owl_status owl_adjoint_source_ex(int type, const owl_misfit_params* prm, int ntr
                                D.get(), nt, aw.nnt, aw.tl);
             fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_FORWARD), "forward FFT");
             fft_check(hipfftExecC2C(plan, dc, dc, HIPFFT_FORWARD), "forward FFT");
-            hipLaunchKernelGGL(k_awi_filter, dim3(ntr), dim3(kThreads), 0, st, U.get(), D.get(), aw.nnt, aw.eps);
+            hipLaunchKernelGGL(k_awi_filter, dim3(ntr), dim3(kThreads), 0, st, U.get(), D.get(), aw.nnt, aw.eps,
+                               static_cast<int*>(nullptr));
             fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_BACKWARD), "inverse FFT");
             hipLaunchKernelGGL(k_awi_misfit, dim3(ntr), dim3(kThreads), 0, st, U.get(), d_w.get(), d_keep.get(),
                                aw.nnt, d_mis.get());
@@ -679,6 +739,8 @@ owl_status owl_adjoint_source_ex(int type, const owl_misfit_params* prm, int ntr
             fft_check(hipfftExecC2C(plan, uc, uc, HIPFFT_BACKWARD), "inverse FFT");
             hipLaunchKernelGGL(k_awi_out, dim3(unsigned(std::min(64, (nt + 255) / 256)), unsigned(ntr)), dim3(256), 0,
                                st, U.get(), d_ramp.get(), d_keep.get(), d_adj.get(), nt, aw.nnt, aw.tl);
+        } else if (type == OWL_MISFIT_ADAPTIVE_SPACETIME) {
+            adaptive_spacetime(aw, *prm, ntr, nt, weight, d_obs.get(), d_syn.get(), d_adj.get(), d_mis.get(), st);
         } else if (type == OWL_MISFIT_LOCAL_ADAPTIVE) {
             local_adaptive(aw, *prm, ntr, nt, d_obs.get(), d_syn.get(), d_keep.get(), d_adj.get(), d_mis.get(), st);
         } else {

@@ -15,7 +15,8 @@ module owl_amd_c
     integer(c_int), parameter :: OWL_CKPT_FULL = 0, OWL_CKPT_UNIFORM = 1, OWL_CKPT_BINOMIAL = 2
     integer(c_int), parameter :: OWL_STORE_F32 = 0, OWL_STORE_F16 = 1, OWL_STORE_BF16 = 2
     integer(c_int), parameter :: OWL_MISFIT_WAVEFORM = 0, OWL_MISFIT_CORR = 1, OWL_MISFIT_ENVELOPE = 2, &
-        OWL_MISFIT_PHASE = 3, OWL_MISFIT_ADAPTIVE = 4, OWL_MISFIT_LOCAL_ADAPTIVE = 5
+        OWL_MISFIT_PHASE = 3, OWL_MISFIT_ADAPTIVE = 4, OWL_MISFIT_LOCAL_ADAPTIVE = 5, &
+        OWL_MISFIT_ADAPTIVE_SPACETIME = 6
     integer(c_int), parameter :: OWL_PENALTY_LINEAR = 0, OWL_PENALTY_POWER = 1, OWL_PENALTY_GAUSSIAN = 2, &
         OWL_PENALTY_EXP = 3
 
@@ -23,6 +24,7 @@ module owl_amd_c
         real(c_float) :: dt, tlag_max
         integer(c_int) :: penalty
         real(c_float) :: penalty_power, deconv_eps, lawi_sigma
+        integer(c_int) :: adaptive_hw
     end type
 
     type, bind(c) :: owl_source_t

@@ -77,6 +77,7 @@ module owl_caller
     ! the adaptive misfit's parameters (module_parameters.f90:262-265, 551-557)
     real :: tlag_max = 0.1, penalty_power = 1.0, deconv_eps = 0.1, lawi_sigma = 0.25
     character(len=32) :: penalty_method = 'linear'
+    integer :: adaptive_hw = 3
     logical :: yn_energy_precond = .false.
     integer :: niter_max = 1
 
@@ -197,6 +198,7 @@ contains
                 call readpar_float(file_parameter, 'deconv_eps', deconv_eps, 0.1)
                 call readpar_float(file_parameter, 'penalty_power', penalty_power, 1.0)
                 call readpar_float(file_parameter, 'lawi_sigma', lawi_sigma, 0.25)
+                call readpar_int(file_parameter, 'adaptive_half_window', adaptive_hw, 3)
                 call readpar_float(file_parameter, 'trace_discard_threshold', trace_discard_threshold, 1.0e-6)
                 call readpar_logical(file_parameter, 'yn_energy_precond', yn_energy_precond, .false.)
                 call readpar_int(file_parameter, 'niter_max', niter_max, 100)

@@ -75,6 +75,8 @@ contains
                 mtype = OWL_MISFIT_ADAPTIVE
             case ('local-adaptive')
                 mtype = OWL_MISFIT_LOCAL_ADAPTIVE
+            case ('adaptive-spacetime')
+                mtype = OWL_MISFIT_ADAPTIVE_SPACETIME
             case default
                 call stop_with('Error: misfit_type = '//tidy(misfit_type)//' is not provided by owl_gradient2')
         end select
@@ -234,6 +236,7 @@ contains
         prm%penalty_power = real(penalty_power, c_float)
         prm%deconv_eps = real(deconv_eps, c_float)
         prm%lawi_sigma = real(lawi_sigma, c_float)
+        prm%adaptive_hw = int(adaptive_hw, c_int)
         st = owl_adjoint_source_ex(int(mtype, c_int), prm, int(nr, c_int), int(nt, c_int), o, s, w, &
             real(trace_discard_threshold, c_float), a, m)
         if (st /= OWL_OK) call stop_with('<owl_adjoint_source> '//owl_error_message())

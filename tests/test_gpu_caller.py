@@ -128,7 +128,10 @@ MISFIT_CASES = {"waveform": ({"misfit_type": "waveform"}, {}),
                               "penalty_power": 2.0, "deconv_eps": 0.05},
                              dict(dt=DT, tlag_max=0.04, penalty="power", penalty_power=2.0, deconv_eps=0.05)),
                 "local-adaptive": ({"misfit_type": "local-adaptive", "tlag_max": 0.04, "lawi_sigma": 0.03},
-                                   dict(dt=DT, tlag_max=0.04, lawi_sigma=0.03))}
+                                   dict(dt=DT, tlag_max=0.04, lawi_sigma=0.03)),
+                "adaptive-spacetime": ({"misfit_type": "adaptive-spacetime", "tlag_max": 0.04,
+                                        "adaptive_half_window": 2},
+                                       dict(dt=DT, tlag_max=0.04, adaptive_hw=2))}
 
 
 @pytest.mark.parametrize("kind", list(MISFIT_CASES))
@@ -158,11 +161,25 @@ def test_owl_gradient2_acoustic(gpu, tmp_path, kind):
         full[0][sl] += gv
         full[1][sl] += gr
     _run(GRADIENT, p, f"owl_gradient2 acoustic-iso, 3 shots, {kind} misfit")
+    # The FFT-based misfits run hipFFT: this python process may bind the
+    # rocFFT build that torch bundles, the Fortran program binds the system's,
+    # and the two may round some transform lengths differently (1-2 ulp).
+    # Without FFTs the two processes agree bit for bit.
+    exact = kind == "waveform"
+    for s, a in zip(shots, adjs):
+        got, _ = read_su(str(tmp_path / "work" / "iteration_1" / "adjoint_source" / f"shot_{s['id']}_seismogram_p.su"))
+        if exact:
+            assert np.array_equal(got, a), s["id"]
+        else:
+            assert np.abs(got - a).max() <= 1e-6 * np.abs(a).max(), (s["id"], float(np.abs(got - a).max()))
     for k, name in enumerate(("vp", "rho")):
         got = np.fromfile(str(tmp_path / "work" / "iteration_1" / "model" / f"grad_{name}.bin"),
                           np.float32).reshape(NX, NZ).T
         assert np.abs(full[k]).max() > 0
-        assert np.array_equal(got, full[k]), name
+        if exact:
+            assert np.array_equal(got, full[k]), name
+        else:
+            assert np.linalg.norm(got - full[k]) <= 1e-5 * np.linalg.norm(full[k]), name
     with open(tmp_path / "work" / "data_misfit.txt") as f:
         it, value = f.read().split()
-    assert int(it) == 1 and float(value) == pytest.approx(misfit, rel=1e-12)
+    assert int(it) == 1 and float(value) == pytest.approx(misfit, rel=1e-12 if exact else 1e-6)

@@ -92,3 +92,25 @@ def test_local_adaptive_matches_the_oracle(gpu, penalty):
     assert np.abs(radj).max() > 0 and rmis.sum() > 0
     assert rel_l2(adj, radj) <= 2e-4, rel_l2(adj, radj)
     assert abs(mis.sum() - rmis.sum()) <= 2e-4 * rmis.sum()
+
+
+@pytest.mark.parametrize("hw", [0, 3])
+def test_adaptive_spacetime_matches_the_oracle(gpu, hw):
+    """Space-time AWI over groups of 2 hw + 1 neighbouring traces (padded
+    length 2800 -> 2880 for 700 samples) against the float64 oracle within
+    1e-4; as OWL's driver, zero weights drop a trace and the discard
+    threshold does not apply."""
+    obs, syn, w = _shot()
+    dt = 1e-3
+    tlag = 0.1 * (obs.shape[1] - 1) * dt
+    print(f"case: space-time adaptive misfit, hw {hw}", flush=True)
+    adj, mis = adjoint_source("adaptive-spacetime", obs, syn, w, 1e-6, dt=dt, adaptive_hw=hw)
+    adj2, mis2 = adjoint_source("adaptive-spacetime", obs, syn, w, 1e-6, dt=dt, adaptive_hw=hw)
+    assert np.array_equal(adj, adj2) and np.array_equal(mis, mis2)
+    radj, rmis = mo.adaptive_spacetime(obs.astype(np.float64), syn.astype(np.float64), dt, float(np.float32(tlag)),
+                                       eps=0.1, hw=hw, weight=w)
+    for r in (3, 30):
+        assert not adj[r].any() and mis[r] == 0
+    assert mis[7] > 0
+    assert rel_l2(adj, radj) <= 1e-4, rel_l2(adj, radj)
+    assert abs(mis.sum() - rmis.sum()) <= 1e-4 * rmis.sum()

@@ -136,3 +136,34 @@ def test_local_adaptive_adjoint_is_a_quarter_of_the_misfit_gradient_over_the_win
     np.testing.assert_allclose(adj[:tl] * gs[:tl] * mo.hann_ramp(tl), 0.25 * g[:tl], atol=1e-8 * np.abs(g).max())
     # equal traces: every local filter is a spike at lag 0, where the linear penalty is 0
     assert mo.local_adaptive(obs, obs, dt, T, sigma=sigma, eps=1e-12)[0] < 1e-12
+
+
+def test_adaptive_spacetime_closed_form_and_gradient():
+    """Space-time AWI: a gather of one broadband trace shifted by k samples in
+    every syn trace gives the spike filter at every group size, so each
+    trace's misfit is penalty(k)^2 / 2 (edge groups are shorter); and the
+    adjoint of trace i is l_i d misfit_i / d syn_i (OWL leaves the average's
+    1/l out of the back-projection), l_i the group's size, outside the start
+    taper."""
+    n, dt, ntr, hw = 160, 0.004, 7, 2
+    T = 0.1 * n * dt
+    x = _broadband(n, 8, 20, 140)
+    obs = np.tile(x, (ntr, 1))
+    for k in (0, 4, -6):
+        _, mis = mo.adaptive_spacetime(obs, np.roll(obs, k, axis=1), dt, T, eps=1e-10, hw=hw)
+        np.testing.assert_allclose(mis, 0.5 * (abs(k) * dt / T) ** 2, atol=1e-12, rtol=1e-8)
+    rng = np.random.default_rng(9)
+    obs = np.stack([np.roll(x, j) + 0.1 * _broadband(n, 20 + j, 0, n) for j in range(ntr)])
+    syn = np.roll(obs, 3, axis=1) + 0.2 * rng.standard_normal((ntr, n))
+    adj, _ = mo.adaptive_spacetime(obs, syn, dt, T, hw=hw)
+    tl = int(np.floor(0.025 * n + 0.5))
+    for i in (0, 3):
+        l = len([j for j in range(i - hw, i + hw + 1) if 0 <= j < ntr])
+        g = np.zeros(n)
+        for t in range(n):
+            sp, sm = syn.copy(), syn.copy()
+            sp[i, t] += 1e-6
+            sm[i, t] -= 1e-6
+            g[t] = (mo.adaptive_spacetime(obs, sp, dt, T, hw=hw)[1][i] -
+                    mo.adaptive_spacetime(obs, sm, dt, T, hw=hw)[1][i]) / 2e-6
+        assert np.linalg.norm(adj[i, tl:] - l * g[tl:]) < 1e-6 * np.linalg.norm(l * g[tl:]), i
