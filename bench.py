@@ -454,8 +454,9 @@ def main():
     ap.add_argument("--elastic3d-steps", type=int, default=2500, help="time steps of the 3D elastic forward")
     ap.add_argument("--elastic3d-shape", default="", metavar="PLANES,ROWS",
                     help="this launch shape of the 3D elastic sweeps instead of the autotuner's (counter runs)")
-    ap.add_argument("--elastic3d-gradient-shots", type=int, default=8,
-                    help="shots of the timed 3D elastic gradient (0 skips it)")
+    ap.add_argument("--elastic3d-gradient-shots", type=int, default=None,
+                    help="shots of the timed 3D elastic gradient (default 8; 0 skips it; with "
+                         "--only-elastic3d default 0)")
     ap.add_argument("--only-fsg", action="store_true", help="only the 2D FSG forward (for profiler runs)")
     ap.add_argument("--fsg-steps", type=int, default=5000, help="time steps of the 2D FSG forward")
     ap.add_argument("--fsg-shape", default="", metavar="PLANES,ROWS",
@@ -501,7 +502,8 @@ def main():
         print(json.dumps({"acoustic3d_forward": bench_3d(sync, args.steps_3d, shape)}))
         return
     if args.only_elastic3d:
-        print(json.dumps({"elastic3d_forward": bench_elastic3d(sync, args.elastic3d_steps, 0, el3_shape)}))
+        print(json.dumps({"elastic3d_forward": bench_elastic3d(sync, args.elastic3d_steps,
+                                                               args.elastic3d_gradient_shots or 0, el3_shape)}))
         return
     if args.only_fsg:
         print(json.dumps({"fsg2d_forward": bench_fsg(sync, args.fsg_steps, not args.no_fsg_gradient,
@@ -599,7 +601,8 @@ def main():
     el3 = None
     if not args.no_elastic3d:
         log("3D elastic batched forward")
-        el3 = bench_elastic3d(sync, args.elastic3d_steps, args.elastic3d_gradient_shots, el3_shape)
+        el3 = bench_elastic3d(sync, args.elastic3d_steps,
+                              8 if args.elastic3d_gradient_shots is None else args.elastic3d_gradient_shots, el3_shape)
 
     fsg = None
     if not args.no_fsg:
