@@ -451,6 +451,8 @@ owl_status owl_elastic3d_set_launch(owl_elastic3d *b, int planes, int rows);
  * keeps the fastest and returns it (planes, rows may be NULL).  No
  * reference counterpart. */
 owl_status owl_elastic3d_autotune(owl_elastic3d *b, int steps, int *planes, int *rows);
+/* As owl_fsg2d_set_graphs, for this batch's forward loop. */
+owl_status owl_elastic3d_set_graphs(owl_elastic3d *b, int on);
 /* As owl_fsg2d_set_record_storage. */
 owl_status owl_elastic3d_set_record_storage(owl_elastic3d *b, int kind);
 /* Timing as for the acoustic batches; kinds: 0 stress sweep, 1 velocity

@@ -190,6 +190,7 @@ _SIGNATURES = {
     "owl_elastic3d_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
     "owl_elastic3d_set_record_storage": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_autotune": (C.c_int, [_P, C.c_int, _P, _P]),
+    "owl_elastic3d_set_graphs": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_set_profiling": (C.c_int, [_P, C.c_int]),
     "owl_elastic3d_kernel_stats": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "owl_elastic3d_last_stats": (C.c_int, [_P, _P, _P]),

@@ -895,6 +895,9 @@ struct owl_elastic3d {
     DevBuf<int> d_adj_beg, d_win;
     DevBuf<Tap> d_adj_taps;
     DevBuf<double> d_gA, d_gB, d_gC, d_fA, d_fB, d_fC, d_misfit, d_scratch;
+    // the forward time loop as one HIP graph (modelling without profiling
+    // or boundary records), re-captured when a launch argument changes
+    LoopGraph graph;
     ~owl_elastic3d() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1370,6 +1373,22 @@ void step_e(owl_elastic3d* b, const EF& F, const EV& O, float dt, double bs, dou
 
 void strips_e(owl_elastic3d* b, int R, float* vx, float* vy, float* vz, int t, int save);
 
+// Every device address and launch parameter the forward loop passes to its
+// kernels: a captured loop is replayed only while these are unchanged.
+std::vector<uintptr_t> forward_signature_e(const owl_elastic3d* b) {
+    std::vector<uintptr_t> v{uintptr_t(b->R), uintptr_t(b->nt), uintptr_t(b->kc), uintptr_t(b->cy),
+                             uintptr_t(b->tile_shot.size()), uintptr_t(b->max_nr), uintptr_t(b->fs),
+                             uintptr_t(float_bits(b->dt)), uintptr_t(float_bits(b->idx)), uintptr_t(float_bits(b->idy)),
+                             uintptr_t(float_bits(b->idz)), uintptr_t(b->d_shots.get()),
+                             uintptr_t(b->d_tile_shot.get()), uintptr_t(b->d_rec_box.get()),
+                             uintptr_t(b->d_rec_w.get()), uintptr_t(b->d_traces.get()),
+                             uintptr_t(b->d_inj_beg.get()), uintptr_t(b->d_inj.get()), uintptr_t(b->d_amp.get()),
+                             uintptr_t(b->d_prof.get())};
+    for (auto& f : b->fld) v.push_back(uintptr_t(f.get()));
+    for (auto& f : b->med) v.push_back(uintptr_t(f.get()));
+    return v;
+}
+
 template <int R>
 void run_forward_e(owl_elastic3d* b, bool save_boundary) {
     hipStream_t st = b->stream;
@@ -1381,31 +1400,41 @@ void run_forward_e(owl_elastic3d* b, bool save_boundary) {
     double bs, bv;
     sweep_bytes(b, bs, bv);
     const unsigned rg = unsigned(std::max(1, (3 * b->max_nr + 255) / 256));
-    OWL_HIP(hipEventRecord(b->ev0, st));
     // step t: [boundary record of the state at the start of the step] ->
     // [free surface: mirror v above it] -> record(t) -> update -> source
-    for (int t = 1; t <= b->nt; ++t) {
-        if (save_boundary) strips_e(b, R, F.vx, F.vy, F.vz, t, 1);
-        if (b->fs) hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 2);
-        int ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(ke_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->d_rec_box.get(),
-                           b->d_rec_w.get(), b->d_traces.get(), b->nt, t);
-        b->prof.end(st, ev);
-        if (t == b->nt) {
-            // the last update changes nothing that is recorded; the state at
-            // the start of step nt is the reverse pass's final state, whose
-            // velocities above a free surface are zero (the mirror undone)
+    auto loop = [&] {
+        for (int t = 1; t <= b->nt; ++t) {
+            if (save_boundary) strips_e(b, R, F.vx, F.vy, F.vz, t, 1);
             if (b->fs)
-                hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 3);
-            break;
+                hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml, 2);
+            int ev = b->prof.begin(st, 2, 0);
+            hipLaunchKernelGGL(ke_record, dim3(rg, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->d_rec_box.get(),
+                               b->d_rec_w.get(), b->d_traces.get(), b->nt, t);
+            b->prof.end(st, ev);
+            if (t == b->nt) {
+                // the last update changes nothing that is recorded; the state
+                // at the start of step nt is the reverse pass's final state,
+                // whose velocities above a free surface are zero (the mirror
+                // undone)
+                if (b->fs)
+                    hipLaunchKernelGGL(ke_surface<R>, dim3(256, nshot), dim3(256), 0, st, b->d_shots.get(), F, b->pml,
+                                       3);
+                break;
+            }
+            step_e<R>(b, F, out_of(F), b->dt, bs, bv);
+            ev = b->prof.begin(st, 2, 0);
+            hipLaunchKernelGGL(ke_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), F, b->d_inj_beg.get(),
+                               b->d_inj.get(), b->d_amp.get(), b->nt, t);
+            b->prof.end(st, ev);
+            if (!b->graph.capturing) maybe_collect(b);
         }
-        step_e<R>(b, F, out_of(F), b->dt, bs, bv);
-        ev = b->prof.begin(st, 2, 0);
-        hipLaunchKernelGGL(ke_inject, dim3(nshot), dim3(1024), 0, st, b->d_shots.get(), F, b->d_inj_beg.get(),
-                           b->d_inj.get(), b->d_amp.get(), b->nt, t);
-        b->prof.end(st, ev);
-        maybe_collect(b);
-    }
+    };
+    const bool graph_ok = b->graph.enabled && !b->prof.on() && !save_boundary;
+    OWL_HIP(hipEventRecord(b->ev0, st));
+    if (graph_ok)
+        b->graph.run(st, forward_signature_e(b), loop);
+    else
+        loop();
     OWL_HIP(hipGetLastError());
     OWL_HIP(hipEventRecord(b->ev1, st));
     OWL_HIP(hipStreamSynchronize(st));
@@ -1769,6 +1798,14 @@ owl_status owl_elastic3d_set_record_storage(owl_elastic3d* b, int kind) {
             b->have_boundary = false;
             b->fwd_state_valid = false;
         }
+    });
+}
+
+owl_status owl_elastic3d_set_graphs(owl_elastic3d* b, int on) {
+    return guarded([&] {
+        if (!b) fail(OWL_ERR_ARG, "null batch");
+        b->graph.enabled = on != 0;
+        b->graph.reset();
     });
 }
 

@@ -80,6 +80,11 @@ class Elastic3DBatch:
         (default), OWL_STORE_F16 or OWL_STORE_BF16 (scaled per record)."""
         check(lib().owl_elastic3d_set_record_storage(self._h, int(store)))
 
+    def set_graphs(self, on=True):
+        """True (default): forward modelling without boundary records or
+        profiling replays its time loop as one captured HIP graph."""
+        check(lib().owl_elastic3d_set_graphs(self._h, int(bool(on))))
+
     def autotune(self, steps=3):
         """Times `steps` steps of the sweeps under each launch shape and keeps
         the fastest; returns (planes, rows)."""

@@ -146,3 +146,35 @@ def test_elastic3d_owl_gradient_matches_oracle(oracle32, gpu, R, fs):
         m = sum(float(np.sum((np.asarray(o, np.float64) - np.asarray(x, np.float64)) ** 2)) for x, o in zip(sy, ob))
         assert abs(mis[s] - m) <= 1e-6 * m
     b.close()
+
+
+def test_elastic3d_graph_replays_match_eager(gpu):
+    """The forward loop captured as one HIP graph (owl_elastic3d_set_graphs):
+    every replay gives the eager traces bit for bit, with profiled and
+    boundary-recording runs (eager) between replays, and a new launch shape
+    re-captures."""
+    nz, ny, nx, dx, dt, tmax, pml, R = 22, 24, 26, 10.0, 0.001, 0.1, 8, 4
+    vp, vs, rho = _model(nz, ny, nx, seed=21)
+    print("case: elastic 3D forward graph replays", flush=True)
+    b = Elastic3DBatch(nx, ny, nz, dx, dx, dx, vp, vs, rho, _geoms(), dt, tmax, pml=pml, fdhalf=R, free_surface=True)
+    b.set_graphs(False)
+    ref = b.forward()
+    assert max(np.abs(c).max() for tr in ref for c in tr) > 0
+    b.set_graphs(True)
+
+    def same(got):
+        for x, y in zip(got, ref):
+            for u, v in zip(x, y):
+                assert np.array_equal(u, v)
+    for k in range(4):
+        same(b.forward())
+        if k == 1:
+            b.set_profiling(True)
+            same(b.forward())
+            b.set_profiling(False)
+        if k == 2:
+            same(b.forward(save_boundary=True))
+    b.set_launch(16, 2)
+    same(b.forward())
+    same(b.forward())
+    b.close()
